@@ -1,0 +1,206 @@
+/*
+ * sphere_hip.h -- C ABI of the MI355X (gfx950) DEM + Darcy library.
+ *
+ * The reference program has no library boundary: python/sphere.py
+ * writes input/<sid>.bin and runs "cd ..; ./sphere [flags] input/<sid>.bin"
+ * (python/sphere.py:4482-4540), and the CUDA binary reads/writes the
+ * versioned binary format (src/file_io.cpp:30-731) and runs the time loop
+ * DEM::startTime (src/device.cu:833-2834).  This header is the drop-in
+ * replacement of that process boundary: every entry point below replaces
+ * one step of that contract and is cited at its declaration.
+ *
+ * Conventions
+ *   - extern "C", plain pointers and sizes, no exceptions across the ABI.
+ *   - int status: 0 = OK, negative = error (sph_last_error() has the text).
+ *   - One context is single threaded and not reentrant; every call runs on
+ *     the context's own HIP stream and returns when its work is complete.
+ *   - Caller-owned buffers are copied in or out and never retained.
+ */
+#ifndef SPHERE_HIP_H_
+#define SPHERE_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_VERSION 2.15      /* src/version.h:5 */
+#define SPH_NC 32             /* contact slots per particle, src/constants.h:21 */
+#define SPH_MAXWALLS 6        /* src/constants.h:12 */
+
+/* Status codes */
+#define SPH_OK 0
+#define SPH_ERR_ARG -1        /* bad argument / inconsistent sizes */
+#define SPH_ERR_FILE -2       /* file missing, short or wrong version */
+#define SPH_ERR_VALUE -3      /* failed value check (src/sphere.cpp:145-345) */
+#define SPH_ERR_HIP -4        /* HIP runtime error or no device */
+#define SPH_ERR_GRID -5       /* a particle left the sorting grid */
+#define SPH_ERR_NAN -6        /* NaN Darcy residual (src/darcy.cpp:276-285) */
+#define SPH_ERR_OVERFLOW -7   /* more than SPH_NC contacts on one particle */
+#define SPH_ERR_STATE -8      /* call out of order (no state loaded, ...) */
+
+/*
+ * In-memory image of one v2.15 binary file (field order of
+ * src/file_io.cpp:420-731 and python/sphere.py:1355-1553).  All arrays are
+ * caller owned.  Per-particle vectors are packed rows; fluid cell arrays
+ * are interior cells in file order (x fastest, then y, then z).
+ */
+typedef struct sph_state {
+    /* header, src/file_io.cpp:47-104 */
+    int32_t nd;
+    uint32_t np;
+    double time_dt, time_current, time_total, time_file_dt;
+    uint32_t time_step_count;
+    double origo[3], L[3];
+    uint32_t num[3];
+    int32_t periodic, adaptive;
+
+    /* particles, src/file_io.cpp:106-164 */
+    double *x;       /* np*4: x, y, z, radius      */
+    double *xyzsum;  /* np*3                        */
+    double *vel;     /* np*4: vx, vy, vz, fixvel    */
+    double *force;   /* np*3 */
+    double *angpos;  /* np*3 */
+    double *angvel;  /* np*3 */
+    double *torque;  /* np*3 */
+    double *es_dot, *es, *ev_dot, *ev, *p;  /* np each */
+
+    /* params, src/file_io.cpp:166-186 (k_r carries Poisson's ratio for the
+     * Hertz-Mindlin model, see contactmodel 3) */
+    double g[3];
+    double k_n, k_t, k_r, E, gamma_n, gamma_t, gamma_r;
+    double mu_s, mu_d, mu_r, gamma_wn, gamma_wt, mu_ws, mu_wd, rho;
+    uint32_t contactmodel;
+    double kappa, db, V_b;
+
+    /* walls, src/file_io.cpp:188-220 */
+    uint32_t nw;
+    int32_t wmode[SPH_MAXWALLS];
+    double w_nx[SPH_MAXWALLS * 4];    /* normal x,y,z and position        */
+    double w_mvfd[SPH_MAXWALLS * 4];  /* mass, velocity, force, sigma0    */
+    double sigma0_A, sigma0_f, tau_x;
+
+    /* bonds, src/file_io.cpp:222-249 */
+    double lambda_bar;
+    uint32_t nb0;
+    double sigma_b, tau_b;
+    uint32_t *bonds;          /* nb0*2 (broken bonds hold nb0 in column 0) */
+    double *bonds_delta_n;    /* nb0   */
+    double *bonds_delta_t;    /* nb0*3 */
+    double *bonds_omega_n;    /* nb0   */
+    double *bonds_omega_t;    /* nb0*3 */
+
+    /* Darcy fluid block, present iff fluid != 0 (src/file_io.cpp:329-398) */
+    int32_t fluid;            /* not in the file: the --fluid flag */
+    int32_t cfd_solver;       /* must be 1 (Darcy) */
+    double mu;
+    double *v_f;              /* cells*3 */
+    double *p_f, *phi, *dphi; /* cells each; dphi as the native solver stores it */
+    double rho_f, p_mod_A, p_mod_f, p_mod_phi;
+    int32_t bc_xn, bc_xp, bc_yn, bc_yp, bc_bot, bc_top;
+    int32_t free_slip_bot, free_slip_top;
+    double bc_bot_flux, bc_top_flux;
+    int32_t *p_constant;      /* cells */
+    double tolerance;
+    uint32_t maxiter, ndem;
+    double c_phi;
+    double *f_p;              /* np*3 */
+    double beta_f, k_c;
+
+    int32_t *color;           /* np */
+} sph_state;
+
+/* Options of sph_run: the CLI flags of src/main.cpp:45-163 that affect a run. */
+typedef struct sph_run_opts {
+    const char *input_path;   /* input/<sid>.bin (relative to cwd)          */
+    const char *output_dir;   /* "output" in the reference layout           */
+    int32_t fluid;            /* --fluid                                    */
+    int32_t verbose;          /* 0 = -q                                     */
+    int32_t dry;              /* --dry: read + check only                   */
+    int32_t device;           /* -d N (negative: device 0)                  */
+} sph_run_opts;
+
+/* Darcy solver statistics since the last sph_load_* call. */
+typedef struct sph_darcy_stats {
+    uint64_t fluid_steps;     /* Jacobi solves performed                     */
+    uint64_t sweeps;          /* total Jacobi sweeps                          */
+    uint64_t nonconverged;    /* solves that hit maxiter (warning, not error) */
+    uint32_t last_nijac;      /* nijac at the last convergence check          */
+    uint32_t pad;
+    double last_max_residual; /* max |norm| at the last check                 */
+    double solve_ms;          /* device time spent in Jacobi loops            */
+} sph_darcy_stats;
+
+/* Kernel identifiers for sph_kernel_time. */
+#define SPH_K_HASH 0
+#define SPH_K_SORT 1
+#define SPH_K_REORDER 2
+#define SPH_K_CONTACT 3
+#define SPH_K_BONDS 4
+#define SPH_K_INTEGRATE 5
+#define SPH_K_WALLS 6
+#define SPH_K_DARCY_SETUP 7
+#define SPH_K_DARCY_SWEEP 8
+#define SPH_K_COUNT 9
+
+/* Context.  Replaces DEM::DEM(..., initCuda=1, device) (src/sphere.cpp:16-95)
+ * and initializeGPU (src/device.cu:65-155); device < 0 picks device 0. */
+typedef struct sph_ctx sph_ctx;
+sph_ctx *sph_create(int device);
+/* Replaces DEM::~DEM + freeing of device memory (src/device.cu:602-664). */
+void sph_destroy(sph_ctx *ctx);
+/* Text of the last error on this context (or of sph_create failure if ctx is NULL). */
+const char *sph_last_error(const sph_ctx *ctx);
+
+/* Load a state: replaces DEM::readbin + checkValues + transfer to the device
+ * (src/file_io.cpp:30-417, src/sphere.cpp:145-345, src/device.cu:667-752).
+ * Resets the time-loop counters (iter = 0, as every reference invocation). */
+int sph_load_state(sph_ctx *ctx, const sph_state *s);
+int sph_load_file(sph_ctx *ctx, const char *path, int fluid);
+
+/* Header of the loaded state (scalars only; array pointers set to NULL), so
+ * that a caller can size buffers for sph_read_state. */
+int sph_get_header(sph_ctx *ctx, sph_state *hdr);
+/* Copy the device state into caller buffers (any NULL array is skipped):
+ * replaces transferFromGlobalDeviceMemory (src/device.cu:754-829). */
+int sph_read_state(sph_ctx *ctx, sph_state *out);
+/* Write the current state as a v2.15 file: replaces DEM::writebin
+ * (src/file_io.cpp:420-731). */
+int sph_write_file(sph_ctx *ctx, const char *path);
+
+/* Advance n DEM time steps (one pass of the loop body of
+ * src/device.cu:1038-2545 each, including the Darcy solve when fluid). */
+int sph_step(sph_ctx *ctx, int nsteps);
+/* Full run: replaces main() + DEM::startTime (src/main.cpp:28-225,
+ * src/device.cu:833-2834): outputs <dir>/<sid>.outputNNNNN.bin, status.dat,
+ * <sid>-conv.log exactly as the reference names and formats them. */
+int sph_run(const sph_run_opts *opts);
+
+/* Diagnostics for bit-exact parity (the reference keeps these on the
+ * device: src/device.cu:1066-1174).  Any NULL pointer is skipped.
+ *   cellid, index: np each (the sorted key/value pairs of the last step)
+ *   cell_start, cell_end: num[0]*num[1]*num[2] each (0xffffffff = empty)  */
+int sph_read_sort(sph_ctx *ctx, uint32_t *cellid, uint32_t *index,
+                  uint32_t *cell_start, uint32_t *cell_end);
+/* Per-contact history (src/device.cu:436-439 layout, orig index * NC + slot):
+ *   contacts: np*NC (empty slot = np), distmod: np*NC*3, delta_t: np*NC*3 */
+int sph_read_contacts(sph_ctx *ctx, uint32_t *contacts, double *distmod,
+                      double *delta_t);
+/* Restore contact history (a restart side car; absent from the v2.15 file). */
+int sph_write_contacts(sph_ctx *ctx, const uint32_t *contacts,
+                       const double *distmod, const double *delta_t);
+/* Number of occupied contact slots summed over all particles. */
+int sph_count_contacts(sph_ctx *ctx, uint64_t *ncontacts);
+
+int sph_get_darcy_stats(sph_ctx *ctx, sph_darcy_stats *out);
+
+/* Per-kernel device timing with HIP events on the context stream. */
+int sph_set_profiling(sph_ctx *ctx, int on);
+int sph_kernel_time(sph_ctx *ctx, int kernel_id, double *total_ms,
+                    uint64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHERE_HIP_H_ */

@@ -1,0 +1,8 @@
+"""MI355X (gfx950) DEM time step and Darcy pressure solve behind the Python
+``sim`` API of the sphere granular simulator.
+
+``sim`` mirrors python/sphere.py's setup, binary I/O and ``run()``;
+``Engine`` is the in-process C ABI context of libsphere_hip.so.
+"""
+from .sim import sim, status, run, cleanup, V_sphere, VERSION  # noqa: F401
+from ._abi import Engine, SphereError, load_library  # noqa: F401

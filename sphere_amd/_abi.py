@@ -1,0 +1,289 @@
+"""ctypes mirror of include/sphere_hip.h and the loader of the HIP library.
+
+The reference reaches its CUDA binary through ``subprocess`` (python/sphere.py
+:4482-4540).  Here the same run goes through the C ABI of
+``libsphere_hip.so`` in process.  The library must have been built
+(``__graft_entry__.build()``); there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsphere_hip.so")
+
+NC = 32
+MAXWALLS = 6
+
+SPH_OK = 0
+ERRORS = {
+    -1: "argument",
+    -2: "file",
+    -3: "value",
+    -4: "hip",
+    -5: "grid",
+    -6: "nan",
+    -7: "overflow",
+    -8: "state",
+}
+
+K_NAMES = ["hash", "sort", "reorder", "contact", "bonds", "integrate", "walls",
+           "darcy_setup", "darcy_sweep"]
+
+_d = ctypes.c_double
+_pd = ctypes.POINTER(ctypes.c_double)
+_pu = ctypes.POINTER(ctypes.c_uint32)
+_pi = ctypes.POINTER(ctypes.c_int32)
+
+
+class SphState(ctypes.Structure):
+    """struct sph_state (include/sphere_hip.h)."""
+    _fields_ = [
+        ("nd", ctypes.c_int32), ("np", ctypes.c_uint32),
+        ("time_dt", _d), ("time_current", _d), ("time_total", _d),
+        ("time_file_dt", _d), ("time_step_count", ctypes.c_uint32),
+        ("origo", _d * 3), ("L", _d * 3), ("num", ctypes.c_uint32 * 3),
+        ("periodic", ctypes.c_int32), ("adaptive", ctypes.c_int32),
+        ("x", _pd), ("xyzsum", _pd), ("vel", _pd), ("force", _pd),
+        ("angpos", _pd), ("angvel", _pd), ("torque", _pd),
+        ("es_dot", _pd), ("es", _pd), ("ev_dot", _pd), ("ev", _pd), ("p", _pd),
+        ("g", _d * 3), ("k_n", _d), ("k_t", _d), ("k_r", _d), ("E", _d),
+        ("gamma_n", _d), ("gamma_t", _d), ("gamma_r", _d),
+        ("mu_s", _d), ("mu_d", _d), ("mu_r", _d), ("gamma_wn", _d),
+        ("gamma_wt", _d), ("mu_ws", _d), ("mu_wd", _d), ("rho", _d),
+        ("contactmodel", ctypes.c_uint32),
+        ("kappa", _d), ("db", _d), ("V_b", _d),
+        ("nw", ctypes.c_uint32), ("wmode", ctypes.c_int32 * MAXWALLS),
+        ("w_nx", _d * (MAXWALLS * 4)), ("w_mvfd", _d * (MAXWALLS * 4)),
+        ("sigma0_A", _d), ("sigma0_f", _d), ("tau_x", _d),
+        ("lambda_bar", _d), ("nb0", ctypes.c_uint32),
+        ("sigma_b", _d), ("tau_b", _d),
+        ("bonds", _pu), ("bonds_delta_n", _pd), ("bonds_delta_t", _pd),
+        ("bonds_omega_n", _pd), ("bonds_omega_t", _pd),
+        ("fluid", ctypes.c_int32), ("cfd_solver", ctypes.c_int32), ("mu", _d),
+        ("v_f", _pd), ("p_f", _pd), ("phi", _pd), ("dphi", _pd),
+        ("rho_f", _d), ("p_mod_A", _d), ("p_mod_f", _d), ("p_mod_phi", _d),
+        ("bc_xn", ctypes.c_int32), ("bc_xp", ctypes.c_int32),
+        ("bc_yn", ctypes.c_int32), ("bc_yp", ctypes.c_int32),
+        ("bc_bot", ctypes.c_int32), ("bc_top", ctypes.c_int32),
+        ("free_slip_bot", ctypes.c_int32), ("free_slip_top", ctypes.c_int32),
+        ("bc_bot_flux", _d), ("bc_top_flux", _d),
+        ("p_constant", _pi), ("tolerance", _d),
+        ("maxiter", ctypes.c_uint32), ("ndem", ctypes.c_uint32),
+        ("c_phi", _d), ("f_p", _pd), ("beta_f", _d), ("k_c", _d),
+        ("color", _pi),
+    ]
+
+
+class SphRunOpts(ctypes.Structure):
+    _fields_ = [("input_path", ctypes.c_char_p), ("output_dir", ctypes.c_char_p),
+                ("fluid", ctypes.c_int32), ("verbose", ctypes.c_int32),
+                ("dry", ctypes.c_int32), ("device", ctypes.c_int32)]
+
+
+class SphDarcyStats(ctypes.Structure):
+    _fields_ = [("fluid_steps", ctypes.c_uint64), ("sweeps", ctypes.c_uint64),
+                ("nonconverged", ctypes.c_uint64), ("last_nijac", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32), ("last_max_residual", _d),
+                ("solve_ms", _d)]
+
+
+# Names of the array members and their (dtype, per-item width, count key).
+PARTICLE_ARRAYS = [("x", numpy.float64, 4), ("xyzsum", numpy.float64, 3),
+                   ("vel", numpy.float64, 4), ("force", numpy.float64, 3),
+                   ("angpos", numpy.float64, 3), ("angvel", numpy.float64, 3),
+                   ("torque", numpy.float64, 3), ("es_dot", numpy.float64, 1),
+                   ("es", numpy.float64, 1), ("ev_dot", numpy.float64, 1),
+                   ("ev", numpy.float64, 1), ("p", numpy.float64, 1),
+                   ("color", numpy.int32, 1)]
+BOND_ARRAYS = [("bonds", numpy.uint32, 2), ("bonds_delta_n", numpy.float64, 1),
+               ("bonds_delta_t", numpy.float64, 3), ("bonds_omega_n", numpy.float64, 1),
+               ("bonds_omega_t", numpy.float64, 3)]
+CELL_ARRAYS = [("v_f", numpy.float64, 3), ("p_f", numpy.float64, 1),
+               ("phi", numpy.float64, 1), ("dphi", numpy.float64, 1),
+               ("p_constant", numpy.int32, 1)]
+FLUID_PARTICLE_ARRAYS = [("f_p", numpy.float64, 3)]
+
+SCALARS = [f[0] for f in SphState._fields_
+           if f[0] not in {a[0] for a in PARTICLE_ARRAYS + BOND_ARRAYS + CELL_ARRAYS
+                           + FLUID_PARTICLE_ARRAYS}]
+
+
+def _ptr(arr, ctype):
+    return arr.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+_CT = {numpy.float64: ctypes.c_double, numpy.uint32: ctypes.c_uint32,
+       numpy.int32: ctypes.c_int32}
+
+
+class StateBuffers:
+    """Owns numpy arrays and a SphState that points at them."""
+
+    def __init__(self, hdr):
+        self.state = SphState()
+        for name in SCALARS:
+            setattr(self.state, name, getattr(hdr, name))
+        self.arrays = {}
+        np_ = hdr.np
+        nb = hdr.nb0
+        cells = hdr.num[0] * hdr.num[1] * hdr.num[2]
+        specs = [(a, np_) for a in PARTICLE_ARRAYS] + [(a, nb) for a in BOND_ARRAYS]
+        if hdr.fluid:
+            specs += [(a, cells) for a in CELL_ARRAYS]
+            specs += [(a, np_) for a in FLUID_PARTICLE_ARRAYS]
+        for (name, dt, w), n in specs:
+            arr = numpy.zeros(max(n * w, 1), dtype=dt)
+            self.arrays[name] = arr
+            setattr(self.state, name, _ptr(arr, _CT[dt]))
+
+    def get(self, name, n, w):
+        a = self.arrays[name][:n * w]
+        return a.reshape(n, w) if w > 1 else a
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Load libsphere_hip.so; raises if it was not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            "libsphere_hip.so is not built (%s); run __graft_entry__.build()" % p)
+    lib = ctypes.CDLL(p)
+    vp = ctypes.c_void_p
+    sig = {
+        "sph_create": (vp, [ctypes.c_int]),
+        "sph_destroy": (None, [vp]),
+        "sph_last_error": (ctypes.c_char_p, [vp]),
+        "sph_load_state": (ctypes.c_int, [vp, ctypes.POINTER(SphState)]),
+        "sph_load_file": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
+        "sph_get_header": (ctypes.c_int, [vp, ctypes.POINTER(SphState)]),
+        "sph_read_state": (ctypes.c_int, [vp, ctypes.POINTER(SphState)]),
+        "sph_write_file": (ctypes.c_int, [vp, ctypes.c_char_p]),
+        "sph_step": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sph_run": (ctypes.c_int, [ctypes.POINTER(SphRunOpts)]),
+        "sph_read_sort": (ctypes.c_int, [vp, _pu, _pu, _pu, _pu]),
+        "sph_read_contacts": (ctypes.c_int, [vp, _pu, _pd, _pd]),
+        "sph_write_contacts": (ctypes.c_int, [vp, _pu, _pd, _pd]),
+        "sph_count_contacts": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "sph_get_darcy_stats": (ctypes.c_int, [vp, ctypes.POINTER(SphDarcyStats)]),
+        "sph_set_profiling": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sph_kernel_time": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_uint64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class SphereError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("sphere_hip error %d (%s): %s" % (code, ERRORS.get(code, "?"), msg))
+        self.code = code
+
+
+class Engine:
+    """One library context (one device, one loaded state)."""
+
+    def __init__(self, device=0, lib=None):
+        self.lib = lib or load_library()
+        self.ctx = self.lib.sph_create(device)
+        if not self.ctx:
+            raise SphereError(-4, self.lib.sph_last_error(None).decode())
+
+    def _check(self, rc):
+        if rc != SPH_OK:
+            raise SphereError(rc, self.lib.sph_last_error(self.ctx).decode())
+
+    def close(self):
+        if self.ctx:
+            self.lib.sph_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, st):
+        self._check(self.lib.sph_load_state(self.ctx, ctypes.byref(st)))
+
+    def load_file(self, path, fluid):
+        self._check(self.lib.sph_load_file(self.ctx, path.encode(), int(fluid)))
+
+    def header(self):
+        h = SphState()
+        self._check(self.lib.sph_get_header(self.ctx, ctypes.byref(h)))
+        return h
+
+    def read(self):
+        buf = StateBuffers(self.header())
+        self._check(self.lib.sph_read_state(self.ctx, ctypes.byref(buf.state)))
+        return buf
+
+    def step(self, n=1):
+        self._check(self.lib.sph_step(self.ctx, int(n)))
+
+    def write_file(self, path):
+        self._check(self.lib.sph_write_file(self.ctx, path.encode()))
+
+    def read_sort(self):
+        h = self.header()
+        ncell = h.num[0] * h.num[1] * h.num[2]
+        cid = numpy.zeros(max(h.np, 1), numpy.uint32)
+        idx = numpy.zeros(max(h.np, 1), numpy.uint32)
+        cs = numpy.zeros(ncell, numpy.uint32)
+        ce = numpy.zeros(ncell, numpy.uint32)
+        self._check(self.lib.sph_read_sort(self.ctx, _ptr(cid, ctypes.c_uint32),
+                                           _ptr(idx, ctypes.c_uint32),
+                                           _ptr(cs, ctypes.c_uint32), _ptr(ce, ctypes.c_uint32)))
+        return cid[:h.np], idx[:h.np], cs, ce
+
+    def read_contacts(self):
+        h = self.header()
+        n = max(h.np, 1) * NC
+        c = numpy.zeros(n, numpy.uint32)
+        dm = numpy.zeros(n * 3, numpy.float64)
+        dt = numpy.zeros(n * 3, numpy.float64)
+        self._check(self.lib.sph_read_contacts(self.ctx, _ptr(c, ctypes.c_uint32),
+                                               _ptr(dm, ctypes.c_double), _ptr(dt, ctypes.c_double)))
+        m = h.np * NC
+        return c[:m], dm[:3 * m].reshape(-1, 3), dt[:3 * m].reshape(-1, 3)
+
+    def write_contacts(self, contacts, distmod, delta_t):
+        contacts = numpy.ascontiguousarray(contacts, numpy.uint32)
+        distmod = numpy.ascontiguousarray(distmod, numpy.float64)
+        delta_t = numpy.ascontiguousarray(delta_t, numpy.float64)
+        self._check(self.lib.sph_write_contacts(self.ctx, _ptr(contacts, ctypes.c_uint32),
+                                                _ptr(distmod, ctypes.c_double),
+                                                _ptr(delta_t, ctypes.c_double)))
+
+    def count_contacts(self):
+        n = ctypes.c_uint64(0)
+        self._check(self.lib.sph_count_contacts(self.ctx, ctypes.byref(n)))
+        return n.value
+
+    def darcy_stats(self):
+        s = SphDarcyStats()
+        self._check(self.lib.sph_get_darcy_stats(self.ctx, ctypes.byref(s)))
+        return s
+
+    def set_profiling(self, on):
+        self._check(self.lib.sph_set_profiling(self.ctx, int(on)))
+
+    def kernel_time(self, kid):
+        t = ctypes.c_double(0)
+        n = ctypes.c_uint64(0)
+        self._check(self.lib.sph_kernel_time(self.ctx, int(kid), ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value

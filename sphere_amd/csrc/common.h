@@ -1,0 +1,92 @@
+// common.h -- device-side types of the gfx950 DEM + Darcy library.
+//
+// Run parameters travel by value in the kernel-argument segment (the
+// reference keeps them in __constant__ memory, src/constants.cuh:10-19).
+// All particle rows are double4 (32 B): one row is two 16-byte loads and a
+// wave reads 2 KiB of consecutive rows as fully used cache lines.
+//
+// Every floating-point expression that has a counterpart in the reference
+// keeps the reference's operation order; the library is compiled with
+// -ffp-contract=off so no multiply-add is fused.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NCS 32                 // contact slots per particle (src/constants.h:21)
+#define EMPTY_CELL 0xffffffffu
+#define MAXW 6
+
+// Error bits raised by kernels (read by the host after a step).
+#define ERRF_GRID 1u       // particle outside the grid
+#define ERRF_OVERFLOW 2u   // more than NCS contacts
+#define ERRF_NAN 4u        // NaN Darcy residual
+
+struct Grid {
+    double origo[3];
+    double L[3];
+    uint32_t num[3];
+    int periodic;
+};
+
+struct Params {
+    double g[3];
+    double k_n, k_t, k_r, E, gamma_n, gamma_t, gamma_r;
+    double mu_s, mu_d, mu_r, gamma_wn, gamma_wt, mu_ws, mu_wd, rho;
+    uint32_t contactmodel;
+    double kappa, db, V_b;
+    double lambda_bar, sigma_b, tau_b;
+    double sigma0_A, sigma0_f;
+};
+
+struct Walls {
+    uint32_t nw;
+    int wmode[MAXW];
+};
+
+// Everything a DEM kernel needs that is constant within one step.
+struct StepArgs {
+    Grid grid;
+    Params prm;
+    Walls walls;
+    uint32_t np;
+    uint32_t nb0;
+    double dt;
+};
+
+struct DarcyArgs {
+    uint32_t nx, ny, nz;
+    double dx, dy, dz;
+    double L0, L1, L2;
+    int bc_xn, bc_xp, bc_yn, bc_yp, bc_bot, bc_top;
+    double bc_bot_flux, bc_top_flux;
+    double mu, beta_f, rho_f, k_c, c_phi;
+    uint32_t ndem;
+    double dt;
+    unsigned int wall0_iz;
+    int periodic;
+};
+
+// ------------------------------------------------------------------
+// FP64 vector helpers (component-wise, like src/vector_arithmetic.h)
+// ------------------------------------------------------------------
+struct d3 {
+    double x, y, z;
+};
+__device__ __forceinline__ d3 mk3(double x, double y, double z) { d3 r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ d3 operator*(double s, d3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ d3 operator/(d3 a, double s) { return mk3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ d3 operator-(d3 a) { return mk3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ double dot3(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double len3(d3 a) { return sqrt(dot3(a, a)); }
+__device__ __forceinline__ d3 cross3(d3 a, d3 b)
+{
+    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ d3 xyz(double4 v) { return mk3(v.x, v.y, v.z); }
+
+// Reference constants
+#define PI_REF 3.14159265358979          // src/constants.h:15
+#define PI_M 3.14159265358979323846      // M_PI, src/darcy.cuh:317

@@ -1,0 +1,1109 @@
+// dem_kernels.hip -- gfx950 kernels of one DEM time step.
+//
+// Reference pipeline (src/device.cu:1038-2545): calcParticleCellID ->
+// thrust::sort_by_key -> reorderArrays -> topology -> interact -> bondsLinear
+// -> integrate -> summation -> integrateWalls.  Here:
+//   k_hash            calcParticleCellID (src/sorting.cuh:25-51)
+//   k_radix_*         stable LSD radix sort, 8-bit digits, wave64 ballot ranking
+//   k_reorder         reorderArrays (src/sorting.cuh:56-132)
+//   k_contact         topology + interact fused per particle
+//                     (src/contactsearch.cuh:194-371, 385-726)
+//   k_bond_compute /  bondsLinear (src/cohesion.cuh:8-233) as a per-bond pass
+//   k_bond_gather     and a per-particle gather in ascending bond order
+//   k_integrate       integrate (src/integration.cuh:14-305) fused with the
+//                     256-wide block reduction of summation (:309-341)
+//   k_integrate_walls integrateWalls (src/integration.cuh:344-450)
+#include "kernels.h"
+
+// ------------------------------------------------------------------
+// A1: cell hash
+// ------------------------------------------------------------------
+
+// src/sorting.cuh:7-20.  Out-of-grid particles raise ERRF_GRID (the
+// reference converts a negative double to unsigned there).
+__device__ __forceinline__ bool cell_of(const Grid &g, double4 x, uint32_t &cell)
+{
+    double fx = floor((x.x - g.origo[0]) / (g.L[0] / g.num[0]));
+    double fy = floor((x.y - g.origo[1]) / (g.L[1] / g.num[1]));
+    double fz = floor((x.z - g.origo[2]) / (g.L[2] / g.num[2]));
+    if (!(fx >= 0.0 && fx < (double)g.num[0] && fy >= 0.0 && fy < (double)g.num[1] &&
+          fz >= 0.0 && fz < (double)g.num[2]))
+        return false;
+    uint32_t ix = (uint32_t)fx, iy = (uint32_t)fy, iz = (uint32_t)fz;
+    cell = (iz * g.num[1]) * g.num[0] + iy * g.num[0] + ix;
+    return true;
+}
+
+__global__ void __launch_bounds__(256) k_hash(StepArgs a, const double4 *__restrict__ x,
+                                              uint32_t *__restrict__ keys,
+                                              uint32_t *__restrict__ vals,
+                                              uint32_t *__restrict__ err)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.np) return;
+    uint32_t c;
+    if (!cell_of(a.grid, x[i], c)) {
+        atomicOr(err, ERRF_GRID);
+        c = 0;
+    }
+    keys[i] = c;
+    vals[i] = i;
+}
+
+void launch_hash(hipStream_t s, const StepArgs &a, const DemBuffers &b)
+{
+    uint32_t nb = (a.np + 255) / 256;
+    hipLaunchKernelGGL(k_hash, dim3(nb), dim3(256), 0, s, a, b.x, b.keys, b.vals, b.errflags);
+}
+
+// ------------------------------------------------------------------
+// A2: stable LSD radix sort (replaces thrust::sort_by_key)
+// ------------------------------------------------------------------
+// Tiles of RS_THREADS * RS_ITEMS keys, item j of thread t at tile position
+// j * RS_THREADS + t.  Per pass: (1) per-tile digit histogram, (2) one
+// exclusive scan over (digit, tile) in digit-major order, (3) a stable
+// scatter that ranks equal digits with wave ballots in position order.
+
+#define RS_THREADS 256
+#define RS_ITEMS 8
+#define RS_TILE (RS_THREADS * RS_ITEMS)
+#define RS_WAVES (RS_THREADS / 64)
+
+__global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t *__restrict__ keys,
+                                                           uint32_t n, int shift,
+                                                           uint32_t *__restrict__ hist,
+                                                           uint32_t ntiles)
+{
+    __shared__ uint32_t h[256];
+    uint32_t t = threadIdx.x;
+    h[t] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * RS_TILE;
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        uint32_t i = base + j * RS_THREADS + t;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[t * ntiles + blockIdx.x] = h[t];
+}
+
+// Exclusive scan of m entries in place, one block of 1024 threads.
+__global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t *__restrict__ a, uint32_t m)
+{
+    __shared__ uint32_t part[1024];
+    uint32_t t = threadIdx.x;
+    uint32_t chunk = (m + 1023) / 1024;
+    uint32_t lo = t * chunk, hi = lo + chunk;
+    if (hi > m) hi = m;
+    uint32_t sum = 0;
+    for (uint32_t i = lo; i < hi; ++i) sum += a[i];
+    part[t] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the 1024 partial sums
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0;
+    for (uint32_t i = lo; i < hi; ++i) {
+        uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, int shift,
+    const uint32_t *__restrict__ offs, uint32_t ntiles)
+{
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_WAVES][256];
+    uint32_t t = threadIdx.x;
+    uint32_t lane = t & 63u, wave = t >> 6;
+    run[t] = offs[t * ntiles + blockIdx.x];
+    for (int w = 0; w < RS_WAVES; ++w) wcnt[w][t] = 0;
+    __syncthreads();
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t base = blockIdx.x * RS_TILE;
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        uint32_t i = base + j * RS_THREADS + t;
+        bool valid = i < n;
+        uint32_t key = valid ? kin[i] : 0u;
+        uint32_t val = valid ? vin[i] : 0u;
+        uint32_t d = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            uint64_t bb = __ballot((d >> bit) & 1u);
+            peers &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        uint32_t rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][d];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < RS_WAVES; ++w) {
+            add += wcnt[w][t];
+            wcnt[w][t] = 0;
+        }
+        run[t] += add;
+        __syncthreads();
+    }
+}
+
+void launch_sort(hipStream_t s, uint32_t n, int bits, DemBuffers &b)
+{
+    if (n == 0) return;
+    uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+    int passes = (bits + 7) / 8;
+    if (passes < 1) passes = 1;
+    for (int pass = 0; pass < passes; ++pass) {
+        int shift = pass * 8;
+        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(RS_THREADS), 0, s, b.keys, n, shift,
+                           b.hist, ntiles);
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, b.hist, 256u * ntiles);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(RS_THREADS), 0, s, b.keys, b.vals,
+                           b.keys_tmp, b.vals_tmp, n, shift, b.hist, ntiles);
+        uint32_t *tk = b.keys;
+        b.keys = b.keys_tmp;
+        b.keys_tmp = tk;
+        uint32_t *tv = b.vals;
+        b.vals = b.vals_tmp;
+        b.vals_tmp = tv;
+    }
+}
+
+// ------------------------------------------------------------------
+// A3: reorder + cellStart/cellEnd
+// ------------------------------------------------------------------
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f64(double *__restrict__ p, size_t n, double v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
+void launch_fill_f64(hipStream_t s, double *p, size_t n, double v)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
+// src/sorting.cuh:56-132
+__global__ void __launch_bounds__(256) k_reorder(
+    uint32_t np, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+    const double4 *__restrict__ x, const double4 *__restrict__ vel,
+    const double4 *__restrict__ angvel, double4 *__restrict__ x_s, double4 *__restrict__ vel_s,
+    double4 *__restrict__ angvel_s, uint32_t *__restrict__ cell_start,
+    uint32_t *__restrict__ cell_end)
+{
+    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= np) return;
+    uint32_t c = keys[idx];
+    if (idx == 0 || c != keys[idx - 1]) {
+        cell_start[c] = idx;
+        if (idx > 0) cell_end[keys[idx - 1]] = idx;
+    }
+    if (idx == np - 1) cell_end[c] = idx + 1;
+    uint32_t s = vals[idx];
+    x_s[idx] = x[s];
+    vel_s[idx] = vel[s];
+    angvel_s[idx] = angvel[s];
+}
+
+void launch_reorder(hipStream_t s, const StepArgs &a, const DemBuffers &b, uint32_t ncells)
+{
+    launch_fill_u32(s, b.cell_start, ncells, EMPTY_CELL);
+    launch_fill_u32(s, b.cell_end, ncells, EMPTY_CELL);
+    uint32_t nb = (a.np + 255) / 256;
+    hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(256), 0, s, a.np, b.keys, b.vals, b.x, b.vel,
+                       b.angvel, b.x_s, b.vel_s, b.angvel_s, b.cell_start, b.cell_end);
+}
+
+// ------------------------------------------------------------------
+// A4: periodic images (src/contactsearch.cuh:11-83)
+// ------------------------------------------------------------------
+// code bits 0-1: x image (1: +L0, 2: -L0), bits 2-3: y image.
+
+__device__ __forceinline__ int dist_mod(const Grid &g, int &tx, int &ty, int tz, d3 &dm,
+                                        uint32_t &code)
+{
+    int nx = (int)g.num[0], ny = (int)g.num[1], nz = (int)g.num[2];
+    if (g.periodic == 1) {
+        if (tx < 0) { tx += nx; dm = dm + mk3(g.L[0], 0.0, 0.0); code |= 1u; }
+        if (tx >= nx) { tx -= nx; dm = dm - mk3(g.L[0], 0.0, 0.0); code |= 2u; }
+        if (ty < 0) { ty += ny; dm = dm + mk3(0.0, g.L[1], 0.0); code |= 4u; }
+        if (ty >= ny) { ty -= ny; dm = dm - mk3(0.0, g.L[1], 0.0); code |= 8u; }
+    }
+    if (g.periodic == 2) {
+        if (tx < 0) { tx += nx; dm = dm + mk3(g.L[0], 0.0, 0.0); code |= 1u; }
+        if (tx >= nx) { tx -= nx; dm = dm - mk3(g.L[0], 0.0, 0.0); code |= 2u; }
+        if (ty < 0 || ty >= ny) return -1;
+    }
+    if (g.periodic > 2 || g.periodic < 1) {
+        if (tx < 0 || tx >= nx) return -1;
+        if (ty < 0 || ty >= ny) return -1;
+    }
+    if (tz < 0 || tz >= nz) return -1;
+    return 0;
+}
+
+__device__ __forceinline__ d3 decode_dm(const Grid &g, uint32_t code)
+{
+    d3 dm = mk3(0.0, 0.0, 0.0);
+    if (code & 1u) dm = dm + mk3(g.L[0], 0.0, 0.0);
+    if (code & 2u) dm = dm - mk3(g.L[0], 0.0, 0.0);
+    if (code & 4u) dm = dm + mk3(0.0, g.L[1], 0.0);
+    if (code & 8u) dm = dm - mk3(0.0, g.L[1], 0.0);
+    return dm;
+}
+
+// ------------------------------------------------------------------
+// A7-A10: contact laws
+// ------------------------------------------------------------------
+
+struct Acc {
+    d3 F, T;
+    double es_dot, ev_dot, p;
+};
+
+// contactLinear, src/contactmodels.cuh:241-408
+__device__ __forceinline__ void law_linear(const StepArgs &a, Acc &A, d3 vel_a, d3 angvel_a,
+                                           double4 vb, double4 wb, double r_a, double r_b,
+                                           d3 x, double x_length, double delta, double *dts)
+{
+    const Params &P = a.prm;
+    const double dt = a.dt;
+    d3 vel_b = xyz(vb), angvel_b = xyz(wb);
+    d3 delta_t0_uncor = mk3(dts[0], dts[1], dts[2]);
+    d3 n = x / x_length;
+    d3 vel_linear = vel_a - vel_b;
+    d3 vel = (vel_linear + (r_a + delta / 2.0) * cross3(n, angvel_a)) +
+             (r_b + delta / 2.0) * cross3(n, angvel_b);
+    double vel_n = -dot3(vel_linear, n);
+    d3 vel_t = vel - dot3(n, vel) * n;
+    double vel_t_length = len3(vel_t);
+    d3 delta_t0 = delta_t0_uncor - dot3(n, delta_t0_uncor) * n;
+    double delta_t0_length = len3(delta_t0);
+    d3 delta_t = delta_t0;
+    double k_n = P.k_n, k_t = P.k_t;
+    if (P.E > .001) {
+        k_n = 3.141592654 / 2.0 * P.E * (r_a + r_b) / 2.;
+        k_t = k_n;
+    }
+    d3 f_n = fmax(0.0, -k_n * delta + P.gamma_n * vel_n) * n;
+    double f_n_length = len3(f_n);
+    A.ev_dot += 0.5 * P.gamma_n * vel_n * vel_n;
+    d3 f_c = (-P.kappa * sqrt(r_a * r_b)) * n;
+    d3 f_t = mk3(0.0, 0.0, 0.0);
+    if (delta_t0_length > 0.0 || vel_t_length > 0.0) {
+        delta_t = delta_t0 + dt * vel_t;
+        d3 f_t_elast = (-k_t) * delta_t;
+        d3 f_t_visc = (-P.gamma_t) * vel_t;
+        f_t = f_t_elast + f_t_visc;
+        double f_t_length = len3(f_t);
+        double f_t_limit = P.mu_s * len3(f_n - f_c);
+        A.ev_dot += 0.5 * P.gamma_t * vel_t_length * vel_t_length;
+        if (f_t_length > f_t_limit) {
+            d3 t = f_t / len3(f_t);
+            f_t = f_t_limit * t;
+            delta_t = (-1.0 / k_t) * ((P.mu_d * len3(f_n - f_c)) * t + P.gamma_t * vel_t);
+            A.es_dot += 0.5 * len3(dt * (len3(f_t) * vel_t)) / dt;
+        }
+    }
+    A.F = A.F + ((f_n + f_t) + f_c);
+    A.T = A.T + cross3((-(r_a + delta * 0.5)) * n, f_t);
+    A.p += f_n_length / (4.0 * PI_REF * r_a * r_a);
+    dts[0] = delta_t.x;
+    dts[1] = delta_t.y;
+    dts[2] = delta_t.z;
+}
+
+// contactmodel 3, Hertz-Mindlin: the structure of contactHertz
+// (src/contactmodels.cuh:414-591) on the overlap magnitude d = -delta with
+// d^(3/2) = d*sqrt(d) and d^(1/4) = sqrt(sqrt(d)).  With E > 0.001 (Poisson's
+// ratio nu in the k_r field): E* = E/(2(1-nu^2)), R* = r_a r_b/(r_a+r_b),
+// G* = E/(2(1+nu)) / (2(2-nu)), k_n = 4/3 E* sqrt(R*), k_t = 8 G* sqrt(R*).
+__device__ __forceinline__ void law_hertz(const StepArgs &a, Acc &A, d3 vel_a, d3 angvel_a,
+                                          double4 vb, double4 wb, double r_a, double r_b,
+                                          d3 x_ab, double x_ab_length, double delta_ab,
+                                          double *dts)
+{
+    const Params &P = a.prm;
+    const double dt = a.dt;
+    d3 vel_b = xyz(vb), angvel_b = xyz(wb);
+    d3 delta_t0_uncor = mk3(dts[0], dts[1], dts[2]);
+    d3 n_ab = x_ab / x_ab_length;
+    d3 vel_ab_linear = vel_a - vel_b;
+    d3 vel_ab = (vel_ab_linear + (r_a + delta_ab / 2.0) * cross3(n_ab, angvel_a)) +
+                (r_b + delta_ab / 2.0) * cross3(n_ab, angvel_b);
+    d3 angvel_ab = angvel_a - angvel_b;
+    double angvel_ab_length = len3(angvel_ab);
+    double vel_n_ab = dot3(vel_ab_linear, n_ab);
+    d3 vel_t_ab = vel_ab - dot3(vel_ab, n_ab) * n_ab;
+    double vel_t_ab_length = len3(vel_t_ab);
+    d3 delta_t0 = delta_t0_uncor - dot3(delta_t0_uncor, n_ab) * n_ab;
+    double delta_t0_length = len3(delta_t0);
+    d3 delta_t = delta_t0;
+    double d = -delta_ab;
+    double sd = sqrt(d);
+    double qd = sqrt(sd);
+    double k_n = P.k_n, k_t = P.k_t;
+    if (P.E > .001) {
+        double nu = P.k_r;
+        double E_star = P.E / (2.0 * (1.0 - nu * nu));
+        double R_star = r_a * r_b / (r_a + r_b);
+        double G = P.E / (2.0 * (1.0 + nu));
+        double G_star = G / (2.0 * (2.0 - nu));
+        double sR = sqrt(R_star);
+        k_n = 4.0 / 3.0 * E_star * sR;
+        k_t = 8.0 * G_star * sR;
+    }
+    d3 f_n = fmax(0.0, k_n * d * sd - P.gamma_n * qd * vel_n_ab) * n_ab;
+    A.ev_dot += 0.5 * P.gamma_n * qd * vel_n_ab * vel_n_ab;
+    double f_n_length = len3(f_n);
+    d3 f_c = (-P.kappa * sqrt(r_a * r_b)) * n_ab;
+    d3 f_t = mk3(0.0, 0.0, 0.0);
+    d3 T_res = mk3(0.0, 0.0, 0.0);
+    if (delta_t0_length > 0.0 || vel_t_ab_length > 0.0) {
+        d3 f_t_elast = (-k_t * sd) * delta_t0;
+        d3 f_t_visc = (-P.gamma_t * qd) * vel_t_ab;
+        double f_t_limit;
+        if (vel_t_ab_length > (double)0.001f)
+            f_t_limit = P.mu_d * len3(f_n - f_c);
+        else
+            f_t_limit = P.mu_s * len3(f_n - f_c);
+        f_t = f_t_elast + f_t_visc;
+        double f_t_length = len3(f_t);
+        A.ev_dot += 0.5 * P.gamma_t * qd * vel_t_ab_length * vel_t_ab_length;
+        if (f_t_length > f_t_limit) {
+            f_t = (f_t_limit / f_t_length) * f_t;
+            delta_t = (f_t + (P.gamma_t * qd) * vel_t_ab) / (-k_t * sd);
+            A.es_dot += 0.5 * len3(dt * (len3(f_t) * vel_t_ab)) / dt;
+        } else {
+            delta_t = delta_t0 + dt * vel_t_ab;
+        }
+    }
+    if (angvel_ab_length > 0.0) {
+        double m = fmin(P.gamma_r * r_a * angvel_ab_length, P.mu_r * r_a * f_n_length);
+        T_res = (-1.0 * m) * (angvel_ab / angvel_ab_length);
+    }
+    A.F = A.F + ((f_n + f_t) + f_c);
+    A.T = A.T + ((-(r_a + delta_ab / 2.0)) * cross3(n_ab, f_t) + T_res);
+    A.p += f_n_length / (4.0 * PI_REF * r_a * r_a);
+    dts[0] = delta_t.x;
+    dts[1] = delta_t.y;
+    dts[2] = delta_t.z;
+}
+
+// contactLinearViscous, src/contactmodels.cuh:119-237 (model 1)
+__device__ __forceinline__ void law_linear_viscous(const StepArgs &a, Acc &A, double4 va4,
+                                                   double4 vb4, double4 wa4, double4 wb4,
+                                                   double r_a, double r_b, d3 x_ab,
+                                                   double x_ab_length, double delta_ab,
+                                                   double kappa)
+{
+    const Params &P = a.prm;
+    d3 vel_a = xyz(va4), vel_b = xyz(vb4), angvel_a = xyz(wa4), angvel_b = xyz(wb4);
+    d3 n_ab = x_ab / x_ab_length;
+    d3 vel_ab_linear = vel_a - vel_b;
+    d3 vel_ab = (vel_ab_linear + (r_a + delta_ab / 2.0) * cross3(n_ab, angvel_a)) +
+                (r_b + delta_ab / 2.0) * cross3(n_ab, angvel_b);
+    double vel_n_ab = dot3(vel_ab_linear, n_ab);
+    d3 vel_t_ab = vel_ab - dot3(vel_ab, n_ab) * n_ab;
+    double vel_t_ab_length = len3(vel_t_ab);
+    double k_n = P.k_n;
+    if (P.E > .001) k_n = 3.141592654 / 2.0 * P.E * (r_a + r_b) / 2.;
+    d3 f_n = fmax(0.0, -k_n * delta_ab - P.gamma_n * vel_n_ab) * n_ab;
+    if (dot3(f_n, n_ab) < 0.0) f_n = mk3(0.0, 0.0, 0.0);
+    double f_n_length = len3(f_n);
+    d3 f_c = (-kappa * sqrt(r_a * r_b)) * n_ab;
+    d3 f_t = mk3(0.0, 0.0, 0.0);
+    if (vel_t_ab_length > 0.0) {
+        double f_t_visc = P.gamma_t * vel_t_ab_length;
+        double f_t_limit;
+        if (vel_t_ab_length > (double)0.001f)
+            f_t_limit = P.mu_d * len3(f_n - f_c);
+        else
+            f_t_limit = P.mu_s * len3(f_n - f_c);
+        if (f_t_visc < f_t_limit)
+            f_t = ((-f_t_visc) * vel_t_ab) / vel_t_ab_length;
+        else
+            f_t = ((-f_t_limit) * vel_t_ab) / vel_t_ab_length;
+    }
+    A.F = A.F + ((f_n + f_t) + f_c);
+    A.T = A.T + (-(r_a + delta_ab / 2.0)) * cross3(n_ab, f_t);
+    A.p += f_n_length / (4.0 * PI_REF * r_a * r_a);
+}
+
+// capillaryCohesion_exp, src/cohesion.cuh:237-277
+__device__ __forceinline__ void law_capillary(const StepArgs &a, Acc &A, double r_a, double r_b,
+                                              double delta_ab, d3 x_ab, double x_ab_length,
+                                              double kappa)
+{
+    d3 n_ab = x_ab / x_ab_length;
+    double r;
+    if ((r_a / r_b) > (r_b / r_a)) r = r_a / r_b;
+    else r = r_b / r_a;
+    double hh = -sqrt(r);
+    double R_har = (2.0 * r_a * r_b) / (r_a + r_b);
+    double R_geo = sqrt(r_a * r_b);
+    double lambda = (double)0.9f * hh * sqrt(a.prm.V_b / R_har);
+    d3 f_c = (-kappa * R_geo * exp(-delta_ab / lambda)) * n_ab;
+    A.F = A.F + f_c;
+}
+
+// contactLinear_wall, src/contactmodels.cuh:10-114
+__device__ __forceinline__ double law_wall(const StepArgs &a, Acc &A, double4 va4, double4 wa4,
+                                           double radius_a, d3 n, double delta, double wvel)
+{
+    const Params &P = a.prm;
+    const double dt = a.dt;
+    d3 vel_linear = xyz(va4), angvel = xyz(wa4);
+    d3 vel = vel_linear + (radius_a + delta / 2.0) * cross3(n, angvel);
+    vel = mk3(vel.x + wvel, vel.y + wvel, vel.z + wvel);
+    double vel_n = dot3(vel_linear, n);
+    d3 vel_t = vel - dot3(n, vel) * n;
+    double vel_t_length = len3(vel_t);
+    double k_n = P.k_n;
+    if (P.E > .001) k_n = 3.141592654 / 2.0 * P.E * radius_a;
+    d3 f_n = fmax(0.0, -k_n * delta - P.gamma_wn * vel_n) * n;
+    double f_n_length = len3(f_n);
+    A.ev_dot += P.gamma_wn * vel_n * vel_n;
+    d3 f_t = mk3(0.0, 0.0, 0.0);
+    if (vel_t_length > 0.0 && P.gamma_wt > 0.0) {
+        d3 f_t_visc = P.gamma_wt * vel_t;
+        double f_t_visc_length = len3(f_t_visc);
+        double f_t_limit;
+        if (vel_t_length > 0.0) f_t_limit = P.mu_wd * f_n_length;
+        else f_t_limit = P.mu_ws * f_n_length;
+        if (f_t_visc_length < f_t_limit) {
+            f_t = (-1.0) * f_t_visc;
+        } else {
+            f_t = ((-f_t_limit) * vel_t) / vel_t_length;
+            A.es_dot += len3(dt * (len3(f_t) * vel_t)) / dt;
+        }
+    }
+    A.F = A.F + (f_n + f_t);
+    A.T = A.T + cross3((-(radius_a + delta * 0.5)) * n, f_t);
+    A.p += f_n_length / (4.0 * PI_REF * radius_a * radius_a);
+    return dot3(f_n, n);
+}
+
+// ------------------------------------------------------------------
+// A5 + A6: contact kernel
+// ------------------------------------------------------------------
+// One thread per particle in sorted order.  Topology (slot bookkeeping of
+// new overlaps in the 27 neighbour cells) and interact (forces over the
+// slots in slot order, removal of separated contacts, walls) run back to
+// back in the same thread: a particle's slot row is private to it, so the
+// fusion is exact.  cmask holds one bit per occupied slot; empty slots keep
+// delta_t == 0, which the reference re-establishes every step.
+//
+// The search skips the sqrt of a candidate whose squared distance s is at
+// least (r_a+r_b)^2 (1+1e-9): then sqrt(s) > r_a+r_b and the reference's
+// test sqrt(s) - (r_a+r_b) < 0 is false, so the skip never changes it.
+
+__global__ void __launch_bounds__(256) k_contact(
+    StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
+    const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x,
+    const double4 *__restrict__ x_s, const double4 *__restrict__ vel_s,
+    const double4 *__restrict__ angvel_s, const double4 *__restrict__ vel,
+    const double4 *__restrict__ angvel, double4 *__restrict__ force,
+    double4 *__restrict__ torque, double *__restrict__ es_dot_out,
+    double *__restrict__ ev_dot_out, double *__restrict__ es, double *__restrict__ ev,
+    double *__restrict__ p_out, const double4 *__restrict__ w_nx,
+    const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
+    uint32_t *__restrict__ contacts, uint32_t *__restrict__ cmask,
+    uint8_t *__restrict__ dmcode, double *__restrict__ delta_t, uint32_t *__restrict__ err)
+{
+    const uint32_t idx_a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx_a >= a.np) return;
+    const Grid &g = a.grid;
+    const uint32_t np = a.np;
+    const uint32_t a_orig = vals[idx_a];
+    const double4 xa = x_s[idx_a];
+    const double radius_a = xa.w;
+    const int model = (int)a.prm.contactmodel;
+
+    Acc A;
+    A.F = mk3(0.0, 0.0, 0.0);
+    A.T = mk3(0.0, 0.0, 0.0);
+    A.es_dot = 0.0;
+    A.ev_dot = 0.0;
+    A.p = 0.0;
+
+    const double4 va4 = vel_s[idx_a];
+    const double4 wa4 = angvel_s[idx_a];
+    // contactsearch.cuh:352-354
+    const int gx = (int)floor((xa.x - g.origo[0]) / (g.L[0] / g.num[0]));
+    const int gy = (int)floor((xa.y - g.origo[1]) / (g.L[1] / g.num[1]));
+    const int gz = (int)floor((xa.z - g.origo[2]) / (g.L[2] / g.num[2]));
+    // a particle outside the grid makes the step invalid (k_hash flags it);
+    // its neighbour search is skipped so that no cell index leaves the grid
+    const bool inside = gx >= 0 && gx < (int)g.num[0] && gy >= 0 && gy < (int)g.num[1] &&
+                        gz >= 0 && gz < (int)g.num[2];
+    if (!inside) atomicOr(err, ERRF_GRID);
+
+    if (inside && (model == 2 || model == 3)) {
+        uint32_t *row = contacts + (size_t)a_orig * NCS;
+        uint8_t *crow = dmcode + (size_t)a_orig * NCS;
+        uint32_t mask = cmask[a_orig];
+        int nc = 0;
+        bool ovf = false;
+        // topology, contactsearch.cuh:358-369 (z, y, x order)
+        for (int zd = -1; zd < 2; ++zd)
+            for (int yd = -1; yd < 2; ++yd)
+                for (int xd = -1; xd < 2; ++xd) {
+                    int tx = gx + xd, ty = gy + yd, tz = gz + zd;
+                    d3 dm = mk3(0.0, 0.0, 0.0);
+                    uint32_t code = 0;
+                    if (dist_mod(g, tx, ty, tz, dm, code) == -1) continue;
+                    uint32_t cid = (uint32_t)tx + (uint32_t)ty * g.num[0] +
+                                   (g.num[0] * g.num[1]) * (uint32_t)tz;
+                    uint32_t start = cell_start[cid];
+                    if (start == EMPTY_CELL) continue;
+                    uint32_t end = cell_end[cid];
+                    for (uint32_t idx_b = start; idx_b < end; ++idx_b) {
+                        if (idx_b == idx_a) continue;
+                        double4 xb = x_s[idx_b];
+                        d3 x_ab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
+                        x_ab = x_ab + dm;
+                        double s2 = dot3(x_ab, x_ab);
+                        double rsum = radius_a + xb.w;
+                        if (s2 >= rsum * rsum * (1.0 + 1e-9)) continue;
+                        double delta = sqrt(s2) - rsum;
+                        if (!(delta < 0.0)) continue;
+                        uint32_t b_orig = vals[idx_b];
+                        // slot rule of contactsearch.cuh:267-286
+                        int cpos = -1;
+                        uint32_t occ = mask;
+                        while (occ) {
+                            int i = __ffs(occ) - 1;
+                            occ &= occ - 1;
+                            if (row[i] == b_orig) { cpos = i; break; }
+                        }
+                        if (cpos < 0) {
+                            uint32_t emp = ~mask;
+                            cpos = emp ? 31 - __clz(emp) : nc;
+                        }
+                        if (cpos < NCS) {
+                            row[cpos] = b_orig;
+                            crow[cpos] = (uint8_t)code;
+                            mask |= 1u << cpos;
+                        } else {
+                            ovf = true;
+                        }
+                        ++nc;
+                        if (nc > NCS) break;
+                    }
+                }
+        if (ovf) atomicOr(err, ERRF_OVERFLOW);
+
+        // interact, contactsearch.cuh:484-559: slots in ascending order
+        const d3 vel_a = xyz(va4), angvel_a = xyz(wa4);
+        uint32_t occ = mask;
+        while (occ) {
+            int i = __ffs(occ) - 1;
+            occ &= occ - 1;
+            size_t mempos = (size_t)a_orig * NCS + i;
+            uint32_t b = row[i];
+            d3 dm = decode_dm(g, crow[i]);
+            double4 xb = x[b];
+            double radius_b = xb.w;
+            d3 x_ab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
+            double l = len3(x_ab);
+            double delta_n = l - (radius_a + radius_b);
+            double *dts = delta_t + 3 * mempos;
+            if (delta_n < 0.0) {
+                double loc[3] = {dts[0], dts[1], dts[2]};
+                if (model == 2)
+                    law_linear(a, A, vel_a, angvel_a, vel[b], angvel[b], radius_a, radius_b,
+                               x_ab, l, delta_n, loc);
+                else
+                    law_hertz(a, A, vel_a, angvel_a, vel[b], angvel[b], radius_a, radius_b,
+                              x_ab, l, delta_n, loc);
+                dts[0] = loc[0];
+                dts[1] = loc[1];
+                dts[2] = loc[2];
+            } else {
+                row[i] = np;
+                dts[0] = 0.0;
+                dts[1] = 0.0;
+                dts[2] = 0.0;
+                mask &= ~(1u << i);
+            }
+        }
+        cmask[a_orig] = mask;
+    } else if (inside && model == 1) {
+        // findAndProcessContactsInCell, contactsearch.cuh:91-186
+        for (int zd = -1; zd < 2; ++zd)
+            for (int yd = -1; yd < 2; ++yd)
+                for (int xd = -1; xd < 2; ++xd) {
+                    int tx = gx + xd, ty = gy + yd, tz = gz + zd;
+                    d3 dm = mk3(0.0, 0.0, 0.0);
+                    uint32_t code = 0;
+                    if (dist_mod(g, tx, ty, tz, dm, code) == -1) continue;
+                    uint32_t cid = (uint32_t)tx + (uint32_t)ty * g.num[0] +
+                                   (g.num[0] * g.num[1]) * (uint32_t)tz;
+                    uint32_t start = cell_start[cid];
+                    if (start == EMPTY_CELL) continue;
+                    uint32_t end = cell_end[cid];
+                    for (uint32_t idx_b = start; idx_b < end; ++idx_b) {
+                        if (idx_b == idx_a) continue;
+                        double4 xb = x_s[idx_b];
+                        double radius_b = xb.w;
+                        double kappa = a.prm.kappa;
+                        d3 x_ab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z) + dm;
+                        double l = len3(x_ab);
+                        double delta_ab = l - (radius_a + radius_b);
+                        if (delta_ab < 0.0)
+                            law_linear_viscous(a, A, va4, vel_s[idx_b], wa4, angvel_s[idx_b],
+                                               radius_a, radius_b, x_ab, l, delta_ab, kappa);
+                        else if (delta_ab < a.prm.db)
+                            law_capillary(a, A, radius_a, radius_b, delta_ab, x_ab, l, kappa);
+                    }
+                }
+    }
+
+    // walls, contactsearch.cuh:420-701
+    double4 wn[5] = {make_double4(0.0, 0.0, -1.0, g.L[2]), make_double4(-1.0, 0.0, 0.0, g.L[0]),
+                     make_double4(1.0, 0.0, 0.0, 0.0), make_double4(0.0, -1.0, 0.0, g.L[1]),
+                     make_double4(0.0, 1.0, 0.0, 0.0)};
+    double wv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const uint32_t nw = a.walls.nw;
+    for (uint32_t w = 0; w < nw && w < 5; ++w) {
+        wn[w] = w_nx[w];
+        wv[w] = w_mvfd[w].y;
+    }
+    double wf[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double delta_w;
+    delta_w = wn[0].w - (xa.z + radius_a);
+    if (delta_w < 0.0) wf[0] = law_wall(a, A, va4, wa4, radius_a, xyz(wn[0]), delta_w, wv[0]);
+    delta_w = xa.z - radius_a - g.origo[2];
+    if (delta_w < 0.0) (void)law_wall(a, A, va4, wa4, radius_a, mk3(0.0, 0.0, 1.0), delta_w, 0.0);
+    if (g.periodic == 0) {
+        delta_w = wn[1].w - (xa.x + radius_a);
+        if (delta_w < 0.0) wf[1] = law_wall(a, A, va4, wa4, radius_a, xyz(wn[1]), delta_w, wv[1]);
+        delta_w = xa.x - radius_a - wn[2].w;
+        if (delta_w < 0.0) wf[2] = law_wall(a, A, va4, wa4, radius_a, xyz(wn[2]), delta_w, wv[2]);
+    }
+    if (g.periodic == 0 || g.periodic == 2) {
+        delta_w = wn[3].w - (xa.y + radius_a);
+        if (delta_w < 0.0) wf[3] = law_wall(a, A, va4, wa4, radius_a, xyz(wn[3]), delta_w, wv[3]);
+        delta_w = xa.y - radius_a - wn[4].w;
+        if (delta_w < 0.0) wf[4] = law_wall(a, A, va4, wa4, radius_a, xyz(wn[4]), delta_w, wv[4]);
+    }
+
+    // contactsearch.cuh:706-724
+    force[a_orig] = make_double4(A.F.x, A.F.y, A.F.z, 0.0);
+    torque[a_orig] = make_double4(A.T.x, A.T.y, A.T.z, 0.0);
+    es_dot_out[a_orig] = A.es_dot;
+    ev_dot_out[a_orig] = A.ev_dot;
+    es[a_orig] += A.es_dot * a.dt;
+    ev[a_orig] += A.ev_dot * a.dt;
+    p_out[a_orig] = A.p;
+    for (uint32_t w = 0; w < nw && w < 5; ++w) walls_force_pp[a_orig + (size_t)np * w] = wf[w];
+}
+
+void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b)
+{
+    uint32_t nb = (a.np + 255) / 256;
+    hipLaunchKernelGGL(k_contact, dim3(nb), dim3(256), 0, s, a, b.vals, b.cell_start, b.cell_end,
+                       b.x, b.x_s, b.vel_s, b.angvel_s, b.vel, b.angvel, b.force, b.torque,
+                       b.es_dot, b.ev_dot, b.es, b.ev, b.p, b.w_nx, b.w_mvfd, b.walls_force_pp,
+                       b.contacts, b.cmask, b.dmcode, b.delta_t, b.errflags);
+}
+
+// ------------------------------------------------------------------
+// A11: bonds
+// ------------------------------------------------------------------
+
+// bondsLinear body, src/cohesion.cuh:24-217, one thread per bond
+__global__ void __launch_bounds__(256) k_bond_compute(
+    StepArgs a, const uint2 *__restrict__ bonds, uint8_t *__restrict__ broken,
+    double4 *__restrict__ bdelta, double4 *__restrict__ bomega, const double4 *__restrict__ x,
+    const double4 *__restrict__ vel, const double4 *__restrict__ angvel,
+    double4 *__restrict__ bf, double4 *__restrict__ bt, uint8_t *__restrict__ applied)
+{
+    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.nb0) return;
+    applied[idx] = 0;
+    if (broken[idx]) return;
+    const Params &P = a.prm;
+    const double dt = a.dt;
+    const uint2 bond = bonds[idx];
+    const double4 d0 = bdelta[idx], o0 = bomega[idx];
+    d3 delta0_t = xyz(d0);
+    const double delta0_n = d0.w;
+    d3 omega0_t = xyz(o0);
+    const double omega0_n = o0.w;
+    const double4 x_i = x[bond.x], x_j = x[bond.y];
+    const d3 vel_i = xyz(vel[bond.x]), vel_j = xyz(vel[bond.y]);
+    const d3 angvel_i = xyz(angvel[bond.x]), angvel_j = xyz(angvel[bond.y]);
+    const double R_bar = P.lambda_bar * fmin(x_i.w, x_j.w);
+    const double Ab = PI_REF * R_bar * R_bar;
+    const double I = 0.25 * PI_REF * R_bar * R_bar * R_bar * R_bar;
+    const double J = I * 2.0;
+    const d3 xv = mk3(x_i.x - x_j.x, x_i.y - x_j.y, x_i.z - x_j.z);
+    const double x_length = len3(xv);
+    const double overlap = fmin(0.0, x_length - (x_i.w + x_j.w));
+    const d3 n = xv / x_length;
+    delta0_t = delta0_t - dot3(n, delta0_t) * n;
+    const d3 ddelta = dt * (((vel_i - vel_j) + (x_i.w + overlap / 2.0) * cross3(n, angvel_i)) +
+                            (x_j.w + overlap / 2.0) * cross3(n, angvel_j));
+    const double ddelta_n = -dot3(ddelta, n);
+    const double delta_n = delta0_n + ddelta_n;
+    const d3 ddelta_t = ddelta - dot3(n, ddelta) * n;
+    const d3 delta_t = delta0_t + ddelta_t;
+    const d3 f_n = (P.k_n * Ab * delta_n + P.gamma_n * ddelta_n / dt) * n;
+    const d3 f_t = (-P.k_t * Ab) * delta_t - (P.gamma_t * ddelta_t) / dt;
+    const d3 f = f_n + f_t;
+    const d3 mn = -n;
+    omega0_t = omega0_t - dot3(omega0_t, mn) * mn;
+    const d3 domega = dt * (angvel_j - angvel_i);
+    const double domega_n = dot3(domega, mn);
+    const double omega_n = omega0_n + domega_n;
+    const d3 domega_t = domega - domega_n * mn;
+    const d3 omega_t = omega0_t + domega_t;
+    const d3 t_n = (-P.k_t * J * omega_n) * mn;
+    const d3 t_t = (-P.k_n * I) * omega_t;
+    const d3 t_j = t_n + t_t;
+    const double sigma_max = len3(f_n) / Ab + len3(t_t) * R_bar / I;
+    const double tau_max = len3(f_t) / Ab + len3(t_n) * R_bar / J;
+    if (sigma_max >= P.sigma_b || tau_max >= P.tau_b) {
+        broken[idx] = 1;
+        return;
+    }
+    bdelta[idx] = make_double4(delta_t.x, delta_t.y, delta_t.z, delta_n);
+    bomega[idx] = make_double4(omega_t.x, omega_t.y, omega_t.z, omega_n);
+    bf[idx] = make_double4(f.x, f.y, f.z, 0.0);
+    bt[idx] = make_double4(t_j.x, t_j.y, t_j.z, 0.0);
+    applied[idx] = 1;
+}
+
+// Per bonded particle: force/torque updates of cohesion.cuh:220-230 applied in
+// ascending bond order.  ent = bond index << 1 | (1 if the particle is bond.y).
+__global__ void __launch_bounds__(256) k_bond_gather(
+    uint32_t nbp, const uint32_t *__restrict__ plist, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ ent, const double4 *__restrict__ bf,
+    const double4 *__restrict__ bt, const uint8_t *__restrict__ applied,
+    double4 *__restrict__ force, double4 *__restrict__ torque)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nbp) return;
+    uint32_t pi = plist[k];
+    double4 F = force[pi], T = torque[pi];
+    for (uint32_t e = off[k]; e < off[k + 1]; ++e) {
+        uint32_t v = ent[e];
+        uint32_t b = v >> 1;
+        if (!applied[b]) continue;
+        double4 f = bf[b], t = bt[b];
+        if ((v & 1u) == 0) {
+            F.x = F.x + f.x; F.y = F.y + f.y; F.z = F.z + f.z;
+            T.x = T.x - t.x; T.y = T.y - t.y; T.z = T.z - t.z;
+        } else {
+            F.x = F.x - f.x; F.y = F.y - f.y; F.z = F.z - f.z;
+            T.x = T.x + t.x; T.y = T.y + t.y; T.z = T.z + t.z;
+        }
+    }
+    force[pi] = F;
+    torque[pi] = T;
+}
+
+void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b)
+{
+    if (a.nb0 == 0) return;
+    hipLaunchKernelGGL(k_bond_compute, dim3((a.nb0 + 255) / 256), dim3(256), 0, s, a, b.bonds,
+                       b.bond_broken, b.bond_delta, b.bond_omega, b.x, b.vel, b.angvel, b.bond_f,
+                       b.bond_t, b.bond_applied);
+    if (b.nbp)
+        hipLaunchKernelGGL(k_bond_gather, dim3((b.nbp + 255) / 256), dim3(256), 0, s, b.nbp,
+                           b.bp_list, b.bp_off, b.bp_ent, b.bond_f, b.bond_t, b.bond_applied,
+                           b.force, b.torque);
+}
+
+// ------------------------------------------------------------------
+// A12-A13: integrate, wall force reduction, walls
+// ------------------------------------------------------------------
+
+// Block tree reduction of summation(), src/integration.cuh:309-341 (256 wide).
+__device__ __forceinline__ double block_sum_256(double v, double *cache)
+{
+    uint32_t t = threadIdx.x;
+    cache[t] = v;
+    __syncthreads();
+    for (uint32_t i = 128; i != 0; i /= 2) {
+        if (t < i) cache[t] += cache[t + i];
+        __syncthreads();
+    }
+    return cache[0];
+}
+
+// findShearStressOnFixedMovingParticles + summation, src/integration.cuh:457-484
+__global__ void __launch_bounds__(256) k_shear_stress(StepArgs a, const double4 *__restrict__ x,
+                                                      const double4 *__restrict__ vel,
+                                                      const double4 *__restrict__ force,
+                                                      double *__restrict__ tau_partial)
+{
+    __shared__ double cache[256];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double temp = 0.0;
+    if (i < a.np) {
+        double z = x[i].z, fixvel = vel[i].w;
+        double f_x = 0.0;
+        if (fixvel > 0.0 && z > a.grid.L[2] * 0.5) f_x = force[i].x;
+        temp += f_x / (a.grid.L[0] * a.grid.L[1]);
+    }
+    double s = block_sum_256(temp, cache);
+    if (threadIdx.x == 0) tau_partial[blockIdx.x] = s;
+}
+
+// Serial sum over block partials (the order of integration.cuh:64-66).
+__global__ void k_serial_sum(const double *__restrict__ part, uint32_t n, double *__restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (uint32_t i = 0; i < n; ++i) s += part[i];
+    *out = s;
+}
+
+void launch_shear_stress(hipStream_t s, const StepArgs &a, const DemBuffers &b)
+{
+    uint32_t nb = (a.np + 255) / 256;
+    hipLaunchKernelGGL(k_shear_stress, dim3(nb), dim3(256), 0, s, a, b.x, b.vel, b.force,
+                       b.tau_partial);
+    hipLaunchKernelGGL(k_serial_sum, dim3(1), dim3(64), 0, s, b.tau_partial, nb, b.tau_eff);
+}
+
+// integrate (TY3), src/integration.cuh:14-305, in original order (the
+// reference reads the *_sorted copies, which hold the same values), with the
+// wall-0 force partial of summation() reduced over the same 256-particle block.
+__global__ void __launch_bounds__(256) k_integrate(
+    StepArgs a, long iter, int change_velocity_state, double velocity_factor, double tau_x,
+    double4 *__restrict__ x, double4 *__restrict__ vel, double4 *__restrict__ angvel,
+    const double4 *__restrict__ force, const double4 *__restrict__ torque,
+    double4 *__restrict__ angpos, double4 *__restrict__ acc_a, double4 *__restrict__ angacc_a,
+    double4 *__restrict__ xyzsum_a, const double4 *__restrict__ w_mvfd,
+    const double *__restrict__ tau_eff, const double *__restrict__ walls_force_pp,
+    double *__restrict__ wall_partial)
+{
+    __shared__ double cache[256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.walls.nw > 0) {
+        double temp = 0.0;
+        if (i < a.np) temp += walls_force_pp[i];
+        double s = block_sum_256(temp, cache);
+        if (threadIdx.x == 0) wall_partial[blockIdx.x] = s;
+    }
+    if (i >= a.np) return;
+    const Grid &g = a.grid;
+    const Params &P = a.prm;
+    const double dt = a.dt;
+    int wall0mode = 0;  // 0 when nw == 0 (the reference reads it uninitialised)
+    double wall0mass = 0.0;
+    if (a.walls.nw > 0) {
+        wall0mode = a.walls.wmode[0];
+        wall0mass = w_mvfd[0].x;
+    }
+    double tau_eff_x = 0.0;
+    if (a.walls.nw > 0 && wall0mode == 3) tau_eff_x = *tau_eff;
+    const double4 f4 = force[i], t4 = torque[i], ap = angpos[i];
+    const double4 xo = x[i], vo = vel[i], wo = angvel[i];
+    const double4 xs = xyzsum_a[i];
+    double acc0[3] = {0.0, 0.0, 0.0}, angacc0[3] = {0.0, 0.0, 0.0};
+    if (iter != 0) {
+        const double4 a0 = acc_a[i], w0 = angacc_a[i];
+        acc0[0] = a0.x; acc0[1] = a0.y; acc0[2] = a0.z;
+        angacc0[0] = w0.x; angacc0[1] = w0.y; angacc0[2] = w0.z;
+    }
+    const double radius = xo.w;
+    const double m = 4.0 / 3.0 * PI_REF * radius * radius * radius * P.rho;
+    double acc[3], angacc[3];
+    acc[0] = f4.x / m + P.g[0];
+    acc[1] = f4.y / m + P.g[1];
+    acc[2] = f4.z / m + P.g[2];
+    angacc[0] = t4.x * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    angacc[1] = t4.y * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    angacc[2] = t4.z * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    if (wall0mode == 3 && vo.w > 0.0001 && xo.z > g.L[2] * 0.5) {
+        const double f_tau_x = (tau_x + tau_eff_x) * g.L[0] * g.L[1];
+        acc[0] = f_tau_x / wall0mass;
+        acc[1] = 0.0;
+        acc[2] -= P.g[2];
+        angacc[0] = angacc[1] = angacc[2] = 0.0;
+    } else if (vo.w > 0.0001 && vo.w < 10.0) {
+        acc[0] = 0.0;
+        acc[1] = 0.0;
+        acc[2] -= P.g[2];
+        angacc[0] = angacc[1] = angacc[2] = 0.0;
+    }
+    if (vo.w >= 10.0) {
+        acc[1] = 0.0;
+        angacc[0] = 0.0;
+        angacc[2] = 0.0;
+    }
+    if (vo.w < -0.0001) {
+        acc[0] = acc[1] = acc[2] = 0.0;
+        angacc[0] = angacc[1] = angacc[2] = 0.0;
+    }
+    const double pos[3] = {xo.x, xo.y, xo.z}, v[3] = {vo.x, vo.y, vo.z};
+    const double w[3] = {wo.x, wo.y, wo.z}, apo[3] = {ap.x, ap.y, ap.z};
+    double xsum[3] = {xs.x, xs.y, xs.z};
+    double xn[3], vn[3], apn[3], avn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double dacc = (acc[c] - acc0[c]) / dt;
+        const double dang = (angacc[c] - angacc0[c]) / dt;
+        xn[c] = pos[c] + v[c] * dt + 0.5 * acc[c] * dt * dt + 1.0 / 6.0 * dacc * dt * dt * dt;
+        vn[c] = v[c] + acc[c] * dt + 0.5 * dacc * dt * dt;
+        apn[c] = apo[c] + w[c] * dt + 0.5 * angacc[c] * dt * dt + 1.0 / 6.0 * dang * dt * dt * dt;
+        avn[c] = w[c] + angacc[c] * dt + 0.5 * dang * dt * dt;
+        xsum[c] += v[c] * dt + 0.5 * acc[c] * dt * dt + 1.0 / 6.0 * dacc * dt * dt * dt;
+    }
+    if (g.periodic == 1) {
+        if (xn[0] < g.origo[0]) xn[0] += g.L[0];
+        if (xn[0] > g.L[0]) xn[0] -= g.L[0];
+        if (xn[1] < g.origo[1]) xn[1] += g.L[1];
+        if (xn[1] > g.L[1]) xn[1] -= g.L[1];
+    } else if (g.periodic == 2) {
+        if (xn[0] < g.origo[0]) xn[0] += g.L[0];
+        if (xn[0] > g.L[0]) xn[0] -= g.L[0];
+    }
+    if (vo.w > 5.0 && vo.w < 10.0) {
+        if (change_velocity_state == 1) vn[0] *= velocity_factor;
+        else if (change_velocity_state == -1) vn[0] /= velocity_factor;
+    }
+    xyzsum_a[i] = make_double4(xsum[0], xsum[1], xsum[2], xs.w);
+    acc_a[i] = make_double4(acc[0], acc[1], acc[2], 0.0);
+    angacc_a[i] = make_double4(angacc[0], angacc[1], angacc[2], 0.0);
+    angvel[i] = make_double4(avn[0], avn[1], avn[2], wo.w);
+    vel[i] = make_double4(vn[0], vn[1], vn[2], vo.w);
+    angpos[i] = make_double4(apn[0], apn[1], apn[2], ap.w);
+    x[i] = make_double4(xn[0], xn[1], xn[2], xo.w);
+}
+
+void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,
+                      int change_velocity_state, double velocity_factor, double tau_x)
+{
+    uint32_t nb = (a.np + 255) / 256;
+    hipLaunchKernelGGL(k_integrate, dim3(nb), dim3(256), 0, s, a, iter, change_velocity_state,
+                       velocity_factor, tau_x, b.x, b.vel, b.angvel, b.force, b.torque, b.angpos,
+                       b.acc, b.angacc, b.xyzsum, b.w_mvfd, b.tau_eff, b.walls_force_pp,
+                       b.wall_partial);
+}
+
+// integrateWalls, src/integration.cuh:344-450.  One wave: lane 0 forms the
+// serial sum over the block partials (the reference's order) once and every
+// wall thread uses it (the reference sums wall 0's partials for each wall).
+__global__ void __launch_bounds__(64) k_integrate_walls(
+    StepArgs a, long iter, double t_current, double4 *__restrict__ w_nx,
+    double4 *__restrict__ w_mvfd, double *__restrict__ w_acc,
+    const double *__restrict__ partial, uint32_t nparts)
+{
+    __shared__ double total;
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (uint32_t k = 0; k < nparts; ++k) s += partial[k];
+        total = s;
+    }
+    __syncthreads();
+    uint32_t idx = threadIdx.x;
+    if (idx >= a.walls.nw) return;
+    int wmode = a.walls.wmode[idx];
+    if (wmode == 0) return;
+    double4 nx = w_nx[idx], mvfd = w_mvfd[idx];
+    const double dt = a.dt;
+    double acc0 = iter == 0 ? 0.0 : w_acc[idx];
+    mvfd.z = total;
+    const double sigma_0 = mvfd.w + a.prm.sigma0_A * sin(2.0 * PI_REF * a.prm.sigma0_f * t_current);
+    const double N = -sigma_0 * a.grid.L[0] * a.grid.L[1];
+    double acc = (mvfd.z + N) / mvfd.x;
+    if (wmode == 2) acc = 0.0;
+    const double dacc_dt = (acc - acc0) / dt;
+    nx.w += mvfd.y * dt + 0.5 * acc * dt * dt + 1.0 / 6.0 * dacc_dt * dt * dt * dt;
+    mvfd.y += acc * dt + 0.5 * dacc_dt * dt * dt;
+    w_nx[idx] = nx;
+    w_mvfd[idx] = mvfd;
+    w_acc[idx] = acc;
+}
+
+void launch_integrate_walls(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,
+                            double t_current)
+{
+    uint32_t nb = (a.np + 255) / 256;
+    hipLaunchKernelGGL(k_integrate_walls, dim3(1), dim3(64), 0, s, a, iter, t_current, b.w_nx,
+                       b.w_mvfd, b.w_acc, b.wall_partial, nb);
+}
+
+// ------------------------------------------------------------------
+// helpers
+// ------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_count_contacts(const uint32_t *__restrict__ cmask,
+                                                        uint32_t np,
+                                                        unsigned long long *__restrict__ out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long c = i < np ? (unsigned long long)__popc(cmask[i]) : 0ull;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
+    if ((threadIdx.x & 63u) == 0 && c) atomicAdd(out, c);
+}
+
+void launch_count_contacts(hipStream_t s, const uint32_t *cmask, uint32_t np,
+                           unsigned long long *out)
+{
+    if (!np) return;
+    hipLaunchKernelGGL(k_count_contacts, dim3((np + 255) / 256), dim3(256), 0, s, cmask, np, out);
+}
+
+// Occupancy bits from a contact row (after a host write of contact history);
+// empty slots get delta_t = 0, the state the reference's interact leaves them in.
+__global__ void __launch_bounds__(256) k_rebuild_mask(uint32_t np,
+                                                      const uint32_t *__restrict__ contacts,
+                                                      uint32_t *__restrict__ cmask,
+                                                      double *__restrict__ delta_t)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    uint32_t m = 0;
+    for (int k = 0; k < NCS; ++k) {
+        size_t pos = (size_t)i * NCS + k;
+        if (contacts[pos] != np) {
+            m |= 1u << k;
+        } else {
+            delta_t[3 * pos] = 0.0;
+            delta_t[3 * pos + 1] = 0.0;
+            delta_t[3 * pos + 2] = 0.0;
+        }
+    }
+    cmask[i] = m;
+}
+
+void launch_rebuild_mask(hipStream_t s, const DemBuffers &b, uint32_t np)
+{
+    if (!np) return;
+    hipLaunchKernelGGL(k_rebuild_mask, dim3((np + 255) / 256), dim3(256), 0, s, np, b.contacts,
+                       b.cmask, b.delta_t);
+}

@@ -1,0 +1,1148 @@
+// driver.hip -- host driver and C ABI (include/sphere_hip.h).
+//
+// Replaces the reference's process boundary and time loop: DEM::DEM
+// (src/sphere.cpp:16-95), DEM::startTime (src/device.cu:833-2834) and main()
+// (src/main.cpp:28-225).  One context owns one device arena and one HIP
+// stream; all kernels of a step are enqueued back to back on that stream
+// (the reference synchronises after every launch).  The host waits only
+// where the algorithm needs a device value: the Darcy convergence check every
+// 10 sweeps, output events, and the end of each call.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "host_state.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int key_bits_for(uint32_t ncells)
+{
+    int bits = 1;
+    while (bits < 32 && (ncells - 1) >> bits) ++bits;
+    return bits;
+}
+
+}  // namespace
+
+struct sph_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool loaded = false;
+    HostState hs;
+    DemBuffers db{};
+    DarcyBuffers dd{};
+    std::vector<void *> allocs;
+    uint32_t np = 0, nb0 = 0, nw = 0, ncells = 0;
+    int key_bits = 1;
+    // time loop host variables, src/device.cu:913-1036
+    long iter = 0;
+    double filetimeclock = 0.0;
+    double host_wx0 = 0.0;
+    unsigned int wall0_iz = 10000000u;
+    double p_top_orig = 0.0;
+    int velocity_state = 1;
+    sph_darcy_stats stats{};
+    std::vector<std::pair<long, unsigned>> conv;  // (iter+1, nijac)
+    // pinned host scratch
+    uint32_t *h_err = nullptr;
+    unsigned long long *h_u64 = nullptr;
+    double *h_d = nullptr;
+    // profiling
+    bool prof = false;
+    double ktime[SPH_K_COUNT] = {0};
+    uint64_t klaunch[SPH_K_COUNT] = {0};
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::tuple<int, size_t, size_t>> ev_pending;
+};
+
+#define HIPCHK(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);         \
+            return SPH_ERR_HIP;                                                   \
+        }                                                                         \
+    } while (0)
+
+static int fail(sph_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+// ------------------------------------------------------------------
+// profiling with HIP events on the context stream
+// ------------------------------------------------------------------
+
+static size_t prof_event(sph_ctx *ctx)
+{
+    if (ctx->ev_used == ctx->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return (size_t)-1;
+        ctx->ev_pool.push_back(e);
+    }
+    size_t i = ctx->ev_used++;
+    (void)hipEventRecord(ctx->ev_pool[i], ctx->stream);
+    return i;
+}
+
+struct ProfScope {
+    sph_ctx *ctx;
+    int k;
+    size_t e0;
+    ProfScope(sph_ctx *c, int kid) : ctx(c), k(kid), e0((size_t)-1)
+    {
+        if (ctx->prof) e0 = prof_event(ctx);
+    }
+    ~ProfScope()
+    {
+        if (ctx->prof && e0 != (size_t)-1) {
+            size_t e1 = prof_event(ctx);
+            if (e1 != (size_t)-1) ctx->ev_pending.emplace_back(k, e0, e1);
+        }
+    }
+};
+
+static void prof_flush(sph_ctx *ctx)
+{
+    for (auto &t : ctx->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev_pool[std::get<1>(t)], ctx->ev_pool[std::get<2>(t)]) ==
+            hipSuccess) {
+            ctx->ktime[std::get<0>(t)] += ms;
+            ctx->klaunch[std::get<0>(t)] += 1;
+        }
+    }
+    ctx->ev_pending.clear();
+    ctx->ev_used = 0;
+}
+
+static int sync(sph_ctx *ctx)
+{
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipGetLastError());
+    if (ctx->prof) prof_flush(ctx);
+    return SPH_OK;
+}
+
+// Reads and clears the kernel error flags.
+static int check_flags(sph_ctx *ctx)
+{
+    if (!ctx->db.errflags) return SPH_OK;
+    HIPCHK(hipMemcpyAsync(ctx->h_err, ctx->db.errflags, 4, hipMemcpyDeviceToHost, ctx->stream));
+    int rc = sync(ctx);
+    if (rc) return rc;
+    uint32_t f = *ctx->h_err;
+    if (!f) return SPH_OK;
+    launch_fill_u32(ctx->stream, ctx->db.errflags, 1, 0u);
+    if (f & ERRF_GRID)
+        return fail(ctx, SPH_ERR_GRID, "One or more particles moved outside the grid (numerical "
+                                       "instability? is the time step too large?)");
+    if (f & ERRF_OVERFLOW)
+        return fail(ctx, SPH_ERR_OVERFLOW, "more than 32 contacts on one particle");
+    return SPH_OK;
+}
+
+// ------------------------------------------------------------------
+// device memory
+// ------------------------------------------------------------------
+
+template <class T>
+static T *dalloc(sph_ctx *ctx, size_t n)
+{
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+    ctx->allocs.push_back(p);
+    return (T *)p;
+}
+
+static void free_device(sph_ctx *ctx)
+{
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->allocs) hipFree(p);
+    ctx->allocs.clear();
+    ctx->db = DemBuffers{};
+    ctx->dd = DarcyBuffers{};
+    ctx->loaded = false;
+}
+
+static uint32_t dcells(const sph_state &h)
+{
+    return (h.num[0] + 2) * (h.num[1] + 2) * (h.num[2] + 2);
+}
+
+static inline uint32_t host_didx(const sph_state &h, uint32_t x, uint32_t y, uint32_t z)
+{
+    return (x + 1) + (h.num[0] + 2) * (y + 1) + (h.num[0] + 2) * (h.num[1] + 2) * (z + 1);
+}
+
+// ------------------------------------------------------------------
+// value checks, src/sphere.cpp:145-345
+// ------------------------------------------------------------------
+
+static int check_values(sph_ctx *ctx, const HostState &hs)
+{
+    const sph_state &s = hs.h;
+    char m[256];
+    if (s.nd != 3) { snprintf(m, sizeof(m), "nd = %d, ND = 3", s.nd); return fail(ctx, SPH_ERR_VALUE, m); }
+    if (s.time_current > s.time_total || s.time_current < 0.0) {
+        snprintf(m, sizeof(m), "time.current = %g s, time.total = %g s", s.time_current, s.time_total);
+        return fail(ctx, SPH_ERR_VALUE, m);
+    }
+    if (s.origo[0] < 0.0 || s.origo[1] < 0.0 || s.origo[2] < 0.0)
+        return fail(ctx, SPH_ERR_VALUE, "Negative values in grid.origo are known to cause problems");
+    if (s.L[0] <= 0.0 || s.L[1] <= 0.0 || s.L[2] <= 0.0)
+        return fail(ctx, SPH_ERR_VALUE, "grid.L must be positive");
+    if (s.num[0] == 0 || s.num[1] == 0 || s.num[2] == 0)
+        return fail(ctx, SPH_ERR_VALUE, "grid.num must be positive");
+    if (s.periodic == 2 && s.num[0] < 3)
+        return fail(ctx, SPH_ERR_VALUE, "When 1st dimension boundaries are periodic, there must be at least 3 cells");
+    if (s.periodic == 1 && (s.num[0] < 3 || s.num[1] < 3))
+        return fail(ctx, SPH_ERR_VALUE, "When 1st and 2nd dimension boundaries are periodic, there must be at least 3 cells");
+    for (uint32_t i = 0; i < s.np; ++i) {
+        const double *x = &hs.x[4 * (size_t)i];
+        if (x[3] <= 0.0) {
+            snprintf(m, sizeof(m), "Particle %u has a radius of %g m", i, x[3]);
+            return fail(ctx, SPH_ERR_VALUE, m);
+        }
+        if (x[0] != x[0] || x[1] != x[1] || x[2] != x[2] || x[3] != x[3]) {
+            snprintf(m, sizeof(m), "Particle %u position is NaN", i);
+            return fail(ctx, SPH_ERR_VALUE, m);
+        }
+        if (x[0] < s.origo[0] || x[1] < s.origo[1] || x[2] < s.origo[2] || x[0] > s.L[0] ||
+            x[1] > s.L[1] || x[2] > s.L[2]) {
+            snprintf(m, sizeof(m), "Particle %u is outside of the computational grid", i);
+            return fail(ctx, SPH_ERR_VALUE, m);
+        }
+    }
+    if (s.nw > SPH_MAXWALLS) return fail(ctx, SPH_ERR_VALUE, "nw > MAXWALLS");
+    if (s.nw > 0 && s.wmode[0] != 0 && s.np > 0) {
+        double zmax = -1e300;
+        for (uint32_t i = 0; i < s.np; ++i) zmax = std::max(zmax, hs.x[4 * (size_t)i + 2]);
+        if (s.w_nx[3] < zmax)
+            return fail(ctx, SPH_ERR_VALUE, "One or more particles have centres above the upper, dynamic wall");
+    }
+    if (s.k_n <= 0.0) { snprintf(m, sizeof(m), "k_n = %g N/m", s.k_n); return fail(ctx, SPH_ERR_VALUE, m); }
+    if (s.E < 0.0) { snprintf(m, sizeof(m), "E = %g", s.E); return fail(ctx, SPH_ERR_VALUE, m); }
+    if (s.rho <= 0.0) { snprintf(m, sizeof(m), "rho = %g kg/m3", s.rho); return fail(ctx, SPH_ERR_VALUE, m); }
+    if (s.fluid) {
+        if (s.cfd_solver != 1) return fail(ctx, SPH_ERR_VALUE, "only the Darcy solver (cfd_solver = 1) is built");
+        if (s.rho_f <= 0.0) return fail(ctx, SPH_ERR_VALUE, "rho_f <= 0");
+        if (s.mu <= 0.0) return fail(ctx, SPH_ERR_VALUE, "mu <= 0");
+        if (s.beta_f <= 0.0) return fail(ctx, SPH_ERR_VALUE, "beta_f <= 0");
+        if (s.k_c <= 0.0) return fail(ctx, SPH_ERR_VALUE, "k_c <= 0");
+        if (s.ndem == 0) return fail(ctx, SPH_ERR_VALUE, "ndem = 0");
+    }
+    return SPH_OK;
+}
+
+// ------------------------------------------------------------------
+// upload / download
+// ------------------------------------------------------------------
+
+static void pack4(const std::vector<double> &src, size_t n, int w, std::vector<double> &dst)
+{
+    dst.assign(n * 4, 0.0);
+    for (size_t i = 0; i < n; ++i)
+        for (int c = 0; c < w; ++c) dst[4 * i + c] = src[(size_t)w * i + c];
+}
+
+static int upload(sph_ctx *ctx)
+{
+    HostState &hs = ctx->hs;
+    sph_state &h = hs.h;
+    const size_t np = h.np, nb = h.nb0;
+    ctx->np = h.np;
+    ctx->nb0 = h.nb0;
+    ctx->nw = h.nw;
+    ctx->ncells = h.num[0] * h.num[1] * h.num[2];
+    ctx->key_bits = key_bits_for(ctx->ncells);
+    DemBuffers &b = ctx->db;
+    const size_t nblocks = (np + 255) / 256;
+    const size_t ntiles = (np + 2047) / 2048;
+#define AL(ptr, T, n)                                                     \
+    do {                                                                  \
+        ptr = dalloc<T>(ctx, n);                                          \
+        if (!ptr) return fail(ctx, SPH_ERR_HIP, "hipMalloc failed for " #ptr); \
+    } while (0)
+    AL(b.x, double4, np); AL(b.vel, double4, np); AL(b.angvel, double4, np);
+    AL(b.force, double4, np); AL(b.torque, double4, np); AL(b.angpos, double4, np);
+    AL(b.acc, double4, np); AL(b.angacc, double4, np); AL(b.xyzsum, double4, np);
+    AL(b.es_dot, double, np); AL(b.es, double, np); AL(b.ev_dot, double, np);
+    AL(b.ev, double, np); AL(b.p, double, np);
+    AL(b.keys, uint32_t, np); AL(b.vals, uint32_t, np);
+    AL(b.keys_tmp, uint32_t, np); AL(b.vals_tmp, uint32_t, np);
+    AL(b.hist, uint32_t, 256 * ntiles);
+    AL(b.cell_start, uint32_t, ctx->ncells); AL(b.cell_end, uint32_t, ctx->ncells);
+    AL(b.x_s, double4, np); AL(b.vel_s, double4, np); AL(b.angvel_s, double4, np);
+    AL(b.contacts, uint32_t, np * NCS); AL(b.cmask, uint32_t, np);
+    AL(b.dmcode, uint8_t, np * NCS); AL(b.delta_t, double, np * NCS * 3);
+    AL(b.w_nx, double4, MAXW); AL(b.w_mvfd, double4, MAXW); AL(b.w_acc, double, MAXW);
+    AL(b.walls_force_pp, double, np * (h.nw ? h.nw : 1));
+    AL(b.wall_partial, double, nblocks);
+    AL(b.tau_pp, double, np); AL(b.tau_partial, double, nblocks); AL(b.tau_eff, double, 1);
+    AL(b.bonds, uint2, nb); AL(b.bond_broken, uint8_t, nb);
+    AL(b.bond_delta, double4, nb); AL(b.bond_omega, double4, nb);
+    AL(b.bond_f, double4, nb); AL(b.bond_t, double4, nb); AL(b.bond_applied, uint8_t, nb);
+    AL(b.errflags, uint32_t, 1);
+    hipStream_t s = ctx->stream;
+    std::vector<double> t4;
+    pack4(hs.x, np, 4, t4);
+    HIPCHK(hipMemcpyAsync(b.x, t4.data(), np * 32, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    pack4(hs.vel, np, 4, t4);
+    HIPCHK(hipMemcpyAsync(b.vel, t4.data(), np * 32, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const std::vector<double> *v3[] = {&hs.angvel, &hs.force, &hs.torque, &hs.angpos, &hs.xyzsum};
+    double4 *d3p[] = {b.angvel, b.force, b.torque, b.angpos, b.xyzsum};
+    for (int k = 0; k < 5; ++k) {
+        pack4(*v3[k], np, 3, t4);
+        HIPCHK(hipMemcpyAsync(d3p[k], t4.data(), np * 32, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    HIPCHK(hipMemcpyAsync(b.es_dot, hs.es_dot.data(), np * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b.es, hs.es.data(), np * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b.ev_dot, hs.ev_dot.data(), np * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b.ev, hs.ev.data(), np * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b.p, hs.p.data(), np * 8, hipMemcpyHostToDevice, s));
+    launch_fill_f64(s, (double *)b.acc, np * 4, 0.0);
+    launch_fill_f64(s, (double *)b.angacc, np * 4, 0.0);
+    launch_fill_u32(s, b.contacts, np * NCS, (uint32_t)np);  // device.cu:471-473
+    launch_fill_u32(s, b.cmask, np, 0u);
+    launch_fill_u32(s, (uint32_t *)b.dmcode, (np * NCS + 3) / 4, 0u);
+    launch_fill_f64(s, b.delta_t, np * NCS * 3, 0.0);
+    launch_fill_f64(s, b.walls_force_pp, np * (h.nw ? h.nw : 1), 0.0);
+    launch_fill_f64(s, b.w_acc, MAXW, 0.0);
+    launch_fill_u32(s, b.errflags, 1, 0u);
+    {
+        double4 wn[MAXW], wm[MAXW];
+        for (int w = 0; w < MAXW; ++w) {
+            wn[w] = make_double4(h.w_nx[4 * w], h.w_nx[4 * w + 1], h.w_nx[4 * w + 2], h.w_nx[4 * w + 3]);
+            wm[w] = make_double4(h.w_mvfd[4 * w], h.w_mvfd[4 * w + 1], h.w_mvfd[4 * w + 2], h.w_mvfd[4 * w + 3]);
+        }
+        HIPCHK(hipMemcpyAsync(b.w_nx, wn, sizeof(wn), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(b.w_mvfd, wm, sizeof(wm), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    // bonds: deltas as (t.x, t.y, t.z, n); CSR of bond endpoints per particle in
+    // ascending bond order (the serial order of the scatter in cohesion.cuh:220-230)
+    if (nb) {
+        std::vector<uint32_t> bnd(hs.bonds);
+        std::vector<uint8_t> broken(nb, 0);
+        std::vector<double> bd(nb * 4), bo(nb * 4);
+        for (size_t k = 0; k < nb; ++k) {
+            if (bnd[2 * k] >= np || bnd[2 * k + 1] >= np) broken[k] = 1;  // cohesion.cuh:30
+            bd[4 * k] = hs.bdt[3 * k]; bd[4 * k + 1] = hs.bdt[3 * k + 1];
+            bd[4 * k + 2] = hs.bdt[3 * k + 2]; bd[4 * k + 3] = hs.bdn[k];
+            bo[4 * k] = hs.bot[3 * k]; bo[4 * k + 1] = hs.bot[3 * k + 1];
+            bo[4 * k + 2] = hs.bot[3 * k + 2]; bo[4 * k + 3] = hs.bon[k];
+            if (broken[k]) { bnd[2 * k] = 0; bnd[2 * k + 1] = 0; }
+        }
+        std::vector<uint32_t> cnt(np + 1, 0);
+        for (size_t k = 0; k < nb; ++k)
+            if (!broken[k]) { cnt[bnd[2 * k]]++; cnt[bnd[2 * k + 1]]++; }
+        std::vector<uint32_t> plist, off(1, 0), ent;
+        std::vector<uint32_t> start(np, 0);
+        for (size_t i = 0; i < np; ++i)
+            if (cnt[i]) {
+                start[i] = (uint32_t)plist.size();
+                plist.push_back((uint32_t)i);
+                off.push_back(off.back() + cnt[i]);
+            }
+        ent.assign(off.back(), 0u);
+        std::vector<uint32_t> fillpos(plist.size(), 0);
+        for (size_t k = 0; k < nb; ++k) {
+            if (broken[k]) continue;
+            for (int side = 0; side < 2; ++side) {
+                uint32_t pi = bnd[2 * k + side];
+                uint32_t slot = start[pi];
+                ent[off[slot] + fillpos[slot]++] = (uint32_t)(k << 1) | (uint32_t)side;
+            }
+        }
+        b.nbp = (uint32_t)plist.size();
+        AL(b.bp_list, uint32_t, plist.size()); AL(b.bp_off, uint32_t, off.size());
+        AL(b.bp_ent, uint32_t, ent.size());
+        HIPCHK(hipMemcpyAsync(b.bonds, bnd.data(), nb * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(b.bond_broken, broken.data(), nb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(b.bond_delta, bd.data(), nb * 32, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(b.bond_omega, bo.data(), nb * 32, hipMemcpyHostToDevice, s));
+        if (!plist.empty()) {
+            HIPCHK(hipMemcpyAsync(b.bp_list, plist.data(), plist.size() * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(b.bp_off, off.data(), off.size() * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(b.bp_ent, ent.data(), ent.size() * 4, hipMemcpyHostToDevice, s));
+        }
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    // Darcy
+    if (h.fluid) {
+        DarcyBuffers &d = ctx->dd;
+        const size_t c = dcells(h);
+        AL(d.p, double, c); AL(d.p_old, double, c); AL(d.p_new, double, c);
+        AL(d.dp_expl, double, c); AL(d.phi, double, c); AL(d.dphi, double, c);
+        AL(d.k, double, c); AL(d.norm, double, c);
+        AL(d.v, double, 3 * c); AL(d.vp_avg, double, 3 * c); AL(d.grad_p, double, 3 * c);
+        AL(d.p_constant, int, c); AL(d.f_p, double4, np);
+        AL(d.maxres, unsigned long long, 4);
+        std::vector<double> p(c, 0.0), phi(c, 0.0), dphi(c, 0.0), v(3 * c, 0.0);
+        std::vector<int> pc(c, 0);
+        for (uint32_t z = 0; z < h.num[2]; ++z)
+            for (uint32_t y = 0; y < h.num[1]; ++y)
+                for (uint32_t x = 0; x < h.num[0]; ++x) {
+                    size_t f = x + (size_t)h.num[0] * (y + (size_t)h.num[1] * z);
+                    uint32_t ci = host_didx(h, x, y, z);
+                    p[ci] = hs.p_f[f];
+                    phi[ci] = hs.phi[f];
+                    dphi[ci] = hs.dphi[f];
+                    pc[ci] = hs.p_constant[f];
+                    v[3 * ci] = hs.v_f[3 * f];
+                    v[3 * ci + 1] = hs.v_f[3 * f + 1];
+                    v[3 * ci + 2] = hs.v_f[3 * f + 2];
+                }
+        std::vector<double> zero(3 * c, 0.0);
+        HIPCHK(hipMemcpyAsync(d.p, p.data(), c * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.phi, phi.data(), c * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.dphi, dphi.data(), c * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.v, v.data(), 3 * c * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.p_constant, pc.data(), c * 4, hipMemcpyHostToDevice, s));
+        double *zf[] = {d.p_old, d.p_new, d.dp_expl, d.k, d.norm};
+        for (double *q : zf) HIPCHK(hipMemcpyAsync(q, zero.data(), c * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.vp_avg, zero.data(), 3 * c * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.grad_p, zero.data(), 3 * c * 8, hipMemcpyHostToDevice, s));
+        std::vector<double> fp4;
+        pack4(hs.f_p, np, 3, fp4);
+        HIPCHK(hipMemcpyAsync(d.f_p, fp4.data(), np * 32, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        // src/device.cu:1017-1021
+        ctx->p_top_orig = p[host_didx(h, 0, 0, h.num[2] - 1)] -
+                          h.p_mod_A * sin(2.0 * M_PI * h.p_mod_f * h.time_current + h.p_mod_phi);
+    }
+#undef AL
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->iter = 0;
+    ctx->filetimeclock = 0.0;
+    ctx->host_wx0 = h.nw > 0 ? h.w_nx[3] : 0.0;
+    ctx->wall0_iz = 10000000u;
+    ctx->velocity_state = 1;
+    ctx->stats = sph_darcy_stats{};
+    ctx->conv.clear();
+    ctx->loaded = true;
+    return SPH_OK;
+}
+
+static void unpack4(const std::vector<double> &src, size_t n, int w, double *dst)
+{
+    for (size_t i = 0; i < n; ++i)
+        for (int c = 0; c < w; ++c) dst[(size_t)w * i + c] = src[4 * i + c];
+}
+
+// Download the device state into ctx->hs (transferFromGlobalDeviceMemory).
+static int download(sph_ctx *ctx, bool particles, bool fluid_fields)
+{
+    HostState &hs = ctx->hs;
+    sph_state &h = hs.h;
+    const size_t np = h.np, nb = h.nb0;
+    DemBuffers &b = ctx->db;
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipStreamSynchronize(s));
+    {
+        double4 wn[MAXW], wm[MAXW];
+        HIPCHK(hipMemcpy(wn, b.w_nx, sizeof(wn), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(wm, b.w_mvfd, sizeof(wm), hipMemcpyDeviceToHost));
+        for (uint32_t w = 0; w < h.nw; ++w) {
+            h.w_nx[4 * w] = wn[w].x; h.w_nx[4 * w + 1] = wn[w].y;
+            h.w_nx[4 * w + 2] = wn[w].z; h.w_nx[4 * w + 3] = wn[w].w;
+            h.w_mvfd[4 * w] = wm[w].x; h.w_mvfd[4 * w + 1] = wm[w].y;
+            h.w_mvfd[4 * w + 2] = wm[w].z; h.w_mvfd[4 * w + 3] = wm[w].w;
+        }
+    }
+    if (particles && np) {
+        std::vector<double> t4(np * 4);
+        HIPCHK(hipMemcpy(t4.data(), b.x, np * 32, hipMemcpyDeviceToHost));
+        unpack4(t4, np, 4, hs.x.data());
+        HIPCHK(hipMemcpy(t4.data(), b.vel, np * 32, hipMemcpyDeviceToHost));
+        unpack4(t4, np, 4, hs.vel.data());
+        std::vector<double> *v3[] = {&hs.angvel, &hs.force, &hs.torque, &hs.angpos, &hs.xyzsum};
+        double4 *d3p[] = {b.angvel, b.force, b.torque, b.angpos, b.xyzsum};
+        for (int k = 0; k < 5; ++k) {
+            HIPCHK(hipMemcpy(t4.data(), d3p[k], np * 32, hipMemcpyDeviceToHost));
+            unpack4(t4, np, 3, v3[k]->data());
+        }
+        HIPCHK(hipMemcpy(hs.es_dot.data(), b.es_dot, np * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hs.es.data(), b.es, np * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hs.ev_dot.data(), b.ev_dot, np * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hs.ev.data(), b.ev, np * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hs.p.data(), b.p, np * 8, hipMemcpyDeviceToHost));
+    }
+    if (particles && nb) {
+        std::vector<uint8_t> broken(nb);
+        std::vector<double> bd(nb * 4), bo(nb * 4);
+        HIPCHK(hipMemcpy(broken.data(), b.bond_broken, nb, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(bd.data(), b.bond_delta, nb * 32, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(bo.data(), b.bond_omega, nb * 32, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nb; ++k) {
+            if (broken[k]) hs.bonds[2 * k] = (uint32_t)nb;  // cohesion.cuh:211
+            hs.bdt[3 * k] = bd[4 * k]; hs.bdt[3 * k + 1] = bd[4 * k + 1];
+            hs.bdt[3 * k + 2] = bd[4 * k + 2]; hs.bdn[k] = bd[4 * k + 3];
+            hs.bot[3 * k] = bo[4 * k]; hs.bot[3 * k + 1] = bo[4 * k + 1];
+            hs.bot[3 * k + 2] = bo[4 * k + 2]; hs.bon[k] = bo[4 * k + 3];
+        }
+    }
+    if (fluid_fields && h.fluid) {
+        DarcyBuffers &d = ctx->dd;
+        const size_t c = dcells(h);
+        std::vector<double> p(c), phi(c), dphi(c), v(3 * c);
+        HIPCHK(hipMemcpy(p.data(), d.p, c * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(phi.data(), d.phi, c * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(dphi.data(), d.dphi, c * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(v.data(), d.v, 3 * c * 8, hipMemcpyDeviceToHost));
+        for (uint32_t z = 0; z < h.num[2]; ++z)
+            for (uint32_t y = 0; y < h.num[1]; ++y)
+                for (uint32_t x = 0; x < h.num[0]; ++x) {
+                    size_t f = x + (size_t)h.num[0] * (y + (size_t)h.num[1] * z);
+                    uint32_t ci = host_didx(h, x, y, z);
+                    hs.p_f[f] = p[ci];
+                    hs.phi[f] = phi[ci];
+                    hs.dphi[f] = dphi[ci];
+                    hs.v_f[3 * f] = v[3 * ci];
+                    hs.v_f[3 * f + 1] = v[3 * ci + 1];
+                    hs.v_f[3 * f + 2] = v[3 * ci + 2];
+                }
+        if (np) {
+            std::vector<double> t4(np * 4);
+            HIPCHK(hipMemcpy(t4.data(), d.f_p, np * 32, hipMemcpyDeviceToHost));
+            unpack4(t4, np, 3, hs.f_p.data());
+        }
+    }
+    return SPH_OK;
+}
+
+// ------------------------------------------------------------------
+// one time step
+// ------------------------------------------------------------------
+
+static StepArgs step_args(const sph_ctx *ctx)
+{
+    const sph_state &h = ctx->hs.h;
+    StepArgs a{};
+    for (int i = 0; i < 3; ++i) {
+        a.grid.origo[i] = h.origo[i];
+        a.grid.L[i] = h.L[i];
+        a.grid.num[i] = h.num[i];
+        a.prm.g[i] = h.g[i];
+    }
+    a.grid.periodic = h.periodic;
+    a.prm.k_n = h.k_n; a.prm.k_t = h.k_t; a.prm.k_r = h.k_r; a.prm.E = h.E;
+    a.prm.gamma_n = h.gamma_n; a.prm.gamma_t = h.gamma_t; a.prm.gamma_r = h.gamma_r;
+    a.prm.mu_s = h.mu_s; a.prm.mu_d = h.mu_d; a.prm.mu_r = h.mu_r;
+    a.prm.gamma_wn = h.gamma_wn; a.prm.gamma_wt = h.gamma_wt;
+    a.prm.mu_ws = h.mu_ws; a.prm.mu_wd = h.mu_wd; a.prm.rho = h.rho;
+    a.prm.contactmodel = h.contactmodel;
+    a.prm.kappa = h.kappa; a.prm.db = h.db; a.prm.V_b = h.V_b;
+    a.prm.lambda_bar = h.lambda_bar; a.prm.sigma_b = h.sigma_b; a.prm.tau_b = h.tau_b;
+    a.prm.sigma0_A = h.sigma0_A; a.prm.sigma0_f = h.sigma0_f;
+    a.walls.nw = h.nw;
+    for (int w = 0; w < MAXW; ++w) a.walls.wmode[w] = w < (int)h.nw ? h.wmode[w] : 0;
+    a.np = h.np;
+    a.nb0 = h.nb0;
+    a.dt = h.time_dt;
+    return a;
+}
+
+static DarcyArgs darcy_args(const sph_ctx *ctx)
+{
+    const sph_state &h = ctx->hs.h;
+    DarcyArgs d{};
+    d.nx = h.num[0]; d.ny = h.num[1]; d.nz = h.num[2];
+    d.L0 = h.L[0]; d.L1 = h.L[1]; d.L2 = h.L[2];
+    d.dx = h.L[0] / h.num[0]; d.dy = h.L[1] / h.num[1]; d.dz = h.L[2] / h.num[2];
+    d.bc_xn = h.bc_xn; d.bc_xp = h.bc_xp; d.bc_yn = h.bc_yn; d.bc_yp = h.bc_yp;
+    d.bc_bot = h.bc_bot; d.bc_top = h.bc_top;
+    d.bc_bot_flux = h.bc_bot_flux; d.bc_top_flux = h.bc_top_flux;
+    d.mu = h.mu; d.beta_f = h.beta_f; d.rho_f = h.rho_f; d.k_c = h.k_c; d.c_phi = h.c_phi;
+    d.ndem = h.ndem; d.dt = h.time_dt;
+    d.wall0_iz = ctx->wall0_iz;
+    d.periodic = h.periodic;
+    return d;
+}
+
+// Darcy branch of the loop body, src/device.cu:1901-2426
+static int darcy_step(sph_ctx *ctx, std::ofstream *convlog)
+{
+    sph_state &h = ctx->hs.h;
+    hipStream_t s = ctx->stream;
+    DarcyBuffers &d = ctx->dd;
+    if (h.nw > 0 && (h.wmode[0] == 1 || h.wmode[0] == 3))
+        ctx->wall0_iz = (unsigned int)(ctx->host_wx0 / (h.L[2] / h.num[2]));
+    const DarcyArgs da = darcy_args(ctx);
+    const StepArgs a = step_args(ctx);
+    if (h.np > 0) {
+        ProfScope ps(ctx, SPH_K_DARCY_SETUP);
+        launch_darcy_ghosts(s, da, d.p, 1);
+        launch_darcy_pressure_gradient(s, da, d);
+        launch_darcy_ghosts(s, da, d.grad_p, 3);
+        launch_darcy_pressure_force(s, a, da, d, ctx->db);
+    }
+    if ((ctx->iter % h.ndem) != 0) return SPH_OK;
+    {
+        ProfScope ps(ctx, SPH_K_DARCY_SETUP);
+        launch_darcy_porosities(s, a, da, d, ctx->db, ctx->iter);
+        if (h.periodic == 2) launch_darcy_porosities_to_edges(s, da, d);
+        if ((h.p_mod_A > 1.0e-5 || h.p_mod_A < -1.0e-5) && h.p_mod_f > 1.0e-7) {
+            double new_pressure = ctx->p_top_orig +
+                                  h.p_mod_A * sin(2.0 * M_PI * h.p_mod_f * h.time_current + h.p_mod_phi);
+            launch_darcy_top_pressure(s, da, d, new_pressure);
+        }
+        launch_darcy_permeabilities(s, da, d);
+        launch_darcy_ghosts(s, da, d.phi, 1);
+        launch_darcy_ghosts(s, da, d.k, 1);
+        if (ctx->iter == 0) launch_darcy_copy_interior(s, da, d.p, d.p_old);
+        launch_darcy_ghosts_flux(s, da, d.p, d.k);
+    }
+    ctx->stats.fluid_steps++;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    (void)hipEventRecord(t0, s);
+    for (unsigned int nijac = 0; nijac < h.maxiter; ++nijac) {
+        const bool check = (nijac % 10 == 0);
+        {
+            ProfScope ps(ctx, SPH_K_DARCY_SWEEP);
+            if (nijac == 0) launch_darcy_copy_interior(s, da, d.p, d.p_old);
+            launch_darcy_ghosts(s, da, d.p, 1);
+            if (nijac == 0) launch_darcy_first(s, da, d);
+            if (check) launch_fill_u32(s, (uint32_t *)d.maxres, 2, 0u);
+            launch_darcy_update(s, da, d, d.p, d.p_new, check ? 0 : -1);
+            // setDarcyTopWallFixedFlow (bc_top == 1) writes dev_darcy_p, which
+            // the copy below overwrites: it has no effect (device.cu:2284-2331)
+            launch_darcy_ghosts_flux(s, da, d.p, d.k);
+            launch_darcy_copy_interior(s, da, d.p_new, d.p);
+        }
+        ctx->stats.sweeps++;
+        if (check) {
+            HIPCHK(hipMemcpyAsync(ctx->h_u64, d.maxres, 8, hipMemcpyDeviceToHost, s));
+            int rc = sync(ctx);
+            if (rc) return rc;
+            double max_norm_res;
+            unsigned long long bits = *ctx->h_u64;
+            std::memcpy(&max_norm_res, &bits, 8);
+            if (max_norm_res != max_norm_res) {
+                (void)hipEventDestroy(t0);
+                (void)hipEventDestroy(t1);
+                return fail(ctx, SPH_ERR_NAN, "normalized residual is NaN: the system has "
+                                              "probably become unstable");
+            }
+            ctx->stats.last_nijac = nijac;
+            ctx->stats.last_max_residual = max_norm_res;
+            if (max_norm_res <= h.tolerance) {
+                if (ctx->iter % 10 == 0) {
+                    ctx->conv.emplace_back(ctx->iter + 1, nijac);
+                    if (convlog) *convlog << ctx->iter + 1 << '\t' << nijac << std::endl;
+                }
+                break;
+            }
+        }
+        if (nijac == h.maxiter - 1) {
+            ctx->conv.emplace_back(ctx->iter + 1, nijac);
+            if (convlog) *convlog << ctx->iter + 1 << '\t' << nijac << std::endl;
+            ctx->stats.nonconverged++;
+            fprintf(stderr, "\nIteration %ld, time %g s: Error, the pressure solution in the fluid "
+                            "calculations did not converge. Try increasing the value of "
+                            "'darcy.maxiter' (%u) or increase 'darcy.tolerance' (%g).\n",
+                    ctx->iter, ctx->iter * h.time_dt, h.maxiter, h.tolerance);
+        }
+    }
+    (void)hipEventRecord(t1, s);
+    (void)hipEventSynchronize(t1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    ctx->stats.solve_ms += ms;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (ctx->filetimeclock + h.time_dt < h.time_file_dt) launch_darcy_zero_interior(s, da, d.dphi);
+    launch_darcy_ghosts(s, da, d.p, 1);
+    launch_darcy_velocities(s, da, d);
+    return SPH_OK;
+}
+
+// Output event (src/device.cu:2576-2680).  With dir == nullptr the state
+// effects are applied without writing files.
+static int output_event(sph_ctx *ctx, const std::string *dir, const std::string &sid)
+{
+    sph_state &h = ctx->hs.h;
+    int rc = download(ctx, dir != nullptr, dir != nullptr);
+    if (rc) return rc;
+    ctx->host_wx0 = h.nw > 0 ? h.w_nx[3] : 0.0;
+    if (h.fluid) launch_darcy_zero_interior(ctx->stream, darcy_args(ctx), ctx->dd.dphi);
+    h.time_step_count += 1;
+    if (dir) {
+        char name[64];
+        snprintf(name, sizeof(name), ".output%05u.bin", h.time_step_count);
+        std::string err;
+        rc = write_state_file((*dir + "/" + sid + name).c_str(), ctx->hs, err);
+        if (rc) return fail(ctx, rc, err);
+        FILE *fp = fopen((*dir + "/" + sid + ".status.dat").c_str(), "w");
+        if (fp) {
+            fprintf(fp, "%2.4e %2.4e %d\n", h.time_current, 100.0 * h.time_current / h.time_total,
+                    (int)h.time_step_count);
+            fclose(fp);
+        }
+    }
+    ctx->filetimeclock = 0.0;
+    return SPH_OK;
+}
+
+static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *dir,
+                     const std::string &sid)
+{
+    sph_state &h = ctx->hs.h;
+    hipStream_t s = ctx->stream;
+    int rc;
+    if (h.np > 0) {
+        if (h.adaptive == 1 && h.nw > 0) {  // updateGridSize, device.cu:370-400
+            HIPCHK(hipMemcpyAsync(ctx->h_d, (const char *)ctx->db.w_nx + 24, 8,
+                                  hipMemcpyDeviceToHost, s));
+            rc = sync(ctx);
+            if (rc) return rc;
+            h.L[2] = *ctx->h_d;
+        }
+        const StepArgs a = step_args(ctx);
+        { ProfScope ps(ctx, SPH_K_HASH); launch_hash(s, a, ctx->db); }
+        { ProfScope ps(ctx, SPH_K_SORT); launch_sort(s, a.np, ctx->key_bits, ctx->db); }
+        { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells); }
+        { ProfScope ps(ctx, SPH_K_CONTACT); launch_contact(s, a, ctx->db); }
+        if (h.nb0 > 0) { ProfScope ps(ctx, SPH_K_BONDS); launch_bonds(s, a, ctx->db); }
+    }
+    if (h.fluid) {
+        rc = darcy_step(ctx, convlog);
+        if (rc) return rc;
+    }
+    if (h.np > 0) {
+        const StepArgs a = step_args(ctx);
+        if (h.nw > 0 && h.wmode[0] == 3) launch_shear_stress(s, a, ctx->db);
+        int change_velocity_state = 0;  // device.cu:2461-2469
+        if (h.time_current >= 10.0 && h.time_current < 15.0 && ctx->velocity_state == 1) {
+            change_velocity_state = 1;
+            ctx->velocity_state = 2;
+        } else if (h.time_current >= 15.0 && ctx->velocity_state == 2) {
+            change_velocity_state = -1;
+            ctx->velocity_state = 1;
+        }
+        { ProfScope ps(ctx, SPH_K_INTEGRATE);
+          launch_integrate(s, a, ctx->db, ctx->iter, change_velocity_state, 10.0, h.tau_x); }
+        if (h.nw > 0) {
+            ProfScope ps(ctx, SPH_K_WALLS);
+            launch_integrate_walls(s, a, ctx->db, ctx->iter, h.time_current);
+        }
+    }
+    h.time_current += h.time_dt;
+    ctx->filetimeclock += h.time_dt;
+    ctx->iter++;
+    if (ctx->filetimeclock >= h.time_file_dt) {
+        rc = output_event(ctx, dir, sid);
+        if (rc) return rc;
+    }
+    return SPH_OK;
+}
+
+// ------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------
+
+extern "C" {
+
+sph_ctx *sph_create(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        g_last_error = "no HIP device available";
+        return nullptr;
+    }
+    if (device < 0) device = 0;
+    if (device >= n) {
+        g_last_error = "device index out of range";
+        return nullptr;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        g_last_error = "hipSetDevice failed";
+        return nullptr;
+    }
+    sph_ctx *ctx = new sph_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->h_err, 64, hipHostMallocDefault) != hipSuccess) {
+        g_last_error = "stream or pinned allocation failed";
+        delete ctx;
+        return nullptr;
+    }
+    ctx->h_u64 = (unsigned long long *)((char *)ctx->h_err + 16);
+    ctx->h_d = (double *)((char *)ctx->h_err + 32);
+    return ctx;
+}
+
+void sph_destroy(sph_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    free_device(ctx);
+    for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
+    if (ctx->h_err) hipHostFree(ctx->h_err);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *sph_last_error(const sph_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_last_error.c_str();
+}
+
+static int copy_in(sph_ctx *ctx, const sph_state *s)
+{
+    HostState &hs = ctx->hs;
+    hs.h = *s;
+    const size_t np = s->np, nb = s->nb0;
+    if (np && (!s->x || !s->vel)) return fail(ctx, SPH_ERR_ARG, "x and vel are required");
+    hs.alloc_particles(np);
+    auto cp = [](std::vector<double> &dst, const double *src, size_t n) {
+        if (src && n) std::memcpy(dst.data(), src, n * 8);
+    };
+    cp(hs.x, s->x, np * 4); cp(hs.vel, s->vel, np * 4); cp(hs.xyzsum, s->xyzsum, np * 3);
+    cp(hs.force, s->force, np * 3); cp(hs.angpos, s->angpos, np * 3);
+    cp(hs.angvel, s->angvel, np * 3); cp(hs.torque, s->torque, np * 3);
+    cp(hs.es_dot, s->es_dot, np); cp(hs.es, s->es, np); cp(hs.ev_dot, s->ev_dot, np);
+    cp(hs.ev, s->ev, np); cp(hs.p, s->p, np);
+    if (s->color && np) std::memcpy(hs.color.data(), s->color, np * 4);
+    hs.alloc_bonds(nb);
+    if (nb) {
+        if (!s->bonds) return fail(ctx, SPH_ERR_ARG, "bonds array missing");
+        std::memcpy(hs.bonds.data(), s->bonds, nb * 8);
+        cp(hs.bdn, s->bonds_delta_n, nb); cp(hs.bdt, s->bonds_delta_t, nb * 3);
+        cp(hs.bon, s->bonds_omega_n, nb); cp(hs.bot, s->bonds_omega_t, nb * 3);
+    }
+    if (s->fluid) {
+        const size_t cells = (size_t)s->num[0] * s->num[1] * s->num[2];
+        hs.alloc_cells(cells);
+        cp(hs.v_f, s->v_f, cells * 3); cp(hs.p_f, s->p_f, cells);
+        cp(hs.phi, s->phi, cells); cp(hs.dphi, s->dphi, cells);
+        if (s->p_constant) std::memcpy(hs.p_constant.data(), s->p_constant, cells * 4);
+        hs.f_p.assign(np * 3, 0.0);
+        cp(hs.f_p, s->f_p, np * 3);
+    }
+    return SPH_OK;
+}
+
+int sph_load_state(sph_ctx *ctx, const sph_state *s)
+{
+    if (!ctx || !s) return fail(ctx, SPH_ERR_ARG, "null argument");
+    (void)hipSetDevice(ctx->device);
+    free_device(ctx);
+    int rc = copy_in(ctx, s);
+    if (rc) return rc;
+    rc = check_values(ctx, ctx->hs);
+    if (rc) return rc;
+    return upload(ctx);
+}
+
+int sph_load_file(sph_ctx *ctx, const char *path, int fluid)
+{
+    if (!ctx || !path) return fail(ctx, SPH_ERR_ARG, "null argument");
+    (void)hipSetDevice(ctx->device);
+    free_device(ctx);
+    std::string err;
+    int rc = read_state_file(path, fluid, ctx->hs, err);
+    if (rc) return fail(ctx, rc, err);
+    rc = check_values(ctx, ctx->hs);
+    if (rc) return rc;
+    return upload(ctx);
+}
+
+int sph_get_header(sph_ctx *ctx, sph_state *hdr)
+{
+    if (!ctx || !hdr) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
+    (void)hipSetDevice(ctx->device);
+    int rc = download(ctx, false, false);
+    if (rc) return rc;
+    *hdr = ctx->hs.h;
+    hdr->x = hdr->xyzsum = hdr->vel = hdr->force = hdr->angpos = hdr->angvel = hdr->torque = nullptr;
+    hdr->es_dot = hdr->es = hdr->ev_dot = hdr->ev = hdr->p = nullptr;
+    hdr->bonds = nullptr;
+    hdr->bonds_delta_n = hdr->bonds_delta_t = hdr->bonds_omega_n = hdr->bonds_omega_t = nullptr;
+    hdr->v_f = hdr->p_f = hdr->phi = hdr->dphi = hdr->f_p = nullptr;
+    hdr->p_constant = nullptr;
+    hdr->color = nullptr;
+    return SPH_OK;
+}
+
+int sph_read_state(sph_ctx *ctx, sph_state *out)
+{
+    if (!ctx || !out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
+    (void)hipSetDevice(ctx->device);
+    int rc = download(ctx, true, true);
+    if (rc) return rc;
+    const HostState &hs = ctx->hs;
+    sph_state keep = *out;
+    *out = hs.h;
+    const size_t np = hs.h.np, nb = hs.h.nb0;
+    auto cp = [](double *dst, const std::vector<double> &src, size_t n) {
+        if (dst && n) std::memcpy(dst, src.data(), n * 8);
+    };
+    cp(keep.x, hs.x, np * 4); cp(keep.xyzsum, hs.xyzsum, np * 3); cp(keep.vel, hs.vel, np * 4);
+    cp(keep.force, hs.force, np * 3); cp(keep.angpos, hs.angpos, np * 3);
+    cp(keep.angvel, hs.angvel, np * 3); cp(keep.torque, hs.torque, np * 3);
+    cp(keep.es_dot, hs.es_dot, np); cp(keep.es, hs.es, np); cp(keep.ev_dot, hs.ev_dot, np);
+    cp(keep.ev, hs.ev, np); cp(keep.p, hs.p, np);
+    if (keep.color && np) std::memcpy(keep.color, hs.color.data(), np * 4);
+    if (keep.bonds && nb) std::memcpy(keep.bonds, hs.bonds.data(), nb * 8);
+    cp(keep.bonds_delta_n, hs.bdn, nb); cp(keep.bonds_delta_t, hs.bdt, nb * 3);
+    cp(keep.bonds_omega_n, hs.bon, nb); cp(keep.bonds_omega_t, hs.bot, nb * 3);
+    if (hs.h.fluid) {
+        const size_t cells = (size_t)hs.h.num[0] * hs.h.num[1] * hs.h.num[2];
+        cp(keep.v_f, hs.v_f, cells * 3); cp(keep.p_f, hs.p_f, cells);
+        cp(keep.phi, hs.phi, cells); cp(keep.dphi, hs.dphi, cells);
+        if (keep.p_constant) std::memcpy(keep.p_constant, hs.p_constant.data(), cells * 4);
+        cp(keep.f_p, hs.f_p, np * 3);
+    }
+    out->x = keep.x; out->xyzsum = keep.xyzsum; out->vel = keep.vel; out->force = keep.force;
+    out->angpos = keep.angpos; out->angvel = keep.angvel; out->torque = keep.torque;
+    out->es_dot = keep.es_dot; out->es = keep.es; out->ev_dot = keep.ev_dot; out->ev = keep.ev;
+    out->p = keep.p; out->color = keep.color; out->bonds = keep.bonds;
+    out->bonds_delta_n = keep.bonds_delta_n; out->bonds_delta_t = keep.bonds_delta_t;
+    out->bonds_omega_n = keep.bonds_omega_n; out->bonds_omega_t = keep.bonds_omega_t;
+    out->v_f = keep.v_f; out->p_f = keep.p_f; out->phi = keep.phi; out->dphi = keep.dphi;
+    out->p_constant = keep.p_constant; out->f_p = keep.f_p;
+    return SPH_OK;
+}
+
+int sph_write_file(sph_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
+    (void)hipSetDevice(ctx->device);
+    int rc = download(ctx, true, true);
+    if (rc) return rc;
+    std::string err;
+    rc = write_state_file(path, ctx->hs, err);
+    return rc ? fail(ctx, rc, err) : SPH_OK;
+}
+
+int sph_step(sph_ctx *ctx, int nsteps)
+{
+    if (!ctx) return fail(ctx, SPH_ERR_ARG, "null context");
+    if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
+    (void)hipSetDevice(ctx->device);
+    for (int i = 0; i < nsteps; ++i) {
+        int rc = step_once(ctx, nullptr, nullptr, std::string());
+        if (rc) return rc;
+    }
+    int rc = check_flags(ctx);
+    if (rc) return rc;
+    return sync(ctx);
+}
+
+int sph_run(const sph_run_opts *opts)
+{
+    if (!opts || !opts->input_path) return fail(nullptr, SPH_ERR_ARG, "null options");
+    std::string in(opts->input_path);
+    size_t dotpos = in.rfind('.');
+    size_t slashpos = in.rfind('/');
+    size_t from = slashpos == std::string::npos ? 0 : slashpos + 1;
+    std::string sid = in.substr(from, (dotpos == std::string::npos || dotpos < from) ? std::string::npos
+                                                                                     : dotpos - from);
+    std::string dir = opts->output_dir ? opts->output_dir : "output";
+    sph_ctx *ctx = sph_create(opts->device);
+    if (!ctx) return SPH_ERR_HIP;
+    int rc = sph_load_file(ctx, opts->input_path, opts->fluid);
+    if (rc) {
+        g_last_error = ctx->err;
+        sph_destroy(ctx);
+        return rc;
+    }
+    sph_state &h = ctx->hs.h;
+    if (opts->dry) {  // reportValues, src/sphere.cpp:349-465
+        printf("  - Number of particles: %u\n  - Time step length: %g s\n  - Total time: %g s\n"
+               "  - File output interval: %g s\n  - Grid: %u*%u*%u, periodic %d\n  - Contact model: %u\n",
+               h.np, h.time_dt, h.time_total, h.time_file_dt, h.num[0], h.num[1], h.num[2],
+               h.periodic, h.contactmodel);
+        sph_destroy(ctx);
+        return SPH_OK;
+    }
+    std::string err;
+    if (h.time_step_count == 0) {
+        rc = write_state_file((dir + "/" + sid + ".output00000.bin").c_str(), ctx->hs, err);
+        if (rc) {
+            g_last_error = err;
+            sph_destroy(ctx);
+            return rc;
+        }
+    }
+    {
+        FILE *fp = fopen((dir + "/" + sid + ".status.dat").c_str(), "w");
+        if (fp) {
+            fprintf(fp, "%2.4e %2.4e %d\n", h.time_current, 100.0 * h.time_current / h.time_total,
+                    (int)h.time_step_count);
+            fclose(fp);
+        }
+    }
+    std::ofstream convlog((dir + "/" + sid + "-conv.log").c_str());
+    long nstep = 0;
+    while (h.time_current <= h.time_total) {
+        rc = step_once(ctx, &convlog, &dir, sid);
+        if (!rc && (++nstep % 64 == 0)) rc = check_flags(ctx);
+        if (rc) break;
+        if (opts->verbose && ctx->iter % 10000 == 0) {
+            printf("\r  Current time: %g/%g s.", h.time_current, h.time_total);
+            fflush(stdout);
+        }
+    }
+    if (!rc) rc = check_flags(ctx);
+    if (!rc) rc = sync(ctx);
+    if (opts->verbose) printf("\n");
+    if (rc) g_last_error = ctx->err;
+    sph_destroy(ctx);
+    return rc;
+}
+
+int sph_read_sort(sph_ctx *ctx, uint32_t *cellid, uint32_t *index, uint32_t *cell_start,
+                  uint32_t *cell_end)
+{
+    if (!ctx) return fail(ctx, SPH_ERR_ARG, "null context");
+    if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const size_t np = ctx->np;
+    if (cellid && np) HIPCHK(hipMemcpy(cellid, ctx->db.keys, np * 4, hipMemcpyDeviceToHost));
+    if (index && np) HIPCHK(hipMemcpy(index, ctx->db.vals, np * 4, hipMemcpyDeviceToHost));
+    if (cell_start) HIPCHK(hipMemcpy(cell_start, ctx->db.cell_start, ctx->ncells * 4, hipMemcpyDeviceToHost));
+    if (cell_end) HIPCHK(hipMemcpy(cell_end, ctx->db.cell_end, ctx->ncells * 4, hipMemcpyDeviceToHost));
+    return SPH_OK;
+}
+
+int sph_read_contacts(sph_ctx *ctx, uint32_t *contacts, double *distmod, double *delta_t)
+{
+    if (!ctx) return fail(ctx, SPH_ERR_ARG, "null context");
+    if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const size_t n = (size_t)ctx->np * NCS;
+    if (!n) return SPH_OK;
+    std::vector<uint32_t> c(n);
+    HIPCHK(hipMemcpy(c.data(), ctx->db.contacts, n * 4, hipMemcpyDeviceToHost));
+    if (contacts) std::memcpy(contacts, c.data(), n * 4);
+    if (distmod) {
+        std::vector<uint8_t> code(n);
+        HIPCHK(hipMemcpy(code.data(), ctx->db.dmcode, n, hipMemcpyDeviceToHost));
+        const sph_state &h = ctx->hs.h;
+        for (size_t i = 0; i < n; ++i) {
+            double dx = 0.0, dy = 0.0;
+            if (c[i] != ctx->np) {
+                if (code[i] & 1u) dx = dx + h.L[0];
+                if (code[i] & 2u) dx = dx - h.L[0];
+                if (code[i] & 4u) dy = dy + h.L[1];
+                if (code[i] & 8u) dy = dy - h.L[1];
+            }
+            distmod[3 * i] = dx;
+            distmod[3 * i + 1] = dy;
+            distmod[3 * i + 2] = 0.0;
+        }
+    }
+    if (delta_t) HIPCHK(hipMemcpy(delta_t, ctx->db.delta_t, n * 24, hipMemcpyDeviceToHost));
+    return SPH_OK;
+}
+
+int sph_write_contacts(sph_ctx *ctx, const uint32_t *contacts, const double *distmod,
+                       const double *delta_t)
+{
+    if (!ctx || !contacts) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
+    (void)hipSetDevice(ctx->device);
+    const size_t n = (size_t)ctx->np * NCS;
+    if (!n) return SPH_OK;
+    const sph_state &h = ctx->hs.h;
+    std::vector<uint8_t> code(n, 0);
+    for (size_t i = 0; i < n; ++i) {
+        if (contacts[i] > ctx->np) return fail(ctx, SPH_ERR_ARG, "contact index out of range");
+        if (distmod && contacts[i] != ctx->np) {
+            double dx = distmod[3 * i], dy = distmod[3 * i + 1];
+            uint8_t cd = 0;
+            if (dx == h.L[0]) cd |= 1; else if (dx == -h.L[0]) cd |= 2; else if (dx != 0.0)
+                return fail(ctx, SPH_ERR_ARG, "distmod x is not 0 or +-L[0]");
+            if (dy == h.L[1]) cd |= 4; else if (dy == -h.L[1]) cd |= 8; else if (dy != 0.0)
+                return fail(ctx, SPH_ERR_ARG, "distmod y is not 0 or +-L[1]");
+            code[i] = cd;
+        }
+    }
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(ctx->db.contacts, contacts, n * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ctx->db.dmcode, code.data(), n, hipMemcpyHostToDevice, s));
+    if (delta_t) HIPCHK(hipMemcpyAsync(ctx->db.delta_t, delta_t, n * 24, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    launch_rebuild_mask(s, ctx->db, ctx->np);
+    return sync(ctx);
+}
+
+int sph_count_contacts(sph_ctx *ctx, uint64_t *ncontacts)
+{
+    if (!ctx || !ncontacts) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
+    (void)hipSetDevice(ctx->device);
+    unsigned long long *d = (unsigned long long *)ctx->dd.maxres;
+    unsigned long long *tmp = nullptr;
+    HIPCHK(hipMalloc((void **)&tmp, 8));
+    (void)d;
+    launch_fill_u32(ctx->stream, (uint32_t *)tmp, 2, 0u);
+    launch_count_contacts(ctx->stream, ctx->db.cmask, ctx->np, tmp);
+    HIPCHK(hipMemcpyAsync(ctx->h_u64, tmp, 8, hipMemcpyDeviceToHost, ctx->stream));
+    int rc = sync(ctx);
+    (void)hipFree(tmp);
+    if (rc) return rc;
+    *ncontacts = *ctx->h_u64;
+    return SPH_OK;
+}
+
+int sph_get_darcy_stats(sph_ctx *ctx, sph_darcy_stats *out)
+{
+    if (!ctx || !out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    *out = ctx->stats;
+    return SPH_OK;
+}
+
+int sph_set_profiling(sph_ctx *ctx, int on)
+{
+    if (!ctx) return fail(ctx, SPH_ERR_ARG, "null context");
+    int rc = sync(ctx);
+    if (rc) return rc;
+    ctx->prof = on != 0;
+    for (int k = 0; k < SPH_K_COUNT; ++k) {
+        ctx->ktime[k] = 0.0;
+        ctx->klaunch[k] = 0;
+    }
+    return SPH_OK;
+}
+
+int sph_kernel_time(sph_ctx *ctx, int kernel_id, double *total_ms, uint64_t *launches)
+{
+    if (!ctx || kernel_id < 0 || kernel_id >= SPH_K_COUNT) return fail(ctx, SPH_ERR_ARG, "bad argument");
+    int rc = sync(ctx);
+    if (rc) return rc;
+    if (total_ms) *total_ms = ctx->ktime[kernel_id];
+    if (launches) *launches = ctx->klaunch[kernel_id];
+    return SPH_OK;
+}
+
+}  // extern "C"

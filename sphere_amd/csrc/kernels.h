@@ -1,0 +1,81 @@
+// kernels.h -- host-side launchers of the gfx950 kernels (dem_kernels.hip,
+// darcy_kernels.hip).  Every launcher enqueues on the given stream and
+// returns immediately.
+#pragma once
+
+#include "common.h"
+
+struct DemBuffers {
+    // particle state, original order (src/datatypes.h:16-32)
+    double4 *x, *vel, *angvel, *force, *torque, *angpos, *acc, *angacc, *xyzsum;
+    double *es_dot, *es, *ev_dot, *ev, *p;
+    // sort (src/datatypes.h:52-60)
+    uint32_t *keys, *vals, *keys_tmp, *vals_tmp, *hist;
+    uint32_t *cell_start, *cell_end;
+    double4 *x_s, *vel_s, *angvel_s;
+    // contact history, original index * NCS + slot
+    uint32_t *contacts;   // np*NCS, empty = np
+    uint32_t *cmask;      // np, bit i set iff contacts[.. + i] != np
+    uint8_t *dmcode;      // np*NCS, periodic image code of each slot
+    double *delta_t;      // np*NCS*3
+    // walls
+    double4 *w_nx, *w_mvfd;
+    double *w_acc;
+    double *walls_force_pp;   // nw*np
+    double *wall_partial;     // nblocks
+    double *tau_pp, *tau_partial, *tau_eff;  // wmode 3
+    // bonds
+    uint2 *bonds;
+    uint8_t *bond_broken;
+    double4 *bond_delta, *bond_omega;  // (t.x, t.y, t.z, n)
+    double4 *bond_f, *bond_t;
+    uint8_t *bond_applied;
+    uint32_t *bp_list, *bp_off, *bp_ent;  // bonded particles, CSR offsets, entries
+    uint32_t nbp;
+    uint32_t *errflags;
+};
+
+void launch_hash(hipStream_t s, const StepArgs &a, const DemBuffers &b);
+// Stable LSD radix sort of (keys, vals) by key (bits = key width).
+void launch_sort(hipStream_t s, uint32_t n, int bits, DemBuffers &b);
+void launch_reorder(hipStream_t s, const StepArgs &a, const DemBuffers &b, uint32_t ncells);
+void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b);
+void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b);
+void launch_shear_stress(hipStream_t s, const StepArgs &a, const DemBuffers &b);
+void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,
+                      int change_velocity_state, double velocity_factor, double tau_x);
+void launch_integrate_walls(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,
+                            double t_current);
+void launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
+void launch_fill_f64(hipStream_t s, double *p, size_t n, double v);
+void launch_count_contacts(hipStream_t s, const uint32_t *cmask, uint32_t np,
+                           unsigned long long *out);
+void launch_rebuild_mask(hipStream_t s, const DemBuffers &b, uint32_t np);
+
+struct DarcyBuffers {
+    double *p, *p_old, *p_new, *dp_expl, *phi, *dphi, *k, *norm;
+    double *v, *vp_avg, *grad_p;  // 3 per cell
+    int *p_constant;
+    double4 *f_p;
+    unsigned long long *maxres;   // per-check slots (max |norm| bit patterns)
+};
+
+void launch_darcy_ghosts(hipStream_t s, const DarcyArgs &d, double *f, int ncomp);
+void launch_darcy_ghosts_flux(hipStream_t s, const DarcyArgs &d, double *f, const double *k);
+void launch_darcy_pressure_gradient(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b);
+void launch_darcy_pressure_force(hipStream_t s, const StepArgs &a, const DarcyArgs &d,
+                                 const DarcyBuffers &b, const DemBuffers &db);
+void launch_darcy_porosities(hipStream_t s, const StepArgs &a, const DarcyArgs &d,
+                             const DarcyBuffers &b, const DemBuffers &db, long iter);
+void launch_darcy_porosities_to_edges(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b);
+void launch_darcy_top_pressure(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b,
+                               double new_pressure);
+void launch_darcy_permeabilities(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b);
+void launch_darcy_copy_interior(hipStream_t s, const DarcyArgs &d, const double *src, double *dst);
+void launch_darcy_zero_interior(hipStream_t s, const DarcyArgs &d, double *f);
+void launch_darcy_first(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b);
+// One Jacobi sweep p -> p_new with norm; if check_slot >= 0 the max |norm|
+// over interior cells is accumulated into maxres[check_slot].
+void launch_darcy_update(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b,
+                         const double *p_in, double *p_out, int check_slot);
+void launch_darcy_velocities(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b);

@@ -1,0 +1,154 @@
+"""Seeded test configurations built with the sphere_amd ``sim`` API.
+
+Every case is small enough for the oracle to finish in seconds.  Dense cases
+start with overlapping neighbours and random velocities so that contacts,
+tangential springs, slips, removals and re-insertions all happen within a few
+steps.
+"""
+import numpy
+
+import sphere_amd
+
+
+def lattice_positions(nx, ny, nz, spacing, jitter, rng, origin=0.0):
+    """Jittered cubic lattice, particle index in z-major order."""
+    g = numpy.stack(numpy.meshgrid(numpy.arange(nx), numpy.arange(ny), numpy.arange(nz),
+                                   indexing="ij"), axis=-1).reshape(-1, 3)
+    g = g[numpy.lexsort((g[:, 0], g[:, 1], g[:, 2]))]
+    x = origin + (g + 0.5) * spacing
+    x += rng.uniform(-jitter, jitter, size=x.shape) * spacing
+    return x
+
+
+def settling_case(n=10000, seed=1, steps=2000):
+    """Dry settling of grains on a jittered cubic lattice with frictional
+    walls (model 2, linear contact, k_n = k_t = 1.16e7 N/m, mu = 0.5).
+    Layers of 10x10 at 0.025 m spacing (jitter 10% of the spacing in total);
+    the box is 0.25 x 0.25 m wide and as tall as the lattice needs."""
+    rng = numpy.random.default_rng(seed)
+    per_layer = 100
+    nz = (n + per_layer - 1) // per_layer
+    spacing = 0.025
+    x = lattice_positions(10, 10, nz, spacing, 0.05, rng)[:n]
+    s = sphere_amd.sim(np=n, sid="settling%d" % n, nw=0)
+    s.x[:] = x
+    s.radius[:] = rng.uniform(0.009, 0.011, n)
+    s.defaultParams(mu_s=0.5, mu_d=0.5, k_n=1.16e7, k_t=1.16e7, gamma_wn=1e4, gamma_wt=1e4)
+    s.g[2] = -9.81
+    s.defineWorldBoundaries(L=[0.25, 0.25, nz * spacing])
+    m_min = 2600.0 * 4.0 / 3.0 * numpy.pi * numpy.min(s.radius) ** 3
+    dt = 0.17 * numpy.sqrt(m_min / 1.16e7)
+    s.initTemporal(total=steps * dt, file_dt=steps * dt, dt=dt)
+    return s
+
+
+def dense_case(n=512, seed=0, model=2, periodic=0, nw=0, vmag=0.05, wmag=5.0, spacing_factor=0.985,
+               E=0.0, nu=0.3, gamma_n=0.0, gamma_t=0.0, mu=0.5, gravity=True, kappa=0.0):
+    """Grains of radius ~U[0.0098, 0.0102] on a cubic lattice slightly
+    tighter than a diameter, with random velocities and spins."""
+    rng = numpy.random.default_rng(seed)
+    side = int(round(n ** (1.0 / 3.0)))
+    nx = ny = side
+    nz = (n + nx * ny - 1) // (nx * ny)
+    r_mean = 0.01
+    spacing = 2 * r_mean * spacing_factor
+    margin = 2.2 * 0.0102
+    x = lattice_positions(nx, ny, nz, spacing, 0.02, rng, origin=0.0)[:n]
+    s = sphere_amd.sim(np=n, sid="dense", nw=nw)
+    s.radius[:] = rng.uniform(0.0098, 0.0102, n)
+    if periodic == 1:
+        L = [nx * spacing, ny * spacing, nz * spacing + 2 * margin]
+        x[:, 2] += margin
+    elif periodic == 2:
+        L = [nx * spacing, ny * spacing + 2 * margin, nz * spacing + 2 * margin]
+        x[:, 1] += margin
+        x[:, 2] += margin
+    else:
+        L = [nx * spacing + 2 * margin, ny * spacing + 2 * margin, nz * spacing + 2 * margin]
+        x += margin
+    s.x[:] = x
+    s.vel[:] = rng.uniform(-vmag, vmag, (n, 3))
+    s.angvel[:] = rng.uniform(-wmag, wmag, (n, 3))
+    s.defaultParams(mu_s=mu, mu_d=mu, k_n=1.16e7, k_t=1.16e7, gamma_n=gamma_n, gamma_t=gamma_t,
+                    gamma_wn=1e3, gamma_wt=1e3)
+    s.mu_ws[0] = 0.4
+    s.mu_wd[0] = 0.4
+    s.contactmodel[0] = model
+    if kappa > 0.0:
+        s.kappa[0] = kappa
+        s.V_b[0] = 1e-9
+        s.db[0] = (1e-9) ** (1.0 / 3.0)
+    if model == 3:
+        s.setYoungsModulus(E)
+        s.setPoissonsRatio(nu)
+    if gravity:
+        s.g[2] = -9.81
+    s.periodic[0] = periodic
+    s.defineWorldBoundaries(L=L, dx=2.1 * 0.0102)
+    if nw >= 1:
+        s.w_n[0, :] = [0.0, 0.0, -1.0]
+        s.w_x[0] = numpy.max(s.x[:, 2] + s.radius) + 0.0005
+        s.w_m[0] = s.totalMass() * 0.1
+        s.wmode[0] = 1
+        s.w_sigma0[0] = 1e3
+    m_min = 2600.0 * 4.0 / 3.0 * numpy.pi * 0.0098 ** 3
+    k = 1.16e7 if model != 3 else numpy.pi / 2.0 * E * 0.0102
+    dt = 0.1 * numpy.sqrt(m_min / k)
+    s.initTemporal(total=1.0, file_dt=1.0, dt=dt)
+    return s
+
+
+def add_bonds(s, fraction, seed):
+    """Bond a fraction of the overlapping pairs (brute force, small n)."""
+    rng = numpy.random.default_rng(seed)
+    d = s.x[:, None, :] - s.x[None, :, :]
+    dist = numpy.sqrt(numpy.sum(d ** 2, axis=2))
+    touch = dist < (s.radius[:, None] + s.radius[None, :])
+    i, j = numpy.nonzero(numpy.triu(touch, 1))
+    pick = rng.random(i.size) < fraction
+    for a, b in zip(i[pick], j[pick]):
+        s.bond(int(a), int(b))
+    return s
+
+
+def darcy_case(n=300, seed=0, nxyz=(6, 6, 12), bc_top=0, bc_bot=1, periodic=1, ndem=1,
+               tolerance=1e-8, maxiter=20000, wall=False, mu=1e-5):
+    """Grains in a Darcy grid (the DEM grid): porosity from particle volumes,
+    Kozeny-Carman permeability, Jacobi to the given tolerance.  The sweep
+    p <- 0.95 p + 0.05 (p_old + (dp_expl + dp_impl(p))/2) contracts only while
+    k dt / (beta_f phi mu dx^2) stays below about 6.5 in every cell; mu is
+    chosen to keep the loose cells (phi = 0.9) well inside that bound."""
+    rng = numpy.random.default_rng(seed)
+    nx, ny, nz = nxyz
+    cell = 0.0231
+    L = [nx * cell, ny * cell, nz * cell]
+    s = sphere_amd.sim(np=n, sid="darcy", nw=1 if wall else 0, fluid=True)
+    s.radius[:] = rng.uniform(0.009, 0.011, n)
+    s.x[:, 0] = rng.uniform(0.011, L[0] - 0.011, n)
+    s.x[:, 1] = rng.uniform(0.011, L[1] - 0.011, n)
+    s.x[:, 2] = rng.uniform(0.011, 0.6 * L[2], n)
+    s.vel[:] = rng.uniform(-0.02, 0.02, (n, 3))
+    s.defaultParams(mu_s=0.5, mu_d=0.5, k_n=1.16e7, k_t=1.16e7, gamma_wn=1e3, gamma_wt=1e3)
+    s.g[2] = -9.81
+    s.periodic[0] = periodic
+    s.defineWorldBoundaries(L=L, dx=cell)
+    if wall:
+        s.w_n[0, :] = [0.0, 0.0, -1.0]
+        s.w_x[0] = 0.75 * L[2]
+        s.w_m[0] = s.totalMass()
+        s.wmode[0] = 1
+        s.w_sigma0[0] = 2e4
+    s.initFluid(mu=mu, rho=1000.0, cfd_solver=1)
+    s.setFluidCompressibility(1.43e-8)
+    s.setPermeabilityPrefactor(3.5e-13, verbose=False)
+    s.bc_top[0] = bc_top
+    s.bc_bot[0] = bc_bot
+    s.setTolerance(tolerance)
+    s.setMaxIterations(maxiter)
+    s.setDEMstepsPerCFDstep(ndem)
+    m_min = 2600.0 * 4.0 / 3.0 * numpy.pi * 0.009 ** 3
+    dt = 0.17 * numpy.sqrt(m_min / 1.16e7)
+    s.time_dt[0] = dt
+    s.time_total[0] = 1.0
+    s.time_file_dt[0] = 1.0
+    return s
