@@ -217,7 +217,8 @@ class Engine:
             pass
 
     def load(self, st):
-        self._check(self.lib.sph_load_state(self.ctx, ctypes.byref(st)))
+        """Load a StateBuffers image (e.g. sim.to_state())."""
+        self._check(self.lib.sph_load_state(self.ctx, ctypes.byref(st.state)))
 
     def load_file(self, path, fluid):
         self._check(self.lib.sph_load_file(self.ctx, path.encode(), int(fluid)))
