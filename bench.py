@@ -1,0 +1,168 @@
+"""Benchmark: particle-steps/s of the FP64 DEM step on the dense periodic
+shear box (1,000,000 grains, Hertz-Mindlin, 5% bonded pairs) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 200] [--n 1000000]
+
+Untimed preparation: a dense packing grown on the GPU (radius growth, then
+quenched relaxation under gravity), the 10 kPa top wall, the driven top
+layer and the bonds.  Timed: exactly --steps steps after --warmup steps,
+with every input resident in HBM; per-kernel device time from HIP events on
+the library's stream.  Printed: one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def contact_bytes_per_particle(zbar):
+    # state in 80 B + slot mask 4 B + outputs 112 B, 52 B per contact (index 4,
+    # tangential spring read 24 + write 24)
+    return 196.0 + 52.0 * zbar
+
+
+def prepare_shear_box(engine, n, seed, stage_steps, z_target, relax_rounds, relax_steps):
+    from sphere_amd import workloads as W
+    s, f0, rng = W.shear_box_base(n=n, seed=seed, phi=0.60)
+    s.gamma_n[0] = 5e3
+    st = W.EngineStepper(engine)
+    t0 = time.time()
+    f, z = W.grow_to_jamming(s, st, f0, z_target=z_target, steps=stage_steps, fine_df=0.005)
+    log("growth to jamming done in %.1f s: f=%.4f, Z=%.2f" % (time.time() - t0, f, z))
+    s.w_x[0] = s.L[2]
+    s.g[2] = -9.81
+    t0 = time.time()
+    W.relax_packing(s, st, relax_rounds, relax_steps)
+    log("relaxation done in %.1f s (%d rounds x %d steps); stages with slot overflow: %d" %
+        (time.time() - t0, relax_rounds, relax_steps, st.overflow_steps))
+    # contacts of the final packing: one step with the run's contact law
+    s.gamma_n[0] = 0.0
+    engine.load(s.to_state())
+    engine.step(1)
+    contacts = engine.read_contacts()[0]
+    per = numpy.count_nonzero(contacts.reshape(n, -1) != n, axis=1)
+    log("contacts per grain after preparation: mean %.2f max %d" % (per.mean(), per.max()))
+    nb = W.shear_box_finish(s, rng, contacts=contacts, bond_fraction=0.05)
+    log("packing: %d grains, %d bonds, grid %s, box %s" % (n, nb, [int(v) for v in s.num],
+                                                          [round(float(v), 4) for v in s.L]))
+    return s
+
+
+def cpu_baseline(state_buf, n, warm, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import Oracle
+    ora = Oracle(state_buf, threads=threads, nc=64)
+    ora.step(warm)
+    t0 = time.perf_counter()
+    ora.step(steps)
+    dt = time.perf_counter() - t0
+    return n * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--stage-steps", type=int, default=400)
+    ap.add_argument("--z-target", type=float, default=4.2)
+    ap.add_argument("--relax-rounds", type=int, default=4)
+    ap.add_argument("--relax-steps", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the DEM step and the Darcy solve run on one "
+              "device (multi-GPU runs are not part of this build); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import sphere_amd
+    # the largest lognormal grains touch more than 32 others: 64 contact slots
+    engine = sphere_amd.Engine(0, nc=64)
+    log("preparing the shear box (%d grains)" % args.n)
+    s = prepare_shear_box(engine, args.n, args.seed, args.stage_steps, args.z_target,
+                          args.relax_rounds, args.relax_steps)
+    state = s.to_state()
+    engine.load(state)
+    log("warm-up %d steps" % args.warmup)
+    engine.step(args.warmup)
+    warm_buf = engine.read()
+    n0 = engine.count_contacts()
+    engine.set_profiling(1)
+    log("timed %d steps" % args.steps)
+    t0 = time.perf_counter()
+    engine.step(args.steps)
+    elapsed = time.perf_counter() - t0
+    n1 = engine.count_contacts()
+    ktimes = {}
+    for k, name in enumerate(sphere_amd._abi.K_NAMES):
+        ms, cnt = engine.kernel_time(k)
+        if cnt:
+            ktimes[name] = {"ms_total": round(ms, 3), "launches": int(cnt),
+                            "us_avg": round(1000.0 * ms / cnt, 2)}
+    engine.set_profiling(0)
+    n = args.n
+    value = n * args.steps / elapsed
+    zbar = 0.5 * (n0 + n1) / n
+    c_ms, c_cnt = ktimes["contact"]["ms_total"], ktimes["contact"]["launches"]
+    c_avg_s = c_ms / c_cnt / 1000.0
+    c_bytes = contact_bytes_per_particle(zbar) * n
+    achieved = c_bytes / c_avg_s / 1e9
+    log("elapsed %.3f s, %.1f us/step, Zbar %.2f, contact kernel %.1f us" %
+        (elapsed, 1e6 * elapsed / args.steps, zbar, 1e6 * c_avg_s))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log("CPU oracle baseline: %d threads, 2 + %d steps" % (threads, args.cpu_steps))
+        cv, cdt = cpu_baseline(warm_buf, n, 2, args.cpu_steps, threads)
+        cpu = {"value": cv, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": "the same 1M-grain shear box after the GPU warm-up: 2 untimed + %d timed "
+                         "steps of the FP64 C oracle (OpenMP, %.1f s)" % (args.cpu_steps, cdt)}
+
+    line = {
+        "metric": "particle-steps/s (1M grains, FP64)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "dense periodic shear box: %d grains, lognormal radii (median 0.01 m, "
+                               "sigma_ln 0.2), Hertz-Mindlin E=7e9 nu=0.3 mu=0.5, 10 kPa top wall, "
+                               "top layer at 0.01 m/s, 5%% bonded pairs" % n,
+                   "grains": n, "bonds": int(s.nb0[0]), "contacts_per_grain": round(zbar, 3),
+                   "grid": [int(v) for v in s.num], "dt": float(s.time_dt[0])},
+        "roofline": {"bound": "hbm", "kernel": "contact", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "bytes_per_launch": c_bytes, "us_per_launch": 1e6 * c_avg_s},
+        "kernels": ktimes,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -27,6 +27,7 @@ extern "C" {
 
 #define SPH_VERSION 2.15      /* src/version.h:5 */
 #define SPH_NC 32             /* contact slots per particle, src/constants.h:21 */
+#define SPH_NC_MAX 64         /* largest slot count sph_set_contact_slots accepts */
 #define SPH_MAXWALLS 6        /* src/constants.h:12 */
 
 /* Status codes */
@@ -153,6 +154,14 @@ void sph_destroy(sph_ctx *ctx);
 /* Text of the last error on this context (or of sph_create failure if ctx is NULL). */
 const char *sph_last_error(const sph_ctx *ctx);
 
+/* Contact slots per particle for the next sph_load_*: 32 (the reference's
+ * compile-time NC, src/constants.h:21; the default) or 64.  Slots are filled
+ * from the top, so the slot order of the contacts -- and the force summation
+ * order -- is the same for both sizes while a particle has at most 32 contacts;
+ * 64 serves packings whose largest grains touch more than 32 others, where the
+ * reference writes past the slot row (src/contactsearch.cuh:298-299). */
+int sph_set_contact_slots(sph_ctx *ctx, int nc);
+
 /* Load a state: replaces DEM::readbin + checkValues + transfer to the device
  * (src/file_io.cpp:30-417, src/sphere.cpp:145-345, src/device.cu:667-752).
  * Resets the time-loop counters (iter = 0, as every reference invocation). */
@@ -183,8 +192,9 @@ int sph_run(const sph_run_opts *opts);
  *   cell_start, cell_end: num[0]*num[1]*num[2] each (0xffffffff = empty)  */
 int sph_read_sort(sph_ctx *ctx, uint32_t *cellid, uint32_t *index,
                   uint32_t *cell_start, uint32_t *cell_end);
-/* Per-contact history (src/device.cu:436-439 layout, orig index * NC + slot):
- *   contacts: np*NC (empty slot = np), distmod: np*NC*3, delta_t: np*NC*3 */
+/* Per-contact history (src/device.cu:436-439 layout, orig index * nc + slot,
+ * nc = the slot count of the context):
+ *   contacts: np*nc (empty slot = np), distmod: np*nc*3, delta_t: np*nc*3 */
 int sph_read_contacts(sph_ctx *ctx, uint32_t *contacts, double *distmod,
                       double *delta_t);
 /* Restore contact history (a restart side car; absent from the v2.15 file). */

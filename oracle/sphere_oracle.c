@@ -39,7 +39,6 @@
 #include <omp.h>
 #endif
 
-#define NC SPH_NC
 #define EMPTY 0xffffffffu
 
 /* src/constants.h:15 */
@@ -70,6 +69,9 @@ static inline void st3(double *p, v3 a) { p[0] = a.x; p[1] = a.y; p[2] = a.z; }
 static int g_threads = 1;
 void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 int orc_get_threads(void) { return g_threads; }
+/* contact slots per particle of the next orc_create (the reference's NC) */
+static int g_nc = SPH_NC;
+void orc_set_contact_slots(int nc) { g_nc = (nc == 64) ? 64 : 32; }
 
 struct orc {
     sph_state h;            /* header scalars; array pointers unused */
@@ -93,6 +95,7 @@ struct orc {
     /* contact history, src/device.cu:436-439 */
     uint32_t *contacts;
     double *distmod, *delta_t;
+    int nc;                 /* contact slots per particle */
     long overflow;
     /* Darcy (cells with one ghost layer) */
     uint32_t dcells;
@@ -241,6 +244,8 @@ orc *orc_create(const sph_state *s, char *err, int errlen)
     o->cnt = xcalloc(o->ncells + 1, 4);
     o->x_s = xcalloc((size_t)np * 4, 8); o->vel_s = xcalloc((size_t)np * 4, 8);
     o->angvel_s = xcalloc((size_t)np * 4, 8);
+    o->nc = g_nc;
+    const int NC = o->nc;
     o->contacts = xcalloc((size_t)np * NC, 4);
     for (size_t i = 0; i < (size_t)np * NC; ++i) o->contacts[i] = np; /* device.cu:471-473 */
     o->distmod = xcalloc((size_t)np * NC * 3, 8);
@@ -411,6 +416,7 @@ static void grid_pos(const orc *o, const double *x, int *g)
 /* src/contactsearch.cuh:194-322 and 329-371 */
 static long topology_one(orc *o, uint32_t idx_a)
 {
+    const int NC = o->nc;
     long overflow = 0;
     const double *xa4 = o->x_s + 4 * (size_t)idx_a;
     v3 x_a = ld4(xa4);
@@ -715,6 +721,7 @@ static double contact_linear_wall(const orc *o, acc_t *A, uint32_t idx_a, double
 /* src/contactsearch.cuh:385-726 */
 static void interact_one(orc *o, uint32_t idx_a)
 {
+    const int NC = o->nc;
     const sph_state *h = &o->h;
     uint32_t np = o->np;
     uint32_t a_orig = o->index[idx_a];
@@ -1532,7 +1539,7 @@ static int step_once(orc *o)
         o->filetimeclock = 0.0;
     }
     if (o->overflow) {
-        snprintf(o->err, sizeof(o->err), "more than %d contacts on a particle (%ld overflows)", NC, o->overflow);
+        snprintf(o->err, sizeof(o->err), "more than %d contacts on a particle (%ld overflows)", o->nc, o->overflow);
         return SPH_ERR_OVERFLOW;
     }
     return SPH_OK;
@@ -1632,7 +1639,7 @@ int orc_read_sort(orc *o, uint32_t *cellid, uint32_t *index, uint32_t *cs, uint3
 
 int orc_read_contacts(orc *o, uint32_t *contacts, double *distmod, double *delta_t)
 {
-    size_t n = (size_t)o->np * NC;
+    size_t n = (size_t)o->np * o->nc;
     CP(contacts, o->contacts, n * 4);
     CP(distmod, o->distmod, n * 24);
     CP(delta_t, o->delta_t, n * 24);
@@ -1641,7 +1648,7 @@ int orc_read_contacts(orc *o, uint32_t *contacts, double *distmod, double *delta
 
 int orc_write_contacts(orc *o, const uint32_t *contacts, const double *distmod, const double *delta_t)
 {
-    size_t n = (size_t)o->np * NC;
+    size_t n = (size_t)o->np * o->nc;
     if (contacts) memcpy(o->contacts, contacts, n * 4);
     if (distmod) memcpy(o->distmod, distmod, n * 24);
     if (delta_t) memcpy(o->delta_t, delta_t, n * 24);

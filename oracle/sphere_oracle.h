@@ -44,6 +44,9 @@ int orc_conv_log(orc *o, uint32_t *pairs, int max_pairs);
  * 3 k, 4 norm, 5 dp_expl, 6 p_old; out has (nx+2)(ny+2)(nz+2) doubles. */
 int orc_read_darcy_field(orc *o, int which, double *out);
 void orc_set_threads(int n);
+/* Contact slots per particle (32, the reference's NC, or 64) of the next
+ * orc_create. */
+void orc_set_contact_slots(int nc);
 int orc_get_threads(void);
 
 #ifdef __cplusplus

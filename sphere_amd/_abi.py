@@ -161,6 +161,7 @@ def load_library(path=None):
         "sph_create": (vp, [ctypes.c_int]),
         "sph_destroy": (None, [vp]),
         "sph_last_error": (ctypes.c_char_p, [vp]),
+        "sph_set_contact_slots": (ctypes.c_int, [vp, ctypes.c_int]),
         "sph_load_state": (ctypes.c_int, [vp, ctypes.POINTER(SphState)]),
         "sph_load_file": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
         "sph_get_header": (ctypes.c_int, [vp, ctypes.POINTER(SphState)]),
@@ -195,11 +196,13 @@ class SphereError(RuntimeError):
 class Engine:
     """One library context (one device, one loaded state)."""
 
-    def __init__(self, device=0, lib=None):
+    def __init__(self, device=0, lib=None, nc=32):
         self.lib = lib or load_library()
         self.ctx = self.lib.sph_create(device)
         if not self.ctx:
             raise SphereError(-4, self.lib.sph_last_error(None).decode())
+        self.nc = int(nc)
+        self._check(self.lib.sph_set_contact_slots(self.ctx, self.nc))
 
     def _check(self, rc):
         if rc != SPH_OK:
@@ -253,13 +256,13 @@ class Engine:
 
     def read_contacts(self):
         h = self.header()
-        n = max(h.np, 1) * NC
+        n = max(h.np, 1) * self.nc
         c = numpy.zeros(n, numpy.uint32)
         dm = numpy.zeros(n * 3, numpy.float64)
         dt = numpy.zeros(n * 3, numpy.float64)
         self._check(self.lib.sph_read_contacts(self.ctx, _ptr(c, ctypes.c_uint32),
                                                _ptr(dm, ctypes.c_double), _ptr(dt, ctypes.c_double)))
-        m = h.np * NC
+        m = h.np * self.nc
         return c[:m], dm[:3 * m].reshape(-1, 3), dt[:3 * m].reshape(-1, 3)
 
     def write_contacts(self, contacts, distmod, delta_t):

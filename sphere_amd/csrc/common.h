@@ -13,7 +13,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define NCS 32                 // contact slots per particle (src/constants.h:21)
 #define EMPTY_CELL 0xffffffffu
 #define MAXW 6
 
@@ -51,6 +50,7 @@ struct StepArgs {
     Walls walls;
     uint32_t np;
     uint32_t nb0;
+    uint32_t nc;  // contact slots per particle (32 or 64)
     double dt;
 };
 

@@ -535,7 +535,7 @@ __global__ void __launch_bounds__(256) k_contact(
     double *__restrict__ ev_dot_out, double *__restrict__ es, double *__restrict__ ev,
     double *__restrict__ p_out, const double4 *__restrict__ w_nx,
     const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
-    uint32_t *__restrict__ contacts, uint32_t *__restrict__ cmask,
+    uint32_t *__restrict__ contacts, uint64_t *__restrict__ cmask,
     uint8_t *__restrict__ dmcode, double *__restrict__ delta_t, uint32_t *__restrict__ err)
 {
     const uint32_t idx_a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -567,9 +567,11 @@ __global__ void __launch_bounds__(256) k_contact(
     if (!inside) atomicOr(err, ERRF_GRID);
 
     if (inside && (model == 2 || model == 3)) {
+        const int NCS = (int)a.nc;
+        const uint64_t full = NCS >= 64 ? ~0ull : ((1ull << NCS) - 1ull);
         uint32_t *row = contacts + (size_t)a_orig * NCS;
         uint8_t *crow = dmcode + (size_t)a_orig * NCS;
-        uint32_t mask = cmask[a_orig];
+        uint64_t mask = cmask[a_orig];
         int nc = 0;
         bool ovf = false;
         // topology, contactsearch.cuh:358-369 (z, y, x order)
@@ -598,20 +600,20 @@ __global__ void __launch_bounds__(256) k_contact(
                         uint32_t b_orig = vals[idx_b];
                         // slot rule of contactsearch.cuh:267-286
                         int cpos = -1;
-                        uint32_t occ = mask;
+                        uint64_t occ = mask;
                         while (occ) {
-                            int i = __ffs(occ) - 1;
+                            int i = __ffsll((unsigned long long)occ) - 1;
                             occ &= occ - 1;
                             if (row[i] == b_orig) { cpos = i; break; }
                         }
                         if (cpos < 0) {
-                            uint32_t emp = ~mask;
-                            cpos = emp ? 31 - __clz(emp) : nc;
+                            uint64_t emp = ~mask & full;
+                            cpos = emp ? 63 - __clzll((long long)emp) : nc;
                         }
                         if (cpos < NCS) {
                             row[cpos] = b_orig;
                             crow[cpos] = (uint8_t)code;
-                            mask |= 1u << cpos;
+                            mask |= 1ull << cpos;
                         } else {
                             ovf = true;
                         }
@@ -623,9 +625,9 @@ __global__ void __launch_bounds__(256) k_contact(
 
         // interact, contactsearch.cuh:484-559: slots in ascending order
         const d3 vel_a = xyz(va4), angvel_a = xyz(wa4);
-        uint32_t occ = mask;
+        uint64_t occ = mask;
         while (occ) {
-            int i = __ffs(occ) - 1;
+            int i = __ffsll((unsigned long long)occ) - 1;
             occ &= occ - 1;
             size_t mempos = (size_t)a_orig * NCS + i;
             uint32_t b = row[i];
@@ -652,7 +654,7 @@ __global__ void __launch_bounds__(256) k_contact(
                 dts[0] = 0.0;
                 dts[1] = 0.0;
                 dts[2] = 0.0;
-                mask &= ~(1u << i);
+                mask &= ~(1ull << i);
             }
         }
         cmask[a_orig] = mask;
@@ -1061,17 +1063,17 @@ void launch_integrate_walls(hipStream_t s, const StepArgs &a, const DemBuffers &
 // helpers
 // ------------------------------------------------------------------
 
-__global__ void __launch_bounds__(256) k_count_contacts(const uint32_t *__restrict__ cmask,
+__global__ void __launch_bounds__(256) k_count_contacts(const uint64_t *__restrict__ cmask,
                                                         uint32_t np,
                                                         unsigned long long *__restrict__ out)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long c = i < np ? (unsigned long long)__popc(cmask[i]) : 0ull;
+    unsigned long long c = i < np ? (unsigned long long)__popcll(cmask[i]) : 0ull;
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
     if ((threadIdx.x & 63u) == 0 && c) atomicAdd(out, c);
 }
 
-void launch_count_contacts(hipStream_t s, const uint32_t *cmask, uint32_t np,
+void launch_count_contacts(hipStream_t s, const uint64_t *cmask, uint32_t np,
                            unsigned long long *out)
 {
     if (!np) return;
@@ -1080,18 +1082,18 @@ void launch_count_contacts(hipStream_t s, const uint32_t *cmask, uint32_t np,
 
 // Occupancy bits from a contact row (after a host write of contact history);
 // empty slots get delta_t = 0, the state the reference's interact leaves them in.
-__global__ void __launch_bounds__(256) k_rebuild_mask(uint32_t np,
+__global__ void __launch_bounds__(256) k_rebuild_mask(uint32_t np, uint32_t nc,
                                                       const uint32_t *__restrict__ contacts,
-                                                      uint32_t *__restrict__ cmask,
+                                                      uint64_t *__restrict__ cmask,
                                                       double *__restrict__ delta_t)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
-    uint32_t m = 0;
-    for (int k = 0; k < NCS; ++k) {
-        size_t pos = (size_t)i * NCS + k;
+    uint64_t m = 0;
+    for (uint32_t k = 0; k < nc; ++k) {
+        size_t pos = (size_t)i * nc + k;
         if (contacts[pos] != np) {
-            m |= 1u << k;
+            m |= 1ull << k;
         } else {
             delta_t[3 * pos] = 0.0;
             delta_t[3 * pos + 1] = 0.0;
@@ -1101,9 +1103,9 @@ __global__ void __launch_bounds__(256) k_rebuild_mask(uint32_t np,
     cmask[i] = m;
 }
 
-void launch_rebuild_mask(hipStream_t s, const DemBuffers &b, uint32_t np)
+void launch_rebuild_mask(hipStream_t s, const DemBuffers &b, uint32_t np, uint32_t nc)
 {
     if (!np) return;
-    hipLaunchKernelGGL(k_rebuild_mask, dim3((np + 255) / 256), dim3(256), 0, s, np, b.contacts,
+    hipLaunchKernelGGL(k_rebuild_mask, dim3((np + 255) / 256), dim3(256), 0, s, np, nc, b.contacts,
                        b.cmask, b.delta_t);
 }

@@ -45,6 +45,7 @@ struct sph_ctx {
     DarcyBuffers dd{};
     std::vector<void *> allocs;
     uint32_t np = 0, nb0 = 0, nw = 0, ncells = 0;
+    uint32_t nc = 32;  // contact slots per particle
     int key_bits = 1;
     // time loop host variables, src/device.cu:913-1036
     long iter = 0;
@@ -153,7 +154,7 @@ static int check_flags(sph_ctx *ctx)
         return fail(ctx, SPH_ERR_GRID, "One or more particles moved outside the grid (numerical "
                                        "instability? is the time step too large?)");
     if (f & ERRF_OVERFLOW)
-        return fail(ctx, SPH_ERR_OVERFLOW, "more than 32 contacts on one particle");
+        return fail(ctx, SPH_ERR_OVERFLOW, "more contacts on one particle than contact slots (" + std::to_string(ctx->nc) + ")");
     return SPH_OK;
 }
 
@@ -290,7 +291,8 @@ static int upload(sph_ctx *ctx)
     AL(b.hist, uint32_t, 256 * ntiles);
     AL(b.cell_start, uint32_t, ctx->ncells); AL(b.cell_end, uint32_t, ctx->ncells);
     AL(b.x_s, double4, np); AL(b.vel_s, double4, np); AL(b.angvel_s, double4, np);
-    AL(b.contacts, uint32_t, np * NCS); AL(b.cmask, uint32_t, np);
+    const size_t NCS = ctx->nc;
+    AL(b.contacts, uint32_t, np * NCS); AL(b.cmask, uint64_t, np);
     AL(b.dmcode, uint8_t, np * NCS); AL(b.delta_t, double, np * NCS * 3);
     AL(b.w_nx, double4, MAXW); AL(b.w_mvfd, double4, MAXW); AL(b.w_acc, double, MAXW);
     AL(b.walls_force_pp, double, np * (h.nw ? h.nw : 1));
@@ -323,7 +325,7 @@ static int upload(sph_ctx *ctx)
     launch_fill_f64(s, (double *)b.acc, np * 4, 0.0);
     launch_fill_f64(s, (double *)b.angacc, np * 4, 0.0);
     launch_fill_u32(s, b.contacts, np * NCS, (uint32_t)np);  // device.cu:471-473
-    launch_fill_u32(s, b.cmask, np, 0u);
+    launch_fill_u32(s, (uint32_t *)b.cmask, 2 * np, 0u);
     launch_fill_u32(s, (uint32_t *)b.dmcode, (np * NCS + 3) / 4, 0u);
     launch_fill_f64(s, b.delta_t, np * NCS * 3, 0.0);
     launch_fill_f64(s, b.walls_force_pp, np * (h.nw ? h.nw : 1), 0.0);
@@ -559,6 +561,7 @@ static StepArgs step_args(const sph_ctx *ctx)
     for (int w = 0; w < MAXW; ++w) a.walls.wmode[w] = w < (int)h.nw ? h.wmode[w] : 0;
     a.np = h.np;
     a.nb0 = h.nb0;
+    a.nc = ctx->nc;
     a.dt = h.time_dt;
     return a;
 }
@@ -845,6 +848,16 @@ static int copy_in(sph_ctx *ctx, const sph_state *s)
     return SPH_OK;
 }
 
+int sph_set_contact_slots(sph_ctx *ctx, int nc)
+{
+    if (!ctx) return fail(ctx, SPH_ERR_ARG, "null context");
+    if (nc != 32 && nc != 64) return fail(ctx, SPH_ERR_ARG, "contact slots must be 32 or 64");
+    if (ctx->loaded && (uint32_t)nc != ctx->nc)
+        return fail(ctx, SPH_ERR_STATE, "set the contact slot count before loading a state");
+    ctx->nc = (uint32_t)nc;
+    return SPH_OK;
+}
+
 int sph_load_state(sph_ctx *ctx, const sph_state *s)
 {
     if (!ctx || !s) return fail(ctx, SPH_ERR_ARG, "null argument");
@@ -1039,7 +1052,7 @@ int sph_read_contacts(sph_ctx *ctx, uint32_t *contacts, double *distmod, double 
     if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
     (void)hipSetDevice(ctx->device);
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    const size_t n = (size_t)ctx->np * NCS;
+    const size_t n = (size_t)ctx->np * ctx->nc;
     if (!n) return SPH_OK;
     std::vector<uint32_t> c(n);
     HIPCHK(hipMemcpy(c.data(), ctx->db.contacts, n * 4, hipMemcpyDeviceToHost));
@@ -1071,7 +1084,7 @@ int sph_write_contacts(sph_ctx *ctx, const uint32_t *contacts, const double *dis
     if (!ctx || !contacts) return fail(ctx, SPH_ERR_ARG, "null argument");
     if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
     (void)hipSetDevice(ctx->device);
-    const size_t n = (size_t)ctx->np * NCS;
+    const size_t n = (size_t)ctx->np * ctx->nc;
     if (!n) return SPH_OK;
     const sph_state &h = ctx->hs.h;
     std::vector<uint8_t> code(n, 0);
@@ -1092,7 +1105,7 @@ int sph_write_contacts(sph_ctx *ctx, const uint32_t *contacts, const double *dis
     HIPCHK(hipMemcpyAsync(ctx->db.dmcode, code.data(), n, hipMemcpyHostToDevice, s));
     if (delta_t) HIPCHK(hipMemcpyAsync(ctx->db.delta_t, delta_t, n * 24, hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
-    launch_rebuild_mask(s, ctx->db, ctx->np);
+    launch_rebuild_mask(s, ctx->db, ctx->np, ctx->nc);
     return sync(ctx);
 }
 

@@ -13,11 +13,11 @@ struct DemBuffers {
     uint32_t *keys, *vals, *keys_tmp, *vals_tmp, *hist;
     uint32_t *cell_start, *cell_end;
     double4 *x_s, *vel_s, *angvel_s;
-    // contact history, original index * NCS + slot
-    uint32_t *contacts;   // np*NCS, empty = np
-    uint32_t *cmask;      // np, bit i set iff contacts[.. + i] != np
-    uint8_t *dmcode;      // np*NCS, periodic image code of each slot
-    double *delta_t;      // np*NCS*3
+    // contact history, original index * nc + slot
+    uint32_t *contacts;   // np*nc, empty = np
+    uint64_t *cmask;      // np, bit i set iff contacts[.. + i] != np
+    uint8_t *dmcode;      // np*nc, periodic image code of each slot
+    double *delta_t;      // np*nc*3
     // walls
     double4 *w_nx, *w_mvfd;
     double *w_acc;
@@ -48,9 +48,9 @@ void launch_integrate_walls(hipStream_t s, const StepArgs &a, const DemBuffers &
                             double t_current);
 void launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
 void launch_fill_f64(hipStream_t s, double *p, size_t n, double v);
-void launch_count_contacts(hipStream_t s, const uint32_t *cmask, uint32_t np,
+void launch_count_contacts(hipStream_t s, const uint64_t *cmask, uint32_t np,
                            unsigned long long *out);
-void launch_rebuild_mask(hipStream_t s, const DemBuffers &b, uint32_t np);
+void launch_rebuild_mask(hipStream_t s, const DemBuffers &b, uint32_t np, uint32_t nc);
 
 struct DarcyBuffers {
     double *p, *p_old, *p_new, *dp_expl, *phi, *dphi, *k, *norm;

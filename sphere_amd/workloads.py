@@ -1,0 +1,305 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+* settling(n)       -- dry settling, linear contacts, frictional walls.
+* shear_box(n)      -- dense periodic shear box, Hertz-Mindlin, 10 kPa top
+                       wall, top layer driven at 0.01 m/s, 5% bonded pairs.
+* consolidation(n)  -- Darcy-coupled consolidation under a 20 kPa wall.
+
+Dense packings are made by radius growth: grains start on a jittered cubic
+lattice with radii scaled down until no two touch, then the radii grow back
+to their drawn values in stages while the DEM (the stepper given) relaxes
+the overlaps; velocities are zeroed at the start of every stage.
+"""
+import numpy
+
+from .sim import sim
+
+
+def lognormal_radii(n, rng, median=0.01, sigma=0.2):
+    return median * numpy.exp(sigma * rng.standard_normal(n))
+
+
+def lattice(n, spacing, nx, ny, jitter, rng, z0=0.0):
+    """n sites of a cubic lattice nx*ny per layer, z-major order, jittered by
+    +-jitter*spacing per axis."""
+    k = numpy.arange(n)
+    ix = k % nx
+    iy = (k // nx) % ny
+    iz = k // (nx * ny)
+    x = numpy.stack([ix, iy, iz], axis=1) * spacing + 0.5 * spacing
+    x = x + rng.uniform(-jitter, jitter, size=x.shape) * spacing
+    x[:, 2] += z0
+    return x
+
+
+class EngineStepper:
+    """Adapter of sphere_amd.Engine for grow_packing.  Packing preparation
+    passes through over-compressed states where a large grain can touch more
+    than 32 small ones; those steps are kept (the extra contacts are left out
+    of the slot row) and counted in ``overflow_steps``."""
+
+    def __init__(self, engine):
+        self.engine = engine
+        self.overflow_steps = 0
+
+    def start(self, buf):
+        self.engine.load(buf)
+
+    def step(self, n):
+        from ._abi import SphereError
+        try:
+            self.engine.step(n)
+        except SphereError as e:
+            if e.code != -7:
+                raise
+            self.overflow_steps += 1
+
+    def count_contacts(self):
+        return self.engine.count_contacts()
+
+    def read(self):
+        return self.engine.read()
+
+
+def grow_packing(s, stepper, f0, stages, steps_per_stage, relax_steps, log=None):
+    """Grow the radii of sim ``s`` from f0 to 1 times their values."""
+    r_final = s.radius.copy()
+    n = int(s.np[0])
+    for k in range(1, stages + 2):
+        # stages crowd towards f = 1, where the packing jams
+        u = min(1.0, k / stages)
+        f = min(1.0, f0 + (1.0 - f0) * (1.0 - (1.0 - u) ** 2))
+        s.radius = r_final * f
+        s.vel[:] = 0.0
+        s.angvel[:] = 0.0
+        stepper.start(s.to_state())
+        stepper.step(steps_per_stage if k <= stages else relax_steps)
+        buf = stepper.read()
+        s.x = buf.get("x", n, 4)[:, :3].copy()
+        s.vel = buf.get("vel", n, 4)[:, :3].copy()
+        if log:
+            log("growth stage %d/%d f=%.3f" % (k, stages + 1, f))
+    s.radius = r_final.copy()
+    s.vel[:] = 0.0
+    s.angvel[:] = 0.0
+    s.force[:] = 0.0
+    s.torque[:] = 0.0
+    return s
+
+
+def grow_to_jamming(s, stepper, f0, z_target=4.5, steps=300, coarse_df=0.02, fine_df=0.0025,
+                    z_fine=1.0, max_stages=400, log=None):
+    """Grow radii r*f from f = f0: by coarse_df per stage while the mean
+    coordination number is below z_fine, then by fine_df until it reaches
+    z_target (a jammed packing).  Positions, box and walls are then rescaled
+    by 1/f so that the radii are the drawn ones; Hertzian stress depends on
+    the relative overlap only, so the rescaled packing carries the same
+    stress.  Each stage zeroes the velocities and runs ``steps`` steps.
+    Returns (f, mean coordination number)."""
+    r_final = s.radius.copy()
+    n = int(s.np[0])
+    f = f0
+    z = 0.0
+    for k in range(max_stages):
+        s.radius = r_final * f
+        s.vel[:] = 0.0
+        s.angvel[:] = 0.0
+        stepper.start(s.to_state())
+        stepper.step(steps)
+        buf = stepper.read()
+        s.x = buf.get("x", n, 4)[:, :3].copy()
+        z = stepper.count_contacts() / n
+        if log:
+            log("stage %d f=%.6f Z=%.3f" % (k, f, z))
+        if z >= z_target:
+            break
+        f *= 1.0 + (coarse_df if z < z_fine else fine_df)
+    scale = 1.0 / f
+    s.x *= scale
+    s.L = s.L * scale
+    s.w_x = s.w_x * scale
+    s.radius = r_final.copy()
+    s.vel[:] = 0.0
+    s.angvel[:] = 0.0
+    s.force[:] = 0.0
+    s.torque[:] = 0.0
+    return f, z
+
+
+def cap_overlaps(s, pairs, cap_rel, iters, relax=0.6):
+    """Geometric overlap capping: each pair (i, j) overlapping by more than
+    cap_rel * min(r_i, r_j) is pushed apart along the centre line by a
+    fraction ``relax`` of the excess, shared equally, for ``iters`` sweeps
+    (Jacobi style, periodic images in x and y when s.periodic == 1; the
+    bottom wall at origo and the top wall keep the grains inside).  Bounds
+    the Hertzian stress of a grown packing at E* cap_rel^(3/2)."""
+    x = s.x
+    r = s.radius
+    i = pairs[:, 0]
+    j = pairs[:, 1]
+    n = x.shape[0]
+    L = numpy.asarray(s.L, dtype=numpy.float64)
+    cap = cap_rel * numpy.minimum(r[i], r[j])
+    z_lo = s.origo[2] + r * (1.0 - cap_rel)
+    z_hi = (s.w_x[0] if s.nw[0] > 0 else s.L[2]) - r * (1.0 - cap_rel)
+    for _ in range(iters):
+        d = x[i] - x[j]
+        if int(s.periodic[0]) == 1:
+            d[:, 0] -= L[0] * numpy.round(d[:, 0] / L[0])
+            d[:, 1] -= L[1] * numpy.round(d[:, 1] / L[1])
+        dist = numpy.sqrt(numpy.sum(d * d, axis=1))
+        excess = numpy.maximum(r[i] + r[j] - dist - cap, 0.0)
+        if not numpy.any(excess > 0.0):
+            break
+        m = (0.5 * relax * excess / dist)[:, None] * d
+        for c in range(3):
+            x[:, c] += numpy.bincount(i, weights=m[:, c], minlength=n)
+            x[:, c] -= numpy.bincount(j, weights=m[:, c], minlength=n)
+        x[:, 2] = numpy.clip(x[:, 2], z_lo, z_hi)
+        if int(s.periodic[0]) == 1:
+            x[:, 0] %= L[0]
+            x[:, 1] %= L[1]
+    s.x = x
+    return s
+
+
+def relax_packing(s, stepper, rounds, steps, log=None):
+    """Quenched relaxation: every round zeroes the velocities and runs
+    ``steps`` steps (a damped descent to a static, gravity-loaded state)."""
+    n = int(s.np[0])
+    for k in range(rounds):
+        s.vel[:] = 0.0
+        s.angvel[:] = 0.0
+        stepper.start(s.to_state())
+        stepper.step(steps)
+        buf = stepper.read()
+        s.x = buf.get("x", n, 4)[:, :3].copy()
+        if log:
+            log("relax round %d/%d" % (k + 1, rounds))
+    s.vel[:] = 0.0
+    s.angvel[:] = 0.0
+    s.force[:] = 0.0
+    s.torque[:] = 0.0
+    return s
+
+
+def touching_pairs(contacts, np_):
+    """Unique (i, j), i < j, of a contact list laid out as i * 32 + slot."""
+    c = contacts.reshape(np_, -1)
+    i = numpy.repeat(numpy.arange(np_, dtype=numpy.int64), c.shape[1])
+    j = c.ravel().astype(numpy.int64)
+    keep = (j != np_) & (i < j)
+    return numpy.stack([i[keep], j[keep]], axis=1)
+
+
+def bond_pairs(s, pairs, fraction, rng):
+    """Bond a seeded fraction of the given pairs (sphere's bond(), vectorised)."""
+    pick = pairs[rng.random(pairs.shape[0]) < fraction]
+    nb = pick.shape[0]
+    s.lambda_bar[0] = 1.0
+    s.bonds = pick.astype(numpy.uint32)
+    s.bonds_delta_n = numpy.zeros(nb)
+    s.bonds_delta_t = numpy.zeros((nb, 3))
+    s.bonds_omega_n = numpy.zeros(nb)
+    s.bonds_omega_t = numpy.zeros((nb, 3))
+    s.nb0 = numpy.ones(1, dtype=numpy.uint32) * nb
+    return nb
+
+
+def shear_box_base(n=1_000_000, seed=2, phi=0.62, footprint=None):
+    """Grains of the dense periodic shear box on their growth lattice.
+
+    Radii: lognormal, median 0.01 m, sigma_ln 0.2.  Box: periodic in x and
+    y, fixed walls at the bottom (z = 0) and top during growth; volume set so
+    the grown grains fill it at solid fraction phi."""
+    rng = numpy.random.default_rng(seed)
+    r = lognormal_radii(n, rng)
+    v_solid = float(numpy.sum(4.0 / 3.0 * numpy.pi * r ** 3))
+    spacing = (v_solid / phi / n) ** (1.0 / 3.0)
+    if footprint is None:
+        nx = int(round((n / 1.05) ** (1.0 / 3.0)))
+    else:
+        nx = int(round(footprint / spacing))
+    ny = nx
+    nz = (n + nx * ny - 1) // (nx * ny)
+    order = rng.permutation(n)  # radii placed at random sites
+    x = lattice(n, spacing, nx, ny, 0.05, rng)
+    s = sim(np=n, sid="shear_box", nw=1)
+    s.radius[:] = r[order]
+    s.x[:] = x
+    s.periodic[0] = 1
+    s.contactmodel[0] = 3
+    s.setYoungsModulus(7e9)
+    s.setPoissonsRatio(0.3)
+    s.defaultParams(mu_s=0.0, mu_d=0.0, rho=2600, k_n=1.16e9, k_t=1.16e9, k_r=0.3,
+                    gamma_n=0.0, gamma_wn=0.0, gamma_wt=0.0)
+    s.g[:] = 0.0
+    L = [nx * spacing, ny * spacing, nz * spacing]
+    r_max = float(numpy.max(r))
+    # cells fit grains grown to 1.1 times their drawn radii
+    s.defineWorldBoundaries(L=[L[0], L[1], L[2] + 4.0 * r_max], dx=2.1 * 1.1 * r_max)
+    s.w_n[0, :] = [0.0, 0.0, -1.0]
+    s.w_x[0] = L[2]
+    s.wmode[0] = 0
+    s.w_m[0] = 1.0
+    r_min = float(numpy.min(r))
+    m_min = 2600.0 * 4.0 / 3.0 * numpy.pi * r_min ** 3
+    k_max = numpy.pi / 2.0 * 7e9 * r_max
+    s.time_dt[0] = 0.17 * numpy.sqrt(m_min / k_max)
+    s.time_total[0] = 1.0
+    s.time_file_dt[0] = 1.0
+    s.gamma_n[0] = 2e2  # damping during growth only
+    f0 = 0.9 * spacing / (2.0 * r_max)
+    return s, f0, rng
+
+
+def shear_box_finish(s, rng, contacts=None, bond_fraction=0.05, sigma0=10e3, v_top=0.01):
+    """Apply the shear-box run configuration to a grown packing: friction
+    mu = 0.5, no damping, gravity, a 10 kPa top wall (consolidate()) and a
+    driven top layer (shear()), then bond a fraction of the touching pairs."""
+    # defaultParams() makes mu_ws/mu_wd aliases of mu_s/mu_d, and shear()
+    # zeroes the wall friction: keep them separate arrays
+    s.mu_ws = s.mu_ws.copy()
+    s.mu_wd = s.mu_wd.copy()
+    s.gamma_n[0] = 0.0
+    s.g[:] = [0.0, 0.0, -9.81]
+    s.consolidate(normal_stress=sigma0)
+    z_extent = float(numpy.max(s.x[:, 2] + s.radius) - numpy.min(s.x[:, 2] - s.radius))
+    s.shear(shear_strain_rate=v_top / z_extent)
+    s.w_sigma0[0] = sigma0
+    s.mu_s[0] = 0.5
+    s.mu_d[0] = 0.5
+    nb = 0
+    if contacts is not None and bond_fraction > 0.0:
+        nb = bond_pairs(s, touching_pairs(contacts, int(s.np[0])), bond_fraction, rng)
+    s.time_current[0] = 0.0
+    s.time_total[0] = 1.0
+    s.time_file_dt[0] = 1.0
+    s.time_step_count[0] = 0
+    return nb
+
+
+def settling(n=10000, seed=1, steps=2000):
+    """Dry settling: radii ~U[0.009, 0.011] m on a jittered cubic lattice
+    (0.025 m spacing, jitter 10% of the spacing, 10 x 10 per layer) in a
+    0.25 x 0.25 m box with frictional walls; linear contacts with
+    k_n = k_t = 1.16e7 N/m, gamma_n = 0, mu = 0.5; dt = 0.17 sqrt(m_min/k_n).
+    The box is as tall as the lattice (2.5 m for 10,000 grains): 0.5 m cannot
+    hold 0.042 m^3 of grains."""
+    rng = numpy.random.default_rng(seed)
+    per_layer = 100
+    nz = (n + per_layer - 1) // per_layer
+    spacing = 0.025
+    x = lattice(n, spacing, 10, 10, 0.05, rng)
+    s = sim(np=n, sid="settling", nw=0)
+    s.x[:] = x
+    s.radius[:] = rng.uniform(0.009, 0.011, n)
+    s.defaultParams(mu_s=0.5, mu_d=0.5, k_n=1.16e7, k_t=1.16e7, gamma_wn=1e4, gamma_wt=1e4)
+    s.g[2] = -9.81
+    s.defineWorldBoundaries(L=[0.25, 0.25, nz * spacing])
+    m_min = 2600.0 * 4.0 / 3.0 * numpy.pi * numpy.min(s.radius) ** 3
+    dt = 0.17 * numpy.sqrt(m_min / 1.16e7)
+    s.time_dt[0] = dt
+    s.time_total[0] = steps * dt
+    s.time_file_dt[0] = steps * dt
+    return s

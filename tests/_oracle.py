@@ -41,6 +41,7 @@ def lib():
         L.orc_conv_log.argtypes = [vp, pu, ctypes.c_int]
         L.orc_read_darcy_field.argtypes = [vp, ctypes.c_int, pd]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_contact_slots.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -59,10 +60,12 @@ class OracleError(RuntimeError):
 class Oracle:
     """Same surface as sphere_amd.Engine, backed by the CPU oracle."""
 
-    def __init__(self, buf, threads=None):
+    def __init__(self, buf, threads=None, nc=32):
         L = lib()
         if threads is not None:
             L.orc_set_threads(int(threads))
+        self.nc = int(nc)
+        L.orc_set_contact_slots(self.nc)
         err = ctypes.create_string_buffer(512)
         self.h = L.orc_create(ctypes.byref(buf.state), err, 512)
         if not self.h:
@@ -110,13 +113,13 @@ class Oracle:
 
     def read_contacts(self):
         h = self.header()
-        n = max(h.np, 1) * _abi.NC
+        n = max(h.np, 1) * self.nc
         c = numpy.zeros(n, numpy.uint32)
         dm = numpy.zeros(3 * n)
         dt = numpy.zeros(3 * n)
         self.L.orc_read_contacts(self.h, _p(c, ctypes.c_uint32), _p(dm, ctypes.c_double),
                                  _p(dt, ctypes.c_double))
-        m = h.np * _abi.NC
+        m = h.np * self.nc
         return c[:m], dm[:3 * m].reshape(-1, 3), dt[:3 * m].reshape(-1, 3)
 
     def write_contacts(self, contacts, distmod, delta_t):

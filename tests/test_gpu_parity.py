@@ -81,7 +81,7 @@ def compare_contacts(engine, ora, steps):
 
 def run_pair(engine, s, chunks, per_chunk, fluid=False, label="", check_contacts=True):
     engine.load(s.to_state())
-    ora = Oracle(s.to_state())
+    ora = Oracle(s.to_state(), nc=engine.nc)
     done = 0
     ncont = 0
     for _ in range(chunks):
@@ -111,6 +111,35 @@ def test_dense_hertz_mindlin(engine):
     s = dense_case(n=729, seed=21, model=3, periodic=1, E=7e9, nu=0.3, gamma_n=0.0)
     _, nc = run_pair(engine, s, 3, 10, label="hertz")
     assert nc > 100
+
+
+def test_dense_hertz_64_slots():
+    """64 contact slots: same contacts, forces and state as the oracle with
+    64 slots, and the same slot order and forces as 32 slots (rows never hold
+    more than 32 contacts here)."""
+    s = dense_case(n=729, seed=22, model=3, periodic=1, E=7e9, nu=0.3)
+    e64 = sphere_amd.Engine(0, nc=64)
+    try:
+        run_pair(e64, s, 2, 10, label="hertz nc64")
+        e32 = sphere_amd.Engine(0, nc=32)
+        try:
+            e32.load(s.to_state())
+            e64.load(s.to_state())
+            e32.step(20)
+            e64.step(20)
+            a = e32.read()
+            b = e64.read()
+            n = int(s.np[0])
+            for name, w in (("x", 4), ("vel", 4), ("force", 3), ("torque", 3)):
+                assert numpy.array_equal(a.get(name, n, w), b.get(name, n, w)), name
+            c32 = e32.read_contacts()[0].reshape(n, 32)
+            c64 = e64.read_contacts()[0].reshape(n, 64)
+            assert numpy.array_equal(c64[:, 32:], c32)
+            assert numpy.all(c64[:, :32] == n)
+        finally:
+            e32.close()
+    finally:
+        e64.close()
 
 
 def test_dense_model1_capillary(engine):
