@@ -143,7 +143,18 @@ typedef struct sph_darcy_stats {
 #define SPH_K_WALLS 6
 #define SPH_K_DARCY_SETUP 7
 #define SPH_K_DARCY_SWEEP 8
-#define SPH_K_COUNT 9
+#define SPH_K_NLIST 9         /* neighbour-list rebuild (no reference counterpart) */
+#define SPH_K_COUNT 10
+
+/* Neighbour-list statistics (sph_get_neighbor_stats). */
+typedef struct {
+    uint32_t enabled;   /* 1 when the contact search uses the lists */
+    uint32_t range;     /* cells searched on each side at a rebuild */
+    uint32_t cap;       /* list entries per particle */
+    uint32_t overflow;  /* 1 once a list overflowed (rebuilt every step since) */
+    double skin;        /* listed surface gap [m] */
+    uint64_t rebuilds;  /* rebuilds since the last load */
+} sph_neighbor_stats;
 
 /* Context.  Replaces DEM::DEM(..., initCuda=1, device) (src/sphere.cpp:16-95)
  * and initializeGPU (src/device.cu:65-155); device < 0 picks device 0. */
@@ -161,6 +172,20 @@ const char *sph_last_error(const sph_ctx *ctx);
  * 64 serves packings whose largest grains touch more than 32 others, where the
  * reference writes past the slot row (src/contactsearch.cuh:298-299). */
 int sph_set_contact_slots(sph_ctx *ctx, int nc);
+
+/* Neighbour lists for contact models 2 and 3.  No counterpart in the
+ * reference, which searches the 27 cells around every particle each step
+ * (src/contactsearch.cuh:358-369); the lists give the same contacts, slot
+ * assignment and forces.  skin_fraction is the listed surface gap as a
+ * fraction of the largest radius (default 0.1; 0 disables the lists, and so
+ * does a grid that leaves no room for it); cap is the number of list entries
+ * per particle (1..128, default 128; a particle with more neighbours than
+ * that is searched over the cells, and the lists are then rebuilt every
+ * step).  Takes effect at the next load.  In list mode the cell sort runs
+ * only on rebuild steps, so sph_read_sort returns the arrays of the last
+ * rebuild. */
+int sph_set_neighbor_lists(sph_ctx *ctx, double skin_fraction, int cap);
+int sph_get_neighbor_stats(sph_ctx *ctx, sph_neighbor_stats *out);
 
 /* Load a state: replaces DEM::readbin + checkValues + transfer to the device
  * (src/file_io.cpp:30-417, src/sphere.cpp:145-345, src/device.cu:667-752).

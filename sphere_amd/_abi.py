@@ -29,7 +29,7 @@ ERRORS = {
 }
 
 K_NAMES = ["hash", "sort", "reorder", "contact", "bonds", "integrate", "walls",
-           "darcy_setup", "darcy_sweep"]
+           "darcy_setup", "darcy_sweep", "nlist"]
 
 _d = ctypes.c_double
 _pd = ctypes.POINTER(ctypes.c_double)
@@ -118,6 +118,12 @@ _CT = {numpy.float64: ctypes.c_double, numpy.uint32: ctypes.c_uint32,
        numpy.int32: ctypes.c_int32}
 
 
+class SphNeighborStats(ctypes.Structure):
+    _fields_ = [("enabled", ctypes.c_uint32), ("range", ctypes.c_uint32),
+                ("cap", ctypes.c_uint32), ("overflow", ctypes.c_uint32),
+                ("skin", ctypes.c_double), ("rebuilds", ctypes.c_uint64)]
+
+
 class StateBuffers:
     """Owns numpy arrays and a SphState that points at them."""
 
@@ -162,6 +168,8 @@ def load_library(path=None):
         "sph_destroy": (None, [vp]),
         "sph_last_error": (ctypes.c_char_p, [vp]),
         "sph_set_contact_slots": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sph_set_neighbor_lists": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int]),
+        "sph_get_neighbor_stats": (ctypes.c_int, [vp, ctypes.POINTER(SphNeighborStats)]),
         "sph_load_state": (ctypes.c_int, [vp, ctypes.POINTER(SphState)]),
         "sph_load_file": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
         "sph_get_header": (ctypes.c_int, [vp, ctypes.POINTER(SphState)]),
@@ -196,13 +204,20 @@ class SphereError(RuntimeError):
 class Engine:
     """One library context (one device, one loaded state)."""
 
-    def __init__(self, device=0, lib=None, nc=32):
+    def __init__(self, device=0, lib=None, nc=32, skin=None, nl_cap=128):
+        """nc: contact slots per particle (32 or 64); skin: neighbour-list
+        skin as a fraction of the largest radius (None keeps the library
+        default, 0 searches the 27 cells every step like the reference);
+        nl_cap: neighbour-list entries per particle."""
         self.lib = lib or load_library()
         self.ctx = self.lib.sph_create(device)
         if not self.ctx:
             raise SphereError(-4, self.lib.sph_last_error(None).decode())
         self.nc = int(nc)
         self._check(self.lib.sph_set_contact_slots(self.ctx, self.nc))
+        if skin is not None or nl_cap != 128:
+            self._check(self.lib.sph_set_neighbor_lists(
+                self.ctx, 0.1 if skin is None else float(skin), int(nl_cap)))
 
     def _check(self, rc):
         if rc != SPH_OK:
@@ -281,6 +296,11 @@ class Engine:
     def darcy_stats(self):
         s = SphDarcyStats()
         self._check(self.lib.sph_get_darcy_stats(self.ctx, ctypes.byref(s)))
+        return s
+
+    def neighbor_stats(self):
+        s = SphNeighborStats()
+        self._check(self.lib.sph_get_neighbor_stats(self.ctx, ctypes.byref(s)))
         return s
 
     def set_profiling(self, on):

@@ -54,6 +54,16 @@ struct StepArgs {
     double dt;
 };
 
+// Neighbour-list parameters (no counterpart in the reference, which searches
+// the 27 cells around every particle each step).
+#define NL_OVF 0xffffffffu  // list count of a particle whose list overflowed
+struct NLArgs {
+    double skin;     // listed pairs: surface gap below skin at the rebuild
+    double disp2;    // rebuild once a particle moved more than sqrt(disp2)
+    uint32_t cap;    // list entries per particle
+    uint32_t range;  // cells searched on each side at a rebuild (1 or 2)
+};
+
 struct DarcyArgs {
     uint32_t nx, ny, nz;
     double dx, dy, dz;

@@ -56,6 +56,58 @@ void launch_hash(hipStream_t s, const StepArgs &a, const DemBuffers &b)
     hipLaunchKernelGGL(k_hash, dim3(nb), dim3(256), 0, s, a, b.x, b.keys, b.vals, b.errflags);
 }
 
+// Neighbour-list mode: the same cell IDs plus the packed cell coordinates of
+// every particle (x | y << 11 | z << 22, all ones outside the grid) and the
+// rebuild decision of this step.  flags[cur] becomes non-zero when any
+// particle has moved more than sqrt(disp2) since the last rebuild, when the
+// host forces a rebuild, or when the sticky list-overflow bit flags[2] is
+// set; flags[cur ^ 1] (read by the previous step's kernels) is cleared here.
+__global__ void __launch_bounds__(256) k_hash_nl(StepArgs a, NLArgs n,
+                                                 const double4 *__restrict__ x,
+                                                 uint32_t *__restrict__ keys,
+                                                 uint32_t *__restrict__ vals,
+                                                 uint32_t *__restrict__ cellc,
+                                                 const double4 *__restrict__ x_ref,
+                                                 uint32_t *__restrict__ flags, uint32_t cur,
+                                                 uint32_t force, uint32_t *__restrict__ err)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) flags[cur ^ 1u] = 0u;
+    if (i >= a.np) return;
+    const Grid &g = a.grid;
+    const double4 xi = x[i];
+    const double fx = floor((xi.x - g.origo[0]) / (g.L[0] / g.num[0]));
+    const double fy = floor((xi.y - g.origo[1]) / (g.L[1] / g.num[1]));
+    const double fz = floor((xi.z - g.origo[2]) / (g.L[2] / g.num[2]));
+    uint32_t c = 0, packed = 0xffffffffu;
+    if (fx >= 0.0 && fx < (double)g.num[0] && fy >= 0.0 && fy < (double)g.num[1] && fz >= 0.0 &&
+        fz < (double)g.num[2]) {
+        const uint32_t ix = (uint32_t)fx, iy = (uint32_t)fy, iz = (uint32_t)fz;
+        c = (iz * g.num[1]) * g.num[0] + iy * g.num[0] + ix;
+        packed = ix | (iy << 11) | (iz << 22);
+    } else {
+        atomicOr(err, ERRF_GRID);
+    }
+    keys[i] = c;
+    vals[i] = i;
+    cellc[i] = packed;
+    const double4 xr = x_ref[i];
+    const double dx = xi.x - xr.x, dy = xi.y - xr.y, dz = xi.z - xr.z;
+    bool need = dx * dx + dy * dy + dz * dz > n.disp2;
+    if (i == 0 && (force || flags[2] != 0u)) need = true;
+    const uint64_t any = __ballot(need);
+    if (any != 0ull && (threadIdx.x & 63u) == (uint32_t)(__ffsll((unsigned long long)any) - 1))
+        atomicOr(&flags[cur], 1u);
+}
+
+void launch_hash_nl(hipStream_t s, const StepArgs &a, const NLArgs &n, const DemBuffers &b,
+                    uint32_t cur, uint32_t force)
+{
+    uint32_t nb = (a.np + 255) / 256;
+    hipLaunchKernelGGL(k_hash_nl, dim3(nb), dim3(256), 0, s, a, n, b.x, b.keys, b.vals, b.cellc,
+                       b.x_ref, b.nl_flags, cur, force, b.errflags);
+}
+
 // ------------------------------------------------------------------
 // A2: stable LSD radix sort (replaces thrust::sort_by_key)
 // ------------------------------------------------------------------
@@ -69,11 +121,18 @@ void launch_hash(hipStream_t s, const StepArgs &a, const DemBuffers &b)
 #define RS_TILE (RS_THREADS * RS_ITEMS)
 #define RS_WAVES (RS_THREADS / 64)
 
+// Kernels that take `cond` return at once when it is non-null and *cond is
+// zero (neighbour-list mode: the cell sort only runs on rebuild steps).
+#define SKIP_UNLESS(cond) \
+    if ((cond) != nullptr && *(cond) == 0u) return
+
 __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t *__restrict__ keys,
                                                            uint32_t n, int shift,
                                                            uint32_t *__restrict__ hist,
-                                                           uint32_t ntiles)
+                                                           uint32_t ntiles,
+                                                           const uint32_t *__restrict__ cond)
 {
+    SKIP_UNLESS(cond);
     __shared__ uint32_t h[256];
     uint32_t t = threadIdx.x;
     h[t] = 0;
@@ -88,8 +147,10 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t *__res
 }
 
 // Exclusive scan of m entries in place, one block of 1024 threads.
-__global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t *__restrict__ a, uint32_t m)
+__global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t *__restrict__ a, uint32_t m,
+                                                         const uint32_t *__restrict__ cond)
 {
+    SKIP_UNLESS(cond);
     __shared__ uint32_t part[1024];
     uint32_t t = threadIdx.x;
     uint32_t chunk = (m + 1023) / 1024;
@@ -117,8 +178,9 @@ __global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t *__restrict__ 
 __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, int shift,
-    const uint32_t *__restrict__ offs, uint32_t ntiles)
+    const uint32_t *__restrict__ offs, uint32_t ntiles, const uint32_t *__restrict__ cond)
 {
+    SKIP_UNLESS(cond);
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[RS_WAVES][256];
     uint32_t t = threadIdx.x;
@@ -160,7 +222,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(
     }
 }
 
-void launch_sort(hipStream_t s, uint32_t n, int bits, DemBuffers &b)
+void launch_sort(hipStream_t s, uint32_t n, int bits, DemBuffers &b, const uint32_t *cond)
 {
     if (n == 0) return;
     uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
@@ -169,10 +231,11 @@ void launch_sort(hipStream_t s, uint32_t n, int bits, DemBuffers &b)
     for (int pass = 0; pass < passes; ++pass) {
         int shift = pass * 8;
         hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(RS_THREADS), 0, s, b.keys, n, shift,
-                           b.hist, ntiles);
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, b.hist, 256u * ntiles);
+                           b.hist, ntiles, cond);
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, b.hist, 256u * ntiles,
+                           cond);
         hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(RS_THREADS), 0, s, b.keys, b.vals,
-                           b.keys_tmp, b.vals_tmp, n, shift, b.hist, ntiles);
+                           b.keys_tmp, b.vals_tmp, n, shift, b.hist, ntiles, cond);
         uint32_t *tk = b.keys;
         b.keys = b.keys_tmp;
         b.keys_tmp = tk;
@@ -185,6 +248,14 @@ void launch_sort(hipStream_t s, uint32_t n, int bits, DemBuffers &b)
 // ------------------------------------------------------------------
 // A3: reorder + cellStart/cellEnd
 // ------------------------------------------------------------------
+
+__global__ void k_fill_u32_if(uint32_t *__restrict__ p, size_t n, uint32_t v,
+                              const uint32_t *__restrict__ cond)
+{
+    SKIP_UNLESS(cond);
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
 
 __global__ void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v)
 {
@@ -216,8 +287,9 @@ __global__ void __launch_bounds__(256) k_reorder(
     const double4 *__restrict__ x, const double4 *__restrict__ vel,
     const double4 *__restrict__ angvel, double4 *__restrict__ x_s, double4 *__restrict__ vel_s,
     double4 *__restrict__ angvel_s, uint32_t *__restrict__ cell_start,
-    uint32_t *__restrict__ cell_end)
+    uint32_t *__restrict__ cell_end, const uint32_t *__restrict__ cond)
 {
+    SKIP_UNLESS(cond);
     uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= np) return;
     uint32_t c = keys[idx];
@@ -232,13 +304,17 @@ __global__ void __launch_bounds__(256) k_reorder(
     angvel_s[idx] = angvel[s];
 }
 
-void launch_reorder(hipStream_t s, const StepArgs &a, const DemBuffers &b, uint32_t ncells)
+void launch_reorder(hipStream_t s, const StepArgs &a, const DemBuffers &b, uint32_t ncells,
+                    const uint32_t *cond)
 {
-    launch_fill_u32(s, b.cell_start, ncells, EMPTY_CELL);
-    launch_fill_u32(s, b.cell_end, ncells, EMPTY_CELL);
+    const dim3 fg((ncells + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u32_if, fg, dim3(256), 0, s, b.cell_start, (size_t)ncells,
+                       EMPTY_CELL, cond);
+    hipLaunchKernelGGL(k_fill_u32_if, fg, dim3(256), 0, s, b.cell_end, (size_t)ncells,
+                       EMPTY_CELL, cond);
     uint32_t nb = (a.np + 255) / 256;
     hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(256), 0, s, a.np, b.keys, b.vals, b.x, b.vel,
-                       b.angvel, b.x_s, b.vel_s, b.angvel_s, b.cell_start, b.cell_end);
+                       b.angvel, b.x_s, b.vel_s, b.angvel_s, b.cell_start, b.cell_end, cond);
 }
 
 // ------------------------------------------------------------------
@@ -282,6 +358,256 @@ __device__ __forceinline__ d3 decode_dm(const Grid &g, uint32_t code)
 // ------------------------------------------------------------------
 // A7-A10: contact laws
 // ------------------------------------------------------------------
+
+// ------------------------------------------------------------------
+// Neighbour (Verlet) lists
+// ------------------------------------------------------------------
+// A rebuild step lists, for every particle, each other particle whose
+// surface gap was below `skin` over the (2R+1)^3 cells around it.  Until a
+// particle moves more than skin/2 (less a margin) no pair outside the lists
+// can come into contact, so the per-step contact search only tests list
+// entries.  The list is column-major, entry k of sorted position p at
+// nl[k * np + p], so a wave reads consecutive words.
+
+__global__ void __launch_bounds__(256) k_nl_build(
+    StepArgs a, NLArgs n, const uint32_t *__restrict__ cond, const uint32_t *__restrict__ vals,
+    const uint32_t *__restrict__ cell_start, const uint32_t *__restrict__ cell_end,
+    const double4 *__restrict__ x_s, uint32_t *__restrict__ nl, uint32_t *__restrict__ nl_cnt,
+    uint32_t *__restrict__ nl_perm, double4 *__restrict__ x_ref, uint32_t *__restrict__ flags)
+{
+    SKIP_UNLESS(cond);
+    const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t np = a.np;
+    if (pos >= np) return;
+    if (pos == 0) flags[3] += 1u;  // rebuild counter
+    const Grid &g = a.grid;
+    const uint32_t a_orig = vals[pos];
+    const double4 xa = x_s[pos];
+    nl_perm[pos] = a_orig;
+    x_ref[a_orig] = xa;
+    const int gx = (int)floor((xa.x - g.origo[0]) / (g.L[0] / g.num[0]));
+    const int gy = (int)floor((xa.y - g.origo[1]) / (g.L[1] / g.num[1]));
+    const int gz = (int)floor((xa.z - g.origo[2]) / (g.L[2] / g.num[2]));
+    const bool inside = gx >= 0 && gx < (int)g.num[0] && gy >= 0 && gy < (int)g.num[1] &&
+                        gz >= 0 && gz < (int)g.num[2];
+    uint32_t cnt = 0;
+    const int R = (int)n.range;
+    if (inside) {
+        for (int zd = -R; zd <= R; ++zd)
+            for (int yd = -R; yd <= R; ++yd)
+                for (int xd = -R; xd <= R; ++xd) {
+                    int tx = gx + xd, ty = gy + yd, tz = gz + zd;
+                    d3 dm = mk3(0.0, 0.0, 0.0);
+                    uint32_t code = 0;
+                    if (dist_mod(g, tx, ty, tz, dm, code) == -1) continue;
+                    const uint32_t cid = (uint32_t)tx + (uint32_t)ty * g.num[0] +
+                                         (g.num[0] * g.num[1]) * (uint32_t)tz;
+                    const uint32_t start = cell_start[cid];
+                    if (start == EMPTY_CELL) continue;
+                    const uint32_t end = cell_end[cid];
+                    for (uint32_t idx_b = start; idx_b < end; ++idx_b) {
+                        if (idx_b == pos) continue;
+                        const double4 xb = x_s[idx_b];
+                        const d3 x_ab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z) + dm;
+                        const double rr = xa.w + xb.w + n.skin;
+                        if (dot3(x_ab, x_ab) < rr * rr) {
+                            if (cnt < n.cap) nl[(size_t)cnt * np + pos] = vals[idx_b];
+                            ++cnt;
+                        }
+                    }
+                }
+    }
+    if (cnt > n.cap) {
+        nl_cnt[pos] = NL_OVF;
+        atomicOr(&flags[2], 1u);
+    } else {
+        nl_cnt[pos] = cnt;
+    }
+}
+
+void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const DemBuffers &b,
+                     const uint32_t *cond)
+{
+    uint32_t nb = (a.np + 255) / 256;
+    hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(256), 0, s, a, n, cond, b.vals, b.cell_start,
+                       b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_perm, b.x_ref, b.nl_flags);
+}
+
+// Image of list entry b as the reference's 27-cell search would reach it
+// this step: b counts only if its cell is one of the 27 around the cell
+// (gx, gy, gz) (periodic axes wrapped to the nearest image; valid for 3 or
+// more cells along a periodic axis).  offc = (dz+1)*9 + (dy+1)*3 + (dx+1) is
+// the position of that cell in the search order of
+// src/contactsearch.cuh:358-369.
+__device__ __forceinline__ bool nl_image(const Grid &g, int gx, int gy, int gz, uint32_t cb,
+                                         uint32_t &offc, uint32_t &code, d3 &dm)
+{
+    if (cb == 0xffffffffu) return false;  // outside the grid (the step reports it)
+    int dx = (int)(cb & 2047u) - gx;
+    int dy = (int)((cb >> 11) & 2047u) - gy;
+    const int dz = (int)(cb >> 22) - gz;
+    if (g.periodic == 1 || g.periodic == 2) {
+        if (dx > 1) dx -= (int)g.num[0];
+        else if (dx < -1) dx += (int)g.num[0];
+    }
+    if (g.periodic == 1) {
+        if (dy > 1) dy -= (int)g.num[1];
+        else if (dy < -1) dy += (int)g.num[1];
+    }
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return false;
+    int tx = gx + dx, ty = gy + dy;
+    dm = mk3(0.0, 0.0, 0.0);
+    code = 0;
+    if (dist_mod(g, tx, ty, gz + dz, dm, code) == -1) return false;
+    offc = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+    return true;
+}
+
+// Slot holding b among the occupied slots of a row, or -1.
+__device__ __forceinline__ int find_slot(const uint32_t *row, uint64_t mask, uint32_t b)
+{
+    while (mask) {
+        const int i = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1;
+        if (row[i] == b) return i;
+    }
+    return -1;
+}
+
+// 128-bit set over list positions
+struct Set128 {
+    uint64_t lo, hi;
+};
+__device__ __forceinline__ void set_bit(Set128 &m, uint32_t k)
+{
+    if (k < 64) m.lo |= 1ull << k;
+    else m.hi |= 1ull << (k - 64);
+}
+__device__ __forceinline__ void clear_bit(Set128 &m, uint32_t k)
+{
+    if (k < 64) m.lo &= ~(1ull << k);
+    else m.hi &= ~(1ull << (k - 64));
+}
+__device__ __forceinline__ bool any_bit(const Set128 &m) { return (m.lo | m.hi) != 0ull; }
+
+// List entry of `m` with the smallest discovery key (offc, b): the order in
+// which the reference's cell search meets the contacts (cells in search
+// order, then ascending original index inside a cell, which is the order of
+// the stable cell sort).
+__device__ __forceinline__ uint32_t nl_first(const Grid &g, int gx, int gy, int gz, Set128 m,
+                                             const uint32_t *__restrict__ nl, uint32_t np,
+                                             uint32_t pos, const uint32_t *__restrict__ cellc,
+                                             uint32_t &b_out, uint32_t &offc_out,
+                                             uint32_t &code_out)
+{
+    uint64_t best = ~0ull;
+    uint32_t bk = 0;
+    for (int part = 0; part < 2; ++part) {
+        uint64_t w = part ? m.hi : m.lo;
+        while (w) {
+            const uint32_t k = (uint32_t)(__ffsll((unsigned long long)w) - 1) + 64u * part;
+            w &= w - 1;
+            const uint32_t b = nl[(size_t)k * np + pos];
+            uint32_t offc = 0, code = 0;
+            d3 dm;
+            (void)nl_image(g, gx, gy, gz, cellc[b], offc, code, dm);
+            const uint64_t key = ((uint64_t)offc << 32) | b;
+            if (key < best) {
+                best = key;
+                bk = k;
+                b_out = b;
+                offc_out = offc;
+                code_out = code;
+            }
+        }
+    }
+    return bk;
+}
+
+// Topology from the neighbour list with the slot rule of
+// src/contactsearch.cuh:267-286.  Returns true on slot overflow.
+__device__ bool topology_list(const Grid &g, uint32_t np, uint32_t pos, uint32_t cnt,
+                              const uint32_t *__restrict__ nl, const uint32_t *__restrict__ cellc,
+                              const double4 *__restrict__ x, double4 xa, int gx, int gy, int gz,
+                              uint32_t *row, uint8_t *crow, uint64_t &mask, int NCS,
+                              uint64_t full)
+{
+    Set128 found = {0ull, 0ull}, fresh = {0ull, 0ull};
+    int n_found = 0, n_new = 0;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const uint32_t b = nl[(size_t)k * np + pos];
+        const uint32_t cb = cellc[b];
+        const double4 xb = x[b];
+        uint32_t offc, code;
+        d3 dm;
+        if (!nl_image(g, gx, gy, gz, cb, offc, code, dm)) continue;
+        const d3 x_ab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z) + dm;
+        const double s2 = dot3(x_ab, x_ab);
+        const double rsum = xa.w + xb.w;
+        if (s2 >= rsum * rsum * (1.0 + 1e-9)) continue;
+        const double delta = sqrt(s2) - rsum;
+        if (!(delta < 0.0)) continue;
+        ++n_found;
+        set_bit(found, k);
+        if (find_slot(row, mask, b) < 0) {
+            set_bit(fresh, k);
+            ++n_new;
+        }
+    }
+    if (n_found <= NCS && __popcll(mask) + n_new <= NCS) {
+        // No slot runs out: contacts already in the row keep their slots
+        // (their image code is refreshed) and the new ones take the highest
+        // empty slot in discovery order.
+        Set128 old = {found.lo & ~fresh.lo, found.hi & ~fresh.hi};
+        for (int part = 0; part < 2; ++part) {
+            uint64_t w = part ? old.hi : old.lo;
+            while (w) {
+                const uint32_t k = (uint32_t)(__ffsll((unsigned long long)w) - 1) + 64u * part;
+                w &= w - 1;
+                const uint32_t b = nl[(size_t)k * np + pos];
+                uint32_t offc = 0, code = 0;
+                d3 dm;
+                (void)nl_image(g, gx, gy, gz, cellc[b], offc, code, dm);
+                crow[find_slot(row, mask, b)] = (uint8_t)code;
+            }
+        }
+        while (any_bit(fresh)) {
+            uint32_t b = 0, offc = 0, code = 0;
+            const uint32_t k = nl_first(g, gx, gy, gz, fresh, nl, np, pos, cellc, b, offc, code);
+            clear_bit(fresh, k);
+            const int cpos = 63 - __clzll((long long)(~mask & full));
+            row[cpos] = b;
+            crow[cpos] = (uint8_t)code;
+            mask |= 1ull << cpos;
+        }
+        return false;
+    }
+    // General case: every contact in discovery order through the full rule.
+    int nc = 0;
+    int broken = -1;  // search cell left early (more than NCS contacts)
+    bool ovf = false;
+    while (any_bit(found)) {
+        uint32_t b = 0, offc = 0, code = 0;
+        const uint32_t k = nl_first(g, gx, gy, gz, found, nl, np, pos, cellc, b, offc, code);
+        clear_bit(found, k);
+        if ((int)offc == broken) continue;
+        int cpos = find_slot(row, mask, b);
+        if (cpos < 0) {
+            const uint64_t emp = ~mask & full;
+            cpos = emp ? 63 - __clzll((long long)emp) : nc;
+        }
+        if (cpos < NCS) {
+            row[cpos] = b;
+            crow[cpos] = (uint8_t)code;
+            mask |= 1ull << cpos;
+        } else {
+            ovf = true;
+        }
+        ++nc;
+        if (nc > NCS) broken = (int)offc;
+    }
+    return ovf;
+}
 
 struct Acc {
     d3 F, T;
@@ -525,6 +851,12 @@ __device__ __forceinline__ double law_wall(const StepArgs &a, Acc &A, double4 va
 // least (r_a+r_b)^2 (1+1e-9): then sqrt(s) > r_a+r_b and the reference's
 // test sqrt(s) - (r_a+r_b) < 0 is false, so the skip never changes it.
 
+// NL = false: one thread per sorted position, 27-cell search every step
+// (the reference's pipeline).  NL = true: one thread per position of the last
+// list rebuild; the search tests the particle's list entries, or the 27
+// cells when its list overflowed (then the sort runs every step, so the
+// sorted arrays are current).
+template <bool NL>
 __global__ void __launch_bounds__(256) k_contact(
     StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
     const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x,
@@ -536,14 +868,16 @@ __global__ void __launch_bounds__(256) k_contact(
     double *__restrict__ p_out, const double4 *__restrict__ w_nx,
     const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
     uint32_t *__restrict__ contacts, uint64_t *__restrict__ cmask,
-    uint8_t *__restrict__ dmcode, double *__restrict__ delta_t, uint32_t *__restrict__ err)
+    uint8_t *__restrict__ dmcode, double *__restrict__ delta_t, uint32_t *__restrict__ err,
+    const uint32_t *__restrict__ nl, const uint32_t *__restrict__ nl_cnt,
+    const uint32_t *__restrict__ nl_perm, const uint32_t *__restrict__ cellc)
 {
     const uint32_t idx_a = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx_a >= a.np) return;
     const Grid &g = a.grid;
     const uint32_t np = a.np;
-    const uint32_t a_orig = vals[idx_a];
-    const double4 xa = x_s[idx_a];
+    const uint32_t a_orig = NL ? nl_perm[idx_a] : vals[idx_a];
+    const double4 xa = NL ? x[a_orig] : x_s[idx_a];
     const double radius_a = xa.w;
     const int model = (int)a.prm.contactmodel;
 
@@ -554,8 +888,8 @@ __global__ void __launch_bounds__(256) k_contact(
     A.ev_dot = 0.0;
     A.p = 0.0;
 
-    const double4 va4 = vel_s[idx_a];
-    const double4 wa4 = angvel_s[idx_a];
+    const double4 va4 = NL ? vel[a_orig] : vel_s[idx_a];
+    const double4 wa4 = NL ? angvel[a_orig] : angvel_s[idx_a];
     // contactsearch.cuh:352-354
     const int gx = (int)floor((xa.x - g.origo[0]) / (g.L[0] / g.num[0]));
     const int gy = (int)floor((xa.y - g.origo[1]) / (g.L[1] / g.num[1]));
@@ -574,6 +908,11 @@ __global__ void __launch_bounds__(256) k_contact(
         uint64_t mask = cmask[a_orig];
         int nc = 0;
         bool ovf = false;
+        const uint32_t lcnt = NL ? nl_cnt[idx_a] : NL_OVF;
+        if (NL && lcnt != NL_OVF) {
+            ovf = topology_list(g, np, idx_a, lcnt, nl, cellc, x, xa, gx, gy, gz, row, crow, mask,
+                                NCS, full);
+        } else {
         // topology, contactsearch.cuh:358-369 (z, y, x order)
         for (int zd = -1; zd < 2; ++zd)
             for (int yd = -1; yd < 2; ++yd)
@@ -621,6 +960,7 @@ __global__ void __launch_bounds__(256) k_contact(
                         if (nc > NCS) break;
                     }
                 }
+        }
         if (ovf) atomicOr(err, ERRF_OVERFLOW);
 
         // interact, contactsearch.cuh:484-559: slots in ascending order
@@ -729,13 +1069,21 @@ __global__ void __launch_bounds__(256) k_contact(
     for (uint32_t w = 0; w < nw && w < 5; ++w) walls_force_pp[a_orig + (size_t)np * w] = wf[w];
 }
 
-void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b)
+void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist)
 {
     uint32_t nb = (a.np + 255) / 256;
-    hipLaunchKernelGGL(k_contact, dim3(nb), dim3(256), 0, s, a, b.vals, b.cell_start, b.cell_end,
-                       b.x, b.x_s, b.vel_s, b.angvel_s, b.vel, b.angvel, b.force, b.torque,
-                       b.es_dot, b.ev_dot, b.es, b.ev, b.p, b.w_nx, b.w_mvfd, b.walls_force_pp,
-                       b.contacts, b.cmask, b.dmcode, b.delta_t, b.errflags);
+    if (nlist)
+        hipLaunchKernelGGL(k_contact<true>, dim3(nb), dim3(256), 0, s, a, b.vals, b.cell_start,
+                           b.cell_end, b.x, b.x_s, b.vel_s, b.angvel_s, b.vel, b.angvel, b.force,
+                           b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p, b.w_nx, b.w_mvfd,
+                           b.walls_force_pp, b.contacts, b.cmask, b.dmcode, b.delta_t,
+                           b.errflags, b.nl, b.nl_cnt, b.nl_perm, b.cellc);
+    else
+        hipLaunchKernelGGL(k_contact<false>, dim3(nb), dim3(256), 0, s, a, b.vals, b.cell_start,
+                           b.cell_end, b.x, b.x_s, b.vel_s, b.angvel_s, b.vel, b.angvel, b.force,
+                           b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p, b.w_nx, b.w_mvfd,
+                           b.walls_force_pp, b.contacts, b.cmask, b.dmcode, b.delta_t,
+                           b.errflags, nullptr, nullptr, nullptr, nullptr);
 }
 
 // ------------------------------------------------------------------

@@ -9,6 +9,7 @@
 // 10 sweeps, output events, and the end of each call.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -47,6 +48,13 @@ struct sph_ctx {
     uint32_t np = 0, nb0 = 0, nw = 0, ncells = 0;
     uint32_t nc = 32;  // contact slots per particle
     int key_bits = 1;
+    // neighbour lists
+    double nl_frac = 0.1;    // requested skin / largest radius
+    uint32_t nl_cap = 128;   // list entries per particle
+    bool nl_on = false;
+    NLArgs nla{};
+    uint32_t nl_parity = 0;
+    uint32_t nl_force = 1;
     // time loop host variables, src/device.cu:913-1036
     long iter = 0;
     double filetimeclock = 0.0;
@@ -433,6 +441,41 @@ static int upload(sph_ctx *ctx)
         ctx->p_top_orig = p[host_didx(h, 0, 0, h.num[2] - 1)] -
                           h.p_mod_A * sin(2.0 * M_PI * h.p_mod_f * h.time_current + h.p_mod_phi);
     }
+    // Neighbour lists: contact models 2 and 3 on a fixed grid whose cells
+    // leave room for the skin within `range` cells, with at least 2 range + 1
+    // cells along periodic axes and cell coordinates that pack into 32 bits.
+    ctx->nl_on = false;
+    if (np > 0 && ctx->nl_frac > 0.0 && (h.contactmodel == 2 || h.contactmodel == 3) &&
+        !(h.adaptive == 1 && h.nw > 0) && h.num[0] <= 2048 && h.num[1] <= 2048 &&
+        h.num[2] <= 1024) {
+        double rmax = 0.0;
+        for (size_t i = 0; i < np; ++i) rmax = std::max(rmax, hs.x[4 * i + 3]);
+        double cmin = 1e300;
+        for (int d = 0; d < 3; ++d) cmin = std::min(cmin, h.L[d] / h.num[d]);
+        const double skin = ctx->nl_frac * rmax;
+        uint32_t range = 0;
+        if (cmin >= (2.0 * rmax + skin) * (1.0 + 1e-9)) range = 1;
+        else if (2.0 * cmin >= (2.0 * rmax + skin) * (1.0 + 1e-9)) range = 2;
+        const uint32_t need = 2 * range + 1;
+        bool ok = range > 0;
+        if (h.periodic == 1 || h.periodic == 2) ok = ok && h.num[0] >= need;
+        if (h.periodic == 1) ok = ok && h.num[1] >= need;
+        if (ok) {
+            ctx->nla.skin = skin;
+            // both partners move at most 0.45 skin before a rebuild
+            ctx->nla.disp2 = (0.45 * skin) * (0.45 * skin);
+            ctx->nla.cap = ctx->nl_cap;
+            ctx->nla.range = range;
+            AL(b.cellc, uint32_t, np); AL(b.x_ref, double4, np);
+            AL(b.nl, uint32_t, (size_t)ctx->nl_cap * np); AL(b.nl_cnt, uint32_t, np);
+            AL(b.nl_perm, uint32_t, np); AL(b.nl_flags, uint32_t, 4);
+            launch_fill_u32(s, b.nl_flags, 4, 0u);
+            launch_fill_f64(s, (double *)b.x_ref, np * 4, 0.0);
+            ctx->nl_on = true;
+        }
+    }
+    ctx->nl_parity = 0;
+    ctx->nl_force = 1;
 #undef AL
     HIPCHK(hipStreamSynchronize(s));
     ctx->iter = 0;
@@ -724,10 +767,26 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             h.L[2] = *ctx->h_d;
         }
         const StepArgs a = step_args(ctx);
-        { ProfScope ps(ctx, SPH_K_HASH); launch_hash(s, a, ctx->db); }
-        { ProfScope ps(ctx, SPH_K_SORT); launch_sort(s, a.np, ctx->key_bits, ctx->db); }
-        { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells); }
-        { ProfScope ps(ctx, SPH_K_CONTACT); launch_contact(s, a, ctx->db); }
+        if (ctx->nl_on) {
+            // Lists are rebuilt (sort, reorder, list build) only on steps
+            // whose device-side flag is set; the Darcy solve reads the
+            // sorted arrays, so a fluid run rebuilds every step.
+            const uint32_t cur = ctx->nl_parity;
+            const uint32_t *cond = ctx->db.nl_flags + cur;
+            const uint32_t force = ctx->nl_force || h.fluid ? 1u : 0u;
+            { ProfScope ps(ctx, SPH_K_HASH); launch_hash_nl(s, a, ctx->nla, ctx->db, cur, force); }
+            { ProfScope ps(ctx, SPH_K_SORT); launch_sort(s, a.np, ctx->key_bits, ctx->db, cond); }
+            { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells, cond); }
+            { ProfScope ps(ctx, SPH_K_NLIST); launch_nl_build(s, a, ctx->nla, ctx->db, cond); }
+            { ProfScope ps(ctx, SPH_K_CONTACT); launch_contact(s, a, ctx->db, true); }
+            ctx->nl_parity ^= 1u;
+            ctx->nl_force = 0;
+        } else {
+            { ProfScope ps(ctx, SPH_K_HASH); launch_hash(s, a, ctx->db); }
+            { ProfScope ps(ctx, SPH_K_SORT); launch_sort(s, a.np, ctx->key_bits, ctx->db, nullptr); }
+            { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells, nullptr); }
+            { ProfScope ps(ctx, SPH_K_CONTACT); launch_contact(s, a, ctx->db, false); }
+        }
         if (h.nb0 > 0) { ProfScope ps(ctx, SPH_K_BONDS); launch_bonds(s, a, ctx->db); }
     }
     if (h.fluid) {
@@ -855,6 +914,35 @@ int sph_set_contact_slots(sph_ctx *ctx, int nc)
     if (ctx->loaded && (uint32_t)nc != ctx->nc)
         return fail(ctx, SPH_ERR_STATE, "set the contact slot count before loading a state");
     ctx->nc = (uint32_t)nc;
+    return SPH_OK;
+}
+
+int sph_set_neighbor_lists(sph_ctx *ctx, double skin_fraction, int cap)
+{
+    if (!ctx) return fail(ctx, SPH_ERR_ARG, "null context");
+    if (!(skin_fraction >= 0.0 && skin_fraction <= 1.0))
+        return fail(ctx, SPH_ERR_ARG, "skin fraction must lie in [0, 1]");
+    if (cap < 1 || cap > 128) return fail(ctx, SPH_ERR_ARG, "list capacity must lie in [1, 128]");
+    ctx->nl_frac = skin_fraction;
+    ctx->nl_cap = (uint32_t)cap;
+    return SPH_OK;
+}
+
+int sph_get_neighbor_stats(sph_ctx *ctx, sph_neighbor_stats *out)
+{
+    if (!ctx || !out) return fail(ctx, SPH_ERR_ARG, "null argument");
+    *out = sph_neighbor_stats{};
+    if (!ctx->loaded || !ctx->nl_on) return SPH_OK;
+    (void)hipSetDevice(ctx->device);
+    uint32_t f[4];
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(f, ctx->db.nl_flags, sizeof(f), hipMemcpyDeviceToHost));
+    out->enabled = 1;
+    out->range = ctx->nla.range;
+    out->cap = ctx->nla.cap;
+    out->overflow = f[2] ? 1u : 0u;
+    out->skin = ctx->nla.skin;
+    out->rebuilds = f[3];
     return SPH_OK;
 }
 

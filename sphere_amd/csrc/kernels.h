@@ -33,13 +33,26 @@ struct DemBuffers {
     uint32_t *bp_list, *bp_off, *bp_ent;  // bonded particles, CSR offsets, entries
     uint32_t nbp;
     uint32_t *errflags;
+    // neighbour lists
+    uint32_t *cellc;     // np, packed cell coordinates of this step
+    double4 *x_ref;      // np, positions at the last rebuild
+    uint32_t *nl;        // cap*np, column-major by rebuild position
+    uint32_t *nl_cnt;    // np, entries per rebuild position (NL_OVF: overflow)
+    uint32_t *nl_perm;   // np, original index at each rebuild position
+    uint32_t *nl_flags;  // [0],[1] rebuild flag by step parity, [2] sticky overflow, [3] rebuilds
 };
 
 void launch_hash(hipStream_t s, const StepArgs &a, const DemBuffers &b);
-// Stable LSD radix sort of (keys, vals) by key (bits = key width).
-void launch_sort(hipStream_t s, uint32_t n, int bits, DemBuffers &b);
-void launch_reorder(hipStream_t s, const StepArgs &a, const DemBuffers &b, uint32_t ncells);
-void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b);
+void launch_hash_nl(hipStream_t s, const StepArgs &a, const NLArgs &n, const DemBuffers &b,
+                    uint32_t cur, uint32_t force);
+// Stable LSD radix sort of (keys, vals) by key (bits = key width).  With a
+// non-null cond every kernel returns at once if *cond == 0 on the device.
+void launch_sort(hipStream_t s, uint32_t n, int bits, DemBuffers &b, const uint32_t *cond);
+void launch_reorder(hipStream_t s, const StepArgs &a, const DemBuffers &b, uint32_t ncells,
+                    const uint32_t *cond);
+void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const DemBuffers &b,
+                     const uint32_t *cond);
+void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist);
 void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b);
 void launch_shear_stress(hipStream_t s, const StepArgs &a, const DemBuffers &b);
 void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,

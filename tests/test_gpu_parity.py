@@ -11,7 +11,7 @@ import pytest
 
 import sphere_amd
 from _cases import add_bonds, darcy_case, dense_case, settling_case
-from _oracle import Oracle
+from _oracle import Oracle, OracleError
 
 pytestmark = pytest.mark.gpu
 
@@ -24,9 +24,24 @@ PARTICLE_FIELDS = [("x", 4), ("vel", 4), ("angvel", 3), ("force", 3), ("torque",
 
 @pytest.fixture(scope="module")
 def engine():
+    """Default engine: neighbour lists on (sorted arrays current only on
+    rebuild steps)."""
     e = sphere_amd.Engine(0)
     yield e
     e.close()
+
+
+@pytest.fixture(scope="module")
+def engine_cells():
+    """The reference's pipeline: sort and 27-cell search every step."""
+    e = sphere_amd.Engine(0, skin=0.0)
+    yield e
+    e.close()
+
+
+@pytest.fixture(params=["lists", "cells"])
+def any_engine(request, engine, engine_cells):
+    return engine if request.param == "lists" else engine_cells
 
 
 def rel_diff(a, b):
@@ -82,35 +97,165 @@ def compare_contacts(engine, ora, steps):
 def run_pair(engine, s, chunks, per_chunk, fluid=False, label="", check_contacts=True):
     engine.load(s.to_state())
     ora = Oracle(s.to_state(), nc=engine.nc)
+    # in list mode the cell sort is only current on rebuild steps
+    check_sort = fluid or not engine.neighbor_stats().enabled
     done = 0
     ncont = 0
     for _ in range(chunks):
         engine.step(per_chunk)
         ora.step(per_chunk)
         done += per_chunk
-        compare_sort(engine, ora)
+        if check_sort:
+            compare_sort(engine, ora)
         if check_contacts:
             ncont = compare_contacts(engine, ora, done)
         compare_state(engine.read(), ora.read(), int(s.np[0]), done, fluid=fluid, label=label)
     return ora, ncont
 
 
-def test_settling_lattice(engine):
+def test_settling_lattice(any_engine):
     s = settling_case(n=2000, seed=5, steps=100)
-    run_pair(engine, s, 2, 50, label="settling")
+    run_pair(any_engine, s, 2, 50, label="settling")
 
 
 @pytest.mark.parametrize("periodic", [0, 1, 2])
-def test_dense_linear(engine, periodic):
+def test_dense_linear(any_engine, periodic):
     s = dense_case(n=729, seed=11 + periodic, model=2, periodic=periodic, gamma_n=50.0, gamma_t=20.0)
-    _, nc = run_pair(engine, s, 3, 10, label="linear p%d" % periodic)
+    _, nc = run_pair(any_engine, s, 3, 10, label="linear p%d" % periodic)
     assert nc > 100
 
 
-def test_dense_hertz_mindlin(engine):
+def test_dense_hertz_mindlin(any_engine):
     s = dense_case(n=729, seed=21, model=3, periodic=1, E=7e9, nu=0.3, gamma_n=0.0)
-    _, nc = run_pair(engine, s, 3, 10, label="hertz")
+    _, nc = run_pair(any_engine, s, 3, 10, label="hertz")
     assert nc > 100
+
+
+def test_lists_enabled_on_test_grids(engine):
+    """Cells narrower than a diameter plus the skin (the dense and settling
+    grids) are searched two cells deep at a rebuild, wider ones one deep."""
+    engine.load(dense_case(n=729, seed=21, model=3, periodic=1, E=7e9).to_state())
+    st = engine.neighbor_stats()
+    assert st.enabled == 1 and st.range == 2 and st.cap == 128
+    engine.load(settling_case(n=2000, seed=5, steps=100).to_state())
+    assert engine.neighbor_stats().range == 2
+    s = crowded_case(12)
+    s.num[:] = 3  # cells of 0.067 m around grains of up to 0.03 m
+    engine.load(s.to_state())
+    assert engine.neighbor_stats().range == 1
+    s = dense_case(n=512, seed=31, model=1, periodic=0, kappa=0.05)
+    engine.load(s.to_state())
+    assert engine.neighbor_stats().enabled == 0  # model 1 searches cells
+
+
+def test_lists_frequent_rebuilds():
+    """A skin of 0.2% of the largest radius: the lists are rebuilt every few
+    steps, and every step still matches the cell search of the oracle."""
+    s = dense_case(n=729, seed=71, model=3, periodic=1, E=7e9, nu=0.3, vmag=0.5)
+    e = sphere_amd.Engine(0, skin=0.002)
+    try:
+        run_pair(e, s, 4, 5, label="frequent rebuilds")
+        assert e.neighbor_stats().rebuilds >= 4
+    finally:
+        e.close()
+
+
+def test_lists_periodic_crossing():
+    """Every grain drifts at 2 m/s along x (plus noise) with a layer sitting on
+    the periodic boundary: grains and contacts cross it, images change."""
+    s = dense_case(n=729, seed=72, model=2, periodic=1, gamma_n=50.0)
+    L0 = float(s.L[0])
+    s.x[:, 0] = numpy.mod(s.x[:, 0] + 0.5 * L0 - 1e-4, L0)
+    s.vel[:, 0] += 2.0
+    e = sphere_amd.Engine(0, skin=0.05)
+    try:
+        _, nc = run_pair(e, s, 4, 5, label="crossing")
+        assert nc > 100
+        assert e.neighbor_stats().rebuilds >= 2
+    finally:
+        e.close()
+
+
+def test_lists_capacity_overflow():
+    """Lists of 4 entries overflow: those grains fall back to the cell
+    search and the lists are rebuilt every step."""
+    s = dense_case(n=729, seed=73, model=3, periodic=1, E=7e9, nu=0.3)
+    e = sphere_amd.Engine(0, nl_cap=4)
+    try:
+        run_pair(e, s, 2, 5, label="list overflow")
+        st = e.neighbor_stats()
+        assert st.overflow == 1 and st.rebuilds == 10
+    finally:
+        e.close()
+
+
+def crowded_case(n_small, r_big=0.03, r_small=0.005, gap=-1e-6, v_out=0.0, arrivals=()):
+    """One big grain at the centre of a box with n_small small grains on a
+    Fibonacci sphere around it at surface gap `gap` (negative: overlap),
+    moving radially outwards at v_out, plus small grains 1e-8 m away from
+    the big one along the unit vectors `arrivals`, approaching at 0.05 m/s."""
+    arrivals = numpy.asarray(arrivals, dtype=numpy.float64).reshape(-1, 3)
+    n = n_small + 1 + arrivals.shape[0]
+    s = sphere_amd.sim(np=n, sid="crowded", nw=0)
+    L = 0.2
+    c = numpy.array([L / 2, L / 2, L / 2])
+    k = numpy.arange(n_small) + 0.5
+    phi = numpy.arccos(1 - 2 * k / n_small)
+    theta = numpy.pi * (1 + 5 ** 0.5) * k
+    u = numpy.stack([numpy.cos(theta) * numpy.sin(phi), numpy.sin(theta) * numpy.sin(phi),
+                     numpy.cos(phi)], axis=1)
+    s.x[0] = c
+    s.radius[0] = r_big
+    s.x[1:n_small + 1] = c + u * (r_big + r_small + gap)
+    s.radius[1:] = r_small
+    s.vel[1:n_small + 1] = u * v_out
+    if arrivals.shape[0]:
+        s.x[n_small + 1:] = c + arrivals * (r_big + r_small + 1e-8)
+        s.vel[n_small + 1:] = -arrivals * 0.05
+    s.defaultParams(mu_s=0.5, mu_d=0.5, k_n=1.16e7, k_t=1.16e7)
+    s.contactmodel[0] = 2
+    s.defineWorldBoundaries(L=[L, L, L], dx=2.2 * r_big)
+    s.g[:] = 0.0
+    m = 2600.0 * 4.0 / 3.0 * numpy.pi * r_small ** 3
+    s.initTemporal(total=1.0, file_dt=1.0, dt=0.05 * numpy.sqrt(m / 1.16e7))
+    return s
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.1])
+def test_slot_overflow_rows(skin):
+    """40 grains touch one: more contacts than 32 slots.  Both sides report
+    the overflow and leave identical slot rows (the reference's rule with
+    cpos = nc and the early exit from a search cell)."""
+    s = crowded_case(40)
+    e = sphere_amd.Engine(0, skin=skin)
+    try:
+        e.load(s.to_state())
+        ora = Oracle(s.to_state(), nc=32)
+        with pytest.raises(sphere_amd.SphereError) as ei:
+            e.step(1)
+        assert ei.value.code == -7
+        with pytest.raises(OracleError) as eo:
+            ora.step(1)
+        assert eo.value.code == -7
+        cg = e.read_contacts()[0]
+        co = ora.read_contacts()[0]
+        assert numpy.array_equal(cg, co)
+    finally:
+        e.close()
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.1])
+def test_full_row_replacement(skin):
+    """32 contacts fill the row of the big grain and separate in the next
+    step while 3 new grains arrive: with no empty slot the new contacts are
+    written to slot nc (the reference's rule), before the stale ones go."""
+    s = crowded_case(32, gap=-2e-8, v_out=0.05,
+                     arrivals=[[0.6, 0.8, 0.0], [-0.8, 0.0, 0.6], [0.0, -0.6, -0.8]])
+    e = sphere_amd.Engine(0, skin=skin)
+    try:
+        run_pair(e, s, 3, 1, label="full row")
+    finally:
+        e.close()
 
 
 def test_dense_hertz_64_slots():
