@@ -85,6 +85,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--save-packing", default="",
+                    help="write the prepared shear box to this folder (<sid>.bin)")
+    ap.add_argument("--load-packing", default="",
+                    help="start from a shear box written by --save-packing")
+    ap.add_argument("--prep-only", action="store_true", help="stop after preparing")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="finish with three device-to-device copies of MIB MiB, the "
+                         "known-size kernel that scales HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the DEM step and the Darcy solve run on one "
@@ -95,9 +103,22 @@ def main():
     import sphere_amd
     # the largest lognormal grains touch more than 32 others: 64 contact slots
     engine = sphere_amd.Engine(0, nc=64)
-    log("preparing the shear box (%d grains)" % args.n)
-    s = prepare_shear_box(engine, args.n, args.seed, args.stage_steps, args.z_target,
-                          args.relax_rounds, args.relax_steps)
+    if args.load_packing:
+        s = sphere_amd.sim()
+        s.readbin(args.load_packing, verbose=False)
+        if int(s.np[0]) != args.n:
+            raise SystemExit("%s holds %d grains, not --n %d" % (args.load_packing, s.np[0], args.n))
+        log("loaded the shear box from %s" % args.load_packing)
+    else:
+        log("preparing the shear box (%d grains)" % args.n)
+        s = prepare_shear_box(engine, args.n, args.seed, args.stage_steps, args.z_target,
+                              args.relax_rounds, args.relax_steps)
+    if args.save_packing:
+        os.makedirs(args.save_packing, exist_ok=True)
+        s.writebin(folder=args.save_packing, verbose=False)
+        log("wrote %s/%s.bin" % (args.save_packing, s.sid))
+    if args.prep_only:
+        return
     state = s.to_state()
     engine.load(state)
     log("warm-up %d steps" % args.warmup)
@@ -162,7 +183,31 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
+
+def copy_calibration(mib):
+    """hipMemcpy device-to-device of a known size (the runtime's plain copy
+    kernel), three times: its FETCH_SIZE / WRITE_SIZE counters give the
+    scale of the HBM traffic counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib * (1 << 20)
+    a = ctypes.c_void_p()
+    b = ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or \
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy calibration: 3 device-to-device copies of %d B (read %d B + write %d B each)" %
+        (n, n, n))
 
 if __name__ == "__main__":
     main()

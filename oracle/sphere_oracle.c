@@ -171,6 +171,10 @@ static int check_values(const sph_state *s, char *err, int errlen)
         if (s->w_nx[3] < zmax) { snprintf(err, errlen, "particles above the upper, dynamic wall"); return SPH_ERR_VALUE; }
     }
     if (s->k_n <= 0.0) { snprintf(err, errlen, "k_n = %g N/m", s->k_n); return SPH_ERR_VALUE; }
+    if (s->contactmodel < 1 || s->contactmodel > 3) { /* src/sphere.cpp:384-393 */
+        snprintf(err, errlen, "contact model value (%u) not understood", s->contactmodel);
+        return SPH_ERR_VALUE;
+    }
     if (s->E < 0.0) { snprintf(err, errlen, "E = %g", s->E); return SPH_ERR_VALUE; }
     if (s->rho <= 0.0) { snprintf(err, errlen, "rho = %g kg/m3", s->rho); return SPH_ERR_VALUE; }
     if (s->fluid) {

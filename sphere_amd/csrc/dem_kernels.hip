@@ -614,8 +614,31 @@ struct Acc {
     double es_dot, ev_dot, p;
 };
 
+// What one contact adds to its particle's sums.  The reference adds these
+// terms to the running sums inside the contact law; the sums see the same
+// additions in the same order when they are applied afterwards by
+// add_contrib (a term a law does not add is +0.0 here, and adding +0.0
+// changes no sum: every sum starts at +0.0 and so is never -0.0).
+struct Contrib {
+    d3 F, T;
+    double p;
+    double ev_n;  // normal damping
+    double ev_t;  // tangential damping (static friction)
+    double es;    // frictional slip
+};
+
+__device__ __forceinline__ void add_contrib(Acc &A, const Contrib &C)
+{
+    A.ev_dot += C.ev_n;
+    A.ev_dot += C.ev_t;
+    A.es_dot += C.es;
+    A.F = A.F + C.F;
+    A.T = A.T + C.T;
+    A.p += C.p;
+}
+
 // contactLinear, src/contactmodels.cuh:241-408
-__device__ __forceinline__ void law_linear(const StepArgs &a, Acc &A, d3 vel_a, d3 angvel_a,
+__device__ __forceinline__ void law_linear(const StepArgs &a, Contrib &C, d3 vel_a, d3 angvel_a,
                                            double4 vb, double4 wb, double r_a, double r_b,
                                            d3 x, double x_length, double delta, double *dts)
 {
@@ -640,7 +663,7 @@ __device__ __forceinline__ void law_linear(const StepArgs &a, Acc &A, d3 vel_a, 
     }
     d3 f_n = fmax(0.0, -k_n * delta + P.gamma_n * vel_n) * n;
     double f_n_length = len3(f_n);
-    A.ev_dot += 0.5 * P.gamma_n * vel_n * vel_n;
+    C.ev_n = 0.5 * P.gamma_n * vel_n * vel_n;
     d3 f_c = (-P.kappa * sqrt(r_a * r_b)) * n;
     d3 f_t = mk3(0.0, 0.0, 0.0);
     if (delta_t0_length > 0.0 || vel_t_length > 0.0) {
@@ -650,17 +673,17 @@ __device__ __forceinline__ void law_linear(const StepArgs &a, Acc &A, d3 vel_a, 
         f_t = f_t_elast + f_t_visc;
         double f_t_length = len3(f_t);
         double f_t_limit = P.mu_s * len3(f_n - f_c);
-        A.ev_dot += 0.5 * P.gamma_t * vel_t_length * vel_t_length;
+        C.ev_t = 0.5 * P.gamma_t * vel_t_length * vel_t_length;
         if (f_t_length > f_t_limit) {
             d3 t = f_t / len3(f_t);
             f_t = f_t_limit * t;
             delta_t = (-1.0 / k_t) * ((P.mu_d * len3(f_n - f_c)) * t + P.gamma_t * vel_t);
-            A.es_dot += 0.5 * len3(dt * (len3(f_t) * vel_t)) / dt;
+            C.es = 0.5 * len3(dt * (len3(f_t) * vel_t)) / dt;
         }
     }
-    A.F = A.F + ((f_n + f_t) + f_c);
-    A.T = A.T + cross3((-(r_a + delta * 0.5)) * n, f_t);
-    A.p += f_n_length / (4.0 * PI_REF * r_a * r_a);
+    C.F = (f_n + f_t) + f_c;
+    C.T = cross3((-(r_a + delta * 0.5)) * n, f_t);
+    C.p = f_n_length / (4.0 * PI_REF * r_a * r_a);
     dts[0] = delta_t.x;
     dts[1] = delta_t.y;
     dts[2] = delta_t.z;
@@ -671,7 +694,7 @@ __device__ __forceinline__ void law_linear(const StepArgs &a, Acc &A, d3 vel_a, 
 // d^(3/2) = d*sqrt(d) and d^(1/4) = sqrt(sqrt(d)).  With E > 0.001 (Poisson's
 // ratio nu in the k_r field): E* = E/(2(1-nu^2)), R* = r_a r_b/(r_a+r_b),
 // G* = E/(2(1+nu)) / (2(2-nu)), k_n = 4/3 E* sqrt(R*), k_t = 8 G* sqrt(R*).
-__device__ __forceinline__ void law_hertz(const StepArgs &a, Acc &A, d3 vel_a, d3 angvel_a,
+__device__ __forceinline__ void law_hertz(const StepArgs &a, Contrib &C, d3 vel_a, d3 angvel_a,
                                           double4 vb, double4 wb, double r_a, double r_b,
                                           d3 x_ab, double x_ab_length, double delta_ab,
                                           double *dts)
@@ -707,7 +730,7 @@ __device__ __forceinline__ void law_hertz(const StepArgs &a, Acc &A, d3 vel_a, d
         k_t = 8.0 * G_star * sR;
     }
     d3 f_n = fmax(0.0, k_n * d * sd - P.gamma_n * qd * vel_n_ab) * n_ab;
-    A.ev_dot += 0.5 * P.gamma_n * qd * vel_n_ab * vel_n_ab;
+    C.ev_n = 0.5 * P.gamma_n * qd * vel_n_ab * vel_n_ab;
     double f_n_length = len3(f_n);
     d3 f_c = (-P.kappa * sqrt(r_a * r_b)) * n_ab;
     d3 f_t = mk3(0.0, 0.0, 0.0);
@@ -722,11 +745,11 @@ __device__ __forceinline__ void law_hertz(const StepArgs &a, Acc &A, d3 vel_a, d
             f_t_limit = P.mu_s * len3(f_n - f_c);
         f_t = f_t_elast + f_t_visc;
         double f_t_length = len3(f_t);
-        A.ev_dot += 0.5 * P.gamma_t * qd * vel_t_ab_length * vel_t_ab_length;
+        C.ev_t = 0.5 * P.gamma_t * qd * vel_t_ab_length * vel_t_ab_length;
         if (f_t_length > f_t_limit) {
             f_t = (f_t_limit / f_t_length) * f_t;
             delta_t = (f_t + (P.gamma_t * qd) * vel_t_ab) / (-k_t * sd);
-            A.es_dot += 0.5 * len3(dt * (len3(f_t) * vel_t_ab)) / dt;
+            C.es = 0.5 * len3(dt * (len3(f_t) * vel_t_ab)) / dt;
         } else {
             delta_t = delta_t0 + dt * vel_t_ab;
         }
@@ -735,9 +758,9 @@ __device__ __forceinline__ void law_hertz(const StepArgs &a, Acc &A, d3 vel_a, d
         double m = fmin(P.gamma_r * r_a * angvel_ab_length, P.mu_r * r_a * f_n_length);
         T_res = (-1.0 * m) * (angvel_ab / angvel_ab_length);
     }
-    A.F = A.F + ((f_n + f_t) + f_c);
-    A.T = A.T + ((-(r_a + delta_ab / 2.0)) * cross3(n_ab, f_t) + T_res);
-    A.p += f_n_length / (4.0 * PI_REF * r_a * r_a);
+    C.F = (f_n + f_t) + f_c;
+    C.T = (-(r_a + delta_ab / 2.0)) * cross3(n_ab, f_t) + T_res;
+    C.p = f_n_length / (4.0 * PI_REF * r_a * r_a);
     dts[0] = delta_t.x;
     dts[1] = delta_t.y;
     dts[2] = delta_t.z;
@@ -838,48 +861,235 @@ __device__ __forceinline__ double law_wall(const StepArgs &a, Acc &A, double4 va
 }
 
 // ------------------------------------------------------------------
-// A5 + A6: contact kernel
+// A5 + A6: contact search (topology) and contact forces (interact)
 // ------------------------------------------------------------------
-// One thread per particle in sorted order.  Topology (slot bookkeeping of
-// new overlaps in the 27 neighbour cells) and interact (forces over the
-// slots in slot order, removal of separated contacts, walls) run back to
-// back in the same thread: a particle's slot row is private to it, so the
-// fusion is exact.  cmask holds one bit per occupied slot; empty slots keep
-// delta_t == 0, which the reference re-establishes every step.
+// Contact models 2 and 3 run as two kernels, like the reference's topology
+// (src/contactsearch.cuh:194-371) and interact (:385-726):
+//   k_topology  one thread per particle: overlaps found this step take
+//               slots of the particle's row by the reference's rule.
+//   k_forces    one wave per 64 particles: the occupied slots of the 64 rows
+//               are dealt out to the 64 lanes, one contact per lane per
+//               round; each lane evaluates its contact law, and the lane
+//               owning the row adds the contributions in slot order.  The
+//               wave stays busy however unevenly its 64 rows are filled.
+// Model 1 (no contact history) runs as one fused thread per particle.
 //
-// The search skips the sqrt of a candidate whose squared distance s is at
-// least (r_a+r_b)^2 (1+1e-9): then sqrt(s) > r_a+r_b and the reference's
-// test sqrt(s) - (r_a+r_b) < 0 is false, so the skip never changes it.
+// cmask holds one bit per occupied slot; empty slots keep delta_t == 0,
+// which the reference re-establishes every step.  The search skips the sqrt
+// of a candidate whose squared distance s is at least (r_a+r_b)^2 (1+1e-9):
+// then sqrt(s) > r_a+r_b and the reference's test sqrt(s) - (r_a+r_b) < 0 is
+// false, so the skip never changes it.
 
-// NL = false: one thread per sorted position, 27-cell search every step
-// (the reference's pipeline).  NL = true: one thread per position of the last
+// contactsearch.cuh:352-354; false outside the grid
+__device__ __forceinline__ bool grid_pos(const Grid &g, double4 xa, int &gx, int &gy, int &gz)
+{
+    gx = (int)floor((xa.x - g.origo[0]) / (g.L[0] / g.num[0]));
+    gy = (int)floor((xa.y - g.origo[1]) / (g.L[1] / g.num[1]));
+    gz = (int)floor((xa.z - g.origo[2]) / (g.L[2] / g.num[2]));
+    return gx >= 0 && gx < (int)g.num[0] && gy >= 0 && gy < (int)g.num[1] && gz >= 0 &&
+           gz < (int)g.num[2];
+}
+
+// Topology over the 27 cells (z, y, x order, contactsearch.cuh:358-369)
+// with the slot rule of contactsearch.cuh:267-286.  Returns true on slot
+// overflow.
+__device__ bool topology_cells(const Grid &g, uint32_t idx_a, double4 xa, int gx, int gy,
+                               int gz, const uint32_t *__restrict__ vals,
+                               const uint32_t *__restrict__ cell_start,
+                               const uint32_t *__restrict__ cell_end,
+                               const double4 *__restrict__ x_s, uint32_t *row, uint8_t *crow,
+                               uint64_t &mask, int NCS, uint64_t full)
+{
+    int nc = 0;
+    bool ovf = false;
+    for (int zd = -1; zd < 2; ++zd)
+        for (int yd = -1; yd < 2; ++yd)
+            for (int xd = -1; xd < 2; ++xd) {
+                int tx = gx + xd, ty = gy + yd, tz = gz + zd;
+                d3 dm = mk3(0.0, 0.0, 0.0);
+                uint32_t code = 0;
+                if (dist_mod(g, tx, ty, tz, dm, code) == -1) continue;
+                const uint32_t cid = (uint32_t)tx + (uint32_t)ty * g.num[0] +
+                                     (g.num[0] * g.num[1]) * (uint32_t)tz;
+                const uint32_t start = cell_start[cid];
+                if (start == EMPTY_CELL) continue;
+                const uint32_t end = cell_end[cid];
+                for (uint32_t idx_b = start; idx_b < end; ++idx_b) {
+                    if (idx_b == idx_a) continue;
+                    const double4 xb = x_s[idx_b];
+                    const d3 x_ab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z) + dm;
+                    const double s2 = dot3(x_ab, x_ab);
+                    const double rsum = xa.w + xb.w;
+                    if (s2 >= rsum * rsum * (1.0 + 1e-9)) continue;
+                    const double delta = sqrt(s2) - rsum;
+                    if (!(delta < 0.0)) continue;
+                    const uint32_t b_orig = vals[idx_b];
+                    int cpos = find_slot(row, mask, b_orig);
+                    if (cpos < 0) {
+                        const uint64_t emp = ~mask & full;
+                        cpos = emp ? 63 - __clzll((long long)emp) : nc;
+                    }
+                    if (cpos < NCS) {
+                        row[cpos] = b_orig;
+                        if (crow[cpos] != (uint8_t)code) crow[cpos] = (uint8_t)code;
+                        mask |= 1ull << cpos;
+                    } else {
+                        ovf = true;
+                    }
+                    ++nc;
+                    if (nc > NCS) break;
+                }
+            }
+    return ovf;
+}
+
+// NL = false: one thread per sorted position, 27-cell search (the
+// reference's pipeline).  NL = true: one thread per position of the last
 // list rebuild; the search tests the particle's list entries, or the 27
-// cells when its list overflowed (then the sort runs every step, so the
+// cells when its list overflowed (the sort then runs every step, so the
 // sorted arrays are current).
 template <bool NL>
-__global__ void __launch_bounds__(256) k_contact(
+__global__ void __launch_bounds__(256) k_topology(
     StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
     const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x,
-    const double4 *__restrict__ x_s, const double4 *__restrict__ vel_s,
-    const double4 *__restrict__ angvel_s, const double4 *__restrict__ vel,
-    const double4 *__restrict__ angvel, double4 *__restrict__ force,
-    double4 *__restrict__ torque, double *__restrict__ es_dot_out,
-    double *__restrict__ ev_dot_out, double *__restrict__ es, double *__restrict__ ev,
-    double *__restrict__ p_out, const double4 *__restrict__ w_nx,
-    const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
-    uint32_t *__restrict__ contacts, uint64_t *__restrict__ cmask,
-    uint8_t *__restrict__ dmcode, double *__restrict__ delta_t, uint32_t *__restrict__ err,
+    const double4 *__restrict__ x_s, uint32_t *__restrict__ contacts,
+    uint64_t *__restrict__ cmask, uint8_t *__restrict__ dmcode, uint32_t *__restrict__ err,
     const uint32_t *__restrict__ nl, const uint32_t *__restrict__ nl_cnt,
     const uint32_t *__restrict__ nl_perm, const uint32_t *__restrict__ cellc)
 {
     const uint32_t idx_a = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx_a >= a.np) return;
     const Grid &g = a.grid;
-    const uint32_t np = a.np;
     const uint32_t a_orig = NL ? nl_perm[idx_a] : vals[idx_a];
     const double4 xa = NL ? x[a_orig] : x_s[idx_a];
+    int gx, gy, gz;
+    if (!grid_pos(g, xa, gx, gy, gz)) {
+        // the step is invalid (k_hash reports it too); no cell index may
+        // leave the grid
+        atomicOr(err, ERRF_GRID);
+        return;
+    }
+    const int NCS = (int)a.nc;
+    const uint64_t full = NCS >= 64 ? ~0ull : ((1ull << NCS) - 1ull);
+    uint32_t *row = contacts + (size_t)a_orig * NCS;
+    uint8_t *crow = dmcode + (size_t)a_orig * NCS;
+    const uint64_t mask0 = cmask[a_orig];
+    uint64_t mask = mask0;
+    bool ovf;
+    const uint32_t lcnt = NL ? nl_cnt[idx_a] : NL_OVF;
+    if (NL && lcnt != NL_OVF)
+        ovf = topology_list(g, a.np, idx_a, lcnt, nl, cellc, x, xa, gx, gy, gz, row, crow, mask,
+                            NCS, full);
+    else
+        ovf = topology_cells(g, idx_a, xa, gx, gy, gz, vals, cell_start, cell_end, x_s, row,
+                             crow, mask, NCS, full);
+    if (ovf) atomicOr(err, ERRF_OVERFLOW);
+    if (mask != mask0) cmask[a_orig] = mask;
+}
+
+// Wall contacts of one particle, contactsearch.cuh:420-701: the top wall
+// (wall 0, or the plane z = L2 without walls), the fixed bottom z = origo2,
+// and for non-periodic axes walls 1-4.  Returns the normal forces on walls
+// 0-4 in wf0..wf4.
+__device__ __forceinline__ void walls_contact(const StepArgs &a, Acc &A, double4 xa, double4 va4,
+                                              double4 wa4, const double4 *__restrict__ w_nx,
+                                              const double4 *__restrict__ w_mvfd, double &wf0,
+                                              double &wf1, double &wf2, double &wf3,
+                                              double &wf4)
+{
+    const Grid &g = a.grid;
+    const uint32_t nw = a.walls.nw;
     const double radius_a = xa.w;
-    const int model = (int)a.prm.contactmodel;
+    wf0 = wf1 = wf2 = wf3 = wf4 = 0.0;
+    const double4 wn0 = nw > 0 ? w_nx[0] : make_double4(0.0, 0.0, -1.0, g.L[2]);
+    const double wv0 = nw > 0 ? w_mvfd[0].y : 0.0;
+    double delta_w = wn0.w - (xa.z + radius_a);
+    if (delta_w < 0.0) wf0 = law_wall(a, A, va4, wa4, radius_a, xyz(wn0), delta_w, wv0);
+    delta_w = xa.z - radius_a - g.origo[2];
+    if (delta_w < 0.0) (void)law_wall(a, A, va4, wa4, radius_a, mk3(0.0, 0.0, 1.0), delta_w, 0.0);
+    if (g.periodic == 0) {
+        const double4 wn1 = nw > 1 ? w_nx[1] : make_double4(-1.0, 0.0, 0.0, g.L[0]);
+        const double4 wn2 = nw > 2 ? w_nx[2] : make_double4(1.0, 0.0, 0.0, 0.0);
+        const double wv1 = nw > 1 ? w_mvfd[1].y : 0.0, wv2 = nw > 2 ? w_mvfd[2].y : 0.0;
+        delta_w = wn1.w - (xa.x + radius_a);
+        if (delta_w < 0.0) wf1 = law_wall(a, A, va4, wa4, radius_a, xyz(wn1), delta_w, wv1);
+        delta_w = xa.x - radius_a - wn2.w;
+        if (delta_w < 0.0) wf2 = law_wall(a, A, va4, wa4, radius_a, xyz(wn2), delta_w, wv2);
+    }
+    if (g.periodic == 0 || g.periodic == 2) {
+        const double4 wn3 = nw > 3 ? w_nx[3] : make_double4(0.0, -1.0, 0.0, g.L[1]);
+        const double4 wn4 = nw > 4 ? w_nx[4] : make_double4(0.0, 1.0, 0.0, 0.0);
+        const double wv3 = nw > 3 ? w_mvfd[3].y : 0.0, wv4 = nw > 4 ? w_mvfd[4].y : 0.0;
+        delta_w = wn3.w - (xa.y + radius_a);
+        if (delta_w < 0.0) wf3 = law_wall(a, A, va4, wa4, radius_a, xyz(wn3), delta_w, wv3);
+        delta_w = xa.y - radius_a - wn4.w;
+        if (delta_w < 0.0) wf4 = law_wall(a, A, va4, wa4, radius_a, xyz(wn4), delta_w, wv4);
+    }
+}
+
+// contactsearch.cuh:706-724
+__device__ __forceinline__ void write_sums(const StepArgs &a, const Acc &A, uint32_t a_orig,
+                                           double4 *__restrict__ force,
+                                           double4 *__restrict__ torque,
+                                           double *__restrict__ es_dot_out,
+                                           double *__restrict__ ev_dot_out,
+                                           double *__restrict__ es, double *__restrict__ ev,
+                                           double *__restrict__ p_out,
+                                           double *__restrict__ walls_force_pp, double wf0,
+                                           double wf1, double wf2, double wf3, double wf4)
+{
+    const size_t np = a.np;
+    force[a_orig] = make_double4(A.F.x, A.F.y, A.F.z, 0.0);
+    torque[a_orig] = make_double4(A.T.x, A.T.y, A.T.z, 0.0);
+    es_dot_out[a_orig] = A.es_dot;
+    ev_dot_out[a_orig] = A.ev_dot;
+    es[a_orig] += A.es_dot * a.dt;
+    ev[a_orig] += A.ev_dot * a.dt;
+    p_out[a_orig] = A.p;
+    const uint32_t nw = a.walls.nw;
+    if (nw > 0) walls_force_pp[a_orig] = wf0;
+    if (nw > 1) walls_force_pp[a_orig + np] = wf1;
+    if (nw > 2) walls_force_pp[a_orig + 2 * np] = wf2;
+    if (nw > 3) walls_force_pp[a_orig + 3 * np] = wf3;
+    if (nw > 4) walls_force_pp[a_orig + 4 * np] = wf4;
+}
+
+#define FW 64  // k_forces: one wave per block
+#define NREC 10
+
+// HERTZ: contact model 3, else 2
+template <bool HERTZ>
+__global__ void __launch_bounds__(FW) k_forces(
+    StepArgs a, const uint32_t *__restrict__ order, const double4 *__restrict__ x,
+    const double4 *__restrict__ vel, const double4 *__restrict__ angvel,
+    double4 *__restrict__ force, double4 *__restrict__ torque, double *__restrict__ es_dot_out,
+    double *__restrict__ ev_dot_out, double *__restrict__ es, double *__restrict__ ev,
+    double *__restrict__ p_out, const double4 *__restrict__ w_nx,
+    const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
+    uint32_t *__restrict__ contacts, uint64_t *__restrict__ cmask,
+    const uint8_t *__restrict__ dmcode, double *__restrict__ delta_t)
+{
+    __shared__ uint32_t s_a[FW];
+    __shared__ uint8_t s_own[FW], s_slot[FW], s_rm[FW];
+    __shared__ double s_rec[NREC][FW];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t pos = blockIdx.x * FW + lane;
+    const uint32_t np = a.np;
+    const bool valid = pos < np;
+    const uint32_t a_orig = valid ? order[pos] : 0u;
+    const uint64_t mask = valid ? cmask[a_orig] : 0ull;
+    const uint32_t n = (uint32_t)__popcll(mask);
+    // exclusive scan of the row sizes over the wave
+    uint32_t incl = n;
+    for (uint32_t off = 1; off < FW; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off, FW);
+        if (lane >= off) incl += t;
+    }
+    const uint32_t excl = incl - n;
+    const uint32_t total = __shfl(incl, FW - 1, FW);
+    s_a[lane] = a_orig;
+    const Grid &g = a.grid;
+    const int NCS = (int)a.nc;
 
     Acc A;
     A.F = mk3(0.0, 0.0, 0.0);
@@ -887,118 +1097,124 @@ __global__ void __launch_bounds__(256) k_contact(
     A.es_dot = 0.0;
     A.ev_dot = 0.0;
     A.p = 0.0;
-
-    const double4 va4 = NL ? vel[a_orig] : vel_s[idx_a];
-    const double4 wa4 = NL ? angvel[a_orig] : angvel_s[idx_a];
-    // contactsearch.cuh:352-354
-    const int gx = (int)floor((xa.x - g.origo[0]) / (g.L[0] / g.num[0]));
-    const int gy = (int)floor((xa.y - g.origo[1]) / (g.L[1] / g.num[1]));
-    const int gz = (int)floor((xa.z - g.origo[2]) / (g.L[2] / g.num[2]));
-    // a particle outside the grid makes the step invalid (k_hash flags it);
-    // its neighbour search is skipped so that no cell index leaves the grid
-    const bool inside = gx >= 0 && gx < (int)g.num[0] && gy >= 0 && gy < (int)g.num[1] &&
-                        gz >= 0 && gz < (int)g.num[2];
-    if (!inside) atomicOr(err, ERRF_GRID);
-
-    if (inside && (model == 2 || model == 3)) {
-        const int NCS = (int)a.nc;
-        const uint64_t full = NCS >= 64 ? ~0ull : ((1ull << NCS) - 1ull);
-        uint32_t *row = contacts + (size_t)a_orig * NCS;
-        uint8_t *crow = dmcode + (size_t)a_orig * NCS;
-        uint64_t mask = cmask[a_orig];
-        int nc = 0;
-        bool ovf = false;
-        const uint32_t lcnt = NL ? nl_cnt[idx_a] : NL_OVF;
-        if (NL && lcnt != NL_OVF) {
-            ovf = topology_list(g, np, idx_a, lcnt, nl, cellc, x, xa, gx, gy, gz, row, crow, mask,
-                                NCS, full);
-        } else {
-        // topology, contactsearch.cuh:358-369 (z, y, x order)
-        for (int zd = -1; zd < 2; ++zd)
-            for (int yd = -1; yd < 2; ++yd)
-                for (int xd = -1; xd < 2; ++xd) {
-                    int tx = gx + xd, ty = gy + yd, tz = gz + zd;
-                    d3 dm = mk3(0.0, 0.0, 0.0);
-                    uint32_t code = 0;
-                    if (dist_mod(g, tx, ty, tz, dm, code) == -1) continue;
-                    uint32_t cid = (uint32_t)tx + (uint32_t)ty * g.num[0] +
-                                   (g.num[0] * g.num[1]) * (uint32_t)tz;
-                    uint32_t start = cell_start[cid];
-                    if (start == EMPTY_CELL) continue;
-                    uint32_t end = cell_end[cid];
-                    for (uint32_t idx_b = start; idx_b < end; ++idx_b) {
-                        if (idx_b == idx_a) continue;
-                        double4 xb = x_s[idx_b];
-                        d3 x_ab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
-                        x_ab = x_ab + dm;
-                        double s2 = dot3(x_ab, x_ab);
-                        double rsum = radius_a + xb.w;
-                        if (s2 >= rsum * rsum * (1.0 + 1e-9)) continue;
-                        double delta = sqrt(s2) - rsum;
-                        if (!(delta < 0.0)) continue;
-                        uint32_t b_orig = vals[idx_b];
-                        // slot rule of contactsearch.cuh:267-286
-                        int cpos = -1;
-                        uint64_t occ = mask;
-                        while (occ) {
-                            int i = __ffsll((unsigned long long)occ) - 1;
-                            occ &= occ - 1;
-                            if (row[i] == b_orig) { cpos = i; break; }
-                        }
-                        if (cpos < 0) {
-                            uint64_t emp = ~mask & full;
-                            cpos = emp ? 63 - __clzll((long long)emp) : nc;
-                        }
-                        if (cpos < NCS) {
-                            row[cpos] = b_orig;
-                            crow[cpos] = (uint8_t)code;
-                            mask |= 1ull << cpos;
-                        } else {
-                            ovf = true;
-                        }
-                        ++nc;
-                        if (nc > NCS) break;
-                    }
-                }
+    uint64_t deal = mask, keep = mask;
+    for (uint32_t base = 0; base < total; base += FW) {
+        // this row's contacts with wave index in [base, base + FW)
+        const uint32_t lo = excl > base ? excl : base;
+        const uint32_t hi = excl + n < base + FW ? excl + n : base + FW;
+        for (uint32_t c = lo; c < hi; ++c) {
+            const int sl = __ffsll((unsigned long long)deal) - 1;
+            deal &= deal - 1;
+            s_own[c - base] = (uint8_t)lane;
+            s_slot[c - base] = (uint8_t)sl;
         }
-        if (ovf) atomicOr(err, ERRF_OVERFLOW);
-
-        // interact, contactsearch.cuh:484-559: slots in ascending order
-        const d3 vel_a = xyz(va4), angvel_a = xyz(wa4);
-        uint64_t occ = mask;
-        while (occ) {
-            int i = __ffsll((unsigned long long)occ) - 1;
-            occ &= occ - 1;
-            size_t mempos = (size_t)a_orig * NCS + i;
-            uint32_t b = row[i];
-            d3 dm = decode_dm(g, crow[i]);
-            double4 xb = x[b];
-            double radius_b = xb.w;
-            d3 x_ab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
-            double l = len3(x_ab);
-            double delta_n = l - (radius_a + radius_b);
+        __syncthreads();
+        if (base + lane < total) {
+            // interact body for one slot, contactsearch.cuh:484-559
+            const uint32_t ao = s_a[s_own[lane]];
+            const uint32_t sl = s_slot[lane];
+            const size_t mempos = (size_t)ao * NCS + sl;
+            const double4 xa = x[ao];
+            const uint32_t b = contacts[mempos];
+            const d3 dm = decode_dm(g, dmcode[mempos]);
+            const double4 xb = x[b];
+            const d3 x_ab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
+            const double l = len3(x_ab);
+            const double delta_n = l - (xa.w + xb.w);
             double *dts = delta_t + 3 * mempos;
+            Contrib C;
+            C.F = mk3(0.0, 0.0, 0.0);
+            C.T = mk3(0.0, 0.0, 0.0);
+            C.p = C.ev_n = C.ev_t = C.es = 0.0;
+            uint8_t rm = 0;
             if (delta_n < 0.0) {
+                const double4 va4 = vel[ao], wa4 = angvel[ao];
                 double loc[3] = {dts[0], dts[1], dts[2]};
-                if (model == 2)
-                    law_linear(a, A, vel_a, angvel_a, vel[b], angvel[b], radius_a, radius_b,
-                               x_ab, l, delta_n, loc);
+                if (HERTZ)
+                    law_hertz(a, C, xyz(va4), xyz(wa4), vel[b], angvel[b], xa.w, xb.w, x_ab, l,
+                              delta_n, loc);
                 else
-                    law_hertz(a, A, vel_a, angvel_a, vel[b], angvel[b], radius_a, radius_b,
-                              x_ab, l, delta_n, loc);
+                    law_linear(a, C, xyz(va4), xyz(wa4), vel[b], angvel[b], xa.w, xb.w, x_ab, l,
+                               delta_n, loc);
                 dts[0] = loc[0];
                 dts[1] = loc[1];
                 dts[2] = loc[2];
             } else {
-                row[i] = np;
+                contacts[mempos] = np;
                 dts[0] = 0.0;
                 dts[1] = 0.0;
                 dts[2] = 0.0;
-                mask &= ~(1ull << i);
+                rm = 1;
+            }
+            s_rec[0][lane] = C.F.x;
+            s_rec[1][lane] = C.F.y;
+            s_rec[2][lane] = C.F.z;
+            s_rec[3][lane] = C.T.x;
+            s_rec[4][lane] = C.T.y;
+            s_rec[5][lane] = C.T.z;
+            s_rec[6][lane] = C.p;
+            s_rec[7][lane] = C.ev_n;
+            s_rec[8][lane] = C.ev_t;
+            s_rec[9][lane] = C.es;
+            s_rm[lane] = rm;
+        }
+        __syncthreads();
+        for (uint32_t c = lo; c < hi; ++c) {
+            const uint32_t k = c - base;
+            if (s_rm[k]) {
+                keep &= ~(1ull << s_slot[k]);
+            } else {
+                Contrib C;
+                C.F = mk3(s_rec[0][k], s_rec[1][k], s_rec[2][k]);
+                C.T = mk3(s_rec[3][k], s_rec[4][k], s_rec[5][k]);
+                C.p = s_rec[6][k];
+                C.ev_n = s_rec[7][k];
+                C.ev_t = s_rec[8][k];
+                C.es = s_rec[9][k];
+                add_contrib(A, C);
             }
         }
-        cmask[a_orig] = mask;
-    } else if (inside && model == 1) {
+        __syncthreads();
+    }
+    if (!valid) return;
+    const double4 xa = x[a_orig], va4 = vel[a_orig], wa4 = angvel[a_orig];
+    double wf0, wf1, wf2, wf3, wf4;
+    walls_contact(a, A, xa, va4, wa4, w_nx, w_mvfd, wf0, wf1, wf2, wf3, wf4);
+    write_sums(a, A, a_orig, force, torque, es_dot_out, ev_dot_out, es, ev, p_out,
+               walls_force_pp, wf0, wf1, wf2, wf3, wf4);
+    if (keep != mask) cmask[a_orig] = keep;
+}
+
+// Model 1 (contactLinearViscous + capillary cohesion, no contact history):
+// one thread per sorted position, cells searched every step.
+__global__ void __launch_bounds__(256) k_contact_viscous(
+    StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
+    const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x_s,
+    const double4 *__restrict__ vel_s, const double4 *__restrict__ angvel_s,
+    double4 *__restrict__ force, double4 *__restrict__ torque, double *__restrict__ es_dot_out,
+    double *__restrict__ ev_dot_out, double *__restrict__ es, double *__restrict__ ev,
+    double *__restrict__ p_out, const double4 *__restrict__ w_nx,
+    const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
+    uint32_t *__restrict__ err)
+{
+    const uint32_t idx_a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx_a >= a.np) return;
+    const Grid &g = a.grid;
+    const uint32_t a_orig = vals[idx_a];
+    const double4 xa = x_s[idx_a];
+    const double radius_a = xa.w;
+    const double4 va4 = vel_s[idx_a];
+    const double4 wa4 = angvel_s[idx_a];
+    Acc A;
+    A.F = mk3(0.0, 0.0, 0.0);
+    A.T = mk3(0.0, 0.0, 0.0);
+    A.es_dot = 0.0;
+    A.ev_dot = 0.0;
+    A.p = 0.0;
+    int gx, gy, gz;
+    const bool inside = grid_pos(g, xa, gx, gy, gz);
+    if (!inside) atomicOr(err, ERRF_GRID);
+    if (inside) {
         // findAndProcessContactsInCell, contactsearch.cuh:91-186
         for (int zd = -1; zd < 2; ++zd)
             for (int yd = -1; yd < 2; ++yd)
@@ -1028,62 +1244,42 @@ __global__ void __launch_bounds__(256) k_contact(
                     }
                 }
     }
-
-    // walls, contactsearch.cuh:420-701
-    double4 wn[5] = {make_double4(0.0, 0.0, -1.0, g.L[2]), make_double4(-1.0, 0.0, 0.0, g.L[0]),
-                     make_double4(1.0, 0.0, 0.0, 0.0), make_double4(0.0, -1.0, 0.0, g.L[1]),
-                     make_double4(0.0, 1.0, 0.0, 0.0)};
-    double wv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    const uint32_t nw = a.walls.nw;
-    for (uint32_t w = 0; w < nw && w < 5; ++w) {
-        wn[w] = w_nx[w];
-        wv[w] = w_mvfd[w].y;
-    }
-    double wf[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double delta_w;
-    delta_w = wn[0].w - (xa.z + radius_a);
-    if (delta_w < 0.0) wf[0] = law_wall(a, A, va4, wa4, radius_a, xyz(wn[0]), delta_w, wv[0]);
-    delta_w = xa.z - radius_a - g.origo[2];
-    if (delta_w < 0.0) (void)law_wall(a, A, va4, wa4, radius_a, mk3(0.0, 0.0, 1.0), delta_w, 0.0);
-    if (g.periodic == 0) {
-        delta_w = wn[1].w - (xa.x + radius_a);
-        if (delta_w < 0.0) wf[1] = law_wall(a, A, va4, wa4, radius_a, xyz(wn[1]), delta_w, wv[1]);
-        delta_w = xa.x - radius_a - wn[2].w;
-        if (delta_w < 0.0) wf[2] = law_wall(a, A, va4, wa4, radius_a, xyz(wn[2]), delta_w, wv[2]);
-    }
-    if (g.periodic == 0 || g.periodic == 2) {
-        delta_w = wn[3].w - (xa.y + radius_a);
-        if (delta_w < 0.0) wf[3] = law_wall(a, A, va4, wa4, radius_a, xyz(wn[3]), delta_w, wv[3]);
-        delta_w = xa.y - radius_a - wn[4].w;
-        if (delta_w < 0.0) wf[4] = law_wall(a, A, va4, wa4, radius_a, xyz(wn[4]), delta_w, wv[4]);
-    }
-
-    // contactsearch.cuh:706-724
-    force[a_orig] = make_double4(A.F.x, A.F.y, A.F.z, 0.0);
-    torque[a_orig] = make_double4(A.T.x, A.T.y, A.T.z, 0.0);
-    es_dot_out[a_orig] = A.es_dot;
-    ev_dot_out[a_orig] = A.ev_dot;
-    es[a_orig] += A.es_dot * a.dt;
-    ev[a_orig] += A.ev_dot * a.dt;
-    p_out[a_orig] = A.p;
-    for (uint32_t w = 0; w < nw && w < 5; ++w) walls_force_pp[a_orig + (size_t)np * w] = wf[w];
+    double wf0, wf1, wf2, wf3, wf4;
+    walls_contact(a, A, xa, va4, wa4, w_nx, w_mvfd, wf0, wf1, wf2, wf3, wf4);
+    write_sums(a, A, a_orig, force, torque, es_dot_out, ev_dot_out, es, ev, p_out,
+               walls_force_pp, wf0, wf1, wf2, wf3, wf4);
 }
 
 void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist)
 {
-    uint32_t nb = (a.np + 255) / 256;
+    const uint32_t nb = (a.np + 255) / 256;
+    if (a.prm.contactmodel == 1) {
+        hipLaunchKernelGGL(k_contact_viscous, dim3(nb), dim3(256), 0, s, a, b.vals, b.cell_start,
+                           b.cell_end, b.x_s, b.vel_s, b.angvel_s, b.force, b.torque, b.es_dot,
+                           b.ev_dot, b.es, b.ev, b.p, b.w_nx, b.w_mvfd, b.walls_force_pp,
+                           b.errflags);
+        return;
+    }
     if (nlist)
-        hipLaunchKernelGGL(k_contact<true>, dim3(nb), dim3(256), 0, s, a, b.vals, b.cell_start,
-                           b.cell_end, b.x, b.x_s, b.vel_s, b.angvel_s, b.vel, b.angvel, b.force,
-                           b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p, b.w_nx, b.w_mvfd,
-                           b.walls_force_pp, b.contacts, b.cmask, b.dmcode, b.delta_t,
-                           b.errflags, b.nl, b.nl_cnt, b.nl_perm, b.cellc);
+        hipLaunchKernelGGL(k_topology<true>, dim3(nb), dim3(256), 0, s, a, b.vals, b.cell_start,
+                           b.cell_end, b.x, b.x_s, b.contacts, b.cmask, b.dmcode, b.errflags,
+                           b.nl, b.nl_cnt, b.nl_perm, b.cellc);
     else
-        hipLaunchKernelGGL(k_contact<false>, dim3(nb), dim3(256), 0, s, a, b.vals, b.cell_start,
-                           b.cell_end, b.x, b.x_s, b.vel_s, b.angvel_s, b.vel, b.angvel, b.force,
-                           b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p, b.w_nx, b.w_mvfd,
-                           b.walls_force_pp, b.contacts, b.cmask, b.dmcode, b.delta_t,
+        hipLaunchKernelGGL(k_topology<false>, dim3(nb), dim3(256), 0, s, a, b.vals,
+                           b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask, b.dmcode,
                            b.errflags, nullptr, nullptr, nullptr, nullptr);
+    const uint32_t nw = (a.np + FW - 1) / FW;
+    const uint32_t *order = nlist ? b.nl_perm : b.vals;
+    if (a.prm.contactmodel == 3)
+        hipLaunchKernelGGL(k_forces<true>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
+                           b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
+                           b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask, b.dmcode,
+                           b.delta_t);
+    else
+        hipLaunchKernelGGL(k_forces<false>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
+                           b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
+                           b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask, b.dmcode,
+                           b.delta_t);
 }
 
 // ------------------------------------------------------------------
@@ -1231,12 +1427,47 @@ __global__ void __launch_bounds__(256) k_shear_stress(StepArgs a, const double4 
 }
 
 // Serial sum over block partials (the order of integration.cuh:64-66).
-__global__ void k_serial_sum(const double *__restrict__ part, uint32_t n, double *__restrict__ out)
+// Left-to-right sum part[0] + part[1] + ... (the reference's serial loop,
+// src/integration.cuh:361-364), one dependent add after another.  The
+// block stages chunks of the partials in LDS with coalesced loads so that
+// thread 0 only waits on the adds, not on global-memory latency.  Result
+// valid in thread 0; every thread of the block must call it.
+#define SUM_THREADS 256
+#define SUM_CHUNK (SUM_THREADS * 8)
+__device__ double serial_sum_staged(const double *__restrict__ part, uint32_t n, double *lds)
 {
-    if (threadIdx.x != 0) return;
     double s = 0.0;
-    for (uint32_t i = 0; i < n; ++i) s += part[i];
-    *out = s;
+    for (uint32_t base = 0; base < n; base += SUM_CHUNK) {
+        const uint32_t m = n - base < SUM_CHUNK ? n - base : SUM_CHUNK;
+        for (uint32_t j = threadIdx.x; j < m; j += SUM_THREADS) lds[j] = part[base + j];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t j = 0;
+            for (; j + 8 <= m; j += 8) {
+                const double v0 = lds[j], v1 = lds[j + 1], v2 = lds[j + 2], v3 = lds[j + 3];
+                const double v4 = lds[j + 4], v5 = lds[j + 5], v6 = lds[j + 6], v7 = lds[j + 7];
+                s += v0;
+                s += v1;
+                s += v2;
+                s += v3;
+                s += v4;
+                s += v5;
+                s += v6;
+                s += v7;
+            }
+            for (; j < m; ++j) s += lds[j];
+        }
+        __syncthreads();
+    }
+    return s;
+}
+
+__global__ void __launch_bounds__(SUM_THREADS) k_serial_sum(const double *__restrict__ part,
+                                                            uint32_t n, double *__restrict__ out)
+{
+    __shared__ double lds[SUM_CHUNK];
+    const double s = serial_sum_staged(part, n, lds);
+    if (threadIdx.x == 0) *out = s;
 }
 
 void launch_shear_stress(hipStream_t s, const StepArgs &a, const DemBuffers &b)
@@ -1244,7 +1475,8 @@ void launch_shear_stress(hipStream_t s, const StepArgs &a, const DemBuffers &b)
     uint32_t nb = (a.np + 255) / 256;
     hipLaunchKernelGGL(k_shear_stress, dim3(nb), dim3(256), 0, s, a, b.x, b.vel, b.force,
                        b.tau_partial);
-    hipLaunchKernelGGL(k_serial_sum, dim3(1), dim3(64), 0, s, b.tau_partial, nb, b.tau_eff);
+    hipLaunchKernelGGL(k_serial_sum, dim3(1), dim3(SUM_THREADS), 0, s, b.tau_partial, nb,
+                       b.tau_eff);
 }
 
 // integrate (TY3), src/integration.cuh:14-305, in original order (the
@@ -1367,17 +1599,15 @@ void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b, lon
 // integrateWalls, src/integration.cuh:344-450.  One wave: lane 0 forms the
 // serial sum over the block partials (the reference's order) once and every
 // wall thread uses it (the reference sums wall 0's partials for each wall).
-__global__ void __launch_bounds__(64) k_integrate_walls(
+__global__ void __launch_bounds__(SUM_THREADS) k_integrate_walls(
     StepArgs a, long iter, double t_current, double4 *__restrict__ w_nx,
     double4 *__restrict__ w_mvfd, double *__restrict__ w_acc,
     const double *__restrict__ partial, uint32_t nparts)
 {
+    __shared__ double lds[SUM_CHUNK];
     __shared__ double total;
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (uint32_t k = 0; k < nparts; ++k) s += partial[k];
-        total = s;
-    }
+    const double s = serial_sum_staged(partial, nparts, lds);
+    if (threadIdx.x == 0) total = s;
     __syncthreads();
     uint32_t idx = threadIdx.x;
     if (idx >= a.walls.nw) return;
@@ -1403,7 +1633,7 @@ void launch_integrate_walls(hipStream_t s, const StepArgs &a, const DemBuffers &
                             double t_current)
 {
     uint32_t nb = (a.np + 255) / 256;
-    hipLaunchKernelGGL(k_integrate_walls, dim3(1), dim3(64), 0, s, a, iter, t_current, b.w_nx,
+    hipLaunchKernelGGL(k_integrate_walls, dim3(1), dim3(SUM_THREADS), 0, s, a, iter, t_current, b.w_nx,
                        b.w_mvfd, b.w_acc, b.wall_partial, nb);
 }
 

@@ -247,6 +247,10 @@ static int check_values(sph_ctx *ctx, const HostState &hs)
             return fail(ctx, SPH_ERR_VALUE, "One or more particles have centres above the upper, dynamic wall");
     }
     if (s.k_n <= 0.0) { snprintf(m, sizeof(m), "k_n = %g N/m", s.k_n); return fail(ctx, SPH_ERR_VALUE, m); }
+    if (s.contactmodel < 1 || s.contactmodel > 3) {  // reportValues, src/sphere.cpp:384-393
+        snprintf(m, sizeof(m), "contact model value (%u) not understood", s.contactmodel);
+        return fail(ctx, SPH_ERR_VALUE, m);
+    }
     if (s.E < 0.0) { snprintf(m, sizeof(m), "E = %g", s.E); return fail(ctx, SPH_ERR_VALUE, m); }
     if (s.rho <= 0.0) { snprintf(m, sizeof(m), "rho = %g kg/m3", s.rho); return fail(ctx, SPH_ERR_VALUE, m); }
     if (s.fluid) {
