@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skin", type=float, default=None,
+                    help="neighbour-list skin as a fraction of the largest radius (0: cell search)")
     ap.add_argument("--save-packing", default="",
                     help="write the prepared shear box to this folder (<sid>.bin)")
     ap.add_argument("--load-packing", default="",
@@ -103,6 +105,7 @@ def main():
     import sphere_amd
     # the largest lognormal grains touch more than 32 others: 64 contact slots
     engine = sphere_amd.Engine(0, nc=64)
+    run_engine = engine if args.skin is None else sphere_amd.Engine(0, nc=64, skin=args.skin)
     if args.load_packing:
         s = sphere_amd.sim()
         s.readbin(args.load_packing, verbose=False)
@@ -120,29 +123,40 @@ def main():
     if args.prep_only:
         return
     state = s.to_state()
-    engine.load(state)
+    run_engine.load(state)
     log("warm-up %d steps" % args.warmup)
-    engine.step(args.warmup)
-    warm_buf = engine.read()
-    n0 = engine.count_contacts()
-    engine.set_profiling(1)
+    run_engine.step(args.warmup)
+    warm_buf = run_engine.read()
+    n0 = run_engine.count_contacts()
+    run_engine.set_profiling(1)
     log("timed %d steps" % args.steps)
     t0 = time.perf_counter()
-    engine.step(args.steps)
+    run_engine.step(args.steps)
     elapsed = time.perf_counter() - t0
-    n1 = engine.count_contacts()
+    n1 = run_engine.count_contacts()
     ktimes = {}
     for k, name in enumerate(sphere_amd._abi.K_NAMES):
-        ms, cnt = engine.kernel_time(k)
+        ms, cnt = run_engine.kernel_time(k)
         if cnt:
             ktimes[name] = {"ms_total": round(ms, 3), "launches": int(cnt),
                             "us_avg": round(1000.0 * ms / cnt, 2)}
-    engine.set_profiling(0)
+    run_engine.set_profiling(0)
+    nls = run_engine.neighbor_stats()
+    lists = None
+    if nls.enabled:
+        lists = {"skin_m": nls.skin, "rebuilds_total": int(nls.rebuilds),
+                 "entries_per_grain": round(nls.entries / float(args.n), 2),
+                 "longest": int(nls.longest)}
+        log("neighbour lists: skin %.3g m, %d rebuilds, %.2f entries per grain, longest %d" %
+            (nls.skin, nls.rebuilds, nls.entries / float(args.n), nls.longest))
     n = args.n
     value = n * args.steps / elapsed
     zbar = 0.5 * (n0 + n1) / n
-    c_ms, c_cnt = ktimes["contact"]["ms_total"], ktimes["contact"]["launches"]
-    c_avg_s = c_ms / c_cnt / 1000.0
+    # the contact stage is two launches per step: topology (search and slot
+    # bookkeeping) and contact (laws, walls, sums); its duration is the sum of
+    # their average launch durations
+    c_avg_s = sum(ktimes[k]["ms_total"] / ktimes[k]["launches"] / 1000.0
+                  for k in ("topology", "contact") if k in ktimes)
     c_bytes = contact_bytes_per_particle(zbar) * n
     achieved = c_bytes / c_avg_s / 1e9
     log("elapsed %.3f s, %.1f us/step, Zbar %.2f, contact kernel %.1f us" %
@@ -174,11 +188,13 @@ def main():
                                "sigma_ln 0.2), Hertz-Mindlin E=7e9 nu=0.3 mu=0.5, 10 kPa top wall, "
                                "top layer at 0.01 m/s, 5%% bonded pairs" % n,
                    "grains": n, "bonds": int(s.nb0[0]), "contacts_per_grain": round(zbar, 3),
-                   "grid": [int(v) for v in s.num], "dt": float(s.time_dt[0])},
-        "roofline": {"bound": "hbm", "kernel": "contact", "achieved": achieved,
+                   "grid": [int(v) for v in s.num], "dt": float(s.time_dt[0]),
+                   "neighbour_lists": lists},
+        "roofline": {"bound": "hbm", "kernel": "contact stage (k_topology_lists + k_forces)",
+                     "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None,
-                     "bytes_per_launch": c_bytes, "us_per_launch": 1e6 * c_avg_s},
+                     "bytes_per_step": c_bytes, "us_per_step": 1e6 * c_avg_s},
         "kernels": ktimes,
         "cpu_baseline": cpu,
     }

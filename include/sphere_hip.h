@@ -137,14 +137,15 @@ typedef struct sph_darcy_stats {
 #define SPH_K_HASH 0
 #define SPH_K_SORT 1
 #define SPH_K_REORDER 2
-#define SPH_K_CONTACT 3
+#define SPH_K_CONTACT 3       /* contact forces, walls and sums (interact) */
 #define SPH_K_BONDS 4
 #define SPH_K_INTEGRATE 5
 #define SPH_K_WALLS 6
 #define SPH_K_DARCY_SETUP 7
 #define SPH_K_DARCY_SWEEP 8
 #define SPH_K_NLIST 9         /* neighbour-list rebuild (no reference counterpart) */
-#define SPH_K_COUNT 10
+#define SPH_K_TOPOLOGY 10     /* contact search and slot bookkeeping (topology) */
+#define SPH_K_COUNT 11
 
 /* Neighbour-list statistics (sph_get_neighbor_stats). */
 typedef struct {
@@ -154,6 +155,9 @@ typedef struct {
     uint32_t overflow;  /* 1 once a list overflowed (rebuilt every step since) */
     double skin;        /* listed surface gap [m] */
     uint64_t rebuilds;  /* rebuilds since the last load */
+    uint64_t entries;   /* list entries over all particles at the last rebuild */
+    uint32_t longest;   /* longest list (cap + 1 if one overflowed) */
+    uint32_t pad;
 } sph_neighbor_stats;
 
 /* Context.  Replaces DEM::DEM(..., initCuda=1, device) (src/sphere.cpp:16-95)

@@ -29,7 +29,7 @@ ERRORS = {
 }
 
 K_NAMES = ["hash", "sort", "reorder", "contact", "bonds", "integrate", "walls",
-           "darcy_setup", "darcy_sweep", "nlist"]
+           "darcy_setup", "darcy_sweep", "nlist", "topology"]
 
 _d = ctypes.c_double
 _pd = ctypes.POINTER(ctypes.c_double)
@@ -121,7 +121,9 @@ _CT = {numpy.float64: ctypes.c_double, numpy.uint32: ctypes.c_uint32,
 class SphNeighborStats(ctypes.Structure):
     _fields_ = [("enabled", ctypes.c_uint32), ("range", ctypes.c_uint32),
                 ("cap", ctypes.c_uint32), ("overflow", ctypes.c_uint32),
-                ("skin", ctypes.c_double), ("rebuilds", ctypes.c_uint64)]
+                ("skin", ctypes.c_double), ("rebuilds", ctypes.c_uint64),
+                ("entries", ctypes.c_uint64), ("longest", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
 
 
 class StateBuffers:

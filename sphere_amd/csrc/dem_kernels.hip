@@ -433,6 +433,33 @@ void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const De
                        b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_perm, b.x_ref, b.nl_flags);
 }
 
+// Sum and maximum of the list lengths (overflowed lists count as cap + 1).
+__global__ void __launch_bounds__(256) k_nl_stats(const uint32_t *__restrict__ nl_cnt, uint32_t np,
+                                                  uint32_t cap, unsigned long long *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c = 0;
+    if (i < np) c = nl_cnt[i] == NL_OVF ? cap + 1u : nl_cnt[i];
+    unsigned long long sum = c;
+    uint32_t mx = c;
+    for (int off = 32; off > 0; off >>= 1) {
+        sum += __shfl_xor(sum, off, 64);
+        const uint32_t o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        atomicAdd(&out[0], sum);
+        atomicMax(&out[1], (unsigned long long)mx);
+    }
+}
+
+void launch_nl_stats(hipStream_t s, const DemBuffers &b, uint32_t np, uint32_t cap,
+                     unsigned long long *out)
+{
+    hipLaunchKernelGGL(k_nl_stats, dim3((np + 255) / 256), dim3(256), 0, s, b.nl_cnt, np, cap,
+                       out);
+}
+
 // Image of list entry b as the reference's 27-cell search would reach it
 // this step: b counts only if its cell is one of the 27 around the cell
 // (gx, gy, gz) (periodic axes wrapped to the nearest image; valid for 3 or
@@ -524,34 +551,26 @@ __device__ __forceinline__ uint32_t nl_first(const Grid &g, int gx, int gy, int 
     return bk;
 }
 
-// Topology from the neighbour list with the slot rule of
-// src/contactsearch.cuh:267-286.  Returns true on slot overflow.
-__device__ bool topology_list(const Grid &g, uint32_t np, uint32_t pos, uint32_t cnt,
-                              const uint32_t *__restrict__ nl, const uint32_t *__restrict__ cellc,
-                              const double4 *__restrict__ x, double4 xa, int gx, int gy, int gz,
-                              uint32_t *row, uint8_t *crow, uint64_t &mask, int NCS,
-                              uint64_t full)
+// Slot rule of src/contactsearch.cuh:267-286 applied to the contacts found
+// this step (`found`: positions in the particle's list).  Returns true on
+// slot overflow.
+__device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 found,
+                            const uint32_t *__restrict__ nl, const uint32_t *__restrict__ cellc,
+                            int gx, int gy, int gz, uint32_t *row, uint8_t *crow,
+                            uint64_t &mask, int NCS, uint64_t full)
 {
-    Set128 found = {0ull, 0ull}, fresh = {0ull, 0ull};
-    int n_found = 0, n_new = 0;
-    for (uint32_t k = 0; k < cnt; ++k) {
-        const uint32_t b = nl[(size_t)k * np + pos];
-        const uint32_t cb = cellc[b];
-        const double4 xb = x[b];
-        uint32_t offc, code;
-        d3 dm;
-        if (!nl_image(g, gx, gy, gz, cb, offc, code, dm)) continue;
-        const d3 x_ab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z) + dm;
-        const double s2 = dot3(x_ab, x_ab);
-        const double rsum = xa.w + xb.w;
-        if (s2 >= rsum * rsum * (1.0 + 1e-9)) continue;
-        const double delta = sqrt(s2) - rsum;
-        if (!(delta < 0.0)) continue;
-        ++n_found;
-        set_bit(found, k);
-        if (find_slot(row, mask, b) < 0) {
-            set_bit(fresh, k);
-            ++n_new;
+    Set128 fresh = {0ull, 0ull};
+    int n_new = 0;
+    const int n_found = __popcll(found.lo) + __popcll(found.hi);
+    for (int part = 0; part < 2; ++part) {
+        uint64_t w = part ? found.hi : found.lo;
+        while (w) {
+            const uint32_t k = (uint32_t)(__ffsll((unsigned long long)w) - 1) + 64u * part;
+            w &= w - 1;
+            if (find_slot(row, mask, nl[(size_t)k * np + pos]) < 0) {
+                set_bit(fresh, k);
+                ++n_new;
+            }
         }
     }
     if (n_found <= NCS && __popcll(mask) + n_new <= NCS) {
@@ -568,7 +587,8 @@ __device__ bool topology_list(const Grid &g, uint32_t np, uint32_t pos, uint32_t
                 uint32_t offc = 0, code = 0;
                 d3 dm;
                 (void)nl_image(g, gx, gy, gz, cellc[b], offc, code, dm);
-                crow[find_slot(row, mask, b)] = (uint8_t)code;
+                const int sl = find_slot(row, mask, b);
+                if (crow[sl] != (uint8_t)code) crow[sl] = (uint8_t)code;
             }
         }
         while (any_bit(fresh)) {
@@ -607,6 +627,21 @@ __device__ bool topology_list(const Grid &g, uint32_t np, uint32_t pos, uint32_t
         if (nc > NCS) broken = (int)offc;
     }
     return ovf;
+}
+
+// Overlap test of list entry b for a particle at xa in cell (gx, gy, gz),
+// with the reference's arithmetic (contactsearch.cuh:225-236).
+__device__ __forceinline__ bool nl_overlap(const Grid &g, double4 xa, int gx, int gy, int gz,
+                                           double4 xb, uint32_t cb)
+{
+    uint32_t offc, code;
+    d3 dm;
+    if (!nl_image(g, gx, gy, gz, cb, offc, code, dm)) return false;
+    const d3 x_ab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z) + dm;
+    const double s2 = dot3(x_ab, x_ab);
+    const double rsum = xa.w + xb.w;
+    if (s2 >= rsum * rsum * (1.0 + 1e-9)) return false;
+    return sqrt(s2) - rsum < 0.0;
 }
 
 struct Acc {
@@ -943,25 +978,19 @@ __device__ bool topology_cells(const Grid &g, uint32_t idx_a, double4 xa, int gx
     return ovf;
 }
 
-// NL = false: one thread per sorted position, 27-cell search (the
-// reference's pipeline).  NL = true: one thread per position of the last
-// list rebuild; the search tests the particle's list entries, or the 27
-// cells when its list overflowed (the sort then runs every step, so the
-// sorted arrays are current).
-template <bool NL>
-__global__ void __launch_bounds__(256) k_topology(
+// Topology with the 27-cell search, one thread per sorted position (the
+// reference's pipeline).
+__global__ void __launch_bounds__(256) k_topology_cells(
     StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
-    const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x,
-    const double4 *__restrict__ x_s, uint32_t *__restrict__ contacts,
-    uint64_t *__restrict__ cmask, uint8_t *__restrict__ dmcode, uint32_t *__restrict__ err,
-    const uint32_t *__restrict__ nl, const uint32_t *__restrict__ nl_cnt,
-    const uint32_t *__restrict__ nl_perm, const uint32_t *__restrict__ cellc)
+    const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x_s,
+    uint32_t *__restrict__ contacts, uint64_t *__restrict__ cmask,
+    uint8_t *__restrict__ dmcode, uint32_t *__restrict__ err)
 {
     const uint32_t idx_a = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx_a >= a.np) return;
     const Grid &g = a.grid;
-    const uint32_t a_orig = NL ? nl_perm[idx_a] : vals[idx_a];
-    const double4 xa = NL ? x[a_orig] : x_s[idx_a];
+    const uint32_t a_orig = vals[idx_a];
+    const double4 xa = x_s[idx_a];
     int gx, gy, gz;
     if (!grid_pos(g, xa, gx, gy, gz)) {
         // the step is invalid (k_hash reports it too); no cell index may
@@ -971,18 +1000,91 @@ __global__ void __launch_bounds__(256) k_topology(
     }
     const int NCS = (int)a.nc;
     const uint64_t full = NCS >= 64 ? ~0ull : ((1ull << NCS) - 1ull);
+    const uint64_t mask0 = cmask[a_orig];
+    uint64_t mask = mask0;
+    const bool ovf = topology_cells(g, idx_a, xa, gx, gy, gz, vals, cell_start, cell_end, x_s,
+                                    contacts + (size_t)a_orig * NCS,
+                                    dmcode + (size_t)a_orig * NCS, mask, NCS, full);
+    if (ovf) atomicOr(err, ERRF_OVERFLOW);
+    if (mask != mask0) cmask[a_orig] = mask;
+}
+
+// Topology from the neighbour lists, one wave per 64 rebuild positions.
+// The list entries of the 64 particles are dealt out to the lanes (one
+// overlap test per lane per round, so long lists of large grains do not
+// hold up the wave); each owner then applies the slot rule to its hits.  A
+// particle whose list overflowed searches its 27 cells instead (the sort
+// then runs every step, so the sorted arrays are current).
+#define TW 64
+__global__ void __launch_bounds__(TW) k_topology_lists(
+    StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
+    const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x,
+    const double4 *__restrict__ x_s, uint32_t *__restrict__ contacts,
+    uint64_t *__restrict__ cmask, uint8_t *__restrict__ dmcode, uint32_t *__restrict__ err,
+    const uint32_t *__restrict__ nl, const uint32_t *__restrict__ nl_cnt,
+    const uint32_t *__restrict__ nl_perm, const uint32_t *__restrict__ cellc)
+{
+    __shared__ double4 s_x[TW];
+    __shared__ int s_g[TW][3];
+    __shared__ uint8_t s_own[TW], s_k[TW], s_hit[TW];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t pos = blockIdx.x * TW + lane;
+    const uint32_t np = a.np;
+    const Grid &g = a.grid;
+    const bool valid = pos < np;
+    const uint32_t a_orig = valid ? nl_perm[pos] : 0u;
+    const double4 xa = valid ? x[a_orig] : make_double4(0.0, 0.0, 0.0, 0.0);
+    int gx = 0, gy = 0, gz = 0;
+    const bool inside = valid && grid_pos(g, xa, gx, gy, gz);
+    if (valid && !inside) atomicOr(err, ERRF_GRID);
+    const uint32_t lcnt = inside ? nl_cnt[pos] : 0u;
+    const bool list_ovf = lcnt == NL_OVF;
+    const uint32_t n = list_ovf ? 0u : lcnt;
+    s_x[lane] = xa;
+    s_g[lane][0] = gx;
+    s_g[lane][1] = gy;
+    s_g[lane][2] = gz;
+    uint32_t incl = n;
+    for (uint32_t off = 1; off < TW; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off, TW);
+        if (lane >= off) incl += t;
+    }
+    const uint32_t excl = incl - n;
+    const uint32_t total = __shfl(incl, TW - 1, TW);
+    Set128 found = {0ull, 0ull};
+    for (uint32_t base = 0; base < total; base += TW) {
+        const uint32_t lo = excl > base ? excl : base;
+        const uint32_t hi = excl + n < base + TW ? excl + n : base + TW;
+        for (uint32_t c = lo; c < hi; ++c) {
+            s_own[c - base] = (uint8_t)lane;
+            s_k[c - base] = (uint8_t)(c - excl);
+        }
+        __syncthreads();
+        if (base + lane < total) {
+            const uint32_t o = s_own[lane];
+            const uint32_t b = nl[(size_t)s_k[lane] * np + blockIdx.x * TW + o];
+            s_hit[lane] = nl_overlap(g, s_x[o], s_g[o][0], s_g[o][1], s_g[o][2], x[b], cellc[b])
+                              ? 1
+                              : 0;
+        }
+        __syncthreads();
+        for (uint32_t c = lo; c < hi; ++c)
+            if (s_hit[c - base]) set_bit(found, c - excl);
+        __syncthreads();
+    }
+    if (!inside) return;
+    const int NCS = (int)a.nc;
+    const uint64_t full = NCS >= 64 ? ~0ull : ((1ull << NCS) - 1ull);
     uint32_t *row = contacts + (size_t)a_orig * NCS;
     uint8_t *crow = dmcode + (size_t)a_orig * NCS;
     const uint64_t mask0 = cmask[a_orig];
     uint64_t mask = mask0;
     bool ovf;
-    const uint32_t lcnt = NL ? nl_cnt[idx_a] : NL_OVF;
-    if (NL && lcnt != NL_OVF)
-        ovf = topology_list(g, a.np, idx_a, lcnt, nl, cellc, x, xa, gx, gy, gz, row, crow, mask,
-                            NCS, full);
+    if (list_ovf)
+        ovf = topology_cells(g, pos, xa, gx, gy, gz, vals, cell_start, cell_end, x_s, row, crow,
+                             mask, NCS, full);
     else
-        ovf = topology_cells(g, idx_a, xa, gx, gy, gz, vals, cell_start, cell_end, x_s, row,
-                             crow, mask, NCS, full);
+        ovf = slot_update(g, np, pos, found, nl, cellc, gx, gy, gz, row, crow, mask, NCS, full);
     if (ovf) atomicOr(err, ERRF_OVERFLOW);
     if (mask != mask0) cmask[a_orig] = mask;
 }
@@ -1250,24 +1352,28 @@ __global__ void __launch_bounds__(256) k_contact_viscous(
                walls_force_pp, wf0, wf1, wf2, wf3, wf4);
 }
 
+void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist)
+{
+    if (a.prm.contactmodel == 1) return;  // model 1 keeps no contact history
+    if (nlist)
+        hipLaunchKernelGGL(k_topology_lists, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s, a,
+                           b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
+                           b.dmcode, b.errflags, b.nl, b.nl_cnt, b.nl_perm, b.cellc);
+    else
+        hipLaunchKernelGGL(k_topology_cells, dim3((a.np + 255) / 256), dim3(256), 0, s, a,
+                           b.vals, b.cell_start, b.cell_end, b.x_s, b.contacts, b.cmask,
+                           b.dmcode, b.errflags);
+}
+
 void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist)
 {
-    const uint32_t nb = (a.np + 255) / 256;
     if (a.prm.contactmodel == 1) {
-        hipLaunchKernelGGL(k_contact_viscous, dim3(nb), dim3(256), 0, s, a, b.vals, b.cell_start,
-                           b.cell_end, b.x_s, b.vel_s, b.angvel_s, b.force, b.torque, b.es_dot,
-                           b.ev_dot, b.es, b.ev, b.p, b.w_nx, b.w_mvfd, b.walls_force_pp,
-                           b.errflags);
+        hipLaunchKernelGGL(k_contact_viscous, dim3((a.np + 255) / 256), dim3(256), 0, s, a,
+                           b.vals, b.cell_start, b.cell_end, b.x_s, b.vel_s, b.angvel_s, b.force,
+                           b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p, b.w_nx, b.w_mvfd,
+                           b.walls_force_pp, b.errflags);
         return;
     }
-    if (nlist)
-        hipLaunchKernelGGL(k_topology<true>, dim3(nb), dim3(256), 0, s, a, b.vals, b.cell_start,
-                           b.cell_end, b.x, b.x_s, b.contacts, b.cmask, b.dmcode, b.errflags,
-                           b.nl, b.nl_cnt, b.nl_perm, b.cellc);
-    else
-        hipLaunchKernelGGL(k_topology<false>, dim3(nb), dim3(256), 0, s, a, b.vals,
-                           b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask, b.dmcode,
-                           b.errflags, nullptr, nullptr, nullptr, nullptr);
     const uint32_t nw = (a.np + FW - 1) / FW;
     const uint32_t *order = nlist ? b.nl_perm : b.vals;
     if (a.prm.contactmodel == 3)

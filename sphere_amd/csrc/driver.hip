@@ -782,6 +782,7 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             { ProfScope ps(ctx, SPH_K_SORT); launch_sort(s, a.np, ctx->key_bits, ctx->db, cond); }
             { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells, cond); }
             { ProfScope ps(ctx, SPH_K_NLIST); launch_nl_build(s, a, ctx->nla, ctx->db, cond); }
+            { ProfScope ps(ctx, SPH_K_TOPOLOGY); launch_topology(s, a, ctx->db, true); }
             { ProfScope ps(ctx, SPH_K_CONTACT); launch_contact(s, a, ctx->db, true); }
             ctx->nl_parity ^= 1u;
             ctx->nl_force = 0;
@@ -789,6 +790,10 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             { ProfScope ps(ctx, SPH_K_HASH); launch_hash(s, a, ctx->db); }
             { ProfScope ps(ctx, SPH_K_SORT); launch_sort(s, a.np, ctx->key_bits, ctx->db, nullptr); }
             { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells, nullptr); }
+            if (h.contactmodel != 1) {
+                ProfScope ps(ctx, SPH_K_TOPOLOGY);
+                launch_topology(s, a, ctx->db, false);
+            }
             { ProfScope ps(ctx, SPH_K_CONTACT); launch_contact(s, a, ctx->db, false); }
         }
         if (h.nb0 > 0) { ProfScope ps(ctx, SPH_K_BONDS); launch_bonds(s, a, ctx->db); }
@@ -941,6 +946,21 @@ int sph_get_neighbor_stats(sph_ctx *ctx, sph_neighbor_stats *out)
     uint32_t f[4];
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(f, ctx->db.nl_flags, sizeof(f), hipMemcpyDeviceToHost));
+    if (f[3] > 0) {  // lists exist
+        unsigned long long *tmp = nullptr;
+        HIPCHK(hipMalloc((void **)&tmp, 16));
+        unsigned long long h2[2] = {0ull, 0ull};
+        hipError_t e = hipMemcpy(tmp, h2, 16, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            launch_nl_stats(ctx->stream, ctx->db, ctx->np, ctx->nla.cap, tmp);
+            e = hipStreamSynchronize(ctx->stream);
+        }
+        if (e == hipSuccess) e = hipMemcpy(h2, tmp, 16, hipMemcpyDeviceToHost);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(ctx, SPH_ERR_HIP, std::string("list statistics: ") + hipGetErrorString(e));
+        out->entries = h2[0];
+        out->longest = (uint32_t)h2[1];
+    }
     out->enabled = 1;
     out->range = ctx->nla.range;
     out->cap = ctx->nla.cap;

@@ -52,6 +52,12 @@ void launch_reorder(hipStream_t s, const StepArgs &a, const DemBuffers &b, uint3
                     const uint32_t *cond);
 void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const DemBuffers &b,
                      const uint32_t *cond);
+// out[0] += sum of list lengths, out[1] = max(out[1], longest list)
+void launch_nl_stats(hipStream_t s, const DemBuffers &b, uint32_t np, uint32_t cap,
+                     unsigned long long *out);
+// Contact search (models 2, 3): new overlaps take slots (topology).
+void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist);
+// Contact forces, walls and per-particle sums (interact).
 void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist);
 void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b);
 void launch_shear_stress(hipStream_t s, const StepArgs &a, const DemBuffers &b);
