@@ -599,16 +599,29 @@ class sim:
         return buf
 
     def from_state(self, buf):
-        """Copy a StateBuffers image (e.g. Engine.read()) into this object."""
+        """Copy a StateBuffers image (e.g. Engine.read()) into this object:
+        the inverse of to_state(), header fields included."""
         h = buf.state
         n = h.np
+        f64 = numpy.float64
+        self.nd = numpy.ones(1, dtype=numpy.int32) * h.nd
         self.np = numpy.ones(1, dtype=numpy.uint32) * n
         self.time_dt = numpy.array([h.time_dt])
         self.time_current = numpy.array([h.time_current])
         self.time_total = numpy.array([h.time_total])
         self.time_file_dt = numpy.array([h.time_file_dt])
         self.time_step_count = numpy.array([h.time_step_count], dtype=numpy.uint32)
-        self.L = numpy.array(list(h.L))
+        self.origo = numpy.array(list(h.origo), dtype=f64)
+        self.L = numpy.array(list(h.L), dtype=f64)
+        self.num = numpy.array(list(h.num), dtype=numpy.uint32)
+        self.periodic = numpy.array([h.periodic], dtype=numpy.int32)
+        self.adaptive = numpy.array([h.adaptive], dtype=numpy.int32)
+        self.g = numpy.array(list(h.g), dtype=f64)
+        for nm in ['k_n', 'k_t', 'k_r', 'E', 'gamma_n', 'gamma_t', 'gamma_r', 'mu_s', 'mu_d',
+                   'mu_r', 'gamma_wn', 'gamma_wt', 'mu_ws', 'mu_wd', 'rho', 'kappa', 'db', 'V_b',
+                   'lambda_bar', 'sigma_b', 'tau_b']:
+            setattr(self, nm, numpy.array([getattr(h, nm)], dtype=f64))
+        self.contactmodel = numpy.array([h.contactmodel], dtype=numpy.uint32)
         xr = buf.get('x', n, 4)
         self.x = xr[:, :3].copy()
         self.radius = xr[:, 3].copy()
@@ -620,28 +633,48 @@ class sim:
             setattr(self, nm, buf.get(nm, n, 3).copy())
         for nm in ['es_dot', 'es', 'ev_dot', 'ev', 'p']:
             setattr(self, nm, buf.get(nm, n, 1).copy())
+        self.color = buf.get('color', n, 1).astype(numpy.int32).copy()
         nw = h.nw
-        for i in range(nw):
-            self.w_x[i] = h.w_nx[4 * i + 3]
-            self.w_m[i] = h.w_mvfd[4 * i + 0]
-            self.w_vel[i] = h.w_mvfd[4 * i + 1]
-            self.w_force[i] = h.w_mvfd[4 * i + 2]
-            self.w_sigma0[i] = h.w_mvfd[4 * i + 3]
+        self.nw = numpy.array([nw], dtype=numpy.uint32)
+        self.wmode = numpy.array([h.wmode[i] for i in range(nw)], dtype=numpy.int32)
+        self.w_n = numpy.array([[h.w_nx[4 * i + c] for c in range(3)] for i in range(nw)],
+                               dtype=f64).reshape(nw, 3)
+        self.w_x = numpy.array([h.w_nx[4 * i + 3] for i in range(nw)], dtype=f64)
+        self.w_m = numpy.array([h.w_mvfd[4 * i] for i in range(nw)], dtype=f64)
+        self.w_vel = numpy.array([h.w_mvfd[4 * i + 1] for i in range(nw)], dtype=f64)
+        self.w_force = numpy.array([h.w_mvfd[4 * i + 2] for i in range(nw)], dtype=f64)
+        self.w_sigma0 = numpy.array([h.w_mvfd[4 * i + 3] for i in range(nw)], dtype=f64)
+        self.w_sigma0_A = numpy.array([h.sigma0_A], dtype=f64)
+        self.w_sigma0_f = numpy.array([h.sigma0_f], dtype=f64)
+        self.w_tau_x = numpy.array([h.tau_x], dtype=f64)
         nb = h.nb0
-        if nb:
-            self.bonds = buf.get('bonds', nb, 2).copy()
-            self.bonds_delta_n = buf.get('bonds_delta_n', nb, 1).copy()
-            self.bonds_delta_t = buf.get('bonds_delta_t', nb, 3).copy()
-            self.bonds_omega_n = buf.get('bonds_omega_n', nb, 1).copy()
-            self.bonds_omega_t = buf.get('bonds_omega_t', nb, 3).copy()
-        if self.fluid and h.fluid:
+        self.nb0 = numpy.array([nb], dtype=numpy.uint32)
+        self.bonds = buf.get('bonds', nb, 2).copy().reshape(nb, 2)
+        self.bonds_delta_n = buf.get('bonds_delta_n', nb, 1).copy()
+        self.bonds_delta_t = buf.get('bonds_delta_t', nb, 3).copy().reshape(nb, 3)
+        self.bonds_omega_n = buf.get('bonds_omega_n', nb, 1).copy()
+        self.bonds_omega_t = buf.get('bonds_omega_t', nb, 3).copy().reshape(nb, 3)
+        self.fluid = bool(h.fluid)
+        if self.fluid:
             nx, ny, nz = h.num[0], h.num[1], h.num[2]
             cells = nx * ny * nz
+            self.cfd_solver = numpy.array([h.cfd_solver], dtype=numpy.int32)
+            self.mu = numpy.array([h.mu], dtype=f64)
+            for nm in ['rho_f', 'p_mod_A', 'p_mod_f', 'p_mod_phi', 'bc_bot_flux', 'bc_top_flux',
+                       'c_phi', 'beta_f', 'k_c', 'tolerance']:
+                setattr(self, nm, numpy.array([getattr(h, nm)], dtype=f64))
+            for nm in ['bc_xn', 'bc_xp', 'bc_yn', 'bc_yp', 'bc_bot', 'bc_top', 'free_slip_bot',
+                       'free_slip_top']:
+                setattr(self, nm, numpy.array([getattr(h, nm)], dtype=numpy.int32))
+            self.maxiter = numpy.array([h.maxiter], dtype=numpy.uint32)
+            self.ndem = numpy.array([h.ndem], dtype=numpy.uint32)
             self.v_f = buf.get('v_f', cells, 3).reshape(nz, ny, nx, 3).transpose(2, 1, 0, 3).copy()
             self.p_f = buf.get('p_f', cells, 1).reshape(nz, ny, nx).transpose(2, 1, 0).copy()
             self.phi = buf.get('phi', cells, 1).reshape(nz, ny, nx).transpose(2, 1, 0).copy()
             self.dphi = (buf.get('dphi', cells, 1).reshape(nz, ny, nx).transpose(2, 1, 0)
                          / (h.time_dt * h.ndem))
+            self.p_f_constant = buf.get('p_constant', cells, 1).reshape(nz, ny, nx) \
+                .transpose(2, 1, 0).astype(numpy.int32).copy()
             self.f_p = buf.get('f_p', n, 3).copy()
 
     # ------------------------------------------------------------------
