@@ -677,6 +677,40 @@ class sim:
                 .transpose(2, 1, 0).astype(numpy.int32).copy()
             self.f_p = buf.get('f_p', n, 3).copy()
 
+    def addParticle(self, x, radius, xyzsum=numpy.zeros(3), vel=numpy.zeros(3),
+                    fixvel=numpy.zeros(1), force=numpy.zeros(3), angpos=numpy.zeros(3),
+                    angvel=numpy.zeros(3), torque=numpy.zeros(3), es_dot=numpy.zeros(1),
+                    es=numpy.zeros(1), ev_dot=numpy.zeros(1), ev=numpy.zeros(1),
+                    p=numpy.zeros(1), color=0):
+        """Append one particle (python/sphere.py:821-891)."""
+        self.np = self.np + 1
+        self.x = numpy.append(self.x.reshape(-1, 3), [x], axis=0)
+        self.radius = numpy.append(self.radius, radius)
+        self.vel = numpy.append(self.vel.reshape(-1, 3), [vel], axis=0)
+        self.xyzsum = numpy.append(self.xyzsum.reshape(-1, 3), [xyzsum], axis=0)
+        self.fixvel = numpy.append(self.fixvel, fixvel)
+        self.force = numpy.append(self.force.reshape(-1, 3), [force], axis=0)
+        self.angpos = numpy.append(self.angpos.reshape(-1, 3), [angpos], axis=0)
+        self.angvel = numpy.append(self.angvel.reshape(-1, 3), [angvel], axis=0)
+        self.torque = numpy.append(self.torque.reshape(-1, 3), [torque], axis=0)
+        self.es_dot = numpy.append(self.es_dot, es_dot)
+        self.es = numpy.append(self.es, es)
+        self.ev_dot = numpy.append(self.ev_dot, ev_dot)
+        self.ev = numpy.append(self.ev, ev)
+        self.p = numpy.append(self.p, p)
+        self.color = numpy.append(self.color, color).astype(numpy.int32)
+        if self.fluid:
+            self.f_p = numpy.append(self.f_p.reshape(-1, 3), [numpy.zeros(3)], axis=0)
+
+    def deleteParticle(self, i):
+        """Delete the particle(s) with index i (python/sphere.py:892-935)."""
+        self.np = self.np - numpy.size(i)
+        for nm in ['x', 'radius', 'vel', 'xyzsum', 'fixvel', 'force', 'angpos', 'angvel',
+                   'torque', 'es_dot', 'es', 'ev_dot', 'ev', 'p', 'color']:
+            setattr(self, nm, numpy.delete(getattr(self, nm), i, axis=0))
+        if self.fluid:
+            self.f_p = numpy.delete(self.f_p, i, axis=0)
+
     # ------------------------------------------------------------------
     # setup helpers (python/sphere.py:2358-3170)
     # ------------------------------------------------------------------

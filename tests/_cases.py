@@ -152,3 +152,39 @@ def darcy_case(n=300, seed=0, nxyz=(6, 6, 12), bc_top=0, bc_bot=1, periodic=1, n
     s.time_total[0] = 1.0
     s.time_file_dt[0] = 1.0
     return s
+
+
+def darcy_empty(p_top=None):
+    """The reference's Darcy test box (tests/cfd_tests_darcy.py:14-27): no
+    grains, 4 x 3 x 4 cells of 0.1 m, default fluid; p_top sets the top
+    layer of the pressure field (a Dirichlet value)."""
+    s = sphere_amd.sim(np=0, nd=3, nw=0, sid="cfdtest", fluid=True)
+    s.defaultParams()
+    s.defineWorldBoundaries([0.4, 0.3, 0.4], dx=0.1)
+    s.initFluid(cfd_solver=1)
+    if p_top is None:
+        s.initTemporal(total=0.2, file_dt=0.01, dt=1.0e-7)
+        s.time_file_dt[0] = s.time_dt[0] * 0.99
+        s.time_total[0] = s.time_dt[0] * 10
+    else:
+        s.p_f[:, :, -1] = p_top
+        s.initTemporal(total=0.5, file_dt=0.01, dt=1.0e-6)
+    return s
+
+
+def fluid_particle(test):
+    """tests/fluid_particle_interaction_darcy.py: one grain in a 1 m box of
+    0.1 m cells; test 1: p = 1 on the top layer, no gravity (x = 0.55,
+    r = 0.05); test 2: gravity, p = 0 on top (x = 0.5, r = 0.01)."""
+    s = sphere_amd.sim(sid="fluid_particle_interaction", fluid=True)
+    s.defineWorldBoundaries([1.0, 1.0, 1.0], dx=0.1)
+    s.initFluid(cfd_solver=1)
+    if test == 1:
+        s.p_f[:, :, -1] = 1.0
+        s.addParticle([0.55, 0.55, 0.55], 0.05)
+    else:
+        s.p_f[:, :, -1] = 0.0
+        s.addParticle([0.5, 0.5, 0.5], 0.01)
+        s.g[2] = -10.0
+    s.initTemporal(total=0.001, file_dt=0.0001)
+    return s

@@ -10,7 +10,7 @@ import numpy
 import pytest
 
 import sphere_amd
-from _cases import add_bonds, darcy_case, dense_case, settling_case
+from _cases import add_bonds, darcy_case, darcy_empty, dense_case, fluid_particle, settling_case
 from _oracle import Oracle, OracleError
 
 pytestmark = pytest.mark.gpu
@@ -329,3 +329,33 @@ def test_darcy_wall_neumann_top(engine):
     s = darcy_case(n=250, seed=4, nxyz=(5, 6, 10), bc_top=1, bc_bot=1, wall=True, tolerance=1e-7)
     ora, _ = run_pair(engine, s, 2, 2, fluid=True, label="darcy wall")
     assert engine.darcy_stats().sweeps == ora.darcy_stats().sweeps
+
+
+def test_darcy_reference_no_forcing(engine):
+    """tests/cfd_tests_darcy.py:14-45 through the C ABI: p stays 0 and no
+    fluid step iterates."""
+    s = darcy_empty()
+    engine.load(s.to_state())
+    engine.step(11)
+    st = engine.read()
+    assert numpy.all(st.get("p_f", 48, 1) == 0.0)
+    assert engine.darcy_stats().sweeps == 0 or engine.darcy_stats().last_nijac == 0
+
+
+def test_darcy_reference_gradient(engine):
+    """tests/cfd_tests_darcy.py:48-69: the linear steady profile, matched
+    against the oracle field by field every 500 steps."""
+    s = darcy_empty(p_top=1.0)
+    ora, _ = run_pair(engine, s, 4, 500, fluid=True, label="darcy gradient", check_contacts=False)
+    p = engine.read().get("p_f", 48, 1).reshape(4, 3, 4)
+    assert numpy.max(numpy.abs(p - numpy.linspace(0.0, 1.0, 4)[:, None, None])) < 1e-3
+    assert engine.darcy_stats().sweeps == ora.darcy_stats().sweeps
+
+
+@pytest.mark.parametrize("test", [1, 2])
+def test_darcy_reference_grain(engine, test):
+    """tests/fluid_particle_interaction_darcy.py: the grain moves to low
+    pressure, with the oracle's trajectory."""
+    s = fluid_particle(test)
+    run_pair(engine, s, 2, 200, fluid=True, label="grain %d" % test, check_contacts=False)
+    assert engine.read().get("vel", 1, 4)[0, 2] < 0.0
