@@ -359,6 +359,17 @@ __device__ __forceinline__ d3 decode_dm(const Grid &g, uint32_t code)
 // A7-A10: contact laws
 // ------------------------------------------------------------------
 
+// Slot holding b among the occupied slots of a row, or -1.
+__device__ __forceinline__ int find_slot(const uint32_t *row, uint64_t mask, uint32_t b)
+{
+    while (mask) {
+        const int i = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1;
+        if (row[i] == b) return i;
+    }
+    return -1;
+}
+
 // ------------------------------------------------------------------
 // Neighbour (Verlet) lists
 // ------------------------------------------------------------------
@@ -373,7 +384,9 @@ __global__ void __launch_bounds__(256) k_nl_build(
     StepArgs a, NLArgs n, const uint32_t *__restrict__ cond, const uint32_t *__restrict__ vals,
     const uint32_t *__restrict__ cell_start, const uint32_t *__restrict__ cell_end,
     const double4 *__restrict__ x_s, uint32_t *__restrict__ nl, uint32_t *__restrict__ nl_cnt,
-    uint32_t *__restrict__ nl_perm, double4 *__restrict__ x_ref, uint32_t *__restrict__ flags)
+    uint32_t *__restrict__ nl_perm, double4 *__restrict__ x_ref, uint32_t *__restrict__ flags,
+    const uint32_t *__restrict__ contacts, const uint64_t *__restrict__ cmask,
+    uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand)
 {
     SKIP_UNLESS(cond);
     const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
@@ -417,11 +430,22 @@ __global__ void __launch_bounds__(256) k_nl_build(
                     }
                 }
     }
+    // maps between list entries and the slots of the particle's row
+    const int NCS = (int)a.nc;
+    uint8_t *scand = slot_cand + (size_t)a_orig * NCS;
+    for (int sl = 0; sl < NCS; ++sl) scand[sl] = 0xffu;
     if (cnt > n.cap) {
         nl_cnt[pos] = NL_OVF;
         atomicOr(&flags[2], 1u);
     } else {
         nl_cnt[pos] = cnt;
+        const uint32_t *row = contacts + (size_t)a_orig * NCS;
+        const uint64_t mask = cmask[a_orig];
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const int sl = find_slot(row, mask, nl[(size_t)k * np + pos]);
+            nl_slot[(size_t)k * np + pos] = sl < 0 ? 0xffu : (uint8_t)sl;
+            if (sl >= 0) scand[sl] = (uint8_t)k;
+        }
     }
 }
 
@@ -430,7 +454,8 @@ void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const De
 {
     uint32_t nb = (a.np + 255) / 256;
     hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(256), 0, s, a, n, cond, b.vals, b.cell_start,
-                       b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_perm, b.x_ref, b.nl_flags);
+                       b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_perm, b.x_ref, b.nl_flags,
+                       b.contacts, b.cmask, b.nl_slot, b.slot_cand);
 }
 
 // Sum and maximum of the list lengths (overflowed lists count as cap + 1).
@@ -490,17 +515,6 @@ __device__ __forceinline__ bool nl_image(const Grid &g, int gx, int gy, int gz, 
     return true;
 }
 
-// Slot holding b among the occupied slots of a row, or -1.
-__device__ __forceinline__ int find_slot(const uint32_t *row, uint64_t mask, uint32_t b)
-{
-    while (mask) {
-        const int i = __ffsll((unsigned long long)mask) - 1;
-        mask &= mask - 1;
-        if (row[i] == b) return i;
-    }
-    return -1;
-}
-
 // 128-bit set over list positions
 struct Set128 {
     uint64_t lo, hi;
@@ -552,43 +566,33 @@ __device__ __forceinline__ uint32_t nl_first(const Grid &g, int gx, int gy, int 
 }
 
 // Slot rule of src/contactsearch.cuh:267-286 applied to the contacts found
-// this step (`found`: positions in the particle's list).  Returns true on
-// slot overflow.
-__device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 found,
-                            const uint32_t *__restrict__ nl, const uint32_t *__restrict__ cellc,
-                            int gx, int gy, int gz, uint32_t *row, uint8_t *crow,
+// this step from the neighbour list.  found: list positions of every contact
+// found; fresh: those not in the row yet; recode: those in the row whose
+// periodic image changed.  The maps nl_slot (slot of each list entry, 0xff
+// if none) and scand (list entry of each slot of the row, 0xff if none) are
+// kept exact.  Returns true on slot overflow.
+__device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 found, Set128 fresh,
+                            Set128 recode, const uint32_t *__restrict__ nl,
+                            uint8_t *__restrict__ nl_slot, const uint32_t *__restrict__ cellc,
+                            int gx, int gy, int gz, uint32_t *row, uint8_t *crow, uint8_t *scand,
                             uint64_t &mask, int NCS, uint64_t full)
 {
-    Set128 fresh = {0ull, 0ull};
-    int n_new = 0;
     const int n_found = __popcll(found.lo) + __popcll(found.hi);
-    for (int part = 0; part < 2; ++part) {
-        uint64_t w = part ? found.hi : found.lo;
-        while (w) {
-            const uint32_t k = (uint32_t)(__ffsll((unsigned long long)w) - 1) + 64u * part;
-            w &= w - 1;
-            if (find_slot(row, mask, nl[(size_t)k * np + pos]) < 0) {
-                set_bit(fresh, k);
-                ++n_new;
-            }
-        }
-    }
+    const int n_new = __popcll(fresh.lo) + __popcll(fresh.hi);
     if (n_found <= NCS && __popcll(mask) + n_new <= NCS) {
         // No slot runs out: contacts already in the row keep their slots
-        // (their image code is refreshed) and the new ones take the highest
-        // empty slot in discovery order.
-        Set128 old = {found.lo & ~fresh.lo, found.hi & ~fresh.hi};
+        // (a changed image code is rewritten) and the new ones take the
+        // highest empty slot in discovery order.
         for (int part = 0; part < 2; ++part) {
-            uint64_t w = part ? old.hi : old.lo;
+            uint64_t w = part ? recode.hi : recode.lo;
             while (w) {
                 const uint32_t k = (uint32_t)(__ffsll((unsigned long long)w) - 1) + 64u * part;
                 w &= w - 1;
-                const uint32_t b = nl[(size_t)k * np + pos];
+                const size_t li = (size_t)k * np + pos;
                 uint32_t offc = 0, code = 0;
                 d3 dm;
-                (void)nl_image(g, gx, gy, gz, cellc[b], offc, code, dm);
-                const int sl = find_slot(row, mask, b);
-                if (crow[sl] != (uint8_t)code) crow[sl] = (uint8_t)code;
+                (void)nl_image(g, gx, gy, gz, cellc[nl[li]], offc, code, dm);
+                crow[nl_slot[li]] = (uint8_t)code;
             }
         }
         while (any_bit(fresh)) {
@@ -599,6 +603,8 @@ __device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 fou
             row[cpos] = b;
             crow[cpos] = (uint8_t)code;
             mask |= 1ull << cpos;
+            nl_slot[(size_t)k * np + pos] = (uint8_t)cpos;
+            scand[cpos] = (uint8_t)k;
         }
         return false;
     }
@@ -617,9 +623,13 @@ __device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 fou
             cpos = emp ? 63 - __clzll((long long)emp) : nc;
         }
         if (cpos < NCS) {
+            const uint8_t prev = scand[cpos];
+            if (prev != 0xffu && prev != (uint8_t)k) nl_slot[(size_t)prev * np + pos] = 0xffu;
             row[cpos] = b;
             crow[cpos] = (uint8_t)code;
             mask |= 1ull << cpos;
+            nl_slot[(size_t)k * np + pos] = (uint8_t)cpos;
+            scand[cpos] = (uint8_t)k;
         } else {
             ovf = true;
         }
@@ -1010,27 +1020,34 @@ __global__ void __launch_bounds__(256) k_topology_cells(
 }
 
 // Topology from the neighbour lists, one wave per 64 rebuild positions.
-// The list entries of the 64 particles are dealt out to the lanes (one
-// overlap test per lane per round, so long lists of large grains do not
-// hold up the wave); each owner then applies the slot rule to its hits.  A
-// particle whose list overflowed searches its 27 cells instead (the sort
-// then runs every step, so the sorted arrays are current).
+// The list entries of the 64 particles are dealt out to the lanes, TU
+// overlap tests per lane per round with their loads issued together (so
+// long lists of large grains do not hold up the wave); a test also tells
+// from nl_slot whether the contact is new and from the row's image code
+// whether its periodic image changed.  Each owner then applies the slot rule
+// to its new contacts.  A particle whose list overflowed searches its 27
+// cells instead (the sort then runs every step, so the sorted arrays are
+// current).
 #define TW 64
+#define TU 4
 __global__ void __launch_bounds__(TW) k_topology_lists(
     StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
     const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x,
     const double4 *__restrict__ x_s, uint32_t *__restrict__ contacts,
     uint64_t *__restrict__ cmask, uint8_t *__restrict__ dmcode, uint32_t *__restrict__ err,
-    const uint32_t *__restrict__ nl, const uint32_t *__restrict__ nl_cnt,
+    const uint32_t *__restrict__ nl, uint8_t *__restrict__ nl_slot,
+    uint8_t *__restrict__ slot_cand, const uint32_t *__restrict__ nl_cnt,
     const uint32_t *__restrict__ nl_perm, const uint32_t *__restrict__ cellc)
 {
     __shared__ double4 s_x[TW];
     __shared__ int s_g[TW][3];
-    __shared__ uint8_t s_own[TW], s_k[TW], s_hit[TW];
+    __shared__ uint32_t s_a[TW];
+    __shared__ uint8_t s_own[TW * TU], s_k[TW * TU], s_fl[TW * TU];
     const uint32_t lane = threadIdx.x;
     const uint32_t pos = blockIdx.x * TW + lane;
     const uint32_t np = a.np;
     const Grid &g = a.grid;
+    const int NCS = (int)a.nc;
     const bool valid = pos < np;
     const uint32_t a_orig = valid ? nl_perm[pos] : 0u;
     const double4 xa = valid ? x[a_orig] : make_double4(0.0, 0.0, 0.0, 0.0);
@@ -1044,6 +1061,7 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     s_g[lane][0] = gx;
     s_g[lane][1] = gy;
     s_g[lane][2] = gz;
+    s_a[lane] = a_orig;
     uint32_t incl = n;
     for (uint32_t off = 1; off < TW; off <<= 1) {
         const uint32_t t = __shfl_up(incl, off, TW);
@@ -1051,40 +1069,86 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     }
     const uint32_t excl = incl - n;
     const uint32_t total = __shfl(incl, TW - 1, TW);
-    Set128 found = {0ull, 0ull};
-    for (uint32_t base = 0; base < total; base += TW) {
+    Set128 found = {0ull, 0ull}, fresh = {0ull, 0ull}, recode = {0ull, 0ull};
+    for (uint32_t base = 0; base < total; base += TW * TU) {
         const uint32_t lo = excl > base ? excl : base;
-        const uint32_t hi = excl + n < base + TW ? excl + n : base + TW;
+        const uint32_t hi = excl + n < base + TW * TU ? excl + n : base + TW * TU;
         for (uint32_t c = lo; c < hi; ++c) {
             s_own[c - base] = (uint8_t)lane;
             s_k[c - base] = (uint8_t)(c - excl);
         }
         __syncthreads();
-        if (base + lane < total) {
-            const uint32_t o = s_own[lane];
-            const uint32_t b = nl[(size_t)s_k[lane] * np + blockIdx.x * TW + o];
-            s_hit[lane] = nl_overlap(g, s_x[o], s_g[o][0], s_g[o][1], s_g[o][2], x[b], cellc[b])
-                              ? 1
-                              : 0;
+        uint32_t bq[TU];
+        uint8_t sq[TU];
+        double4 xb[TU];
+        uint32_t cb[TU];
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            const uint32_t t = lane + (uint32_t)u * TW;
+            bq[u] = 0u;
+            sq[u] = 0xffu;
+            if (base + t < total) {
+                const size_t li = (size_t)s_k[t] * np + blockIdx.x * TW + s_own[t];
+                bq[u] = nl[li];
+                sq[u] = nl_slot[li];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            const uint32_t t = lane + (uint32_t)u * TW;
+            if (base + t < total) {
+                xb[u] = x[bq[u]];
+                cb[u] = cellc[bq[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            const uint32_t t = lane + (uint32_t)u * TW;
+            if (base + t < total) {
+                const uint32_t o = s_own[t];
+                const double4 xo = s_x[o];
+                uint32_t offc, code;
+                d3 dm;
+                uint8_t fl = 0;
+                if (nl_image(g, s_g[o][0], s_g[o][1], s_g[o][2], cb[u], offc, code, dm)) {
+                    const d3 x_ab = mk3(xo.x - xb[u].x, xo.y - xb[u].y, xo.z - xb[u].z) + dm;
+                    const double s2 = dot3(x_ab, x_ab);
+                    const double rsum = xo.w + xb[u].w;
+                    if (s2 < rsum * rsum * (1.0 + 1e-9) && sqrt(s2) - rsum < 0.0) {
+                        fl = 1;
+                        if (sq[u] == 0xffu)
+                            fl |= 2;
+                        else if (dmcode[(size_t)s_a[o] * NCS + sq[u]] != (uint8_t)code)
+                            fl |= 4;
+                    }
+                }
+                s_fl[t] = fl;
+            }
         }
         __syncthreads();
-        for (uint32_t c = lo; c < hi; ++c)
-            if (s_hit[c - base]) set_bit(found, c - excl);
+        for (uint32_t c = lo; c < hi; ++c) {
+            const uint8_t fl = s_fl[c - base];
+            const uint32_t k = c - excl;
+            if (fl & 1) set_bit(found, k);
+            if (fl & 2) set_bit(fresh, k);
+            if (fl & 4) set_bit(recode, k);
+        }
         __syncthreads();
     }
     if (!inside) return;
-    const int NCS = (int)a.nc;
     const uint64_t full = NCS >= 64 ? ~0ull : ((1ull << NCS) - 1ull);
     uint32_t *row = contacts + (size_t)a_orig * NCS;
     uint8_t *crow = dmcode + (size_t)a_orig * NCS;
     const uint64_t mask0 = cmask[a_orig];
     uint64_t mask = mask0;
-    bool ovf;
+    bool ovf = false;
     if (list_ovf)
         ovf = topology_cells(g, pos, xa, gx, gy, gz, vals, cell_start, cell_end, x_s, row, crow,
                              mask, NCS, full);
-    else
-        ovf = slot_update(g, np, pos, found, nl, cellc, gx, gy, gz, row, crow, mask, NCS, full);
+    else if (any_bit(fresh) || any_bit(recode) ||
+             __popcll(found.lo) + __popcll(found.hi) > NCS)
+        ovf = slot_update(g, np, pos, found, fresh, recode, nl, nl_slot, cellc, gx, gy, gz, row,
+                          crow, slot_cand + (size_t)a_orig * NCS, mask, NCS, full);
     if (ovf) atomicOr(err, ERRF_OVERFLOW);
     if (mask != mask0) cmask[a_orig] = mask;
 }
@@ -1169,7 +1233,8 @@ __global__ void __launch_bounds__(FW) k_forces(
     double *__restrict__ p_out, const double4 *__restrict__ w_nx,
     const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
     uint32_t *__restrict__ contacts, uint64_t *__restrict__ cmask,
-    const uint8_t *__restrict__ dmcode, double *__restrict__ delta_t)
+    const uint8_t *__restrict__ dmcode, double *__restrict__ delta_t,
+    uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand)
 {
     __shared__ uint32_t s_a[FW];
     __shared__ uint8_t s_own[FW], s_slot[FW], s_rm[FW];
@@ -1247,6 +1312,13 @@ __global__ void __launch_bounds__(FW) k_forces(
                 dts[1] = 0.0;
                 dts[2] = 0.0;
                 rm = 1;
+                if (slot_cand) {  // list mode: the list entry loses its slot
+                    const uint8_t kk = slot_cand[mempos];
+                    if (kk != 0xffu) {
+                        nl_slot[(size_t)kk * np + blockIdx.x * FW + s_own[lane]] = 0xffu;
+                        slot_cand[mempos] = 0xffu;
+                    }
+                }
             }
             s_rec[0][lane] = C.F.x;
             s_rec[1][lane] = C.F.y;
@@ -1358,7 +1430,8 @@ void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool
     if (nlist)
         hipLaunchKernelGGL(k_topology_lists, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s, a,
                            b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
-                           b.dmcode, b.errflags, b.nl, b.nl_cnt, b.nl_perm, b.cellc);
+                           b.dmcode, b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
+                           b.nl_perm, b.cellc);
     else
         hipLaunchKernelGGL(k_topology_cells, dim3((a.np + 255) / 256), dim3(256), 0, s, a,
                            b.vals, b.cell_start, b.cell_end, b.x_s, b.contacts, b.cmask,
@@ -1376,16 +1449,18 @@ void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool 
     }
     const uint32_t nw = (a.np + FW - 1) / FW;
     const uint32_t *order = nlist ? b.nl_perm : b.vals;
+    uint8_t *nls = nlist ? b.nl_slot : nullptr;
+    uint8_t *scd = nlist ? b.slot_cand : nullptr;
     if (a.prm.contactmodel == 3)
         hipLaunchKernelGGL(k_forces<true>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
                            b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
                            b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask, b.dmcode,
-                           b.delta_t);
+                           b.delta_t, nls, scd);
     else
         hipLaunchKernelGGL(k_forces<false>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
                            b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
                            b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask, b.dmcode,
-                           b.delta_t);
+                           b.delta_t, nls, scd);
 }
 
 // ------------------------------------------------------------------

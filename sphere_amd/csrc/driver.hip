@@ -182,8 +182,8 @@ static T *dalloc(sph_ctx *ctx, size_t n)
 
 static void free_device(sph_ctx *ctx)
 {
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    for (void *p : ctx->allocs) hipFree(p);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
     ctx->db = DemBuffers{};
     ctx->dd = DarcyBuffers{};
@@ -473,7 +473,10 @@ static int upload(sph_ctx *ctx)
             AL(b.cellc, uint32_t, np); AL(b.x_ref, double4, np);
             AL(b.nl, uint32_t, (size_t)ctx->nl_cap * np); AL(b.nl_cnt, uint32_t, np);
             AL(b.nl_perm, uint32_t, np); AL(b.nl_flags, uint32_t, 4);
+            AL(b.nl_slot, uint8_t, (size_t)ctx->nl_cap * np);
+            AL(b.slot_cand, uint8_t, np * NCS);
             launch_fill_u32(s, b.nl_flags, 4, 0u);
+            launch_fill_u32(s, (uint32_t *)b.slot_cand, (np * NCS + 3) / 4, 0xffffffffu);
             launch_fill_f64(s, (double *)b.x_ref, np * 4, 0.0);
             ctx->nl_on = true;
         }
@@ -870,9 +873,9 @@ void sph_destroy(sph_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     free_device(ctx);
-    for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
-    if (ctx->h_err) hipHostFree(ctx->h_err);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->h_err) (void)hipHostFree(ctx->h_err);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -1218,6 +1221,7 @@ int sph_write_contacts(sph_ctx *ctx, const uint32_t *contacts, const double *dis
     if (delta_t) HIPCHK(hipMemcpyAsync(ctx->db.delta_t, delta_t, n * 24, hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
     launch_rebuild_mask(s, ctx->db, ctx->np, ctx->nc);
+    ctx->nl_force = 1;  // the list/slot maps are rebuilt from the new rows
     return sync(ctx);
 }
 

@@ -39,6 +39,8 @@ struct DemBuffers {
     uint32_t *nl;        // cap*np, column-major by rebuild position
     uint32_t *nl_cnt;    // np, entries per rebuild position (NL_OVF: overflow)
     uint32_t *nl_perm;   // np, original index at each rebuild position
+    uint8_t *nl_slot;    // cap*np like nl: slot of each list entry in the row (0xff: none)
+    uint8_t *slot_cand;  // np*nc like contacts: list entry of each slot (0xff: none)
     uint32_t *nl_flags;  // [0],[1] rebuild flag by step parity, [2] sticky overflow, [3] rebuilds
 };
 
