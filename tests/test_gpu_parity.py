@@ -97,8 +97,9 @@ def compare_contacts(engine, ora, steps):
 def run_pair(engine, s, chunks, per_chunk, fluid=False, label="", check_contacts=True):
     engine.load(s.to_state())
     ora = Oracle(s.to_state(), nc=engine.nc)
-    # in list mode the cell sort is only current on rebuild steps
-    check_sort = fluid or not engine.neighbor_stats().enabled
+    # in list mode the cell sort is only current on rebuild steps; without
+    # grains no sort runs (the reference leaves the arrays unset too)
+    check_sort = (fluid or not engine.neighbor_stats().enabled) and int(s.np[0]) > 0
     done = 0
     ncont = 0
     for _ in range(chunks):
