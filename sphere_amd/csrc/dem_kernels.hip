@@ -377,14 +377,16 @@ __device__ __forceinline__ int find_slot(const uint32_t *row, uint64_t mask, uin
 // surface gap was below `skin` over the (2R+1)^3 cells around it.  Until a
 // particle moves more than skin/2 (less a margin) no pair outside the lists
 // can come into contact, so the per-step contact search only tests list
-// entries.  The list is column-major, entry k of sorted position p at
-// nl[k * np + p], so a wave reads consecutive words.
+// entries.  The list is column-major, entry k of particle a (original
+// index) at nl[k * np + a], so a wave over consecutive particles reads
+// consecutive words.  nl_pos[a] is a's position in the cell sort of the
+// rebuild.
 
 __global__ void __launch_bounds__(256) k_nl_build(
     StepArgs a, NLArgs n, const uint32_t *__restrict__ cond, const uint32_t *__restrict__ vals,
     const uint32_t *__restrict__ cell_start, const uint32_t *__restrict__ cell_end,
     const double4 *__restrict__ x_s, uint32_t *__restrict__ nl, uint32_t *__restrict__ nl_cnt,
-    uint32_t *__restrict__ nl_perm, double4 *__restrict__ x_ref, uint32_t *__restrict__ flags,
+    uint32_t *__restrict__ nl_pos, double4 *__restrict__ x_ref, uint32_t *__restrict__ flags,
     const uint32_t *__restrict__ contacts, const uint64_t *__restrict__ cmask,
     uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand)
 {
@@ -396,7 +398,7 @@ __global__ void __launch_bounds__(256) k_nl_build(
     const Grid &g = a.grid;
     const uint32_t a_orig = vals[pos];
     const double4 xa = x_s[pos];
-    nl_perm[pos] = a_orig;
+    nl_pos[a_orig] = pos;
     x_ref[a_orig] = xa;
     const int gx = (int)floor((xa.x - g.origo[0]) / (g.L[0] / g.num[0]));
     const int gy = (int)floor((xa.y - g.origo[1]) / (g.L[1] / g.num[1]));
@@ -424,7 +426,7 @@ __global__ void __launch_bounds__(256) k_nl_build(
                         const d3 x_ab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z) + dm;
                         const double rr = xa.w + xb.w + n.skin;
                         if (dot3(x_ab, x_ab) < rr * rr) {
-                            if (cnt < n.cap) nl[(size_t)cnt * np + pos] = vals[idx_b];
+                            if (cnt < n.cap) nl[(size_t)cnt * np + a_orig] = vals[idx_b];
                             ++cnt;
                         }
                     }
@@ -435,15 +437,15 @@ __global__ void __launch_bounds__(256) k_nl_build(
     uint8_t *scand = slot_cand + (size_t)a_orig * NCS;
     for (int sl = 0; sl < NCS; ++sl) scand[sl] = 0xffu;
     if (cnt > n.cap) {
-        nl_cnt[pos] = NL_OVF;
+        nl_cnt[a_orig] = NL_OVF;
         atomicOr(&flags[2], 1u);
     } else {
-        nl_cnt[pos] = cnt;
+        nl_cnt[a_orig] = cnt;
         const uint32_t *row = contacts + (size_t)a_orig * NCS;
         const uint64_t mask = cmask[a_orig];
         for (uint32_t k = 0; k < cnt; ++k) {
-            const int sl = find_slot(row, mask, nl[(size_t)k * np + pos]);
-            nl_slot[(size_t)k * np + pos] = sl < 0 ? 0xffu : (uint8_t)sl;
+            const int sl = find_slot(row, mask, nl[(size_t)k * np + a_orig]);
+            nl_slot[(size_t)k * np + a_orig] = sl < 0 ? 0xffu : (uint8_t)sl;
             if (sl >= 0) scand[sl] = (uint8_t)k;
         }
     }
@@ -454,7 +456,7 @@ void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const De
 {
     uint32_t nb = (a.np + 255) / 256;
     hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(256), 0, s, a, n, cond, b.vals, b.cell_start,
-                       b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_perm, b.x_ref, b.nl_flags,
+                       b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_pos, b.x_ref, b.nl_flags,
                        b.contacts, b.cmask, b.nl_slot, b.slot_cand);
 }
 
@@ -1019,7 +1021,7 @@ __global__ void __launch_bounds__(256) k_topology_cells(
     if (mask != mask0) cmask[a_orig] = mask;
 }
 
-// Topology from the neighbour lists, one wave per 64 rebuild positions.
+// Topology from the neighbour lists, one wave per 64 consecutive particles.
 // The list entries of the 64 particles are dealt out to the lanes, TU
 // overlap tests per lane per round with their loads issued together (so
 // long lists of large grains do not hold up the wave); a test also tells
@@ -1028,8 +1030,33 @@ __global__ void __launch_bounds__(256) k_topology_cells(
 // to its new contacts.  A particle whose list overflowed searches its 27
 // cells instead (the sort then runs every step, so the sorted arrays are
 // current).
+// Periodic image of b for a grid searched one cell deep (cells at least as
+// wide as a diameter plus the skin, 3 or more cells along periodic axes):
+// every pair that can overlap lies in neighbouring cells, and the image the
+// reference's cell search uses for such a pair is the nearest one, so it
+// follows from the coordinate difference alone.  code as in dist_mod.
+__device__ __forceinline__ void nearest_image(const Grid &g, double4 xa, double4 xb, d3 &dm,
+                                              uint32_t &code)
+{
+    dm = mk3(0.0, 0.0, 0.0);
+    code = 0;
+    if (g.periodic == 1 || g.periodic == 2) {
+        const double dx = xa.x - xb.x;
+        if (dx < -0.5 * g.L[0]) { dm.x = g.L[0]; code |= 1u; }
+        else if (dx > 0.5 * g.L[0]) { dm.x = -g.L[0]; code |= 2u; }
+    }
+    if (g.periodic == 1) {
+        const double dy = xa.y - xb.y;
+        if (dy < -0.5 * g.L[1]) { dm.y = g.L[1]; code |= 4u; }
+        else if (dy > 0.5 * g.L[1]) { dm.y = -g.L[1]; code |= 8u; }
+    }
+}
+
 #define TW 64
 #define TU 4
+// NEAR: grid searched one cell deep (nearest_image applies; the cell code of
+// list entries is only read for new contacts)
+template <bool NEAR>
 __global__ void __launch_bounds__(TW) k_topology_lists(
     StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
     const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x,
@@ -1037,19 +1064,20 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     uint64_t *__restrict__ cmask, uint8_t *__restrict__ dmcode, uint32_t *__restrict__ err,
     const uint32_t *__restrict__ nl, uint8_t *__restrict__ nl_slot,
     uint8_t *__restrict__ slot_cand, const uint32_t *__restrict__ nl_cnt,
-    const uint32_t *__restrict__ nl_perm, const uint32_t *__restrict__ cellc)
+    const uint32_t *__restrict__ nl_pos, const uint32_t *__restrict__ cellc)
 {
     __shared__ double4 s_x[TW];
     __shared__ int s_g[TW][3];
     __shared__ uint32_t s_a[TW];
     __shared__ uint8_t s_own[TW * TU], s_k[TW * TU], s_fl[TW * TU];
     const uint32_t lane = threadIdx.x;
-    const uint32_t pos = blockIdx.x * TW + lane;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t pos = tile * TW + lane;
     const uint32_t np = a.np;
     const Grid &g = a.grid;
     const int NCS = (int)a.nc;
     const bool valid = pos < np;
-    const uint32_t a_orig = valid ? nl_perm[pos] : 0u;
+    const uint32_t a_orig = valid ? pos : 0u;  // lists are by original index
     const double4 xa = valid ? x[a_orig] : make_double4(0.0, 0.0, 0.0, 0.0);
     int gx = 0, gy = 0, gz = 0;
     const bool inside = valid && grid_pos(g, xa, gx, gy, gz);
@@ -1088,7 +1116,7 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
             bq[u] = 0u;
             sq[u] = 0xffu;
             if (base + t < total) {
-                const size_t li = (size_t)s_k[t] * np + blockIdx.x * TW + s_own[t];
+                const size_t li = (size_t)s_k[t] * np + tile * TW + s_own[t];
                 bq[u] = nl[li];
                 sq[u] = nl_slot[li];
             }
@@ -1098,7 +1126,7 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
             const uint32_t t = lane + (uint32_t)u * TW;
             if (base + t < total) {
                 xb[u] = x[bq[u]];
-                cb[u] = cellc[bq[u]];
+                cb[u] = NEAR ? 0u : cellc[bq[u]];
             }
         }
 #pragma unroll
@@ -1107,10 +1135,17 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
             if (base + t < total) {
                 const uint32_t o = s_own[t];
                 const double4 xo = s_x[o];
-                uint32_t offc, code;
+                uint32_t offc = 0, code = 0;
                 d3 dm;
                 uint8_t fl = 0;
-                if (nl_image(g, s_g[o][0], s_g[o][1], s_g[o][2], cb[u], offc, code, dm)) {
+                bool adjacent;
+                if (NEAR) {
+                    nearest_image(g, xo, xb[u], dm, code);
+                    adjacent = true;
+                } else {
+                    adjacent = nl_image(g, s_g[o][0], s_g[o][1], s_g[o][2], cb[u], offc, code, dm);
+                }
+                if (adjacent) {
                     const d3 x_ab = mk3(xo.x - xb[u].x, xo.y - xb[u].y, xo.z - xb[u].z) + dm;
                     const double s2 = dot3(x_ab, x_ab);
                     const double rsum = xo.w + xb[u].w;
@@ -1143,8 +1178,8 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     uint64_t mask = mask0;
     bool ovf = false;
     if (list_ovf)
-        ovf = topology_cells(g, pos, xa, gx, gy, gz, vals, cell_start, cell_end, x_s, row, crow,
-                             mask, NCS, full);
+        ovf = topology_cells(g, nl_pos[a_orig], xa, gx, gy, gz, vals, cell_start, cell_end, x_s,
+                             row, crow, mask, NCS, full);
     else if (any_bit(fresh) || any_bit(recode) ||
              __popcll(found.lo) + __popcll(found.hi) > NCS)
         ovf = slot_update(g, np, pos, found, fresh, recode, nl, nl_slot, cellc, gx, gy, gz, row,
@@ -1240,10 +1275,11 @@ __global__ void __launch_bounds__(FW) k_forces(
     __shared__ uint8_t s_own[FW], s_slot[FW], s_rm[FW];
     __shared__ double s_rec[NREC][FW];
     const uint32_t lane = threadIdx.x;
-    const uint32_t pos = blockIdx.x * FW + lane;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t pos = tile * FW + lane;
     const uint32_t np = a.np;
     const bool valid = pos < np;
-    const uint32_t a_orig = valid ? order[pos] : 0u;
+    const uint32_t a_orig = valid ? (order ? order[pos] : pos) : 0u;
     const uint64_t mask = valid ? cmask[a_orig] : 0ull;
     const uint32_t n = (uint32_t)__popcll(mask);
     // exclusive scan of the row sizes over the wave
@@ -1315,7 +1351,7 @@ __global__ void __launch_bounds__(FW) k_forces(
                 if (slot_cand) {  // list mode: the list entry loses its slot
                     const uint8_t kk = slot_cand[mempos];
                     if (kk != 0xffu) {
-                        nl_slot[(size_t)kk * np + blockIdx.x * FW + s_own[lane]] = 0xffu;
+                        nl_slot[(size_t)kk * np + tile * FW + s_own[lane]] = 0xffu;
                         slot_cand[mempos] = 0xffu;
                     }
                 }
@@ -1424,14 +1460,20 @@ __global__ void __launch_bounds__(256) k_contact_viscous(
                walls_force_pp, wf0, wf1, wf2, wf3, wf4);
 }
 
-void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist)
+void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist,
+                     bool near)
 {
     if (a.prm.contactmodel == 1) return;  // model 1 keeps no contact history
-    if (nlist)
-        hipLaunchKernelGGL(k_topology_lists, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s, a,
+    if (nlist && near)
+        hipLaunchKernelGGL(k_topology_lists<true>, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s, a,
                            b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
                            b.dmcode, b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
-                           b.nl_perm, b.cellc);
+                           b.nl_pos, b.cellc);
+    else if (nlist)
+        hipLaunchKernelGGL(k_topology_lists<false>, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s,
+                           a, b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
+                           b.dmcode, b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
+                           b.nl_pos, b.cellc);
     else
         hipLaunchKernelGGL(k_topology_cells, dim3((a.np + 255) / 256), dim3(256), 0, s, a,
                            b.vals, b.cell_start, b.cell_end, b.x_s, b.contacts, b.cmask,
@@ -1448,7 +1490,8 @@ void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool 
         return;
     }
     const uint32_t nw = (a.np + FW - 1) / FW;
-    const uint32_t *order = nlist ? b.nl_perm : b.vals;
+    // list mode runs in original order (the lists are by original index)
+    const uint32_t *order = nlist ? nullptr : b.vals;
     uint8_t *nls = nlist ? b.nl_slot : nullptr;
     uint8_t *scd = nlist ? b.slot_cand : nullptr;
     if (a.prm.contactmodel == 3)

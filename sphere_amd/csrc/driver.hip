@@ -472,7 +472,7 @@ static int upload(sph_ctx *ctx)
             ctx->nla.range = range;
             AL(b.cellc, uint32_t, np); AL(b.x_ref, double4, np);
             AL(b.nl, uint32_t, (size_t)ctx->nl_cap * np); AL(b.nl_cnt, uint32_t, np);
-            AL(b.nl_perm, uint32_t, np); AL(b.nl_flags, uint32_t, 4);
+            AL(b.nl_pos, uint32_t, np); AL(b.nl_flags, uint32_t, 4);
             AL(b.nl_slot, uint8_t, (size_t)ctx->nl_cap * np);
             AL(b.slot_cand, uint8_t, np * NCS);
             launch_fill_u32(s, b.nl_flags, 4, 0u);
@@ -785,7 +785,8 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             { ProfScope ps(ctx, SPH_K_SORT); launch_sort(s, a.np, ctx->key_bits, ctx->db, cond); }
             { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells, cond); }
             { ProfScope ps(ctx, SPH_K_NLIST); launch_nl_build(s, a, ctx->nla, ctx->db, cond); }
-            { ProfScope ps(ctx, SPH_K_TOPOLOGY); launch_topology(s, a, ctx->db, true); }
+            { ProfScope ps(ctx, SPH_K_TOPOLOGY);
+              launch_topology(s, a, ctx->db, true, ctx->nla.range == 1); }
             { ProfScope ps(ctx, SPH_K_CONTACT); launch_contact(s, a, ctx->db, true); }
             ctx->nl_parity ^= 1u;
             ctx->nl_force = 0;
@@ -795,7 +796,7 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells, nullptr); }
             if (h.contactmodel != 1) {
                 ProfScope ps(ctx, SPH_K_TOPOLOGY);
-                launch_topology(s, a, ctx->db, false);
+                launch_topology(s, a, ctx->db, false, false);
             }
             { ProfScope ps(ctx, SPH_K_CONTACT); launch_contact(s, a, ctx->db, false); }
         }

@@ -36,9 +36,9 @@ struct DemBuffers {
     // neighbour lists
     uint32_t *cellc;     // np, packed cell coordinates of this step
     double4 *x_ref;      // np, positions at the last rebuild
-    uint32_t *nl;        // cap*np, column-major by rebuild position
-    uint32_t *nl_cnt;    // np, entries per rebuild position (NL_OVF: overflow)
-    uint32_t *nl_perm;   // np, original index at each rebuild position
+    uint32_t *nl;        // cap*np, column-major by original index
+    uint32_t *nl_cnt;    // np, entries per particle (NL_OVF: overflow)
+    uint32_t *nl_pos;    // np, position in the cell sort of the last rebuild
     uint8_t *nl_slot;    // cap*np like nl: slot of each list entry in the row (0xff: none)
     uint8_t *slot_cand;  // np*nc like contacts: list entry of each slot (0xff: none)
     uint32_t *nl_flags;  // [0],[1] rebuild flag by step parity, [2] sticky overflow, [3] rebuilds
@@ -57,8 +57,10 @@ void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const De
 // out[0] += sum of list lengths, out[1] = max(out[1], longest list)
 void launch_nl_stats(hipStream_t s, const DemBuffers &b, uint32_t np, uint32_t cap,
                      unsigned long long *out);
-// Contact search (models 2, 3): new overlaps take slots (topology).
-void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist);
+// Contact search (models 2, 3): new overlaps take slots (topology).  near:
+// list mode on a grid searched one cell deep.
+void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist,
+                     bool near);
 // Contact forces, walls and per-particle sums (interact).
 void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist);
 void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b);

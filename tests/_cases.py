@@ -43,9 +43,11 @@ def settling_case(n=10000, seed=1, steps=2000):
 
 
 def dense_case(n=512, seed=0, model=2, periodic=0, nw=0, vmag=0.05, wmag=5.0, spacing_factor=0.985,
-               E=0.0, nu=0.3, gamma_n=0.0, gamma_t=0.0, mu=0.5, gravity=True, kappa=0.0):
+               E=0.0, nu=0.3, gamma_n=0.0, gamma_t=0.0, mu=0.5, gravity=True, kappa=0.0,
+               cell_factor=2.1):
     """Grains of radius ~U[0.0098, 0.0102] on a cubic lattice slightly
-    tighter than a diameter, with random velocities and spins."""
+    tighter than a diameter, with random velocities and spins; cells of at
+    least cell_factor * 0.0102 m."""
     rng = numpy.random.default_rng(seed)
     side = int(round(n ** (1.0 / 3.0)))
     nx = ny = side
@@ -84,7 +86,7 @@ def dense_case(n=512, seed=0, model=2, periodic=0, nw=0, vmag=0.05, wmag=5.0, sp
     if gravity:
         s.g[2] = -9.81
     s.periodic[0] = periodic
-    s.defineWorldBoundaries(L=L, dx=2.1 * 0.0102)
+    s.defineWorldBoundaries(L=L, dx=cell_factor * 0.0102)
     if nw >= 1:
         s.w_n[0, :] = [0.0, 0.0, -1.0]
         s.w_x[0] = numpy.max(s.x[:, 2] + s.radius) + 0.0005

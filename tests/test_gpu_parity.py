@@ -177,6 +177,31 @@ def test_lists_periodic_crossing():
         e.close()
 
 
+@pytest.mark.parametrize("periodic", [1, 2])
+def test_lists_one_cell_deep_crossing(periodic):
+    """Cells wider than a diameter plus the skin: the list search takes the
+    nearest periodic image without reading cells; grains drift across the
+    periodic boundaries (images change) and all results match the oracle."""
+    s = dense_case(n=729, seed=74 + periodic, model=3, periodic=periodic, E=7e9, nu=0.3,
+                   cell_factor=2.5)
+    L0 = float(s.L[0])
+    s.x[:, 0] = numpy.mod(s.x[:, 0] + 0.5 * L0 - 1e-4, L0)
+    s.vel[:, 0] += 2.0
+    if periodic == 1:
+        L1 = float(s.L[1])
+        s.x[:, 1] = numpy.mod(s.x[:, 1] + 0.5 * L1 - 1e-4, L1)
+        s.vel[:, 1] -= 2.0
+    e = sphere_amd.Engine(0, skin=0.05)
+    try:
+        e.load(s.to_state())
+        assert e.neighbor_stats().range == 1
+        _, nc = run_pair(e, s, 4, 5, label="one deep p%d" % periodic)
+        assert nc > 100
+        assert e.neighbor_stats().rebuilds >= 2
+    finally:
+        e.close()
+
+
 def test_lists_capacity_overflow():
     """Lists of 4 entries overflow: those grains fall back to the cell
     search and the lists are rebuilt every step."""
