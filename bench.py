@@ -27,6 +27,23 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic():
+    """HBM bytes per step of the contact stage from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE passes, calibrated against a
+    copy kernel of known size), or None if it is missing."""
+    path = os.path.join(ROOT, "profiles", "shear_box_1m_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    total = 0.0
+    found = 0
+    for line in open(path):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) > 4 and ("k_forces" in cells[1] or "k_topology" in cells[1]):
+            total += float(cells[4].split()[0]) * 1e6
+            found += 1
+    return total if found == 2 else None
+
+
 def contact_bytes_per_particle(zbar):
     # state in 80 B + slot mask 4 B + outputs 112 B, 52 B per contact (index 4,
     # tangential spring read 24 + write 24)
@@ -193,7 +210,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "contact stage (k_topology_lists + k_forces)",
                      "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": measured_traffic(),
+                     "traffic_source": "profiles/shear_box_1m_hbm_traffic.md",
                      "bytes_per_step": c_bytes, "us_per_step": 1e6 * c_avg_s},
         "kernels": ktimes,
         "cpu_baseline": cpu,
