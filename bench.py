@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--prof-steps", type=int, default=200,
+                    help="steps after the timed ones with per-kernel HIP-event timing")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--stage-steps", type=int, default=400)
@@ -145,12 +147,16 @@ def main():
     run_engine.step(args.warmup)
     warm_buf = run_engine.read()
     n0 = run_engine.count_contacts()
-    run_engine.set_profiling(1)
     log("timed %d steps" % args.steps)
     t0 = time.perf_counter()
     run_engine.step(args.steps)
     elapsed = time.perf_counter() - t0
     n1 = run_engine.count_contacts()
+    # per-kernel device times (HIP events between the launches) in a run of
+    # their own after the timed one, so that the events do not slow it
+    run_engine.set_profiling(1)
+    log("profiled %d steps" % args.prof_steps)
+    run_engine.step(args.prof_steps)
     ktimes = {}
     for k, name in enumerate(sphere_amd._abi.K_NAMES):
         ms, cnt = run_engine.kernel_time(k)
@@ -162,10 +168,12 @@ def main():
     lists = None
     if nls.enabled:
         lists = {"skin_m": nls.skin, "rebuilds_total": int(nls.rebuilds),
+                 "full_search_steps_total": int(nls.full_steps),
                  "entries_per_grain": round(nls.entries / float(args.n), 2),
                  "longest": int(nls.longest)}
-        log("neighbour lists: skin %.3g m, %d rebuilds, %.2f entries per grain, longest %d" %
-            (nls.skin, nls.rebuilds, nls.entries / float(args.n), nls.longest))
+        log("neighbour lists: skin %.3g m, %d rebuilds, %d full-search steps, %.2f entries per "
+            "grain, longest %d" % (nls.skin, nls.rebuilds, nls.full_steps,
+                                   nls.entries / float(args.n), nls.longest))
     n = args.n
     value = n * args.steps / elapsed
     zbar = 0.5 * (n0 + n1) / n

@@ -157,7 +157,9 @@ typedef struct {
     uint64_t rebuilds;  /* rebuilds since the last load */
     uint64_t entries;   /* list entries over all particles at the last rebuild */
     uint32_t longest;   /* longest list (cap + 1 if one overflowed) */
-    uint32_t pad;
+    uint32_t full_steps; /* steps whose search tested every list entry (rebuilds,
+                            periodic wraps); the others test only entries that
+                            can have come into contact */
 } sph_neighbor_stats;
 
 /* Context.  Replaces DEM::DEM(..., initCuda=1, device) (src/sphere.cpp:16-95)

@@ -123,7 +123,7 @@ class SphNeighborStats(ctypes.Structure):
                 ("cap", ctypes.c_uint32), ("overflow", ctypes.c_uint32),
                 ("skin", ctypes.c_double), ("rebuilds", ctypes.c_uint64),
                 ("entries", ctypes.c_uint64), ("longest", ctypes.c_uint32),
-                ("pad", ctypes.c_uint32)]
+                ("full_steps", ctypes.c_uint32)]
 
 
 class StateBuffers:

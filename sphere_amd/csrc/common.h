@@ -36,6 +36,9 @@ struct Params {
     double kappa, db, V_b;
     double lambda_bar, sigma_b, tau_b;
     double sigma0_A, sigma0_f;
+    // Hertz-Mindlin moduli E* = E/(2(1-nu^2)), G* = (E/(2(1+nu)))/(2(2-nu)),
+    // nu = k_r, formed once on the host
+    double E_star, G_star;
 };
 
 struct Walls {
@@ -57,6 +60,19 @@ struct StepArgs {
 // Neighbour-list parameters (no counterpart in the reference, which searches
 // the 27 cells around every particle each step).
 #define NL_OVF 0xffffffffu  // list count of a particle whose list overflowed
+// nl_flags words: [0], [1] per step (alternating), [2] sticky list overflow,
+// [3] rebuild counter, [4] steps whose search tested every list entry,
+// [5] padding, then NLF_DSLOTS 64-bit words whose maximum is the largest
+// squared displacement since the last rebuild (FP64 bit patterns; waves
+// spread their atomic maxima over the words)
+#define NLF_STICKY 2
+#define NLF_COUNT 3
+#define NLF_FULL 4
+#define NLF_DMAX 6
+#define NLF_DSLOTS 64
+#define NLF_WORDS (NLF_DMAX + 2 * NLF_DSLOTS)
+#define NLF_REBUILD 1u  // per-step bit: rebuild the lists
+#define NLF_WRAP 2u     // per-step bit: a grain crossed a periodic boundary
 struct NLArgs {
     double skin;     // listed pairs: surface gap below skin at the rebuild
     double disp2;    // rebuild once a particle moved more than sqrt(disp2)

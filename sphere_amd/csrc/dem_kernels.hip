@@ -56,6 +56,63 @@ void launch_hash(hipStream_t s, const StepArgs &a, const DemBuffers &b)
     hipLaunchKernelGGL(k_hash, dim3(nb), dim3(256), 0, s, a, b.x, b.keys, b.vals, b.errflags);
 }
 
+// Squared displacement of a grain since the last rebuild, along periodic
+// axes to the nearest image (a grain that wraps around has not moved far).
+__device__ __forceinline__ double disp2_since_rebuild(const Grid &g, double4 xi, double4 xr)
+{
+    double dx = xi.x - xr.x, dy = xi.y - xr.y;
+    const double dz = xi.z - xr.z;
+    if (g.periodic == 1 || g.periodic == 2) {
+        if (dx > 0.5 * g.L[0]) dx -= g.L[0];
+        else if (dx < -0.5 * g.L[0]) dx += g.L[0];
+    }
+    if (g.periodic == 1) {
+        if (dy > 0.5 * g.L[1]) dy -= g.L[1];
+        else if (dy < -0.5 * g.L[1]) dy += g.L[1];
+    }
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// One wave's share of the step's list bookkeeping: the rebuild bit (any lane
+// moved too far), the wrap bit (any lane crossed a periodic boundary; the
+// search then tests every list entry) and the largest squared displacement
+// since the rebuild (atomic max on the bit pattern, monotonic for d2 >= 0).
+__device__ __forceinline__ void record_displacement(uint32_t *flags, uint32_t cur, bool need,
+                                                    bool wrapped, double d2)
+{
+    const uint64_t any_need = __ballot(need);
+    const uint64_t any_wrap = __ballot(wrapped);
+    double m = d2 == d2 ? d2 : 0.0;  // NaN already asks for a rebuild
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    const uint64_t active = __ballot(true);
+    const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)active) - 1);
+    if ((threadIdx.x & 63u) == leader) {
+        const uint32_t bits = (any_need ? NLF_REBUILD : 0u) | (any_wrap ? NLF_WRAP : 0u);
+        if (bits) atomicOr(&flags[cur], bits);
+        const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        unsigned long long *slot =
+            (unsigned long long *)(flags + NLF_DMAX) + (wave % NLF_DSLOTS);
+        const unsigned long long v = (unsigned long long)__double_as_longlong(m);
+        if (v > *slot) atomicMax(slot, v);  // most waves see a larger value already
+    }
+}
+
+// Largest squared displacement since the last rebuild: maximum of the
+// NLF_DSLOTS words, formed by one wave (every lane gets it).
+__device__ __forceinline__ double dmax_since_rebuild(const uint32_t *flags)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    double m = __longlong_as_double(((const long long *)(flags + NLF_DMAX))[lane % NLF_DSLOTS]);
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    return m;
+}
+
 // Neighbour-list mode: the same cell IDs plus the packed cell coordinates of
 // every particle (x | y << 11 | z << 22, all ones outside the grid) and the
 // rebuild decision of this step.  flags[cur] becomes non-zero when any
@@ -91,13 +148,10 @@ __global__ void __launch_bounds__(256) k_hash_nl(StepArgs a, NLArgs n,
     keys[i] = c;
     vals[i] = i;
     cellc[i] = packed;
-    const double4 xr = x_ref[i];
-    const double dx = xi.x - xr.x, dy = xi.y - xr.y, dz = xi.z - xr.z;
-    bool need = dx * dx + dy * dy + dz * dz > n.disp2;
-    if (i == 0 && (force || flags[2] != 0u)) need = true;
-    const uint64_t any = __ballot(need);
-    if (any != 0ull && (threadIdx.x & 63u) == (uint32_t)(__ffsll((unsigned long long)any) - 1))
-        atomicOr(&flags[cur], 1u);
+    const double d2 = disp2_since_rebuild(g, xi, x_ref[i]);
+    bool need = !(d2 <= n.disp2);
+    if (i == 0 && (force || flags[NLF_STICKY] != 0u)) need = true;
+    record_displacement(flags, cur, need, false, d2);
 }
 
 void launch_hash_nl(hipStream_t s, const StepArgs &a, const NLArgs &n, const DemBuffers &b,
@@ -121,10 +175,11 @@ void launch_hash_nl(hipStream_t s, const StepArgs &a, const NLArgs &n, const Dem
 #define RS_TILE (RS_THREADS * RS_ITEMS)
 #define RS_WAVES (RS_THREADS / 64)
 
-// Kernels that take `cond` return at once when it is non-null and *cond is
-// zero (neighbour-list mode: the cell sort only runs on rebuild steps).
+// Kernels that take `cond` return at once when it is non-null and its
+// NLF_REBUILD bit is clear (neighbour-list mode: the cell sort only runs on
+// rebuild steps).
 #define SKIP_UNLESS(cond) \
-    if ((cond) != nullptr && *(cond) == 0u) return
+    if ((cond) != nullptr && (*(cond) & NLF_REBUILD) == 0u) return
 
 __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t *__restrict__ keys,
                                                            uint32_t n, int shift,
@@ -290,29 +345,48 @@ __global__ void __launch_bounds__(256) k_reorder(
     uint32_t *__restrict__ cell_end, const uint32_t *__restrict__ cond)
 {
     SKIP_UNLESS(cond);
-    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= np) return;
-    uint32_t c = keys[idx];
-    if (idx == 0 || c != keys[idx - 1]) {
-        cell_start[c] = idx;
-        if (idx > 0) cell_end[keys[idx - 1]] = idx;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < np;
+         idx += gridDim.x * blockDim.x) {
+        uint32_t c = keys[idx];
+        if (idx == 0 || c != keys[idx - 1]) {
+            cell_start[c] = idx;
+            if (idx > 0) cell_end[keys[idx - 1]] = idx;
+        }
+        if (idx == np - 1) cell_end[c] = idx + 1;
+        uint32_t s = vals[idx];
+        x_s[idx] = x[s];
+        vel_s[idx] = vel[s];
+        angvel_s[idx] = angvel[s];
     }
-    if (idx == np - 1) cell_end[c] = idx + 1;
-    uint32_t s = vals[idx];
-    x_s[idx] = x[s];
-    vel_s[idx] = vel[s];
-    angvel_s[idx] = angvel[s];
 }
+
+// cell_start = cell_end = EMPTY_CELL for every cell
+__global__ void __launch_bounds__(256) k_clear_cells(uint32_t *__restrict__ cell_start,
+                                                     uint32_t *__restrict__ cell_end,
+                                                     uint32_t ncells,
+                                                     const uint32_t *__restrict__ cond)
+{
+    SKIP_UNLESS(cond);
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells;
+         c += gridDim.x * blockDim.x) {
+        cell_start[c] = EMPTY_CELL;
+        cell_end[c] = EMPTY_CELL;
+    }
+}
+
+// Kernels that may be skipped run on at most this many blocks (grid-stride
+// loops): a skipped launch then costs little more than its dispatch.
+#define COND_BLOCKS 1024u
 
 void launch_reorder(hipStream_t s, const StepArgs &a, const DemBuffers &b, uint32_t ncells,
                     const uint32_t *cond)
 {
-    const dim3 fg((ncells + 255) / 256);
-    hipLaunchKernelGGL(k_fill_u32_if, fg, dim3(256), 0, s, b.cell_start, (size_t)ncells,
-                       EMPTY_CELL, cond);
-    hipLaunchKernelGGL(k_fill_u32_if, fg, dim3(256), 0, s, b.cell_end, (size_t)ncells,
-                       EMPTY_CELL, cond);
+    uint32_t fb = (ncells + 255) / 256;
+    if (fb > COND_BLOCKS) fb = COND_BLOCKS;
+    hipLaunchKernelGGL(k_clear_cells, dim3(fb), dim3(256), 0, s, b.cell_start, b.cell_end,
+                       ncells, cond);
     uint32_t nb = (a.np + 255) / 256;
+    if (nb > COND_BLOCKS) nb = COND_BLOCKS;
     hipLaunchKernelGGL(k_reorder, dim3(nb), dim3(256), 0, s, a.np, b.keys, b.vals, b.x, b.vel,
                        b.angvel, b.x_s, b.vel_s, b.angvel_s, b.cell_start, b.cell_end, cond);
 }
@@ -382,19 +456,16 @@ __device__ __forceinline__ int find_slot(const uint32_t *row, uint64_t mask, uin
 // consecutive words.  nl_pos[a] is a's position in the cell sort of the
 // rebuild.
 
-__global__ void __launch_bounds__(256) k_nl_build(
-    StepArgs a, NLArgs n, const uint32_t *__restrict__ cond, const uint32_t *__restrict__ vals,
+__device__ void nl_build_one(
+    const StepArgs &a, const NLArgs &n, uint32_t pos, const uint32_t *__restrict__ vals,
     const uint32_t *__restrict__ cell_start, const uint32_t *__restrict__ cell_end,
     const double4 *__restrict__ x_s, uint32_t *__restrict__ nl, uint32_t *__restrict__ nl_cnt,
     uint32_t *__restrict__ nl_pos, double4 *__restrict__ x_ref, uint32_t *__restrict__ flags,
     const uint32_t *__restrict__ contacts, const uint64_t *__restrict__ cmask,
-    uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand)
+    uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand, float *__restrict__ nl_gap,
+    float *__restrict__ nl_gmin)
 {
-    SKIP_UNLESS(cond);
-    const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t np = a.np;
-    if (pos >= np) return;
-    if (pos == 0) flags[3] += 1u;  // rebuild counter
     const Grid &g = a.grid;
     const uint32_t a_orig = vals[pos];
     const double4 xa = x_s[pos];
@@ -425,8 +496,14 @@ __global__ void __launch_bounds__(256) k_nl_build(
                         const double4 xb = x_s[idx_b];
                         const d3 x_ab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z) + dm;
                         const double rr = xa.w + xb.w + n.skin;
-                        if (dot3(x_ab, x_ab) < rr * rr) {
-                            if (cnt < n.cap) nl[(size_t)cnt * np + a_orig] = vals[idx_b];
+                        const double s2 = dot3(x_ab, x_ab);
+                        if (s2 < rr * rr) {
+                            if (cnt < n.cap) {
+                                nl[(size_t)cnt * np + a_orig] = vals[idx_b];
+                                // surface gap, rounded down
+                                nl_gap[(size_t)cnt * np + a_orig] =
+                                    __double2float_rd(sqrt(s2) - (xa.w + xb.w));
+                            }
                             ++cnt;
                         }
                     }
@@ -438,26 +515,53 @@ __global__ void __launch_bounds__(256) k_nl_build(
     for (int sl = 0; sl < NCS; ++sl) scand[sl] = 0xffu;
     if (cnt > n.cap) {
         nl_cnt[a_orig] = NL_OVF;
-        atomicOr(&flags[2], 1u);
+        atomicOr(&flags[NLF_STICKY], 1u);
     } else {
         nl_cnt[a_orig] = cnt;
         const uint32_t *row = contacts + (size_t)a_orig * NCS;
         const uint64_t mask = cmask[a_orig];
+        float gmin = INFINITY;
         for (uint32_t k = 0; k < cnt; ++k) {
             const int sl = find_slot(row, mask, nl[(size_t)k * np + a_orig]);
             nl_slot[(size_t)k * np + a_orig] = sl < 0 ? 0xffu : (uint8_t)sl;
             if (sl >= 0) scand[sl] = (uint8_t)k;
+            else gmin = fminf(gmin, nl_gap[(size_t)k * np + a_orig]);
         }
+        nl_gmin[a_orig] = gmin;
     }
+}
+
+
+__global__ void __launch_bounds__(256) k_nl_build(
+    StepArgs a, NLArgs n, const uint32_t *__restrict__ cond, const uint32_t *__restrict__ vals,
+    const uint32_t *__restrict__ cell_start, const uint32_t *__restrict__ cell_end,
+    const double4 *__restrict__ x_s, uint32_t *__restrict__ nl, uint32_t *__restrict__ nl_cnt,
+    uint32_t *__restrict__ nl_pos, double4 *__restrict__ x_ref, uint32_t *__restrict__ flags,
+    const uint32_t *__restrict__ contacts, const uint64_t *__restrict__ cmask,
+    uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand, float *__restrict__ nl_gap,
+    float *__restrict__ nl_gmin)
+{
+    SKIP_UNLESS(cond);
+    const uint32_t np = a.np;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        flags[NLF_COUNT] += 1u;  // rebuild counter
+    }
+    if (blockIdx.x == 0 && threadIdx.x < NLF_DSLOTS)  // displacements restart
+        ((unsigned long long *)(flags + NLF_DMAX))[threadIdx.x] = 0ull;
+    for (uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x; pos < np;
+         pos += gridDim.x * blockDim.x)
+        nl_build_one(a, n, pos, vals, cell_start, cell_end, x_s, nl, nl_cnt, nl_pos, x_ref, flags,
+                     contacts, cmask, nl_slot, slot_cand, nl_gap, nl_gmin);
 }
 
 void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const DemBuffers &b,
                      const uint32_t *cond)
 {
     uint32_t nb = (a.np + 255) / 256;
+    if (nb > COND_BLOCKS) nb = COND_BLOCKS;
     hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(256), 0, s, a, n, cond, b.vals, b.cell_start,
                        b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_pos, b.x_ref, b.nl_flags,
-                       b.contacts, b.cmask, b.nl_slot, b.slot_cand);
+                       b.contacts, b.cmask, b.nl_slot, b.slot_cand, b.nl_gap, b.nl_gmin);
 }
 
 // Sum and maximum of the list lengths (overflowed lists count as cap + 1).
@@ -684,6 +788,13 @@ __device__ __forceinline__ void add_contrib(Acc &A, const Contrib &C)
     A.p += C.p;
 }
 
+// -kappa sqrt(r_a r_b): for kappa = +0 it is -0 for any positive radii, so
+// the square root is skipped without changing a bit of f_c = factor * n.
+__device__ __forceinline__ double cohesion_factor(double kappa, double r_a, double r_b)
+{
+    return __double_as_longlong(kappa) == 0ll ? -0.0 : -kappa * sqrt(r_a * r_b);
+}
+
 // contactLinear, src/contactmodels.cuh:241-408
 __device__ __forceinline__ void law_linear(const StepArgs &a, Contrib &C, d3 vel_a, d3 angvel_a,
                                            double4 vb, double4 wb, double r_a, double r_b,
@@ -711,7 +822,7 @@ __device__ __forceinline__ void law_linear(const StepArgs &a, Contrib &C, d3 vel
     d3 f_n = fmax(0.0, -k_n * delta + P.gamma_n * vel_n) * n;
     double f_n_length = len3(f_n);
     C.ev_n = 0.5 * P.gamma_n * vel_n * vel_n;
-    d3 f_c = (-P.kappa * sqrt(r_a * r_b)) * n;
+    d3 f_c = cohesion_factor(P.kappa, r_a, r_b) * n;
     d3 f_t = mk3(0.0, 0.0, 0.0);
     if (delta_t0_length > 0.0 || vel_t_length > 0.0) {
         delta_t = delta_t0 + dt * vel_t;
@@ -755,7 +866,6 @@ __device__ __forceinline__ void law_hertz(const StepArgs &a, Contrib &C, d3 vel_
     d3 vel_ab = (vel_ab_linear + (r_a + delta_ab / 2.0) * cross3(n_ab, angvel_a)) +
                 (r_b + delta_ab / 2.0) * cross3(n_ab, angvel_b);
     d3 angvel_ab = angvel_a - angvel_b;
-    double angvel_ab_length = len3(angvel_ab);
     double vel_n_ab = dot3(vel_ab_linear, n_ab);
     d3 vel_t_ab = vel_ab - dot3(vel_ab, n_ab) * n_ab;
     double vel_t_ab_length = len3(vel_t_ab);
@@ -767,19 +877,17 @@ __device__ __forceinline__ void law_hertz(const StepArgs &a, Contrib &C, d3 vel_
     double qd = sqrt(sd);
     double k_n = P.k_n, k_t = P.k_t;
     if (P.E > .001) {
-        double nu = P.k_r;
-        double E_star = P.E / (2.0 * (1.0 - nu * nu));
+        // E* = E/(2(1-nu^2)) and G* = (E/(2(1+nu)))/(2(2-nu)) are formed on
+        // the host with the same IEEE operations
         double R_star = r_a * r_b / (r_a + r_b);
-        double G = P.E / (2.0 * (1.0 + nu));
-        double G_star = G / (2.0 * (2.0 - nu));
         double sR = sqrt(R_star);
-        k_n = 4.0 / 3.0 * E_star * sR;
-        k_t = 8.0 * G_star * sR;
+        k_n = 4.0 / 3.0 * P.E_star * sR;
+        k_t = 8.0 * P.G_star * sR;
     }
     d3 f_n = fmax(0.0, k_n * d * sd - P.gamma_n * qd * vel_n_ab) * n_ab;
     C.ev_n = 0.5 * P.gamma_n * qd * vel_n_ab * vel_n_ab;
     double f_n_length = len3(f_n);
-    d3 f_c = (-P.kappa * sqrt(r_a * r_b)) * n_ab;
+    d3 f_c = cohesion_factor(P.kappa, r_a, r_b) * n_ab;
     d3 f_t = mk3(0.0, 0.0, 0.0);
     d3 T_res = mk3(0.0, 0.0, 0.0);
     if (delta_t0_length > 0.0 || vel_t_ab_length > 0.0) {
@@ -801,9 +909,19 @@ __device__ __forceinline__ void law_hertz(const StepArgs &a, Contrib &C, d3 vel_
             delta_t = delta_t0 + dt * vel_t_ab;
         }
     }
-    if (angvel_ab_length > 0.0) {
-        double m = fmin(P.gamma_r * r_a * angvel_ab_length, P.mu_r * r_a * f_n_length);
-        T_res = (-1.0 * m) * (angvel_ab / angvel_ab_length);
+    const double w2 = dot3(angvel_ab, angvel_ab);
+    if (__double_as_longlong(P.gamma_r) == 0ll && __double_as_longlong(P.mu_r) == 0ll &&
+        w2 < INFINITY && f_n_length < INFINITY) {
+        // gamma_r = mu_r = +0: m = fmin(+0, +0) = +0 and T_res = -0 (w/|w|),
+        // whose components have the signs of -0 * w (|w| = sqrt(w2) > 0 is
+        // finite), so no square root or division is needed
+        if (w2 > 0.0) T_res = (-0.0) * angvel_ab;
+    } else {
+        const double angvel_ab_length = sqrt(w2);
+        if (angvel_ab_length > 0.0) {
+            double m = fmin(P.gamma_r * r_a * angvel_ab_length, P.mu_r * r_a * f_n_length);
+            T_res = (-1.0 * m) * (angvel_ab / angvel_ab_length);
+        }
     }
     C.F = (f_n + f_t) + f_c;
     C.T = (-(r_a + delta_ab / 2.0)) * cross3(n_ab, f_t) + T_res;
@@ -1052,6 +1170,43 @@ __device__ __forceinline__ void nearest_image(const Grid &g, double4 xa, double4
     }
 }
 
+// Every list entry of one particle tested by its owner lane (the rare case
+// in which new contacts find no empty slot on a step that tested only the
+// candidates for new contacts).
+template <bool NEAR>
+__device__ void owner_test_all(const Grid &g, uint32_t np, uint32_t pos, uint32_t lcnt,
+                               const uint32_t *__restrict__ nl,
+                               const uint8_t *__restrict__ nl_slot,
+                               const uint32_t *__restrict__ cellc, const double4 *__restrict__ x,
+                               double4 xa, int gx, int gy, int gz, const uint8_t *crow,
+                               Set128 &found, Set128 &fresh, Set128 &recode)
+{
+    found = fresh = recode = Set128{0ull, 0ull};
+    for (uint32_t k = 0; k < lcnt; ++k) {
+        const size_t li = (size_t)k * np + pos;
+        const uint32_t b = nl[li];
+        const double4 xb = x[b];
+        uint32_t offc = 0, code = 0;
+        d3 dm;
+        bool adjacent;
+        if (NEAR) {
+            nearest_image(g, xa, xb, dm, code);
+            adjacent = true;
+        } else {
+            adjacent = nl_image(g, gx, gy, gz, cellc[b], offc, code, dm);
+        }
+        if (!adjacent) continue;
+        const d3 x_ab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z) + dm;
+        const double s2 = dot3(x_ab, x_ab);
+        const double rsum = xa.w + xb.w;
+        if (!(s2 < rsum * rsum * (1.0 + 1e-9) && sqrt(s2) - rsum < 0.0)) continue;
+        set_bit(found, k);
+        const uint8_t sl = nl_slot[li];
+        if (sl == 0xffu) set_bit(fresh, k);
+        else if (crow[sl] != (uint8_t)code) set_bit(recode, k);
+    }
+}
+
 #define TW 64
 #define TU 4
 // NEAR: grid searched one cell deep (nearest_image applies; the cell code of
@@ -1064,8 +1219,11 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     uint64_t *__restrict__ cmask, uint8_t *__restrict__ dmcode, uint32_t *__restrict__ err,
     const uint32_t *__restrict__ nl, uint8_t *__restrict__ nl_slot,
     uint8_t *__restrict__ slot_cand, const uint32_t *__restrict__ nl_cnt,
-    const uint32_t *__restrict__ nl_pos, const uint32_t *__restrict__ cellc)
+    const uint32_t *__restrict__ nl_pos, const uint32_t *__restrict__ cellc,
+    const float *__restrict__ nl_gap, const float *__restrict__ nl_gmin,
+    uint32_t *__restrict__ flags_rw, uint32_t cur)
 {
+    const uint32_t *flags = flags_rw;
     __shared__ double4 s_x[TW];
     __shared__ int s_g[TW][3];
     __shared__ uint32_t s_a[TW];
@@ -1078,13 +1236,60 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     const int NCS = (int)a.nc;
     const bool valid = pos < np;
     const uint32_t a_orig = valid ? pos : 0u;  // lists are by original index
-    const double4 xa = valid ? x[a_orig] : make_double4(0.0, 0.0, 0.0, 0.0);
-    int gx = 0, gy = 0, gz = 0;
-    const bool inside = valid && grid_pos(g, xa, gx, gy, gz);
+    // this step's cell from the hash (all ones outside the grid)
+    const uint32_t cc = valid ? cellc[pos] : 0xffffffffu;
+    const bool inside = cc != 0xffffffffu;
     if (valid && !inside) atomicOr(err, ERRF_GRID);
+    const int gx = (int)(cc & 2047u), gy = (int)((cc >> 11) & 2047u), gz = (int)(cc >> 22);
     const uint32_t lcnt = inside ? nl_cnt[pos] : 0u;
     const bool list_ovf = lcnt == NL_OVF;
-    const uint32_t n = list_ovf ? 0u : lcnt;
+    // Entries to test.  On steps that rebuilt the lists, or on which a grain
+    // wrapped around a periodic boundary (images may change), every entry.
+    // Otherwise the row can only change by a new contact: an entry not in the
+    // row whose surface gap at the rebuild is at most twice the largest
+    // displacement since then (plus a rounding margin).
+    const bool full = (flags[cur] & (NLF_REBUILD | NLF_WRAP)) != 0u || flags[NLF_STICKY] != 0u;
+    if (full && blockIdx.x == 0 && lane == 0) flags_rw[NLF_FULL] += 1u;
+    const double dmax = dmax_since_rebuild(flags);
+    Set128 q = {0ull, 0ull};
+    uint32_t n = 0;
+    if (!list_ovf && lcnt > 0) {
+        if (full) {
+            n = lcnt;
+            q.lo = lcnt >= 64 ? ~0ull : ((1ull << lcnt) - 1ull);
+            q.hi = lcnt > 64 ? (lcnt >= 128 ? ~0ull : ((1ull << (lcnt - 64)) - 1ull)) : 0ull;
+        } else {
+            const float thr = (float)(2.0 * sqrt(dmax) * (1.0 + 1e-9) +
+                                      1e-11 * (g.L[0] + g.L[1] + g.L[2]));
+            // nl_gmin bounds the gaps of the entries not in the row
+            if (!(nl_gmin[pos] > thr)) {
+                // eight entries per iteration, their loads issued together
+                for (uint32_t k0 = 0; k0 < lcnt; k0 += 8) {
+                    uint8_t sl[8];
+                    float gp[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        sl[u] = 0u;
+                        gp[u] = INFINITY;
+                        if (k0 + u < lcnt) {
+                            const size_t li = (size_t)(k0 + u) * np + pos;
+                            sl[u] = nl_slot[li];
+                            gp[u] = nl_gap[li];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (sl[u] == 0xffu && !(gp[u] > thr)) {
+                            set_bit(q, k0 + u);
+                            ++n;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    const bool busy = inside && (full || list_ovf || n > 0);
+    const double4 xa = busy ? x[a_orig] : make_double4(0.0, 0.0, 0.0, 0.0);
     s_x[lane] = xa;
     s_g[lane][0] = gx;
     s_g[lane][1] = gy;
@@ -1098,12 +1303,21 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     const uint32_t excl = incl - n;
     const uint32_t total = __shfl(incl, TW - 1, TW);
     Set128 found = {0ull, 0ull}, fresh = {0ull, 0ull}, recode = {0ull, 0ull};
+    Set128 deal = q;  // entries not yet dealt, in list order
     for (uint32_t base = 0; base < total; base += TW * TU) {
         const uint32_t lo = excl > base ? excl : base;
         const uint32_t hi = excl + n < base + TW * TU ? excl + n : base + TW * TU;
         for (uint32_t c = lo; c < hi; ++c) {
+            uint32_t k;
+            if (deal.lo) {
+                k = (uint32_t)(__ffsll((unsigned long long)deal.lo) - 1);
+                deal.lo &= deal.lo - 1;
+            } else {
+                k = 64u + (uint32_t)(__ffsll((unsigned long long)deal.hi) - 1);
+                deal.hi &= deal.hi - 1;
+            }
             s_own[c - base] = (uint8_t)lane;
-            s_k[c - base] = (uint8_t)(c - excl);
+            s_k[c - base] = (uint8_t)k;
         }
         __syncthreads();
         uint32_t bq[TU];
@@ -1163,27 +1377,39 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
         __syncthreads();
         for (uint32_t c = lo; c < hi; ++c) {
             const uint8_t fl = s_fl[c - base];
-            const uint32_t k = c - excl;
+            const uint32_t k = s_k[c - base];
             if (fl & 1) set_bit(found, k);
             if (fl & 2) set_bit(fresh, k);
             if (fl & 4) set_bit(recode, k);
         }
         __syncthreads();
     }
-    if (!inside) return;
-    const uint64_t full = NCS >= 64 ? ~0ull : ((1ull << NCS) - 1ull);
+    if (!busy) return;
+    const uint64_t fullmask = NCS >= 64 ? ~0ull : ((1ull << NCS) - 1ull);
     uint32_t *row = contacts + (size_t)a_orig * NCS;
     uint8_t *crow = dmcode + (size_t)a_orig * NCS;
     const uint64_t mask0 = cmask[a_orig];
     uint64_t mask = mask0;
     bool ovf = false;
-    if (list_ovf)
+    if (list_ovf) {
         ovf = topology_cells(g, nl_pos[a_orig], xa, gx, gy, gz, vals, cell_start, cell_end, x_s,
-                             row, crow, mask, NCS, full);
-    else if (any_bit(fresh) || any_bit(recode) ||
-             __popcll(found.lo) + __popcll(found.hi) > NCS)
+                             row, crow, mask, NCS, fullmask);
+    } else if (full) {
+        if (any_bit(fresh) || any_bit(recode) || __popcll(found.lo) + __popcll(found.hi) > NCS)
+            ovf = slot_update(g, np, pos, found, fresh, recode, nl, nl_slot, cellc, gx, gy, gz,
+                              row, crow, slot_cand + (size_t)a_orig * NCS, mask, NCS, fullmask);
+    } else if (any_bit(fresh)) {
+        // only new contacts were tested; if they do not all fit into empty
+        // slots the full rule needs every contact found: test all entries
+        const int n_new = __popcll(fresh.lo) + __popcll(fresh.hi);
+        if (__popcll(mask) + n_new > NCS)
+            owner_test_all<NEAR>(g, np, pos, lcnt, nl, nl_slot, cellc, x, xa, gx, gy, gz, crow,
+                                 found, fresh, recode);
+        else
+            found = fresh;
         ovf = slot_update(g, np, pos, found, fresh, recode, nl, nl_slot, cellc, gx, gy, gz, row,
-                          crow, slot_cand + (size_t)a_orig * NCS, mask, NCS, full);
+                          crow, slot_cand + (size_t)a_orig * NCS, mask, NCS, fullmask);
+    }
     if (ovf) atomicOr(err, ERRF_OVERFLOW);
     if (mask != mask0) cmask[a_orig] = mask;
 }
@@ -1255,6 +1481,30 @@ __device__ __forceinline__ void write_sums(const StepArgs &a, const Acc &A, uint
     if (nw > 4) walls_force_pp[a_orig + 4 * np] = wf4;
 }
 
+__device__ __forceinline__ Acc load_acc(const double (*s)[64], uint32_t lane)
+{
+    Acc A;
+    A.F = mk3(s[0][lane], s[1][lane], s[2][lane]);
+    A.T = mk3(s[3][lane], s[4][lane], s[5][lane]);
+    A.es_dot = s[6][lane];
+    A.ev_dot = s[7][lane];
+    A.p = s[8][lane];
+    return A;
+}
+
+__device__ __forceinline__ void store_acc(double (*s)[64], uint32_t lane, const Acc &A)
+{
+    s[0][lane] = A.F.x;
+    s[1][lane] = A.F.y;
+    s[2][lane] = A.F.z;
+    s[3][lane] = A.T.x;
+    s[4][lane] = A.T.y;
+    s[5][lane] = A.T.z;
+    s[6][lane] = A.es_dot;
+    s[7][lane] = A.ev_dot;
+    s[8][lane] = A.p;
+}
+
 #define FW 64  // k_forces: one wave per block
 #define NREC 10
 
@@ -1269,11 +1519,13 @@ __global__ void __launch_bounds__(FW) k_forces(
     const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
     uint32_t *__restrict__ contacts, uint64_t *__restrict__ cmask,
     const uint8_t *__restrict__ dmcode, double *__restrict__ delta_t,
-    uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand)
+    uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand,
+    const float *__restrict__ nl_gap, float *__restrict__ nl_gmin)
 {
     __shared__ uint32_t s_a[FW];
-    __shared__ uint8_t s_own[FW], s_slot[FW], s_rm[FW];
+    __shared__ uint8_t s_own[FW], s_slot[FW], s_rm[FW], s_rmk[FW];
     __shared__ double s_rec[NREC][FW];
+    __shared__ double s_acc[9][FW];  // the owners' running sums (F, T, es_dot, ev_dot, p)
     const uint32_t lane = threadIdx.x;
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t pos = tile * FW + lane;
@@ -1294,13 +1546,12 @@ __global__ void __launch_bounds__(FW) k_forces(
     const Grid &g = a.grid;
     const int NCS = (int)a.nc;
 
-    Acc A;
-    A.F = mk3(0.0, 0.0, 0.0);
-    A.T = mk3(0.0, 0.0, 0.0);
-    A.es_dot = 0.0;
-    A.ev_dot = 0.0;
-    A.p = 0.0;
+    // The sums live in LDS between rounds, which keeps the contact law's
+    // registers free of them.
+#pragma unroll
+    for (int c = 0; c < 9; ++c) s_acc[c][lane] = 0.0;
     uint64_t deal = mask, keep = mask;
+    float gfree = INFINITY;  // smallest rebuild gap among list entries freed here
     for (uint32_t base = 0; base < total; base += FW) {
         // this row's contacts with wave index in [base, base + FW)
         const uint32_t lo = excl > base ? excl : base;
@@ -1329,7 +1580,7 @@ __global__ void __launch_bounds__(FW) k_forces(
             C.F = mk3(0.0, 0.0, 0.0);
             C.T = mk3(0.0, 0.0, 0.0);
             C.p = C.ev_n = C.ev_t = C.es = 0.0;
-            uint8_t rm = 0;
+            uint8_t rm = 0, rmk = 0xffu;
             if (delta_n < 0.0) {
                 const double4 va4 = vel[ao], wa4 = angvel[ao];
                 double loc[3] = {dts[0], dts[1], dts[2]};
@@ -1353,6 +1604,7 @@ __global__ void __launch_bounds__(FW) k_forces(
                     if (kk != 0xffu) {
                         nl_slot[(size_t)kk * np + tile * FW + s_own[lane]] = 0xffu;
                         slot_cand[mempos] = 0xffu;
+                        rmk = kk;
                     }
                 }
             }
@@ -1367,32 +1619,41 @@ __global__ void __launch_bounds__(FW) k_forces(
             s_rec[8][lane] = C.ev_t;
             s_rec[9][lane] = C.es;
             s_rm[lane] = rm;
+            s_rmk[lane] = rmk;
         }
         __syncthreads();
-        for (uint32_t c = lo; c < hi; ++c) {
-            const uint32_t k = c - base;
-            if (s_rm[k]) {
-                keep &= ~(1ull << s_slot[k]);
-            } else {
-                Contrib C;
-                C.F = mk3(s_rec[0][k], s_rec[1][k], s_rec[2][k]);
-                C.T = mk3(s_rec[3][k], s_rec[4][k], s_rec[5][k]);
-                C.p = s_rec[6][k];
-                C.ev_n = s_rec[7][k];
-                C.ev_t = s_rec[8][k];
-                C.es = s_rec[9][k];
-                add_contrib(A, C);
+        if (lo < hi) {
+            Acc A = load_acc(s_acc, lane);
+            for (uint32_t c = lo; c < hi; ++c) {
+                const uint32_t k = c - base;
+                if (s_rm[k]) {
+                    keep &= ~(1ull << s_slot[k]);
+                    if (s_rmk[k] != 0xffu)  // the freed list entry may touch again
+                        gfree = fminf(gfree, nl_gap[(size_t)s_rmk[k] * np + pos]);
+                } else {
+                    Contrib C;
+                    C.F = mk3(s_rec[0][k], s_rec[1][k], s_rec[2][k]);
+                    C.T = mk3(s_rec[3][k], s_rec[4][k], s_rec[5][k]);
+                    C.p = s_rec[6][k];
+                    C.ev_n = s_rec[7][k];
+                    C.ev_t = s_rec[8][k];
+                    C.es = s_rec[9][k];
+                    add_contrib(A, C);
+                }
             }
+            store_acc(s_acc, lane, A);
         }
         __syncthreads();
     }
     if (!valid) return;
+    Acc A = load_acc(s_acc, lane);
     const double4 xa = x[a_orig], va4 = vel[a_orig], wa4 = angvel[a_orig];
     double wf0, wf1, wf2, wf3, wf4;
     walls_contact(a, A, xa, va4, wa4, w_nx, w_mvfd, wf0, wf1, wf2, wf3, wf4);
     write_sums(a, A, a_orig, force, torque, es_dot_out, ev_dot_out, es, ev, p_out,
                walls_force_pp, wf0, wf1, wf2, wf3, wf4);
     if (keep != mask) cmask[a_orig] = keep;
+    if (gfree < INFINITY && nl_gmin[a_orig] > gfree) nl_gmin[a_orig] = gfree;
 }
 
 // Model 1 (contactLinearViscous + capillary cohesion, no contact history):
@@ -1461,19 +1722,19 @@ __global__ void __launch_bounds__(256) k_contact_viscous(
 }
 
 void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist,
-                     bool near)
+                     bool near, uint32_t cur)
 {
     if (a.prm.contactmodel == 1) return;  // model 1 keeps no contact history
     if (nlist && near)
         hipLaunchKernelGGL(k_topology_lists<true>, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s, a,
                            b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
                            b.dmcode, b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
-                           b.nl_pos, b.cellc);
+                           b.nl_pos, b.cellc, b.nl_gap, b.nl_gmin, b.nl_flags, cur);
     else if (nlist)
         hipLaunchKernelGGL(k_topology_lists<false>, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s,
                            a, b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
                            b.dmcode, b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
-                           b.nl_pos, b.cellc);
+                           b.nl_pos, b.cellc, b.nl_gap, b.nl_gmin, b.nl_flags, cur);
     else
         hipLaunchKernelGGL(k_topology_cells, dim3((a.np + 255) / 256), dim3(256), 0, s, a,
                            b.vals, b.cell_start, b.cell_end, b.x_s, b.contacts, b.cmask,
@@ -1494,16 +1755,18 @@ void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool 
     const uint32_t *order = nlist ? nullptr : b.vals;
     uint8_t *nls = nlist ? b.nl_slot : nullptr;
     uint8_t *scd = nlist ? b.slot_cand : nullptr;
+    const float *gap = nlist ? b.nl_gap : nullptr;
+    float *gmin = nlist ? b.nl_gmin : nullptr;
     if (a.prm.contactmodel == 3)
         hipLaunchKernelGGL(k_forces<true>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
                            b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
                            b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask, b.dmcode,
-                           b.delta_t, nls, scd);
+                           b.delta_t, nls, scd, gap, gmin);
     else
         hipLaunchKernelGGL(k_forces<false>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
                            b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
                            b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask, b.dmcode,
-                           b.delta_t, nls, scd);
+                           b.delta_t, nls, scd, gap, gmin);
 }
 
 // ------------------------------------------------------------------
@@ -1602,13 +1865,13 @@ __global__ void __launch_bounds__(256) k_bond_gather(
     torque[pi] = T;
 }
 
-void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b)
+void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool gather)
 {
     if (a.nb0 == 0) return;
     hipLaunchKernelGGL(k_bond_compute, dim3((a.nb0 + 255) / 256), dim3(256), 0, s, a, b.bonds,
                        b.bond_broken, b.bond_delta, b.bond_omega, b.x, b.vel, b.angvel, b.bond_f,
                        b.bond_t, b.bond_applied);
-    if (b.nbp)
+    if (gather && b.nbp)
         hipLaunchKernelGGL(k_bond_gather, dim3((b.nbp + 255) / 256), dim3(256), 0, s, b.nbp,
                            b.bp_list, b.bp_off, b.bp_ent, b.bond_f, b.bond_t, b.bond_applied,
                            b.force, b.torque);
@@ -1703,17 +1966,65 @@ void launch_shear_stress(hipStream_t s, const StepArgs &a, const DemBuffers &b)
                        b.tau_eff);
 }
 
+// Bond contributions of entries [e0, e1) of one particle, in ascending bond
+// order (what k_bond_gather adds).
+__device__ __forceinline__ void bond_sum(const BondGather &bg, uint32_t e0, uint32_t e1,
+                                         double4 &F, double4 &T)
+{
+    for (uint32_t e = e0; e < e1; ++e) {
+        const uint32_t v = bg.ent[e];
+        const uint32_t b = v >> 1;
+        if (!bg.applied[b]) continue;
+        const double4 f = bg.f[b], t = bg.t[b];
+        if ((v & 1u) == 0) {
+            F.x = F.x + f.x; F.y = F.y + f.y; F.z = F.z + f.z;
+            T.x = T.x - t.x; T.y = T.y - t.y; T.z = T.z - t.z;
+        } else {
+            F.x = F.x - f.x; F.y = F.y - f.y; F.z = F.z - f.z;
+            T.x = T.x + t.x; T.y = T.y + t.y; T.z = T.z + t.z;
+        }
+    }
+}
+
+// The next step's cell hash of a new position (k_hash_nl's work done where
+// the position is formed).  Thread 0 clears the current step's rebuild flag,
+// which no later kernel of this step reads.  A grain outside the grid gets
+// the all-ones cell code; the next step's topology reports it.
+__device__ __forceinline__ void next_hash(const StepArgs &a, const NextHash &nh, uint32_t i,
+                                          double4 xi, bool wrapped)
+{
+    const Grid &g = a.grid;
+    if (i == 0) nh.flags[nh.cur ^ 1u] = 0u;
+    const double fx = floor((xi.x - g.origo[0]) / (g.L[0] / g.num[0]));
+    const double fy = floor((xi.y - g.origo[1]) / (g.L[1] / g.num[1]));
+    const double fz = floor((xi.z - g.origo[2]) / (g.L[2] / g.num[2]));
+    uint32_t c = 0, packed = 0xffffffffu;
+    if (fx >= 0.0 && fx < (double)g.num[0] && fy >= 0.0 && fy < (double)g.num[1] && fz >= 0.0 &&
+        fz < (double)g.num[2]) {
+        const uint32_t ix = (uint32_t)fx, iy = (uint32_t)fy, iz = (uint32_t)fz;
+        c = (iz * g.num[1]) * g.num[0] + iy * g.num[0] + ix;
+        packed = ix | (iy << 11) | (iz << 22);
+    }
+    nh.keys[i] = c;
+    nh.vals[i] = i;
+    nh.cellc[i] = packed;
+    const double d2 = disp2_since_rebuild(g, xi, nh.x_ref[i]);
+    bool need = !(d2 <= nh.disp2);
+    if (i == 0 && nh.flags[NLF_STICKY] != 0u) need = true;
+    record_displacement(nh.flags, nh.cur, need, wrapped, d2);
+}
+
 // integrate (TY3), src/integration.cuh:14-305, in original order (the
 // reference reads the *_sorted copies, which hold the same values), with the
 // wall-0 force partial of summation() reduced over the same 256-particle block.
 __global__ void __launch_bounds__(256) k_integrate(
     StepArgs a, long iter, int change_velocity_state, double velocity_factor, double tau_x,
     double4 *__restrict__ x, double4 *__restrict__ vel, double4 *__restrict__ angvel,
-    const double4 *__restrict__ force, const double4 *__restrict__ torque,
+    double4 *__restrict__ force, double4 *__restrict__ torque,
     double4 *__restrict__ angpos, double4 *__restrict__ acc_a, double4 *__restrict__ angacc_a,
     double4 *__restrict__ xyzsum_a, const double4 *__restrict__ w_mvfd,
     const double *__restrict__ tau_eff, const double *__restrict__ walls_force_pp,
-    double *__restrict__ wall_partial)
+    double *__restrict__ wall_partial, BondGather bg, NextHash nh)
 {
     __shared__ double cache[256];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1735,7 +2046,18 @@ __global__ void __launch_bounds__(256) k_integrate(
     }
     double tau_eff_x = 0.0;
     if (a.walls.nw > 0 && wall0mode == 3) tau_eff_x = *tau_eff;
-    const double4 f4 = force[i], t4 = torque[i], ap = angpos[i];
+    double4 f4 = force[i], t4 = torque[i];
+    if (bg.off) {
+        // bond forces in ascending bond order (cohesion.cuh:220-226, D3),
+        // added before integration as the separate gather would
+        const uint32_t e0 = bg.off[i], e1 = bg.off[i + 1];
+        if (e0 < e1) {
+            bond_sum(bg, e0, e1, f4, t4);
+            force[i] = f4;
+            torque[i] = t4;
+        }
+    }
+    const double4 ap = angpos[i];
     const double4 xo = x[i], vo = vel[i], wo = angvel[i];
     const double4 xs = xyzsum_a[i];
     double acc0[3] = {0.0, 0.0, 0.0}, angacc0[3] = {0.0, 0.0, 0.0};
@@ -1788,14 +2110,15 @@ __global__ void __launch_bounds__(256) k_integrate(
         avn[c] = w[c] + angacc[c] * dt + 0.5 * dang * dt * dt;
         xsum[c] += v[c] * dt + 0.5 * acc[c] * dt * dt + 1.0 / 6.0 * dacc * dt * dt * dt;
     }
+    bool wrapped = false;
     if (g.periodic == 1) {
-        if (xn[0] < g.origo[0]) xn[0] += g.L[0];
-        if (xn[0] > g.L[0]) xn[0] -= g.L[0];
-        if (xn[1] < g.origo[1]) xn[1] += g.L[1];
-        if (xn[1] > g.L[1]) xn[1] -= g.L[1];
+        if (xn[0] < g.origo[0]) { xn[0] += g.L[0]; wrapped = true; }
+        if (xn[0] > g.L[0]) { xn[0] -= g.L[0]; wrapped = true; }
+        if (xn[1] < g.origo[1]) { xn[1] += g.L[1]; wrapped = true; }
+        if (xn[1] > g.L[1]) { xn[1] -= g.L[1]; wrapped = true; }
     } else if (g.periodic == 2) {
-        if (xn[0] < g.origo[0]) xn[0] += g.L[0];
-        if (xn[0] > g.L[0]) xn[0] -= g.L[0];
+        if (xn[0] < g.origo[0]) { xn[0] += g.L[0]; wrapped = true; }
+        if (xn[0] > g.L[0]) { xn[0] -= g.L[0]; wrapped = true; }
     }
     if (vo.w > 5.0 && vo.w < 10.0) {
         if (change_velocity_state == 1) vn[0] *= velocity_factor;
@@ -1807,17 +2130,28 @@ __global__ void __launch_bounds__(256) k_integrate(
     angvel[i] = make_double4(avn[0], avn[1], avn[2], wo.w);
     vel[i] = make_double4(vn[0], vn[1], vn[2], vo.w);
     angpos[i] = make_double4(apn[0], apn[1], apn[2], ap.w);
-    x[i] = make_double4(xn[0], xn[1], xn[2], xo.w);
+    const double4 x_new = make_double4(xn[0], xn[1], xn[2], xo.w);
+    x[i] = x_new;
+    if (nh.keys) next_hash(a, nh, i, x_new, wrapped);
 }
 
 void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,
-                      int change_velocity_state, double velocity_factor, double tau_x)
+                      int change_velocity_state, double velocity_factor, double tau_x,
+                      bool gather_bonds, const NextHash &nh)
 {
     uint32_t nb = (a.np + 255) / 256;
+    BondGather bg{};
+    if (gather_bonds && a.nb0 > 0) {
+        bg.off = b.pp_off;
+        bg.ent = b.bp_ent;
+        bg.f = b.bond_f;
+        bg.t = b.bond_t;
+        bg.applied = b.bond_applied;
+    }
     hipLaunchKernelGGL(k_integrate, dim3(nb), dim3(256), 0, s, a, iter, change_velocity_state,
                        velocity_factor, tau_x, b.x, b.vel, b.angvel, b.force, b.torque, b.angpos,
                        b.acc, b.angacc, b.xyzsum, b.w_mvfd, b.tau_eff, b.walls_force_pp,
-                       b.wall_partial);
+                       b.wall_partial, bg, nh);
 }
 
 // integrateWalls, src/integration.cuh:344-450.  One wave: lane 0 forms the

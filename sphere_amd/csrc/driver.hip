@@ -39,6 +39,11 @@ int key_bits_for(uint32_t ncells)
 struct sph_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // integrateWalls runs on `aux`, overlapped with the next step's contact
+    // search (which does not read the walls); the contact forces wait for it
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_integrated = nullptr, ev_walls = nullptr;
+    bool walls_pending = false;
     std::string err;
     bool loaded = false;
     HostState hs;
@@ -55,6 +60,7 @@ struct sph_ctx {
     NLArgs nla{};
     uint32_t nl_parity = 0;
     uint32_t nl_force = 1;
+    bool hash_ready = false;  // the last integrate formed this step's hash
     // time loop host variables, src/device.cu:913-1036
     long iter = 0;
     double filetimeclock = 0.0;
@@ -97,7 +103,7 @@ static int fail(sph_ctx *ctx, int code, const std::string &msg)
 // profiling with HIP events on the context stream
 // ------------------------------------------------------------------
 
-static size_t prof_event(sph_ctx *ctx)
+static size_t prof_event(sph_ctx *ctx, hipStream_t s)
 {
     if (ctx->ev_used == ctx->ev_pool.size()) {
         hipEvent_t e;
@@ -105,22 +111,24 @@ static size_t prof_event(sph_ctx *ctx)
         ctx->ev_pool.push_back(e);
     }
     size_t i = ctx->ev_used++;
-    (void)hipEventRecord(ctx->ev_pool[i], ctx->stream);
+    (void)hipEventRecord(ctx->ev_pool[i], s);
     return i;
 }
 
 struct ProfScope {
     sph_ctx *ctx;
     int k;
+    hipStream_t s;
     size_t e0;
-    ProfScope(sph_ctx *c, int kid) : ctx(c), k(kid), e0((size_t)-1)
+    ProfScope(sph_ctx *c, int kid, hipStream_t st = nullptr)
+        : ctx(c), k(kid), s(st ? st : c->stream), e0((size_t)-1)
     {
-        if (ctx->prof) e0 = prof_event(ctx);
+        if (ctx->prof) e0 = prof_event(ctx, s);
     }
     ~ProfScope()
     {
         if (ctx->prof && e0 != (size_t)-1) {
-            size_t e1 = prof_event(ctx);
+            size_t e1 = prof_event(ctx, s);
             if (e1 != (size_t)-1) ctx->ev_pending.emplace_back(k, e0, e1);
         }
     }
@@ -140,9 +148,23 @@ static void prof_flush(sph_ctx *ctx)
     ctx->ev_used = 0;
 }
 
+// The main stream waits for the pending wall update (work enqueued after
+// this sees the new wall state).
+static int wait_walls(sph_ctx *ctx)
+{
+    if (ctx->walls_pending) {
+        HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_walls, 0));
+        ctx->walls_pending = false;
+    }
+    return SPH_OK;
+}
+
 static int sync(sph_ctx *ctx)
 {
+    int rc = wait_walls(ctx);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->aux));
     HIPCHK(hipGetLastError());
     if (ctx->prof) prof_flush(ctx);
     return SPH_OK;
@@ -183,6 +205,8 @@ static T *dalloc(sph_ctx *ctx, size_t n)
 static void free_device(sph_ctx *ctx)
 {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
+    ctx->walls_pending = false;
     for (void *p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
     ctx->db = DemBuffers{};
@@ -391,6 +415,12 @@ static int upload(sph_ctx *ctx)
         b.nbp = (uint32_t)plist.size();
         AL(b.bp_list, uint32_t, plist.size()); AL(b.bp_off, uint32_t, off.size());
         AL(b.bp_ent, uint32_t, ent.size());
+        // the same entries indexed by every particle (empty ranges for the
+        // unbonded ones), for the gather inside integrate
+        std::vector<uint32_t> pp_off(np + 1, 0);
+        for (size_t i = 0; i < np; ++i) pp_off[i + 1] = pp_off[i] + cnt[i];
+        AL(b.pp_off, uint32_t, np + 1);
+        HIPCHK(hipMemcpyAsync(b.pp_off, pp_off.data(), (np + 1) * 4, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(b.bonds, bnd.data(), nb * 8, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(b.bond_broken, broken.data(), nb, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(b.bond_delta, bd.data(), nb * 32, hipMemcpyHostToDevice, s));
@@ -472,10 +502,12 @@ static int upload(sph_ctx *ctx)
             ctx->nla.range = range;
             AL(b.cellc, uint32_t, np); AL(b.x_ref, double4, np);
             AL(b.nl, uint32_t, (size_t)ctx->nl_cap * np); AL(b.nl_cnt, uint32_t, np);
-            AL(b.nl_pos, uint32_t, np); AL(b.nl_flags, uint32_t, 4);
+            AL(b.nl_pos, uint32_t, np); AL(b.nl_flags, uint32_t, NLF_WORDS);
+            AL(b.nl_gap, float, (size_t)ctx->nl_cap * np);
+            AL(b.nl_gmin, float, np);
             AL(b.nl_slot, uint8_t, (size_t)ctx->nl_cap * np);
             AL(b.slot_cand, uint8_t, np * NCS);
-            launch_fill_u32(s, b.nl_flags, 4, 0u);
+            launch_fill_u32(s, b.nl_flags, NLF_WORDS, 0u);
             launch_fill_u32(s, (uint32_t *)b.slot_cand, (np * NCS + 3) / 4, 0xffffffffu);
             launch_fill_f64(s, (double *)b.x_ref, np * 4, 0.0);
             ctx->nl_on = true;
@@ -483,6 +515,7 @@ static int upload(sph_ctx *ctx)
     }
     ctx->nl_parity = 0;
     ctx->nl_force = 1;
+    ctx->hash_ready = false;
 #undef AL
     HIPCHK(hipStreamSynchronize(s));
     ctx->iter = 0;
@@ -509,8 +542,10 @@ static int download(sph_ctx *ctx, bool particles, bool fluid_fields)
     sph_state &h = hs.h;
     const size_t np = h.np, nb = h.nb0;
     DemBuffers &b = ctx->db;
-    hipStream_t s = ctx->stream;
-    HIPCHK(hipStreamSynchronize(s));
+    {
+        int rc = sync(ctx);
+        if (rc) return rc;
+    }
     {
         double4 wn[MAXW], wm[MAXW];
         HIPCHK(hipMemcpy(wn, b.w_nx, sizeof(wn), hipMemcpyDeviceToHost));
@@ -607,6 +642,16 @@ static StepArgs step_args(const sph_ctx *ctx)
     a.prm.kappa = h.kappa; a.prm.db = h.db; a.prm.V_b = h.V_b;
     a.prm.lambda_bar = h.lambda_bar; a.prm.sigma_b = h.sigma_b; a.prm.tau_b = h.tau_b;
     a.prm.sigma0_A = h.sigma0_A; a.prm.sigma0_f = h.sigma0_f;
+    {
+        // the same IEEE operations as the per-contact form of the law
+        // (volatile keeps the host compiler from reassociating or contracting)
+        volatile double nu = h.k_r, E = h.E;
+        volatile double E_star = E / (2.0 * (1.0 - nu * nu));
+        volatile double G = E / (2.0 * (1.0 + nu));
+        volatile double G_star = G / (2.0 * (2.0 - nu));
+        a.prm.E_star = E_star;
+        a.prm.G_star = G_star;
+    }
     a.walls.nw = h.nw;
     for (int w = 0; w < MAXW; ++w) a.walls.wmode[w] = w < (int)h.nw ? h.wmode[w] : 0;
     a.np = h.np;
@@ -759,6 +804,15 @@ static int output_event(sph_ctx *ctx, const std::string *dir, const std::string 
     return SPH_OK;
 }
 
+// The bond forces are added inside integrate unless something between the
+// bonds and integrate reads or adds to the forces: the Darcy pressure force
+// (added after the bonds, src/device.cu:1970-1990) or the wmode-3 shear
+// stress (reads the forces, src/integration.cuh:457-484).
+static bool gather_in_integrate(const sph_state &h)
+{
+    return !h.fluid && !(h.nw > 0 && h.wmode[0] == 3);
+}
+
 static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *dir,
                      const std::string &sid)
 {
@@ -767,6 +821,8 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
     int rc;
     if (h.np > 0) {
         if (h.adaptive == 1 && h.nw > 0) {  // updateGridSize, device.cu:370-400
+            rc = wait_walls(ctx);
+            if (rc) return rc;
             HIPCHK(hipMemcpyAsync(ctx->h_d, (const char *)ctx->db.w_nx + 24, 8,
                                   hipMemcpyDeviceToHost, s));
             rc = sync(ctx);
@@ -781,12 +837,17 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             const uint32_t cur = ctx->nl_parity;
             const uint32_t *cond = ctx->db.nl_flags + cur;
             const uint32_t force = ctx->nl_force || h.fluid ? 1u : 0u;
-            { ProfScope ps(ctx, SPH_K_HASH); launch_hash_nl(s, a, ctx->nla, ctx->db, cur, force); }
+            if (force || !ctx->hash_ready) {  // else the last integrate formed the hash
+                ProfScope ps(ctx, SPH_K_HASH);
+                launch_hash_nl(s, a, ctx->nla, ctx->db, cur, force);
+            }
             { ProfScope ps(ctx, SPH_K_SORT); launch_sort(s, a.np, ctx->key_bits, ctx->db, cond); }
             { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells, cond); }
             { ProfScope ps(ctx, SPH_K_NLIST); launch_nl_build(s, a, ctx->nla, ctx->db, cond); }
             { ProfScope ps(ctx, SPH_K_TOPOLOGY);
-              launch_topology(s, a, ctx->db, true, ctx->nla.range == 1); }
+              launch_topology(s, a, ctx->db, true, ctx->nla.range == 1, cur); }
+            rc = wait_walls(ctx);
+            if (rc) return rc;
             { ProfScope ps(ctx, SPH_K_CONTACT); launch_contact(s, a, ctx->db, true); }
             ctx->nl_parity ^= 1u;
             ctx->nl_force = 0;
@@ -796,11 +857,16 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells, nullptr); }
             if (h.contactmodel != 1) {
                 ProfScope ps(ctx, SPH_K_TOPOLOGY);
-                launch_topology(s, a, ctx->db, false, false);
+                launch_topology(s, a, ctx->db, false, false, 0u);
             }
+            rc = wait_walls(ctx);
+            if (rc) return rc;
             { ProfScope ps(ctx, SPH_K_CONTACT); launch_contact(s, a, ctx->db, false); }
         }
-        if (h.nb0 > 0) { ProfScope ps(ctx, SPH_K_BONDS); launch_bonds(s, a, ctx->db); }
+        if (h.nb0 > 0) {
+            ProfScope ps(ctx, SPH_K_BONDS);
+            launch_bonds(s, a, ctx->db, !gather_in_integrate(h));
+        }
     }
     if (h.fluid) {
         rc = darcy_step(ctx, convlog);
@@ -817,11 +883,32 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             change_velocity_state = -1;
             ctx->velocity_state = 1;
         }
+        // Without fluid the next step's cell hash is formed by integrate (list
+        // mode); the fluid step rebuilds and hashes every step.
+        NextHash nh{};
+        const bool fuse_hash = ctx->nl_on && !h.fluid;
+        if (fuse_hash) {
+            nh.keys = ctx->db.keys;
+            nh.vals = ctx->db.vals;
+            nh.cellc = ctx->db.cellc;
+            nh.x_ref = ctx->db.x_ref;
+            nh.flags = ctx->db.nl_flags;
+            nh.cur = ctx->nl_parity;  // already the next step's word
+            nh.disp2 = ctx->nla.disp2;
+        }
         { ProfScope ps(ctx, SPH_K_INTEGRATE);
-          launch_integrate(s, a, ctx->db, ctx->iter, change_velocity_state, 10.0, h.tau_x); }
+          launch_integrate(s, a, ctx->db, ctx->iter, change_velocity_state, 10.0, h.tau_x,
+                           gather_in_integrate(h), nh); }
+        ctx->hash_ready = fuse_hash;
         if (h.nw > 0) {
-            ProfScope ps(ctx, SPH_K_WALLS);
-            launch_integrate_walls(s, a, ctx->db, ctx->iter, h.time_current);
+            HIPCHK(hipEventRecord(ctx->ev_integrated, s));
+            HIPCHK(hipStreamWaitEvent(ctx->aux, ctx->ev_integrated, 0));
+            {
+                ProfScope ps(ctx, SPH_K_WALLS, ctx->aux);
+                launch_integrate_walls(ctx->aux, a, ctx->db, ctx->iter, h.time_current);
+            }
+            HIPCHK(hipEventRecord(ctx->ev_walls, ctx->aux));
+            ctx->walls_pending = true;
         }
     }
     h.time_current += h.time_dt;
@@ -859,6 +946,9 @@ sph_ctx *sph_create(int device)
     sph_ctx *ctx = new sph_ctx();
     ctx->device = device;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_integrated, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_walls, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&ctx->h_err, 64, hipHostMallocDefault) != hipSuccess) {
         g_last_error = "stream or pinned allocation failed";
         delete ctx;
@@ -875,7 +965,10 @@ void sph_destroy(sph_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     free_device(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->ev_integrated) (void)hipEventDestroy(ctx->ev_integrated);
+    if (ctx->ev_walls) (void)hipEventDestroy(ctx->ev_walls);
     if (ctx->h_err) (void)hipHostFree(ctx->h_err);
+    if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -947,7 +1040,7 @@ int sph_get_neighbor_stats(sph_ctx *ctx, sph_neighbor_stats *out)
     *out = sph_neighbor_stats{};
     if (!ctx->loaded || !ctx->nl_on) return SPH_OK;
     (void)hipSetDevice(ctx->device);
-    uint32_t f[4];
+    uint32_t f[NLF_DMAX];
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(f, ctx->db.nl_flags, sizeof(f), hipMemcpyDeviceToHost));
     if (f[3] > 0) {  // lists exist
@@ -968,9 +1061,10 @@ int sph_get_neighbor_stats(sph_ctx *ctx, sph_neighbor_stats *out)
     out->enabled = 1;
     out->range = ctx->nla.range;
     out->cap = ctx->nla.cap;
-    out->overflow = f[2] ? 1u : 0u;
+    out->overflow = f[NLF_STICKY] ? 1u : 0u;
     out->skin = ctx->nla.skin;
-    out->rebuilds = f[3];
+    out->rebuilds = f[NLF_COUNT];
+    out->full_steps = f[NLF_FULL];
     return SPH_OK;
 }
 

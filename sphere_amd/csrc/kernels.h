@@ -31,6 +31,7 @@ struct DemBuffers {
     double4 *bond_f, *bond_t;
     uint8_t *bond_applied;
     uint32_t *bp_list, *bp_off, *bp_ent;  // bonded particles, CSR offsets, entries
+    uint32_t *pp_off;                     // np+1: CSR offsets into bp_ent per particle
     uint32_t nbp;
     uint32_t *errflags;
     // neighbour lists
@@ -41,7 +42,25 @@ struct DemBuffers {
     uint32_t *nl_pos;    // np, position in the cell sort of the last rebuild
     uint8_t *nl_slot;    // cap*np like nl: slot of each list entry in the row (0xff: none)
     uint8_t *slot_cand;  // np*nc like contacts: list entry of each slot (0xff: none)
-    uint32_t *nl_flags;  // [0],[1] rebuild flag by step parity, [2] sticky overflow, [3] rebuilds
+    uint32_t *nl_flags;  // NLF_WORDS words, see common.h
+    float *nl_gap;       // cap*np like nl: surface gap at the rebuild, rounded down
+    float *nl_gmin;      // np: lower bound of nl_gap over the entries not in the row
+};
+
+// Bond contributions gathered inside integrate (off == nullptr: none).
+struct BondGather {
+    const uint32_t *off, *ent;  // per-particle CSR of (bond << 1 | side)
+    const double4 *f, *t;
+    const uint8_t *applied;
+};
+
+// The next step's cell hash, formed inside integrate (keys == nullptr: none).
+struct NextHash {
+    uint32_t *keys, *vals, *cellc;
+    const double4 *x_ref;
+    uint32_t *flags;
+    uint32_t cur;   // rebuild-flag word of the next step
+    double disp2;
 };
 
 void launch_hash(hipStream_t s, const StepArgs &a, const DemBuffers &b);
@@ -58,15 +77,17 @@ void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const De
 void launch_nl_stats(hipStream_t s, const DemBuffers &b, uint32_t np, uint32_t cap,
                      unsigned long long *out);
 // Contact search (models 2, 3): new overlaps take slots (topology).  near:
-// list mode on a grid searched one cell deep.
+// list mode on a grid searched one cell deep; cur: the step's flag word.
 void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist,
-                     bool near);
+                     bool near, uint32_t cur);
 // Contact forces, walls and per-particle sums (interact).
 void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist);
-void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b);
+// gather: add the bond forces to force/torque here (else integrate does it)
+void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool gather);
 void launch_shear_stress(hipStream_t s, const StepArgs &a, const DemBuffers &b);
 void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,
-                      int change_velocity_state, double velocity_factor, double tau_x);
+                      int change_velocity_state, double velocity_factor, double tau_x,
+                      bool gather_bonds, const NextHash &nh);
 void launch_integrate_walls(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,
                             double t_current);
 void launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);

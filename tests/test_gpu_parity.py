@@ -132,6 +132,18 @@ def test_dense_hertz_mindlin(any_engine):
     assert nc > 100
 
 
+def test_dense_hertz_rolling_cohesion(engine):
+    """Rolling resistance and linear cohesion on (the laws' general branches;
+    with gamma_r = mu_r = kappa = 0 the library uses closed forms of the zero
+    terms)."""
+    s = dense_case(n=729, seed=23, model=3, periodic=1, E=7e9, nu=0.3)
+    s.gamma_r[0] = 1e-3
+    s.mu_r[0] = 0.1
+    s.kappa[0] = 10.0
+    _, nc = run_pair(engine, s, 3, 10, label="hertz rolling")
+    assert nc > 100
+
+
 def test_lists_enabled_on_test_grids(engine):
     """Cells narrower than a diameter plus the skin (the dense and settling
     grids) are searched two cells deep at a rebuild, wider ones one deep."""
@@ -341,6 +353,29 @@ def test_shear_fixvel(engine):
     s.vel[top, 0] = 0.01
     s.vel[bot, :2] = 0.0
     run_pair(engine, s, 2, 10, label="shear")
+
+
+@pytest.mark.parametrize("mode", ["lists", "cells"])
+def test_shear_stress_wall_with_bonds(engine, engine_cells, mode):
+    """wmode 3 (stress-driven top layer, src/integration.cuh:116-129,
+    457-484) with bonds: the shear-stress sum reads the forces after the bond
+    forces, so bonds are gathered before it."""
+    e = engine if mode == "lists" else engine_cells
+    s = dense_case(n=729, seed=81, model=2, periodic=1, nw=1, vmag=0.0, wmag=0.0,
+                   spacing_factor=0.99)
+    add_bonds(s, 0.2, seed=3)
+    s.shear(shear_stress=2e3)
+    assert s.wmode[0] == 3 and s.nb0[0] > 20
+    run_pair(e, s, 2, 10, label="wmode 3")
+
+
+def test_darcy_with_bonds(engine):
+    """Bonds under fluid coupling: the pressure force is added after the bond
+    forces (src/device.cu:1970-1990)."""
+    s = darcy_case(n=300, seed=9, nxyz=(6, 6, 12), tolerance=1e-8)
+    add_bonds(s, 1.0, seed=4)
+    assert s.nb0[0] > 5
+    run_pair(engine, s, 2, 3, fluid=True, label="darcy bonds")
 
 
 def test_darcy_coupled(engine):
