@@ -812,7 +812,7 @@ __device__ __forceinline__ void law_linear(const StepArgs &a, Contrib &C, d3 vel
     d3 vel_t = vel - dot3(n, vel) * n;
     double vel_t_length = len3(vel_t);
     d3 delta_t0 = delta_t0_uncor - dot3(n, delta_t0_uncor) * n;
-    double delta_t0_length = len3(delta_t0);
+    const double delta_t0_sq = dot3(delta_t0, delta_t0);  // only tested > 0
     d3 delta_t = delta_t0;
     double k_n = P.k_n, k_t = P.k_t;
     if (P.E > .001) {
@@ -824,18 +824,20 @@ __device__ __forceinline__ void law_linear(const StepArgs &a, Contrib &C, d3 vel
     C.ev_n = 0.5 * P.gamma_n * vel_n * vel_n;
     d3 f_c = cohesion_factor(P.kappa, r_a, r_b) * n;
     d3 f_t = mk3(0.0, 0.0, 0.0);
-    if (delta_t0_length > 0.0 || vel_t_length > 0.0) {
+    if (delta_t0_sq > 0.0 || vel_t_length > 0.0) {
         delta_t = delta_t0 + dt * vel_t;
         d3 f_t_elast = (-k_t) * delta_t;
         d3 f_t_visc = (-P.gamma_t) * vel_t;
         f_t = f_t_elast + f_t_visc;
         double f_t_length = len3(f_t);
-        double f_t_limit = P.mu_s * len3(f_n - f_c);
+        const bool no_cohesion = __double_as_longlong(P.kappa) == 0ll;
+        const double fnc = no_cohesion ? f_n_length : len3(f_n - f_c);
+        double f_t_limit = P.mu_s * fnc;
         C.ev_t = 0.5 * P.gamma_t * vel_t_length * vel_t_length;
         if (f_t_length > f_t_limit) {
             d3 t = f_t / len3(f_t);
             f_t = f_t_limit * t;
-            delta_t = (-1.0 / k_t) * ((P.mu_d * len3(f_n - f_c)) * t + P.gamma_t * vel_t);
+            delta_t = (-1.0 / k_t) * ((P.mu_d * fnc) * t + P.gamma_t * vel_t);
             C.es = 0.5 * len3(dt * (len3(f_t) * vel_t)) / dt;
         }
     }
@@ -870,7 +872,8 @@ __device__ __forceinline__ void law_hertz(const StepArgs &a, Contrib &C, d3 vel_
     d3 vel_t_ab = vel_ab - dot3(vel_ab, n_ab) * n_ab;
     double vel_t_ab_length = len3(vel_t_ab);
     d3 delta_t0 = delta_t0_uncor - dot3(delta_t0_uncor, n_ab) * n_ab;
-    double delta_t0_length = len3(delta_t0);
+    // |delta_t0| only enters as |delta_t0| > 0, which equals dot > 0
+    const double delta_t0_sq = dot3(delta_t0, delta_t0);
     d3 delta_t = delta_t0;
     double d = -delta_ab;
     double sd = sqrt(d);
@@ -890,14 +893,16 @@ __device__ __forceinline__ void law_hertz(const StepArgs &a, Contrib &C, d3 vel_
     d3 f_c = cohesion_factor(P.kappa, r_a, r_b) * n_ab;
     d3 f_t = mk3(0.0, 0.0, 0.0);
     d3 T_res = mk3(0.0, 0.0, 0.0);
-    if (delta_t0_length > 0.0 || vel_t_ab_length > 0.0) {
+    if (delta_t0_sq > 0.0 || vel_t_ab_length > 0.0) {
         d3 f_t_elast = (-k_t * sd) * delta_t0;
         d3 f_t_visc = (-P.gamma_t * qd) * vel_t_ab;
+        // with kappa = +0, f_c is a vector of zeros and |f_n - f_c| = |f_n|
+        const double fnc = __double_as_longlong(P.kappa) == 0ll ? f_n_length : len3(f_n - f_c);
         double f_t_limit;
         if (vel_t_ab_length > (double)0.001f)
-            f_t_limit = P.mu_d * len3(f_n - f_c);
+            f_t_limit = P.mu_d * fnc;
         else
-            f_t_limit = P.mu_s * len3(f_n - f_c);
+            f_t_limit = P.mu_s * fnc;
         f_t = f_t_elast + f_t_visc;
         double f_t_length = len3(f_t);
         C.ev_t = 0.5 * P.gamma_t * qd * vel_t_ab_length * vel_t_ab_length;
