@@ -1,13 +1,20 @@
-"""Benchmark: particle-steps/s of the FP64 DEM step on the dense periodic
-shear box (1,000,000 grains, Hertz-Mindlin, 5% bonded pairs) on one MI355X.
+"""Benchmark on one MI355X, printed as one JSON line.
+
+1. particle-steps/s of the FP64 DEM step on the dense periodic shear box
+   (1,000,000 grains, Hertz-Mindlin, 5% bonded pairs): the line's value.
+2. Darcy ms-to-1e-8 on the consolidation box (200,000 grains, 48 x 48 x 96
+   Darcy grid, 20 kPa load): the "darcy" object of the line.
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 200] [--n 1000000]
+                    [--darcy-steps 500] [--no-darcy]
 
-Untimed preparation: a dense packing grown on the GPU (radius growth, then
-quenched relaxation under gravity), the 10 kPa top wall, the driven top
-layer and the bonds.  Timed: exactly --steps steps after --warmup steps,
-with every input resident in HBM; per-kernel device time from HIP events on
-the library's stream.  Printed: one JSON line.
+Untimed preparation: dense packings grown on the GPU (radius growth, then
+quenched relaxation under gravity), then the run configuration (walls,
+driven layer, bonds, fluid).  Timed: exactly --steps steps after --warmup
+steps, with every input resident in HBM; per-kernel device time from HIP
+events on the library's stream in a separate run.  The Darcy solve time of
+each fluid step runs from the first Jacobi sweep to convergence (HIP events
+around the sweep loop).
 """
 import argparse
 import json
@@ -77,6 +84,111 @@ def prepare_shear_box(engine, n, seed, stage_steps, z_target, relax_rounds, rela
     return s
 
 
+def prepare_consolidation(engine, n, seed, stage_steps, relax_rounds, relax_steps):
+    from sphere_amd import workloads as W
+    s, f0, rng = W.consolidation_base(n=n, seed=seed)
+    st = W.EngineStepper(engine)
+    t0 = time.time()
+    f, z = W.grow_to_jamming(s, st, f0, z_target=4.2, steps=stage_steps, fine_df=0.005)
+    log("consolidation box: growth to jamming done in %.1f s: f=%.4f, Z=%.2f" %
+        (time.time() - t0, f, z))
+    s.w_x[0] = s.L[2]
+    s.g[2] = -9.81
+    t0 = time.time()
+    W.relax_packing(s, st, relax_rounds, relax_steps)
+    log("consolidation box: relaxation done in %.1f s" % (time.time() - t0))
+    W.consolidation_finish(s)
+    log("consolidation box: %d grains, grid %s, box %s, wall at %.4f m" %
+        (n, [int(v) for v in s.num], [round(float(v), 4) for v in s.L], float(s.w_x[0])))
+    return s
+
+
+def percentiles(v):
+    v = numpy.asarray(v, dtype=numpy.float64)
+    return {"mean": float(v.mean()), "p50": float(numpy.percentile(v, 50)),
+            "p95": float(numpy.percentile(v, 95)), "min": float(v.min()), "max": float(v.max())}
+
+
+def run_darcy(engine, s, warmup, steps):
+    """Fluid steps one at a time; per step the Jacobi solve time (first sweep
+    to convergence, device clock) and its sweep count."""
+    import sphere_amd
+    engine.load(s.to_state())
+    for _ in range(warmup):
+        engine.step(1)
+    ms = []
+    sweeps = []
+    prev = engine.darcy_stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        engine.step(1)
+        cur = engine.darcy_stats()
+        ms.append(cur.solve_ms - prev.solve_ms)
+        sweeps.append(cur.sweeps - prev.sweeps)
+        prev = cur
+    elapsed = time.perf_counter() - t0
+    return numpy.array(ms), numpy.array(sweeps), elapsed, prev
+
+
+def darcy_packing(args, engine):
+    import sphere_amd
+    if args.darcy_load:
+        s = sphere_amd.sim(fluid=True)
+        s.readbin(args.darcy_load, verbose=False)
+        s.fluid = True
+        log("loaded the consolidation box from %s" % args.darcy_load)
+    else:
+        log("preparing the consolidation box (%d grains)" % args.darcy_n)
+        s = prepare_consolidation(engine, args.darcy_n, args.darcy_seed, 300, 3, 400)
+    if args.darcy_save:
+        os.makedirs(args.darcy_save, exist_ok=True)
+        s.writebin(folder=args.darcy_save, verbose=False)
+        log("wrote %s/%s.bin" % (args.darcy_save, s.sid))
+    return s
+
+
+def darcy_section(args):
+    import sphere_amd
+    engine = sphere_amd.Engine(0, nc=32)
+    s = darcy_packing(args, engine)
+    n = int(s.np[0])
+    log("consolidation box: %d warm-up + %d timed fluid steps" % (args.darcy_warmup,
+                                                                   args.darcy_steps))
+    ms, sweeps, elapsed, st = run_darcy(engine, s, args.darcy_warmup, args.darcy_steps)
+    cells = int(s.num[0]) * int(s.num[1]) * int(s.num[2])
+    sweep_bytes = 40.0 * cells
+    per_sweep_s = ms.sum() / 1000.0 / max(1, sweeps.sum())
+    hist_edges = [0, 1, 11, 21, 51, 101, 201, 501, 1001, 2001, 5001, 10**9]
+    hist = numpy.histogram(sweeps, bins=hist_edges)[0]
+    log("Darcy: %.3f ms per solve (p50 %.3f, p95 %.3f), %.1f sweeps per solve, %.2f us per "
+        "sweep, %.1f ms per step overall" % (ms.mean(), numpy.percentile(ms, 50),
+                                             numpy.percentile(ms, 95), sweeps.mean(),
+                                             1e6 * per_sweep_s, 1000.0 * elapsed / len(ms)))
+    out = {
+        "metric": "Darcy ms-to-1e-8 (Jacobi solve per fluid step, first sweep to convergence)",
+        "plain": {"ms": percentiles(ms), "sweeps": percentiles(sweeps),
+                  "sweeps_histogram": {"edges": hist_edges[:-1], "counts": [int(c) for c in hist]},
+                  "nonconverged": int(st.nonconverged)},
+        "particle_steps_per_s": n * len(ms) / elapsed,
+        "ms_per_step": 1000.0 * elapsed / len(ms),
+        "steps": len(ms), "warmup": args.darcy_warmup,
+        "config": {"workload": "Darcy-coupled consolidation: %d grains, radii U[0.009, 0.011] m, "
+                               "20 kPa top wall, linear contacts k_n=k_t=1.16e9 mu=0.5, Darcy grid "
+                               "%dx%dx%d, k_c=3.5e-13, mu_f=1.04e-7, beta_f=1.43e-8, p=0 top, "
+                               "no flow at the base, periodic x/y, tolerance 1e-8"
+                               % (n, s.num[0], s.num[1], s.num[2]),
+                   "grains": n, "grid": [int(v) for v in s.num], "dt": float(s.time_dt[0])},
+        "sweep_roofline": {"bound": "hbm", "achieved": sweep_bytes / per_sweep_s / 1e9,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": sweep_bytes / per_sweep_s / 1e9 / HBM_PEAK_GBS,
+                           "bytes_per_sweep": sweep_bytes, "us_per_sweep": 1e6 * per_sweep_s,
+                           "note": "the sweep's fields (%.1f MB) stay in the Infinity Cache"
+                                   % (sweep_bytes / 1e6)},
+    }
+    engine.close()
+    return out
+
+
 def cpu_baseline(state_buf, n, warm, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from _oracle import Oracle
@@ -111,6 +223,15 @@ def main():
     ap.add_argument("--load-packing", default="",
                     help="start from a shear box written by --save-packing")
     ap.add_argument("--prep-only", action="store_true", help="stop after preparing")
+    ap.add_argument("--no-darcy", action="store_true", help="skip the consolidation box")
+    ap.add_argument("--no-shear", action="store_true",
+                    help="skip the shear box (the line then carries only the Darcy object)")
+    ap.add_argument("--darcy-n", type=int, default=200_000)
+    ap.add_argument("--darcy-seed", type=int, default=3)
+    ap.add_argument("--darcy-steps", type=int, default=500)
+    ap.add_argument("--darcy-warmup", type=int, default=20)
+    ap.add_argument("--darcy-save", default="", help="write the prepared consolidation box here")
+    ap.add_argument("--darcy-load", default="", help="start from a box written by --darcy-save")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="finish with three device-to-device copies of MIB MiB, the "
                          "known-size kernel that scales HBM counters")
@@ -122,6 +243,15 @@ def main():
         sys.exit(2)
 
     import sphere_amd
+    if args.no_shear:
+        if args.prep_only:
+            darcy_packing(args, sphere_amd.Engine(0, nc=32))
+            return
+        d = darcy_section(args)
+        print(json.dumps({"metric": d["metric"], "value": d["plain"]["ms"]["mean"], "unit": "ms",
+                          "n_gpus": 1, "higher_is_better": False, "dtype": "f64",
+                          "data": "synthetic", "darcy": d}), flush=True)
+        return
     # the largest lognormal grains touch more than 32 others: 64 contact slots
     engine = sphere_amd.Engine(0, nc=64)
     run_engine = engine if args.skin is None else sphere_amd.Engine(0, nc=64, skin=args.skin)
@@ -140,6 +270,8 @@ def main():
         s.writebin(folder=args.save_packing, verbose=False)
         log("wrote %s/%s.bin" % (args.save_packing, s.sid))
     if args.prep_only:
+        if not args.no_darcy:
+            darcy_packing(args, sphere_amd.Engine(0, nc=32))
         return
     state = s.to_state()
     run_engine.load(state)
@@ -197,7 +329,7 @@ def main():
                          "steps of the FP64 C oracle (OpenMP, %.1f s)" % (args.cpu_steps, cdt)}
 
     line = {
-        "metric": "particle-steps/s (1M grains, FP64)",
+        "metric": "particle-steps/s (1M grains, FP64); Darcy ms-to-1e-8, CNN-precond vs plain",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -224,6 +356,7 @@ def main():
         "kernels": ktimes,
         "cpu_baseline": cpu,
     }
+    line["darcy"] = None if args.no_darcy else darcy_section(args)
     print(json.dumps(line), flush=True)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

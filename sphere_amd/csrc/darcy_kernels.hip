@@ -459,53 +459,213 @@ __device__ __forceinline__ double darcy_dp(const DarcyArgs &d, int x, int y, int
     return dp;
 }
 
-__global__ void __launch_bounds__(256) k_darcy_first(DarcyArgs d, const double *__restrict__ p,
-                                                     const double *__restrict__ k,
-                                                     const double *__restrict__ phi,
-                                                     const double *__restrict__ dphi,
-                                                     const double *__restrict__ vp,
-                                                     const int *__restrict__ pconst,
-                                                     double *__restrict__ dp_expl)
+// Layout of DarcyBuffers::jc: face conductances on the x, y and z faces
+// ((nx+1) ny nz, nx (ny+1) nz and nx ny (nz+1) values, face f of a row lying
+// between cells f-1 and f, the ghost cells at -1 and n), then the per-cell
+// A, F and base planes (nx ny nz each, x fastest).
+struct JLayout {
+    size_t hy, hz, A, F, base;
+    uint32_t sxf, syf;  // row strides of the x- and y-face planes
+};
+
+__device__ __forceinline__ JLayout jlayout(const DarcyArgs &d)
+{
+    JLayout j;
+    const size_t n = (size_t)d.nx * d.ny * d.nz;
+    j.hy = (size_t)(d.nx + 1) * d.ny * d.nz;
+    j.hz = j.hy + (size_t)d.nx * (d.ny + 1) * d.nz;
+    j.A = j.hz + (size_t)d.nx * d.ny * (d.nz + 1);
+    j.F = j.A + n;
+    j.base = j.F + n;
+    j.sxf = d.nx + 1;
+    j.syf = d.ny + 1;
+    return j;
+}
+
+// firstDarcySolution (src/darcy.cuh:1425-1598) plus the constants of the
+// solve's sweeps.  Within one solve k, phi, dphi, vp_avg, p_old and dp_expl
+// are fixed, so each sweep of updateDarcySolution only needs the iterate
+// p.  The sweep's face factor 2 k_nb k / (k_nb + k) is the same number seen
+// from either cell of a face (2x is exact, and * and + commute in IEEE
+// arithmetic), so it is stored once per face.  With
+//   A = ndt / (beta_f phi mu),
+//   F = ndt / (beta_f phi (1 - phi)) (dphi / ndt + vp_avg . grad phi),
+//   base = p_old + (1 - epsilon) dp_expl,
+// the sweep's dp_impl = A div - F and p* = base + epsilon dp_impl are the
+// reference's expressions, operation for operation.
+__global__ void __launch_bounds__(256) k_darcy_prepare(DarcyArgs d, const double *__restrict__ p_a,
+                                                       const double *__restrict__ k_a,
+                                                       const double *__restrict__ phi_a,
+                                                       const double *__restrict__ dphi_a,
+                                                       const double *__restrict__ vp_a,
+                                                       const int *__restrict__ pconst,
+                                                       const double *__restrict__ p_old,
+                                                       double *__restrict__ dp_expl,
+                                                       double *__restrict__ jc,
+                                                       uint8_t *__restrict__ frozen)
 {
     int x, y, z;
     if (!cell_xyz(d, x, y, z)) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = (int)d.nx, ny = (int)d.ny, nz = (int)d.nz;
+    const double dx = d.dx, dy = d.dy, dz = d.dz;
+    const JLayout j = jlayout(d);
     double pc;
-    dp_expl[didx(d, x, y, z)] = darcy_dp(d, x, y, z, p, k, phi, dphi, vp, pconst, pc);
+    const double dpe = darcy_dp(d, x, y, z, p_a, k_a, phi_a, dphi_a, vp_a, pconst, pc);
+    const uint32_t c = didx(d, x, y, z);
+    dp_expl[c] = dpe;
+    const uint32_t cxn = didx(d, x - 1, y, z), cxp = didx(d, x + 1, y, z);
+    const uint32_t cyn = didx(d, x, y - 1, z), cyp = didx(d, x, y + 1, z);
+    const uint32_t czn = didx(d, x, y, z - 1), czp = didx(d, x, y, z + 1);
+    const double k = k_a[c];
+    const size_t fx = (size_t)x + j.sxf * ((size_t)y + d.ny * (size_t)z);
+    const size_t fy = (size_t)x + d.nx * ((size_t)y + j.syf * (size_t)z);
+    jc[fx] = 2. * k_a[cxn] * k / (k_a[cxn] + k);
+    if (x == nx - 1) jc[fx + 1] = 2. * k_a[cxp] * k / (k_a[cxp] + k);
+    jc[j.hy + fy] = 2. * k_a[cyn] * k / (k_a[cyn] + k);
+    if (y == ny - 1) jc[j.hy + fy + d.nx] = 2. * k_a[cyp] * k / (k_a[cyp] + k);
+    jc[j.hz + i] = 2. * k_a[czn] * k / (k_a[czn] + k);
+    if (z == nz - 1) jc[j.hz + i + (size_t)d.nx * d.ny] = 2. * k_a[czp] * k / (k_a[czp] + k);
+    const double phi = phi_a[c];
+    const d3 grad_phi = mk3((phi_a[cxp] - phi_a[cxn]) / (dx + dx), (phi_a[cyp] - phi_a[cyn]) / (dy + dy),
+                            (phi_a[czp] - phi_a[czn]) / (dz + dz));
+    const d3 vp_avg = mk3(vp_a[3 * (size_t)c], vp_a[3 * (size_t)c + 1], vp_a[3 * (size_t)c + 2]);
+    const double ndt = d.ndem * d.dt;
+    jc[j.A + i] = ndt / (d.beta_f * phi * d.mu);
+    jc[j.F + i] = (ndt / (d.beta_f * phi * (1.0 - phi))) * (dphi_a[c] / ndt + dot3(vp_avg, grad_phi));
+    const double epsilon = 0.5;
+    jc[j.base + i] = p_old[c] + (1.0 - epsilon) * dpe;
+    frozen[i] = ((d.bc_bot == 0 && z == 0) || (d.bc_top == 0 && z == nz - 1) ||
+                 ((uint32_t)z >= d.wall0_iz && d.bc_top == 0) || (d.bc_xn == 0 && x == 0) ||
+                 (d.bc_xp == 0 && x == nx - 1) || (d.bc_yn == 0 && y == 0) ||
+                 (d.bc_yp == 0 && y == nx - 1) || pconst[c] == 1) ? 1 : 0;
 }
 
-void launch_darcy_first(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b)
+void launch_darcy_prepare(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b)
 {
-    hipLaunchKernelGGL(k_darcy_first, cell_grid(d), dim3(256), 0, s, d, b.p, b.k, b.phi, b.dphi,
-                       b.vp_avg, b.p_constant, b.dp_expl);
+    hipLaunchKernelGGL(k_darcy_prepare, cell_grid(d), dim3(256), 0, s, d, b.p, b.k, b.phi, b.dphi,
+                       b.vp_avg, b.p_constant, b.p_old, b.dp_expl, b.jc, b.jfrozen);
 }
 
-// updateDarcySolution, src/darcy.cuh:1602-1799, with the max |norm| of
-// maxNormResDarcy (src/darcy.cpp:266-292) reduced per block and folded with
-// an atomic max on the IEEE bit pattern (monotone for non-negative doubles;
-// NaN patterns sort above +inf, so a NaN residual survives the max).
-__global__ void __launch_bounds__(256) k_darcy_update(
-    DarcyArgs d, const double *__restrict__ p_old, const double *__restrict__ dp_expl,
-    const double *__restrict__ p_in, const double *__restrict__ k, const double *__restrict__ phi,
-    const double *__restrict__ dphi, const double *__restrict__ vp, const int *__restrict__ pconst,
-    double *__restrict__ p_out, double *__restrict__ norm, unsigned long long *__restrict__ maxres)
+// The convergence test of the Jacobi loop (src/device.cu:2342-2386,
+// maxNormResDarcy src/darcy.cpp:266-292) for check sweep nijac: a NaN max
+// ends the solve with an error, max <= tolerance ends it converged.
+// state: [0] 0 running, 1 converged, 2 NaN; [1] nijac of the last check;
+// [2] bit pattern of its max |norm|.  Returns whether the solve ends.
+__device__ __forceinline__ bool darcy_decide(const unsigned long long *__restrict__ slot,
+                                             double tolerance, uint32_t nijac,
+                                             unsigned long long *__restrict__ state, bool writer)
+{
+    const unsigned long long bits = *slot;
+    const double m = __longlong_as_double((long long)bits);
+    const bool nan = m != m;
+    const bool stop = nan || m <= tolerance;
+    if (writer) {
+        state[1] = nijac;
+        state[2] = bits;
+        if (stop) state[0] = nan ? 2ull : 1ull;
+    }
+    return stop;
+}
+
+// One sweep of the Jacobi loop body (src/device.cu:2184-2331) in a single
+// launch: updateDarcySolution on iterate p_in into p_out, then, into p_out,
+// the ghost values that the next sweep's setDarcyGhostNodes would copy from
+// p_out's interior (src/darcy.cuh:194-264) and the flux ghosts that
+// setDarcyGhostNodesFlux computes from p_in's interior
+// (src/darcy.cuh:270-312; the reference writes them after the update and
+// before copying p_new into p, so they lag one sweep).  Ping-ponging p_in
+// and p_out therefore leaves exactly the state the reference's
+// ghosts / update / flux ghosts / copy sequence leaves.  The residual
+// field itself is not stored, only its max |.| for the convergence check
+// (check != null): a per-block max of the bit pattern of |norm| (monotone
+// for non-negative doubles; NaN patterns sort above +inf, so a NaN residual
+// survives the max) folded with one atomic max per block; a check sweep
+// also zeroes the slot of the next check (clear).  The sweep after a check
+// sweep first applies that check's test (prev != null; every block reaches
+// the same verdict from the same slot) and returns if the solve has ended;
+// once state[0] is set, every later sweep returns at once, so the host may
+// enqueue sweeps beyond the convergence point.
+__global__ void __launch_bounds__(256) k_darcy_sweep(DarcyArgs d, const double *__restrict__ jc,
+                                                     const uint8_t *__restrict__ frozen,
+                                                     const double *__restrict__ p_in,
+                                                     const double *__restrict__ k_a,
+                                                     double *__restrict__ p_out,
+                                                     unsigned long long *__restrict__ check,
+                                                     unsigned long long *__restrict__ clear,
+                                                     const unsigned long long *__restrict__ prev,
+                                                     double tolerance, uint32_t prev_nijac,
+                                                     unsigned long long *__restrict__ state)
 {
     __shared__ unsigned long long wmax[4];
-    int x, y, z;
+    if (state[0] != 0ull) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (prev && darcy_decide(prev, tolerance, prev_nijac, state, i == 0)) return;
+    const uint32_t n = d.nx * d.ny * d.nz;
     unsigned long long bits = 0;
-    if (cell_xyz(d, x, y, z)) {
+    if (clear && i == 0) *clear = 0ull;
+    if (i < n) {
+        const int nx = (int)d.nx, ny = (int)d.ny, nz = (int)d.nz;
+        const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / (d.nx * d.ny));
+        const JLayout j = jlayout(d);
         const uint32_t c = didx(d, x, y, z);
-        double p;
-        const double dp_impl = darcy_dp(d, x, y, z, p_in, k, phi, dphi, vp, pconst, p);
+        const uint32_t sy = d.nx + 2, sz = (d.nx + 2) * (d.ny + 2);
+        const double p = p_in[c];
+        double p_xn = p_in[c - 1], p_xp = p_in[c + 1];
+        double p_yn = p_in[c - sy], p_yp = p_in[c + sy];
+        double p_zn = p_in[c - sz], p_zp = p_in[c + sz];
+        if (x == 0 && d.bc_xn == 1) p_xn = p;
+        if (x == nx - 1 && d.bc_xp == 1) p_xp = p;
+        if (y == 0 && d.bc_yn == 1) p_yn = p;
+        if (y == ny - 1 && d.bc_yp == 1) p_yp = p;
+        if (z == 0 && d.bc_bot == 1) p_zn = p;
+        if (z == nz - 1 && d.bc_top == 1) p_zp = p;
+        const double dx = d.dx, dy = d.dy, dz = d.dz;
+        const size_t fx = (size_t)x + j.sxf * ((size_t)y + d.ny * (size_t)z);
+        const size_t fy = j.hy + (size_t)x + d.nx * ((size_t)y + j.syf * (size_t)z);
+        const double h_xn = jc[fx], h_xp = jc[fx + 1];
+        const double h_yn = jc[fy], h_yp = jc[fy + d.nx];
+        const double h_zn = jc[j.hz + i], h_zp = jc[j.hz + i + (size_t)d.nx * d.ny];
+        const double div_k_grad_p = (h_xp * (p_xp - p) / dx - h_xn * (p - p_xn) / dx) / dx +
+                                    (h_yp * (p_yp - p) / dy - h_yn * (p - p_yn) / dy) / dy +
+                                    (h_zp * (p_zp - p) / dz - h_zn * (p - p_zn) / dz) / dz;
+        double dp_impl = jc[j.A + i] * div_k_grad_p - jc[j.F + i];
+        if (frozen[i]) dp_impl = 0.0;
         const double epsilon = 0.5;
-        double p_new = p_old[c] + (1.0 - epsilon) * dp_expl[c] + epsilon * dp_impl;
+        double p_new = jc[j.base + i] + epsilon * dp_impl;
         const double theta = 0.05;
         p_new = p * (1.0 - theta) + p_new * theta;
         const double res_norm = (p_new - p) / (p + 1.0e-16);
         p_out[c] = p_new;
-        norm[c] = res_norm;
+#define GW(X, Y, Z) p_out[didx(d, X, Y, Z)] = p_new
+        if (x == 0 && d.bc_xn == 0) GW(-1, y, z);
+        if (x == 1 && d.bc_xn == 1) GW(-1, y, z);
+        if (x == 0 && d.bc_xn == 2) GW(nx, y, z);
+        if (x == nx - 1 && d.bc_xp == 0) GW(nx, y, z);
+        if (x == nx - 2 && d.bc_xp == 1) GW(nx, y, z);
+        if (x == nx - 1 && d.bc_xp == 2) GW(-1, y, z);
+        if (y == 0 && d.bc_yn == 0) GW(x, -1, z);
+        if (y == 1 && d.bc_yn == 1) GW(x, -1, z);
+        if (y == 0 && d.bc_yn == 2) GW(x, ny, z);
+        if (y == ny - 1 && d.bc_yp == 0) GW(x, ny, z);
+        if (y == ny - 2 && d.bc_yp == 1) GW(x, ny, z);
+        if (y == ny - 1 && d.bc_yp == 2) GW(x, -1, z);
+        if (z == 0 && d.bc_bot == 0) GW(x, y, -1);
+        if (z == 1 && d.bc_bot == 1) GW(x, y, -1);
+        if (z == 0 && d.bc_bot == 2) GW(x, y, nz);
+        if (z == nz - 1 && d.bc_top == 0) GW(x, y, nz);
+        if (z == nz - 2 && d.bc_top == 1) GW(x, y, nz);
+        if (z == nz - 1 && d.bc_top == 2) GW(x, y, -1);
+#undef GW
+        if ((z == 0 && d.bc_bot == 4) || (z == nz - 1 && d.bc_top == 4)) {
+            const double q_z = z == 0 ? d.bc_bot_flux : d.bc_top_flux;
+            const double p_ghost = -d.mu / k_a[c] * q_z * (d.L2 / nz) + p;
+            if (z == 0 && d.bc_bot == 4) p_out[didx(d, x, y, -1)] = p_ghost;
+            if (z == nz - 1 && d.bc_top == 4) p_out[didx(d, x, y, nz)] = p_ghost;
+        }
         bits = (unsigned long long)__double_as_longlong(fabs(res_norm));
     }
-    if (maxres) {
+    if (check) {
         for (int off = 32; off > 0; off >>= 1) {
             unsigned long long o = __shfl_down(bits, off);
             bits = o > bits ? o : bits;
@@ -515,17 +675,34 @@ __global__ void __launch_bounds__(256) k_darcy_update(
         if (threadIdx.x == 0) {
             unsigned long long m = wmax[0];
             for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = wmax[w] > m ? wmax[w] : m;
-            atomicMax(maxres, m);
+            atomicMax(check, m);
         }
     }
 }
 
-void launch_darcy_update(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b,
-                         const double *p_in, double *p_out, int check_slot)
+void launch_darcy_sweep(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b,
+                        const double *p_in, double *p_out, unsigned long long *check,
+                        unsigned long long *clear, const unsigned long long *prev,
+                        double tolerance, uint32_t prev_nijac)
 {
-    unsigned long long *slot = check_slot >= 0 ? b.maxres + check_slot : nullptr;
-    hipLaunchKernelGGL(k_darcy_update, cell_grid(d), dim3(256), 0, s, d, b.p_old, b.dp_expl, p_in,
-                       b.k, b.phi, b.dphi, b.vp_avg, b.p_constant, p_out, b.norm, slot);
+    hipLaunchKernelGGL(k_darcy_sweep, cell_grid(d), dim3(256), 0, s, d, b.jc, b.jfrozen, p_in,
+                       b.k, p_out, check, clear, prev, tolerance, prev_nijac,
+                       b.maxres + DARCY_STATE);
+}
+
+// The test of a check sweep that no sweep follows (the last of maxiter).
+__global__ void k_darcy_decide(const unsigned long long *__restrict__ slot, double tolerance,
+                               uint32_t nijac, unsigned long long *__restrict__ state)
+{
+    if (threadIdx.x != 0 || state[0] != 0ull) return;
+    (void)darcy_decide(slot, tolerance, nijac, state, true);
+}
+
+void launch_darcy_decide(hipStream_t s, const DarcyBuffers &b, const unsigned long long *slot,
+                         double tolerance, uint32_t nijac)
+{
+    hipLaunchKernelGGL(k_darcy_decide, dim3(1), dim3(64), 0, s, slot, tolerance, nijac,
+                       b.maxres + DARCY_STATE);
 }
 
 // findDarcyVelocities, src/darcy.cuh:1839-1897

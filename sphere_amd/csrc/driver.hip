@@ -73,6 +73,8 @@ struct sph_ctx {
     // pinned host scratch
     uint32_t *h_err = nullptr;
     unsigned long long *h_u64 = nullptr;
+    unsigned long long *h_state = nullptr;  // Jacobi solve state (3 words)
+    unsigned int jacobi_hint = 0;           // sweeps of the last solve
     double *h_d = nullptr;
     // profiling
     bool prof = false;
@@ -438,10 +440,12 @@ static int upload(sph_ctx *ctx)
         const size_t c = dcells(h);
         AL(d.p, double, c); AL(d.p_old, double, c); AL(d.p_new, double, c);
         AL(d.dp_expl, double, c); AL(d.phi, double, c); AL(d.dphi, double, c);
-        AL(d.k, double, c); AL(d.norm, double, c);
+        AL(d.k, double, c);
+        const size_t ci = (size_t)h.num[0] * h.num[1] * h.num[2];
+        AL(d.jc, double, 9 * ci); AL(d.jfrozen, uint8_t, ci);
         AL(d.v, double, 3 * c); AL(d.vp_avg, double, 3 * c); AL(d.grad_p, double, 3 * c);
         AL(d.p_constant, int, c); AL(d.f_p, double4, np);
-        AL(d.maxres, unsigned long long, 4);
+        AL(d.maxres, unsigned long long, DARCY_MAXRES_WORDS);
         std::vector<double> p(c, 0.0), phi(c, 0.0), dphi(c, 0.0), v(3 * c, 0.0);
         std::vector<int> pc(c, 0);
         for (uint32_t z = 0; z < h.num[2]; ++z)
@@ -463,7 +467,7 @@ static int upload(sph_ctx *ctx)
         HIPCHK(hipMemcpyAsync(d.dphi, dphi.data(), c * 8, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d.v, v.data(), 3 * c * 8, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d.p_constant, pc.data(), c * 4, hipMemcpyHostToDevice, s));
-        double *zf[] = {d.p_old, d.p_new, d.dp_expl, d.k, d.norm};
+        double *zf[] = {d.p_old, d.p_new, d.dp_expl, d.k};
         for (double *q : zf) HIPCHK(hipMemcpyAsync(q, zero.data(), c * 8, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d.vp_avg, zero.data(), 3 * c * 8, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d.grad_p, zero.data(), 3 * c * 8, hipMemcpyHostToDevice, s));
@@ -716,53 +720,82 @@ static int darcy_step(sph_ctx *ctx, std::ofstream *convlog)
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
     (void)hipEventRecord(t0, s);
-    for (unsigned int nijac = 0; nijac < h.maxiter; ++nijac) {
-        const bool check = (nijac % 10 == 0);
-        {
+    // nijac == 0 (src/device.cu:2195-2240): p_old, the ghosts of p and the
+    // explicit solution, with the constants every sweep of this solve shares
+    {
+        ProfScope ps(ctx, SPH_K_DARCY_SWEEP);
+        launch_darcy_copy_interior(s, da, d.p, d.p_old);
+        launch_darcy_ghosts(s, da, d.p, 1);
+        launch_darcy_prepare(s, da, d);
+        launch_fill_u32(s, (uint32_t *)d.maxres, 2 * DARCY_MAXRES_WORDS, 0u);
+    }
+    // The sweeps (src/device.cu:2184-2386) ping-pong between the two
+    // buffers.  The convergence test of check sweep j (j % 10 == 0) runs on
+    // the device at the start of sweep j + 1, and once it ends the solve
+    // every later sweep returns at once.  The host enqueues sweeps in batches
+    // (the first as long as the last solve, each ending with a deciding
+    // sweep) and reads the solve state once per batch.
+    double *buf[2] = {d.p, d.p_new};
+    unsigned int nijac = 0;
+    unsigned long long state = 0, last_nijac = 0, last_bits = 0;
+    unsigned long long want = std::max(12u, (ctx->jacobi_hint + 9u) / 10u * 10u + 2u);
+    while (nijac < h.maxiter && state == 0) {
+        const unsigned int end = (unsigned int)std::min<unsigned long long>(h.maxiter, want);
+        for (; nijac < end; ++nijac) {
+            const bool check = (nijac % 10 == 0);
+            // two check slots alternate: a check sweep zeroes the other one
+            unsigned long long *slot = d.maxres + ((nijac / 10) & 1u);
+            unsigned long long *next = d.maxres + (((nijac / 10) & 1u) ^ 1u);
+            const bool decides = nijac > 0 && (nijac - 1) % 10 == 0;
+            const unsigned long long *prev = decides ? d.maxres + (((nijac - 1) / 10) & 1u) : nullptr;
             ProfScope ps(ctx, SPH_K_DARCY_SWEEP);
-            if (nijac == 0) launch_darcy_copy_interior(s, da, d.p, d.p_old);
-            launch_darcy_ghosts(s, da, d.p, 1);
-            if (nijac == 0) launch_darcy_first(s, da, d);
-            if (check) launch_fill_u32(s, (uint32_t *)d.maxres, 2, 0u);
-            launch_darcy_update(s, da, d, d.p, d.p_new, check ? 0 : -1);
             // setDarcyTopWallFixedFlow (bc_top == 1) writes dev_darcy_p, which
-            // the copy below overwrites: it has no effect (device.cu:2284-2331)
-            launch_darcy_ghosts_flux(s, da, d.p, d.k);
-            launch_darcy_copy_interior(s, da, d.p_new, d.p);
+            // the copy of p_new into p overwrites: it has no effect
+            // (device.cu:2284-2331)
+            launch_darcy_sweep(s, da, d, buf[nijac & 1u], buf[(nijac & 1u) ^ 1u],
+                               check ? slot : nullptr, check ? next : nullptr, prev, h.tolerance,
+                               nijac - 1u);
         }
-        ctx->stats.sweeps++;
-        if (check) {
-            HIPCHK(hipMemcpyAsync(ctx->h_u64, d.maxres, 8, hipMemcpyDeviceToHost, s));
-            int rc = sync(ctx);
-            if (rc) return rc;
-            double max_norm_res;
-            unsigned long long bits = *ctx->h_u64;
-            std::memcpy(&max_norm_res, &bits, 8);
-            if (max_norm_res != max_norm_res) {
-                (void)hipEventDestroy(t0);
-                (void)hipEventDestroy(t1);
-                return fail(ctx, SPH_ERR_NAN, "normalized residual is NaN: the system has "
-                                              "probably become unstable");
-            }
-            ctx->stats.last_nijac = nijac;
-            ctx->stats.last_max_residual = max_norm_res;
-            if (max_norm_res <= h.tolerance) {
-                if (ctx->iter % 10 == 0) {
-                    ctx->conv.emplace_back(ctx->iter + 1, nijac);
-                    if (convlog) *convlog << ctx->iter + 1 << '\t' << nijac << std::endl;
-                }
-                break;
-            }
+        if (nijac == h.maxiter && (h.maxiter - 1) % 10 == 0)
+            launch_darcy_decide(s, d, d.maxres + (((h.maxiter - 1) / 10) & 1u), h.tolerance,
+                                h.maxiter - 1);
+        HIPCHK(hipMemcpyAsync(ctx->h_state, d.maxres + DARCY_STATE, 24, hipMemcpyDeviceToHost, s));
+        int rc = sync(ctx);
+        if (rc) return rc;
+        state = ctx->h_state[0];
+        last_nijac = ctx->h_state[1];
+        last_bits = ctx->h_state[2];
+        want = (unsigned long long)nijac + 20u;
+    }
+    // sweeps that ran: up to the deciding check, or all maxiter
+    const unsigned int ran = state != 0 ? (unsigned int)last_nijac + 1u : h.maxiter;
+    d.p = buf[ran & 1u];
+    d.p_new = buf[(ran & 1u) ^ 1u];
+    ctx->stats.sweeps += ran;
+    ctx->jacobi_hint = ran;
+    double max_norm_res;
+    std::memcpy(&max_norm_res, &last_bits, 8);
+    ctx->stats.last_nijac = (uint32_t)last_nijac;
+    ctx->stats.last_max_residual = max_norm_res;
+    if (state == 2) {
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        return fail(ctx, SPH_ERR_NAN, "normalized residual is NaN: the system has "
+                                      "probably become unstable");
+    }
+    if (state == 1) {
+        if (ctx->iter % 10 == 0) {
+            ctx->conv.emplace_back(ctx->iter + 1, (unsigned int)last_nijac);
+            if (convlog) *convlog << ctx->iter + 1 << '\t' << last_nijac << std::endl;
         }
-        if (nijac == h.maxiter - 1) {
-            ctx->conv.emplace_back(ctx->iter + 1, nijac);
-            if (convlog) *convlog << ctx->iter + 1 << '\t' << nijac << std::endl;
-            ctx->stats.nonconverged++;
-            fprintf(stderr, "\nIteration %ld, time %g s: Error, the pressure solution in the fluid "
-                            "calculations did not converge. Try increasing the value of "
-                            "'darcy.maxiter' (%u) or increase 'darcy.tolerance' (%g).\n",
-                    ctx->iter, ctx->iter * h.time_dt, h.maxiter, h.tolerance);
-        }
+    } else if (h.maxiter > 0) {
+        ctx->conv.emplace_back(ctx->iter + 1, h.maxiter - 1);
+        if (convlog) *convlog << ctx->iter + 1 << '\t' << h.maxiter - 1 << std::endl;
+        ctx->stats.nonconverged++;
+        fprintf(stderr, "\nIteration %ld, time %g s: Error, the pressure solution in the fluid "
+                        "calculations did not converge. Try increasing the value of "
+                        "'darcy.maxiter' (%u) or increase 'darcy.tolerance' (%g).\n",
+                ctx->iter, ctx->iter * h.time_dt, h.maxiter, h.tolerance);
     }
     (void)hipEventRecord(t1, s);
     (void)hipEventSynchronize(t1);
@@ -949,13 +982,14 @@ sph_ctx *sph_create(int device)
         hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_integrated, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_walls, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc((void **)&ctx->h_err, 64, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&ctx->h_err, 128, hipHostMallocDefault) != hipSuccess) {
         g_last_error = "stream or pinned allocation failed";
         delete ctx;
         return nullptr;
     }
     ctx->h_u64 = (unsigned long long *)((char *)ctx->h_err + 16);
     ctx->h_d = (double *)((char *)ctx->h_err + 32);
+    ctx->h_state = (unsigned long long *)((char *)ctx->h_err + 64);
     return ctx;
 }
 

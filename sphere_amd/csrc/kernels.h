@@ -96,12 +96,23 @@ void launch_count_contacts(hipStream_t s, const uint64_t *cmask, uint32_t np,
                            unsigned long long *out);
 void launch_rebuild_mask(hipStream_t s, const DemBuffers &b, uint32_t np, uint32_t nc);
 
+#define DARCY_STATE 2     // first solve-state word in DarcyBuffers::maxres
+#define DARCY_MAXRES_WORDS 5
+
 struct DarcyBuffers {
-    double *p, *p_old, *p_new, *dp_expl, *phi, *dphi, *k, *norm;
+    double *p, *p_old, *p_new, *dp_expl, *phi, *dphi, *k;
     double *v, *vp_avg, *grad_p;  // 3 per cell
     int *p_constant;
     double4 *f_p;
-    unsigned long long *maxres;   // per-check slots (max |norm| bit patterns)
+    // [0], [1]: alternating check slots (max |norm| bit patterns);
+    // [DARCY_STATE..+2]: solve state written by k_darcy_decide
+    unsigned long long *maxres;
+    // constants of one solve's Jacobi sweeps: face conductances on the x, y
+    // and z faces, then per interior cell (x fastest) the factor A of
+    // div(k grad p), the forcing F and the base p_old + dp_expl/2 (at most
+    // 9 nx*ny*nz doubles); and per cell whether dp is held at zero
+    double *jc;
+    uint8_t *jfrozen;
 };
 
 void launch_darcy_ghosts(hipStream_t s, const DarcyArgs &d, double *f, int ncomp);
@@ -117,9 +128,11 @@ void launch_darcy_top_pressure(hipStream_t s, const DarcyArgs &d, const DarcyBuf
 void launch_darcy_permeabilities(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b);
 void launch_darcy_copy_interior(hipStream_t s, const DarcyArgs &d, const double *src, double *dst);
 void launch_darcy_zero_interior(hipStream_t s, const DarcyArgs &d, double *f);
-void launch_darcy_first(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b);
-// One Jacobi sweep p -> p_new with norm; if check_slot >= 0 the max |norm|
-// over interior cells is accumulated into maxres[check_slot].
-void launch_darcy_update(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b,
-                         const double *p_in, double *p_out, int check_slot);
+void launch_darcy_prepare(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b);
+void launch_darcy_sweep(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b,
+                        const double *p_in, double *p_out, unsigned long long *check,
+                        unsigned long long *clear, const unsigned long long *prev,
+                        double tolerance, uint32_t prev_nijac);
+void launch_darcy_decide(hipStream_t s, const DarcyBuffers &b, const unsigned long long *slot,
+                         double tolerance, uint32_t nijac);
 void launch_darcy_velocities(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b);

@@ -3,7 +3,9 @@
 * settling(n)       -- dry settling, linear contacts, frictional walls.
 * shear_box(n)      -- dense periodic shear box, Hertz-Mindlin, 10 kPa top
                        wall, top layer driven at 0.01 m/s, 5% bonded pairs.
-* consolidation(n)  -- Darcy-coupled consolidation under a 20 kPa wall.
+* consolidation_base(n) / consolidation_finish(s)
+                    -- Darcy-coupled consolidation: 200,000 grains under a
+                       20 kPa wall on a 48 x 48 x 96 Darcy grid.
 
 Dense packings are made by radius growth: grains start on a jittered cubic
 lattice with radii scaled down until no two touch, then the radii grow back
@@ -302,4 +304,96 @@ def settling(n=10000, seed=1, steps=2000):
     s.time_dt[0] = dt
     s.time_total[0] = steps * dt
     s.time_file_dt[0] = steps * dt
+    return s
+
+
+# linear contact stiffness of the consolidation box (sphere's defaultParams)
+K_N_CONSOLIDATION = 1.16e9
+
+
+def consolidation_base(n=200_000, seed=3, phi=0.6, cells_xy=48):
+    """Grains of the Darcy consolidation box on their growth lattice.
+
+    Radii ~U[0.009, 0.011] m; periodic in x and y; a fixed bottom (z = 0) and
+    a fixed top wall during growth; linear contacts k_n = k_t = 1.16e9 N/m
+    (sphere's default stiffness), frictionless and damped while growing.  The lattice footprint is chosen
+    so that, after the packing is rescaled to the drawn radii, cells_xy cells
+    of about 2.1 r_max span it."""
+    rng = numpy.random.default_rng(seed)
+    r = rng.uniform(0.009, 0.011, n)
+    r_max = float(numpy.max(r))
+    v_solid = float(numpy.sum(4.0 / 3.0 * numpy.pi * r ** 3))
+    spacing = (v_solid / phi / n) ** (1.0 / 3.0)
+    nx = int(round(cells_xy * 2.1 * r_max / spacing))
+    ny = nx
+    nz = (n + nx * ny - 1) // (nx * ny)
+    x = lattice(n, spacing, nx, ny, 0.05, rng)
+    s = sim(np=n, sid="consolidation", nw=1)
+    s.radius[:] = r
+    s.x[:] = x
+    s.periodic[0] = 1
+    s.contactmodel[0] = 2
+    s.defaultParams(mu_s=0.0, mu_d=0.0, rho=2600, k_n=K_N_CONSOLIDATION, k_t=K_N_CONSOLIDATION,
+                    gamma_n=3e3, gamma_wn=1e4, gamma_wt=0.0)
+    s.g[:] = 0.0
+    L = [nx * spacing, ny * spacing, nz * spacing]
+    s.defineWorldBoundaries(L=[L[0], L[1], L[2] + 4.0 * r_max], dx=2.1 * 1.1 * r_max)
+    s.w_n[0, :] = [0.0, 0.0, -1.0]
+    s.w_x[0] = L[2]
+    s.wmode[0] = 0
+    s.w_m[0] = 1.0
+    m_min = 2600.0 * 4.0 / 3.0 * numpy.pi * float(numpy.min(r)) ** 3
+    s.time_dt[0] = 0.17 * numpy.sqrt(m_min / K_N_CONSOLIDATION)
+    s.time_total[0] = 1.0
+    s.time_file_dt[0] = 1.0
+    f0 = 0.9 * spacing / (2.0 * r_max)
+    return s, f0, rng
+
+
+def consolidation_finish(s, sigma0=2e4, cells=(48, 48, 96), mu_f=1.04e-7, beta_f=1.43e-8,
+                         k_c=3.5e-13, tolerance=1e-8, maxiter=200000):
+    """Apply the consolidation run configuration to a settled packing.
+
+    consolidate() puts the top wall on the grains with sigma0 = 20 kPa and
+    the grains' total mass; friction mu = 0.5, no contact damping, gravity.
+    The Darcy grid is the DEM grid, cells[0] x cells[1] x cells[2] cells of
+    side L_x / cells[0] (at least 2 r_max).  Fluid: Darcy solver, viscosity
+    1.04e-7 Pa s, compressibility 1.43e-8 1/Pa (1/K_w of
+    python/halfshear-darcy-starter.py:41-44,57), Kozeny-Carman prefactor
+    3.5e-13 m^2, p = 0 at the top, no flow through the base, periodic in x
+    and y, Jacobi to a relative change of 1e-8, one DEM step per fluid step.
+    dt = 0.17 sqrt(m_min / k_n) with k_n = 1.16e9 N/m.
+
+    The Jacobi sweep p <- 0.95 p + 0.05 (p_old + (dp_expl + dp_impl(p))/2)
+    contracts while c = k dt / (beta_f phi mu dx^2) stays below about 6.5 in
+    the iterated cells; with these values that holds for phi below about
+    0.84 (the cells above the top wall are not iterated)."""
+    s.gamma_n[0] = 0.0
+    s.mu_s[0] = 0.5
+    s.mu_d[0] = 0.5
+    s.g[:] = [0.0, 0.0, -9.81]
+    s.consolidate(normal_stress=sigma0)
+    s.gamma_wn[0] = 1e4
+    cell = float(s.L[0]) / cells[0]
+    r_max = float(numpy.max(s.radius))
+    if cell < 2.0 * r_max:
+        raise ValueError("Darcy cell %.5g m is below 2 r_max = %.5g m" % (cell, 2.0 * r_max))
+    s.num = numpy.array(cells, dtype=numpy.uint32)
+    s.L = numpy.array([cells[0] * cell, cells[1] * cell, cells[2] * cell])
+    if float(s.w_x[0]) >= s.L[2]:
+        raise ValueError("the packing is taller than the Darcy grid")
+    s.initFluid(mu=mu_f, rho=1000.0, p=0.0, cfd_solver=1)
+    s.setFluidCompressibility(beta_f)
+    s.setPermeabilityPrefactor(k_c, verbose=False)
+    s.setFluidBottomNoFlow()
+    s.setFluidTopFixedPressure()
+    s.setTolerance(tolerance)
+    s.setMaxIterations(maxiter)
+    s.setDEMstepsPerCFDstep(1)
+    m_min = 2600.0 * 4.0 / 3.0 * numpy.pi * float(numpy.min(s.radius)) ** 3
+    s.time_dt[0] = 0.17 * numpy.sqrt(m_min / K_N_CONSOLIDATION)
+    s.time_current[0] = 0.0
+    s.time_total[0] = 1.0
+    s.time_file_dt[0] = 1.0
+    s.time_step_count[0] = 0
     return s
