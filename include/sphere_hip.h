@@ -131,6 +131,7 @@ typedef struct sph_darcy_stats {
     uint32_t pad;
     double last_max_residual; /* max |norm| at the last check                 */
     double solve_ms;          /* device time spent in Jacobi loops            */
+    uint64_t corrections;     /* learned corrections applied (sph_set_darcy_net) */
 } sph_darcy_stats;
 
 /* Kernel identifiers for sph_kernel_time. */
@@ -240,6 +241,51 @@ int sph_get_darcy_stats(sph_ctx *ctx, sph_darcy_stats *out);
 int sph_set_profiling(sph_ctx *ctx, int on);
 int sph_kernel_time(sph_ctx *ctx, int kernel_id, double *total_ms,
                     uint64_t *launches);
+
+/* ------------------------------------------------------------------------
+ * Learned correction of the Darcy solve.  No counterpart in the reference:
+ * it changes only how many Jacobi sweeps a solve takes.  The sweeps, their
+ * convergence test (max relative change <= tolerance every 10 sweeps) and
+ * maxiter stay those of the reference (src/device.cu:2184-2386); after the
+ * test of sweeps 0, 10, 20, ... has not ended the solve, a 3-D U-Net (BF16
+ * on the matrix cores, FP32 accumulation) maps the fixed-point residual of
+ * the current iterate to an estimate of its error, which is added to the
+ * iterate.  Plain Jacobi stays the fallback and the judge of convergence.
+ * ---------------------------------------------------------------------- */
+typedef struct sph_net sph_net;
+
+/* Number of FP32 parameters of the network (per layer w[27][cout][cin],
+ * then b[cout]). */
+uint64_t sph_net_param_count(void);
+/* Network for an nx*ny*nz grid (each a multiple of 8) on a device. */
+sph_net *sph_net_create(int device, uint32_t nx, uint32_t ny, uint32_t nz);
+void sph_net_destroy(sph_net *net);
+const char *sph_net_last_error(void);
+/* Device pointers; stream is a hipStream_t (NULL: the null stream).
+ * forward: input[V][3] -> output[V], V = nx*ny*nz, x fastest.
+ * backward: of the last forward; adds dLoss/dparams to grad_params given
+ * grad_output = dLoss/doutput. */
+int sph_net_forward(sph_net *net, const float *params, const float *input,
+                    float *output, void *stream);
+int sph_net_backward(sph_net *net, const float *params, const float *grad_output,
+                     float *grad_params, void *stream);
+
+/* Use the network in the context's Darcy solves: params on the host
+ * (count = sph_net_param_count()), at most max_corrections per solve;
+ * params == NULL or max_corrections == 0 switches it off. */
+int sph_set_darcy_net(sph_ctx *ctx, const float *params, uint64_t count,
+                      int max_corrections);
+
+/* Training data from the last Darcy solve of the context (device
+ * pointers; fields with the ghost layer, (nx+2)(ny+2)(nz+2) doubles). */
+#define SPH_DARCY_P_START 0   /* the iterate the solve started from (p_old) */
+#define SPH_DARCY_P_FINAL 1   /* the converged pressure                     */
+int sph_darcy_export(sph_ctx *ctx, int which, double *dst);
+/* nsweeps plain Jacobi sweeps of the last solve's system applied to p. */
+int sph_darcy_sweeps(sph_ctx *ctx, double *p, int nsweeps);
+/* Network input [V][3] for iterate p, and the residual scale s (host):
+ * the network's target for p is (p_fixed - p) / s on the cells. */
+int sph_darcy_net_input(sph_ctx *ctx, const double *p, float *input, double *scale);
 
 #ifdef __cplusplus
 }

@@ -86,7 +86,7 @@ class SphDarcyStats(ctypes.Structure):
     _fields_ = [("fluid_steps", ctypes.c_uint64), ("sweeps", ctypes.c_uint64),
                 ("nonconverged", ctypes.c_uint64), ("last_nijac", ctypes.c_uint32),
                 ("pad", ctypes.c_uint32), ("last_max_residual", _d),
-                ("solve_ms", _d)]
+                ("solve_ms", _d), ("corrections", ctypes.c_uint64)]
 
 
 # Names of the array members and their (dtype, per-item width, count key).
@@ -187,6 +187,17 @@ def load_library(path=None):
         "sph_set_profiling": (ctypes.c_int, [vp, ctypes.c_int]),
         "sph_kernel_time": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_uint64)]),
+        "sph_net_param_count": (ctypes.c_uint64, []),
+        "sph_net_create": (vp, [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+        "sph_net_destroy": (None, [vp]),
+        "sph_net_last_error": (ctypes.c_char_p, []),
+        "sph_net_forward": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+        "sph_net_backward": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+        "sph_set_darcy_net": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_uint64,
+                                             ctypes.c_int]),
+        "sph_darcy_export": (ctypes.c_int, [vp, ctypes.c_int, vp]),
+        "sph_darcy_sweeps": (ctypes.c_int, [vp, vp, ctypes.c_int]),
+        "sph_darcy_net_input": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -308,8 +319,79 @@ class Engine:
     def set_profiling(self, on):
         self._check(self.lib.sph_set_profiling(self.ctx, int(on)))
 
+    # learned Darcy correction (device buffers are torch CUDA tensors)
+    def set_darcy_net(self, params, max_corrections=8):
+        """params: FP32 host array of Net.param_count() values, or None."""
+        if params is None:
+            self._check(self.lib.sph_set_darcy_net(self.ctx, None, 0, 0))
+            return
+        p = numpy.ascontiguousarray(params, dtype=numpy.float32)
+        self._check(self.lib.sph_set_darcy_net(self.ctx, _ptr(p, ctypes.c_float), p.size,
+                                               int(max_corrections)))
+
+    def darcy_export(self, which, dst):
+        """Copy the last solve's start (0) or final (1) pressure, ghost layer
+        included, into the float64 CUDA tensor dst."""
+        self._check(self.lib.sph_darcy_export(self.ctx, int(which), ctypes.c_void_p(dst.data_ptr())))
+
+    def darcy_sweeps(self, p, n):
+        self._check(self.lib.sph_darcy_sweeps(self.ctx, ctypes.c_void_p(p.data_ptr()), int(n)))
+
+    def darcy_net_input(self, p, inp):
+        """Network input for iterate p (float64 CUDA tensor with ghosts) into
+        the float32 CUDA tensor inp [V, 3]; returns the residual scale."""
+        s = ctypes.c_double(0)
+        self._check(self.lib.sph_darcy_net_input(self.ctx, ctypes.c_void_p(p.data_ptr()),
+                                                 ctypes.c_void_p(inp.data_ptr()), ctypes.byref(s)))
+        return s.value
+
     def kernel_time(self, kid):
         t = ctypes.c_double(0)
         n = ctypes.c_uint64(0)
         self._check(self.lib.sph_kernel_time(self.ctx, int(kid), ctypes.byref(t), ctypes.byref(n)))
         return t.value, n.value
+
+
+class Net:
+    """The Darcy correction network (sph_net_*) on one device.  Buffers are
+    torch CUDA tensors; calls run on torch's current stream."""
+
+    def __init__(self, nx, ny, nz, device=0, lib=None):
+        self.lib = lib or load_library()
+        self.h = self.lib.sph_net_create(int(device), int(nx), int(ny), int(nz))
+        if not self.h:
+            raise SphereError(-3, self.lib.sph_net_last_error().decode())
+        self.shape = (int(nz), int(ny), int(nx))
+
+    @staticmethod
+    def param_count(lib=None):
+        return int((lib or load_library()).sph_net_param_count())
+
+    def _stream(self):
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def _check(self, rc):
+        if rc != SPH_OK:
+            raise SphereError(rc, self.lib.sph_net_last_error().decode())
+
+    def forward(self, params, inp, out):
+        self._check(self.lib.sph_net_forward(self.h, ctypes.c_void_p(params.data_ptr()),
+                                             ctypes.c_void_p(inp.data_ptr()),
+                                             ctypes.c_void_p(out.data_ptr()), self._stream()))
+
+    def backward(self, params, grad_out, grad_params):
+        self._check(self.lib.sph_net_backward(self.h, ctypes.c_void_p(params.data_ptr()),
+                                              ctypes.c_void_p(grad_out.data_ptr()),
+                                              ctypes.c_void_p(grad_params.data_ptr()), self._stream()))
+
+    def close(self):
+        if self.h:
+            self.lib.sph_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

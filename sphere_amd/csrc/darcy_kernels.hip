@@ -568,6 +568,43 @@ __device__ __forceinline__ bool darcy_decide(const unsigned long long *__restric
     return stop;
 }
 
+// p* of updateDarcySolution (src/darcy.cuh:1646-1739) for interior cell i
+// = (x, y, z) (ghost-layer index c) of iterate p_in, with the solve's
+// constants: p* = base + epsilon (A div(k grad p) - F), dp_impl held at 0
+// where frozen.  The sweep relaxes p toward it by theta.
+__device__ __forceinline__ double jacobi_target(const DarcyArgs &d, const double *__restrict__ jc,
+                                                const uint8_t *__restrict__ frozen,
+                                                const double *__restrict__ p_in, uint32_t i, int x, int y,
+                                                int z, uint32_t c)
+{
+    const int nx = (int)d.nx, ny = (int)d.ny, nz = (int)d.nz;
+    const JLayout j = jlayout(d);
+    const uint32_t sy = d.nx + 2, sz = (d.nx + 2) * (d.ny + 2);
+    const double p = p_in[c];
+    double p_xn = p_in[c - 1], p_xp = p_in[c + 1];
+    double p_yn = p_in[c - sy], p_yp = p_in[c + sy];
+    double p_zn = p_in[c - sz], p_zp = p_in[c + sz];
+    if (x == 0 && d.bc_xn == 1) p_xn = p;
+    if (x == nx - 1 && d.bc_xp == 1) p_xp = p;
+    if (y == 0 && d.bc_yn == 1) p_yn = p;
+    if (y == ny - 1 && d.bc_yp == 1) p_yp = p;
+    if (z == 0 && d.bc_bot == 1) p_zn = p;
+    if (z == nz - 1 && d.bc_top == 1) p_zp = p;
+    const double dx = d.dx, dy = d.dy, dz = d.dz;
+    const size_t fx = (size_t)x + j.sxf * ((size_t)y + d.ny * (size_t)z);
+    const size_t fy = j.hy + (size_t)x + d.nx * ((size_t)y + j.syf * (size_t)z);
+    const double h_xn = jc[fx], h_xp = jc[fx + 1];
+    const double h_yn = jc[fy], h_yp = jc[fy + d.nx];
+    const double h_zn = jc[j.hz + i], h_zp = jc[j.hz + i + (size_t)d.nx * d.ny];
+    const double div_k_grad_p = (h_xp * (p_xp - p) / dx - h_xn * (p - p_xn) / dx) / dx +
+                                (h_yp * (p_yp - p) / dy - h_yn * (p - p_yn) / dy) / dy +
+                                (h_zp * (p_zp - p) / dz - h_zn * (p - p_zn) / dz) / dz;
+    double dp_impl = jc[j.A + i] * div_k_grad_p - jc[j.F + i];
+    if (frozen[i]) dp_impl = 0.0;
+    const double epsilon = 0.5;
+    return jc[j.base + i] + epsilon * dp_impl;
+}
+
 // One sweep of the Jacobi loop body (src/device.cu:2184-2331) in a single
 // launch: updateDarcySolution on iterate p_in into p_out, then, into p_out,
 // the ghost values that the next sweep's setDarcyGhostNodes would copy from
@@ -607,32 +644,9 @@ __global__ void __launch_bounds__(256) k_darcy_sweep(DarcyArgs d, const double *
     if (i < n) {
         const int nx = (int)d.nx, ny = (int)d.ny, nz = (int)d.nz;
         const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / (d.nx * d.ny));
-        const JLayout j = jlayout(d);
         const uint32_t c = didx(d, x, y, z);
-        const uint32_t sy = d.nx + 2, sz = (d.nx + 2) * (d.ny + 2);
         const double p = p_in[c];
-        double p_xn = p_in[c - 1], p_xp = p_in[c + 1];
-        double p_yn = p_in[c - sy], p_yp = p_in[c + sy];
-        double p_zn = p_in[c - sz], p_zp = p_in[c + sz];
-        if (x == 0 && d.bc_xn == 1) p_xn = p;
-        if (x == nx - 1 && d.bc_xp == 1) p_xp = p;
-        if (y == 0 && d.bc_yn == 1) p_yn = p;
-        if (y == ny - 1 && d.bc_yp == 1) p_yp = p;
-        if (z == 0 && d.bc_bot == 1) p_zn = p;
-        if (z == nz - 1 && d.bc_top == 1) p_zp = p;
-        const double dx = d.dx, dy = d.dy, dz = d.dz;
-        const size_t fx = (size_t)x + j.sxf * ((size_t)y + d.ny * (size_t)z);
-        const size_t fy = j.hy + (size_t)x + d.nx * ((size_t)y + j.syf * (size_t)z);
-        const double h_xn = jc[fx], h_xp = jc[fx + 1];
-        const double h_yn = jc[fy], h_yp = jc[fy + d.nx];
-        const double h_zn = jc[j.hz + i], h_zp = jc[j.hz + i + (size_t)d.nx * d.ny];
-        const double div_k_grad_p = (h_xp * (p_xp - p) / dx - h_xn * (p - p_xn) / dx) / dx +
-                                    (h_yp * (p_yp - p) / dy - h_yn * (p - p_yn) / dy) / dy +
-                                    (h_zp * (p_zp - p) / dz - h_zn * (p - p_zn) / dz) / dz;
-        double dp_impl = jc[j.A + i] * div_k_grad_p - jc[j.F + i];
-        if (frozen[i]) dp_impl = 0.0;
-        const double epsilon = 0.5;
-        double p_new = jc[j.base + i] + epsilon * dp_impl;
+        double p_new = jacobi_target(d, jc, frozen, p_in, i, x, y, z, c);
         const double theta = 0.05;
         p_new = p * (1.0 - theta) + p_new * theta;
         const double res_norm = (p_new - p) / (p + 1.0e-16);
@@ -683,11 +697,11 @@ __global__ void __launch_bounds__(256) k_darcy_sweep(DarcyArgs d, const double *
 void launch_darcy_sweep(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b,
                         const double *p_in, double *p_out, unsigned long long *check,
                         unsigned long long *clear, const unsigned long long *prev,
-                        double tolerance, uint32_t prev_nijac)
+                        double tolerance, uint32_t prev_nijac, bool free_running)
 {
     hipLaunchKernelGGL(k_darcy_sweep, cell_grid(d), dim3(256), 0, s, d, b.jc, b.jfrozen, p_in,
                        b.k, p_out, check, clear, prev, tolerance, prev_nijac,
-                       b.maxres + DARCY_STATE);
+                       b.maxres + (free_running ? DARCY_ZERO : DARCY_STATE));
 }
 
 // The test of a check sweep that no sweep follows (the last of maxiter).
@@ -703,6 +717,125 @@ void launch_darcy_decide(hipStream_t s, const DarcyBuffers &b, const unsigned lo
 {
     hipLaunchKernelGGL(k_darcy_decide, dim3(1), dim3(64), 0, s, slot, tolerance, nijac,
                        b.maxres + DARCY_STATE);
+}
+
+// ---------------------------------------------------------------------------
+// Learned correction between sweeps (no counterpart in the reference).  For
+// iterate p the fixed-point residual r = p*(p) - p, with p* as above, and
+// the error e = p_fixed - p satisfy (I - epsilon A L) e = r, L the
+// conductance-weighted Laplacian; the network maps (r / s, log(1 + D),
+// frozen) to e / s, s = max |r|, D = epsilon A sum_faces h / dx^2 the
+// cell's coupling.  Every kernel below returns at once when state != null
+// and state[0] != 0 (the solve has ended), so corrections can be enqueued
+// ahead of the convergence decision.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_darcy_residual_max(DarcyArgs d, const double *__restrict__ jc,
+                                                            const uint8_t *__restrict__ frozen,
+                                                            const double *__restrict__ p,
+                                                            const unsigned long long *__restrict__ state,
+                                                            unsigned long long *__restrict__ smax)
+{
+    __shared__ unsigned long long wmax[4];
+    if (state && state[0] != 0ull) return;
+    const uint32_t n = d.nx * d.ny * d.nz;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long bits = 0;
+    if (i < n) {
+        const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / (d.nx * d.ny));
+        const uint32_t c = didx(d, x, y, z);
+        const double r = jacobi_target(d, jc, frozen, p, i, x, y, z, c) - p[c];
+        bits = (unsigned long long)__double_as_longlong(fabs(r));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_down(bits, off);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63u) == 0) wmax[threadIdx.x >> 6] = bits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = wmax[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = wmax[w] > m ? wmax[w] : m;
+        atomicMax(smax, m);
+    }
+}
+
+__device__ __forceinline__ bool usable_scale(unsigned long long bits, double &s)
+{
+    s = __longlong_as_double((long long)bits);
+    return s > 0.0 && s < 1.0e300;  // false for 0, inf and NaN
+}
+
+// network input [V][3] (x fastest) for iterate p
+__global__ void __launch_bounds__(256) k_darcy_net_input(DarcyArgs d, const double *__restrict__ jc,
+                                                         const uint8_t *__restrict__ frozen,
+                                                         const double *__restrict__ p,
+                                                         const unsigned long long *__restrict__ state,
+                                                         const unsigned long long *__restrict__ smax,
+                                                         float *__restrict__ input)
+{
+    if (state && state[0] != 0ull) return;
+    const uint32_t n = d.nx * d.ny * d.nz;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / (d.nx * d.ny));
+    const uint32_t c = didx(d, x, y, z);
+    double s;
+    const bool ok = usable_scale(*smax, s);
+    const double r = jacobi_target(d, jc, frozen, p, i, x, y, z, c) - p[c];
+    const JLayout j = jlayout(d);
+    const size_t fx = (size_t)x + j.sxf * ((size_t)y + d.ny * (size_t)z);
+    const size_t fy = j.hy + (size_t)x + d.nx * ((size_t)y + j.syf * (size_t)z);
+    const double hsum = (jc[fx] + jc[fx + 1]) / (d.dx * d.dx) + (jc[fy] + jc[fy + d.nx]) / (d.dy * d.dy) +
+                        (jc[j.hz + i] + jc[j.hz + i + (size_t)d.nx * d.ny]) / (d.dz * d.dz);
+    const double D = 0.5 * jc[j.A + i] * hsum;
+    const bool fz = frozen[i] != 0;
+    input[3 * (size_t)i] = ok ? (float)(r / s) : 0.f;
+    input[3 * (size_t)i + 1] = (fz || !(D >= 0.0)) ? 0.f : (float)log1p(D);
+    input[3 * (size_t)i + 2] = fz ? 1.f : 0.f;
+}
+
+// p += s e on the cells that are not held fixed; counts applied corrections
+// in state[3]
+__global__ void __launch_bounds__(256) k_darcy_net_apply(DarcyArgs d, const uint8_t *__restrict__ frozen,
+                                                         double *__restrict__ p,
+                                                         const float *__restrict__ e,
+                                                         const unsigned long long *__restrict__ smax,
+                                                         unsigned long long *__restrict__ state)
+{
+    if (state[0] != 0ull) return;
+    double s;
+    if (!usable_scale(*smax, s)) return;
+    const uint32_t n = d.nx * d.ny * d.nz;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) state[3] += 1ull;
+    if (i >= n || frozen[i]) return;
+    const float ev = e[i];
+    if (!(ev == ev)) return;
+    const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / (d.nx * d.ny));
+    p[didx(d, x, y, z)] += s * (double)ev;
+}
+
+void launch_darcy_residual_max(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, const double *p,
+                               bool free_running)
+{
+    hipLaunchKernelGGL(k_darcy_residual_max, cell_grid(d), dim3(256), 0, s, d, b.jc, b.jfrozen, p,
+                       free_running ? (const unsigned long long *)nullptr : b.maxres + DARCY_STATE,
+                       b.maxres + DARCY_SCALE);
+}
+
+void launch_darcy_net_input(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, const double *p,
+                            float *input, bool free_running)
+{
+    hipLaunchKernelGGL(k_darcy_net_input, cell_grid(d), dim3(256), 0, s, d, b.jc, b.jfrozen, p,
+                       free_running ? (const unsigned long long *)nullptr : b.maxres + DARCY_STATE,
+                       b.maxres + DARCY_SCALE, input);
+}
+
+void launch_darcy_net_apply(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, double *p,
+                            const float *e)
+{
+    hipLaunchKernelGGL(k_darcy_net_apply, cell_grid(d), dim3(256), 0, s, d, b.jfrozen, p, e,
+                       b.maxres + DARCY_SCALE, b.maxres + DARCY_STATE);
 }
 
 // findDarcyVelocities, src/darcy.cuh:1839-1897

@@ -9,6 +9,8 @@
 // 10 sweeps, output events, and the end of each call.
 #include <hip/hip_runtime.h>
 
+#include "net.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -73,9 +75,15 @@ struct sph_ctx {
     // pinned host scratch
     uint32_t *h_err = nullptr;
     unsigned long long *h_u64 = nullptr;
-    unsigned long long *h_state = nullptr;  // Jacobi solve state (3 words)
+    unsigned long long *h_state = nullptr;  // Jacobi solve state (4 words)
     unsigned int jacobi_hint = 0;           // sweeps of the last solve
     double *h_d = nullptr;
+    // learned Darcy correction: host parameters, and the network with its
+    // device buffers for the loaded grid (made at the first solve)
+    std::vector<float> net_host;
+    int net_max_corr = 0;
+    sph_net *net = nullptr;
+    float *net_params = nullptr, *net_in = nullptr, *net_out = nullptr;
     // profiling
     bool prof = false;
     double ktime[SPH_K_COUNT] = {0};
@@ -204,8 +212,20 @@ static T *dalloc(sph_ctx *ctx, size_t n)
     return (T *)p;
 }
 
+static void net_release(sph_ctx *ctx)
+{
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    sph_net_destroy(ctx->net);
+    ctx->net = nullptr;
+    float *bufs[] = {ctx->net_params, ctx->net_in, ctx->net_out};
+    for (float *b : bufs)
+        if (b) (void)hipFree(b);
+    ctx->net_params = ctx->net_in = ctx->net_out = nullptr;
+}
+
 static void free_device(sph_ctx *ctx)
 {
+    net_release(ctx);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
     ctx->walls_pending = false;
@@ -682,6 +702,26 @@ static DarcyArgs darcy_args(const sph_ctx *ctx)
     return d;
 }
 
+// The network of sph_set_darcy_net for the loaded grid, made on first use.
+static int net_ready(sph_ctx *ctx)
+{
+    if (ctx->net || ctx->net_host.empty() || ctx->net_max_corr <= 0) return SPH_OK;
+    const sph_state &h = ctx->hs.h;
+    ctx->net = sph_net_create(ctx->device, h.num[0], h.num[1], h.num[2]);
+    if (!ctx->net) return fail(ctx, SPH_ERR_VALUE, std::string("Darcy network: ") + sph_net_last_error());
+    const size_t V = (size_t)h.num[0] * h.num[1] * h.num[2];
+    if (hipMalloc((void **)&ctx->net_params, ctx->net_host.size() * 4) != hipSuccess ||
+        hipMalloc((void **)&ctx->net_in, V * 3 * 4) != hipSuccess ||
+        hipMalloc((void **)&ctx->net_out, V * 4) != hipSuccess) {
+        net_release(ctx);
+        return fail(ctx, SPH_ERR_HIP, "Darcy network: device allocation failed");
+    }
+    HIPCHK(hipMemcpyAsync(ctx->net_params, ctx->net_host.data(), ctx->net_host.size() * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    net_pack(ctx->net, ctx->net_params, ctx->stream);
+    return sync(ctx);
+}
+
 // Darcy branch of the loop body, src/device.cu:1901-2426
 static int darcy_step(sph_ctx *ctx, std::ofstream *convlog)
 {
@@ -736,6 +776,9 @@ static int darcy_step(sph_ctx *ctx, std::ofstream *convlog)
     // (the first as long as the last solve, each ending with a deciding
     // sweep) and reads the solve state once per batch.
     double *buf[2] = {d.p, d.p_new};
+    int rc0 = net_ready(ctx);
+    if (rc0) return rc0;
+    int corrections = 0;
     unsigned int nijac = 0;
     unsigned long long state = 0, last_nijac = 0, last_bits = 0;
     unsigned long long want = std::max(12u, (ctx->jacobi_hint + 9u) / 10u * 10u + 2u);
@@ -755,11 +798,24 @@ static int darcy_step(sph_ctx *ctx, std::ofstream *convlog)
             launch_darcy_sweep(s, da, d, buf[nijac & 1u], buf[(nijac & 1u) ^ 1u],
                                check ? slot : nullptr, check ? next : nullptr, prev, h.tolerance,
                                nijac - 1u);
+            // learned correction of the iterate this deciding sweep wrote
+            // (its kernels do nothing once the solve has ended)
+            if (decides && ctx->net && corrections < ctx->net_max_corr) {
+                double *pc = buf[(nijac & 1u) ^ 1u];
+                launch_fill_u32(s, (uint32_t *)(d.maxres + DARCY_SCALE), 2, 0u);
+                launch_darcy_residual_max(s, da, d, pc, false);
+                launch_darcy_net_input(s, da, d, pc, ctx->net_in, false);
+                int nrc = net_forward_packed(ctx->net, ctx->net_params, ctx->net_in, ctx->net_out, s);
+                if (nrc) return fail(ctx, nrc, sph_net_last_error());
+                launch_darcy_net_apply(s, da, d, pc, ctx->net_out);
+                launch_darcy_ghosts(s, da, pc, 1);
+                ++corrections;
+            }
         }
         if (nijac == h.maxiter && (h.maxiter - 1) % 10 == 0)
             launch_darcy_decide(s, d, d.maxres + (((h.maxiter - 1) / 10) & 1u), h.tolerance,
                                 h.maxiter - 1);
-        HIPCHK(hipMemcpyAsync(ctx->h_state, d.maxres + DARCY_STATE, 24, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(ctx->h_state, d.maxres + DARCY_STATE, 32, hipMemcpyDeviceToHost, s));
         int rc = sync(ctx);
         if (rc) return rc;
         state = ctx->h_state[0];
@@ -767,6 +823,7 @@ static int darcy_step(sph_ctx *ctx, std::ofstream *convlog)
         last_bits = ctx->h_state[2];
         want = (unsigned long long)nijac + 20u;
     }
+    ctx->stats.corrections += ctx->h_state[3];
     // sweeps that ran: up to the deciding check, or all maxiter
     const unsigned int ran = state != 0 ? (unsigned int)last_nijac + 1u : h.maxiter;
     d.p = buf[ran & 1u];
@@ -1370,6 +1427,79 @@ int sph_count_contacts(sph_ctx *ctx, uint64_t *ncontacts)
     (void)hipFree(tmp);
     if (rc) return rc;
     *ncontacts = *ctx->h_u64;
+    return SPH_OK;
+}
+
+int sph_set_darcy_net(sph_ctx *ctx, const float *params, uint64_t count, int max_corrections)
+{
+    if (!ctx) return fail(nullptr, SPH_ERR_ARG, "null context");
+    (void)hipSetDevice(ctx->device);
+    net_release(ctx);
+    if (!params || max_corrections <= 0) {
+        ctx->net_host.clear();
+        ctx->net_max_corr = 0;
+        return SPH_OK;
+    }
+    if (count != sph_net_param_count())
+        return fail(ctx, SPH_ERR_ARG, "Darcy network: expected " + std::to_string(sph_net_param_count()) +
+                                          " parameters, got " + std::to_string(count));
+    ctx->net_host.assign(params, params + count);
+    ctx->net_max_corr = max_corrections;
+    return SPH_OK;
+}
+
+static int darcy_system_ready(sph_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, SPH_ERR_ARG, "null context");
+    if (!ctx->loaded || !ctx->hs.h.fluid) return fail(ctx, SPH_ERR_STATE, "no fluid state loaded");
+    if (ctx->stats.fluid_steps == 0) return fail(ctx, SPH_ERR_STATE, "no Darcy solve has run yet");
+    (void)hipSetDevice(ctx->device);
+    return SPH_OK;
+}
+
+int sph_darcy_export(sph_ctx *ctx, int which, double *dst)
+{
+    int rc = darcy_system_ready(ctx);
+    if (rc) return rc;
+    if (!dst || (which != SPH_DARCY_P_START && which != SPH_DARCY_P_FINAL))
+        return fail(ctx, SPH_ERR_ARG, "bad field or destination");
+    const double *src = which == SPH_DARCY_P_START ? ctx->dd.p_old : ctx->dd.p;
+    HIPCHK(hipMemcpyAsync(dst, src, (size_t)dcells(ctx->hs.h) * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    return sync(ctx);
+}
+
+int sph_darcy_sweeps(sph_ctx *ctx, double *p, int nsweeps)
+{
+    int rc = darcy_system_ready(ctx);
+    if (rc) return rc;
+    if (!p || nsweeps < 0) return fail(ctx, SPH_ERR_ARG, "bad iterate or sweep count");
+    const DarcyArgs da = darcy_args(ctx);
+    DarcyBuffers &d = ctx->dd;
+    hipStream_t s = ctx->stream;
+    launch_darcy_ghosts(s, da, p, 1);
+    double *buf[2] = {p, d.p_new};
+    for (int k = 0; k < nsweeps; ++k)
+        launch_darcy_sweep(s, da, d, buf[k & 1], buf[(k & 1) ^ 1], nullptr, nullptr, nullptr, 0.0, 0u, true);
+    if (nsweeps & 1)
+        HIPCHK(hipMemcpyAsync(p, d.p_new, (size_t)dcells(ctx->hs.h) * 8, hipMemcpyDeviceToDevice, s));
+    return sync(ctx);
+}
+
+int sph_darcy_net_input(sph_ctx *ctx, const double *p, float *input, double *scale)
+{
+    int rc = darcy_system_ready(ctx);
+    if (rc) return rc;
+    if (!p || !input || !scale) return fail(ctx, SPH_ERR_ARG, "null argument");
+    const DarcyArgs da = darcy_args(ctx);
+    DarcyBuffers &d = ctx->dd;
+    hipStream_t s = ctx->stream;
+    launch_fill_u32(s, (uint32_t *)(d.maxres + DARCY_SCALE), 2, 0u);
+    launch_darcy_residual_max(s, da, d, p, true);
+    launch_darcy_net_input(s, da, d, p, input, true);
+    HIPCHK(hipMemcpyAsync(ctx->h_u64, d.maxres + DARCY_SCALE, 8, hipMemcpyDeviceToHost, s));
+    rc = sync(ctx);
+    if (rc) return rc;
+    std::memcpy(scale, ctx->h_u64, 8);
     return SPH_OK;
 }
 

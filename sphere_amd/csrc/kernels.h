@@ -96,17 +96,21 @@ void launch_count_contacts(hipStream_t s, const uint64_t *cmask, uint32_t np,
                            unsigned long long *out);
 void launch_rebuild_mask(hipStream_t s, const DemBuffers &b, uint32_t np, uint32_t nc);
 
-#define DARCY_STATE 2     // first solve-state word in DarcyBuffers::maxres
-#define DARCY_MAXRES_WORDS 5
+// words of DarcyBuffers::maxres: [0], [1] alternating check slots;
+// DARCY_STATE: solve state (ended, nijac of the last check, its max |norm|
+// bits, corrections applied); DARCY_ZERO: always 0 (the state of sweeps run
+// outside a solve); DARCY_SCALE: max |residual| bits of a correction
+#define DARCY_STATE 2
+#define DARCY_ZERO 6
+#define DARCY_SCALE 7
+#define DARCY_MAXRES_WORDS 8
 
 struct DarcyBuffers {
     double *p, *p_old, *p_new, *dp_expl, *phi, *dphi, *k;
     double *v, *vp_avg, *grad_p;  // 3 per cell
     int *p_constant;
     double4 *f_p;
-    // [0], [1]: alternating check slots (max |norm| bit patterns);
-    // [DARCY_STATE..+2]: solve state written by k_darcy_decide
-    unsigned long long *maxres;
+    unsigned long long *maxres;  // DARCY_MAXRES_WORDS words, see above
     // constants of one solve's Jacobi sweeps: face conductances on the x, y
     // and z faces, then per interior cell (x fastest) the factor A of
     // div(k grad p), the forcing F and the base p_old + dp_expl/2 (at most
@@ -132,7 +136,13 @@ void launch_darcy_prepare(hipStream_t s, const DarcyArgs &d, const DarcyBuffers 
 void launch_darcy_sweep(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b,
                         const double *p_in, double *p_out, unsigned long long *check,
                         unsigned long long *clear, const unsigned long long *prev,
-                        double tolerance, uint32_t prev_nijac);
+                        double tolerance, uint32_t prev_nijac, bool free_running = false);
+void launch_darcy_residual_max(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, const double *p,
+                               bool free_running);
+void launch_darcy_net_input(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, const double *p,
+                            float *input, bool free_running);
+void launch_darcy_net_apply(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, double *p,
+                            const float *e);
 void launch_darcy_decide(hipStream_t s, const DarcyBuffers &b, const unsigned long long *slot,
                          double tolerance, uint32_t nijac);
 void launch_darcy_velocities(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b);

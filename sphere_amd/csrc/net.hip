@@ -1,0 +1,284 @@
+// net.hip -- the Darcy correction network: a four-level 3-D U-Net whose
+// forward and backward passes run on the kernels of net_kernels.hip, behind
+// the sph_net_* entry points (device pointers, caller's stream).
+//
+// Layers (3x3x3 convolutions, ReLU except the last; levels L0 = the Darcy
+// grid, L1..L3 halved by 2x average pooling):
+//   x0 = c0(in)   3 -> 16 @L0      a3 = c4(x3)   64 -> 48 @L3
+//   x1 = c1(pool x0) 16 -> 32 @L1   y2 = up(a3) + x2     @L2
+//   x2 = c2(pool x1) 32 -> 48 @L2   a2 = c5(y2)   48 -> 32 @L2,  y1 = up(a2) + x1
+//   x3 = c3(pool x2) 48 -> 64 @L3   a1 = c6(y1)   32 -> 16 @L1,  y0 = up(a1) + x0
+//   h0 = c7(y0) 16 -> 16 @L0,   out = c8(h0) 16 -> 1 @L0 (no ReLU, FP32)
+// Parameters: per layer w[27][cout][cin] then b[cout], FP32, 285,393 in all.
+#include "net.h"
+#include "net_kernels.h"
+
+#include <string>
+#include <vector>
+
+namespace {
+
+struct LayerSpec {
+    int cin, cout, level;
+};
+
+const LayerSpec kLayers[9] = {{3, 16, 0},  {16, 32, 1}, {32, 48, 2}, {48, 64, 3}, {64, 48, 3},
+                              {48, 32, 2}, {32, 16, 1}, {16, 16, 0}, {16, 1, 0}};
+
+int pad_in(int c) { return c <= 4 ? 4 : (c + 15) / 16 * 16; }
+int pad_out(int c) { return (c + 15) / 16 * 16; }
+
+std::string g_net_error;
+
+}  // namespace
+
+struct sph_net {
+    int device = 0;
+    NetGeom g[4];
+    uint32_t V[4];
+    size_t woff[9], boff[9];
+    uint16_t *wf[9], *wt[9];
+    std::vector<void *> allocs;
+    // activations (BF16, channels-last)
+    uint16_t *in4, *x0, *p0, *x1, *p1, *x2, *p2, *x3, *a3, *y2, *a2, *y1, *a1, *y0, *h0;
+    // gradients (FP32) and the BF16 operand of the weight gradient
+    float *d_h0, *d_y0, *d_a1, *d_y1, *d_a2, *d_y2, *d_a3, *d_x3, *d_p2, *d_p1, *d_p0;
+    uint16_t *dz;
+    float *gw;
+};
+
+static bool net_alloc(sph_net *n, void **p, size_t bytes)
+{
+    if (hipMalloc(p, bytes ? bytes : 16) != hipSuccess) return false;
+    n->allocs.push_back(*p);
+    return true;
+}
+
+extern "C" {
+
+uint64_t sph_net_param_count(void)
+{
+    uint64_t n = 0;
+    for (const LayerSpec &l : kLayers) n += 27ull * l.cout * l.cin + l.cout;
+    return n;
+}
+
+const char *sph_net_last_error(void) { return g_net_error.c_str(); }
+
+void sph_net_destroy(sph_net *n)
+{
+    if (!n) return;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(n->device);
+    for (void *p : n->allocs) (void)hipFree(p);
+    (void)hipSetDevice(prev);
+    delete n;
+}
+
+sph_net *sph_net_create(int device, uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    if (nx == 0 || ny == 0 || nz == 0 || nx % 8 || ny % 8 || nz % 8) {
+        g_net_error = "the network grid must be a positive multiple of 8 along each axis";
+        return nullptr;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        g_net_error = "no HIP device available";
+        return nullptr;
+    }
+    if (device < 0 || device >= count) {
+        g_net_error = "device index out of range";
+        return nullptr;
+    }
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (hipSetDevice(device) != hipSuccess) {
+        g_net_error = "hipSetDevice failed";
+        return nullptr;
+    }
+    sph_net *n = new sph_net();
+    n->device = device;
+    for (int l = 0; l < 4; ++l) {
+        n->g[l] = NetGeom{nx >> l, ny >> l, nz >> l};
+        n->V[l] = n->g[l].nx * n->g[l].ny * n->g[l].nz;
+    }
+    size_t off = 0;
+    bool ok = true;
+    for (int l = 0; l < 9; ++l) {
+        const LayerSpec &s = kLayers[l];
+        n->woff[l] = off;
+        off += 27ull * s.cout * s.cin;
+        n->boff[l] = off;
+        off += s.cout;
+        ok = ok && net_alloc(n, (void **)&n->wf[l], 27ull * pad_out(s.cout) * pad_in(s.cin) * 2);
+        // adjoint: input = this layer's output gradient (cout channels,
+        // padded to 4 or 16), output = cin channels
+        ok = ok && net_alloc(n, (void **)&n->wt[l], 27ull * pad_out(s.cin) * pad_in(s.cout) * 2);
+    }
+    const uint32_t *V = n->V;
+    auto A16 = [&](uint16_t **p, size_t count) { ok = ok && net_alloc(n, (void **)p, count * 2); };
+    auto A32 = [&](float **p, size_t count) { ok = ok && net_alloc(n, (void **)p, count * 4); };
+    A16(&n->in4, (size_t)V[0] * 4);
+    A16(&n->x0, (size_t)V[0] * 16); A16(&n->p0, (size_t)V[1] * 16);
+    A16(&n->x1, (size_t)V[1] * 32); A16(&n->p1, (size_t)V[2] * 32);
+    A16(&n->x2, (size_t)V[2] * 48); A16(&n->p2, (size_t)V[3] * 48);
+    A16(&n->x3, (size_t)V[3] * 64); A16(&n->a3, (size_t)V[3] * 48);
+    A16(&n->y2, (size_t)V[2] * 48); A16(&n->a2, (size_t)V[2] * 32);
+    A16(&n->y1, (size_t)V[1] * 32); A16(&n->a1, (size_t)V[1] * 16);
+    A16(&n->y0, (size_t)V[0] * 16); A16(&n->h0, (size_t)V[0] * 16);
+    A32(&n->d_h0, (size_t)V[0] * 16); A32(&n->d_y0, (size_t)V[0] * 16);
+    A32(&n->d_a1, (size_t)V[1] * 16); A32(&n->d_y1, (size_t)V[1] * 32);
+    A32(&n->d_a2, (size_t)V[2] * 32); A32(&n->d_y2, (size_t)V[2] * 48);
+    A32(&n->d_a3, (size_t)V[3] * 48); A32(&n->d_x3, (size_t)V[3] * 64);
+    A32(&n->d_p2, (size_t)V[3] * 48); A32(&n->d_p1, (size_t)V[2] * 32);
+    A32(&n->d_p0, (size_t)V[1] * 16);
+    A16(&n->dz, (size_t)V[0] * 16);
+    A32(&n->gw, 27ull * 64 * 64);
+    (void)hipSetDevice(prev);
+    if (!ok) {
+        g_net_error = "device allocation failed";
+        sph_net_destroy(n);
+        return nullptr;
+    }
+    return n;
+}
+
+}  // extern "C"
+
+// BF16 operand copies of the FP32 parameters (forward and adjoint layouts)
+void net_pack(sph_net *n, const float *params, hipStream_t s)
+{
+    for (int l = 0; l < 9; ++l) {
+        const LayerSpec &L = kLayers[l];
+        launch_pack_weights(s, params + n->woff[l], L.cin, L.cout, n->wf[l], pad_in(L.cin), pad_out(L.cout),
+                            n->wt[l], pad_in(L.cout), pad_out(L.cin));
+    }
+}
+
+// forward pass with the weights of the last net_pack
+int net_forward_packed(sph_net *n, const float *params, const float *input, float *output, hipStream_t s)
+{
+    const NetGeom *g = n->g;
+    const float *b[9];
+    for (int l = 0; l < 9; ++l) b[l] = params + n->boff[l];
+    int rc = 0;
+    launch_to_bf16_pad(s, n->V[0], input, 3, 4, n->in4);
+    rc |= launch_conv_fwd(s, g[0], n->in4, 4, n->wf[0], 16, b[0], 16, n->x0, NET_OUT_BF16_RELU);
+    launch_pool_fwd(s, g[1], n->x0, 16, n->p0);
+    rc |= launch_conv_fwd(s, g[1], n->p0, 16, n->wf[1], 32, b[1], 32, n->x1, NET_OUT_BF16_RELU);
+    launch_pool_fwd(s, g[2], n->x1, 32, n->p1);
+    rc |= launch_conv_fwd(s, g[2], n->p1, 32, n->wf[2], 48, b[2], 48, n->x2, NET_OUT_BF16_RELU);
+    launch_pool_fwd(s, g[3], n->x2, 48, n->p2);
+    rc |= launch_conv_fwd(s, g[3], n->p2, 48, n->wf[3], 64, b[3], 64, n->x3, NET_OUT_BF16_RELU);
+    rc |= launch_conv_fwd(s, g[3], n->x3, 64, n->wf[4], 48, b[4], 48, n->a3, NET_OUT_BF16_RELU);
+    launch_up_add(s, g[2], n->a3, n->x2, 48, n->y2);
+    rc |= launch_conv_fwd(s, g[2], n->y2, 48, n->wf[5], 32, b[5], 32, n->a2, NET_OUT_BF16_RELU);
+    launch_up_add(s, g[1], n->a2, n->x1, 32, n->y1);
+    rc |= launch_conv_fwd(s, g[1], n->y1, 32, n->wf[6], 16, b[6], 16, n->a1, NET_OUT_BF16_RELU);
+    launch_up_add(s, g[0], n->a1, n->x0, 16, n->y0);
+    rc |= launch_conv_fwd(s, g[0], n->y0, 16, n->wf[7], 16, b[7], 16, n->h0, NET_OUT_BF16_RELU);
+    rc |= launch_conv_fwd(s, g[0], n->h0, 16, n->wf[8], 16, b[8], 1, output, NET_OUT_F32);
+    if (rc) {
+        g_net_error = "unsupported layer shape";
+        return SPH_ERR_VALUE;
+    }
+    if (hipGetLastError() != hipSuccess) {
+        g_net_error = "kernel launch failed";
+        return SPH_ERR_HIP;
+    }
+    return SPH_OK;
+}
+
+extern "C" {
+
+// Forward pass: input[V0][3] FP32 -> output[V0] FP32.  params: FP32 device
+// vector of sph_net_param_count() values.  Keeps the activations for
+// sph_net_backward.
+int sph_net_forward(sph_net *n, const float *params, const float *input, float *output, void *stream)
+{
+    if (!n || !params || !input || !output) {
+        g_net_error = "null argument";
+        return SPH_ERR_ARG;
+    }
+    net_pack(n, params, (hipStream_t)stream);
+    return net_forward_packed(n, params, input, output, (hipStream_t)stream);
+}
+
+// Backward pass of the last forward: grad_output[V0] FP32 (dLoss/dout);
+// adds dLoss/dparams into grad_params (FP32, param_count values).
+int sph_net_backward(sph_net *n, const float *params, const float *grad_output, float *grad_params,
+                     void *stream)
+{
+    if (!n || !params || !grad_output || !grad_params) {
+        g_net_error = "null argument";
+        return SPH_ERR_ARG;
+    }
+    (void)params;  // the adjoint weights were packed by the forward pass
+    hipStream_t s = (hipStream_t)stream;
+    const NetGeom *g = n->g;
+    const uint32_t *V = n->V;
+    int rc = 0;
+    // weight and bias gradient of layer l from input x (cin_x channels) and
+    // the BF16 output gradient in n->dz (cdz channels)
+    auto wgrad = [&](int l, const uint16_t *x, int cin_x, int cdz, int level) {
+        const LayerSpec &L = kLayers[l];
+        launch_fill_f32(s, n->gw, 27ull * cdz * cin_x, 0.f);
+        rc |= launch_conv_wgrad(s, g[level], x, cin_x, n->dz, cdz, n->gw);
+        launch_unpack_wgrad(s, n->gw, L.cin, cin_x, L.cout, cdz, grad_params + n->woff[l]);
+        launch_bias_grad(s, V[level], n->dz, cdz, L.cout, grad_params + n->boff[l]);
+    };
+    // c8: out = conv(h0) (no ReLU)
+    launch_relu_mask(s, V[0], grad_output, nullptr, 1, 4, n->dz);
+    wgrad(8, n->h0, 16, 4, 0);
+    rc |= launch_conv_fwd(s, g[0], n->dz, 4, n->wt[8], 16, nullptr, 16, n->d_h0, NET_OUT_F32);
+    // c7: h0 = relu(conv(y0))
+    launch_relu_mask(s, V[0], n->d_h0, n->h0, 16, 16, n->dz);
+    wgrad(7, n->y0, 16, 16, 0);
+    rc |= launch_conv_fwd(s, g[0], n->dz, 16, n->wt[7], 16, nullptr, 16, n->d_y0, NET_OUT_F32);
+    // y0 = up(a1) + x0: d_y0 is also the gradient of x0 (accumulated below)
+    launch_up_bwd(s, g[1], n->d_y0, 16, n->d_a1);
+    // c6: a1 = relu(conv(y1))
+    launch_relu_mask(s, V[1], n->d_a1, n->a1, 16, 16, n->dz);
+    wgrad(6, n->y1, 32, 16, 1);
+    rc |= launch_conv_fwd(s, g[1], n->dz, 16, n->wt[6], 32, nullptr, 32, n->d_y1, NET_OUT_F32);
+    launch_up_bwd(s, g[2], n->d_y1, 32, n->d_a2);
+    // c5: a2 = relu(conv(y2))
+    launch_relu_mask(s, V[2], n->d_a2, n->a2, 32, 32, n->dz);
+    wgrad(5, n->y2, 48, 32, 2);
+    rc |= launch_conv_fwd(s, g[2], n->dz, 32, n->wt[5], 48, nullptr, 48, n->d_y2, NET_OUT_F32);
+    launch_up_bwd(s, g[3], n->d_y2, 48, n->d_a3);
+    // c4: a3 = relu(conv(x3))
+    launch_relu_mask(s, V[3], n->d_a3, n->a3, 48, 48, n->dz);
+    wgrad(4, n->x3, 64, 48, 3);
+    rc |= launch_conv_fwd(s, g[3], n->dz, 48, n->wt[4], 64, nullptr, 64, n->d_x3, NET_OUT_F32);
+    // c3: x3 = relu(conv(p2))
+    launch_relu_mask(s, V[3], n->d_x3, n->x3, 64, 64, n->dz);
+    wgrad(3, n->p2, 48, 64, 3);
+    rc |= launch_conv_fwd(s, g[3], n->dz, 64, n->wt[3], 48, nullptr, 48, n->d_p2, NET_OUT_F32);
+    launch_pool_bwd(s, g[2], n->d_p2, 48, n->d_y2);  // x2's gradient: skip + pool paths
+    // c2: x2 = relu(conv(p1))
+    launch_relu_mask(s, V[2], n->d_y2, n->x2, 48, 48, n->dz);
+    wgrad(2, n->p1, 32, 48, 2);
+    rc |= launch_conv_fwd(s, g[2], n->dz, 48, n->wt[2], 32, nullptr, 32, n->d_p1, NET_OUT_F32);
+    launch_pool_bwd(s, g[1], n->d_p1, 32, n->d_y1);
+    // c1: x1 = relu(conv(p0))
+    launch_relu_mask(s, V[1], n->d_y1, n->x1, 32, 32, n->dz);
+    wgrad(1, n->p0, 16, 32, 1);
+    rc |= launch_conv_fwd(s, g[1], n->dz, 32, n->wt[1], 16, nullptr, 16, n->d_p0, NET_OUT_F32);
+    launch_pool_bwd(s, g[0], n->d_p0, 16, n->d_y0);
+    // c0: x0 = relu(conv(in))
+    launch_relu_mask(s, V[0], n->d_y0, n->x0, 16, 16, n->dz);
+    wgrad(0, n->in4, 4, 16, 0);
+    if (rc) {
+        g_net_error = "unsupported layer shape";
+        return SPH_ERR_VALUE;
+    }
+    if (hipGetLastError() != hipSuccess) {
+        g_net_error = "kernel launch failed";
+        return SPH_ERR_HIP;
+    }
+    return SPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+// net_kernels.h -- launchers of the Darcy correction network's kernels
+// (net_kernels.hip).  Device pointers; every launcher enqueues on stream s.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+struct NetGeom {
+    uint32_t nx, ny, nz;
+};
+
+#define NET_OUT_BF16_RELU 0  // out = bf16(relu(acc + bias))
+#define NET_OUT_F32 1        // out = acc + bias
+#define NET_OUT_F32_ACC 2    // out += acc + bias
+
+// in[V][cin] (cin 4 or a multiple of 16), w[27][cout_pad][cin]; out has
+// cout_real channels.  Returns -1 for an unsupported shape.
+int launch_conv_fwd(hipStream_t s, const NetGeom &g, const uint16_t *in, int cin, const uint16_t *w,
+                    int cout_pad, const float *bias, int cout_real, void *out, int mode);
+// dw[27][cout][cin] += sum_v dz[v][co] x[v + tap][ci]; x[V][cin] (cin 4 or
+// a multiple of 16), dz[V][cout]
+int launch_conv_wgrad(hipStream_t s, const NetGeom &g, const uint16_t *x, int cin, const uint16_t *dz,
+                      int cout, float *dw);
+void launch_bias_grad(hipStream_t s, uint32_t V, const uint16_t *dz, int c_stride, int c_real, float *db);
+void launch_pool_fwd(hipStream_t s, const NetGeom &gc, const uint16_t *in, int c, uint16_t *out);
+void launch_pool_bwd(hipStream_t s, const NetGeom &gf, const float *dcoarse, int c, float *dfine);
+void launch_up_add(hipStream_t s, const NetGeom &gf, const uint16_t *coarse, const uint16_t *skip, int c,
+                   uint16_t *out);
+void launch_up_bwd(hipStream_t s, const NetGeom &gc, const float *dfine, int c, float *dcoarse);
+void launch_relu_mask(hipStream_t s, uint32_t V, const float *dy, const uint16_t *y, int c, int cpad,
+                      uint16_t *dz);
+void launch_to_bf16_pad(hipStream_t s, uint32_t V, const float *in, int c, int cpad, uint16_t *out);
+void launch_pack_weights(hipStream_t s, const float *w, int cin, int cout, uint16_t *wf, int cin_pad,
+                         int cout_pad, uint16_t *wt, int tin_pad, int tout_pad);
+void launch_unpack_wgrad(hipStream_t s, const float *gw, int cin, int cin_pad, int cout, int cout_pad,
+                         float *g);
+void launch_fill_f32(hipStream_t s, float *p, size_t n, float v);

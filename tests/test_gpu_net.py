@@ -1,0 +1,131 @@
+"""GPU checks of the Darcy correction network and its training-data path.
+
+* The network's forward and backward kernels (BF16 matrix-core operands,
+  FP32 accumulation) against the FP64 oracle (oracle/darcy_net.py) run with
+  the same BF16 roundings of weights, input and activations.  Tolerance:
+  BF16 keeps 8 significant bits (relative spacing 2^-7), so an activation
+  that lands on the other side of a rounding tie moves by up to 0.8%; the
+  test allows 2% of the output's largest magnitude anywhere, 0.2% on average,
+  and 3% (relative Frobenius norm) per layer of the parameter gradient.
+* The training-data entry points: free-running sweeps from the start of a
+  solve reproduce the solve bit for bit, and the network input is the
+  normalized fixed-point residual.
+"""
+import os
+import sys
+
+import numpy
+import pytest
+
+import sphere_amd
+from _cases import darcy_case
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import darcy_net as N  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.skip("no GPU")
+    return t
+
+
+def _net_case(shape, seed):
+    rng = numpy.random.default_rng(seed)
+    params = N.init_params(rng)
+    # small non-zero biases exercise the bias paths
+    for w, b in N.split_params(params):
+        b[:] = rng.uniform(-0.05, 0.1, b.shape)
+    inp = rng.standard_normal(shape + (3,))
+    inp[..., 2] = (rng.random(shape) < 0.2).astype(numpy.float64)
+    return params, inp, rng
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (24, 16, 8)])
+def test_net_forward_backward_match_oracle(torch, shape):
+    nz, ny, nx = shape
+    params, inp, rng = _net_case(shape, 11)
+    # the kernels see FP32 parameters and input
+    params = params.astype(numpy.float32).astype(numpy.float64)
+    inp = inp.astype(numpy.float32).astype(numpy.float64)
+    print("net case", shape, flush=True)
+    net = sphere_amd.Net(nx, ny, nz)
+    dev = torch.device("cuda:0")
+    p_t = torch.tensor(params, dtype=torch.float32, device=dev)
+    in_t = torch.tensor(inp.reshape(-1, 3), dtype=torch.float32, device=dev)
+    out_t = torch.empty(nx * ny * nz, dtype=torch.float32, device=dev)
+    net.forward(p_t, in_t, out_t)
+    torch.cuda.synchronize()
+    out_g = out_t.cpu().numpy().astype(numpy.float64).reshape(shape)
+    ref, cache = N.forward(params, inp, bf16=True)
+    scale = numpy.max(numpy.abs(ref))
+    assert scale > 0
+    err = numpy.abs(out_g - ref)
+    assert err.max() <= 2e-2 * scale, (err.max(), scale)
+    assert err.mean() <= 2e-3 * scale, (err.mean(), scale)
+
+    R = rng.standard_normal(shape)
+    g_t = torch.zeros(params.size, dtype=torch.float32, device=dev)
+    net.backward(p_t, torch.tensor(R.ravel(), dtype=torch.float32, device=dev), g_t)
+    torch.cuda.synchronize()
+    g_gpu = g_t.cpu().numpy().astype(numpy.float64)
+    g_ref = N.backward(cache, R)
+    off = 0
+    for k, (ci, co) in enumerate(N.LAYERS):
+        for n in (27 * co * ci, co):
+            a, b = g_gpu[off:off + n], g_ref[off:off + n]
+            off += n
+            den = numpy.linalg.norm(b)
+            assert den > 0, k
+            assert numpy.linalg.norm(a - b) <= 3e-2 * den, (k, n, numpy.linalg.norm(a - b) / den)
+    # backward accumulates
+    net.backward(p_t, torch.tensor(R.ravel(), dtype=torch.float32, device=dev), g_t)
+    torch.cuda.synchronize()
+    assert numpy.allclose(g_t.cpu().numpy(), 2 * g_gpu, rtol=1e-5, atol=1e-6 * numpy.abs(g_gpu).max())
+    net.close()
+
+
+def test_net_rejects_bad_grid():
+    with pytest.raises(sphere_amd.SphereError):
+        sphere_amd.Net(12, 16, 16)
+
+
+def test_darcy_training_path(torch):
+    s = darcy_case(n=400, seed=5, nxyz=(8, 8, 16))
+    e = sphere_amd.Engine(0)
+    e.load(s.to_state())
+    e.step(3)
+    st = e.darcy_stats()
+    ran = st.last_nijac + 1
+    assert st.last_max_residual <= 1e-8
+    h = e.header()
+    nx, ny, nz = h.num[0], h.num[1], h.num[2]
+    cells = (nx + 2) * (ny + 2) * (nz + 2)
+    dev = torch.device("cuda:0")
+    p0 = torch.zeros(cells, dtype=torch.float64, device=dev)
+    pf = torch.zeros(cells, dtype=torch.float64, device=dev)
+    e.darcy_export(0, p0)
+    e.darcy_export(1, pf)
+
+    def interior(t):
+        return t.reshape(nz + 2, ny + 2, nx + 2)[1:-1, 1:-1, 1:-1].cpu().numpy()
+
+    pk = p0.clone()
+    e.darcy_sweeps(pk, ran)
+    assert numpy.array_equal(interior(pk), interior(pf))
+    inp = torch.zeros((nx * ny * nz, 3), dtype=torch.float32, device=dev)
+    sc = e.darcy_net_input(p0, inp)
+    x = inp.cpu().numpy()
+    assert sc > 0.0
+    assert numpy.max(numpy.abs(x[:, 0])) == pytest.approx(1.0, rel=1e-6)
+    assert set(numpy.unique(x[:, 2])) <= {0.0, 1.0}
+    assert numpy.all(x[:, 1] >= 0.0)
+    # at the converged iterate the residual is tiny against the start's
+    sf = e.darcy_net_input(pf, inp)
+    assert sf < 1e-3 * sc
+    e.close()
