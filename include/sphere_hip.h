@@ -257,8 +257,10 @@ typedef struct sph_net sph_net;
 /* Number of FP32 parameters of the network (per layer w[27][cout][cin],
  * then b[cout]). */
 uint64_t sph_net_param_count(void);
-/* Network for an nx*ny*nz grid (each a multiple of 8) on a device. */
-sph_net *sph_net_create(int device, uint32_t nx, uint32_t ny, uint32_t nz);
+/* Network for an nx*ny*nz grid (each a multiple of 8) on a device;
+ * periodic: bit 0 x, bit 1 y wrap around (else zero padding). */
+sph_net *sph_net_create(int device, uint32_t nx, uint32_t ny, uint32_t nz,
+                        uint32_t periodic);
 void sph_net_destroy(sph_net *net);
 const char *sph_net_last_error(void);
 /* Device pointers; stream is a hipStream_t (NULL: the null stream).
@@ -270,6 +272,13 @@ int sph_net_forward(sph_net *net, const float *params, const float *input,
 int sph_net_backward(sph_net *net, const float *params, const float *grad_output,
                      float *grad_params, void *stream);
 
+/* One of the network's 3x3x3 convolutions on its own (for tests):
+ * in[V][cin] and out[V][cout] FP32, w[27][cout][cin], b[cout] or NULL;
+ * operands rounded to BF16, FP32 sums; device pointers. */
+int sph_net_conv3d(const float *in, int cin, const float *w, const float *b, int cout,
+                   uint32_t nx, uint32_t ny, uint32_t nz, uint32_t periodic, float *out,
+                   void *stream);
+
 /* Use the network in the context's Darcy solves: params on the host
  * (count = sph_net_param_count()), at most max_corrections per solve;
  * params == NULL or max_corrections == 0 switches it off. */
@@ -277,7 +286,9 @@ int sph_set_darcy_net(sph_ctx *ctx, const float *params, uint64_t count,
                       int max_corrections);
 
 /* Training data from the last Darcy solve of the context (device
- * pointers; fields with the ghost layer, (nx+2)(ny+2)(nz+2) doubles). */
+ * pointers; fields with the ghost layer, (nx+2)(ny+2)(nz+2) doubles).  They
+ * run on the context's stream and return when done; the caller finishes
+ * its own pending work on the buffers first. */
 #define SPH_DARCY_P_START 0   /* the iterate the solve started from (p_old) */
 #define SPH_DARCY_P_FINAL 1   /* the converged pressure                     */
 int sph_darcy_export(sph_ctx *ctx, int which, double *dst);

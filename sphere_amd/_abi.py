@@ -188,11 +188,14 @@ def load_library(path=None):
         "sph_kernel_time": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_uint64)]),
         "sph_net_param_count": (ctypes.c_uint64, []),
-        "sph_net_create": (vp, [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+        "sph_net_create": (vp, [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_uint32]),
         "sph_net_destroy": (None, [vp]),
         "sph_net_last_error": (ctypes.c_char_p, []),
         "sph_net_forward": (ctypes.c_int, [vp, vp, vp, vp, vp]),
         "sph_net_backward": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+        "sph_net_conv3d": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_uint32,
+                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp]),
         "sph_set_darcy_net": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_uint64,
                                              ctypes.c_int]),
         "sph_darcy_export": (ctypes.c_int, [vp, ctypes.c_int, vp]),
@@ -329,17 +332,27 @@ class Engine:
         self._check(self.lib.sph_set_darcy_net(self.ctx, _ptr(p, ctypes.c_float), p.size,
                                                int(max_corrections)))
 
+    # The context runs on its own stream and returns when its work is done;
+    # torch's pending work on the buffers is finished first.
+    @staticmethod
+    def _torch_done():
+        import torch
+        torch.cuda.current_stream().synchronize()
+
     def darcy_export(self, which, dst):
         """Copy the last solve's start (0) or final (1) pressure, ghost layer
         included, into the float64 CUDA tensor dst."""
+        self._torch_done()
         self._check(self.lib.sph_darcy_export(self.ctx, int(which), ctypes.c_void_p(dst.data_ptr())))
 
     def darcy_sweeps(self, p, n):
+        self._torch_done()
         self._check(self.lib.sph_darcy_sweeps(self.ctx, ctypes.c_void_p(p.data_ptr()), int(n)))
 
     def darcy_net_input(self, p, inp):
         """Network input for iterate p (float64 CUDA tensor with ghosts) into
         the float32 CUDA tensor inp [V, 3]; returns the residual scale."""
+        self._torch_done()
         s = ctypes.c_double(0)
         self._check(self.lib.sph_darcy_net_input(self.ctx, ctypes.c_void_p(p.data_ptr()),
                                                  ctypes.c_void_p(inp.data_ptr()), ctypes.byref(s)))
@@ -356,9 +369,10 @@ class Net:
     """The Darcy correction network (sph_net_*) on one device.  Buffers are
     torch CUDA tensors; calls run on torch's current stream."""
 
-    def __init__(self, nx, ny, nz, device=0, lib=None):
+    def __init__(self, nx, ny, nz, periodic=0, device=0, lib=None):
+        """periodic: bit 0 x, bit 1 y wrap around."""
         self.lib = lib or load_library()
-        self.h = self.lib.sph_net_create(int(device), int(nx), int(ny), int(nz))
+        self.h = self.lib.sph_net_create(int(device), int(nx), int(ny), int(nz), int(periodic))
         if not self.h:
             raise SphereError(-3, self.lib.sph_net_last_error().decode())
         self.shape = (int(nz), int(ny), int(nx))
@@ -384,6 +398,23 @@ class Net:
         self._check(self.lib.sph_net_backward(self.h, ctypes.c_void_p(params.data_ptr()),
                                               ctypes.c_void_p(grad_out.data_ptr()),
                                               ctypes.c_void_p(grad_params.data_ptr()), self._stream()))
+
+    @staticmethod
+    def conv3d(inp, w, b, out, shape, periodic=0, lib=None):
+        """One convolution of the network's kernels (torch CUDA tensors):
+        inp [V, cin], w [27, cout, cin], b [cout] or None, out [V, cout]."""
+        import torch
+        lib = lib or load_library()
+        nz, ny, nx = shape
+        torch.cuda.current_stream().synchronize()
+        rc = lib.sph_net_conv3d(ctypes.c_void_p(inp.data_ptr()), int(inp.shape[1]),
+                                ctypes.c_void_p(w.data_ptr()),
+                                ctypes.c_void_p(b.data_ptr()) if b is not None else None,
+                                int(w.shape[1]), nx, ny, nz, int(periodic),
+                                ctypes.c_void_p(out.data_ptr()),
+                                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        if rc != SPH_OK:
+            raise SphereError(rc, lib.sph_net_last_error().decode())
 
     def close(self):
         if self.h:

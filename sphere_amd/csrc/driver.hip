@@ -707,7 +707,8 @@ static int net_ready(sph_ctx *ctx)
 {
     if (ctx->net || ctx->net_host.empty() || ctx->net_max_corr <= 0) return SPH_OK;
     const sph_state &h = ctx->hs.h;
-    ctx->net = sph_net_create(ctx->device, h.num[0], h.num[1], h.num[2]);
+    const uint32_t periodic = (h.bc_xn == 2 && h.bc_xp == 2 ? 1u : 0u) | (h.bc_yn == 2 && h.bc_yp == 2 ? 2u : 0u);
+    ctx->net = sph_net_create(ctx->device, h.num[0], h.num[1], h.num[2], periodic);
     if (!ctx->net) return fail(ctx, SPH_ERR_VALUE, std::string("Darcy network: ") + sph_net_last_error());
     const size_t V = (size_t)h.num[0] * h.num[1] * h.num[2];
     if (hipMalloc((void **)&ctx->net_params, ctx->net_host.size() * 4) != hipSuccess ||

@@ -2,14 +2,16 @@
 // forward and backward passes run on the kernels of net_kernels.hip, behind
 // the sph_net_* entry points (device pointers, caller's stream).
 //
-// Layers (3x3x3 convolutions, ReLU except the last; levels L0 = the Darcy
-// grid, L1..L3 halved by 2x average pooling):
+// Layers (3x3x3 convolutions, zero padding or wrap-around along periodic
+// axes, ReLU except the last; levels L0 = the Darcy grid, L1..L3 halved by
+// 2x average pooling):
 //   x0 = c0(in)   3 -> 16 @L0      a3 = c4(x3)   64 -> 48 @L3
 //   x1 = c1(pool x0) 16 -> 32 @L1   y2 = up(a3) + x2     @L2
 //   x2 = c2(pool x1) 32 -> 48 @L2   a2 = c5(y2)   48 -> 32 @L2,  y1 = up(a2) + x1
 //   x3 = c3(pool x2) 48 -> 64 @L3   a1 = c6(y1)   32 -> 16 @L1,  y0 = up(a1) + x0
-//   h0 = c7(y0) 16 -> 16 @L0,   out = c8(h0) 16 -> 1 @L0 (no ReLU, FP32)
-// Parameters: per layer w[27][cout][cin] then b[cout], FP32, 285,393 in all.
+//   h0 = c7(y0) 16 -> 16 @L0,   out = c8 . h0 + b8 (pointwise 16 -> 1, FP32)
+// Parameters: per layer w[taps][cout][cin] then b[cout] (taps 27, 1 for
+// c8), FP32, 284,977 in all.
 #include "net.h"
 #include "net_kernels.h"
 
@@ -19,11 +21,12 @@
 namespace {
 
 struct LayerSpec {
-    int cin, cout, level;
+    int cin, cout, level, taps;
 };
 
-const LayerSpec kLayers[9] = {{3, 16, 0},  {16, 32, 1}, {32, 48, 2}, {48, 64, 3}, {64, 48, 3},
-                              {48, 32, 2}, {32, 16, 1}, {16, 16, 0}, {16, 1, 0}};
+const LayerSpec kLayers[9] = {{3, 16, 0, 27},  {16, 32, 1, 27}, {32, 48, 2, 27}, {48, 64, 3, 27},
+                              {64, 48, 3, 27}, {48, 32, 2, 27}, {32, 16, 1, 27}, {16, 16, 0, 27},
+                              {16, 1, 0, 1}};
 
 int pad_in(int c) { return c <= 4 ? 4 : (c + 15) / 16 * 16; }
 int pad_out(int c) { return (c + 15) / 16 * 16; }
@@ -59,7 +62,7 @@ extern "C" {
 uint64_t sph_net_param_count(void)
 {
     uint64_t n = 0;
-    for (const LayerSpec &l : kLayers) n += 27ull * l.cout * l.cin + l.cout;
+    for (const LayerSpec &l : kLayers) n += (uint64_t)l.taps * l.cout * l.cin + l.cout;
     return n;
 }
 
@@ -76,7 +79,7 @@ void sph_net_destroy(sph_net *n)
     delete n;
 }
 
-sph_net *sph_net_create(int device, uint32_t nx, uint32_t ny, uint32_t nz)
+sph_net *sph_net_create(int device, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t periodic)
 {
     if (nx == 0 || ny == 0 || nz == 0 || nx % 8 || ny % 8 || nz % 8) {
         g_net_error = "the network grid must be a positive multiple of 8 along each axis";
@@ -100,7 +103,7 @@ sph_net *sph_net_create(int device, uint32_t nx, uint32_t ny, uint32_t nz)
     sph_net *n = new sph_net();
     n->device = device;
     for (int l = 0; l < 4; ++l) {
-        n->g[l] = NetGeom{nx >> l, ny >> l, nz >> l};
+        n->g[l] = NetGeom{nx >> l, ny >> l, nz >> l, periodic & 3u};
         n->V[l] = n->g[l].nx * n->g[l].ny * n->g[l].nz;
     }
     size_t off = 0;
@@ -108,9 +111,11 @@ sph_net *sph_net_create(int device, uint32_t nx, uint32_t ny, uint32_t nz)
     for (int l = 0; l < 9; ++l) {
         const LayerSpec &s = kLayers[l];
         n->woff[l] = off;
-        off += 27ull * s.cout * s.cin;
+        off += (size_t)s.taps * s.cout * s.cin;
         n->boff[l] = off;
         off += s.cout;
+        n->wf[l] = n->wt[l] = nullptr;
+        if (s.taps != 27) continue;  // the pointwise head reads its FP32 weights
         ok = ok && net_alloc(n, (void **)&n->wf[l], 27ull * pad_out(s.cout) * pad_in(s.cin) * 2);
         // adjoint: input = this layer's output gradient (cout channels,
         // padded to 4 or 16), output = cin channels
@@ -151,6 +156,7 @@ void net_pack(sph_net *n, const float *params, hipStream_t s)
 {
     for (int l = 0; l < 9; ++l) {
         const LayerSpec &L = kLayers[l];
+        if (L.taps != 27) continue;
         launch_pack_weights(s, params + n->woff[l], L.cin, L.cout, n->wf[l], pad_in(L.cin), pad_out(L.cout),
                             n->wt[l], pad_in(L.cout), pad_out(L.cin));
     }
@@ -171,14 +177,16 @@ int net_forward_packed(sph_net *n, const float *params, const float *input, floa
     rc |= launch_conv_fwd(s, g[2], n->p1, 32, n->wf[2], 48, b[2], 48, n->x2, NET_OUT_BF16_RELU);
     launch_pool_fwd(s, g[3], n->x2, 48, n->p2);
     rc |= launch_conv_fwd(s, g[3], n->p2, 48, n->wf[3], 64, b[3], 64, n->x3, NET_OUT_BF16_RELU);
-    rc |= launch_conv_fwd(s, g[3], n->x3, 64, n->wf[4], 48, b[4], 48, n->a3, NET_OUT_BF16_RELU);
-    launch_up_add(s, g[2], n->a3, n->x2, 48, n->y2);
-    rc |= launch_conv_fwd(s, g[2], n->y2, 48, n->wf[5], 32, b[5], 32, n->a2, NET_OUT_BF16_RELU);
-    launch_up_add(s, g[1], n->a2, n->x1, 32, n->y1);
-    rc |= launch_conv_fwd(s, g[1], n->y1, 32, n->wf[6], 16, b[6], 16, n->a1, NET_OUT_BF16_RELU);
-    launch_up_add(s, g[0], n->a1, n->x0, 16, n->y0);
-    rc |= launch_conv_fwd(s, g[0], n->y0, 16, n->wf[7], 16, b[7], 16, n->h0, NET_OUT_BF16_RELU);
-    rc |= launch_conv_fwd(s, g[0], n->h0, 16, n->wf[8], 16, b[8], 1, output, NET_OUT_F32);
+    // decoder convolutions also write up(a) + skip on the finer grid
+    rc |= launch_conv_fwd(s, g[3], n->x3, 64, n->wf[4], 48, b[4], 48, n->a3, NET_OUT_RELU_UP, nullptr,
+                          nullptr, n->x2, n->y2);
+    rc |= launch_conv_fwd(s, g[2], n->y2, 48, n->wf[5], 32, b[5], 32, n->a2, NET_OUT_RELU_UP, nullptr,
+                          nullptr, n->x1, n->y1);
+    rc |= launch_conv_fwd(s, g[1], n->y1, 32, n->wf[6], 16, b[6], 16, n->a1, NET_OUT_RELU_UP, nullptr,
+                          nullptr, n->x0, n->y0);
+    // c7 with the pointwise head c8 (weights then bias, contiguous) in its epilogue
+    rc |= launch_conv_fwd(s, g[0], n->y0, 16, n->wf[7], 16, b[7], 16, n->h0, NET_OUT_PROJ,
+                          params + n->woff[8], output);
     if (rc) {
         g_net_error = "unsupported layer shape";
         return SPH_ERR_VALUE;
@@ -214,7 +222,7 @@ int sph_net_backward(sph_net *n, const float *params, const float *grad_output, 
         g_net_error = "null argument";
         return SPH_ERR_ARG;
     }
-    (void)params;  // the adjoint weights were packed by the forward pass
+    // the adjoint weights were packed by the forward pass
     hipStream_t s = (hipStream_t)stream;
     const NetGeom *g = n->g;
     const uint32_t *V = n->V;
@@ -228,10 +236,8 @@ int sph_net_backward(sph_net *n, const float *params, const float *grad_output, 
         launch_unpack_wgrad(s, n->gw, L.cin, cin_x, L.cout, cdz, grad_params + n->woff[l]);
         launch_bias_grad(s, V[level], n->dz, cdz, L.cout, grad_params + n->boff[l]);
     };
-    // c8: out = conv(h0) (no ReLU)
-    launch_relu_mask(s, V[0], grad_output, nullptr, 1, 4, n->dz);
-    wgrad(8, n->h0, 16, 4, 0);
-    rc |= launch_conv_fwd(s, g[0], n->dz, 4, n->wt[8], 16, nullptr, 16, n->d_h0, NET_OUT_F32);
+    // c8: out = w8 . h0 + b8 (weights and bias contiguous)
+    launch_proj_bwd(s, V[0], grad_output, n->h0, params + n->woff[8], n->d_h0, grad_params + n->woff[8]);
     // c7: h0 = relu(conv(y0))
     launch_relu_mask(s, V[0], n->d_h0, n->h0, 16, 16, n->dz);
     wgrad(7, n->y0, 16, 16, 0);
@@ -277,6 +283,46 @@ int sph_net_backward(sph_net *n, const float *params, const float *grad_output, 
     if (hipGetLastError() != hipSuccess) {
         g_net_error = "kernel launch failed";
         return SPH_ERR_HIP;
+    }
+    return SPH_OK;
+}
+
+// One 3x3x3 convolution as the network runs it, for tests: in[V][cin]
+// FP32 (cin 3 or 4: padded to 4 channels; else a multiple of 16), weights
+// w[27][cout][cin] FP32, b[cout] or NULL, out[V][cout] FP32 (cout a
+// multiple of 16 or less than 16).  Operands are rounded to BF16, sums are
+// FP32.  Device pointers; allocates and frees its own temporaries.
+int sph_net_conv3d(const float *in, int cin, const float *w, const float *b, int cout, uint32_t nx,
+                   uint32_t ny, uint32_t nz, uint32_t periodic, float *out, void *stream)
+{
+    if (!in || !w || !out || cin <= 0 || cout <= 0 || cout > 64 || nx == 0 || ny == 0 || nz == 0) {
+        g_net_error = "bad argument";
+        return SPH_ERR_ARG;
+    }
+    const int cpad = pad_in(cin), opad = pad_out(cout);
+    if (cpad > 64) {
+        g_net_error = "too many input channels";
+        return SPH_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const NetGeom g{nx, ny, nz, periodic & 3u};
+    const uint32_t V = nx * ny * nz;
+    uint16_t *inb = nullptr, *wf = nullptr;
+    if (hipMalloc((void **)&inb, (size_t)V * cpad * 2) != hipSuccess ||
+        hipMalloc((void **)&wf, 27ull * opad * cpad * 2) != hipSuccess) {
+        (void)hipFree(inb);
+        g_net_error = "device allocation failed";
+        return SPH_ERR_HIP;
+    }
+    launch_to_bf16_pad(s, V, in, cin, cpad, inb);
+    launch_pack_weights(s, w, cin, cout, wf, cpad, opad, nullptr, 0, 0);
+    int rc = launch_conv_fwd(s, g, inb, cpad, wf, opad, b, cout, out, NET_OUT_F32);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(inb);
+    (void)hipFree(wf);
+    if (rc) {
+        g_net_error = "unsupported shape";
+        return SPH_ERR_VALUE;
     }
     return SPH_OK;
 }

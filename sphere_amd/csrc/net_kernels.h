@@ -7,16 +7,25 @@
 
 struct NetGeom {
     uint32_t nx, ny, nz;
+    uint32_t wrap;  // bit 0: x periodic, bit 1: y periodic (else zero padding)
 };
 
 #define NET_OUT_BF16_RELU 0  // out = bf16(relu(acc + bias))
 #define NET_OUT_F32 1        // out = acc + bias
 #define NET_OUT_F32_ACC 2    // out += acc + bias
+#define NET_OUT_PROJ 3       // 16 channels: out = bf16(relu(acc + bias)) and
+                             // out2[v] = sum_c out[v][c] proj[c] + proj[16]
+#define NET_OUT_RELU_UP 4    // out = bf16(relu(acc + bias)) and, on the grid
+                             // twice as fine, yfine = bf16(up(out) + skip)
 
 // in[V][cin] (cin 4 or a multiple of 16), w[27][cout_pad][cin]; out has
 // cout_real channels.  Returns -1 for an unsupported shape.
 int launch_conv_fwd(hipStream_t s, const NetGeom &g, const uint16_t *in, int cin, const uint16_t *w,
-                    int cout_pad, const float *bias, int cout_real, void *out, int mode);
+                    int cout_pad, const float *bias, int cout_real, void *out, int mode,
+                    const float *proj = nullptr, float *out2 = nullptr,
+                    const uint16_t *skip = nullptr, uint16_t *yfine = nullptr);
+void launch_proj_bwd(hipStream_t s, uint32_t V, const float *dout, const uint16_t *h, const float *w, float *dh,
+                     float *g);
 // dw[27][cout][cin] += sum_v dz[v][co] x[v + tap][ci]; x[V][cin] (cin 4 or
 // a multiple of 16), dz[V][cout]
 int launch_conv_wgrad(hipStream_t s, const NetGeom &g, const uint16_t *x, int cin, const uint16_t *dz,
@@ -24,8 +33,6 @@ int launch_conv_wgrad(hipStream_t s, const NetGeom &g, const uint16_t *x, int ci
 void launch_bias_grad(hipStream_t s, uint32_t V, const uint16_t *dz, int c_stride, int c_real, float *db);
 void launch_pool_fwd(hipStream_t s, const NetGeom &gc, const uint16_t *in, int c, uint16_t *out);
 void launch_pool_bwd(hipStream_t s, const NetGeom &gf, const float *dcoarse, int c, float *dfine);
-void launch_up_add(hipStream_t s, const NetGeom &gf, const uint16_t *coarse, const uint16_t *skip, int c,
-                   uint16_t *out);
 void launch_up_bwd(hipStream_t s, const NetGeom &gc, const float *dfine, int c, float *dcoarse);
 void launch_relu_mask(hipStream_t s, uint32_t V, const float *dy, const uint16_t *y, int c, int cpad,
                       uint16_t *dz);

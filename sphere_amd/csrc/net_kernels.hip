@@ -29,6 +29,16 @@ __device__ __forceinline__ uint16_t f2bf(float f)
     return (uint16_t)(u >> 16);
 }
 
+// neighbour (x, y, z) of the 3x3x3 stencil: wrapped along periodic axes,
+// false where it falls in the zero padding
+__device__ __forceinline__ bool nb_in(const NetGeom &g, int &x, int &y, int z)
+{
+    const int nx = (int)g.nx, ny = (int)g.ny;
+    if (g.wrap & 1u) x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+    if (g.wrap & 2u) y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+    return x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < (int)g.nz;
+}
+
 __device__ __forceinline__ void vox_xyz(const NetGeom &g, uint32_t v, int &x, int &y, int &z)
 {
     x = (int)(v % g.nx);
@@ -38,97 +48,56 @@ __device__ __forceinline__ void vox_xyz(const NetGeom &g, uint32_t v, int &x, in
 }
 
 // ---------------------------------------------------------------------------
-// Forward convolution.  One wave computes 64 voxels (four 16-voxel M tiles)
-// by all 16*NT output channels; K runs over taps and 16-channel chunks of
-// the input (CIN4: inputs of 4 channels, four taps per K step, one per
-// 16-lane group).  MFMA 16x16x16 lane layout: A[m][k] with m = lane % 16,
+// Forward convolution.  A wave computes MT 16-voxel M tiles by all 16*NT
+// output channels; K runs over steps = (tap, 16-channel chunk of the input)
+// (CIN4: inputs of 4 channels, four taps per step, one per 16-lane group).
+// With KS > 1 the KS waves of a block share the M tiles and take every
+// KS-th step, and their partial sums meet in LDS (small grids: more waves,
+// shorter chains).  Each wave issues the loads of G steps before their
+// MFMAs.  MFMA 16x16x16 lane layout: A[m][k] with m = lane % 16,
 // k = 4 (lane / 16) + 0..3; B[k][n] with n = lane % 16, k likewise; the
 // accumulator holds D[4 (lane / 16) + i][lane % 16].
 // ---------------------------------------------------------------------------
-template <int NT, bool CIN4>
-__global__ void __launch_bounds__(256) k_conv_fwd(NetGeom g, const uint16_t *__restrict__ in, int cin,
-                                                  const uint16_t *__restrict__ w,
-                                                  const float *__restrict__ bias, int cout_real,
-                                                  void *__restrict__ out, int mode)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
-    const uint32_t V = g.nx * g.ny * g.nz;
-    const uint32_t v0 = wave * 64u;
-    if (v0 >= V) return;
-    const uint32_t m = lane & 15u, grp = lane >> 4;
-    const int cout = 16 * NT;
-    int vx[4], vy[4], vz[4];
-    bool vin[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const uint32_t v = v0 + 16u * j + m;
-        vin[j] = v < V;
-        vox_xyz(g, vin[j] ? v : 0u, vx[j], vy[j], vz[j]);
-    }
-    f32x4 acc[4][NT];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[j][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#define CONV_G 4
 
-    if (CIN4) {
-        // K step s covers taps 4s .. 4s+3 (tap 27 is padding), 4 channels each
-        for (int s = 0; s < 7; ++s) {
-            const int tap = 4 * s + (int)grp;
-            const bool tok = tap < 27;
-            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-            bf16x4 b[NT];
+// Epilogue of the convolutions: acc[j][t] holds, for M tile j (16
+// voxels from vbase[j] along x), voxels vbase[j] + 4 grp + i (i = 0..3) and
+// output channel 16 t + m.
+template <int NT, int MT>
+__device__ __forceinline__ void conv_store(const f32x4 (&acc)[MT][NT], const uint32_t (&vbase)[MT], uint32_t V,
+                                           uint32_t m, uint32_t grp, const float *__restrict__ bias,
+                                           int cout_real, void *__restrict__ out, int mode,
+                                           const float *__restrict__ proj, float *__restrict__ out2,
+                                           const NetGeom &g, const uint16_t *__restrict__ skip,
+                                           uint16_t *__restrict__ yfine)
+{
+    if (NT == 1 && mode == NET_OUT_PROJ) {
+        // h = bf16(relu(acc + bias)) stored, and out2[v] = sum_c h[v][c]
+        // proj[c] + proj[16]: the 16 channels of a voxel sit in the 16 lanes
+        // of one lane group, summed by xor shuffles within the group
+        const float bv = bias[m];
+        const float wp = proj[m];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                b[t] = bf16x4{0, 0, 0, 0};
-                if (tok) b[t] = *(const bf16x4 *)(w + ((size_t)(tap * cout + t * 16 + (int)m) * 4));
-            }
+        for (int j = 0; j < MT; ++j)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int x = vx[j] + dx, y = vy[j] + dy, z = vz[j] + dz;
-                bf16x4 a = bf16x4{0, 0, 0, 0};
-                if (tok && vin[j] && x >= 0 && x < (int)g.nx && y >= 0 && y < (int)g.ny && z >= 0 &&
-                    z < (int)g.nz)
-                    a = *(const bf16x4 *)(in + ((size_t)((z * (int)g.ny + y) * (int)g.nx + x) * 4));
-#pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc[j][t], 0, 0, 0);
-            }
-        }
-    } else {
-        const int chunks = cin >> 4;
-        for (int tap = 0; tap < 27; ++tap) {
-            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-            size_t nb[4];
-            bool ok[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int x = vx[j] + dx, y = vy[j] + dy, z = vz[j] + dz;
-                ok[j] = vin[j] && x >= 0 && x < (int)g.nx && y >= 0 && y < (int)g.ny && z >= 0 &&
-                        z < (int)g.nz;
-                nb[j] = ok[j] ? (size_t)((z * (int)g.ny + y) * (int)g.nx + x) * (size_t)cin : 0;
-            }
-            for (int c = 0; c < chunks; ++c) {
-                const int ci = c * 16 + 4 * (int)grp;
-                bf16x4 b[NT];
-#pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    b[t] = *(const bf16x4 *)(w + ((size_t)(tap * cout + t * 16 + (int)m) * cin + ci));
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    bf16x4 a = bf16x4{0, 0, 0, 0};
-                    if (ok[j]) a = *(const bf16x4 *)(in + nb[j] + ci);
-#pragma unroll
-                    for (int t = 0; t < NT; ++t)
-                        acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc[j][t], 0, 0, 0);
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t v = vbase[j] + 4u * grp + (uint32_t)i;
+                const float r = acc[j][0][i] + bv;
+                const uint16_t h = f2bf(r > 0.f ? r : 0.f);
+                float sum = bf2f(h) * wp;
+                sum += __shfl_xor(sum, 1);
+                sum += __shfl_xor(sum, 2);
+                sum += __shfl_xor(sum, 4);
+                sum += __shfl_xor(sum, 8);
+                if (v < V) {
+                    ((uint16_t *)out)[(size_t)v * 16 + m] = h;
+                    if (m == 0) out2[v] = sum + proj[16];
                 }
             }
-        }
+        return;
     }
-    // epilogue: lane holds voxels v0 + 16 j + 4 grp + i, channel 16 t + m
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < MT; ++j)
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int co = t * 16 + (int)m;
@@ -136,12 +105,27 @@ __global__ void __launch_bounds__(256) k_conv_fwd(NetGeom g, const uint16_t *__r
             const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const uint32_t v = v0 + 16u * j + 4u * grp + (uint32_t)i;
+                const uint32_t v = vbase[j] + 4u * grp + (uint32_t)i;
                 if (v >= V) continue;
                 float r = acc[j][t][i] + bv;
                 const size_t o = (size_t)v * cout_real + co;
-                if (mode == NET_OUT_BF16_RELU) {
-                    ((uint16_t *)out)[o] = f2bf(r > 0.f ? r : 0.f);
+                if (mode == NET_OUT_BF16_RELU || mode == NET_OUT_RELU_UP) {
+                    const uint16_t a = f2bf(r > 0.f ? r : 0.f);
+                    ((uint16_t *)out)[o] = a;
+                    if (mode == NET_OUT_RELU_UP) {
+                        // the 8 children on the grid twice as fine: a + skip
+                        int x, y, z;
+                        vox_xyz(g, v, x, y, z);
+                        const uint32_t fnx = 2u * g.nx, fny = 2u * g.ny;
+                        const float av = bf2f(a);
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            const uint32_t fv = ((2u * z + (k >> 2)) * fny + 2u * y + ((k >> 1) & 1)) * fnx +
+                                                2u * x + (k & 1);
+                            const size_t fo = (size_t)fv * cout_real + co;
+                            yfine[fo] = f2bf(av + bf2f(skip[fo]));
+                        }
+                    }
                 } else if (mode == NET_OUT_F32) {
                     ((float *)out)[o] = r;
                 } else {
@@ -151,33 +135,267 @@ __global__ void __launch_bounds__(256) k_conv_fwd(NetGeom g, const uint16_t *__r
         }
 }
 
+
+template <int NT, int MT, int KS, bool CIN4>
+__global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd(NetGeom g, const uint16_t *__restrict__ in, int cin,
+                                                  const uint16_t *__restrict__ w,
+                                                  const float *__restrict__ bias, int cout_real,
+                                                  void *__restrict__ out, int mode,
+                                                  const float *__restrict__ proj, float *__restrict__ out2,
+                                                  const uint16_t *__restrict__ skip, uint16_t *__restrict__ yfine)
+{
+    __shared__ float red[KS > 1 ? (KS - 1) * MT * NT * 4 * 64 : 1];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t tile = KS == 1 ? blockIdx.x * 4u + wv : blockIdx.x;
+    const int kpart = KS == 1 ? 0 : (int)wv;
+    const uint32_t V = g.nx * g.ny * g.nz;
+    const uint32_t v0 = tile * 16u * MT;
+    if (v0 >= V) return;  // uniform over the block when KS > 1
+    const uint32_t m = lane & 15u, grp = lane >> 4;
+    const int cout = 16 * NT;
+    const int nx = (int)g.nx, ny = (int)g.ny;
+    int vx[MT], vy[MT], vz[MT];
+    bool vin[MT];
+#pragma unroll
+    for (int j = 0; j < MT; ++j) {
+        const uint32_t v = v0 + 16u * j + m;
+        vin[j] = v < V;
+        vox_xyz(g, vin[j] ? v : 0u, vx[j], vy[j], vz[j]);
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int j = 0; j < MT; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[j][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int chunks = CIN4 ? 1 : (cin >> 4);
+    const int S = CIN4 ? 7 : 27 * chunks;
+    for (int s0 = kpart; s0 < S; s0 += KS * CONV_G) {
+        bf16x4 a[CONV_G][MT], b[CONV_G][NT];
+#pragma unroll
+        for (int q = 0; q < CONV_G; ++q) {
+            const int st = s0 + q * KS;
+            int tap, ci;
+            if (CIN4) {
+                tap = 4 * st + (int)grp;
+                ci = 0;
+            } else {
+                tap = st / chunks;
+                ci = (st - tap * chunks) * 16 + 4 * (int)grp;
+            }
+            const bool tok = st < S && tap < 27;
+            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+            const int cs = CIN4 ? 4 : cin;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                b[q][t] = bf16x4{0, 0, 0, 0};
+                if (tok) b[q][t] = *(const bf16x4 *)(w + ((size_t)(tap * cout + t * 16 + (int)m) * cs + ci));
+            }
+#pragma unroll
+            for (int j = 0; j < MT; ++j) {
+                int x = vx[j] + dx, y = vy[j] + dy;
+                const int z = vz[j] + dz;
+                a[q][j] = bf16x4{0, 0, 0, 0};
+                if (tok && vin[j] && nb_in(g, x, y, z))
+                    a[q][j] = *(const bf16x4 *)(in + ((size_t)((z * ny + y) * nx + x) * cs + ci));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CONV_G; ++q) {
+            if (s0 + q * KS >= S) break;
+#pragma unroll
+            for (int j = 0; j < MT; ++j)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q][j], b[q][t], acc[j][t], 0, 0, 0);
+        }
+    }
+    if (KS > 1) {
+        if (wv > 0) {
+#pragma unroll
+            for (int j = 0; j < MT; ++j)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        red[((((wv - 1) * MT + j) * NT + t) * 4 + i) * 64 + lane] = acc[j][t][i];
+        }
+        __syncthreads();
+        if (wv > 0) return;
+#pragma unroll 1
+        for (int k = 0; k < KS - 1; ++k)
+#pragma unroll
+            for (int j = 0; j < MT; ++j)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[j][t][i] += red[(((k * MT + j) * NT + t) * 4 + i) * 64 + lane];
+    }
+    uint32_t vbase[MT];
+#pragma unroll
+    for (int j = 0; j < MT; ++j) vbase[j] = v0 + 16u * j;
+    conv_store<NT, MT>(acc, vbase, V, m, grp, bias, cout_real, out, mode, proj, out2, g, skip, yfine);
+}
+
+// Full-resolution convolution with its operands staged in LDS: a block
+// computes a 16 x 4 x 4 voxel tile (wave w: y = w, M tiles z = 0..3) by 16
+// output channels; the input tile with its one-voxel halo (18 x 6 x 6
+// voxels) and the layer's weights are loaded once, so each input voxel is
+// read from HBM/L2 about once instead of 27 times.  Requires nx % 16 == 0,
+// ny % 4 == 0, nz % 4 == 0 and 16 output channels; cin 4 or 16.
+template <bool CIN4>
+__global__ void __launch_bounds__(256) k_conv_fwd_lds(NetGeom g, const uint16_t *__restrict__ in, int cin,
+                                                      const uint16_t *__restrict__ w,
+                                                      const float *__restrict__ bias, int cout_real,
+                                                      void *__restrict__ out, int mode,
+                                                      const float *__restrict__ proj, float *__restrict__ out2,
+                                                      const uint16_t *__restrict__ skip, uint16_t *__restrict__ yfine)
+{
+    constexpr int CS = CIN4 ? 4 : 16;         // channels per voxel
+    constexpr int HX = 18, HY = 6, HZ = 6;    // tile with halo
+    __shared__ bf16x4 s_in[HX * HY * HZ * CS / 4];
+    __shared__ bf16x4 s_w[27 * 16 * CS / 4];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u, wv = tid >> 6;
+    const uint32_t tx = g.nx / 16u, ty = g.ny / 4u;
+    const uint32_t bx = blockIdx.x % tx, by = (blockIdx.x / tx) % ty, bz = blockIdx.x / (tx * ty);
+    const int x0 = (int)bx * 16, y0 = (int)by * 4, z0 = (int)bz * 4;
+    // stage the input tile (zeros or wrap-around outside the grid)
+    for (int q = (int)tid; q < HX * HY * HZ * (CS / 4); q += 256) {
+        const int vox = q / (CS / 4), part = q % (CS / 4);
+        const int lx = vox % HX, ly = (vox / HX) % HY, lz = vox / (HX * HY);
+        int x = x0 - 1 + lx, y = y0 - 1 + ly;
+        const int z = z0 - 1 + lz;
+        bf16x4 v = bf16x4{0, 0, 0, 0};
+        if (nb_in(g, x, y, z)) v = *(const bf16x4 *)(in + ((size_t)((z * (int)g.ny + y) * (int)g.nx + x) * CS + part * 4));
+        s_in[q] = v;
+    }
+    for (int q = (int)tid; q < 27 * 16 * CS / 4; q += 256) s_w[q] = *(const bf16x4 *)(w + (size_t)q * 4);
+    __syncthreads();
+    const uint32_t m = lane & 15u, grp = lane >> 4;
+    f32x4 acc[4][1];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (CIN4) {
+        // every lane takes part in every MFMA; the padding tap 27 of the
+        // last step reads tap 26's operands and zeroes them
+        for (int st = 0; st < 7; ++st) {
+            const int tap = 4 * st + (int)grp;
+            const bool tok = tap < 27;
+            const int tc = tok ? tap : 26;
+            const int dz = tc / 9, dy = (tc / 3) % 3, dx = tc % 3;  // halo offsets 0..2
+            const bf16x4 zero4 = bf16x4{0, 0, 0, 0};
+            const bf16x4 b = tok ? s_w[tc * 16 + (int)m] : zero4;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bf16x4 a = s_in[((j + dz) * HY + ((int)wv + dy)) * HX + ((int)m + dx)];
+                acc[j][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(tok ? a : zero4, b, acc[j][0], 0, 0, 0);
+            }
+        }
+    } else {
+        for (int tap = 0; tap < 27; ++tap) {
+            const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+            const bf16x4 b = s_w[(tap * 16 + (int)m) * 4 + (int)grp];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bf16x4 a = s_in[(((j + dz) * HY + ((int)wv + dy)) * HX + ((int)m + dx)) * 4 + (int)grp];
+                acc[j][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j][0], 0, 0, 0);
+            }
+        }
+    }
+    (void)cin;
+    const uint32_t V = g.nx * g.ny * g.nz;
+    uint32_t vbase[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) vbase[j] = ((uint32_t)(z0 + j) * g.ny + (uint32_t)(y0 + (int)wv)) * g.nx + (uint32_t)x0;
+    conv_store<1, 4>(acc, vbase, V, m, grp, bias, cout_real, out, mode, proj, out2, g, skip, yfine);
+}
+
 template <int NT, bool CIN4>
 static void conv_fwd_nt(hipStream_t s, const NetGeom &g, const uint16_t *in, int cin, const uint16_t *w,
-                        const float *bias, int cout_real, void *out, int mode)
+                        const float *bias, int cout_real, void *out, int mode, const float *proj,
+                        float *out2, const uint16_t *skip, uint16_t *yfine)
 {
     const uint32_t V = g.nx * g.ny * g.nz;
-    const uint32_t waves = (V + 63u) / 64u;
-    hipLaunchKernelGGL((k_conv_fwd<NT, CIN4>), dim3((waves + 3u) / 4u), dim3(256), 0, s, g, in, cin, w,
-                       bias, cout_real, out, mode);
+    if (NT == 1 && V >= 65536u && g.nx % 16u == 0 && g.ny % 4u == 0 && g.nz % 4u == 0 &&
+        (cin == 4 || cin == 16)) {
+        const uint32_t blocks = (g.nx / 16u) * (g.ny / 4u) * (g.nz / 4u);
+        hipLaunchKernelGGL((k_conv_fwd_lds<CIN4>), dim3(blocks), dim3(256), 0, s, g, in, cin, w, bias,
+                           cout_real, out, mode, proj, out2, skip, yfine);
+    } else if (V >= 65536u) {
+        const uint32_t tiles = (V + 63u) / 64u;
+        hipLaunchKernelGGL((k_conv_fwd<NT, 4, 1, CIN4>), dim3((tiles + 3u) / 4u), dim3(256), 0, s, g, in, cin,
+                           w, bias, cout_real, out, mode, proj, out2, skip, yfine);
+    } else if (V >= 8192u) {
+        const uint32_t tiles = (V + 15u) / 16u;
+        hipLaunchKernelGGL((k_conv_fwd<NT, 1, 1, CIN4>), dim3((tiles + 3u) / 4u), dim3(256), 0, s, g, in, cin,
+                           w, bias, cout_real, out, mode, proj, out2, skip, yfine);
+    } else {
+        // small grids: the 4 waves of a block split K for one 16-voxel tile
+        const uint32_t tiles = (V + 15u) / 16u;
+        hipLaunchKernelGGL((k_conv_fwd<NT, 1, 4, CIN4>), dim3(tiles), dim3(256), 0, s, g, in, cin, w, bias,
+                           cout_real, out, mode, proj, out2, skip, yfine);
+    }
 }
 
 int launch_conv_fwd(hipStream_t s, const NetGeom &g, const uint16_t *in, int cin, const uint16_t *w,
-                    int cout_pad, const float *bias, int cout_real, void *out, int mode)
+                    int cout_pad, const float *bias, int cout_real, void *out, int mode, const float *proj,
+                    float *out2, const uint16_t *skip, uint16_t *yfine)
 {
     const bool c4 = cin == 4;
     if (!c4 && (cin % 16) != 0) return -1;
+    if (mode == NET_OUT_PROJ && (cout_pad != 16 || cout_real != 16 || !bias || !proj || !out2)) return -1;
+    if (mode == NET_OUT_RELU_UP && (!skip || !yfine)) return -1;
+#define CONV_NT(NTV)                                                                                         \
+    (c4 ? conv_fwd_nt<NTV, true>(s, g, in, cin, w, bias, cout_real, out, mode, proj, out2, skip, yfine)       \
+        : conv_fwd_nt<NTV, false>(s, g, in, cin, w, bias, cout_real, out, mode, proj, out2, skip, yfine))
     switch (cout_pad) {
-    case 16: c4 ? conv_fwd_nt<1, true>(s, g, in, cin, w, bias, cout_real, out, mode)
-                : conv_fwd_nt<1, false>(s, g, in, cin, w, bias, cout_real, out, mode); break;
-    case 32: c4 ? conv_fwd_nt<2, true>(s, g, in, cin, w, bias, cout_real, out, mode)
-                : conv_fwd_nt<2, false>(s, g, in, cin, w, bias, cout_real, out, mode); break;
-    case 48: c4 ? conv_fwd_nt<3, true>(s, g, in, cin, w, bias, cout_real, out, mode)
-                : conv_fwd_nt<3, false>(s, g, in, cin, w, bias, cout_real, out, mode); break;
-    case 64: c4 ? conv_fwd_nt<4, true>(s, g, in, cin, w, bias, cout_real, out, mode)
-                : conv_fwd_nt<4, false>(s, g, in, cin, w, bias, cout_real, out, mode); break;
+    case 16: CONV_NT(1); break;
+    case 32: CONV_NT(2); break;
+    case 48: CONV_NT(3); break;
+    case 64: CONV_NT(4); break;
     default: return -1;
     }
+#undef CONV_NT
     return 0;
+}
+
+// Backward of the pointwise head out[v] = sum_c h[v][c] w[c] + w[16]:
+// dh[v][c] = dout[v] w[c], g[c] += sum_v dout[v] h[v][c], g[16] += sum_v
+// dout[v].  One block per 1024 voxels, 16 voxels x 16 channels per pass.
+__global__ void __launch_bounds__(256) k_proj_bwd(uint32_t V, const float *__restrict__ dout,
+                                                  const uint16_t *__restrict__ h, const float *__restrict__ w,
+                                                  float *__restrict__ dh, float *__restrict__ g)
+{
+    __shared__ float part[256], pb[16];
+    const uint32_t c = threadIdx.x & 15u, vl = threadIdx.x >> 4;
+    const uint32_t v0 = blockIdx.x * 1024u;
+    const uint32_t ve = v0 + 1024u < V ? v0 + 1024u : V;
+    const float wc = w[c];
+    float sw = 0.f, sb = 0.f;
+    for (uint32_t v = v0 + vl; v < ve; v += 16u) {
+        const float d = dout[v];
+        dh[(size_t)v * 16 + c] = d * wc;
+        sw += d * bf2f(h[(size_t)v * 16 + c]);
+        if (c == 0) sb += d;
+    }
+    part[threadIdx.x] = sw;
+    if (c == 0) pb[vl] = sb;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        float t = 0.f;
+        for (int k = 0; k < 16; ++k) t += part[k * 16 + threadIdx.x];
+        atomicAdd(g + threadIdx.x, t);
+    } else if (threadIdx.x == 16) {
+        float t = 0.f;
+        for (int k = 0; k < 16; ++k) t += pb[k];
+        atomicAdd(g + 16, t);
+    }
+}
+
+void launch_proj_bwd(hipStream_t s, uint32_t V, const float *dout, const uint16_t *h, const float *w, float *dh,
+                     float *g)
+{
+    hipLaunchKernelGGL(k_proj_bwd, dim3((V + 1023u) / 1024u), dim3(256), 0, s, V, dout, h, w, dh, g);
 }
 
 // ---------------------------------------------------------------------------
@@ -234,8 +452,9 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(NetGeom g, const uint16_t *_
                 if (tok) {
                     int vx, vy, vz;
                     vox_xyz(g, v, vx, vy, vz);
-                    const int xx = vx + dxx, yy = vy + dyy, zz = vz + dzz;
-                    if (xx >= 0 && xx < (int)g.nx && yy >= 0 && yy < (int)g.ny && zz >= 0 && zz < (int)g.nz)
+                    int xx = vx + dxx, yy = vy + dyy;
+                    const int zz = vz + dzz;
+                    if (nb_in(g, xx, yy, zz))
                         bv = (short)x[(size_t)((zz * (int)g.ny + yy) * (int)g.nx + xx) * cin + ci];
                 }
             }
@@ -274,17 +493,27 @@ int launch_conv_wgrad(hipStream_t s, const NetGeom &g, const uint16_t *x, int ci
     return 0;
 }
 
-// db[c] += sum_v dz[v][c] for c < c_real (dz has c_stride channels); one
-// block per 1024 voxels
+// db[c] += sum_v dz[v][c] for c < c_real (dz has c_stride <= 256
+// channels); one block per 1024 voxels, threads over (voxel lane, channel)
+// so that a block reads dz rows contiguously, partial sums folded in LDS
 __global__ void __launch_bounds__(256) k_bias_grad(uint32_t V, const uint16_t *__restrict__ dz, int c_stride,
                                                    int c_real, float *__restrict__ db)
 {
+    __shared__ float part[256];
+    const uint32_t C = (uint32_t)c_stride;
+    const uint32_t lanes = 256u / C;
+    const uint32_t ch = threadIdx.x % C, vl = threadIdx.x / C;
     const uint32_t v0 = blockIdx.x * 1024u;
     const uint32_t ve = v0 + 1024u < V ? v0 + 1024u : V;
-    for (int ch = (int)threadIdx.x; ch < c_real; ch += 256) {
-        float sum = 0.f;
-        for (uint32_t v = v0; v < ve; ++v) sum += bf2f(dz[(size_t)v * c_stride + ch]);
-        atomicAdd(db + ch, sum);
+    float sum = 0.f;
+    if (vl < lanes)
+        for (uint32_t v = v0 + vl; v < ve; v += lanes) sum += bf2f(dz[(size_t)v * C + ch]);
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x < C && (int)threadIdx.x < c_real) {
+        float t = 0.f;
+        for (uint32_t k = 0; k < lanes; ++k) t += part[k * C + threadIdx.x];
+        atomicAdd(db + threadIdx.x, t);
     }
 }
 
@@ -339,29 +568,6 @@ void launch_pool_bwd(hipStream_t s, const NetGeom &gf, const float *dcoarse, int
 {
     const uint32_t n = gf.nx * gf.ny * gf.nz * (uint32_t)c;
     hipLaunchKernelGGL(k_pool_bwd, dim3((n + 255u) / 256u), dim3(256), 0, s, gf, dcoarse, c, dfine);
-}
-
-// out[v][c] = coarse[parent(v)][c] + skip[v][c]
-__global__ void __launch_bounds__(256) k_up_add(NetGeom gf, const uint16_t *__restrict__ coarse,
-                                                const uint16_t *__restrict__ skip, int c,
-                                                uint16_t *__restrict__ out)
-{
-    const uint32_t idx = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t Vf = gf.nx * gf.ny * gf.nz;
-    if (idx >= Vf * (uint32_t)c) return;
-    const uint32_t vf = idx / (uint32_t)c, ch = idx % (uint32_t)c;
-    int x, y, z;
-    vox_xyz(gf, vf, x, y, z);
-    const uint32_t cnx = gf.nx / 2u, cny = gf.ny / 2u;
-    const size_t p = ((size_t)((z >> 1) * cny + (y >> 1)) * cnx + (x >> 1)) * c + ch;
-    out[idx] = f2bf(bf2f(coarse[p]) + bf2f(skip[idx]));
-}
-
-void launch_up_add(hipStream_t s, const NetGeom &gf, const uint16_t *coarse, const uint16_t *skip, int c,
-                   uint16_t *out)
-{
-    const uint32_t n = gf.nx * gf.ny * gf.nz * (uint32_t)c;
-    hipLaunchKernelGGL(k_up_add, dim3((n + 255u) / 256u), dim3(256), 0, s, gf, coarse, skip, c, out);
 }
 
 // d_coarse[v][c] = sum of the 8 children's d_fine

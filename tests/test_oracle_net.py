@@ -13,7 +13,7 @@ import darcy_net as N  # noqa: E402
 
 
 def test_param_count():
-    assert N.param_count() == 285393
+    assert N.param_count() == 284977
     assert N.init_params(numpy.random.default_rng(0)).size == N.param_count()
 
 
@@ -31,15 +31,16 @@ def test_identity_kernel_and_shift():
     assert numpy.all(y[:, :, -1] == 0.0)
 
 
-def test_conv_adjoint_identities():
+@pytest.mark.parametrize("wrap", [0, 1, 2, 3])
+def test_conv_adjoint_identities(wrap):
     rng = numpy.random.default_rng(2)
     x = rng.standard_normal((4, 6, 8, 5))
     w = rng.standard_normal((27, 7, 5))
     y = rng.standard_normal((4, 6, 8, 7))
-    lhs = numpy.sum(N.conv(x, w) * y)
-    assert numpy.isclose(lhs, numpy.sum(x * N.conv_t(y, w)), rtol=1e-12)
+    lhs = numpy.sum(N.conv(x, w, wrap=wrap) * y)
+    assert numpy.isclose(lhs, numpy.sum(x * N.conv_t(y, w, wrap=wrap)), rtol=1e-12)
     # weight gradient: d<conv(x, w), y>/dw = conv_wgrad(x, y)
-    g = N.conv_wgrad(x, y)
+    g = N.conv_wgrad(x, y, wrap=wrap)
     assert numpy.isclose(lhs, numpy.sum(g * w), rtol=1e-12)
     a = rng.standard_normal((4, 6, 8, 2))
     b = rng.standard_normal((2, 3, 4, 2))
@@ -47,7 +48,21 @@ def test_conv_adjoint_identities():
     assert numpy.isclose(numpy.sum(N.up(b) * a), numpy.sum(b * N.up_t(a)), rtol=1e-12)
 
 
-def test_backward_matches_finite_differences():
+def test_wrap_padding_is_periodic():
+    rng = numpy.random.default_rng(4)
+    x = rng.standard_normal((4, 6, 8, 2))
+    w = rng.standard_normal((27, 3, 2))
+    y = N.conv(x, w, wrap=3)
+    # shifting the input by a period along x and y shifts the output alike
+    xs = numpy.roll(numpy.roll(x, 3, axis=2), 2, axis=1)
+    ys = numpy.roll(numpy.roll(y, 3, axis=2), 2, axis=1)
+    assert numpy.allclose(N.conv(xs, w, wrap=3), ys, rtol=1e-12, atol=1e-12)
+    # but not along z (zero padding)
+    assert not numpy.allclose(N.conv(numpy.roll(x, 1, axis=0), w, wrap=3), numpy.roll(y, 1, axis=0))
+
+
+@pytest.mark.parametrize("wrap", [0, 3])
+def test_backward_matches_finite_differences(wrap):
     rng = numpy.random.default_rng(3)
     shape = (8, 8, 8)
     params = N.init_params(rng)
@@ -56,18 +71,18 @@ def test_backward_matches_finite_differences():
         b[:] = 0.05
     inp = rng.standard_normal(shape + (3,))
     R = rng.standard_normal(shape)
-    out, cache = N.forward(params, inp)
+    out, cache = N.forward(params, inp, wrap=wrap)
     g = N.backward(cache, R)
     assert g.shape == params.shape
 
     def loss(p):
-        return numpy.sum(N.forward(p, inp)[0] * R)
+        return numpy.sum(N.forward(p, inp, wrap=wrap)[0] * R)
 
     # a few coordinates of every layer's weights and bias
     offs = []
     off = 0
-    for ci, co in N.LAYERS:
-        nw = 27 * co * ci
+    for (ci, co), t in zip(N.LAYERS, N.TAPS_OF):
+        nw = t * co * ci
         offs += list(off + rng.choice(nw, 3, replace=False))
         offs.append(off + nw + rng.integers(co))
         off += nw + co
