@@ -186,7 +186,102 @@ def darcy_section(args):
                                    % (sweep_bytes / 1e6)},
     }
     engine.close()
+    if args.no_net:
+        return out
+    out["learned"] = learned_section(args, s, ms, sweeps)
     return out
+
+
+def net_flops(nx, ny, nz):
+    """Multiply-adds x 2 of one forward pass of the correction network."""
+    from sphere_amd import learn
+    levels = [0, 1, 2, 3, 3, 2, 1, 0, 0]
+    V0 = nx * ny * nz
+    return sum(2.0 * taps * ci * co * V0 / 8 ** lv
+               for (ci, co, taps), lv in zip(learn.LAYERS, levels))
+
+
+def learned_section(args, s, plain_ms, plain_sweeps):
+    """Train the correction network on the consolidation box's own solves,
+    then time the same run with it (same packing, warm-up and step count)."""
+    import torch
+    import sphere_amd
+    from sphere_amd import learn
+    t0 = time.time()
+    trainer = sphere_amd.Engine(0, nc=32)
+    trainer.load(s.to_state())
+    trainer.step(args.darcy_warmup)
+    net = learn.make_net(trainer)
+    X, Y = learn.collect(trainer, args.net_steps)
+    params, _ = learn.train(net, X, Y, iters=args.net_iters, lr=1e-3)
+    rounds = 1
+    for _ in range(args.net_rounds):
+        X2, Y2 = learn.collect(trainer, args.net_steps // 2, sweeps=(2, 42, 162), net=net, params=params,
+                               corrections=args.net_corrections)
+        X = torch.cat([X, X2])
+        Y = torch.cat([Y, Y2])
+        params, _ = learn.train(net, X, Y, iters=args.net_iters, lr=5e-4, params=params)
+        rounds += 1
+    pairs = int(X.shape[0])
+    del X, Y
+    torch.cuda.synchronize()
+    train_s = time.time() - t0
+    trainer.close()
+    log("Darcy network: %d rounds on %d (input, error) pairs in %.1f s" % (rounds, pairs, train_s))
+    # inference time of one forward pass (HIP events on torch's stream)
+    nx, ny, nz = int(s.num[0]), int(s.num[1]), int(s.num[2])
+    V = nx * ny * nz
+    dev = torch.device("cuda:0")
+    pt = torch.tensor(params, device=dev)
+    inp = torch.randn((V, 3), device=dev)
+    o = torch.empty(V, device=dev)
+    for _ in range(3):
+        net.forward(pt, inp, o)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        net.forward(pt, inp, o)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_s = e0.elapsed_time(e1) / 20 / 1000.0
+    net.close()
+    # the timed run with the network
+    engine = sphere_amd.Engine(0, nc=32)
+    engine.set_darcy_net(params, args.net_corrections)
+    ms, sweeps, elapsed, st = run_darcy(engine, s, args.darcy_warmup, args.darcy_steps)
+    engine.close()
+    # the converged pressure of one solve from the same state, with and without
+    pf = []
+    for use in (False, True):
+        e = sphere_amd.Engine(0, nc=32)
+        if use:
+            e.set_darcy_net(params, args.net_corrections)
+        e.load(s.to_state())
+        e.step(args.darcy_warmup + 1)
+        pf.append(e.read().get("p_f", V, 1).copy())
+        e.close()
+    p_err = float(numpy.max(numpy.abs(pf[1] - pf[0])) / max(1e-300, numpy.max(numpy.abs(pf[0]))))
+    flops = net_flops(nx, ny, nz)
+    log("Darcy with the network: %.3f ms per solve (p50 %.3f), %.1f sweeps, %.1f corrections per "
+        "solve; plain %.3f ms, %.1f sweeps; forward %.1f us; pressure difference %.2e" %
+        (ms.mean(), numpy.percentile(ms, 50), sweeps.mean(), st.corrections / float(args.darcy_warmup +
+                                                                                     args.darcy_steps),
+         plain_ms.mean(), plain_sweeps.mean(), 1e6 * fwd_s, p_err))
+    return {
+        "ms": percentiles(ms), "sweeps": percentiles(sweeps),
+        "corrections_per_solve": st.corrections / float(args.darcy_warmup + args.darcy_steps),
+        "nonconverged": int(st.nonconverged),
+        "sweep_ratio_plain_over_learned": float(plain_sweeps.mean() / sweeps.mean()),
+        "speedup_ms": float(plain_ms.mean() / ms.mean()),
+        "final_pressure_rel_diff_vs_plain": p_err,
+        "max_corrections": args.net_corrections,
+        "training": {"pairs": pairs, "rounds": rounds, "iters_per_round": args.net_iters,
+                     "seconds": train_s, "params": int(sphere_amd.Net.param_count())},
+        "net_roofline": {"bound": "mfma", "achieved": flops / fwd_s / 1e12, "peak": 2500.0,
+                         "unit": "TFLOP/s", "frac": flops / fwd_s / 1e12 / 2500.0,
+                         "flops_per_forward": flops, "us_per_forward": 1e6 * fwd_s,
+                         "dtype": "bf16 operands, fp32 accumulation"},
+    }
 
 
 def cpu_baseline(state_buf, n, warm, steps, threads):
@@ -230,6 +325,13 @@ def main():
     ap.add_argument("--darcy-seed", type=int, default=3)
     ap.add_argument("--darcy-steps", type=int, default=500)
     ap.add_argument("--darcy-warmup", type=int, default=20)
+    ap.add_argument("--no-net", action="store_true", help="skip the learned Darcy correction")
+    ap.add_argument("--net-steps", type=int, default=150,
+                    help="fluid steps whose solves make the first training round's pairs")
+    ap.add_argument("--net-iters", type=int, default=3000, help="Adam steps per training round")
+    ap.add_argument("--net-rounds", type=int, default=2,
+                    help="training rounds on iterates of the corrected solve")
+    ap.add_argument("--net-corrections", type=int, default=8, help="corrections per solve at most")
     ap.add_argument("--darcy-save", default="", help="write the prepared consolidation box here")
     ap.add_argument("--darcy-load", default="", help="start from a box written by --darcy-save")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -242,6 +344,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if not args.no_darcy and not args.no_net:
+        # torch carries its own HIP runtime: initialise it before the library
+        # loads, so that both use that one runtime in this process
+        import torch
+        torch.cuda.init()
     import sphere_amd
     if args.no_shear:
         if args.prep_only:
