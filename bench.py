@@ -358,6 +358,8 @@ def main():
         print(json.dumps({"metric": d["metric"], "value": d["plain"]["ms"]["mean"], "unit": "ms",
                           "n_gpus": 1, "higher_is_better": False, "dtype": "f64",
                           "data": "synthetic", "darcy": d}), flush=True)
+        if args.copy_calibration:
+            copy_calibration(args.copy_calibration)
         return
     # the largest lognormal grains touch more than 32 others: 64 contact slots
     engine = sphere_amd.Engine(0, nc=64)
