@@ -298,6 +298,25 @@ int sph_darcy_sweeps(sph_ctx *ctx, double *p, int nsweeps);
  * the network's target for p is (p_fixed - p) / s on the cells. */
 int sph_darcy_net_input(sph_ctx *ctx, const double *p, float *input, double *scale);
 
+/* ------------------------------------------------------------------------
+ * Ensembles of independent boxes (no counterpart in the reference, which
+ * runs one simulation per process).  The boxes are contexts loaded as usual
+ * (any sizes and parameters, one device); sph_ensemble_step advances every
+ * box by nsteps steps, enqueuing all boxes' kernels on their own streams
+ * without waiting in between, and returns when all are done.  Each box's
+ * results equal those of sph_step on that box alone.  Boxes with fluid, an
+ * adaptive grid or an output event inside the steps are refused
+ * (SPH_ERR_VALUE).
+ * ---------------------------------------------------------------------- */
+typedef struct sph_ensemble sph_ensemble;
+
+sph_ensemble *sph_ensemble_create(sph_ctx **boxes, int n);
+void sph_ensemble_destroy(sph_ensemble *ens);
+const char *sph_ensemble_last_error(const sph_ensemble *ens);
+int sph_ensemble_step(sph_ensemble *ens, int nsteps);
+/* steps taken so far by every box */
+int sph_ensemble_stats(const sph_ensemble *ens, uint64_t *steps);
+
 #ifdef __cplusplus
 }
 #endif

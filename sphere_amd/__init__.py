@@ -2,8 +2,9 @@
 ``sim`` API of the sphere granular simulator.
 
 ``sim`` mirrors python/sphere.py's setup, binary I/O and ``run()``;
-``Engine`` is the in-process C ABI context of libsphere_hip.so; ``Net`` is
-the Darcy correction network on its kernels.
+``Engine`` is the in-process C ABI context of libsphere_hip.so; ``Ensemble``
+steps many independent Engines together; ``Net`` is the Darcy correction
+network on its kernels.
 """
 from .sim import sim, status, run, cleanup, V_sphere, VERSION  # noqa: F401
-from ._abi import Engine, Net, SphereError, load_library  # noqa: F401
+from ._abi import Engine, Ensemble, Net, SphereError, load_library  # noqa: F401

@@ -196,6 +196,11 @@ def load_library(path=None):
         "sph_net_backward": (ctypes.c_int, [vp, vp, vp, vp, vp]),
         "sph_net_conv3d": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_uint32,
                                           ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp]),
+        "sph_ensemble_create": (vp, [ctypes.POINTER(vp), ctypes.c_int]),
+        "sph_ensemble_destroy": (None, [vp]),
+        "sph_ensemble_last_error": (ctypes.c_char_p, [vp]),
+        "sph_ensemble_step": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sph_ensemble_stats": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64)]),
         "sph_set_darcy_net": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_uint64,
                                              ctypes.c_int]),
         "sph_darcy_export": (ctypes.c_int, [vp, ctypes.c_int, vp]),
@@ -419,6 +424,41 @@ class Net:
     def close(self):
         if self.h:
             self.lib.sph_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Independent boxes (Engines with loaded states, one device) stepped
+    together: all boxes' kernels are enqueued on their own streams before
+    the host waits."""
+
+    def __init__(self, engines, lib=None):
+        self.lib = lib or load_library()
+        self.engines = list(engines)  # keeps the contexts alive
+        arr = (ctypes.c_void_p * len(self.engines))(*[e.ctx for e in self.engines])
+        self.h = self.lib.sph_ensemble_create(arr, len(self.engines))
+        if not self.h:
+            raise SphereError(-1, self.lib.sph_ensemble_last_error(None).decode())
+
+    def step(self, n=1):
+        rc = self.lib.sph_ensemble_step(self.h, int(n))
+        if rc != SPH_OK:
+            raise SphereError(rc, self.lib.sph_ensemble_last_error(self.h).decode())
+
+    def stats(self):
+        a = ctypes.c_uint64(0)
+        self.lib.sph_ensemble_stats(self.h, ctypes.byref(a))
+        return {"steps": a.value}
+
+    def close(self):
+        if self.h:
+            self.lib.sph_ensemble_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -2023,7 +2023,8 @@ __device__ __forceinline__ void next_hash(const StepArgs &a, const NextHash &nh,
 // reference reads the *_sorted copies, which hold the same values), with the
 // wall-0 force partial of summation() reduced over the same 256-particle block.
 __global__ void __launch_bounds__(256) k_integrate(
-    StepArgs a, long iter, int change_velocity_state, double velocity_factor, double tau_x,
+    StepArgs a, const StepClock *__restrict__ clock, int change_velocity_state, double velocity_factor,
+    double tau_x,
     double4 *__restrict__ x, double4 *__restrict__ vel, double4 *__restrict__ angvel,
     double4 *__restrict__ force, double4 *__restrict__ torque,
     double4 *__restrict__ angpos, double4 *__restrict__ acc_a, double4 *__restrict__ angacc_a,
@@ -2066,7 +2067,7 @@ __global__ void __launch_bounds__(256) k_integrate(
     const double4 xo = x[i], vo = vel[i], wo = angvel[i];
     const double4 xs = xyzsum_a[i];
     double acc0[3] = {0.0, 0.0, 0.0}, angacc0[3] = {0.0, 0.0, 0.0};
-    if (iter != 0) {
+    if (clock->iter != 0) {
         const double4 a0 = acc_a[i], w0 = angacc_a[i];
         acc0[0] = a0.x; acc0[1] = a0.y; acc0[2] = a0.z;
         angacc0[0] = w0.x; angacc0[1] = w0.y; angacc0[2] = w0.z;
@@ -2140,7 +2141,7 @@ __global__ void __launch_bounds__(256) k_integrate(
     if (nh.keys) next_hash(a, nh, i, x_new, wrapped);
 }
 
-void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,
+void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b,
                       int change_velocity_state, double velocity_factor, double tau_x,
                       bool gather_bonds, const NextHash &nh)
 {
@@ -2153,7 +2154,7 @@ void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b, lon
         bg.t = b.bond_t;
         bg.applied = b.bond_applied;
     }
-    hipLaunchKernelGGL(k_integrate, dim3(nb), dim3(256), 0, s, a, iter, change_velocity_state,
+    hipLaunchKernelGGL(k_integrate, dim3(nb), dim3(256), 0, s, a, b.clock, change_velocity_state,
                        velocity_factor, tau_x, b.x, b.vel, b.angvel, b.force, b.torque, b.angpos,
                        b.acc, b.angacc, b.xyzsum, b.w_mvfd, b.tau_eff, b.walls_force_pp,
                        b.wall_partial, bg, nh);
@@ -2163,10 +2164,12 @@ void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b, lon
 // serial sum over the block partials (the reference's order) once and every
 // wall thread uses it (the reference sums wall 0's partials for each wall).
 __global__ void __launch_bounds__(SUM_THREADS) k_integrate_walls(
-    StepArgs a, long iter, double t_current, double4 *__restrict__ w_nx,
+    StepArgs a, const StepClock *__restrict__ clock, double4 *__restrict__ w_nx,
     double4 *__restrict__ w_mvfd, double *__restrict__ w_acc,
     const double *__restrict__ partial, uint32_t nparts)
 {
+    const long long iter = clock->iter;
+    const double t_current = clock->time;
     __shared__ double lds[SUM_CHUNK];
     __shared__ double total;
     const double s = serial_sum_staged(partial, nparts, lds);
@@ -2192,12 +2195,25 @@ __global__ void __launch_bounds__(SUM_THREADS) k_integrate_walls(
     w_acc[idx] = acc;
 }
 
-void launch_integrate_walls(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,
-                            double t_current)
+void launch_integrate_walls(hipStream_t s, const StepArgs &a, const DemBuffers &b)
 {
     uint32_t nb = (a.np + 255) / 256;
-    hipLaunchKernelGGL(k_integrate_walls, dim3(1), dim3(SUM_THREADS), 0, s, a, iter, t_current, b.w_nx,
+    hipLaunchKernelGGL(k_integrate_walls, dim3(1), dim3(SUM_THREADS), 0, s, a, b.clock, b.w_nx,
                        b.w_mvfd, b.w_acc, b.wall_partial, nb);
+}
+
+// iter += 1, time += dt: the host's time loop update (src/device.cu:2541-2545)
+__global__ void k_clock_advance(StepClock *__restrict__ clock, double dt)
+{
+    if (threadIdx.x == 0) {
+        clock->iter += 1;
+        clock->time += dt;
+    }
+}
+
+void launch_clock_advance(hipStream_t s, StepClock *clock, double dt)
+{
+    hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, s, clock, dt);
 }
 
 // ------------------------------------------------------------------

@@ -48,6 +48,7 @@ struct sph_ctx {
     bool walls_pending = false;
     std::string err;
     bool loaded = false;
+    uint64_t gen = 0;  // incremented by every load (captured graphs hold the old buffers)
     HostState hs;
     DemBuffers db{};
     DarcyBuffers dd{};
@@ -347,6 +348,7 @@ static int upload(sph_ctx *ctx)
     AL(b.keys, uint32_t, np); AL(b.vals, uint32_t, np);
     AL(b.keys_tmp, uint32_t, np); AL(b.vals_tmp, uint32_t, np);
     AL(b.hist, uint32_t, 256 * ntiles);
+    AL(b.clock, StepClock, 1);
     AL(b.cell_start, uint32_t, ctx->ncells); AL(b.cell_end, uint32_t, ctx->ncells);
     AL(b.x_s, double4, np); AL(b.vel_s, double4, np); AL(b.angvel_s, double4, np);
     const size_t NCS = ctx->nc;
@@ -541,8 +543,13 @@ static int upload(sph_ctx *ctx)
     ctx->nl_force = 1;
     ctx->hash_ready = false;
 #undef AL
-    HIPCHK(hipStreamSynchronize(s));
     ctx->iter = 0;
+    ctx->gen++;
+    {
+        StepClock c0{0, h.time_current};
+        HIPCHK(hipMemcpyAsync(ctx->db.clock, &c0, sizeof(c0), hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
     ctx->filetimeclock = 0.0;
     ctx->host_wx0 = h.nw > 0 ? h.w_nx[3] : 0.0;
     ctx->wall0_iz = 10000000u;
@@ -988,7 +995,7 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             nh.disp2 = ctx->nla.disp2;
         }
         { ProfScope ps(ctx, SPH_K_INTEGRATE);
-          launch_integrate(s, a, ctx->db, ctx->iter, change_velocity_state, 10.0, h.tau_x,
+          launch_integrate(s, a, ctx->db, change_velocity_state, 10.0, h.tau_x,
                            gather_in_integrate(h), nh); }
         ctx->hash_ready = fuse_hash;
         if (h.nw > 0) {
@@ -996,11 +1003,17 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             HIPCHK(hipStreamWaitEvent(ctx->aux, ctx->ev_integrated, 0));
             {
                 ProfScope ps(ctx, SPH_K_WALLS, ctx->aux);
-                launch_integrate_walls(ctx->aux, a, ctx->db, ctx->iter, h.time_current);
+                launch_integrate_walls(ctx->aux, a, ctx->db);
             }
+            // the clock's last readers this step are integrate and the walls
+            launch_clock_advance(ctx->aux, ctx->db.clock, h.time_dt);
             HIPCHK(hipEventRecord(ctx->ev_walls, ctx->aux));
             ctx->walls_pending = true;
+        } else {
+            launch_clock_advance(s, ctx->db.clock, h.time_dt);
         }
+    } else if (ctx->db.clock) {
+        launch_clock_advance(s, ctx->db.clock, h.time_dt);
     }
     h.time_current += h.time_dt;
     ctx->filetimeclock += h.time_dt;
@@ -1531,6 +1544,96 @@ int sph_kernel_time(sph_ctx *ctx, int kernel_id, double *total_ms, uint64_t *lau
     if (rc) return rc;
     if (total_ms) *total_ms = ctx->ktime[kernel_id];
     if (launches) *launches = ctx->klaunch[kernel_id];
+    return SPH_OK;
+}
+
+// ------------------------------------------------------------------
+// Ensembles of independent boxes (no reference counterpart): the steps of
+// all boxes are enqueued without waiting, box after box for each step, on
+// every box's own streams, so that the device runs the boxes' kernels side
+// by side; the host waits once at the end.  Each box runs exactly the
+// kernels of sph_step, so its results equal those of sph_step on that box.
+// ------------------------------------------------------------------
+
+struct sph_ensemble {
+    std::vector<sph_ctx *> boxes;
+    int device = 0;
+    std::string err;
+    uint64_t steps = 0;
+};
+
+static int ens_fail(sph_ensemble *e, int code, const std::string &msg)
+{
+    e->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+sph_ensemble *sph_ensemble_create(sph_ctx **boxes, int n)
+{
+    if (!boxes || n <= 0) {
+        g_last_error = "no boxes";
+        return nullptr;
+    }
+    for (int i = 0; i < n; ++i)
+        if (!boxes[i] || boxes[i]->device != boxes[0]->device) {
+            g_last_error = "null context or contexts on different devices";
+            return nullptr;
+        }
+    sph_ensemble *e = new sph_ensemble();
+    e->boxes.assign(boxes, boxes + n);
+    e->device = boxes[0]->device;
+    return e;
+}
+
+void sph_ensemble_destroy(sph_ensemble *e) { delete e; }
+
+const char *sph_ensemble_last_error(const sph_ensemble *e)
+{
+    return e ? e->err.c_str() : g_last_error.c_str();
+}
+
+int sph_ensemble_stats(const sph_ensemble *e, uint64_t *steps)
+{
+    if (!e) return SPH_ERR_ARG;
+    if (steps) *steps = e->steps;
+    return SPH_OK;
+}
+
+// Enqueued steps must not need the host between them: no fluid (the Darcy
+// loop decides on the host), no adaptive grid (host read-back) and no output
+// event inside the steps.
+static int ens_check(sph_ensemble *e, int nsteps)
+{
+    for (size_t i = 0; i < e->boxes.size(); ++i) {
+        sph_ctx *c = e->boxes[i];
+        const sph_state &h = c->hs.h;
+        const std::string b = "box " + std::to_string(i) + ": ";
+        if (!c->loaded) return ens_fail(e, SPH_ERR_STATE, b + "no state loaded");
+        if (h.fluid) return ens_fail(e, SPH_ERR_VALUE, b + "fluid boxes are stepped with sph_step");
+        if (h.adaptive == 1 && h.nw > 0) return ens_fail(e, SPH_ERR_VALUE, b + "adaptive grid");
+        double clk = c->filetimeclock;
+        for (int k = 0; k < nsteps; ++k) clk += h.time_dt;
+        if (clk >= h.time_file_dt) return ens_fail(e, SPH_ERR_VALUE, b + "an output event falls in the steps");
+    }
+    return SPH_OK;
+}
+
+int sph_ensemble_step(sph_ensemble *e, int nsteps)
+{
+    if (!e) return SPH_ERR_ARG;
+    if (nsteps <= 0) return SPH_OK;
+    (void)hipSetDevice(e->device);
+    int rc = ens_check(e, nsteps);
+    if (rc) return rc;
+    for (int k = 0; k < nsteps; ++k)
+        for (sph_ctx *c : e->boxes)
+            if ((rc = step_once(c, nullptr, nullptr, std::string()))) return ens_fail(e, rc, c->err);
+    for (sph_ctx *c : e->boxes) {
+        if ((rc = check_flags(c))) return ens_fail(e, rc, c->err);
+        if ((rc = sync(c))) return ens_fail(e, rc, c->err);
+    }
+    e->steps += (uint64_t)nsteps;
     return SPH_OK;
 }
 

@@ -5,12 +5,21 @@
 
 #include "common.h"
 
+// Step counter and simulation time on the device (read by integrate and the
+// wall integration, advanced by k_clock_advance after them), so that a step
+// replayed from a captured graph sees its own values.
+struct StepClock {
+    long long iter;
+    double time;
+};
+
 struct DemBuffers {
     // particle state, original order (src/datatypes.h:16-32)
     double4 *x, *vel, *angvel, *force, *torque, *angpos, *acc, *angacc, *xyzsum;
     double *es_dot, *es, *ev_dot, *ev, *p;
     // sort (src/datatypes.h:52-60)
     uint32_t *keys, *vals, *keys_tmp, *vals_tmp, *hist;
+    StepClock *clock;
     uint32_t *cell_start, *cell_end;
     double4 *x_s, *vel_s, *angvel_s;
     // contact history, original index * nc + slot
@@ -85,11 +94,11 @@ void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool 
 // gather: add the bond forces to force/torque here (else integrate does it)
 void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool gather);
 void launch_shear_stress(hipStream_t s, const StepArgs &a, const DemBuffers &b);
-void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,
+void launch_clock_advance(hipStream_t s, StepClock *clock, double dt);
+void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b,
                       int change_velocity_state, double velocity_factor, double tau_x,
                       bool gather_bonds, const NextHash &nh);
-void launch_integrate_walls(hipStream_t s, const StepArgs &a, const DemBuffers &b, long iter,
-                            double t_current);
+void launch_integrate_walls(hipStream_t s, const StepArgs &a, const DemBuffers &b);
 void launch_fill_u32(hipStream_t s, uint32_t *p, size_t n, uint32_t v);
 void launch_fill_f64(hipStream_t s, double *p, size_t n, double v);
 void launch_count_contacts(hipStream_t s, const uint64_t *cmask, uint32_t np,
