@@ -284,6 +284,69 @@ def learned_section(args, s, plain_ms, plain_sweeps):
     }
 
 
+def ensemble_section(args):
+    """Independent shear boxes stepped together (Ensemble) against the same
+    boxes stepped one at a time (sph_step per box), from the same states."""
+    import sphere_amd
+    from sphere_amd import workloads as W
+    t0 = time.time()
+    sims, params = W.ensemble_boxes(count=args.ens_boxes, seed=args.ens_seed)
+    total = sum(p["grains"] for p in params)
+    log("ensemble: %d boxes, %d grains in all, prepared in %.1f s" % (len(sims), total, time.time() - t0))
+    states = [s.to_state() for s in sims]
+
+    def engines():
+        out = []
+        for st in states:
+            e = sphere_amd.Engine(0, nc=64)
+            e.load(st)
+            out.append(e)
+        return out
+
+    # together
+    eng = engines()
+    ens = sphere_amd.Ensemble(eng)
+    ens.step(args.ens_warmup)
+    t0 = time.perf_counter()
+    ens.step(args.ens_steps)
+    t_ens = time.perf_counter() - t0
+    ens.close()
+    # one at a time
+    one = engines()
+    for e in one:
+        e.step(args.ens_warmup)
+    t0 = time.perf_counter()
+    for e in one:
+        e.step(args.ens_steps)
+    t_one = time.perf_counter() - t0
+    # per-box results
+    diff = 0.0
+    for k, (a, b) in enumerate(zip(eng, one)):
+        n = params[k]["grains"]
+        xa = a.read().get("x", n, 4)
+        xb = b.read().get("x", n, 4)
+        diff = max(diff, float(numpy.max(numpy.abs(xa - xb))))
+    for e in eng + one:
+        e.close()
+    v_ens = total * args.ens_steps / t_ens
+    v_one = total * args.ens_steps / t_one
+    log("ensemble: %.4g particle-steps/s together, %.4g one at a time (%.2fx); max |x| difference %g" %
+        (v_ens, v_one, v_ens / v_one, diff))
+    sizes = sorted(p["grains"] for p in params)
+    return {
+        "metric": "aggregate particle-steps/s (independent boxes, FP64)",
+        "together": v_ens, "one_at_a_time": v_one, "speedup": v_ens / v_one,
+        "steps": args.ens_steps, "warmup": args.ens_warmup,
+        "max_abs_position_difference_vs_one_at_a_time": diff,
+        "config": {"workload": "%d independent periodic shear boxes, grains log-uniform in [5000, 60000] "
+                               "(lognormal radii, Hertz-Mindlin E=7e9 nu=0.3), mu ~ U[0.3, 0.7], "
+                               "sigma0 ~ U[5, 80] kPa, top-layer velocity ~ U[1e-3, 1e-1] m/s; "
+                               "DEM only" % len(params),
+                   "boxes": len(params), "grains_total": total,
+                   "grains_min_median_max": [sizes[0], sizes[len(sizes) // 2], sizes[-1]]},
+    }
+
+
 def cpu_baseline(state_buf, n, warm, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from _oracle import Oracle
@@ -326,6 +389,12 @@ def main():
     ap.add_argument("--darcy-steps", type=int, default=500)
     ap.add_argument("--darcy-warmup", type=int, default=20)
     ap.add_argument("--no-net", action="store_true", help="skip the learned Darcy correction")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="also measure independent boxes stepped together")
+    ap.add_argument("--ens-boxes", type=int, default=64)
+    ap.add_argument("--ens-seed", type=int, default=5)
+    ap.add_argument("--ens-steps", type=int, default=200)
+    ap.add_argument("--ens-warmup", type=int, default=20)
     ap.add_argument("--net-steps", type=int, default=150,
                     help="fluid steps whose solves make the first training round's pairs")
     ap.add_argument("--net-iters", type=int, default=3000, help="Adam steps per training round")
@@ -350,6 +419,12 @@ def main():
         import torch
         torch.cuda.init()
     import sphere_amd
+    if args.no_shear and args.ensemble and args.no_darcy:
+        d = ensemble_section(args)
+        print(json.dumps({"metric": d["metric"], "value": d["together"], "unit": "particle-steps/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+                          "data": "synthetic", "ensemble": d}), flush=True)
+        return
     if args.no_shear:
         if args.prep_only:
             darcy_packing(args, sphere_amd.Engine(0, nc=32))
@@ -466,6 +541,7 @@ def main():
         "cpu_baseline": cpu,
     }
     line["darcy"] = None if args.no_darcy else darcy_section(args)
+    line["ensemble"] = ensemble_section(args) if args.ensemble else None
     print(json.dumps(line), flush=True)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

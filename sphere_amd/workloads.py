@@ -3,6 +3,10 @@
 * settling(n)       -- dry settling, linear contacts, frictional walls.
 * shear_box(n)      -- dense periodic shear box, Hertz-Mindlin, 10 kPa top
                        wall, top layer driven at 0.01 m/s, 5% bonded pairs.
+* ensemble_boxes(count)
+                    -- independent shear boxes of log-uniform size (5,000 to
+                       60,000 grains) and drawn friction, load and shear
+                       velocity, grown together through an Ensemble.
 * consolidation_base(n) / consolidation_finish(s)
                     -- Darcy-coupled consolidation: 200,000 grains under a
                        20 kPa wall on a 48 x 48 x 96 Darcy grid.
@@ -397,3 +401,83 @@ def consolidation_finish(s, sigma0=2e4, cells=(48, 48, 96), mu_f=1.04e-7, beta_f
     s.time_file_dt[0] = 1.0
     s.time_step_count[0] = 0
     return s
+
+
+def ensemble_boxes(count=64, seed=5, n_min=5000, n_max=60000, steps=300, z_target=4.2, relax_steps=400,
+                   device=0, log=None):
+    """Seeded shear boxes for the ensemble benchmark.  Box b: N_b log-uniform
+    in [n_min, n_max], mu ~ U[0.3, 0.7], sigma0 ~ U[5, 80] kPa, top-layer
+    velocity ~ U[1e-3, 1e-1] m/s; the shear box's radii, contact law and
+    preparation (growth to jamming, relaxation under gravity, 10 kPa-style
+    wall and driven layer with the drawn values), all boxes stepped together
+    through one Ensemble per stage.  Returns the sims and their parameters."""
+    from ._abi import Engine, Ensemble
+    rng = numpy.random.default_rng(seed)
+    sizes = numpy.exp(rng.uniform(numpy.log(n_min), numpy.log(n_max), count)).astype(int)
+    mus = rng.uniform(0.3, 0.7, count)
+    sig = rng.uniform(5e3, 80e3, count)
+    vel = rng.uniform(1e-3, 1e-1, count)
+    boxes = []
+    for b in range(count):
+        s, f0, brng = shear_box_base(n=int(sizes[b]), seed=int(seed * 1000 + b), phi=0.60)
+        s.gamma_n[0] = 5e3
+        r_final = s.radius.copy()
+        boxes.append({"s": s, "f": f0, "r": r_final, "z": 0.0, "done": False, "rng": brng})
+    engines = [Engine(device, nc=64) for _ in range(count)]
+    ens = Ensemble(engines)
+    for stage in range(400):
+        active = [k for k, bx in enumerate(boxes) if not bx["done"]]
+        if not active:
+            break
+        for k in active:
+            bx = boxes[k]
+            s = bx["s"]
+            s.radius = bx["r"] * bx["f"]
+            s.vel[:] = 0.0
+            s.angvel[:] = 0.0
+            engines[k].load(s.to_state())
+        ens.step(steps)
+        for k in active:
+            bx = boxes[k]
+            s = bx["s"]
+            n = int(s.np[0])
+            s.x = engines[k].read().get("x", n, 4)[:, :3].copy()
+            bx["z"] = engines[k].count_contacts() / n
+            if bx["z"] >= z_target:
+                bx["done"] = True
+            else:
+                bx["f"] *= 1.0 + (0.02 if bx["z"] < 1.0 else 0.005)
+        if log:
+            log("ensemble growth stage %d: %d boxes still growing" % (stage, len(active)))
+    for bx in boxes:
+        s = bx["s"]
+        scale = 1.0 / bx["f"]
+        s.x *= scale
+        s.L = s.L * scale
+        s.w_x = s.w_x * scale
+        s.radius = bx["r"].copy()
+        s.w_x[0] = s.L[2]
+        s.g[2] = -9.81
+        s.vel[:] = 0.0
+        s.angvel[:] = 0.0
+    for k, bx in enumerate(boxes):
+        engines[k].load(bx["s"].to_state())
+    ens.step(relax_steps)
+    for k, bx in enumerate(boxes):
+        s = bx["s"]
+        n = int(s.np[0])
+        s.x = engines[k].read().get("x", n, 4)[:, :3].copy()
+        s.vel[:] = 0.0
+        s.angvel[:] = 0.0
+        s.force[:] = 0.0
+        s.torque[:] = 0.0
+        shear_box_finish(s, bx["rng"], contacts=None, bond_fraction=0.0, sigma0=float(sig[k]),
+                         v_top=float(vel[k]))
+        s.mu_s[0] = mus[k]
+        s.mu_d[0] = mus[k]
+    ens.close()
+    for e in engines:
+        e.close()
+    params = [{"grains": int(sizes[k]), "mu": float(mus[k]), "sigma0": float(sig[k]),
+               "v_top": float(vel[k])} for k in range(count)]
+    return [bx["s"] for bx in boxes], params
