@@ -376,6 +376,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skin", type=float, default=None,
                     help="neighbour-list skin as a fraction of the largest radius (0: cell search)")
+    ap.add_argument("--separate-rebuild", action="store_true",
+                    help="list rebuilds as separate sort, reorder and list launches")
     ap.add_argument("--save-packing", default="",
                     help="write the prepared shear box to this folder (<sid>.bin)")
     ap.add_argument("--load-packing", default="",
@@ -438,7 +440,10 @@ def main():
         return
     # the largest lognormal grains touch more than 32 others: 64 contact slots
     engine = sphere_amd.Engine(0, nc=64)
-    run_engine = engine if args.skin is None else sphere_amd.Engine(0, nc=64, skin=args.skin)
+    run_engine = engine
+    if args.skin is not None or args.separate_rebuild:
+        run_engine = sphere_amd.Engine(0, nc=64, skin=args.skin,
+                                       fused_rebuild=not args.separate_rebuild)
     if args.load_packing:
         s = sphere_amd.sim()
         s.readbin(args.load_packing, verbose=False)

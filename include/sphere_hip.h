@@ -194,6 +194,14 @@ int sph_set_contact_slots(sph_ctx *ctx, int nc);
 int sph_set_neighbor_lists(sph_ctx *ctx, double skin_fraction, int cap);
 int sph_get_neighbor_stats(sph_ctx *ctx, sph_neighbor_stats *out);
 
+/* How list mode rebuilds its lists.  No counterpart in the reference.
+ * fused = 1 (default): the cell sort, reorder and list build run as one
+ * kernel with grid barriers, launched every step, that returns at once
+ * unless the rebuild flag is set; fused = 0: the sort passes, reorder and
+ * list build are separate launches that each return at once between
+ * rebuilds.  Both give the same arrays.  Takes effect at the next load. */
+int sph_set_rebuild_kernel(sph_ctx *ctx, int fused);
+
 /* Load a state: replaces DEM::readbin + checkValues + transfer to the device
  * (src/file_io.cpp:30-417, src/sphere.cpp:145-345, src/device.cu:667-752).
  * Resets the time-loop counters (iter = 0, as every reference invocation). */

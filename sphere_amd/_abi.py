@@ -171,6 +171,7 @@ def load_library(path=None):
         "sph_last_error": (ctypes.c_char_p, [vp]),
         "sph_set_contact_slots": (ctypes.c_int, [vp, ctypes.c_int]),
         "sph_set_neighbor_lists": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int]),
+        "sph_set_rebuild_kernel": (ctypes.c_int, [vp, ctypes.c_int]),
         "sph_get_neighbor_stats": (ctypes.c_int, [vp, ctypes.POINTER(SphNeighborStats)]),
         "sph_load_state": (ctypes.c_int, [vp, ctypes.POINTER(SphState)]),
         "sph_load_file": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
@@ -225,17 +226,21 @@ class SphereError(RuntimeError):
 class Engine:
     """One library context (one device, one loaded state)."""
 
-    def __init__(self, device=0, lib=None, nc=32, skin=None, nl_cap=128):
+    def __init__(self, device=0, lib=None, nc=32, skin=None, nl_cap=128, fused_rebuild=True):
         """nc: contact slots per particle (32 or 64); skin: neighbour-list
         skin as a fraction of the largest radius (None keeps the library
         default, 0 searches the 27 cells every step like the reference);
-        nl_cap: neighbour-list entries per particle."""
+        nl_cap: neighbour-list entries per particle; fused_rebuild: list
+        rebuilds as one kernel (True) or as separate sort, reorder and list
+        launches (False)."""
         self.lib = lib or load_library()
         self.ctx = self.lib.sph_create(device)
         if not self.ctx:
             raise SphereError(-4, self.lib.sph_last_error(None).decode())
         self.nc = int(nc)
         self._check(self.lib.sph_set_contact_slots(self.ctx, self.nc))
+        if not fused_rebuild:
+            self._check(self.lib.sph_set_rebuild_kernel(self.ctx, 0))
         if skin is not None or nl_cap != 128:
             self._check(self.lib.sph_set_neighbor_lists(
                 self.ctx, 0.1 if skin is None else float(skin), int(nl_cap)))

@@ -20,6 +20,7 @@
 #define ERRF_GRID 1u       // particle outside the grid
 #define ERRF_OVERFLOW 2u   // more than NCS contacts
 #define ERRF_NAN 4u        // NaN Darcy residual
+#define ERRF_BARRIER 8u    // a grid barrier wait ran past its bound
 
 struct Grid {
     double origo[3];

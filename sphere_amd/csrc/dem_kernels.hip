@@ -181,6 +181,25 @@ void launch_hash_nl(hipStream_t s, const StepArgs &a, const NLArgs &n, const Dem
 #define SKIP_UNLESS(cond) \
     if ((cond) != nullptr && (*(cond) & NLF_REBUILD) == 0u) return
 
+// Digit histogram of one tile of RS_TILE keys (RS_THREADS threads), written
+// digit-major: hist[digit * ntiles + tile].
+__device__ __forceinline__ void radix_hist_tile(const uint32_t *__restrict__ keys, uint32_t n,
+                                                int shift, uint32_t *__restrict__ hist,
+                                                uint32_t ntiles, uint32_t tile, uint32_t *h)
+{
+    const uint32_t t = threadIdx.x;
+    h[t] = 0;
+    __syncthreads();
+    const uint32_t base = tile * RS_TILE;
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const uint32_t i = base + j * RS_THREADS + t;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[t * ntiles + tile] = h[t];
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t *__restrict__ keys,
                                                            uint32_t n, int shift,
                                                            uint32_t *__restrict__ hist,
@@ -189,16 +208,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t *__res
 {
     SKIP_UNLESS(cond);
     __shared__ uint32_t h[256];
-    uint32_t t = threadIdx.x;
-    h[t] = 0;
-    __syncthreads();
-    uint32_t base = blockIdx.x * RS_TILE;
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        uint32_t i = base + j * RS_THREADS + t;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    hist[t * ntiles + blockIdx.x] = h[t];
+    radix_hist_tile(keys, n, shift, hist, ntiles, blockIdx.x, h);
 }
 
 // Exclusive scan of m entries in place, one block of 1024 threads.
@@ -230,34 +240,35 @@ __global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t *__restrict__ 
     }
 }
 
-__global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(
+// Stable scatter of one tile (RS_THREADS threads): a key's place is its
+// tile's digit offset plus its rank among the tile's earlier keys of the
+// same digit (wave ballots, then the counts of the earlier waves).
+__device__ __forceinline__ void radix_scatter_tile(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, int shift,
-    const uint32_t *__restrict__ offs, uint32_t ntiles, const uint32_t *__restrict__ cond)
+    const uint32_t *__restrict__ offs, uint32_t ntiles, uint32_t tile, uint32_t *run,
+    uint32_t (*wcnt)[256])
 {
-    SKIP_UNLESS(cond);
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[RS_WAVES][256];
-    uint32_t t = threadIdx.x;
-    uint32_t lane = t & 63u, wave = t >> 6;
-    run[t] = offs[t * ntiles + blockIdx.x];
+    const uint32_t t = threadIdx.x;
+    const uint32_t lane = t & 63u, wave = t >> 6;
+    run[t] = offs[t * ntiles + tile];
     for (int w = 0; w < RS_WAVES; ++w) wcnt[w][t] = 0;
     __syncthreads();
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    uint32_t base = blockIdx.x * RS_TILE;
+    const uint32_t base = tile * RS_TILE;
     for (int j = 0; j < RS_ITEMS; ++j) {
-        uint32_t i = base + j * RS_THREADS + t;
-        bool valid = i < n;
-        uint32_t key = valid ? kin[i] : 0u;
-        uint32_t val = valid ? vin[i] : 0u;
-        uint32_t d = (key >> shift) & 255u;
+        const uint32_t i = base + j * RS_THREADS + t;
+        const bool valid = i < n;
+        const uint32_t key = valid ? kin[i] : 0u;
+        const uint32_t val = valid ? vin[i] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; ++bit) {
-            uint64_t bb = __ballot((d >> bit) & 1u);
+            const uint64_t bb = __ballot((d >> bit) & 1u);
             peers &= ((d >> bit) & 1u) ? bb : ~bb;
         }
-        uint32_t rank = __popcll(peers & lt);
+        const uint32_t rank = __popcll(peers & lt);
         if (valid && rank == 0) wcnt[wave][d] = __popcll(peers);
         __syncthreads();
         if (valid) {
@@ -275,6 +286,17 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(
         run[t] += add;
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, int shift,
+    const uint32_t *__restrict__ offs, uint32_t ntiles, const uint32_t *__restrict__ cond)
+{
+    SKIP_UNLESS(cond);
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_WAVES][256];
+    radix_scatter_tile(kin, vin, kout, vout, n, shift, offs, ntiles, blockIdx.x, run, wcnt);
 }
 
 void launch_sort(hipStream_t s, uint32_t n, int bits, DemBuffers &b, const uint32_t *cond)
@@ -562,6 +584,208 @@ void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const De
     hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(256), 0, s, a, n, cond, b.vals, b.cell_start,
                        b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_pos, b.x_ref, b.nl_flags,
                        b.contacts, b.cmask, b.nl_slot, b.slot_cand, b.nl_gap, b.nl_gmin);
+}
+
+// ------------------------------------------------------------------
+// The whole list rebuild (cell sort, reorder, list build) in one launch
+// ------------------------------------------------------------------
+// Between rebuilds the sort, reorder and list kernels only read the rebuild
+// flag and return, and each such launch still costs its dispatch.  On the
+// list path they run as one kernel whose phases are separated by grid
+// barriers; a step without a rebuild then costs one dispatch.  The grid is
+// at most one block per compute unit, so every block is resident.
+
+// Grid barrier.  bar[0] counts arrivals, bar[1] is the generation; `gen` is
+// the generation this block waits to see end.  A waiter that spins longer
+// than RB_SPIN_LIMIT sleeps stops waiting and raises ERRF_BARRIER (the
+// step then fails), so the kernel always ends.
+#define RB_SPIN_LIMIT (1u << 22)
+__device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t nblocks, uint32_t &gen,
+                                             uint32_t *err)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t arrived =
+            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblocks - 1u) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (++spins > RB_SPIN_LIMIT) {
+                    atomicOr(err, ERRF_BARRIER);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __threadfence();
+    }
+    gen += 1u;
+    __syncthreads();
+}
+
+// Exclusive scan of 256 values over the block; returns the prefix of this
+// thread's value and the block total in *total.
+__device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t *lds, uint32_t *total)
+{
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    uint32_t incl = v;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) lds[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; ++w) before += lds[w];
+    *total = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return before + incl - v;
+}
+
+// Exclusive scan of a[0, m) in place by all blocks: block sums, one
+// barrier, then each block scans its chunk from the sum of the chunks before
+// it.
+__device__ void grid_scan(uint32_t *__restrict__ a, uint32_t m, uint32_t *__restrict__ part,
+                          uint32_t *bar, uint32_t nblocks, uint32_t &gen, uint32_t *err,
+                          uint32_t *lds)
+{
+    const uint32_t t = threadIdx.x;
+    const uint32_t chunk = (m + nblocks - 1u) / nblocks;
+    const uint32_t lo = min(blockIdx.x * chunk, m), hi = min(lo + chunk, m);
+    uint32_t sum = 0, tot;
+    for (uint32_t i = lo + t; i < hi; i += 256u) sum += a[i];
+    (void)block_scan_256(sum, lds, &tot);
+    if (t == 0) part[blockIdx.x] = tot;
+    grid_barrier(bar, nblocks, gen, err);
+    uint32_t off = 0;
+    for (uint32_t b = t; b < blockIdx.x; b += 256u) off += part[b];
+    (void)block_scan_256(off, lds, &off);
+    // each thread takes `per` consecutive entries of the chunk
+    const uint32_t per = (chunk + 255u) / 256u;
+    const uint32_t s0 = min(lo + t * per, hi), s1 = min(s0 + per, hi);
+    uint32_t mine = 0;
+    for (uint32_t i = s0; i < s1; ++i) mine += a[i];
+    uint32_t run = off + block_scan_256(mine, lds, &tot);
+    for (uint32_t i = s0; i < s1; ++i) {
+        const uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_rebuild(StepArgs a, NLArgs n, RebuildArgs r)
+{
+    if ((*r.cond & NLF_REBUILD) == 0u) return;
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_WAVES][256];
+    const uint32_t G = gridDim.x;
+    uint32_t gen = 0;
+    if (threadIdx.x == 0)
+        gen = __hip_atomic_load(&r.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t gtid = blockIdx.x * 256u + threadIdx.x, gsz = G * 256u;
+    for (uint32_t c = gtid; c < r.ncells; c += gsz) {
+        r.cell_start[c] = EMPTY_CELL;
+        r.cell_end[c] = EMPTY_CELL;
+    }
+    // LSD radix passes, keys and values ping-ponged between the two buffers
+    for (uint32_t pass = 0; pass < r.passes; ++pass) {
+        const int shift = (int)(8u * pass);
+        const uint32_t src = pass & 1u;
+        for (uint32_t tile = blockIdx.x; tile < r.ntiles; tile += G)
+            radix_hist_tile(r.keys[src], r.n, shift, r.hist, r.ntiles, tile, run);
+        grid_barrier(r.bar, G, gen, r.err);
+        grid_scan(r.hist, 256u * r.ntiles, r.part, r.bar, G, gen, r.err, run);
+        grid_barrier(r.bar, G, gen, r.err);
+        for (uint32_t tile = blockIdx.x; tile < r.ntiles; tile += G)
+            radix_scatter_tile(r.keys[src], r.vals[src], r.keys[src ^ 1u], r.vals[src ^ 1u], r.n,
+                               shift, r.hist, r.ntiles, tile, run, wcnt);
+        grid_barrier(r.bar, G, gen, r.err);
+    }
+    const uint32_t *keys = r.keys[r.passes & 1u], *vals = r.vals[r.passes & 1u];
+    const uint32_t np = a.np;
+    for (uint32_t idx = gtid; idx < np; idx += gsz) {
+        const uint32_t c = keys[idx];
+        if (idx == 0 || c != keys[idx - 1]) {
+            r.cell_start[c] = idx;
+            if (idx > 0) r.cell_end[keys[idx - 1]] = idx;
+        }
+        if (idx == np - 1) r.cell_end[c] = idx + 1;
+        const uint32_t sidx = vals[idx];
+        r.x_s[idx] = r.x[sidx];
+        r.vel_s[idx] = r.vel[sidx];
+        r.angvel_s[idx] = r.angvel[sidx];
+    }
+    grid_barrier(r.bar, G, gen, r.err);
+    if (blockIdx.x == 0 && threadIdx.x == 0) r.flags[NLF_COUNT] += 1u;  // rebuild counter
+    if (blockIdx.x == 0 && threadIdx.x < NLF_DSLOTS)  // displacements restart
+        ((unsigned long long *)(r.flags + NLF_DMAX))[threadIdx.x] = 0ull;
+    for (uint32_t pos = gtid; pos < np; pos += gsz)
+        nl_build_one(a, n, pos, vals, r.cell_start, r.cell_end, r.x_s, r.nl, r.nl_cnt, r.nl_pos,
+                     r.x_ref, r.flags, r.contacts, r.cmask, r.nl_slot, r.slot_cand, r.nl_gap,
+                     r.nl_gmin);
+}
+
+int rebuild_blocks_per_cu()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_rebuild, 256, 0) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+void launch_rebuild(hipStream_t s, const StepArgs &a, const NLArgs &n, DemBuffers &b,
+                    uint32_t ncells, int bits, uint32_t blocks, const uint32_t *cond)
+{
+    if (a.np == 0) return;
+    int passes = (bits + 7) / 8;
+    if (passes < 1) passes = 1;
+    RebuildArgs r;
+    r.cond = cond;
+    r.keys[0] = b.keys;
+    r.keys[1] = b.keys_tmp;
+    r.vals[0] = b.vals;
+    r.vals[1] = b.vals_tmp;
+    r.hist = b.hist;
+    r.part = b.rb_part;
+    r.n = a.np;
+    r.ntiles = (a.np + RS_TILE - 1) / RS_TILE;
+    r.passes = (uint32_t)passes;
+    r.ncells = ncells;
+    r.cell_start = b.cell_start;
+    r.cell_end = b.cell_end;
+    r.x = b.x;
+    r.vel = b.vel;
+    r.angvel = b.angvel;
+    r.x_s = b.x_s;
+    r.vel_s = b.vel_s;
+    r.angvel_s = b.angvel_s;
+    r.bar = b.rb_bar;
+    r.err = b.errflags;
+    r.nl = b.nl;
+    r.nl_cnt = b.nl_cnt;
+    r.nl_pos = b.nl_pos;
+    r.x_ref = b.x_ref;
+    r.flags = b.nl_flags;
+    r.contacts = b.contacts;
+    r.cmask = b.cmask;
+    r.nl_slot = b.nl_slot;
+    r.slot_cand = b.slot_cand;
+    r.nl_gap = b.nl_gap;
+    r.nl_gmin = b.nl_gmin;
+    hipLaunchKernelGGL(k_rebuild, dim3(blocks), dim3(256), 0, s, a, n, r);
+    // the sorted keys and values end in the buffer the last pass wrote
+    if (passes & 1) {
+        uint32_t *tk = b.keys;
+        b.keys = b.keys_tmp;
+        b.keys_tmp = tk;
+        uint32_t *tv = b.vals;
+        b.vals = b.vals_tmp;
+        b.vals_tmp = tv;
+    }
 }
 
 // Sum and maximum of the list lengths (overflowed lists count as cap + 1).

@@ -59,6 +59,8 @@ struct sph_ctx {
     // neighbour lists
     double nl_frac = 0.1;    // requested skin / largest radius
     uint32_t nl_cap = 128;   // list entries per particle
+    bool rb_fused = true;    // list rebuild as one launch (k_rebuild)
+    uint32_t rb_blocks = 0;  // its grid; 0: separate sort, reorder and list launches
     bool nl_on = false;
     NLArgs nla{};
     uint32_t nl_parity = 0;
@@ -196,6 +198,11 @@ static int check_flags(sph_ctx *ctx)
                                        "instability? is the time step too large?)");
     if (f & ERRF_OVERFLOW)
         return fail(ctx, SPH_ERR_OVERFLOW, "more contacts on one particle than contact slots (" + std::to_string(ctx->nc) + ")");
+    if (f & ERRF_BARRIER) {
+        if (ctx->db.rb_bar) launch_fill_u32(ctx->stream, ctx->db.rb_bar, 2, 0u);
+        return fail(ctx, SPH_ERR_HIP, "the list rebuild kernel's grid barrier timed out "
+                                      "(its blocks were not all resident)");
+    }
     return SPH_OK;
 }
 
@@ -537,6 +544,22 @@ static int upload(sph_ctx *ctx)
             launch_fill_u32(s, (uint32_t *)b.slot_cand, (np * NCS + 3) / 4, 0xffffffffu);
             launch_fill_f64(s, (double *)b.x_ref, np * 4, 0.0);
             ctx->nl_on = true;
+            // one resident block per compute unit at most (the rebuild
+            // kernel's grid barriers need every block running)
+            ctx->rb_blocks = 0;
+            int cus = 0;
+            if (ctx->rb_fused && rebuild_blocks_per_cu() > 0 &&
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                      ctx->device) == hipSuccess && cus > 0) {
+                const uint32_t want = (uint32_t)((np + 255) / 256);
+                const uint32_t cap = (uint32_t)(rebuild_blocks_per_cu() * cus / 4);
+                const uint32_t blocks =
+                    std::max<uint32_t>(1u, std::min<uint32_t>(std::min<uint32_t>((uint32_t)cus, cap), want));
+                AL(b.rb_bar, uint32_t, 2);
+                AL(b.rb_part, uint32_t, blocks);
+                launch_fill_u32(s, b.rb_bar, 2, 0u);
+                ctx->rb_blocks = blocks;
+            }
         }
     }
     ctx->nl_parity = 0;
@@ -939,9 +962,15 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
                 ProfScope ps(ctx, SPH_K_HASH);
                 launch_hash_nl(s, a, ctx->nla, ctx->db, cur, force);
             }
-            { ProfScope ps(ctx, SPH_K_SORT); launch_sort(s, a.np, ctx->key_bits, ctx->db, cond); }
-            { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells, cond); }
-            { ProfScope ps(ctx, SPH_K_NLIST); launch_nl_build(s, a, ctx->nla, ctx->db, cond); }
+            if (ctx->rb_blocks) {
+                ProfScope ps(ctx, SPH_K_NLIST);
+                launch_rebuild(s, a, ctx->nla, ctx->db, ctx->ncells, ctx->key_bits,
+                               ctx->rb_blocks, cond);
+            } else {
+                { ProfScope ps(ctx, SPH_K_SORT); launch_sort(s, a.np, ctx->key_bits, ctx->db, cond); }
+                { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells, cond); }
+                { ProfScope ps(ctx, SPH_K_NLIST); launch_nl_build(s, a, ctx->nla, ctx->db, cond); }
+            }
             { ProfScope ps(ctx, SPH_K_TOPOLOGY);
               launch_topology(s, a, ctx->db, true, ctx->nla.range == 1, cur); }
             rc = wait_walls(ctx);
@@ -1136,6 +1165,13 @@ int sph_set_neighbor_lists(sph_ctx *ctx, double skin_fraction, int cap)
     if (cap < 1 || cap > 128) return fail(ctx, SPH_ERR_ARG, "list capacity must lie in [1, 128]");
     ctx->nl_frac = skin_fraction;
     ctx->nl_cap = (uint32_t)cap;
+    return SPH_OK;
+}
+
+int sph_set_rebuild_kernel(sph_ctx *ctx, int fused)
+{
+    if (!ctx) return fail(ctx, SPH_ERR_ARG, "null context");
+    ctx->rb_fused = fused != 0;
     return SPH_OK;
 }
 

@@ -43,6 +43,8 @@ struct DemBuffers {
     uint32_t *pp_off;                     // np+1: CSR offsets into bp_ent per particle
     uint32_t nbp;
     uint32_t *errflags;
+    uint32_t *rb_bar;    // 2 words: the rebuild kernel's grid barrier (arrivals, generation)
+    uint32_t *rb_part;   // one word per block of the rebuild kernel
     // neighbour lists
     uint32_t *cellc;     // np, packed cell coordinates of this step
     double4 *x_ref;      // np, positions at the last rebuild
@@ -80,6 +82,28 @@ void launch_hash_nl(hipStream_t s, const StepArgs &a, const NLArgs &n, const Dem
 void launch_sort(hipStream_t s, uint32_t n, int bits, DemBuffers &b, const uint32_t *cond);
 void launch_reorder(hipStream_t s, const StepArgs &a, const DemBuffers &b, uint32_t ncells,
                     const uint32_t *cond);
+// The list rebuild (sort, reorder, list build) as one launch of `blocks`
+// resident blocks; returns at once unless the rebuild flag at cond is set.
+struct RebuildArgs {
+    const uint32_t *cond;
+    uint32_t *keys[2], *vals[2];
+    uint32_t *hist, *part;
+    uint32_t n, ntiles, passes, ncells;
+    uint32_t *cell_start, *cell_end;
+    const double4 *x, *vel, *angvel;
+    double4 *x_s, *vel_s, *angvel_s;
+    uint32_t *bar, *err;
+    uint32_t *nl, *nl_cnt, *nl_pos;
+    double4 *x_ref;
+    uint32_t *flags;
+    const uint32_t *contacts;
+    const uint64_t *cmask;
+    uint8_t *nl_slot, *slot_cand;
+    float *nl_gap, *nl_gmin;
+};
+int rebuild_blocks_per_cu();
+void launch_rebuild(hipStream_t s, const StepArgs &a, const NLArgs &n, DemBuffers &b,
+                    uint32_t ncells, int bits, uint32_t blocks, const uint32_t *cond);
 void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const DemBuffers &b,
                      const uint32_t *cond);
 // out[0] += sum of list lengths, out[1] = max(out[1], longest list)
