@@ -22,6 +22,18 @@
 #define ERRF_NAN 4u        // NaN Darcy residual
 #define ERRF_BARRIER 8u    // a grid barrier wait ran past its bound
 
+// A contact slot holds one 32-bit word: the partner's index in bits 0-27
+// and the periodic image code in bits 28-31 (bit 0: +L0, 1: -L0, 2: +L1,
+// 3: -L1).  An empty slot holds np with code 0.  Particle counts stay below
+// 2^28.
+#define CT_IDX_MASK 0x0fffffffu
+#define CT_CODE_SHIFT 28
+#define CT_MAX_NP (1u << 28)
+__host__ __device__ inline uint32_t ct_word(uint32_t b, uint32_t code)
+{
+    return b | (code << CT_CODE_SHIFT);
+}
+
 struct Grid {
     double origo[3];
     double L[3];

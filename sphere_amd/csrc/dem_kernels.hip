@@ -461,7 +461,7 @@ __device__ __forceinline__ int find_slot(const uint32_t *row, uint64_t mask, uin
     while (mask) {
         const int i = __ffsll((unsigned long long)mask) - 1;
         mask &= mask - 1;
-        if (row[i] == b) return i;
+        if ((row[i] & CT_IDX_MASK) == b) return i;
     }
     return -1;
 }
@@ -904,7 +904,7 @@ __device__ __forceinline__ uint32_t nl_first(const Grid &g, int gx, int gy, int 
 __device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 found, Set128 fresh,
                             Set128 recode, const uint32_t *__restrict__ nl,
                             uint8_t *__restrict__ nl_slot, const uint32_t *__restrict__ cellc,
-                            int gx, int gy, int gz, uint32_t *row, uint8_t *crow, uint8_t *scand,
+                            int gx, int gy, int gz, uint32_t *row, uint8_t *scand,
                             uint64_t &mask, int NCS, uint64_t full)
 {
     const int n_found = __popcll(found.lo) + __popcll(found.hi);
@@ -922,7 +922,8 @@ __device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 fou
                 uint32_t offc = 0, code = 0;
                 d3 dm;
                 (void)nl_image(g, gx, gy, gz, cellc[nl[li]], offc, code, dm);
-                crow[nl_slot[li]] = (uint8_t)code;
+                const uint8_t sl = nl_slot[li];
+                row[sl] = ct_word(row[sl] & CT_IDX_MASK, code);
             }
         }
         while (any_bit(fresh)) {
@@ -930,8 +931,7 @@ __device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 fou
             const uint32_t k = nl_first(g, gx, gy, gz, fresh, nl, np, pos, cellc, b, offc, code);
             clear_bit(fresh, k);
             const int cpos = 63 - __clzll((long long)(~mask & full));
-            row[cpos] = b;
-            crow[cpos] = (uint8_t)code;
+            row[cpos] = ct_word(b, code);
             mask |= 1ull << cpos;
             nl_slot[(size_t)k * np + pos] = (uint8_t)cpos;
             scand[cpos] = (uint8_t)k;
@@ -955,8 +955,7 @@ __device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 fou
         if (cpos < NCS) {
             const uint8_t prev = scand[cpos];
             if (prev != 0xffu && prev != (uint8_t)k) nl_slot[(size_t)prev * np + pos] = 0xffu;
-            row[cpos] = b;
-            crow[cpos] = (uint8_t)code;
+            row[cpos] = ct_word(b, code);
             mask |= 1ull << cpos;
             nl_slot[(size_t)k * np + pos] = (uint8_t)cpos;
             scand[cpos] = (uint8_t)k;
@@ -1291,7 +1290,7 @@ __device__ bool topology_cells(const Grid &g, uint32_t idx_a, double4 xa, int gx
                                int gz, const uint32_t *__restrict__ vals,
                                const uint32_t *__restrict__ cell_start,
                                const uint32_t *__restrict__ cell_end,
-                               const double4 *__restrict__ x_s, uint32_t *row, uint8_t *crow,
+                               const double4 *__restrict__ x_s, uint32_t *row,
                                uint64_t &mask, int NCS, uint64_t full)
 {
     int nc = 0;
@@ -1324,8 +1323,7 @@ __device__ bool topology_cells(const Grid &g, uint32_t idx_a, double4 xa, int gx
                         cpos = emp ? 63 - __clzll((long long)emp) : nc;
                     }
                     if (cpos < NCS) {
-                        row[cpos] = b_orig;
-                        if (crow[cpos] != (uint8_t)code) crow[cpos] = (uint8_t)code;
+                        row[cpos] = ct_word(b_orig, code);
                         mask |= 1ull << cpos;
                     } else {
                         ovf = true;
@@ -1343,7 +1341,7 @@ __global__ void __launch_bounds__(256) k_topology_cells(
     StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
     const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x_s,
     uint32_t *__restrict__ contacts, uint64_t *__restrict__ cmask,
-    uint8_t *__restrict__ dmcode, uint32_t *__restrict__ err)
+    uint32_t *__restrict__ err)
 {
     const uint32_t idx_a = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx_a >= a.np) return;
@@ -1362,8 +1360,7 @@ __global__ void __launch_bounds__(256) k_topology_cells(
     const uint64_t mask0 = cmask[a_orig];
     uint64_t mask = mask0;
     const bool ovf = topology_cells(g, idx_a, xa, gx, gy, gz, vals, cell_start, cell_end, x_s,
-                                    contacts + (size_t)a_orig * NCS,
-                                    dmcode + (size_t)a_orig * NCS, mask, NCS, full);
+                                    contacts + (size_t)a_orig * NCS, mask, NCS, full);
     if (ovf) atomicOr(err, ERRF_OVERFLOW);
     if (mask != mask0) cmask[a_orig] = mask;
 }
@@ -1407,7 +1404,7 @@ __device__ void owner_test_all(const Grid &g, uint32_t np, uint32_t pos, uint32_
                                const uint32_t *__restrict__ nl,
                                const uint8_t *__restrict__ nl_slot,
                                const uint32_t *__restrict__ cellc, const double4 *__restrict__ x,
-                               double4 xa, int gx, int gy, int gz, const uint8_t *crow,
+                               double4 xa, int gx, int gy, int gz, const uint32_t *row,
                                Set128 &found, Set128 &fresh, Set128 &recode)
 {
     found = fresh = recode = Set128{0ull, 0ull};
@@ -1432,7 +1429,7 @@ __device__ void owner_test_all(const Grid &g, uint32_t np, uint32_t pos, uint32_
         set_bit(found, k);
         const uint8_t sl = nl_slot[li];
         if (sl == 0xffu) set_bit(fresh, k);
-        else if (crow[sl] != (uint8_t)code) set_bit(recode, k);
+        else if ((row[sl] >> CT_CODE_SHIFT) != code) set_bit(recode, k);
     }
 }
 
@@ -1445,7 +1442,7 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
     const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x,
     const double4 *__restrict__ x_s, uint32_t *__restrict__ contacts,
-    uint64_t *__restrict__ cmask, uint8_t *__restrict__ dmcode, uint32_t *__restrict__ err,
+    uint64_t *__restrict__ cmask, uint32_t *__restrict__ err,
     const uint32_t *__restrict__ nl, uint8_t *__restrict__ nl_slot,
     uint8_t *__restrict__ slot_cand, const uint32_t *__restrict__ nl_cnt,
     const uint32_t *__restrict__ nl_pos, const uint32_t *__restrict__ cellc,
@@ -1596,7 +1593,8 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
                         fl = 1;
                         if (sq[u] == 0xffu)
                             fl |= 2;
-                        else if (dmcode[(size_t)s_a[o] * NCS + sq[u]] != (uint8_t)code)
+                        else if ((contacts[(size_t)s_a[o] * NCS + sq[u]] >> CT_CODE_SHIFT) !=
+                                 code)
                             fl |= 4;
                     }
                 }
@@ -1616,28 +1614,27 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     if (!busy) return;
     const uint64_t fullmask = NCS >= 64 ? ~0ull : ((1ull << NCS) - 1ull);
     uint32_t *row = contacts + (size_t)a_orig * NCS;
-    uint8_t *crow = dmcode + (size_t)a_orig * NCS;
     const uint64_t mask0 = cmask[a_orig];
     uint64_t mask = mask0;
     bool ovf = false;
     if (list_ovf) {
         ovf = topology_cells(g, nl_pos[a_orig], xa, gx, gy, gz, vals, cell_start, cell_end, x_s,
-                             row, crow, mask, NCS, fullmask);
+                             row, mask, NCS, fullmask);
     } else if (full) {
         if (any_bit(fresh) || any_bit(recode) || __popcll(found.lo) + __popcll(found.hi) > NCS)
             ovf = slot_update(g, np, pos, found, fresh, recode, nl, nl_slot, cellc, gx, gy, gz,
-                              row, crow, slot_cand + (size_t)a_orig * NCS, mask, NCS, fullmask);
+                              row, slot_cand + (size_t)a_orig * NCS, mask, NCS, fullmask);
     } else if (any_bit(fresh)) {
         // only new contacts were tested; if they do not all fit into empty
         // slots the full rule needs every contact found: test all entries
         const int n_new = __popcll(fresh.lo) + __popcll(fresh.hi);
         if (__popcll(mask) + n_new > NCS)
-            owner_test_all<NEAR>(g, np, pos, lcnt, nl, nl_slot, cellc, x, xa, gx, gy, gz, crow,
+            owner_test_all<NEAR>(g, np, pos, lcnt, nl, nl_slot, cellc, x, xa, gx, gy, gz, row,
                                  found, fresh, recode);
         else
             found = fresh;
         ovf = slot_update(g, np, pos, found, fresh, recode, nl, nl_slot, cellc, gx, gy, gz, row,
-                          crow, slot_cand + (size_t)a_orig * NCS, mask, NCS, fullmask);
+                          slot_cand + (size_t)a_orig * NCS, mask, NCS, fullmask);
     }
     if (ovf) atomicOr(err, ERRF_OVERFLOW);
     if (mask != mask0) cmask[a_orig] = mask;
@@ -1747,7 +1744,7 @@ __global__ void __launch_bounds__(FW) k_forces(
     double *__restrict__ p_out, const double4 *__restrict__ w_nx,
     const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
     uint32_t *__restrict__ contacts, uint64_t *__restrict__ cmask,
-    const uint8_t *__restrict__ dmcode, double *__restrict__ delta_t,
+    double *__restrict__ delta_t,
     uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand,
     const float *__restrict__ nl_gap, float *__restrict__ nl_gmin)
 {
@@ -1798,8 +1795,9 @@ __global__ void __launch_bounds__(FW) k_forces(
             const uint32_t sl = s_slot[lane];
             const size_t mempos = (size_t)ao * NCS + sl;
             const double4 xa = x[ao];
-            const uint32_t b = contacts[mempos];
-            const d3 dm = decode_dm(g, dmcode[mempos]);
+            const uint32_t cw = contacts[mempos];
+            const uint32_t b = cw & CT_IDX_MASK;
+            const d3 dm = decode_dm(g, cw >> CT_CODE_SHIFT);
             const double4 xb = x[b];
             const d3 x_ab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
             const double l = len3(x_ab);
@@ -1957,17 +1955,17 @@ void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool
     if (nlist && near)
         hipLaunchKernelGGL(k_topology_lists<true>, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s, a,
                            b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
-                           b.dmcode, b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
+                           b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
                            b.nl_pos, b.cellc, b.nl_gap, b.nl_gmin, b.nl_flags, cur);
     else if (nlist)
         hipLaunchKernelGGL(k_topology_lists<false>, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s,
                            a, b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
-                           b.dmcode, b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
+                           b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
                            b.nl_pos, b.cellc, b.nl_gap, b.nl_gmin, b.nl_flags, cur);
     else
         hipLaunchKernelGGL(k_topology_cells, dim3((a.np + 255) / 256), dim3(256), 0, s, a,
                            b.vals, b.cell_start, b.cell_end, b.x_s, b.contacts, b.cmask,
-                           b.dmcode, b.errflags);
+                           b.errflags);
 }
 
 void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool nlist)
@@ -1989,12 +1987,12 @@ void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool 
     if (a.prm.contactmodel == 3)
         hipLaunchKernelGGL(k_forces<true>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
                            b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
-                           b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask, b.dmcode,
+                           b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask,
                            b.delta_t, nls, scd, gap, gmin);
     else
         hipLaunchKernelGGL(k_forces<false>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
                            b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
-                           b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask, b.dmcode,
+                           b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask,
                            b.delta_t, nls, scd, gap, gmin);
 }
 

@@ -263,6 +263,10 @@ static int check_values(sph_ctx *ctx, const HostState &hs)
     const sph_state &s = hs.h;
     char m[256];
     if (s.nd != 3) { snprintf(m, sizeof(m), "nd = %d, ND = 3", s.nd); return fail(ctx, SPH_ERR_VALUE, m); }
+    if (s.np >= CT_MAX_NP) {  // contact slot words keep 28 bits for the partner
+        snprintf(m, sizeof(m), "np = %u; at most %u particles", s.np, CT_MAX_NP - 1u);
+        return fail(ctx, SPH_ERR_VALUE, m);
+    }
     if (s.time_current > s.time_total || s.time_current < 0.0) {
         snprintf(m, sizeof(m), "time.current = %g s, time.total = %g s", s.time_current, s.time_total);
         return fail(ctx, SPH_ERR_VALUE, m);
@@ -360,7 +364,7 @@ static int upload(sph_ctx *ctx)
     AL(b.x_s, double4, np); AL(b.vel_s, double4, np); AL(b.angvel_s, double4, np);
     const size_t NCS = ctx->nc;
     AL(b.contacts, uint32_t, np * NCS); AL(b.cmask, uint64_t, np);
-    AL(b.dmcode, uint8_t, np * NCS); AL(b.delta_t, double, np * NCS * 3);
+    AL(b.delta_t, double, np * NCS * 3);
     AL(b.w_nx, double4, MAXW); AL(b.w_mvfd, double4, MAXW); AL(b.w_acc, double, MAXW);
     AL(b.walls_force_pp, double, np * (h.nw ? h.nw : 1));
     AL(b.wall_partial, double, nblocks);
@@ -393,7 +397,6 @@ static int upload(sph_ctx *ctx)
     launch_fill_f64(s, (double *)b.angacc, np * 4, 0.0);
     launch_fill_u32(s, b.contacts, np * NCS, (uint32_t)np);  // device.cu:471-473
     launch_fill_u32(s, (uint32_t *)b.cmask, 2 * np, 0u);
-    launch_fill_u32(s, (uint32_t *)b.dmcode, (np * NCS + 3) / 4, 0u);
     launch_fill_f64(s, b.delta_t, np * NCS * 3, 0.0);
     launch_fill_f64(s, b.walls_force_pp, np * (h.nw ? h.nw : 1), 0.0);
     launch_fill_f64(s, b.w_acc, MAXW, 0.0);
@@ -1405,12 +1408,14 @@ int sph_read_contacts(sph_ctx *ctx, uint32_t *contacts, double *distmod, double 
     HIPCHK(hipStreamSynchronize(ctx->stream));
     const size_t n = (size_t)ctx->np * ctx->nc;
     if (!n) return SPH_OK;
-    std::vector<uint32_t> c(n);
+    std::vector<uint32_t> c(n), code(n);
     HIPCHK(hipMemcpy(c.data(), ctx->db.contacts, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+        code[i] = c[i] >> CT_CODE_SHIFT;
+        c[i] &= CT_IDX_MASK;
+    }
     if (contacts) std::memcpy(contacts, c.data(), n * 4);
     if (distmod) {
-        std::vector<uint8_t> code(n);
-        HIPCHK(hipMemcpy(code.data(), ctx->db.dmcode, n, hipMemcpyDeviceToHost));
         const sph_state &h = ctx->hs.h;
         for (size_t i = 0; i < n; ++i) {
             double dx = 0.0, dy = 0.0;
@@ -1438,7 +1443,7 @@ int sph_write_contacts(sph_ctx *ctx, const uint32_t *contacts, const double *dis
     const size_t n = (size_t)ctx->np * ctx->nc;
     if (!n) return SPH_OK;
     const sph_state &h = ctx->hs.h;
-    std::vector<uint8_t> code(n, 0);
+    std::vector<uint32_t> words(contacts, contacts + n);
     for (size_t i = 0; i < n; ++i) {
         if (contacts[i] > ctx->np) return fail(ctx, SPH_ERR_ARG, "contact index out of range");
         if (distmod && contacts[i] != ctx->np) {
@@ -1448,12 +1453,11 @@ int sph_write_contacts(sph_ctx *ctx, const uint32_t *contacts, const double *dis
                 return fail(ctx, SPH_ERR_ARG, "distmod x is not 0 or +-L[0]");
             if (dy == h.L[1]) cd |= 4; else if (dy == -h.L[1]) cd |= 8; else if (dy != 0.0)
                 return fail(ctx, SPH_ERR_ARG, "distmod y is not 0 or +-L[1]");
-            code[i] = cd;
+            words[i] = ct_word(contacts[i], cd);
         }
     }
     hipStream_t s = ctx->stream;
-    HIPCHK(hipMemcpyAsync(ctx->db.contacts, contacts, n * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(ctx->db.dmcode, code.data(), n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ctx->db.contacts, words.data(), n * 4, hipMemcpyHostToDevice, s));
     if (delta_t) HIPCHK(hipMemcpyAsync(ctx->db.delta_t, delta_t, n * 24, hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
     launch_rebuild_mask(s, ctx->db, ctx->np, ctx->nc);

@@ -23,9 +23,8 @@ struct DemBuffers {
     uint32_t *cell_start, *cell_end;
     double4 *x_s, *vel_s, *angvel_s;
     // contact history, original index * nc + slot
-    uint32_t *contacts;   // np*nc, empty = np
+    uint32_t *contacts;   // np*nc: partner index | image code << 28 (ct_word), empty = np
     uint64_t *cmask;      // np, bit i set iff contacts[.. + i] != np
-    uint8_t *dmcode;      // np*nc, periodic image code of each slot
     double *delta_t;      // np*nc*3
     // walls
     double4 *w_nx, *w_mvfd;
