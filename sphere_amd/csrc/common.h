@@ -125,6 +125,7 @@ __device__ __forceinline__ d3 cross3(d3 a, d3 b)
     return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 __device__ __forceinline__ d3 xyz(double4 v) { return mk3(v.x, v.y, v.z); }
+__device__ __forceinline__ d3 xyz(double3 v) { return mk3(v.x, v.y, v.z); }
 
 // Tile index of block b of nb such that each of the 8 XCDs, which receive
 // the blocks of a launch round-robin (b mod 8), works through one contiguous

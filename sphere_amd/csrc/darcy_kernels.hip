@@ -127,7 +127,7 @@ __device__ __forceinline__ double sphere_volume(double r) { return 4.0 / 3.0 * P
 __global__ void __launch_bounds__(256) k_pressure_force(StepArgs a, DarcyArgs d,
                                                         const double4 *__restrict__ xarr,
                                                         const double *__restrict__ grad_parr,
-                                                        double4 *__restrict__ force,
+                                                        double3 *__restrict__ force,
                                                         double4 *__restrict__ f_parr,
                                                         uint32_t *__restrict__ err)
 {
@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(256) k_pressure_force(StepArgs a, DarcyArgs d,
     const double v = sphere_volume(x4.w);
     d3 f_p = v * ((-1.0) * grad_p) - (d.rho_f * v) * mk3(a.prm.g[0], a.prm.g[1], a.prm.g[2]);
     if (d.bc_top == 0 && i_z >= d.wall0_iz) f_p.z = 0.0;
-    double4 F = force[i];
+    double3 F = force[i];
     F.x = F.x + f_p.x;
     F.y = F.y + f_p.y;
     F.z = F.z + f_p.z;

@@ -362,8 +362,8 @@ void launch_fill_f64(hipStream_t s, double *p, size_t n, double v)
 __global__ void __launch_bounds__(256) k_reorder(
     uint32_t np, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
     const double4 *__restrict__ x, const double4 *__restrict__ vel,
-    const double4 *__restrict__ angvel, double4 *__restrict__ x_s, double4 *__restrict__ vel_s,
-    double4 *__restrict__ angvel_s, uint32_t *__restrict__ cell_start,
+    const double3 *__restrict__ angvel, double4 *__restrict__ x_s, double4 *__restrict__ vel_s,
+    double3 *__restrict__ angvel_s, uint32_t *__restrict__ cell_start,
     uint32_t *__restrict__ cell_end, const uint32_t *__restrict__ cond)
 {
     SKIP_UNLESS(cond);
@@ -1020,7 +1020,7 @@ __device__ __forceinline__ double cohesion_factor(double kappa, double r_a, doub
 
 // contactLinear, src/contactmodels.cuh:241-408
 __device__ __forceinline__ void law_linear(const StepArgs &a, Contrib &C, d3 vel_a, d3 angvel_a,
-                                           double4 vb, double4 wb, double r_a, double r_b,
+                                           double4 vb, double3 wb, double r_a, double r_b,
                                            d3 x, double x_length, double delta, double *dts)
 {
     const Params &P = a.prm;
@@ -1078,7 +1078,7 @@ __device__ __forceinline__ void law_linear(const StepArgs &a, Contrib &C, d3 vel
 // ratio nu in the k_r field): E* = E/(2(1-nu^2)), R* = r_a r_b/(r_a+r_b),
 // G* = E/(2(1+nu)) / (2(2-nu)), k_n = 4/3 E* sqrt(R*), k_t = 8 G* sqrt(R*).
 __device__ __forceinline__ void law_hertz(const StepArgs &a, Contrib &C, d3 vel_a, d3 angvel_a,
-                                          double4 vb, double4 wb, double r_a, double r_b,
+                                          double4 vb, double3 wb, double r_a, double r_b,
                                           d3 x_ab, double x_ab_length, double delta_ab,
                                           double *dts)
 {
@@ -1161,7 +1161,7 @@ __device__ __forceinline__ void law_hertz(const StepArgs &a, Contrib &C, d3 vel_
 
 // contactLinearViscous, src/contactmodels.cuh:119-237 (model 1)
 __device__ __forceinline__ void law_linear_viscous(const StepArgs &a, Acc &A, double4 va4,
-                                                   double4 vb4, double4 wa4, double4 wb4,
+                                                   double4 vb4, double3 wa4, double3 wb4,
                                                    double r_a, double r_b, d3 x_ab,
                                                    double x_ab_length, double delta_ab,
                                                    double kappa)
@@ -1217,7 +1217,7 @@ __device__ __forceinline__ void law_capillary(const StepArgs &a, Acc &A, double 
 }
 
 // contactLinear_wall, src/contactmodels.cuh:10-114
-__device__ __forceinline__ double law_wall(const StepArgs &a, Acc &A, double4 va4, double4 wa4,
+__device__ __forceinline__ double law_wall(const StepArgs &a, Acc &A, double4 va4, double3 wa4,
                                            double radius_a, d3 n, double delta, double wvel)
 {
     const Params &P = a.prm;
@@ -1645,7 +1645,7 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
 // and for non-periodic axes walls 1-4.  Returns the normal forces on walls
 // 0-4 in wf0..wf4.
 __device__ __forceinline__ void walls_contact(const StepArgs &a, Acc &A, double4 xa, double4 va4,
-                                              double4 wa4, const double4 *__restrict__ w_nx,
+                                              double3 wa4, const double4 *__restrict__ w_nx,
                                               const double4 *__restrict__ w_mvfd, double &wf0,
                                               double &wf1, double &wf2, double &wf3,
                                               double &wf4)
@@ -1682,8 +1682,8 @@ __device__ __forceinline__ void walls_contact(const StepArgs &a, Acc &A, double4
 
 // contactsearch.cuh:706-724
 __device__ __forceinline__ void write_sums(const StepArgs &a, const Acc &A, uint32_t a_orig,
-                                           double4 *__restrict__ force,
-                                           double4 *__restrict__ torque,
+                                           double3 *__restrict__ force,
+                                           double3 *__restrict__ torque,
                                            double *__restrict__ es_dot_out,
                                            double *__restrict__ ev_dot_out,
                                            double *__restrict__ es, double *__restrict__ ev,
@@ -1692,8 +1692,8 @@ __device__ __forceinline__ void write_sums(const StepArgs &a, const Acc &A, uint
                                            double wf1, double wf2, double wf3, double wf4)
 {
     const size_t np = a.np;
-    force[a_orig] = make_double4(A.F.x, A.F.y, A.F.z, 0.0);
-    torque[a_orig] = make_double4(A.T.x, A.T.y, A.T.z, 0.0);
+    force[a_orig] = make_double3(A.F.x, A.F.y, A.F.z);
+    torque[a_orig] = make_double3(A.T.x, A.T.y, A.T.z);
     es_dot_out[a_orig] = A.es_dot;
     ev_dot_out[a_orig] = A.ev_dot;
     es[a_orig] += A.es_dot * a.dt;
@@ -1738,8 +1738,8 @@ __device__ __forceinline__ void store_acc(double (*s)[64], uint32_t lane, const 
 template <bool HERTZ>
 __global__ void __launch_bounds__(FW) k_forces(
     StepArgs a, const uint32_t *__restrict__ order, const double4 *__restrict__ x,
-    const double4 *__restrict__ vel, const double4 *__restrict__ angvel,
-    double4 *__restrict__ force, double4 *__restrict__ torque, double *__restrict__ es_dot_out,
+    const double4 *__restrict__ vel, const double3 *__restrict__ angvel,
+    double3 *__restrict__ force, double3 *__restrict__ torque, double *__restrict__ es_dot_out,
     double *__restrict__ ev_dot_out, double *__restrict__ es, double *__restrict__ ev,
     double *__restrict__ p_out, const double4 *__restrict__ w_nx,
     const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
@@ -1809,7 +1809,8 @@ __global__ void __launch_bounds__(FW) k_forces(
             C.p = C.ev_n = C.ev_t = C.es = 0.0;
             uint8_t rm = 0, rmk = 0xffu;
             if (delta_n < 0.0) {
-                const double4 va4 = vel[ao], wa4 = angvel[ao];
+                const double4 va4 = vel[ao];
+                const double3 wa4 = angvel[ao];
                 double loc[3] = {dts[0], dts[1], dts[2]};
                 if (HERTZ)
                     law_hertz(a, C, xyz(va4), xyz(wa4), vel[b], angvel[b], xa.w, xb.w, x_ab, l,
@@ -1874,7 +1875,8 @@ __global__ void __launch_bounds__(FW) k_forces(
     }
     if (!valid) return;
     Acc A = load_acc(s_acc, lane);
-    const double4 xa = x[a_orig], va4 = vel[a_orig], wa4 = angvel[a_orig];
+    const double4 xa = x[a_orig], va4 = vel[a_orig];
+    const double3 wa4 = angvel[a_orig];
     double wf0, wf1, wf2, wf3, wf4;
     walls_contact(a, A, xa, va4, wa4, w_nx, w_mvfd, wf0, wf1, wf2, wf3, wf4);
     write_sums(a, A, a_orig, force, torque, es_dot_out, ev_dot_out, es, ev, p_out,
@@ -1888,8 +1890,8 @@ __global__ void __launch_bounds__(FW) k_forces(
 __global__ void __launch_bounds__(256) k_contact_viscous(
     StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
     const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x_s,
-    const double4 *__restrict__ vel_s, const double4 *__restrict__ angvel_s,
-    double4 *__restrict__ force, double4 *__restrict__ torque, double *__restrict__ es_dot_out,
+    const double4 *__restrict__ vel_s, const double3 *__restrict__ angvel_s,
+    double3 *__restrict__ force, double3 *__restrict__ torque, double *__restrict__ es_dot_out,
     double *__restrict__ ev_dot_out, double *__restrict__ es, double *__restrict__ ev,
     double *__restrict__ p_out, const double4 *__restrict__ w_nx,
     const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
@@ -1902,7 +1904,7 @@ __global__ void __launch_bounds__(256) k_contact_viscous(
     const double4 xa = x_s[idx_a];
     const double radius_a = xa.w;
     const double4 va4 = vel_s[idx_a];
-    const double4 wa4 = angvel_s[idx_a];
+    const double3 wa4 = angvel_s[idx_a];
     Acc A;
     A.F = mk3(0.0, 0.0, 0.0);
     A.T = mk3(0.0, 0.0, 0.0);
@@ -2004,7 +2006,7 @@ void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool 
 __global__ void __launch_bounds__(256) k_bond_compute(
     StepArgs a, const uint2 *__restrict__ bonds, uint8_t *__restrict__ broken,
     double4 *__restrict__ bdelta, double4 *__restrict__ bomega, const double4 *__restrict__ x,
-    const double4 *__restrict__ vel, const double4 *__restrict__ angvel,
+    const double4 *__restrict__ vel, const double3 *__restrict__ angvel,
     double4 *__restrict__ bf, double4 *__restrict__ bt, uint8_t *__restrict__ applied)
 {
     uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2069,12 +2071,12 @@ __global__ void __launch_bounds__(256) k_bond_gather(
     uint32_t nbp, const uint32_t *__restrict__ plist, const uint32_t *__restrict__ off,
     const uint32_t *__restrict__ ent, const double4 *__restrict__ bf,
     const double4 *__restrict__ bt, const uint8_t *__restrict__ applied,
-    double4 *__restrict__ force, double4 *__restrict__ torque)
+    double3 *__restrict__ force, double3 *__restrict__ torque)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nbp) return;
     uint32_t pi = plist[k];
-    double4 F = force[pi], T = torque[pi];
+    double3 F = force[pi], T = torque[pi];
     for (uint32_t e = off[k]; e < off[k + 1]; ++e) {
         uint32_t v = ent[e];
         uint32_t b = v >> 1;
@@ -2124,7 +2126,7 @@ __device__ __forceinline__ double block_sum_256(double v, double *cache)
 // findShearStressOnFixedMovingParticles + summation, src/integration.cuh:457-484
 __global__ void __launch_bounds__(256) k_shear_stress(StepArgs a, const double4 *__restrict__ x,
                                                       const double4 *__restrict__ vel,
-                                                      const double4 *__restrict__ force,
+                                                      const double3 *__restrict__ force,
                                                       double *__restrict__ tau_partial)
 {
     __shared__ double cache[256];
@@ -2150,26 +2152,34 @@ __global__ void __launch_bounds__(256) k_shear_stress(StepArgs a, const double4 
 #define SUM_CHUNK (SUM_THREADS * 8)
 __device__ double serial_sum_staged(const double *__restrict__ part, uint32_t n, double *lds)
 {
+    // Terms equal to +0.0 are skipped: the running sum starts at +0.0 and can
+    // never become -0.0 (a sum is -0.0 only when both addends are), and
+    // s + (+0.0) == s for every other s, so the result is the left-to-right
+    // sum bit for bit.  Most partials of a wall force are zero (only grains
+    // touching the wall contribute), which leaves lane 0 a short chain.
+    __shared__ uint32_t nzm[SUM_THREADS / 4];  // one byte per thread
     double s = 0.0;
+    const uint32_t t = threadIdx.x;
     for (uint32_t base = 0; base < n; base += SUM_CHUNK) {
         const uint32_t m = n - base < SUM_CHUNK ? n - base : SUM_CHUNK;
-        for (uint32_t j = threadIdx.x; j < m; j += SUM_THREADS) lds[j] = part[base + j];
+        for (uint32_t j = t; j < m; j += SUM_THREADS) lds[j] = part[base + j];
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t j = 0;
-            for (; j + 8 <= m; j += 8) {
-                const double v0 = lds[j], v1 = lds[j + 1], v2 = lds[j + 2], v3 = lds[j + 3];
-                const double v4 = lds[j + 4], v5 = lds[j + 5], v6 = lds[j + 6], v7 = lds[j + 7];
-                s += v0;
-                s += v1;
-                s += v2;
-                s += v3;
-                s += v4;
-                s += v5;
-                s += v6;
-                s += v7;
+        uint32_t bits = 0;
+        for (uint32_t k = 0; k < 8; ++k) {
+            const uint32_t j = 8u * t + k;
+            if (j < m && __double_as_longlong(lds[j]) != 0ll) bits |= 1u << k;
+        }
+        ((uint8_t *)nzm)[t] = (uint8_t)bits;
+        __syncthreads();
+        if (t == 0) {
+            for (uint32_t q = 0; q < SUM_THREADS; q += 4) {
+                uint32_t w = nzm[q / 4u];
+                while (w) {
+                    const uint32_t bit = (uint32_t)__ffs(w) - 1u;
+                    w &= w - 1u;
+                    s += lds[8u * q + bit];  // bit = 8 (thread - q) + k
+                }
             }
-            for (; j < m; ++j) s += lds[j];
         }
         __syncthreads();
     }
@@ -2196,7 +2206,7 @@ void launch_shear_stress(hipStream_t s, const StepArgs &a, const DemBuffers &b)
 // Bond contributions of entries [e0, e1) of one particle, in ascending bond
 // order (what k_bond_gather adds).
 __device__ __forceinline__ void bond_sum(const BondGather &bg, uint32_t e0, uint32_t e1,
-                                         double4 &F, double4 &T)
+                                         double3 &F, double3 &T)
 {
     for (uint32_t e = e0; e < e1; ++e) {
         const uint32_t v = bg.ent[e];
@@ -2247,10 +2257,10 @@ __device__ __forceinline__ void next_hash(const StepArgs &a, const NextHash &nh,
 __global__ void __launch_bounds__(256) k_integrate(
     StepArgs a, const StepClock *__restrict__ clock, int change_velocity_state, double velocity_factor,
     double tau_x,
-    double4 *__restrict__ x, double4 *__restrict__ vel, double4 *__restrict__ angvel,
-    double4 *__restrict__ force, double4 *__restrict__ torque,
-    double4 *__restrict__ angpos, double4 *__restrict__ acc_a, double4 *__restrict__ angacc_a,
-    double4 *__restrict__ xyzsum_a, const double4 *__restrict__ w_mvfd,
+    double4 *__restrict__ x, double4 *__restrict__ vel, double3 *__restrict__ angvel,
+    double3 *__restrict__ force, double3 *__restrict__ torque,
+    double3 *__restrict__ angpos, double3 *__restrict__ acc_a, double3 *__restrict__ angacc_a,
+    double3 *__restrict__ xyzsum_a, const double4 *__restrict__ w_mvfd,
     const double *__restrict__ tau_eff, const double *__restrict__ walls_force_pp,
     double *__restrict__ wall_partial, BondGather bg, NextHash nh)
 {
@@ -2274,7 +2284,7 @@ __global__ void __launch_bounds__(256) k_integrate(
     }
     double tau_eff_x = 0.0;
     if (a.walls.nw > 0 && wall0mode == 3) tau_eff_x = *tau_eff;
-    double4 f4 = force[i], t4 = torque[i];
+    double3 f4 = force[i], t4 = torque[i];
     if (bg.off) {
         // bond forces in ascending bond order (cohesion.cuh:220-226, D3),
         // added before integration as the separate gather would
@@ -2285,12 +2295,13 @@ __global__ void __launch_bounds__(256) k_integrate(
             torque[i] = t4;
         }
     }
-    const double4 ap = angpos[i];
-    const double4 xo = x[i], vo = vel[i], wo = angvel[i];
-    const double4 xs = xyzsum_a[i];
+    const double3 ap = angpos[i];
+    const double4 xo = x[i], vo = vel[i];
+    const double3 wo = angvel[i];
+    const double3 xs = xyzsum_a[i];
     double acc0[3] = {0.0, 0.0, 0.0}, angacc0[3] = {0.0, 0.0, 0.0};
     if (clock->iter != 0) {
-        const double4 a0 = acc_a[i], w0 = angacc_a[i];
+        const double3 a0 = acc_a[i], w0 = angacc_a[i];
         acc0[0] = a0.x; acc0[1] = a0.y; acc0[2] = a0.z;
         angacc0[0] = w0.x; angacc0[1] = w0.y; angacc0[2] = w0.z;
     }
@@ -2352,12 +2363,12 @@ __global__ void __launch_bounds__(256) k_integrate(
         if (change_velocity_state == 1) vn[0] *= velocity_factor;
         else if (change_velocity_state == -1) vn[0] /= velocity_factor;
     }
-    xyzsum_a[i] = make_double4(xsum[0], xsum[1], xsum[2], xs.w);
-    acc_a[i] = make_double4(acc[0], acc[1], acc[2], 0.0);
-    angacc_a[i] = make_double4(angacc[0], angacc[1], angacc[2], 0.0);
-    angvel[i] = make_double4(avn[0], avn[1], avn[2], wo.w);
+    xyzsum_a[i] = make_double3(xsum[0], xsum[1], xsum[2]);
+    acc_a[i] = make_double3(acc[0], acc[1], acc[2]);
+    angacc_a[i] = make_double3(angacc[0], angacc[1], angacc[2]);
+    angvel[i] = make_double3(avn[0], avn[1], avn[2]);
     vel[i] = make_double4(vn[0], vn[1], vn[2], vo.w);
-    angpos[i] = make_double4(apn[0], apn[1], apn[2], ap.w);
+    angpos[i] = make_double3(apn[0], apn[1], apn[2]);
     const double4 x_new = make_double4(xn[0], xn[1], xn[2], xo.w);
     x[i] = x_new;
     if (nh.keys) next_hash(a, nh, i, x_new, wrapped);

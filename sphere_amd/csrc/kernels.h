@@ -15,13 +15,16 @@ struct StepClock {
 
 struct DemBuffers {
     // particle state, original order (src/datatypes.h:16-32)
-    double4 *x, *vel, *angvel, *force, *torque, *angpos, *acc, *angacc, *xyzsum;
+    // rows without a fourth component are double3 (24 B)
+    double4 *x, *vel;
+    double3 *angvel, *force, *torque, *angpos, *acc, *angacc, *xyzsum;
     double *es_dot, *es, *ev_dot, *ev, *p;
     // sort (src/datatypes.h:52-60)
     uint32_t *keys, *vals, *keys_tmp, *vals_tmp, *hist;
     StepClock *clock;
     uint32_t *cell_start, *cell_end;
-    double4 *x_s, *vel_s, *angvel_s;
+    double4 *x_s, *vel_s;
+    double3 *angvel_s;
     // contact history, original index * nc + slot
     uint32_t *contacts;   // np*nc: partner index | image code << 28 (ct_word), empty = np
     uint64_t *cmask;      // np, bit i set iff contacts[.. + i] != np
@@ -89,8 +92,10 @@ struct RebuildArgs {
     uint32_t *hist, *part;
     uint32_t n, ntiles, passes, ncells;
     uint32_t *cell_start, *cell_end;
-    const double4 *x, *vel, *angvel;
-    double4 *x_s, *vel_s, *angvel_s;
+    const double4 *x, *vel;
+    const double3 *angvel;
+    double4 *x_s, *vel_s;
+    double3 *angvel_s;
     uint32_t *bar, *err;
     uint32_t *nl, *nl_cnt, *nl_pos;
     double4 *x_ref;
