@@ -351,9 +351,9 @@ static int upload(sph_ctx *ctx)
         ptr = dalloc<T>(ctx, n);                                          \
         if (!ptr) return fail(ctx, SPH_ERR_HIP, "hipMalloc failed for " #ptr); \
     } while (0)
-    AL(b.x, double4, np); AL(b.vel, double4, np); AL(b.angvel, double4, np);
-    AL(b.force, double4, np); AL(b.torque, double4, np); AL(b.angpos, double4, np);
-    AL(b.acc, double4, np); AL(b.angacc, double4, np); AL(b.xyzsum, double4, np);
+    AL(b.x, double4, np); AL(b.vel, double4, np); AL(b.angvel, double3, np);
+    AL(b.force, double3, np); AL(b.torque, double3, np); AL(b.angpos, double3, np);
+    AL(b.acc, double3, np); AL(b.angacc, double3, np); AL(b.xyzsum, double3, np);
     AL(b.es_dot, double, np); AL(b.es, double, np); AL(b.ev_dot, double, np);
     AL(b.ev, double, np); AL(b.p, double, np);
     AL(b.keys, uint32_t, np); AL(b.vals, uint32_t, np);
@@ -361,7 +361,7 @@ static int upload(sph_ctx *ctx)
     AL(b.hist, uint32_t, 256 * ntiles);
     AL(b.clock, StepClock, 1);
     AL(b.cell_start, uint32_t, ctx->ncells); AL(b.cell_end, uint32_t, ctx->ncells);
-    AL(b.x_s, double4, np); AL(b.vel_s, double4, np); AL(b.angvel_s, double4, np);
+    AL(b.x_s, double4, np); AL(b.vel_s, double4, np); AL(b.angvel_s, double3, np);
     const size_t NCS = ctx->nc;
     AL(b.contacts, uint32_t, np * NCS); AL(b.cmask, uint64_t, np);
     AL(b.delta_t, double, np * NCS * 3);
@@ -382,19 +382,17 @@ static int upload(sph_ctx *ctx)
     HIPCHK(hipMemcpyAsync(b.vel, t4.data(), np * 32, hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
     const std::vector<double> *v3[] = {&hs.angvel, &hs.force, &hs.torque, &hs.angpos, &hs.xyzsum};
-    double4 *d3p[] = {b.angvel, b.force, b.torque, b.angpos, b.xyzsum};
-    for (int k = 0; k < 5; ++k) {
-        pack4(*v3[k], np, 3, t4);
-        HIPCHK(hipMemcpyAsync(d3p[k], t4.data(), np * 32, hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
-    }
+    double3 *d3p[] = {b.angvel, b.force, b.torque, b.angpos, b.xyzsum};
+    for (int k = 0; k < 5; ++k)  // three-component rows: the host layout as is
+        HIPCHK(hipMemcpyAsync(d3p[k], v3[k]->data(), np * 24, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipMemcpyAsync(b.es_dot, hs.es_dot.data(), np * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(b.es, hs.es.data(), np * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(b.ev_dot, hs.ev_dot.data(), np * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(b.ev, hs.ev.data(), np * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(b.p, hs.p.data(), np * 8, hipMemcpyHostToDevice, s));
-    launch_fill_f64(s, (double *)b.acc, np * 4, 0.0);
-    launch_fill_f64(s, (double *)b.angacc, np * 4, 0.0);
+    launch_fill_f64(s, (double *)b.acc, np * 3, 0.0);
+    launch_fill_f64(s, (double *)b.angacc, np * 3, 0.0);
     launch_fill_u32(s, b.contacts, np * NCS, (uint32_t)np);  // device.cu:471-473
     launch_fill_u32(s, (uint32_t *)b.cmask, 2 * np, 0u);
     launch_fill_f64(s, b.delta_t, np * NCS * 3, 0.0);
@@ -621,11 +619,9 @@ static int download(sph_ctx *ctx, bool particles, bool fluid_fields)
         HIPCHK(hipMemcpy(t4.data(), b.vel, np * 32, hipMemcpyDeviceToHost));
         unpack4(t4, np, 4, hs.vel.data());
         std::vector<double> *v3[] = {&hs.angvel, &hs.force, &hs.torque, &hs.angpos, &hs.xyzsum};
-        double4 *d3p[] = {b.angvel, b.force, b.torque, b.angpos, b.xyzsum};
-        for (int k = 0; k < 5; ++k) {
-            HIPCHK(hipMemcpy(t4.data(), d3p[k], np * 32, hipMemcpyDeviceToHost));
-            unpack4(t4, np, 3, v3[k]->data());
-        }
+        double3 *d3p[] = {b.angvel, b.force, b.torque, b.angpos, b.xyzsum};
+        for (int k = 0; k < 5; ++k)
+            HIPCHK(hipMemcpy(v3[k]->data(), d3p[k], np * 24, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(hs.es_dot.data(), b.es_dot, np * 8, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(hs.es.data(), b.es, np * 8, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(hs.ev_dot.data(), b.ev_dot, np * 8, hipMemcpyDeviceToHost));
