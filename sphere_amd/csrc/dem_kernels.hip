@@ -1644,8 +1644,11 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
 // (wall 0, or the plane z = L2 without walls), the fixed bottom z = origo2,
 // and for non-periodic axes walls 1-4.  Returns the normal forces on walls
 // 0-4 in wf0..wf4.
-__device__ __forceinline__ void walls_contact(const StepArgs &a, Acc &A, double4 xa, double4 va4,
-                                              double3 wa4, const double4 *__restrict__ w_nx,
+// The grain's velocities are read only by a grain that touches a wall.
+__device__ __forceinline__ void walls_contact(const StepArgs &a, Acc &A, double4 xa,
+                                              const double4 *__restrict__ vel,
+                                              const double3 *__restrict__ angvel, uint32_t i,
+                                              const double4 *__restrict__ w_nx,
                                               const double4 *__restrict__ w_mvfd, double &wf0,
                                               double &wf1, double &wf2, double &wf3,
                                               double &wf4)
@@ -1657,26 +1660,26 @@ __device__ __forceinline__ void walls_contact(const StepArgs &a, Acc &A, double4
     const double4 wn0 = nw > 0 ? w_nx[0] : make_double4(0.0, 0.0, -1.0, g.L[2]);
     const double wv0 = nw > 0 ? w_mvfd[0].y : 0.0;
     double delta_w = wn0.w - (xa.z + radius_a);
-    if (delta_w < 0.0) wf0 = law_wall(a, A, va4, wa4, radius_a, xyz(wn0), delta_w, wv0);
+    if (delta_w < 0.0) wf0 = law_wall(a, A, vel[i], angvel[i], radius_a, xyz(wn0), delta_w, wv0);
     delta_w = xa.z - radius_a - g.origo[2];
-    if (delta_w < 0.0) (void)law_wall(a, A, va4, wa4, radius_a, mk3(0.0, 0.0, 1.0), delta_w, 0.0);
+    if (delta_w < 0.0) (void)law_wall(a, A, vel[i], angvel[i], radius_a, mk3(0.0, 0.0, 1.0), delta_w, 0.0);
     if (g.periodic == 0) {
         const double4 wn1 = nw > 1 ? w_nx[1] : make_double4(-1.0, 0.0, 0.0, g.L[0]);
         const double4 wn2 = nw > 2 ? w_nx[2] : make_double4(1.0, 0.0, 0.0, 0.0);
         const double wv1 = nw > 1 ? w_mvfd[1].y : 0.0, wv2 = nw > 2 ? w_mvfd[2].y : 0.0;
         delta_w = wn1.w - (xa.x + radius_a);
-        if (delta_w < 0.0) wf1 = law_wall(a, A, va4, wa4, radius_a, xyz(wn1), delta_w, wv1);
+        if (delta_w < 0.0) wf1 = law_wall(a, A, vel[i], angvel[i], radius_a, xyz(wn1), delta_w, wv1);
         delta_w = xa.x - radius_a - wn2.w;
-        if (delta_w < 0.0) wf2 = law_wall(a, A, va4, wa4, radius_a, xyz(wn2), delta_w, wv2);
+        if (delta_w < 0.0) wf2 = law_wall(a, A, vel[i], angvel[i], radius_a, xyz(wn2), delta_w, wv2);
     }
     if (g.periodic == 0 || g.periodic == 2) {
         const double4 wn3 = nw > 3 ? w_nx[3] : make_double4(0.0, -1.0, 0.0, g.L[1]);
         const double4 wn4 = nw > 4 ? w_nx[4] : make_double4(0.0, 1.0, 0.0, 0.0);
         const double wv3 = nw > 3 ? w_mvfd[3].y : 0.0, wv4 = nw > 4 ? w_mvfd[4].y : 0.0;
         delta_w = wn3.w - (xa.y + radius_a);
-        if (delta_w < 0.0) wf3 = law_wall(a, A, va4, wa4, radius_a, xyz(wn3), delta_w, wv3);
+        if (delta_w < 0.0) wf3 = law_wall(a, A, vel[i], angvel[i], radius_a, xyz(wn3), delta_w, wv3);
         delta_w = xa.y - radius_a - wn4.w;
-        if (delta_w < 0.0) wf4 = law_wall(a, A, va4, wa4, radius_a, xyz(wn4), delta_w, wv4);
+        if (delta_w < 0.0) wf4 = law_wall(a, A, vel[i], angvel[i], radius_a, xyz(wn4), delta_w, wv4);
     }
 }
 
@@ -1875,10 +1878,9 @@ __global__ void __launch_bounds__(FW) k_forces(
     }
     if (!valid) return;
     Acc A = load_acc(s_acc, lane);
-    const double4 xa = x[a_orig], va4 = vel[a_orig];
-    const double3 wa4 = angvel[a_orig];
+    const double4 xa = x[a_orig];
     double wf0, wf1, wf2, wf3, wf4;
-    walls_contact(a, A, xa, va4, wa4, w_nx, w_mvfd, wf0, wf1, wf2, wf3, wf4);
+    walls_contact(a, A, xa, vel, angvel, a_orig, w_nx, w_mvfd, wf0, wf1, wf2, wf3, wf4);
     write_sums(a, A, a_orig, force, torque, es_dot_out, ev_dot_out, es, ev, p_out,
                walls_force_pp, wf0, wf1, wf2, wf3, wf4);
     if (keep != mask) cmask[a_orig] = keep;
@@ -1945,7 +1947,7 @@ __global__ void __launch_bounds__(256) k_contact_viscous(
                 }
     }
     double wf0, wf1, wf2, wf3, wf4;
-    walls_contact(a, A, xa, va4, wa4, w_nx, w_mvfd, wf0, wf1, wf2, wf3, wf4);
+    walls_contact(a, A, xa, vel_s, angvel_s, idx_a, w_nx, w_mvfd, wf0, wf1, wf2, wf3, wf4);
     write_sums(a, A, a_orig, force, torque, es_dot_out, ev_dot_out, es, ev, p_out,
                walls_force_pp, wf0, wf1, wf2, wf3, wf4);
 }
