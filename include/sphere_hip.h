@@ -195,11 +195,13 @@ int sph_set_neighbor_lists(sph_ctx *ctx, double skin_fraction, int cap);
 int sph_get_neighbor_stats(sph_ctx *ctx, sph_neighbor_stats *out);
 
 /* How list mode rebuilds its lists.  No counterpart in the reference.
- * fused = 1 (default): the cell sort, reorder and list build run as one
- * kernel with grid barriers, launched every step, that returns at once
- * unless the rebuild flag is set; fused = 0: the sort passes, reorder and
- * list build are separate launches that each return at once between
- * rebuilds.  Both give the same arrays.  Takes effect at the next load. */
+ * fused = 1 (default): the cell sort and reorder run as one kernel with grid
+ * barriers, followed by the list build; both are launched every step and
+ * return at once unless the rebuild flag is set.  fused = 0: the sort
+ * passes, reorder and list build are separate launches that each return at
+ * once between rebuilds.  Runs with fluid rebuild every step and always use
+ * the separate launches.  Both give the same arrays.  Takes effect at the
+ * next load. */
 int sph_set_rebuild_kernel(sph_ctx *ctx, int fused);
 
 /* Load a state: replaces DEM::readbin + checkValues + transfer to the device

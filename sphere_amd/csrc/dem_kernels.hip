@@ -591,9 +591,11 @@ void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const De
 // ------------------------------------------------------------------
 // Between rebuilds the sort, reorder and list kernels only read the rebuild
 // flag and return, and each such launch still costs its dispatch.  On the
-// list path they run as one kernel whose phases are separated by grid
-// barriers; a step without a rebuild then costs one dispatch.  The grid is
-// at most one block per compute unit, so every block is resident.
+// list path the radix passes, the cell clear and the reorder run as one
+// kernel whose phases are separated by grid barriers, and the list build
+// follows as one more launch; a step without a rebuild then costs two
+// dispatches instead of nine.  The grid is at most one block per compute
+// unit, so every block is resident.
 
 // Grid barrier.  bar[0] counts arrivals, bar[1] is the generation; `gen` is
 // the generation this block waits to see end.  A waiter that spins longer
@@ -677,7 +679,7 @@ __device__ void grid_scan(uint32_t *__restrict__ a, uint32_t m, uint32_t *__rest
     }
 }
 
-__global__ void __launch_bounds__(256) k_rebuild(StepArgs a, NLArgs n, RebuildArgs r)
+__global__ void __launch_bounds__(256) k_rebuild(StepArgs a, RebuildArgs r)
 {
     if ((*r.cond & NLF_REBUILD) == 0u) return;
     __shared__ uint32_t run[256];
@@ -719,14 +721,6 @@ __global__ void __launch_bounds__(256) k_rebuild(StepArgs a, NLArgs n, RebuildAr
         r.vel_s[idx] = r.vel[sidx];
         r.angvel_s[idx] = r.angvel[sidx];
     }
-    grid_barrier(r.bar, G, gen, r.err);
-    if (blockIdx.x == 0 && threadIdx.x == 0) r.flags[NLF_COUNT] += 1u;  // rebuild counter
-    if (blockIdx.x == 0 && threadIdx.x < NLF_DSLOTS)  // displacements restart
-        ((unsigned long long *)(r.flags + NLF_DMAX))[threadIdx.x] = 0ull;
-    for (uint32_t pos = gtid; pos < np; pos += gsz)
-        nl_build_one(a, n, pos, vals, r.cell_start, r.cell_end, r.x_s, r.nl, r.nl_cnt, r.nl_pos,
-                     r.x_ref, r.flags, r.contacts, r.cmask, r.nl_slot, r.slot_cand, r.nl_gap,
-                     r.nl_gmin);
 }
 
 int rebuild_blocks_per_cu()
@@ -765,18 +759,7 @@ void launch_rebuild(hipStream_t s, const StepArgs &a, const NLArgs &n, DemBuffer
     r.angvel_s = b.angvel_s;
     r.bar = b.rb_bar;
     r.err = b.errflags;
-    r.nl = b.nl;
-    r.nl_cnt = b.nl_cnt;
-    r.nl_pos = b.nl_pos;
-    r.x_ref = b.x_ref;
-    r.flags = b.nl_flags;
-    r.contacts = b.contacts;
-    r.cmask = b.cmask;
-    r.nl_slot = b.nl_slot;
-    r.slot_cand = b.slot_cand;
-    r.nl_gap = b.nl_gap;
-    r.nl_gmin = b.nl_gmin;
-    hipLaunchKernelGGL(k_rebuild, dim3(blocks), dim3(256), 0, s, a, n, r);
+    hipLaunchKernelGGL(k_rebuild, dim3(blocks), dim3(256), 0, s, a, r);
     // the sorted keys and values end in the buffer the last pass wrote
     if (passes & 1) {
         uint32_t *tk = b.keys;
@@ -786,6 +769,12 @@ void launch_rebuild(hipStream_t s, const StepArgs &a, const NLArgs &n, DemBuffer
         b.vals = b.vals_tmp;
         b.vals_tmp = tv;
     }
+    // the list build, the heaviest phase, on one thread per grain (it
+    // needs no barrier after it, so its grid is not bounded by residency)
+    hipLaunchKernelGGL(k_nl_build, dim3((a.np + 255) / 256), dim3(256), 0, s, a, n, cond, b.vals,
+                       b.cell_start, b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_pos, b.x_ref,
+                       b.nl_flags, b.contacts, b.cmask, b.nl_slot, b.slot_cand, b.nl_gap,
+                       b.nl_gmin);
 }
 
 // Sum and maximum of the list lengths (overflowed lists count as cap + 1).
@@ -1485,8 +1474,9 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
             q.lo = lcnt >= 64 ? ~0ull : ((1ull << lcnt) - 1ull);
             q.hi = lcnt > 64 ? (lcnt >= 128 ? ~0ull : ((1ull << (lcnt - 64)) - 1ull)) : 0ull;
         } else {
-            const float thr = (float)(2.0 * sqrt(dmax) * (1.0 + 1e-9) +
-                                      1e-11 * (g.L[0] + g.L[1] + g.L[2]));
+            // rounded up to float, as the stored gaps are rounded down
+            const float thr = __double2float_ru(2.0 * sqrt(dmax) * (1.0 + 1e-9) +
+                                                1e-11 * (g.L[0] + g.L[1] + g.L[2]));
             // nl_gmin bounds the gaps of the entries not in the row
             if (!(nl_gmin[pos] > thr)) {
                 // eight entries per iteration, their loads issued together

@@ -549,7 +549,9 @@ static int upload(sph_ctx *ctx)
             // kernel's grid barriers need every block running)
             ctx->rb_blocks = 0;
             int cus = 0;
-            if (ctx->rb_fused && rebuild_blocks_per_cu() > 0 &&
+            // (a fluid run rebuilds every step: there the separate launches,
+            // on full grids, are faster)
+            if (ctx->rb_fused && !h.fluid && rebuild_blocks_per_cu() > 0 &&
                 hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                       ctx->device) == hipSuccess && cus > 0) {
                 const uint32_t want = (uint32_t)((np + 255) / 256);

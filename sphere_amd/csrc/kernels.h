@@ -84,8 +84,9 @@ void launch_hash_nl(hipStream_t s, const StepArgs &a, const NLArgs &n, const Dem
 void launch_sort(hipStream_t s, uint32_t n, int bits, DemBuffers &b, const uint32_t *cond);
 void launch_reorder(hipStream_t s, const StepArgs &a, const DemBuffers &b, uint32_t ncells,
                     const uint32_t *cond);
-// The list rebuild (sort, reorder, list build) as one launch of `blocks`
-// resident blocks; returns at once unless the rebuild flag at cond is set.
+// The list rebuild: the cell sort and reorder as one launch of `blocks`
+// resident blocks, then the list build; both return at once unless the
+// rebuild flag at cond is set.
 struct RebuildArgs {
     const uint32_t *cond;
     uint32_t *keys[2], *vals[2];
@@ -97,13 +98,6 @@ struct RebuildArgs {
     double4 *x_s, *vel_s;
     double3 *angvel_s;
     uint32_t *bar, *err;
-    uint32_t *nl, *nl_cnt, *nl_pos;
-    double4 *x_ref;
-    uint32_t *flags;
-    const uint32_t *contacts;
-    const uint64_t *cmask;
-    uint8_t *nl_slot, *slot_cand;
-    float *nl_gap, *nl_gmin;
 };
 int rebuild_blocks_per_cu();
 void launch_rebuild(hipStream_t s, const StepArgs &a, const NLArgs &n, DemBuffers &b,
