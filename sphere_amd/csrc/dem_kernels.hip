@@ -125,6 +125,7 @@ __global__ void __launch_bounds__(256) k_hash_nl(StepArgs a, NLArgs n,
                                                  uint32_t *__restrict__ vals,
                                                  uint32_t *__restrict__ cellc,
                                                  const double4 *__restrict__ x_ref,
+                                                 float *__restrict__ disp,
                                                  uint32_t *__restrict__ flags, uint32_t cur,
                                                  uint32_t force, uint32_t *__restrict__ err)
 {
@@ -149,6 +150,7 @@ __global__ void __launch_bounds__(256) k_hash_nl(StepArgs a, NLArgs n,
     vals[i] = i;
     cellc[i] = packed;
     const double d2 = disp2_since_rebuild(g, xi, x_ref[i]);
+    disp[i] = __double2float_ru(sqrt(d2));
     bool need = !(d2 <= n.disp2);
     if (i == 0 && (force || flags[NLF_STICKY] != 0u)) need = true;
     record_displacement(flags, cur, need, false, d2);
@@ -159,7 +161,7 @@ void launch_hash_nl(hipStream_t s, const StepArgs &a, const NLArgs &n, const Dem
 {
     uint32_t nb = (a.np + 255) / 256;
     hipLaunchKernelGGL(k_hash_nl, dim3(nb), dim3(256), 0, s, a, n, b.x, b.keys, b.vals, b.cellc,
-                       b.x_ref, b.nl_flags, cur, force, b.errflags);
+                       b.x_ref, b.nl_disp, b.nl_flags, cur, force, b.errflags);
 }
 
 // ------------------------------------------------------------------
@@ -1436,7 +1438,7 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     uint8_t *__restrict__ slot_cand, const uint32_t *__restrict__ nl_cnt,
     const uint32_t *__restrict__ nl_pos, const uint32_t *__restrict__ cellc,
     const float *__restrict__ nl_gap, const float *__restrict__ nl_gmin,
-    uint32_t *__restrict__ flags_rw, uint32_t cur)
+    const float *__restrict__ nl_disp, uint32_t *__restrict__ flags_rw, uint32_t cur)
 {
     const uint32_t *flags = flags_rw;
     __shared__ double4 s_x[TW];
@@ -1474,9 +1476,14 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
             q.lo = lcnt >= 64 ? ~0ull : ((1ull << lcnt) - 1ull);
             q.hi = lcnt > 64 ? (lcnt >= 128 ? ~0ull : ((1ull << (lcnt - 64)) - 1ull)) : 0ull;
         } else {
-            // rounded up to float, as the stored gaps are rounded down
-            const float thr = __double2float_ru(2.0 * sqrt(dmax) * (1.0 + 1e-9) +
-                                                1e-11 * (g.L[0] + g.L[1] + g.L[2]));
+            // An entry's gap can have closed by at most the two grains'
+            // displacements since the rebuild: this grain's (nl_disp, rounded
+            // up) and at most sqrt(dmax) for the partner.  Bounds are rounded
+            // up to float, as the stored gaps are rounded down, with a margin
+            // for the rounding of the displacements themselves.
+            const double marg = 1e-11 * (g.L[0] + g.L[1] + g.L[2]);
+            const double da = (double)nl_disp[pos];
+            const float thr = __double2float_ru((da + sqrt(dmax)) * (1.0 + 1e-9) + marg);
             // nl_gmin bounds the gaps of the entries not in the row
             if (!(nl_gmin[pos] > thr)) {
                 // eight entries per iteration, their loads issued together
@@ -1493,9 +1500,22 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
                             gp[u] = nl_gap[li];
                         }
                     }
+                    // the partner's own displacement bounds the pair more tightly
+                    uint32_t bq[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        if (sl[u] == 0xffu && !(gp[u] > thr)) {
+                        bq[u] = np;
+                        if (sl[u] == 0xffu && !(gp[u] > thr))
+                            bq[u] = nl[(size_t)(k0 + u) * np + pos];
+                    }
+                    float db[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) db[u] = bq[u] < np ? nl_disp[bq[u]] : 0.0f;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (bq[u] < np &&
+                            !(gp[u] > __double2float_ru((da + (double)db[u]) * (1.0 + 1e-9) +
+                                                        marg))) {
                             set_bit(q, k0 + u);
                             ++n;
                         }
@@ -1950,12 +1970,12 @@ void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool
         hipLaunchKernelGGL(k_topology_lists<true>, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s, a,
                            b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
                            b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
-                           b.nl_pos, b.cellc, b.nl_gap, b.nl_gmin, b.nl_flags, cur);
+                           b.nl_pos, b.cellc, b.nl_gap, b.nl_gmin, b.nl_disp, b.nl_flags, cur);
     else if (nlist)
         hipLaunchKernelGGL(k_topology_lists<false>, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s,
                            a, b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
                            b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
-                           b.nl_pos, b.cellc, b.nl_gap, b.nl_gmin, b.nl_flags, cur);
+                           b.nl_pos, b.cellc, b.nl_gap, b.nl_gmin, b.nl_disp, b.nl_flags, cur);
     else
         hipLaunchKernelGGL(k_topology_cells, dim3((a.np + 255) / 256), dim3(256), 0, s, a,
                            b.vals, b.cell_start, b.cell_end, b.x_s, b.contacts, b.cmask,
@@ -2238,6 +2258,7 @@ __device__ __forceinline__ void next_hash(const StepArgs &a, const NextHash &nh,
     nh.vals[i] = i;
     nh.cellc[i] = packed;
     const double d2 = disp2_since_rebuild(g, xi, nh.x_ref[i]);
+    nh.disp[i] = __double2float_ru(sqrt(d2));
     bool need = !(d2 <= nh.disp2);
     if (i == 0 && nh.flags[NLF_STICKY] != 0u) need = true;
     record_displacement(nh.flags, nh.cur, need, wrapped, d2);

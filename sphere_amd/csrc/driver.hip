@@ -538,12 +538,13 @@ static int upload(sph_ctx *ctx)
             AL(b.nl, uint32_t, (size_t)ctx->nl_cap * np); AL(b.nl_cnt, uint32_t, np);
             AL(b.nl_pos, uint32_t, np); AL(b.nl_flags, uint32_t, NLF_WORDS);
             AL(b.nl_gap, float, (size_t)ctx->nl_cap * np);
-            AL(b.nl_gmin, float, np);
+            AL(b.nl_gmin, float, np); AL(b.nl_disp, float, np);
             AL(b.nl_slot, uint8_t, (size_t)ctx->nl_cap * np);
             AL(b.slot_cand, uint8_t, np * NCS);
             launch_fill_u32(s, b.nl_flags, NLF_WORDS, 0u);
             launch_fill_u32(s, (uint32_t *)b.slot_cand, (np * NCS + 3) / 4, 0xffffffffu);
             launch_fill_f64(s, (double *)b.x_ref, np * 4, 0.0);
+            launch_fill_u32(s, (uint32_t *)b.nl_disp, np, 0u);
             ctx->nl_on = true;
             // one resident block per compute unit at most (the rebuild
             // kernel's grid barriers need every block running)
@@ -1020,6 +1021,7 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             nh.vals = ctx->db.vals;
             nh.cellc = ctx->db.cellc;
             nh.x_ref = ctx->db.x_ref;
+            nh.disp = ctx->db.nl_disp;
             nh.flags = ctx->db.nl_flags;
             nh.cur = ctx->nl_parity;  // already the next step's word
             nh.disp2 = ctx->nla.disp2;

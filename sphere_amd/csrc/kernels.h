@@ -58,6 +58,7 @@ struct DemBuffers {
     uint32_t *nl_flags;  // NLF_WORDS words, see common.h
     float *nl_gap;       // cap*np like nl: surface gap at the rebuild, rounded down
     float *nl_gmin;      // np: lower bound of nl_gap over the entries not in the row
+    float *nl_disp;      // np: displacement since the rebuild, rounded up
 };
 
 // Bond contributions gathered inside integrate (off == nullptr: none).
@@ -71,6 +72,7 @@ struct BondGather {
 struct NextHash {
     uint32_t *keys, *vals, *cellc;
     const double4 *x_ref;
+    float *disp;
     uint32_t *flags;
     uint32_t cur;   // rebuild-flag word of the next step
     double disp2;
