@@ -63,6 +63,22 @@ __device__ __forceinline__ void vox_xyz(const NetGeom &g, uint32_t v, int &x, in
 // Epilogue of the convolutions: acc[j][t] holds, for M tile j (16
 // voxels from vbase[j] along x), voxels vbase[j] + 4 grp + i (i = 0..3) and
 // output channel 16 t + m.
+// The children of coarse voxel v on the grid twice as fine, channel co:
+// yfine = a + skip, for children k0, k0 + dk, ... (k < 8)
+__device__ __forceinline__ void up_children(const NetGeom &g, uint32_t v, int co, int cout_real, float av,
+                                            int k0, int dk, const uint16_t *__restrict__ skip,
+                                            uint16_t *__restrict__ yfine)
+{
+    int x, y, z;
+    vox_xyz(g, v, x, y, z);
+    const uint32_t fnx = 2u * g.nx, fny = 2u * g.ny;
+    for (int k = k0; k < 8; k += dk) {
+        const uint32_t fv = ((2u * z + (k >> 2)) * fny + 2u * y + ((k >> 1) & 1)) * fnx + 2u * x + (k & 1);
+        const size_t fo = (size_t)fv * cout_real + co;
+        yfine[fo] = f2bf(av + bf2f(skip[fo]));
+    }
+}
+
 template <int NT, int MT>
 __device__ __forceinline__ void conv_store(const f32x4 (&acc)[MT][NT], const uint32_t (&vbase)[MT], uint32_t V,
                                            uint32_t m, uint32_t grp, const float *__restrict__ bias,
@@ -112,20 +128,8 @@ __device__ __forceinline__ void conv_store(const f32x4 (&acc)[MT][NT], const uin
                 if (mode == NET_OUT_BF16_RELU || mode == NET_OUT_RELU_UP) {
                     const uint16_t a = f2bf(r > 0.f ? r : 0.f);
                     ((uint16_t *)out)[o] = a;
-                    if (mode == NET_OUT_RELU_UP) {
-                        // the 8 children on the grid twice as fine: a + skip
-                        int x, y, z;
-                        vox_xyz(g, v, x, y, z);
-                        const uint32_t fnx = 2u * g.nx, fny = 2u * g.ny;
-                        const float av = bf2f(a);
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            const uint32_t fv = ((2u * z + (k >> 2)) * fny + 2u * y + ((k >> 1) & 1)) * fnx +
-                                                2u * x + (k & 1);
-                            const size_t fo = (size_t)fv * cout_real + co;
-                            yfine[fo] = f2bf(av + bf2f(skip[fo]));
-                        }
-                    }
+                    if (mode == NET_OUT_RELU_UP && yfine)  // the 8 children: a + skip
+                        up_children(g, v, co, cout_real, bf2f(a), 0, 1, skip, yfine);
                 } else if (mode == NET_OUT_F32) {
                     ((float *)out)[o] = r;
                 } else {
@@ -220,15 +224,58 @@ __global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd(NetGeom g, 
                         red[((((wv - 1) * MT + j) * NT + t) * 4 + i) * 64 + lane] = acc[j][t][i];
         }
         __syncthreads();
-        if (wv > 0) return;
+        if (mode != NET_OUT_RELU_UP && wv > 0) return;
+        if (wv == 0) {
 #pragma unroll 1
-        for (int k = 0; k < KS - 1; ++k)
+            for (int k = 0; k < KS - 1; ++k)
+#pragma unroll
+                for (int j = 0; j < MT; ++j)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            acc[j][t][i] += red[(((k * MT + j) * NT + t) * 4 + i) * 64 + lane];
+        }
+        if (mode == NET_OUT_RELU_UP) {
+            // wave 0 stores the coarse output and hands a = bf16(relu(acc +
+            // bias)) to the other waves, which share the 8 children per voxel
+            __shared__ float s_a[MT * NT * 4][64];
+            uint32_t vb[MT];
+#pragma unroll
+            for (int j = 0; j < MT; ++j) vb[j] = v0 + 16u * j;
+            if (wv == 0) {
+                conv_store<NT, MT>(acc, vb, V, m, grp, bias, cout_real, out, NET_OUT_RELU_UP, proj, out2, g,
+                                   skip, nullptr);
+#pragma unroll
+                for (int j = 0; j < MT; ++j)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const int co = t * 16 + (int)m;
+                        const float bv = (bias && co < cout_real) ? bias[co] : 0.f;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const float r = acc[j][t][i] + bv;
+                            s_a[(j * NT + t) * 4 + i][lane] = bf2f(f2bf(r > 0.f ? r : 0.f));
+                        }
+                    }
+            }
+            __syncthreads();
 #pragma unroll
             for (int j = 0; j < MT; ++j)
 #pragma unroll
-                for (int t = 0; t < NT; ++t)
+                for (int t = 0; t < NT; ++t) {
+                    const int co = t * 16 + (int)m;
+                    if (co >= cout_real) continue;
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) acc[j][t][i] += red[(((k * MT + j) * NT + t) * 4 + i) * 64 + lane];
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t v = vb[j] + 4u * grp + (uint32_t)i;
+                        if (v < V)
+                            up_children(g, v, co, cout_real, s_a[(j * NT + t) * 4 + i][lane], (int)wv, KS, skip,
+                                        yfine);
+                    }
+                }
+            return;
+        }
     }
     uint32_t vbase[MT];
 #pragma unroll
