@@ -372,12 +372,10 @@ static void conv_fwd_nt(hipStream_t s, const NetGeom &g, const uint16_t *in, int
         const uint32_t tiles = (V + 63u) / 64u;
         hipLaunchKernelGGL((k_conv_fwd<NT, 4, 1, CIN4>), dim3((tiles + 3u) / 4u), dim3(256), 0, s, g, in, cin,
                            w, bias, cout_real, out, mode, proj, out2, skip, yfine);
-    } else if (V >= 8192u) {
-        const uint32_t tiles = (V + 15u) / 16u;
-        hipLaunchKernelGGL((k_conv_fwd<NT, 1, 1, CIN4>), dim3((tiles + 3u) / 4u), dim3(256), 0, s, g, in, cin,
-                           w, bias, cout_real, out, mode, proj, out2, skip, yfine);
     } else {
-        // small grids: the 4 waves of a block split K for one 16-voxel tile
+        // below 65536 voxels the 4 waves of a block split K for one 16-voxel
+        // tile (and share an upsampling epilogue): four times the waves of
+        // one tile per wave, which these latency-bound grids need
         const uint32_t tiles = (V + 15u) / 16u;
         hipLaunchKernelGGL((k_conv_fwd<NT, 1, 4, CIN4>), dim3(tiles), dim3(256), 0, s, g, in, cin, w, bias,
                            cout_real, out, mode, proj, out2, skip, yfine);
