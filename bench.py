@@ -37,7 +37,8 @@ def log(msg):
 def measured_traffic():
     """HBM bytes per step of the contact stage from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE passes, calibrated against a
-    copy kernel of known size), or None if it is missing."""
+    copy kernel of known size, cross-checked by the 128-B read request
+    count), or None if it is missing."""
     path = os.path.join(ROOT, "profiles", "shear_box_1m_hbm_traffic.md")
     if not os.path.exists(path):
         return None
@@ -46,7 +47,8 @@ def measured_traffic():
     for line in open(path):
         cells = [c.strip() for c in line.split("|")]
         if len(cells) > 4 and ("k_forces" in cells[1] or "k_topology" in cells[1]):
-            total += float(cells[4].split()[0]) * 1e6
+            mb = [c for c in cells if c.endswith(" MB")]  # the scaled bytes column
+            total += float(mb[-1].split()[0]) * 1e6
             found += 1
     return total if found == 2 else None
 
@@ -415,9 +417,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    if not args.no_darcy and not args.no_net:
+    if args.no_shear and not args.no_darcy and not args.no_net:
         # torch carries its own HIP runtime: initialise it before the library
-        # loads, so that both use that one runtime in this process
+        # loads, so that both use that one runtime in this process.  (With
+        # the shear box, the Darcy section runs in a child process, so that
+        # the DEM step is timed on the system HIP runtime the library is
+        # built against, without torch.)
         import torch
         torch.cuda.init()
     import sphere_amd
@@ -545,11 +550,37 @@ def main():
         "kernels": ktimes,
         "cpu_baseline": cpu,
     }
-    line["darcy"] = None if args.no_darcy else darcy_section(args)
+    if run_engine is not engine:
+        run_engine.close()
+    engine.close()
+    line["darcy"] = None if args.no_darcy else darcy_child()
     line["ensemble"] = ensemble_section(args) if args.ensemble else None
     print(json.dumps(line), flush=True)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+
+
+def darcy_child():
+    """The Darcy section (plain and learned solves) in a child process with
+    torch, which trains the correction network; returns its record."""
+    import subprocess
+    argv = []
+    skip = False
+    for a in sys.argv[1:]:
+        if skip:
+            skip = False
+            continue
+        if a == "--copy-calibration":
+            skip = True
+            continue
+        if a.startswith("--copy-calibration="):
+            continue
+        argv.append(a)
+    cmd = [sys.executable, "-u", os.path.abspath(__file__)] + argv + ["--no-shear"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, universal_newlines=True)
+    if r.returncode != 0:
+        raise SystemExit("the Darcy section failed (exit status %d)" % r.returncode)
+    return json.loads(r.stdout.strip().splitlines()[-1])["darcy"]
 
 
 def copy_calibration(mib):

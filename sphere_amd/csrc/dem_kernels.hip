@@ -1500,22 +1500,9 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
                             gp[u] = nl_gap[li];
                         }
                     }
-                    // the partner's own displacement bounds the pair more tightly
-                    uint32_t bq[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        bq[u] = np;
-                        if (sl[u] == 0xffu && !(gp[u] > thr))
-                            bq[u] = nl[(size_t)(k0 + u) * np + pos];
-                    }
-                    float db[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) db[u] = bq[u] < np ? nl_disp[bq[u]] : 0.0f;
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        if (bq[u] < np &&
-                            !(gp[u] > __double2float_ru((da + (double)db[u]) * (1.0 + 1e-9) +
-                                                        marg))) {
+                        if (sl[u] == 0xffu && !(gp[u] > thr)) {
                             set_bit(q, k0 + u);
                             ++n;
                         }
