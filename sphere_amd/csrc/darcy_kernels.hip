@@ -4,6 +4,7 @@
 // kernels run one thread per interior cell with x fastest, so a wave reads
 // consecutive doubles of a row.
 #include "kernels.h"
+#include "net_kernels.h"
 
 __device__ __forceinline__ uint32_t didx(const DarcyArgs &d, int x, int y, int z)
 {
@@ -632,11 +633,13 @@ __global__ void __launch_bounds__(256) k_darcy_sweep(DarcyArgs d, const double *
                                                      unsigned long long *__restrict__ clear,
                                                      const unsigned long long *__restrict__ prev,
                                                      double tolerance, uint32_t prev_nijac,
-                                                     unsigned long long *__restrict__ state)
+                                                     unsigned long long *__restrict__ state,
+                                                     unsigned long long *__restrict__ zero_scale)
 {
     __shared__ unsigned long long wmax[4];
-    if (state[0] != 0ull) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (zero_scale && i == 0) *zero_scale = 0ull;  // the scale slot of the correction that follows
+    if (state[0] != 0ull) return;
     if (prev && darcy_decide(prev, tolerance, prev_nijac, state, i == 0)) return;
     const uint32_t n = d.nx * d.ny * d.nz;
     unsigned long long bits = 0;
@@ -697,11 +700,12 @@ __global__ void __launch_bounds__(256) k_darcy_sweep(DarcyArgs d, const double *
 void launch_darcy_sweep(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b,
                         const double *p_in, double *p_out, unsigned long long *check,
                         unsigned long long *clear, const unsigned long long *prev,
-                        double tolerance, uint32_t prev_nijac, bool free_running)
+                        double tolerance, uint32_t prev_nijac, bool free_running, bool zero_scale)
 {
     hipLaunchKernelGGL(k_darcy_sweep, cell_grid(d), dim3(256), 0, s, d, b.jc, b.jfrozen, p_in,
                        b.k, p_out, check, clear, prev, tolerance, prev_nijac,
-                       b.maxres + (free_running ? DARCY_ZERO : DARCY_STATE));
+                       b.maxres + (free_running ? DARCY_ZERO : DARCY_STATE),
+                       zero_scale ? b.maxres + DARCY_SCALE : (unsigned long long *)nullptr);
 }
 
 // The test of a check sweep that no sweep follows (the last of maxiter).
@@ -771,7 +775,8 @@ __global__ void __launch_bounds__(256) k_darcy_net_input(DarcyArgs d, const doub
                                                          const double *__restrict__ p,
                                                          const unsigned long long *__restrict__ state,
                                                          const unsigned long long *__restrict__ smax,
-                                                         float *__restrict__ input)
+                                                         float *__restrict__ input,
+                                                         uint16_t *__restrict__ in4)
 {
     if (state && state[0] != 0ull) return;
     const uint32_t n = d.nx * d.ny * d.nz;
@@ -789,9 +794,20 @@ __global__ void __launch_bounds__(256) k_darcy_net_input(DarcyArgs d, const doub
                         (jc[j.hz + i] + jc[j.hz + i + (size_t)d.nx * d.ny]) / (d.dz * d.dz);
     const double D = 0.5 * jc[j.A + i] * hsum;
     const bool fz = frozen[i] != 0;
-    input[3 * (size_t)i] = ok ? (float)(r / s) : 0.f;
-    input[3 * (size_t)i + 1] = (fz || !(D >= 0.0)) ? 0.f : (float)log1p(D);
-    input[3 * (size_t)i + 2] = fz ? 1.f : 0.f;
+    const float v0 = ok ? (float)(r / s) : 0.f;
+    const float v1 = (fz || !(D >= 0.0)) ? 0.f : (float)log1p(D);
+    const float v2 = fz ? 1.f : 0.f;
+    if (input) {
+        input[3 * (size_t)i] = v0;
+        input[3 * (size_t)i + 1] = v1;
+        input[3 * (size_t)i + 2] = v2;
+    }
+    if (in4) {  // the network's own input layout: BF16, padded to 4 channels
+        in4[4 * (size_t)i] = f2bf(v0);
+        in4[4 * (size_t)i + 1] = f2bf(v1);
+        in4[4 * (size_t)i + 2] = f2bf(v2);
+        in4[4 * (size_t)i + 3] = 0;
+    }
 }
 
 // p += s e on the cells that are not held fixed; counts applied corrections
@@ -812,7 +828,32 @@ __global__ void __launch_bounds__(256) k_darcy_net_apply(DarcyArgs d, const uint
     const float ev = e[i];
     if (!(ev == ev)) return;
     const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / (d.nx * d.ny));
-    p[didx(d, x, y, z)] += s * (double)ev;
+    double &pc = p[didx(d, x, y, z)];
+    pc += s * (double)ev;
+    // the ghost cells that copy this cell (k_ghosts' rules for a scalar
+    // field); the other ghosts already hold the unchanged cells' values
+    const double v = pc;
+    const int nx = (int)d.nx, ny = (int)d.ny, nz = (int)d.nz;
+#define GW(X, Y, Z) p[didx(d, X, Y, Z)] = v
+    if (x == 0 && d.bc_xn == 0) GW(-1, y, z);
+    if (x == 1 && d.bc_xn == 1) GW(-1, y, z);
+    if (x == 0 && d.bc_xn == 2) GW(nx, y, z);
+    if (x == nx - 1 && d.bc_xp == 0) GW(nx, y, z);
+    if (x == nx - 2 && d.bc_xp == 1) GW(nx, y, z);
+    if (x == nx - 1 && d.bc_xp == 2) GW(-1, y, z);
+    if (y == 0 && d.bc_yn == 0) GW(x, -1, z);
+    if (y == 1 && d.bc_yn == 1) GW(x, -1, z);
+    if (y == 0 && d.bc_yn == 2) GW(x, ny, z);
+    if (y == ny - 1 && d.bc_yp == 0) GW(x, ny, z);
+    if (y == ny - 2 && d.bc_yp == 1) GW(x, ny, z);
+    if (y == ny - 1 && d.bc_yp == 2) GW(x, -1, z);
+    if (z == 0 && d.bc_bot == 0) GW(x, y, -1);
+    if (z == 1 && d.bc_bot == 1) GW(x, y, -1);
+    if (z == 0 && d.bc_bot == 2) GW(x, y, nz);
+    if (z == nz - 1 && d.bc_top == 0) GW(x, y, nz);
+    if (z == nz - 2 && d.bc_top == 1) GW(x, y, nz);
+    if (z == nz - 1 && d.bc_top == 2) GW(x, y, -1);
+#undef GW
 }
 
 void launch_darcy_residual_max(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, const double *p,
@@ -824,11 +865,11 @@ void launch_darcy_residual_max(hipStream_t s, const DarcyArgs &d, const DarcyBuf
 }
 
 void launch_darcy_net_input(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, const double *p,
-                            float *input, bool free_running)
+                            float *input, bool free_running, uint16_t *in4)
 {
     hipLaunchKernelGGL(k_darcy_net_input, cell_grid(d), dim3(256), 0, s, d, b.jc, b.jfrozen, p,
                        free_running ? (const unsigned long long *)nullptr : b.maxres + DARCY_STATE,
-                       b.maxres + DARCY_SCALE, input);
+                       b.maxres + DARCY_SCALE, input, in4);
 }
 
 void launch_darcy_net_apply(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, double *p,

@@ -828,20 +828,22 @@ static int darcy_step(sph_ctx *ctx, std::ofstream *convlog)
             // setDarcyTopWallFixedFlow (bc_top == 1) writes dev_darcy_p, which
             // the copy of p_new into p overwrites: it has no effect
             // (device.cu:2284-2331)
+            // learned correction of the iterate this deciding sweep wrote
+            // (its kernels do nothing once the solve has ended); the sweep
+            // clears the correction's scale slot
+            const bool correct = decides && ctx->net && corrections < ctx->net_max_corr;
             launch_darcy_sweep(s, da, d, buf[nijac & 1u], buf[(nijac & 1u) ^ 1u],
                                check ? slot : nullptr, check ? next : nullptr, prev, h.tolerance,
-                               nijac - 1u);
-            // learned correction of the iterate this deciding sweep wrote
-            // (its kernels do nothing once the solve has ended)
-            if (decides && ctx->net && corrections < ctx->net_max_corr) {
+                               nijac - 1u, false, correct);
+            if (correct) {
                 double *pc = buf[(nijac & 1u) ^ 1u];
-                launch_fill_u32(s, (uint32_t *)(d.maxres + DARCY_SCALE), 2, 0u);
                 launch_darcy_residual_max(s, da, d, pc, false);
-                launch_darcy_net_input(s, da, d, pc, ctx->net_in, false);
-                int nrc = net_forward_packed(ctx->net, ctx->net_params, ctx->net_in, ctx->net_out, s);
+                // the input goes straight into the network's BF16 rows, and
+                // the correction refreshes the ghosts of the cells it changes
+                launch_darcy_net_input(s, da, d, pc, nullptr, false, net_input_bf16(ctx->net));
+                int nrc = net_forward_packed(ctx->net, ctx->net_params, nullptr, ctx->net_out, s);
                 if (nrc) return fail(ctx, nrc, sph_net_last_error());
                 launch_darcy_net_apply(s, da, d, pc, ctx->net_out);
-                launch_darcy_ghosts(s, da, pc, 1);
                 ++corrections;
             }
         }

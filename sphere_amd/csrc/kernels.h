@@ -169,11 +169,14 @@ void launch_darcy_prepare(hipStream_t s, const DarcyArgs &d, const DarcyBuffers 
 void launch_darcy_sweep(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b,
                         const double *p_in, double *p_out, unsigned long long *check,
                         unsigned long long *clear, const unsigned long long *prev,
-                        double tolerance, uint32_t prev_nijac, bool free_running = false);
+                        double tolerance, uint32_t prev_nijac, bool free_running = false,
+                        bool zero_scale = false);
 void launch_darcy_residual_max(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, const double *p,
                                bool free_running);
+// the network input of iterate p: FP32 rows of 3 (input) and/or the
+// network's BF16 rows padded to 4 (in4); either may be null
 void launch_darcy_net_input(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, const double *p,
-                            float *input, bool free_running);
+                            float *input, bool free_running, uint16_t *in4 = nullptr);
 void launch_darcy_net_apply(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, double *p,
                             const float *e);
 void launch_darcy_decide(hipStream_t s, const DarcyBuffers &b, const unsigned long long *slot,

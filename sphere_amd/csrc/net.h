@@ -7,3 +7,4 @@
 void net_pack(sph_net *n, const float *params, hipStream_t s);
 // forward pass (input [V][3] -> output [V]) with the last packed weights
 int net_forward_packed(sph_net *n, const float *params, const float *input, float *output, hipStream_t s);
+uint16_t *net_input_bf16(sph_net *n);

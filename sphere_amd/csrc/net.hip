@@ -162,6 +162,10 @@ void net_pack(sph_net *n, const float *params, hipStream_t s)
     }
 }
 
+// the network's BF16 input rows (V x 4), which a caller may fill itself and
+// then pass a null input to net_forward_packed
+uint16_t *net_input_bf16(sph_net *n) { return n->in4; }
+
 // forward pass with the weights of the last net_pack
 int net_forward_packed(sph_net *n, const float *params, const float *input, float *output, hipStream_t s)
 {
@@ -169,7 +173,7 @@ int net_forward_packed(sph_net *n, const float *params, const float *input, floa
     const float *b[9];
     for (int l = 0; l < 9; ++l) b[l] = params + n->boff[l];
     int rc = 0;
-    launch_to_bf16_pad(s, n->V[0], input, 3, 4, n->in4);
+    if (input) launch_to_bf16_pad(s, n->V[0], input, 3, 4, n->in4);  // else in4 is filled
     rc |= launch_conv_fwd(s, g[0], n->in4, 4, n->wf[0], 16, b[0], 16, n->x0, NET_OUT_BF16_RELU);
     launch_pool_fwd(s, g[1], n->x0, 16, n->p0);
     rc |= launch_conv_fwd(s, g[1], n->p0, 16, n->wf[1], 32, b[1], 32, n->x1, NET_OUT_BF16_RELU);

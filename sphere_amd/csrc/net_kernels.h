@@ -5,6 +5,20 @@
 #include <cstddef>
 #include <cstdint>
 
+__device__ __forceinline__ float bf2f(uint16_t b)
+{
+    return __uint_as_float((uint32_t)b << 16);
+}
+
+// round to nearest even; NaN stays NaN
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
 struct NetGeom {
     uint32_t nx, ny, nz;
     uint32_t wrap;  // bit 0: x periodic, bit 1: y periodic (else zero padding)

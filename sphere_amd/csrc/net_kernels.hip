@@ -15,19 +15,6 @@
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float bf2f(uint16_t b)
-{
-    return __uint_as_float((uint32_t)b << 16);
-}
-
-// round to nearest even; NaN stays NaN
-__device__ __forceinline__ uint16_t f2bf(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
 
 // neighbour (x, y, z) of the 3x3x3 stencil: wrapped along periodic axes,
 // false where it falls in the zero padding
