@@ -624,7 +624,8 @@ __device__ __forceinline__ double jacobi_target(const DarcyArgs &d, const double
 // the same verdict from the same slot) and returns if the solve has ended;
 // once state[0] is set, every later sweep returns at once, so the host may
 // enqueue sweeps beyond the convergence point.
-__global__ void __launch_bounds__(256) k_darcy_sweep(DarcyArgs d, const double *__restrict__ jc,
+#define SWEEP_THREADS 512
+__global__ void __launch_bounds__(SWEEP_THREADS) k_darcy_sweep(DarcyArgs d, const double *__restrict__ jc,
                                                      const uint8_t *__restrict__ frozen,
                                                      const double *__restrict__ p_in,
                                                      const double *__restrict__ k_a,
@@ -636,7 +637,7 @@ __global__ void __launch_bounds__(256) k_darcy_sweep(DarcyArgs d, const double *
                                                      unsigned long long *__restrict__ state,
                                                      unsigned long long *__restrict__ zero_scale)
 {
-    __shared__ unsigned long long wmax[4];
+    __shared__ unsigned long long wmax[SWEEP_THREADS / 64];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (zero_scale && i == 0) *zero_scale = 0ull;  // the scale slot of the correction that follows
     if (state[0] != 0ull) return;
@@ -702,7 +703,11 @@ void launch_darcy_sweep(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b
                         unsigned long long *clear, const unsigned long long *prev,
                         double tolerance, uint32_t prev_nijac, bool free_running, bool zero_scale)
 {
-    hipLaunchKernelGGL(k_darcy_sweep, cell_grid(d), dim3(256), 0, s, d, b.jc, b.jfrozen, p_in,
+    // 512-thread blocks: 6.7 us per sweep on the 48 x 48 x 96 grid, against
+    // 7.2 with 256 and 6.8 with 1024
+    const uint32_t n = d.nx * d.ny * d.nz;
+    hipLaunchKernelGGL(k_darcy_sweep, dim3((n + SWEEP_THREADS - 1) / SWEEP_THREADS), dim3(SWEEP_THREADS), 0, s,
+                       d, b.jc, b.jfrozen, p_in,
                        b.k, p_out, check, clear, prev, tolerance, prev_nijac,
                        b.maxres + (free_running ? DARCY_ZERO : DARCY_STATE),
                        zero_scale ? b.maxres + DARCY_SCALE : (unsigned long long *)nullptr);
