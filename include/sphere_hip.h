@@ -260,7 +260,7 @@ int sph_kernel_time(sph_ctx *ctx, int kernel_id, double *total_ms,
  * test of sweeps 0, 10, 20, ... has not ended the solve, a 3-D U-Net (BF16
  * on the matrix cores, FP32 accumulation) maps the fixed-point residual of
  * the current iterate to an estimate of its error, which is added to the
- * iterate.  Plain Jacobi stays the fallback and the judge of convergence.
+ * iterate.  Plain Jacobi stays the fallback and decides convergence.
  * ---------------------------------------------------------------------- */
 typedef struct sph_net sph_net;
 
