@@ -293,17 +293,40 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(NetGeom g, const uint16_t 
     const uint32_t tx = g.nx / 16u, ty = g.ny / 4u;
     const uint32_t bx = blockIdx.x % tx, by = (blockIdx.x / tx) % ty, bz = blockIdx.x / (tx * ty);
     const int x0 = (int)bx * 16, y0 = (int)by * 4, z0 = (int)bz * 4;
-    // stage the input tile (zeros or wrap-around outside the grid)
-    for (int q = (int)tid; q < HX * HY * HZ * (CS / 4); q += 256) {
-        const int vox = q / (CS / 4), part = q % (CS / 4);
-        const int lx = vox % HX, ly = (vox / HX) % HY, lz = vox / (HX * HY);
-        int x = x0 - 1 + lx, y = y0 - 1 + ly;
-        const int z = z0 - 1 + lz;
-        bf16x4 v = bf16x4{0, 0, 0, 0};
-        if (nb_in(g, x, y, z)) v = *(const bf16x4 *)(in + ((size_t)((z * (int)g.ny + y) * (int)g.nx + x) * CS + part * 4));
-        s_in[q] = v;
+    // stage the input tile (zeros or wrap-around outside the grid) and the
+    // weights: every load of a thread is issued before its LDS stores
+    constexpr int NIN = HX * HY * HZ * (CS / 4), NW = 27 * 16 * CS / 4;
+    constexpr int UIN = (NIN + 255) / 256, UW = (NW + 255) / 256;
+    bf16x4 vin[UIN], vw[UW];
+#pragma unroll
+    for (int u = 0; u < UIN; ++u) {
+        const int q = (int)tid + 256 * u;
+        vin[u] = bf16x4{0, 0, 0, 0};
+        if (q < NIN) {
+            const int vox = q / (CS / 4), part = q % (CS / 4);
+            const int lx = vox % HX, ly = (vox / HX) % HY, lz = vox / (HX * HY);
+            int x = x0 - 1 + lx, y = y0 - 1 + ly;
+            const int z = z0 - 1 + lz;
+            if (nb_in(g, x, y, z))
+                vin[u] = *(const bf16x4 *)(in + ((size_t)((z * (int)g.ny + y) * (int)g.nx + x) * CS + part * 4));
+        }
     }
-    for (int q = (int)tid; q < 27 * 16 * CS / 4; q += 256) s_w[q] = *(const bf16x4 *)(w + (size_t)q * 4);
+#pragma unroll
+    for (int u = 0; u < UW; ++u) {
+        const int q = (int)tid + 256 * u;
+        vw[u] = bf16x4{0, 0, 0, 0};
+        if (q < NW) vw[u] = *(const bf16x4 *)(w + (size_t)q * 4);
+    }
+#pragma unroll
+    for (int u = 0; u < UIN; ++u) {
+        const int q = (int)tid + 256 * u;
+        if (q < NIN) s_in[q] = vin[u];
+    }
+#pragma unroll
+    for (int u = 0; u < UW; ++u) {
+        const int q = (int)tid + 256 * u;
+        if (q < NW) s_w[q] = vw[u];
+    }
     __syncthreads();
     const uint32_t m = lane & 15u, grp = lane >> 4;
     f32x4 acc[4][1];
