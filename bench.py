@@ -48,6 +48,8 @@ def measured_traffic():
         cells = [c.strip() for c in line.split("|")]
         if len(cells) > 4 and ("k_forces" in cells[1] or "k_topology" in cells[1]):
             mb = [c for c in cells if c.endswith(" MB")]  # the scaled bytes column
+            if not mb:
+                continue  # a row of another table (instruction counters)
             total += float(mb[-1].split()[0]) * 1e6
             found += 1
     return total if found == 2 else None
