@@ -46,6 +46,13 @@ __device__ __forceinline__ void vox_xyz(const NetGeom &g, uint32_t v, int &x, in
 // accumulator holds D[4 (lane / 16) + i][lane % 16].
 // ---------------------------------------------------------------------------
 #define CONV_G 4
+// K split of the grids of at most CONV_SMALL_V voxels: CONV_KS_SMALL waves
+// per 16-voxel tile (4 on larger grids below 65536 voxels).  On the network's
+// 48 x 48 x 96 grid, 8 waves on its two coarsest levels took one forward pass
+// from 154 to 144 us; 16 waves, or 8 on the 27,648-voxel level too, were
+// slower.
+#define CONV_SMALL_V 8192u
+#define CONV_KS_SMALL 8
 
 // Epilogue of the convolutions: acc[j][t] holds, for M tile j (16
 // voxels from vbase[j] along x), voxels vbase[j] + 4 grp + i (i = 0..3) and
@@ -224,8 +231,8 @@ __global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd(NetGeom g, 
                             acc[j][t][i] += red[(((k * MT + j) * NT + t) * 4 + i) * 64 + lane];
         }
         if (mode == NET_OUT_RELU_UP) {
-            // wave 0 stores the coarse output and hands a = bf16(relu(acc +
-            // bias)) to the other waves, which share the 8 children per voxel
+            // wave 0 stores the coarse output and puts a = bf16(relu(acc +
+            // bias)) in LDS, from which all waves write the 8 children
             __shared__ float s_a[MT * NT * 4][64];
             uint32_t vb[MT];
 #pragma unroll
@@ -247,20 +254,38 @@ __global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd(NetGeom g, 
                     }
             }
             __syncthreads();
+            // then each lane writes 8 channels (16 B) of one child: item =
+            // ((voxel * 8 + child) * cout_real / 8 + channel octet), so
+            // neighbouring lanes fill whole fine-voxel rows (cout_real % 8 == 0)
+            const int octets = cout_real >> 3;
+            const int items = 16 * MT * 8 * octets;
+            const uint32_t fnx = 2u * g.nx, fny = 2u * g.ny;
+            for (int it = (int)threadIdx.x; it < items; it += 64 * KS) {
+                const int h = it % octets, r = it / octets;
+                const int k = r & 7, vv = r >> 3;
+                const uint32_t v = v0 + (uint32_t)vv;
+                if (v >= V) continue;
+                int x, y, z;
+                vox_xyz(g, v, x, y, z);
+                const uint32_t fv = ((2u * z + (k >> 2)) * fny + 2u * y + ((k >> 1) & 1)) * fnx + 2u * x + (k & 1);
+                const size_t fo = (size_t)fv * cout_real + 8 * h;
+                const uint4 sk = *(const uint4 *)(skip + fo);
+                const uint32_t sw[4] = {sk.x, sk.y, sk.z, sk.w};
+                uint32_t ow[4];
+                const int j = vv >> 4, vl = vv & 15;
 #pragma unroll
-            for (int j = 0; j < MT; ++j)
+                for (int q = 0; q < 4; ++q) {
+                    uint16_t o[2];
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    const int co = t * 16 + (int)m;
-                    if (co >= cout_real) continue;
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const uint32_t v = vb[j] + 4u * grp + (uint32_t)i;
-                        if (v < V)
-                            up_children(g, v, co, cout_real, s_a[(j * NT + t) * 4 + i][lane], (int)wv, KS, skip,
-                                        yfine);
+                    for (int e = 0; e < 2; ++e) {
+                        const int co = 8 * h + 2 * q + e;
+                        const float av = s_a[(j * NT + (co >> 4)) * 4 + (vl & 3)][(vl >> 2) * 16 + (co & 15)];
+                        o[e] = f2bf(av + bf2f((uint16_t)(sw[q] >> (16 * e))));
                     }
+                    ow[q] = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
                 }
+                *(uint4 *)(yfine + fo) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
+            }
             return;
         }
     }
@@ -382,13 +407,17 @@ static void conv_fwd_nt(hipStream_t s, const NetGeom &g, const uint16_t *in, int
         const uint32_t tiles = (V + 63u) / 64u;
         hipLaunchKernelGGL((k_conv_fwd<NT, 4, 1, CIN4>), dim3((tiles + 3u) / 4u), dim3(256), 0, s, g, in, cin,
                            w, bias, cout_real, out, mode, proj, out2, skip, yfine);
-    } else {
+    } else if (V > CONV_SMALL_V) {
         // below 65536 voxels the 4 waves of a block split K for one 16-voxel
         // tile (and share an upsampling epilogue): four times the waves of
         // one tile per wave, which these latency-bound grids need
         const uint32_t tiles = (V + 15u) / 16u;
         hipLaunchKernelGGL((k_conv_fwd<NT, 1, 4, CIN4>), dim3(tiles), dim3(256), 0, s, g, in, cin, w, bias,
                            cout_real, out, mode, proj, out2, skip, yfine);
+    } else {
+        const uint32_t tiles = (V + 15u) / 16u;
+        hipLaunchKernelGGL((k_conv_fwd<NT, 1, CONV_KS_SMALL, CIN4>), dim3(tiles), dim3(64 * CONV_KS_SMALL), 0, s,
+                           g, in, cin, w, bias, cout_real, out, mode, proj, out2, skip, yfine);
     }
 }
 
@@ -399,7 +428,7 @@ int launch_conv_fwd(hipStream_t s, const NetGeom &g, const uint16_t *in, int cin
     const bool c4 = cin == 4;
     if (!c4 && (cin % 16) != 0) return -1;
     if (mode == NET_OUT_PROJ && (cout_pad != 16 || cout_real != 16 || !bias || !proj || !out2)) return -1;
-    if (mode == NET_OUT_RELU_UP && (!skip || !yfine)) return -1;
+    if (mode == NET_OUT_RELU_UP && (!skip || !yfine || (cout_real & 7) != 0)) return -1;
 #define CONV_NT(NTV)                                                                                         \
     (c4 ? conv_fwd_nt<NTV, true>(s, g, in, cin, w, bias, cout_real, out, mode, proj, out2, skip, yfine)       \
         : conv_fwd_nt<NTV, false>(s, g, in, cin, w, bias, cout_real, out, mode, proj, out2, skip, yfine))
