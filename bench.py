@@ -406,7 +406,7 @@ def main():
     ap.add_argument("--net-iters", type=int, default=3000, help="Adam steps per training round")
     ap.add_argument("--net-rounds", type=int, default=2,
                     help="training rounds on iterates of the corrected solve")
-    ap.add_argument("--net-corrections", type=int, default=6, help="corrections per solve at most")
+    ap.add_argument("--net-corrections", type=int, default=8, help="corrections per solve at most")
     ap.add_argument("--darcy-save", default="", help="write the prepared consolidation box here")
     ap.add_argument("--darcy-load", default="", help="start from a box written by --darcy-save")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
