@@ -50,7 +50,7 @@ __device__ __forceinline__ void vox_xyz(const NetGeom &g, uint32_t v, int &x, in
 // per 16-voxel tile (4 on larger grids below 65536 voxels).  On the network's
 // 48 x 48 x 96 grid, 8 waves on its two coarsest levels took one forward pass
 // from 154 to 144 us; 16 waves, or 8 on the 27,648-voxel level too, were
-// slower.
+// slower, except for inputs of 64 channels (143.4 us).
 #define CONV_SMALL_V 8192u
 #define CONV_KS_SMALL 8
 
@@ -413,6 +413,11 @@ static void conv_fwd_nt(hipStream_t s, const NetGeom &g, const uint16_t *in, int
         // one tile per wave, which these latency-bound grids need
         const uint32_t tiles = (V + 15u) / 16u;
         hipLaunchKernelGGL((k_conv_fwd<NT, 1, 4, CIN4>), dim3(tiles), dim3(256), 0, s, g, in, cin, w, bias,
+                           cout_real, out, mode, proj, out2, skip, yfine);
+    } else if (cin >= 64) {
+        // the longest K (27 x 64) of the small grids: 16 waves per tile
+        const uint32_t tiles = (V + 15u) / 16u;
+        hipLaunchKernelGGL((k_conv_fwd<NT, 1, 16, CIN4>), dim3(tiles), dim3(1024), 0, s, g, in, cin, w, bias,
                            cout_real, out, mode, proj, out2, skip, yfine);
     } else {
         const uint32_t tiles = (V + 15u) / 16u;
