@@ -204,6 +204,14 @@ int sph_get_neighbor_stats(sph_ctx *ctx, sph_neighbor_stats *out);
  * next load. */
 int sph_set_rebuild_kernel(sph_ctx *ctx, int fused);
 
+/* Bound on the rebuild kernel's grid-barrier waits, in sleeps of a waiting
+ * block (0: the default, 2^22).  No counterpart in the reference.  A wait
+ * that runs past it ends the kernel on every block, the list and contact
+ * kernels after it skip their work, and the step returns SPH_ERR_HIP; a
+ * small bound forces that path from the host, for tests of the failure
+ * handling.  Takes effect at once. */
+int sph_set_barrier_spin_limit(sph_ctx *ctx, uint32_t sleeps);
+
 /* Load a state: replaces DEM::readbin + checkValues + transfer to the device
  * (src/file_io.cpp:30-417, src/sphere.cpp:145-345, src/device.cu:667-752).
  * Resets the time-loop counters (iter = 0, as every reference invocation). */

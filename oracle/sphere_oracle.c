@@ -544,8 +544,9 @@ static void contact_linear(const orc *o, acc_t *A, uint32_t b_orig, v3 vel_a, v3
  *   k_n = 4/3 E* sqrt(R*)  (normal force k_n d^(3/2)),
  *   k_t = 8 G* sqrt(R*)    (tangential stiffness k_t sqrt(d)).
  * Otherwise k_n, k_t are taken from the parameters.  The viscous terms are
- * gamma_n d^(1/4) v_n and gamma_t d^(1/4) v_t; half of each damping power is
- * booked on each particle as contactLinear does.
+ * gamma_n d^(1/4) v_n and gamma_t d^(1/4) v_t.  The energy rates are the
+ * reference's (contactmodels.cuh:506,560): ev_dot += gamma_n v_n^2, and on a
+ * slip es_dot += |delta_t0 - delta_t| k_t / dt with this law's k_t.
  */
 static void contact_hertz(const orc *o, acc_t *A, uint32_t b_orig, v3 vel_a, v3 angvel_a,
                           double r_a, double r_b, v3 x_ab, double x_ab_length, double delta_ab,
@@ -583,7 +584,8 @@ static void contact_hertz(const orc *o, acc_t *A, uint32_t b_orig, v3 vel_a, v3 
         k_t = 8.0 * G_star * sR;
     }
     v3 f_n = smul(fmax(0.0, k_n * d * sd - h->gamma_n * qd * vel_n_ab), n_ab);
-    A->ev_dot += 0.5 * h->gamma_n * qd * vel_n_ab * vel_n_ab;
+    /* contactmodels.cuh:506: the normal damping power gamma_n v_n^2 */
+    A->ev_dot += h->gamma_n * vel_n_ab * vel_n_ab;
     double f_n_length = len(f_n);
     v3 f_c = smul(-h->kappa * sqrt(r_a * r_b), n_ab);
     v3 f_t = mk(0.0, 0.0, 0.0);
@@ -598,11 +600,11 @@ static void contact_hertz(const orc *o, acc_t *A, uint32_t b_orig, v3 vel_a, v3 
             f_t_limit = h->mu_s * len(sub(f_n, f_c));
         f_t = add(f_t_elast, f_t_visc);
         double f_t_length = len(f_t);
-        A->ev_dot += 0.5 * h->gamma_t * qd * vel_t_ab_length * vel_t_ab_length;
         if (f_t_length > f_t_limit) {
             f_t = smul(f_t_limit / f_t_length, f_t);
             delta_t = sdiv(add(f_t, smul(h->gamma_t * qd, vel_t_ab)), -k_t * sd);
-            A->es_dot += 0.5 * len(smul(dt, smul(len(f_t), vel_t_ab))) / dt;
+            /* contactmodels.cuh:560: the spring energy released by the slip */
+            A->es_dot += len(sub(delta_t0, delta_t)) * k_t / dt;
         } else {
             delta_t = add(delta_t0, smul(dt, vel_t_ab));
         }

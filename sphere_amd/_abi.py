@@ -172,6 +172,7 @@ def load_library(path=None):
         "sph_set_contact_slots": (ctypes.c_int, [vp, ctypes.c_int]),
         "sph_set_neighbor_lists": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int]),
         "sph_set_rebuild_kernel": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sph_set_barrier_spin_limit": (ctypes.c_int, [vp, ctypes.c_uint32]),
         "sph_get_neighbor_stats": (ctypes.c_int, [vp, ctypes.POINTER(SphNeighborStats)]),
         "sph_load_state": (ctypes.c_int, [vp, ctypes.POINTER(SphState)]),
         "sph_load_file": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
@@ -328,6 +329,10 @@ class Engine:
         s = SphNeighborStats()
         self._check(self.lib.sph_get_neighbor_stats(self.ctx, ctypes.byref(s)))
         return s
+
+    def set_barrier_spin_limit(self, sleeps):
+        """Bound of the list rebuild kernel's grid-barrier waits (0: default)."""
+        self._check(self.lib.sph_set_barrier_spin_limit(self.ctx, int(sleeps)))
 
     def set_profiling(self, on):
         self._check(self.lib.sph_set_profiling(self.ctx, int(on)))

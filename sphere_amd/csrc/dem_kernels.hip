@@ -563,9 +563,10 @@ __global__ void __launch_bounds__(256) k_nl_build(
     uint32_t *__restrict__ nl_pos, double4 *__restrict__ x_ref, uint32_t *__restrict__ flags,
     const uint32_t *__restrict__ contacts, const uint64_t *__restrict__ cmask,
     uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand, float *__restrict__ nl_gap,
-    float *__restrict__ nl_gmin)
+    float *__restrict__ nl_gmin, const uint32_t *__restrict__ err)
 {
     SKIP_UNLESS(cond);
+    if ((*err & ERRF_BARRIER) != 0u) return;  // the sort before it did not finish
     const uint32_t np = a.np;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         flags[NLF_COUNT] += 1u;  // rebuild counter
@@ -585,7 +586,8 @@ void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const De
     if (nb > COND_BLOCKS) nb = COND_BLOCKS;
     hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(256), 0, s, a, n, cond, b.vals, b.cell_start,
                        b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_pos, b.x_ref, b.nl_flags,
-                       b.contacts, b.cmask, b.nl_slot, b.slot_cand, b.nl_gap, b.nl_gmin);
+                       b.contacts, b.cmask, b.nl_slot, b.slot_cand, b.nl_gap, b.nl_gmin,
+                       b.errflags);
 }
 
 // ------------------------------------------------------------------
@@ -601,14 +603,18 @@ void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const De
 
 // Grid barrier.  bar[0] counts arrivals, bar[1] is the generation; `gen` is
 // the generation this block waits to see end.  A waiter that spins longer
-// than RB_SPIN_LIMIT sleeps stops waiting and raises ERRF_BARRIER (the
-// step then fails), so the kernel always ends.
+// than `limit` sleeps, or that sees ERRF_BARRIER raised by another block,
+// stops waiting, raises ERRF_BARRIER and returns false; the caller then
+// returns, so every block of the kernel ends and the step fails with an
+// error.  Kernels launched after it skip their work while the flag is set.
 #define RB_SPIN_LIMIT (1u << 22)
-__device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t nblocks, uint32_t &gen,
-                                             uint32_t *err)
+__device__ __forceinline__ bool grid_barrier(uint32_t *bar, uint32_t nblocks, uint32_t &gen,
+                                             uint32_t *err, uint32_t limit)
 {
+    __shared__ uint32_t s_ok;
     __syncthreads();
     if (threadIdx.x == 0) {
+        uint32_t ok = 1u;
         __threadfence();
         const uint32_t arrived =
             __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -618,17 +624,22 @@ __device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t nblocks, ui
         } else {
             uint32_t spins = 0;
             while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                if (++spins > RB_SPIN_LIMIT) {
+                if (++spins > limit ||
+                    (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                     ERRF_BARRIER) != 0u) {
                     atomicOr(err, ERRF_BARRIER);
+                    ok = 0u;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
         }
         __threadfence();
+        s_ok = ok;
     }
     gen += 1u;
     __syncthreads();
+    return s_ok != 0u;
 }
 
 // Exclusive scan of 256 values over the block; returns the prefix of this
@@ -653,9 +664,9 @@ __device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t *lds, ui
 // Exclusive scan of a[0, m) in place by all blocks: block sums, one
 // barrier, then each block scans its chunk from the sum of the chunks before
 // it.
-__device__ void grid_scan(uint32_t *__restrict__ a, uint32_t m, uint32_t *__restrict__ part,
+__device__ bool grid_scan(uint32_t *__restrict__ a, uint32_t m, uint32_t *__restrict__ part,
                           uint32_t *bar, uint32_t nblocks, uint32_t &gen, uint32_t *err,
-                          uint32_t *lds)
+                          uint32_t limit, uint32_t *lds)
 {
     const uint32_t t = threadIdx.x;
     const uint32_t chunk = (m + nblocks - 1u) / nblocks;
@@ -664,7 +675,7 @@ __device__ void grid_scan(uint32_t *__restrict__ a, uint32_t m, uint32_t *__rest
     for (uint32_t i = lo + t; i < hi; i += 256u) sum += a[i];
     (void)block_scan_256(sum, lds, &tot);
     if (t == 0) part[blockIdx.x] = tot;
-    grid_barrier(bar, nblocks, gen, err);
+    if (!grid_barrier(bar, nblocks, gen, err, limit)) return false;
     uint32_t off = 0;
     for (uint32_t b = t; b < blockIdx.x; b += 256u) off += part[b];
     (void)block_scan_256(off, lds, &off);
@@ -679,11 +690,13 @@ __device__ void grid_scan(uint32_t *__restrict__ a, uint32_t m, uint32_t *__rest
         a[i] = run;
         run += v;
     }
+    return true;
 }
 
 __global__ void __launch_bounds__(256) k_rebuild(StepArgs a, RebuildArgs r)
 {
     if ((*r.cond & NLF_REBUILD) == 0u) return;
+    if ((*r.err & ERRF_BARRIER) != 0u) return;  // an earlier barrier timed out
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[RS_WAVES][256];
     const uint32_t G = gridDim.x;
@@ -701,13 +714,14 @@ __global__ void __launch_bounds__(256) k_rebuild(StepArgs a, RebuildArgs r)
         const uint32_t src = pass & 1u;
         for (uint32_t tile = blockIdx.x; tile < r.ntiles; tile += G)
             radix_hist_tile(r.keys[src], r.n, shift, r.hist, r.ntiles, tile, run);
-        grid_barrier(r.bar, G, gen, r.err);
-        grid_scan(r.hist, 256u * r.ntiles, r.part, r.bar, G, gen, r.err, run);
-        grid_barrier(r.bar, G, gen, r.err);
+        if (!grid_barrier(r.bar, G, gen, r.err, r.spin_limit)) return;
+        if (!grid_scan(r.hist, 256u * r.ntiles, r.part, r.bar, G, gen, r.err, r.spin_limit, run))
+            return;
+        if (!grid_barrier(r.bar, G, gen, r.err, r.spin_limit)) return;
         for (uint32_t tile = blockIdx.x; tile < r.ntiles; tile += G)
             radix_scatter_tile(r.keys[src], r.vals[src], r.keys[src ^ 1u], r.vals[src ^ 1u], r.n,
                                shift, r.hist, r.ntiles, tile, run, wcnt);
-        grid_barrier(r.bar, G, gen, r.err);
+        if (!grid_barrier(r.bar, G, gen, r.err, r.spin_limit)) return;
     }
     const uint32_t *keys = r.keys[r.passes & 1u], *vals = r.vals[r.passes & 1u];
     const uint32_t np = a.np;
@@ -734,7 +748,8 @@ int rebuild_blocks_per_cu()
 }
 
 void launch_rebuild(hipStream_t s, const StepArgs &a, const NLArgs &n, DemBuffers &b,
-                    uint32_t ncells, int bits, uint32_t blocks, const uint32_t *cond)
+                    uint32_t ncells, int bits, uint32_t blocks, const uint32_t *cond,
+                    uint32_t spin_limit)
 {
     if (a.np == 0) return;
     int passes = (bits + 7) / 8;
@@ -761,6 +776,7 @@ void launch_rebuild(hipStream_t s, const StepArgs &a, const NLArgs &n, DemBuffer
     r.angvel_s = b.angvel_s;
     r.bar = b.rb_bar;
     r.err = b.errflags;
+    r.spin_limit = spin_limit ? spin_limit : RB_SPIN_LIMIT;
     hipLaunchKernelGGL(k_rebuild, dim3(blocks), dim3(256), 0, s, a, r);
     // the sorted keys and values end in the buffer the last pass wrote
     if (passes & 1) {
@@ -776,7 +792,7 @@ void launch_rebuild(hipStream_t s, const StepArgs &a, const NLArgs &n, DemBuffer
     hipLaunchKernelGGL(k_nl_build, dim3((a.np + 255) / 256), dim3(256), 0, s, a, n, cond, b.vals,
                        b.cell_start, b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_pos, b.x_ref,
                        b.nl_flags, b.contacts, b.cmask, b.nl_slot, b.slot_cand, b.nl_gap,
-                       b.nl_gmin);
+                       b.nl_gmin, b.errflags);
 }
 
 // Sum and maximum of the list lengths (overflowed lists count as cap + 1).
@@ -1102,7 +1118,7 @@ __device__ __forceinline__ void law_hertz(const StepArgs &a, Contrib &C, d3 vel_
         k_t = 8.0 * P.G_star * sR;
     }
     d3 f_n = fmax(0.0, k_n * d * sd - P.gamma_n * qd * vel_n_ab) * n_ab;
-    C.ev_n = 0.5 * P.gamma_n * qd * vel_n_ab * vel_n_ab;
+    C.ev_n = P.gamma_n * vel_n_ab * vel_n_ab;  // contactmodels.cuh:506
     double f_n_length = len3(f_n);
     d3 f_c = cohesion_factor(P.kappa, r_a, r_b) * n_ab;
     d3 f_t = mk3(0.0, 0.0, 0.0);
@@ -1119,11 +1135,11 @@ __device__ __forceinline__ void law_hertz(const StepArgs &a, Contrib &C, d3 vel_
             f_t_limit = P.mu_s * fnc;
         f_t = f_t_elast + f_t_visc;
         double f_t_length = len3(f_t);
-        C.ev_t = 0.5 * P.gamma_t * qd * vel_t_ab_length * vel_t_ab_length;
         if (f_t_length > f_t_limit) {
             f_t = (f_t_limit / f_t_length) * f_t;
             delta_t = (f_t + (P.gamma_t * qd) * vel_t_ab) / (-k_t * sd);
-            C.es = 0.5 * len3(dt * (len3(f_t) * vel_t_ab)) / dt;
+            // the spring energy released by the slip, contactmodels.cuh:560
+            C.es = len3(delta_t0 - delta_t) * k_t / dt;
         } else {
             delta_t = delta_t0 + dt * vel_t_ab;
         }
@@ -1440,6 +1456,8 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     const float *__restrict__ nl_gap, const float *__restrict__ nl_gmin,
     const float *__restrict__ nl_disp, uint32_t *__restrict__ flags_rw, uint32_t cur)
 {
+    // the list rebuild before it did not finish (its grid barrier timed out)
+    if ((*err & ERRF_BARRIER) != 0u) return;
     const uint32_t *flags = flags_rw;
     __shared__ double4 s_x[TW];
     __shared__ int s_g[TW][3];
@@ -1746,8 +1764,11 @@ __global__ void __launch_bounds__(FW) k_forces(
     uint32_t *__restrict__ contacts, uint64_t *__restrict__ cmask,
     double *__restrict__ delta_t,
     uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand,
-    const float *__restrict__ nl_gap, float *__restrict__ nl_gmin)
+    const float *__restrict__ nl_gap, float *__restrict__ nl_gmin,
+    const uint32_t *__restrict__ err)
 {
+    // list mode: the list rebuild of this step did not finish
+    if (err && (*err & ERRF_BARRIER) != 0u) return;
     __shared__ uint32_t s_a[FW];
     __shared__ uint8_t s_own[FW], s_slot[FW], s_rm[FW], s_rmk[FW];
     __shared__ double s_rec[NREC][FW];
@@ -1989,12 +2010,12 @@ void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool 
         hipLaunchKernelGGL(k_forces<true>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
                            b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
                            b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask,
-                           b.delta_t, nls, scd, gap, gmin);
+                           b.delta_t, nls, scd, gap, gmin, nlist ? b.errflags : nullptr);
     else
         hipLaunchKernelGGL(k_forces<false>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
                            b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
                            b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask,
-                           b.delta_t, nls, scd, gap, gmin);
+                           b.delta_t, nls, scd, gap, gmin, nlist ? b.errflags : nullptr);
 }
 
 // ------------------------------------------------------------------

@@ -61,6 +61,7 @@ struct sph_ctx {
     uint32_t nl_cap = 128;   // list entries per particle
     bool rb_fused = true;    // list rebuild as one launch (k_rebuild)
     uint32_t rb_blocks = 0;  // its grid; 0: separate sort, reorder and list launches
+    uint32_t rb_spin = 0;    // its barrier wait bound in sleeps (0: the default)
     bool nl_on = false;
     NLArgs nla{};
     uint32_t nl_parity = 0;
@@ -969,7 +970,7 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             if (ctx->rb_blocks) {
                 ProfScope ps(ctx, SPH_K_NLIST);
                 launch_rebuild(s, a, ctx->nla, ctx->db, ctx->ncells, ctx->key_bits,
-                               ctx->rb_blocks, cond);
+                               ctx->rb_blocks, cond, ctx->rb_spin);
             } else {
                 { ProfScope ps(ctx, SPH_K_SORT); launch_sort(s, a.np, ctx->key_bits, ctx->db, cond); }
                 { ProfScope ps(ctx, SPH_K_REORDER); launch_reorder(s, a, ctx->db, ctx->ncells, cond); }
@@ -1177,6 +1178,13 @@ int sph_set_rebuild_kernel(sph_ctx *ctx, int fused)
 {
     if (!ctx) return fail(ctx, SPH_ERR_ARG, "null context");
     ctx->rb_fused = fused != 0;
+    return SPH_OK;
+}
+
+int sph_set_barrier_spin_limit(sph_ctx *ctx, uint32_t sleeps)
+{
+    if (!ctx) return fail(ctx, SPH_ERR_ARG, "null context");
+    ctx->rb_spin = sleeps;
     return SPH_OK;
 }
 

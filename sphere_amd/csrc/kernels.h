@@ -100,10 +100,12 @@ struct RebuildArgs {
     double4 *x_s, *vel_s;
     double3 *angvel_s;
     uint32_t *bar, *err;
+    uint32_t spin_limit;  // sleeps a barrier waiter takes before it gives up
 };
 int rebuild_blocks_per_cu();
 void launch_rebuild(hipStream_t s, const StepArgs &a, const NLArgs &n, DemBuffers &b,
-                    uint32_t ncells, int bits, uint32_t blocks, const uint32_t *cond);
+                    uint32_t ncells, int bits, uint32_t blocks, const uint32_t *cond,
+                    uint32_t spin_limit);
 void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const DemBuffers &b,
                      const uint32_t *cond);
 // out[0] += sum of list lengths, out[1] = max(out[1], longest list)

@@ -57,130 +57,153 @@ class _Reader:
         return a
 
 
+# ----------------------------------------------------------------------
+# What a new ``sim`` holds (python/sphere.py:57-455).  These initial values
+# are part of the interface: writebin puts them in the file for every field
+# a script leaves alone.
+# ----------------------------------------------------------------------
+_F64, _U32, _I32 = numpy.float64, numpy.uint32, numpy.int32
+
+# Per-particle arrays: name, columns ("nd": one per spatial dimension, None:
+# a flat array), dtype, initial value.
+_PARTICLE_ARRAYS = (
+    ("x", "nd", _F64, 0.0), ("radius", None, _F64, 1.0), ("xyzsum", 3, _F64, 0.0),
+    ("vel", "nd", _F64, 0.0), ("fixvel", None, _F64, 0.0), ("force", "nd", _F64, 0.0),
+    ("angpos", "nd", _F64, 0.0), ("angvel", "nd", _F64, 0.0), ("torque", "nd", _F64, 0.0),
+    ("es_dot", None, _F64, 0.0), ("es", None, _F64, 0.0), ("ev_dot", None, _F64, 0.0),
+    ("ev", None, _F64, 0.0), ("p", None, _F64, 0.0), ("color", None, _I32, 0),
+)
+
+# One-value fields: name -> (dtype, initial value).
+_SCALARS = {
+    "time_dt": (_F64, 0.0), "time_current": (_F64, 0.0), "time_total": (_F64, 0.0),
+    "time_file_dt": (_F64, 0.0), "time_step_count": (_U32, 0),
+    "periodic": (_U32, 0), "adaptive": (_U32, 0),
+    # contact law: model 2 (linear-viscous) with stiff springs and friction 0.5
+    "k_n": (_F64, 1.16e9), "k_t": (_F64, 1.16e9), "k_r": (_F64, 0.0), "E": (_F64, 0.0),
+    "gamma_n": (_F64, 0.0), "gamma_t": (_F64, 0.0), "gamma_r": (_F64, 0.0),
+    "mu_s": (_F64, 0.5), "mu_d": (_F64, 0.5), "mu_r": (_F64, 0.0),
+    "gamma_wn": (_F64, 1.0e3), "gamma_wt": (_F64, 0.0),
+    "mu_ws": (_F64, 0.5), "mu_wd": (_F64, 0.5), "rho": (_F64, 2600.0),
+    "contactmodel": (_U32, 2), "kappa": (_F64, 0.0), "db": (_F64, 0.0), "V_b": (_F64, 0.0),
+    # top-wall stress modulation and the wmode-3 shear stress
+    "w_sigma0_A": (_F64, 0.0), "w_sigma0_f": (_F64, 0.0), "w_tau_x": (_F64, 0.0),
+    # bonds: none, unbreakable
+    "lambda_bar": (_F64, 1.0), "nb0": (_U32, 0),
+    "sigma_b": (_F64, numpy.inf), "tau_b": (_F64, numpy.inf),
+}
+
+# Normal of dynamic wall i: the top wall faces down, then the -x, +x, -y and
+# +y side walls face outwards along their axis.
+_WALL_NORMALS = numpy.array([[0.0, 0.0, -1.0], [-1.0, 0.0, 0.0], [1.0, 0.0, 0.0],
+                             [0.0, -1.0, 0.0], [0.0, 1.0, 0.0]])
+
+# Per-wall arrays other than the normals: name -> (dtype, initial value).
+_WALL_ARRAYS = {"wmode": (_I32, 0), "w_x": (_F64, 1.0), "w_m": (_F64, 0.0),
+                "w_vel": (_F64, 0.0), "w_force": (_F64, 0.0), "w_sigma0": (_F64, 0.0)}
+
+# Fluid fields both solvers share, besides the cell fields.
+_FLUID_SCALARS = {
+    "rho_f": (_F64, 1.0e3), "p_mod_A": (_F64, 0.0), "p_mod_f": (_F64, 0.0),
+    "p_mod_phi": (_F64, 0.0), "bc_bot": (_I32, 0), "bc_top": (_I32, 0),
+    "free_slip_bot": (_I32, 1), "free_slip_top": (_I32, 1),
+    "bc_bot_flux": (_F64, 0.0), "bc_top_flux": (_F64, 0.0),
+}
+
+# Solver-specific fields (cfd_solver 0: Navier-Stokes, 1: Darcy); the
+# per-particle fluid forces listed in _SOLVER_FORCES start at zero.
+_SOLVER_SCALARS = {
+    0: {"gamma": (_F64, 0.0), "theta": (_F64, 1.0), "beta": (_F64, 0.0),
+        "tolerance": (_F64, 1.0e-3), "maxiter": (_U32, 10000), "ndem": (_U32, 1),
+        "c_phi": (_F64, 1.0), "c_v": (_F64, 1.0), "dt_dem_fac": (_F64, 1.0)},
+    1: {"tolerance": (_F64, 1.0e-3), "maxiter": (_U32, 10000), "ndem": (_U32, 1),
+        "c_phi": (_F64, 1.0), "beta_f": (_F64, 4.5e-10), "k_c": (_F64, 4.6e-10)},
+}
+_SOLVER_FORCES = {0: ("f_d", "f_p", "f_v", "f_sum"), 1: ("f_d", "f_p")}
+# Darcy's lateral boundaries are periodic unless a script says otherwise
+_DARCY_LATERAL = ("bc_xn", "bc_xp", "bc_yn", "bc_yp")
+
+
+def _one(dtype, value):
+    return numpy.full(1, value, dtype=dtype)
+
+
 class sim:
     """Class containing all ``sphere`` data (python/sphere.py:33-455)."""
 
     def __init__(self, sid='unnamed', np=0, nd=3, nw=0, fluid=False, cfd_solver=0):
-        self.version = numpy.ones(1, dtype=numpy.float64) * VERSION
-        self.nd = numpy.ones(1, dtype=numpy.int32) * int(nd)
-        self.np = numpy.ones(1, dtype=numpy.uint32) * int(np)
         self.sid = sid
-        self.time_dt = numpy.zeros(1, dtype=numpy.float64)
-        self.time_current = numpy.zeros(1, dtype=numpy.float64)
-        self.time_total = numpy.zeros(1, dtype=numpy.float64)
-        self.time_file_dt = numpy.zeros(1, dtype=numpy.float64)
-        self.time_step_count = numpy.zeros(1, dtype=numpy.uint32)
-        self.origo = numpy.zeros(self.nd, dtype=numpy.float64)
-        self.L = numpy.zeros(self.nd, dtype=numpy.float64)
-        self.num = numpy.zeros(self.nd, dtype=numpy.uint32)
-        self.periodic = numpy.zeros(1, dtype=numpy.uint32)
-        self.adaptive = numpy.zeros(1, dtype=numpy.uint32)
-        n = int(self.np[0])
-        d = int(self.nd[0])
-        self.x = numpy.zeros((n, d), dtype=numpy.float64)
-        self.radius = numpy.ones(n, dtype=numpy.float64)
-        self.xyzsum = numpy.zeros((n, 3), dtype=numpy.float64)
-        self.vel = numpy.zeros((n, d), dtype=numpy.float64)
-        self.fixvel = numpy.zeros(n, dtype=numpy.float64)
-        self.force = numpy.zeros((n, d), dtype=numpy.float64)
-        self.angpos = numpy.zeros((n, d), dtype=numpy.float64)
-        self.angvel = numpy.zeros((n, d), dtype=numpy.float64)
-        self.torque = numpy.zeros((n, d), dtype=numpy.float64)
-        self.es_dot = numpy.zeros(n, dtype=numpy.float64)
-        self.es = numpy.zeros(n, dtype=numpy.float64)
-        self.ev_dot = numpy.zeros(n, dtype=numpy.float64)
-        self.ev = numpy.zeros(n, dtype=numpy.float64)
-        self.p = numpy.zeros(n, dtype=numpy.float64)
-        self.g = numpy.array([0.0, 0.0, 0.0], dtype=numpy.float64)
-        self.k_n = numpy.ones(1, dtype=numpy.float64) * 1.16e9
-        self.k_t = numpy.ones(1, dtype=numpy.float64) * 1.16e9
-        self.k_r = numpy.zeros(1, dtype=numpy.float64)
-        self.E = numpy.zeros(1, dtype=numpy.float64)
-        self.gamma_n = numpy.zeros(1, dtype=numpy.float64)
-        self.gamma_t = numpy.zeros(1, dtype=numpy.float64)
-        self.gamma_r = numpy.zeros(1, dtype=numpy.float64)
-        self.mu_s = numpy.ones(1, dtype=numpy.float64) * 0.5
-        self.mu_d = numpy.ones(1, dtype=numpy.float64) * 0.5
-        self.mu_r = numpy.zeros(1, dtype=numpy.float64)
-        self.gamma_wn = numpy.ones(1, dtype=numpy.float64) * 1.0e3
-        self.gamma_wt = numpy.zeros(1, dtype=numpy.float64)
-        self.mu_ws = numpy.ones(1, dtype=numpy.float64) * 0.5
-        self.mu_wd = numpy.ones(1, dtype=numpy.float64) * 0.5
-        self.rho = numpy.ones(1, dtype=numpy.float64) * 2600.0
-        self.contactmodel = numpy.ones(1, dtype=numpy.uint32) * 2
-        self.kappa = numpy.zeros(1, dtype=numpy.float64)
-        self.db = numpy.zeros(1, dtype=numpy.float64)
-        self.V_b = numpy.zeros(1, dtype=numpy.float64)
-        self.nw = numpy.ones(1, dtype=numpy.uint32) * int(nw)
-        w = int(self.nw[0])
-        self.wmode = numpy.zeros(w, dtype=numpy.int32)
-        self.w_n = numpy.zeros((w, d), dtype=numpy.float64)
-        if w >= 1:
-            self.w_n[0, 2] = -1.0
-        if w >= 2:
-            self.w_n[1, 0] = -1.0
-        if w >= 3:
-            self.w_n[2, 0] = 1.0
-        if w >= 4:
-            self.w_n[3, 1] = -1.0
-        if w >= 5:
-            self.w_n[4, 1] = 1.0
-        self.w_x = numpy.ones(w, dtype=numpy.float64)
-        self.w_m = numpy.zeros(w, dtype=numpy.float64)
-        self.w_vel = numpy.zeros(w, dtype=numpy.float64)
-        self.w_force = numpy.zeros(w, dtype=numpy.float64)
-        self.w_sigma0 = numpy.zeros(w, dtype=numpy.float64)
-        self.w_sigma0_A = numpy.zeros(1, dtype=numpy.float64)
-        self.w_sigma0_f = numpy.zeros(1, dtype=numpy.float64)
-        self.w_tau_x = numpy.zeros(1, dtype=numpy.float64)
-        self.lambda_bar = numpy.ones(1, dtype=numpy.float64)
-        self.nb0 = numpy.zeros(1, dtype=numpy.uint32)
-        self.sigma_b = numpy.ones(1, dtype=numpy.float64) * numpy.inf
-        self.tau_b = numpy.ones(1, dtype=numpy.float64) * numpy.inf
-        self.bonds = numpy.zeros((0, 2), dtype=numpy.uint32)
-        self.bonds_delta_n = numpy.zeros(0, dtype=numpy.float64)
-        self.bonds_delta_t = numpy.zeros((0, d), dtype=numpy.float64)
-        self.bonds_omega_n = numpy.zeros(0, dtype=numpy.float64)
-        self.bonds_omega_t = numpy.zeros((0, d), dtype=numpy.float64)
+        self.version = _one(_F64, VERSION)
+        self.nd = _one(_I32, int(nd))
+        self.np = _one(_U32, int(np))
+        d = int(nd)
+        for name, (dt, v) in _SCALARS.items():
+            setattr(self, name, _one(dt, v))
+        self.g = numpy.zeros(3)
+        self.origo = numpy.zeros(d)
+        self.L = numpy.zeros(d)
+        self.num = numpy.zeros(d, dtype=_U32)
+        self._alloc_particles(int(np))
+        self._alloc_walls(int(nw))
+        self._alloc_bonds(0)
         self.fluid = fluid
-        if self.fluid:
-            # python/sphere.py:316-451: the constructor always starts from the
-            # Navier-Stokes defaults; initFluid(cfd_solver=1) selects Darcy.
-            self.cfd_solver = numpy.zeros(1, dtype=numpy.int32)
-            self.mu = numpy.zeros(1, dtype=numpy.float64)
-            self._alloc_fluid(p=0.0)
-            self.rho_f = numpy.ones(1, dtype=numpy.float64) * 1.0e3
-            self.gamma = numpy.array(0.0)
-            self.theta = numpy.array(1.0)
-            self.beta = numpy.array(0.0)
-            self.tolerance = numpy.array(1.0e-3)
-            self.maxiter = numpy.array(1e4)
-            self.ndem = numpy.array(1)
-            self.c_phi = numpy.ones(1, dtype=numpy.float64)
-            self.c_v = numpy.ones(1, dtype=numpy.float64)
-            self.dt_dem_fac = numpy.ones(1, dtype=numpy.float64)
-            self.f_d = numpy.zeros((n, d), dtype=numpy.float64)
-            self.f_p = numpy.zeros((n, d), dtype=numpy.float64)
-            self.f_v = numpy.zeros((n, d), dtype=numpy.float64)
-            self.f_sum = numpy.zeros((n, d), dtype=numpy.float64)
-        self.color = numpy.zeros(n, dtype=numpy.int32)
+        if fluid:
+            # the constructor always starts from the Navier-Stokes fields
+            # (its cfd_solver argument is not read there); initFluid selects
+            # the solver
+            self.mu = _one(_F64, 0.0)
+            self._fluid_fields(p=0.0, phi=0.0)
+            self._solver_fields(0, lateral=False)
 
-    def _alloc_fluid(self, p):
-        nn = (int(self.num[0]), int(self.num[1]), int(self.num[2]))
-        self.v_f = numpy.zeros(nn + (int(self.nd[0]),), dtype=numpy.float64)
-        self.p_f = numpy.ones(nn, dtype=numpy.float64) * p
-        self.phi = numpy.zeros(nn, dtype=numpy.float64)
-        self.dphi = numpy.zeros(nn, dtype=numpy.float64)
-        self.p_mod_A = numpy.zeros(1, dtype=numpy.float64)
-        self.p_mod_f = numpy.zeros(1, dtype=numpy.float64)
-        self.p_mod_phi = numpy.zeros(1, dtype=numpy.float64)
-        self.bc_bot = numpy.zeros(1, dtype=numpy.int32)
-        self.bc_top = numpy.zeros(1, dtype=numpy.int32)
-        self.free_slip_bot = numpy.ones(1, dtype=numpy.int32)
-        self.free_slip_top = numpy.ones(1, dtype=numpy.int32)
-        self.bc_bot_flux = numpy.zeros(1, dtype=numpy.float64)
-        self.bc_top_flux = numpy.zeros(1, dtype=numpy.float64)
-        self.p_f_constant = numpy.zeros(nn, dtype=numpy.int32)
+    # -- allocation --------------------------------------------------------
+    def _alloc_particles(self, n):
+        d = int(self.nd[0])
+        for name, cols, dt, v in _PARTICLE_ARRAYS:
+            shape = (n,) if cols is None else (n, d if cols == "nd" else cols)
+            setattr(self, name, numpy.full(shape, v, dtype=dt))
+
+    def _alloc_walls(self, nw):
+        self.nw = _one(_U32, nw)
+        for name, (dt, v) in _WALL_ARRAYS.items():
+            setattr(self, name, numpy.full(nw, v, dtype=dt))
+        self.w_n = numpy.zeros((nw, int(self.nd[0])))
+        k = min(nw, len(_WALL_NORMALS))
+        self.w_n[:k] = _WALL_NORMALS[:k]
+
+    def _alloc_bonds(self, nb):
+        d = int(self.nd[0])
+        self.bonds = numpy.zeros((nb, 2), dtype=_U32)
+        self.bonds_delta_n = numpy.zeros(nb)
+        self.bonds_delta_t = numpy.zeros((nb, d))
+        self.bonds_omega_n = numpy.zeros(nb)
+        self.bonds_omega_t = numpy.zeros((nb, d))
+
+    def _fluid_fields(self, p, phi):
+        """Cell fields on the current grid (pressure p, porosity phi, zero
+        velocities and porosity change, no fixed-pressure cells) and the
+        fluid scalars both solvers share."""
+        cells = tuple(int(v) for v in self.num[:3])
+        self.v_f = numpy.zeros(cells + (int(self.nd[0]),))
+        self.p_f = numpy.full(cells, p, dtype=_F64)
+        self.phi = numpy.full(cells, phi, dtype=_F64)
+        self.dphi = numpy.zeros(cells)
+        self.p_f_constant = numpy.zeros(cells, dtype=_I32)
+        for name, (dt, v) in _FLUID_SCALARS.items():
+            setattr(self, name, _one(dt, v))
+
+    def _solver_fields(self, solver, lateral):
+        if solver not in _SOLVER_SCALARS:
+            raise Exception('Value of cfd_solver not understood (' + str(solver) + ')')
+        self.cfd_solver = _one(_I32, solver)
+        for name, (dt, v) in _SOLVER_SCALARS[solver].items():
+            setattr(self, name, _one(dt, v))
+        n, d = int(self.np[0]), int(self.nd[0])
+        for name in _SOLVER_FORCES[solver]:
+            setattr(self, name, numpy.zeros((n, d)))
+        if solver == 1 and lateral:
+            for name in _DARCY_LATERAL:
+                setattr(self, name, _one(_I32, 2))
+
 
     # ------------------------------------------------------------------
     # comparison (python/sphere.py:456-793; __cmp__ is Python 2 only)
@@ -677,64 +700,80 @@ class sim:
                 .transpose(2, 1, 0).astype(numpy.int32).copy()
             self.f_p = buf.get('f_p', n, 3).copy()
 
-    def addParticle(self, x, radius, xyzsum=numpy.zeros(3), vel=numpy.zeros(3),
-                    fixvel=numpy.zeros(1), force=numpy.zeros(3), angpos=numpy.zeros(3),
-                    angvel=numpy.zeros(3), torque=numpy.zeros(3), es_dot=numpy.zeros(1),
-                    es=numpy.zeros(1), ev_dot=numpy.zeros(1), ev=numpy.zeros(1),
-                    p=numpy.zeros(1), color=0):
-        """Append one particle (python/sphere.py:821-891)."""
-        self.np = self.np + 1
-        self.x = numpy.append(self.x.reshape(-1, 3), [x], axis=0)
-        self.radius = numpy.append(self.radius, radius)
-        self.vel = numpy.append(self.vel.reshape(-1, 3), [vel], axis=0)
-        self.xyzsum = numpy.append(self.xyzsum.reshape(-1, 3), [xyzsum], axis=0)
-        self.fixvel = numpy.append(self.fixvel, fixvel)
-        self.force = numpy.append(self.force.reshape(-1, 3), [force], axis=0)
-        self.angpos = numpy.append(self.angpos.reshape(-1, 3), [angpos], axis=0)
-        self.angvel = numpy.append(self.angvel.reshape(-1, 3), [angvel], axis=0)
-        self.torque = numpy.append(self.torque.reshape(-1, 3), [torque], axis=0)
-        self.es_dot = numpy.append(self.es_dot, es_dot)
-        self.es = numpy.append(self.es, es)
-        self.ev_dot = numpy.append(self.ev_dot, ev_dot)
-        self.ev = numpy.append(self.ev, ev)
-        self.p = numpy.append(self.p, p)
-        self.color = numpy.append(self.color, color).astype(numpy.int32)
+    # ------------------------------------------------------------------
+    # particles
+    # ------------------------------------------------------------------
+    def _particle_fields(self):
+        """(name, columns) of every per-particle array, the fluid forces
+        included; columns is None for a flat array."""
+        d = int(self.nd[0])
+        out = [(name, None if cols is None else (d if cols == "nd" else cols))
+               for name, cols, _, _ in _PARTICLE_ARRAYS]
         if self.fluid:
-            self.f_p = numpy.append(self.f_p.reshape(-1, 3), [numpy.zeros(3)], axis=0)
+            solver = int(numpy.asarray(self.cfd_solver).ravel()[0])
+            out += [(f, d) for f in _SOLVER_FORCES.get(solver, ()) if hasattr(self, f)]
+        return out
+
+    def addParticle(self, x, radius, xyzsum=None, vel=None, fixvel=0.0, force=None,
+                    angpos=None, angvel=None, torque=None, es_dot=0.0, es=0.0, ev_dot=0.0,
+                    ev=0.0, p=0.0, color=0):
+        """Append one particle; omitted vectors are zero (python/sphere.py
+        :821-891)."""
+        given = dict(x=x, radius=radius, xyzsum=xyzsum, vel=vel, fixvel=fixvel, force=force,
+                     angpos=angpos, angvel=angvel, torque=torque, es_dot=es_dot, es=es,
+                     ev_dot=ev_dot, ev=ev, p=p, color=color)
+        for name, cols in self._particle_fields():
+            old = numpy.asarray(getattr(self, name))
+            value = given.get(name)
+            shape = (1,) if cols is None else (1, cols)
+            row = numpy.zeros(shape, old.dtype) if value is None else \
+                numpy.asarray(value, old.dtype).reshape(-1)[:int(numpy.prod(shape))].reshape(shape)
+            setattr(self, name, numpy.concatenate([old.reshape((-1,) + shape[1:]), row]))
+        self.np = self.np + 1
 
     def deleteParticle(self, i):
-        """Delete the particle(s) with index i (python/sphere.py:892-935)."""
+        """Remove the particle(s) with index i (python/sphere.py:892-935)."""
+        for name, _ in self._particle_fields():
+            setattr(self, name, numpy.delete(getattr(self, name), i, axis=0))
         self.np = self.np - numpy.size(i)
-        for nm in ['x', 'radius', 'vel', 'xyzsum', 'fixvel', 'force', 'angpos', 'angvel',
-                   'torque', 'es_dot', 'es', 'ev_dot', 'ev', 'p', 'color']:
-            setattr(self, nm, numpy.delete(getattr(self, nm), i, axis=0))
-        if self.fluid:
-            self.f_p = numpy.delete(self.f_p, i, axis=0)
 
-    # ------------------------------------------------------------------
-    # setup helpers (python/sphere.py:2358-3170)
-    # ------------------------------------------------------------------
     def generateRadii(self, psd='logn', mean=440e-6, variance=8.8e-9, histogram=False):
-        """python/sphere.py:2358-2416"""
+        """Draw the radii from numpy's global generator (python/sphere.py
+        :2358-2416): 'logn' is the lognormal distribution with the given mean
+        and variance, 'uni' the uniform distribution on mean -+ variance."""
+        n = int(self.np[0])
         if psd == 'logn':
-            mu = math.log((mean**2) / math.sqrt(variance + mean**2))
-            sigma = math.sqrt(math.log(variance / (mean**2) + 1))
-            self.radius = numpy.random.lognormal(mu, sigma, self.np)
+            # the lognormal whose mean and variance are the given ones:
+            # s^2 = ln(1 + var/mean^2), m = ln(mean) - s^2/2
+            s2 = math.log(1.0 + variance / mean**2)
+            self.radius = numpy.random.lognormal(math.log(mean) - 0.5 * s2, math.sqrt(s2), n)
         elif psd == 'uni':
-            radius_min = mean - variance
-            radius_max = mean + variance
-            self.radius = numpy.random.uniform(radius_min, radius_max, self.np)
+            self.radius = numpy.random.uniform(mean - variance, mean + variance, n)
         else:
-            raise Exception('Particle size distribution type not understood (' + str(psd)
-                            + '). Valid values are \'uni\' or \'logn\'')
+            raise Exception("generateRadii: psd must be 'logn' or 'uni', not %r" % (psd,))
 
     def contactModel(self, contactmodel):
         self.contactmodel[0] = contactmodel
 
+    def zeroKinematics(self):
+        """Zero forces, torques, linear and angular velocities and positions,
+        accumulated energies and displacement sums (python/sphere.py:2930-2947);
+        es_dot, ev_dot, p and fixvel are kept."""
+        n, d = int(self.np[0]), int(self.nd[0])
+        for name in ("force", "torque", "vel", "angvel", "angpos"):
+            setattr(self, name, numpy.zeros((n, d)))
+        self.es = numpy.zeros(n)
+        self.ev = numpy.zeros(n)
+        self.xyzsum = numpy.zeros((n, 3))
+
+    # ------------------------------------------------------------------
+    # grid and boundaries (python/sphere.py:2418-2785)
+    # ------------------------------------------------------------------
     def wall0iz(self):
-        if self.nw[0] > 0:
-            return int(self.w_x[0] / (self.L[2] / self.num[2]))
-        raise Exception('No dynamic top wall present!')
+        """Grid layer of the top wall."""
+        if self.nw[0] < 1:
+            raise Exception('No dynamic top wall present!')
+        return int(self.w_x[0] / (self.L[2] / self.num[2]))
 
     def normalBoundariesXY(self):
         self.periodic[0] = 0
@@ -751,330 +790,308 @@ class sim:
     def staticGrid(self):
         self.adaptive[0] = 0
 
+    def _size_grid(self, cell, smallest):
+        """num = ceil((L - origo) / cell) per axis; at least `smallest` cells
+        along every axis."""
+        self.num[:3] = numpy.ceil((self.L[:3] - self.origo[:3]) / cell)
+        if numpy.any(self.num[:3] < smallest):
+            raise Exception("Error: the grid has %s cells; it needs at least %d along each axis"
+                            % (list(self.num[:3]), smallest))
+
+    def _default_cell(self, dx):
+        """dx when positive, else 2.1 times the largest radius."""
+        return dx if dx > 0.0 else 2.1 * numpy.amax(self.radius)
+
     def defineWorldBoundaries(self, L, origo=[0.0, 0.0, 0.0], dx=-1):
-        """python/sphere.py:2607-2653"""
-        if dx > 0.0:
-            cellsize_min = dx
-        else:
-            if self.np[0] < 1:
-                raise Exception('Error: You need to define dx in defineWorldBoundaries if '
-                                'there are no particles in the simulation.')
-            cellsize_min = 2.1 * numpy.amax(self.radius)
+        """World from origo to L with cells of at least dx (python/sphere.py
+        :2607-2653)."""
+        if dx <= 0.0 and self.np[0] < 1:
+            raise Exception('defineWorldBoundaries: without particles the cell size dx is '
+                            'required')
+        cell = self._default_cell(dx)
         self.origo[:] = origo[:]
         self.L[:] = L[:]
-        self.num[0] = numpy.ceil((self.L[0] - self.origo[0]) / cellsize_min)
-        self.num[1] = numpy.ceil((self.L[1] - self.origo[1]) / cellsize_min)
-        self.num[2] = numpy.ceil((self.L[2] - self.origo[2]) / cellsize_min)
-        if self.num[0] < 3 or self.num[1] < 3 or self.num[2] < 3:
-            raise Exception("Error: The grid must be at least 3 cells in each direction\n"
-                            "Grid: x={}, y={}, z={}\n".format(self.num[0], self.num[1], self.num[2])
-                            + "Please increase the world size.")
+        self._size_grid(cell, 3)
 
     def initGrid(self, dx=-1):
-        """python/sphere.py:2655-2685 (upper wall put at L[0], as there)"""
-        if dx > 0.0:
-            cellsize_min = dx
-        else:
-            cellsize_min = 2.1 * numpy.amax(self.radius)
-        self.num[0] = numpy.ceil((self.L[0] - self.origo[0]) / cellsize_min)
-        self.num[1] = numpy.ceil((self.L[1] - self.origo[1]) / cellsize_min)
-        self.num[2] = numpy.ceil((self.L[2] - self.origo[2]) / cellsize_min)
-        if self.num[0] < 4 or self.num[1] < 4 or self.num[2] < 4:
-            raise Exception("Error: The grid must be at least 3 cells in each direction\n"
-                            "Grid: x={}, y={}, z={}".format(self.num[0], self.num[1], self.num[2]))
+        """Cell counts for the current world; a top wall is put at L[0] (sic)
+        (python/sphere.py:2655-2685)."""
+        self._size_grid(self._default_cell(dx), 4)
         if self.nw > 0:
             self.w_x[0] = self.L[0]
 
     def initGridAndWorldsize(self, margin=2.0):
-        """python/sphere.py:2687-2723"""
+        """World spanning the grains plus `margin` largest radii on every
+        side (python/sphere.py:2687-2723)."""
         r_max = numpy.amax(self.radius)
-        self.origo = numpy.array([numpy.amin(self.x[:, 0] - self.radius[:]),
-                                  numpy.amin(self.x[:, 1] - self.radius[:]),
-                                  numpy.amin(self.x[:, 2] - self.radius[:])]) - margin * r_max
-        self.L = numpy.array([numpy.amax(self.x[:, 0] + self.radius[:]),
-                              numpy.amax(self.x[:, 1] + self.radius[:]),
-                              numpy.amax(self.x[:, 2] + self.radius[:])]) + margin * r_max
-        cellsize_min = 2.1 * r_max
-        self.num[0] = numpy.ceil((self.L[0] - self.origo[0]) / cellsize_min)
-        self.num[1] = numpy.ceil((self.L[1] - self.origo[1]) / cellsize_min)
-        self.num[2] = numpy.ceil((self.L[2] - self.origo[2]) / cellsize_min)
-        if self.num[0] < 4 or self.num[1] < 4 or self.num[2] < 4:
-            raise Exception("Error: The grid must be at least 3 cells in each direction, num = "
-                            + str(self.num))
+        pad = margin * r_max
+        self.origo = numpy.amin(self.x - self.radius[:, None], axis=0) - pad
+        self.L = numpy.amax(self.x + self.radius[:, None], axis=0) + pad
+        self._size_grid(2.1 * r_max, 4)
         if self.nw > 0:
             self.w_x[0] = self.L[0]
 
     def initGridPos(self, gridnum=numpy.array([12, 12, 36])):
-        """python/sphere.py:2725-2785"""
-        self.num = numpy.asarray(gridnum).astype(numpy.uint32)
-        r_max = numpy.amax(self.radius)
-        cellsize = 2.1 * r_max
-        self.L = self.num * cellsize
-        if (self.num[0] * self.num[1] * self.num[2] - (2**3)) < self.np:
-            raise NameError('Error! The grid is not sufficiently large.')
-        gridpos = numpy.zeros(self.nd, dtype=numpy.uint32)
+        """One grain per cell of a regular grid of 2.1 largest radii, filled
+        x first (python/sphere.py:2725-2785).  With periodic x and y the grid
+        gives up its last column and row to the wrap, and odd layers are
+        shifted by half a cell in x and y."""
+        self.num = numpy.asarray(gridnum).astype(_U32)
+        cell = 2.1 * numpy.amax(self.radius)
+        self.L = self.num * cell
+        n = int(self.np[0])
+        used = self.num.astype(numpy.int64)
         if self.periodic[0] == 1:
-            self.num[0] -= 1
-            self.num[1] -= 1
-        if (self.num[0] * self.num[1] * self.num[2] - (2 * 3 * 3)) < self.np:
+            used[:2] -= 1
+        if numpy.prod(self.num.astype(numpy.int64)) - 8 < n or numpy.prod(used) - 18 < n:
             raise NameError('Error! The grid is not sufficiently large.')
-        for i in range(int(self.np[0])):
-            gridpos[0] = (i % (self.num[0]))
-            gridpos[1] = numpy.floor(i / (self.num[0])) % (self.num[0])
-            gridpos[2] = numpy.floor(i / ((self.num[0]) * (self.num[1])))
-            for d in range(int(self.nd[0])):
-                self.x[i, d] = gridpos[d] * cellsize + 0.5 * cellsize
-            if self.periodic[0] == 1:
-                if gridpos[2] % 2:
-                    self.x[i, 0] += 0.5 * cellsize
-                    self.x[i, 1] += 0.5 * cellsize
+        i = numpy.arange(n)
+        nx = int(used[0])
+        # (the row index wraps at the x count, as in the reference)
+        g = numpy.stack([i % nx, (i // nx) % nx, i // (nx * int(used[1]))], axis=1)
+        self.x[:n, :3] = g * cell + 0.5 * cell
         if self.periodic[0] == 1:
-            self.num[0] += 1
-            self.num[1] += 1
+            odd = (g[:, 2] % 2) == 1
+            self.x[:n][odd, :2] += 0.5 * cell
 
     def initRandomGridPos(self, gridnum=numpy.array([12, 12, 32]), padding=2.1):
-        """python/sphere.py:2787-2846"""
-        coarsegrid = numpy.floor(numpy.asarray(gridnum) / 2)
-        r_max = numpy.amax(self.radius)
-        cellsize = padding * r_max * 2
-        if ((coarsegrid[0] - 1) * (coarsegrid[1] - 1) * (coarsegrid[2] - 1)) < self.np:
+        """One grain per cell of a grid half as fine as gridnum, at a uniform
+        random position that keeps 1.05 radii from the cell faces; the world
+        then ends at the cell holding the outermost grain surface
+        (python/sphere.py:2787-2846)."""
+        coarse = numpy.floor(numpy.asarray(gridnum) / 2).astype(numpy.int64)
+        cell = padding * numpy.amax(self.radius) * 2
+        n = int(self.np[0])
+        if numpy.prod(coarse - 1) < n:
             raise NameError('Error! The grid is not sufficiently large.')
-        gridpos = numpy.zeros(self.nd, dtype=numpy.uint32)
-        for i in range(int(self.np[0])):
-            gridpos[0] = (i % (coarsegrid[0]))
-            gridpos[1] = numpy.floor(i / (coarsegrid[0])) % (coarsegrid[1])
-            gridpos[2] = numpy.floor(i / ((coarsegrid[0]) * (coarsegrid[1])))
-            for d in range(int(self.nd[0])):
-                r = self.radius[i] * 1.05
-                self.x[i, d] = gridpos[d] * cellsize + ((cellsize - r) - r) * numpy.random.random_sample() + r
-        x_max = numpy.max(self.x[:, 0] + self.radius)
-        y_max = numpy.max(self.x[:, 1] + self.radius)
-        z_max = numpy.max(self.x[:, 2] + self.radius)
-        self.num[0] = numpy.ceil(x_max / cellsize)
-        self.num[1] = numpy.ceil(y_max / cellsize)
-        self.num[2] = numpy.ceil(z_max / cellsize)
-        self.L = self.num * cellsize
+        i = numpy.arange(n)
+        g = numpy.stack([i % coarse[0], (i // coarse[0]) % coarse[1],
+                         i // (coarse[0] * coarse[1])], axis=1)
+        r = 1.05 * self.radius[:n, None]
+        u = numpy.random.random_sample((n, int(self.nd[0])))  # drawn grain by grain
+        self.x[:n] = g * cell + ((cell - r) - r) * u + r
+        top = numpy.max(self.x[:n] + self.radius[:n, None], axis=0)
+        self.num = numpy.ceil(top / cell).astype(_U32)
+        self.L = self.num * cell
+
+    # ------------------------------------------------------------------
+    # bonds (python/sphere.py:2848-2928, 4075-4119)
+    # ------------------------------------------------------------------
+    def bond(self, i, j):
+        """Add an unstrained bond between particles i and j."""
+        self.lambda_bar[0] = 1.0
+        nb = int(self.nb0[0])
+        old = (self.bonds, self.bonds_delta_n, self.bonds_delta_t, self.bonds_omega_n,
+               self.bonds_omega_t)
+        self._alloc_bonds(nb + 1)
+        for new, prev in zip((self.bonds, self.bonds_delta_n, self.bonds_delta_t,
+                              self.bonds_omega_n, self.bonds_omega_t), old):
+            new[:nb] = numpy.asarray(prev).reshape(new[:nb].shape)
+        self.bonds[nb] = (i, j)
+        self.nb0 = self.nb0 + 1
 
     def createBondPair(self, i, j, spacing=-0.1):
-        """python/sphere.py:2848-2903"""
-        x_i = self.x[i]
-        r_i = self.radius[i]
-        r_j = self.radius[j]
-        dist_ij = (r_i + r_j) * (1.0 + spacing)
-        dazi = numpy.random.rand(1) * 360.0
-        azi = numpy.radians(dazi)
-        dang = numpy.random.rand(1) * 180.0 - 90.0
-        ang = numpy.radians(dang)
-        x_j = numpy.copy(x_i)
-        x_j[0] = x_j[0] + dist_ij * numpy.cos(azi[0]) * numpy.cos(ang[0])
-        x_j[1] = x_j[1] + dist_ij * numpy.sin(azi[0]) * numpy.cos(ang[0])
-        x_j[2] = x_j[2] + dist_ij * numpy.sin(ang[0]) * numpy.cos(azi[0])
-        self.x[j] = x_j
-        if self.x[j, 0] < self.origo[0]:
-            self.x[j, 0] += x_i[0] - x_j[0]
-        if self.x[j, 1] < self.origo[1]:
-            self.x[j, 1] += x_i[1] - x_j[1]
-        if self.x[j, 2] < self.origo[2]:
-            self.x[j, 2] += x_i[2] - x_j[2]
-        if self.x[j, 0] > self.L[0]:
-            self.x[j, 0] -= abs(x_j[0] - x_i[0])
-        if self.x[j, 1] > self.L[1]:
-            self.x[j, 1] -= abs(x_j[1] - x_i[1])
-        if self.x[j, 2] > self.L[2]:
-            self.x[j, 2] -= abs(x_j[2] - x_i[2])
+        """Move particle j next to particle i, its centre (r_i + r_j)(1 +
+        spacing) away along a random direction, and bond the two
+        (python/sphere.py:2848-2903).  The direction uses the reference's
+        formula (its z component carries the cosine of the azimuth).  A
+        coordinate that would leave the world is pulled back towards i."""
+        dist = (self.radius[i] + self.radius[j]) * (1.0 + spacing)
+        azi = numpy.radians(numpy.random.rand(1) * 360.0)[0]
+        ele = numpy.radians(numpy.random.rand(1) * 180.0 - 90.0)[0]
+        x_i = self.x[i].copy()
+        step = numpy.array([dist * math.cos(azi) * math.cos(ele),
+                            dist * math.sin(azi) * math.cos(ele),
+                            dist * math.sin(ele) * math.cos(azi)])
+        x_j = x_i + step
+        new = x_j.copy()
+        below = x_j < self.origo[:3]
+        new[below] = x_i[below]
+        above = new > self.L[:3]
+        new[above] -= numpy.abs(x_j - x_i)[above]
+        self.x[j] = new
         self.bond(i, j)
-        x_ij = self.x[i] - self.x[j]
-        x_ij_length = numpy.sqrt(x_ij.dot(x_ij))
-        if (x_ij_length - dist_ij) > dist_ij * 0.01:
-            raise Exception("Error, something went wrong in createBondPair")
+        if numpy.linalg.norm(self.x[i] - self.x[j]) - dist > 0.01 * dist:
+            raise Exception("createBondPair: the pair ended up too far apart")
 
     def randomBondPairs(self, ratio=0.3, spacing=-0.1):
-        """python/sphere.py:2905-2928 (random_integers(0, np-1) is randint(0, np))"""
-        bondparticles = numpy.unique(numpy.random.randint(0, int(self.np[0]),
-                                                          size=int(self.np[0] * ratio)))
-        if bondparticles.size % 2 > 0:
-            bondparticles = bondparticles[:-1].copy()
-        bondparticles = bondparticles.reshape(int(bondparticles.size / 2), 2).copy()
-        for n in numpy.arange(bondparticles.shape[0]):
-            self.createBondPair(bondparticles[n, 0], bondparticles[n, 1], spacing)
-
-    def bond(self, i, j):
-        """python/sphere.py:4075-4119"""
-        self.lambda_bar[0] = 1.0
-        self.bonds = numpy.vstack((numpy.asarray(self.bonds).reshape(-1, 2),
-                                   [i, j])).astype(numpy.uint32)
-        self.bonds_delta_n = numpy.append(self.bonds_delta_n, [0.0])
-        self.bonds_delta_t = numpy.vstack((numpy.asarray(self.bonds_delta_t).reshape(-1, 3),
-                                           [0.0, 0.0, 0.0]))
-        self.bonds_omega_n = numpy.append(self.bonds_omega_n, [0.0])
-        self.bonds_omega_t = numpy.vstack((numpy.asarray(self.bonds_omega_t).reshape(-1, 3),
-                                           [0.0, 0.0, 0.0]))
-        self.nb0 += 1
-
-    def zeroKinematics(self):
-        """python/sphere.py:2930-2947"""
+        """Bond disjoint random pairs drawn from about ratio * np distinct
+        particles (python/sphere.py:2905-2928)."""
         n = int(self.np[0])
-        d = int(self.nd[0])
-        self.force = numpy.zeros((n, d))
-        self.torque = numpy.zeros((n, d))
-        self.vel = numpy.zeros((n, d), dtype=numpy.float64)
-        self.angvel = numpy.zeros((n, d), dtype=numpy.float64)
-        self.angpos = numpy.zeros((n, d), dtype=numpy.float64)
-        self.es = numpy.zeros(n, dtype=numpy.float64)
-        self.ev = numpy.zeros(n, dtype=numpy.float64)
-        self.xyzsum = numpy.zeros((n, 3), dtype=numpy.float64)
+        picks = numpy.unique(numpy.random.randint(0, n, size=int(n * ratio)))
+        picks = picks[:picks.size - picks.size % 2].reshape(-1, 2)
+        for a, b in picks:
+            self.createBondPair(a, b, spacing)
 
-    def adjustUpperWall(self, z_adjust=1.1):
-        """python/sphere.py:2949-2970"""
-        self.nw = numpy.ones(1, dtype=numpy.uint32)
-        self.wmode = numpy.zeros(1, dtype=numpy.int32)
-        self.w_n = numpy.zeros((1, int(self.nd[0])), dtype=numpy.float64)
-        self.w_n[0, 2] = -1.0
-        self.w_vel = numpy.zeros(1)
-        self.w_force = numpy.zeros(1)
-        self.w_sigma0 = numpy.zeros(1)
-        self.w_x = numpy.zeros(1)
-        self.w_m = numpy.zeros(1)
-        self.adjustWall(idx=0, adjust=z_adjust)
+    # ------------------------------------------------------------------
+    # walls and loading (python/sphere.py:2949-3165)
+    # ------------------------------------------------------------------
+    _WALL_AXIS = {0: 2, 1: 0, 2: 0, 3: 1, 4: 1}
 
     def adjustWall(self, idx, adjust=1.1, wall_mass_factor=1.0):
-        """python/sphere.py:2972-3017"""
-        if idx == 0:
-            dim = 2
-        elif idx == 1 or idx == 2:
-            dim = 0
-        elif idx == 3 or idx == 4:
-            dim = 1
-        else:
+        """Fit wall idx (0 top, 1/2 -x/+x, 3/4 -y/+y) to the grains: the
+        world along its axis ends at lo + adjust (hi - lo), lo and hi being
+        the outermost grain surfaces; walls 0, 1 and 3 sit at hi, 2 and 4 at
+        lo; the wall mass is the total grain mass (python/sphere.py
+        :2972-3017; wall_mass_factor is accepted and, as there, not used)."""
+        if idx not in self._WALL_AXIS:
             raise Exception("adjustWall: idx value not understood")
-        xmin = numpy.min(self.x[:, dim] - self.radius)
-        xmax = numpy.max(self.x[:, dim] + self.radius)
-        cellsize = self.L[0] / self.num[0]
-        self.num[dim] = numpy.ceil(((xmax - xmin) * adjust + xmin) / cellsize)
-        self.L[dim] = (xmax - xmin) * adjust + xmin
-        if idx == 0 or idx == 1 or idx == 3:
-            self.w_x[idx] = xmax
-        else:
-            self.w_x[idx] = xmin
+        axis = self._WALL_AXIS[idx]
+        lo = numpy.min(self.x[:, axis] - self.radius)
+        hi = numpy.max(self.x[:, axis] + self.radius)
+        cell = self.L[0] / self.num[0]
+        end = (hi - lo) * adjust + lo
+        self.num[axis] = numpy.ceil(end / cell)
+        self.L[axis] = end
+        self.w_x[idx] = hi if idx in (0, 1, 3) else lo
         self.w_m[idx] = self.totalMass()
 
+    def adjustUpperWall(self, z_adjust=1.1):
+        """One wall, the top one (fixed), fitted with adjustWall."""
+        self._alloc_walls(1)
+        self.w_x[0] = 0.0
+        self.adjustWall(idx=0, adjust=z_adjust)
+
     def consolidate(self, normal_stress=10e3):
-        """python/sphere.py:3019-3048"""
+        """Top wall under a constant normal stress (wmode 1)
+        (python/sphere.py:3019-3048)."""
         self.nw[0] = 1
         if normal_stress <= 0.0:
             raise Exception('consolidate() error: The normal stress should be a positive '
                             'value, but is ' + str(normal_stress) + ' Pa')
-        self.zeroKinematics()
-        self.adjustUpperWall()
-        self.wmode = numpy.array([1], dtype=numpy.int32)
-        self.w_sigma0 = numpy.ones(1) * normal_stress
-        self.w_m[0] = self.totalMass()
+        self._loaded_top_wall(1, sigma0=normal_stress)
 
     def uniaxialStrainRate(self, wvel=-0.001):
-        """python/sphere.py:3050-3066"""
+        """Top wall moving at wvel (wmode 2) (python/sphere.py:3050-3066)."""
+        self._loaded_top_wall(2, vel=wvel)
+
+    def _loaded_top_wall(self, mode, sigma0=0.0, vel=0.0):
         self.zeroKinematics()
         self.adjustUpperWall()
-        self.wmode = numpy.array([2], dtype=numpy.int32)
-        self.w_vel = numpy.array([wvel])
+        self.wmode = numpy.array([mode], dtype=_I32)
+        self.w_sigma0 = numpy.array([sigma0])
+        self.w_vel = numpy.array([vel])
 
     def triaxial(self, wvel=-0.001, normal_stress=10.0e3):
-        """python/sphere.py:3068-3095"""
+        """Five walls: the top one moving at wvel (wmode 2), the four side
+        walls under normal_stress (wmode 1), each fitted with adjustWall
+        (python/sphere.py:3068-3095)."""
         self.zeroKinematics()
-        self.nw[0] = 5
-        self.wmode = numpy.array([2, 1, 1, 1, 1], dtype=numpy.int32)
-        self.w_vel = numpy.array([1, 0, 0, 0, 0]) * wvel
-        self.w_sigma0 = numpy.array([0, 1, 1, 1, 1]) * normal_stress
-        self.w_n = numpy.array(([0, 0, -1], [-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0]),
-                               dtype=numpy.float64)
-        self.w_x = numpy.zeros(5)
-        self.w_m = numpy.zeros(5)
-        self.w_force = numpy.zeros(5)
-        for i in range(5):
-            self.adjustWall(idx=i)
+        self._alloc_walls(5)
+        top = numpy.arange(5) == 0
+        self.wmode[:] = numpy.where(top, 2, 1)
+        # products, so that the side walls' velocities carry the sign of
+        # zero that 0 * wvel has (it is written to the file)
+        self.w_vel[:] = top * wvel
+        self.w_sigma0[:] = ~top * normal_stress
+        self.w_x[:] = 0.0
+        for idx in range(5):
+            self.adjustWall(idx=idx)
+
+    def _fix_layer(self, members, vx):
+        """Drive grains at a prescribed velocity (fixvel = 1): x velocity vx,
+        y velocity 0, no rotation; they are coloured -1."""
+        self.fixvel[members] = 1.0
+        self.angvel[members, :] = 0.0
+        self.vel[members, 0] = vx
+        self.vel[members, 1] = 0.0
+        self.color[members] = -1
 
     def shear(self, shear_strain_rate=1.0, shear_stress=False):
-        """python/sphere.py:3097-3165"""
+        """Simple shear of a consolidated packing (python/sphere.py
+        :3097-3165).  The grid grows by one cell layer; kinematics are zeroed;
+        the bottom grains (centre below z_min + one largest diameter of the
+        grains whose centre lies below 0.3 h) are held still, and the top
+        grains (centre above z_max - one largest diameter of those above
+        0.7 h) move at h * shear_strain_rate in x, h = z_max - z_min being the
+        packing height.  With shear_stress, the top grains are instead driven
+        by that shear stress (wmode 3).  Wall friction is switched off."""
         self.nw[0] = 1
-        z_min = numpy.min(self.x[:, 2] - self.radius)
-        z_max = numpy.max(self.x[:, 2] + self.radius)
-        cellsize = self.L[0] / self.num[0]
+        z_lo = numpy.min(self.x[:, 2] - self.radius)
+        z_hi = numpy.max(self.x[:, 2] + self.radius)
+        h = z_hi - z_lo
+        cell = self.L[0] / self.num[0]
         self.num[2] += 1
-        self.L[2] = self.num[2] * cellsize
+        self.L[2] = self.num[2] * cell
         self.zeroKinematics()
-        self.wmode = numpy.array([1], dtype=numpy.int32)
-        d_max_below = numpy.max(self.radius[numpy.nonzero(self.x[:, 2] < (z_max - z_min) * 0.3)]) * 2.0
-        I = numpy.nonzero(self.x[:, 2] < (z_min + d_max_below))
-        self.fixvel[I] = 1
-        self.angvel[I, 0] = 0.0
-        self.angvel[I, 1] = 0.0
-        self.angvel[I, 2] = 0.0
-        self.vel[I, 0] = 0.0
-        self.vel[I, 1] = 0.0
-        self.color[I] = -1
-        d_max_top = numpy.max(self.radius[numpy.nonzero(self.x[:, 2] > (z_max - z_min) * 0.7)]) * 2.0
-        I = numpy.nonzero(self.x[:, 2] > (z_max - d_max_top))
-        self.fixvel[I] = 1
-        self.angvel[I, 0] = 0.0
-        self.angvel[I, 1] = 0.0
-        self.angvel[I, 2] = 0.0
+        self.wmode = numpy.array([1], dtype=_I32)
+        z = self.x[:, 2]
+        # (the 0.3 h and 0.7 h bounds are heights above z = 0)
+        d_bottom = 2.0 * numpy.max(self.radius[z < 0.3 * h])
+        self._fix_layer(numpy.nonzero(z < z_lo + d_bottom), 0.0)
+        d_top = 2.0 * numpy.max(self.radius[z > 0.7 * h])
+        top = numpy.nonzero(z > z_hi - d_top)
         if shear_stress is False:
-            self.vel[I, 0] = (z_max - z_min) * shear_strain_rate
+            self._fix_layer(top, h * shear_strain_rate)
         else:
-            self.vel[I, 0] = 0.0
+            self._fix_layer(top, 0.0)
             self.wmode[0] = 3
             self.w_tau_x[0] = float(shear_stress)
-        self.vel[I, 1] = 0.0
-        self.color[I] = -1
         self.gamma_wt[0] = 0.0
         self.mu_ws[0] = 0.0
         self.mu_wd[0] = 0.0
 
-    def largestFluidTimeStep(self, safety=0.5, v_max=-1.0):
-        """python/sphere.py:3167-3252 (Darcy: the CFL bound)"""
-        if self.fluid:
-            v_norm = numpy.sqrt(numpy.sum(self.v_f**2, axis=3)).ravel()
-            v_max_obs = numpy.amax(v_norm) if v_norm.size else 0.0
-            if v_max_obs == 0:
-                v_max_obs = 1.0e-7
-            if v_max < 0.0:
-                v_max = v_max_obs
-            dx_min = numpy.min(self.L / self.num)
-            dt_min_cfl = dx_min / v_max
-            if self.cfd_solver[0] == 0:
-                dt_min_von_neumann = 0.5 * dx_min**2 / (self.mu[0] + 1.0e-16)
-                return numpy.min([dt_min_von_neumann, dt_min_cfl]) * safety
-            elif self.cfd_solver[0] == 1:
-                self.cellSize()
-                return dt_min_cfl
-
+    # ------------------------------------------------------------------
+    # time (python/sphere.py:3167-3376)
+    # ------------------------------------------------------------------
     def cellSize(self):
         self.dx = self.L / self.num
 
+    def largestFluidTimeStep(self, safety=0.5, v_max=-1.0):
+        """Largest stable fluid time step for the current fluid velocities:
+        the CFL bound dx_min / v_max (Darcy), or safety times the smaller of
+        it and the von Neumann bound 0.5 dx_min^2 / mu (Navier-Stokes).  An
+        all-zero velocity field counts as 1e-7 m/s."""
+        if not self.fluid:
+            return None
+        speeds = numpy.sqrt(numpy.sum(numpy.asarray(self.v_f) ** 2, axis=-1))
+        observed = float(numpy.max(speeds)) if speeds.size else 0.0
+        if observed == 0.0:
+            observed = 1.0e-7
+        v = observed if v_max < 0.0 else v_max
+        dx_min = numpy.min(self.L / self.num)
+        cfl = dx_min / v
+        if int(self.cfd_solver[0]) == 0:
+            return min(0.5 * dx_min ** 2 / (self.mu[0] + 1.0e-16), cfl) * safety
+        self.cellSize()
+        return cfl
+
     def initTemporal(self, total, current=0.0, file_dt=0.05, step_count=0, dt=-1, epsilon=0.01):
-        """python/sphere.py:3307-3376"""
+        """Time step and run length (python/sphere.py:3307-3376).  Without an
+        explicit dt the step is epsilon / sqrt(k_max / m_min): m_min the mass
+        of the smallest grain, k_max = max(pi/2 E r) when E > 0.001, else the
+        larger of k_n and k_t; with fluid, the fluid's CFL step if smaller.
+        The output counter restarts at 0 (step_count is not used)."""
+        n = int(self.np[0])
         if dt > 0.0:
             self.time_dt[0] = dt
-            if self.np[0] > 0:
+            if n > 0:
                 print("Warning: Manually specifying the time step length when simulating "
                       "particles may produce instabilities.")
-        elif self.np[0] > 0:
-            r_min = numpy.min(self.radius)
-            m_min = self.rho * 4.0 / 3.0 * numpy.pi * r_min**3
-            if self.E > 0.001:
-                k_max = numpy.max(numpy.pi / 2.0 * self.E * self.radius)
+        elif n > 0:
+            m_min = self.rho[0] * V_sphere(numpy.min(self.radius))
+            if self.E[0] > 0.001:
+                k_max = numpy.max(numpy.pi / 2.0 * self.E[0] * self.radius)
             else:
-                k_max = numpy.max([self.k_n[:], self.k_t[:]])
-            self.time_dt[0] = (epsilon / (numpy.sqrt(k_max / m_min))).ravel()[0]
-        elif self.fluid is False:
+                k_max = max(self.k_n[0], self.k_t[0])
+            self.time_dt[0] = epsilon / math.sqrt(k_max / m_min)
+        elif not self.fluid:
             raise Exception('Error: Could not automatically set a time step.')
         if self.fluid:
-            fluid_time_dt = self.largestFluidTimeStep()
-            self.time_dt[0] = numpy.min([fluid_time_dt, self.time_dt[0]])
+            self.time_dt[0] = min(self.largestFluidTimeStep(), self.time_dt[0])
         self.time_current[0] = current
         self.time_total[0] = total
         self.time_file_dt[0] = file_dt
         self.time_step_count[0] = 0
 
+    def currentTime(self, value=-1):
+        if value == -1:
+            return self.time_current[0]
+        self.time_current[0] = value
+
+    # ------------------------------------------------------------------
+    # fluid (python/sphere.py:3378-3760)
+    # ------------------------------------------------------------------
     def dry(self):
         self.fluid = False
 
@@ -1083,136 +1100,109 @@ class sim:
         self.initFluid()
 
     def initFluid(self, mu=8.9e-4, rho=1.0e3, p=0.0, hydrostatic=False, cfd_solver=0):
-        """python/sphere.py:3395-3508"""
+        """Fluid on the current grid (python/sphere.py:3395-3508): pressure p
+        (hydrostatic below the top wall or the world top when asked),
+        porosity 1, the given viscosity and density, and the defaults of the
+        chosen solver (0 Navier-Stokes, 1 Darcy with periodic lateral
+        boundaries)."""
         self.fluid = True
-        self.mu = numpy.ones(1, dtype=numpy.float64) * mu
-        self.rho_f = numpy.ones(1, dtype=numpy.float64) * rho
-        self._alloc_fluid(p)
-        self.phi = numpy.ones(self.p_f.shape, dtype=numpy.float64)
+        self.mu = _one(_F64, mu)
+        self._fluid_fields(p=p, phi=1.0)
+        self.rho_f = _one(_F64, rho)
         if hydrostatic:
-            dz = self.L[2] / self.num[2]
-            if self.nw == 1:
-                wall0_iz = int(self.w_x[0] / (self.L[2] / self.num[2]))
-                self.p_f[:, :, wall0_iz:] = p
-                for iz in numpy.arange(wall0_iz - 1):
-                    z = dz * iz + 0.5 * dz
-                    depth = self.w_x[0] - z
-                    self.p_f[:, :, iz] = p + (depth - dz) * rho * -self.g[2]
-            else:
-                for iz in numpy.arange(self.num[2] - 1):
-                    z = dz * iz + 0.5 * dz
-                    depth = self.L[2] - z
-                    self.p_f[:, :, iz] = p + (depth - dz) * rho * -self.g[2]
-        self.cfd_solver = numpy.ones(1) * cfd_solver
-        n = int(self.np[0])
-        d = int(self.nd[0])
-        if self.cfd_solver[0] == 0:
-            self.gamma = numpy.array(0.0)
-            self.theta = numpy.array(1.0)
-            self.beta = numpy.array(0.0)
-            self.tolerance = numpy.array(1.0e-3)
-            self.maxiter = numpy.array(1e4)
-            self.ndem = numpy.array(1)
-            self.c_phi = numpy.ones(1, dtype=numpy.float64)
-            self.c_v = numpy.ones(1, dtype=numpy.float64)
-            self.dt_dem_fac = numpy.ones(1, dtype=numpy.float64)
-            self.f_d = numpy.zeros((n, d), dtype=numpy.float64)
-            self.f_p = numpy.zeros((n, d), dtype=numpy.float64)
-            self.f_v = numpy.zeros((n, d), dtype=numpy.float64)
-            self.f_sum = numpy.zeros((n, d), dtype=numpy.float64)
-        elif self.cfd_solver[0] == 1:
-            self.tolerance = numpy.array(1.0e-3)
-            self.maxiter = numpy.array(1e4)
-            self.ndem = numpy.array(1)
-            self.c_phi = numpy.ones(1, dtype=numpy.float64)
-            self.f_d = numpy.zeros((n, d), dtype=numpy.float64)
-            self.beta_f = numpy.ones(1, dtype=numpy.float64) * 4.5e-10
-            self.f_p = numpy.zeros((n, d), dtype=numpy.float64)
-            self.k_c = numpy.ones(1, dtype=numpy.float64) * 4.6e-10
-            self.bc_xn = numpy.ones(1, dtype=numpy.int32) * 2
-            self.bc_xp = numpy.ones(1, dtype=numpy.int32) * 2
-            self.bc_yn = numpy.ones(1, dtype=numpy.int32) * 2
-            self.bc_yp = numpy.ones(1, dtype=numpy.int32) * 2
-        else:
-            raise Exception('Value of cfd_solver not understood (' + str(self.cfd_solver[0]) + ')')
+            self._hydrostatic(p, rho)
+        self._solver_fields(int(cfd_solver), lateral=True)
 
-    def currentTime(self, value=-1):
-        if value != -1:
-            self.time_current[0] = value
+    def _hydrostatic(self, p, rho):
+        """Pressure p + (depth - dz) rho |g_z| in layer iz at depth below the
+        top wall (one wall) or the world top, for the layers 0 .. top - 2;
+        p from the wall's layer up."""
+        dz = self.L[2] / self.num[2]
+        if self.nw == 1:
+            top_iz = int(self.w_x[0] / dz)
+            surface = self.w_x[0]
+            self.p_f[:, :, top_iz:] = p
         else:
-            return self.time_current[0]
+            top_iz = int(self.num[2])
+            surface = self.L[2]
+        for iz in range(max(top_iz - 1, 0)):
+            depth = surface - (dz * iz + 0.5 * dz)
+            self.p_f[:, :, iz] = p + (depth - dz) * rho * -self.g[2]
 
-    # fluid boundary conditions, python/sphere.py:3527-3686
+    # fluid boundary conditions, python/sphere.py:3527-3686: bc 0 fixed
+    # pressure, 1 no flow, 2 no flow and no slip (bottom/top) or periodic
+    # (sides), 4 fixed flux
+    def _bc(self, names, code, flux=None, flux_name=None):
+        for nm in names:
+            getattr(self, nm)[0] = code
+        if flux_name is not None:
+            getattr(self, flux_name)[0] = flux
+
     def setFluidBottomNoFlow(self):
-        self.bc_bot[0] = 1
+        self._bc(("bc_bot",), 1)
 
     def setFluidBottomNoFlowNoSlip(self):
-        self.bc_bot[0] = 2
+        self._bc(("bc_bot",), 2)
 
     def setFluidBottomFixedPressure(self):
-        self.bc_bot[0] = 0
+        self._bc(("bc_bot",), 0)
 
     def setFluidBottomFixedFlux(self, specific_flux):
-        self.bc_bot[0] = 4
-        self.bc_bot_flux[0] = specific_flux
+        self._bc(("bc_bot",), 4, specific_flux, "bc_bot_flux")
 
     def setFluidTopNoFlow(self):
-        self.bc_top[0] = 1
+        self._bc(("bc_top",), 1)
 
     def setFluidTopNoFlowNoSlip(self):
-        self.bc_top[0] = 2
+        self._bc(("bc_top",), 2)
 
     def setFluidTopFixedPressure(self):
-        self.bc_top[0] = 0
+        self._bc(("bc_top",), 0)
 
     def setFluidTopFixedFlux(self, specific_flux):
-        self.bc_top[0] = 4
-        self.bc_top_flux[0] = specific_flux
+        self._bc(("bc_top",), 4, specific_flux, "bc_top_flux")
 
     def setFluidXFixedPressure(self):
-        self.bc_xn[0] = 0
-        self.bc_xp[0] = 0
+        self._bc(("bc_xn", "bc_xp"), 0)
 
     def setFluidXNoFlow(self):
-        self.bc_xn[0] = 1
-        self.bc_xp[0] = 1
+        self._bc(("bc_xn", "bc_xp"), 1)
 
     def setFluidXPeriodic(self):
-        self.bc_xn[0] = 2
-        self.bc_xp[0] = 2
+        self._bc(("bc_xn", "bc_xp"), 2)
 
     def setFluidYFixedPressure(self):
-        self.bc_yn[0] = 0
-        self.bc_yp[0] = 0
+        self._bc(("bc_yn", "bc_yp"), 0)
 
     def setFluidYNoFlow(self):
-        self.bc_yn[0] = 1
-        self.bc_yp[0] = 1
+        self._bc(("bc_yn", "bc_yp"), 1)
 
     def setFluidYPeriodic(self):
-        self.bc_yn[0] = 2
-        self.bc_yp[0] = 2
+        self._bc(("bc_yn", "bc_yp"), 2)
+
+    def _darcy_only(self, what):
+        if int(numpy.asarray(self.cfd_solver).ravel()[0]) != 1:
+            raise Exception(what + '() only relevant for the Darcy solver (cfd_solver = 1)')
 
     def setPermeabilityGrainSize(self, verbose=True):
-        self.setPermeabilityPrefactor(k_c=numpy.mean(self.radius * 2.0)**2.0 / 180.0,
+        """k_c = d_mean^2 / 180 (Kozeny-Carman with the mean diameter)."""
+        self.setPermeabilityPrefactor(k_c=numpy.mean(2.0 * self.radius) ** 2 / 180.0,
                                       verbose=verbose)
 
     def setPermeabilityPrefactor(self, k_c, verbose=True):
-        if self.cfd_solver[0] == 1:
-            self.k_c[0] = k_c
-        else:
-            raise Exception('setPermeabilityPrefactor() only relevant for the Darcy solver '
-                            '(cfd_solver = 1)')
+        self._darcy_only('setPermeabilityPrefactor')
+        self.k_c[0] = k_c
 
     def findPermeabilities(self):
-        """python/sphere.py:3730-3744 (the Python helper's phi^3/(1-phi^2))"""
-        if self.cfd_solver[0] == 1:
-            phi = numpy.clip(self.phi, 0.1, 0.9)
-            self.k = self.k_c * phi**3 / (1.0 - phi**2)
-        else:
-            raise Exception('findPermeabilities() only relevant for the Darcy solver')
+        """Permeability of every cell from its porosity with the Python
+        helper's form k_c phi^3 / (1 - phi^2), phi clipped to [0.1, 0.9]
+        (python/sphere.py:3730-3744; the solver itself uses phi^3/(1-phi)^2)."""
+        self._darcy_only('findPermeabilities')
+        phi = numpy.clip(self.phi, 0.1, 0.9)
+        self.k = self.k_c * phi ** 3 / (1.0 - phi ** 2)
 
     def setFluidPressureModulation(self, A, f, phi=0.0, plot=False):
+        """Top pressure p_top + A sin(2 pi f t + phi) (Darcy)."""
         self.p_mod_A[0] = A
         self.p_mod_f[0] = f
         self.p_mod_phi[0] = phi
@@ -1221,6 +1211,7 @@ class sim:
         self.setFluidPressureModulation(A=0.0, f=0.0)
 
     def setTopWallNormalStressModulation(self, A, f, plot=False):
+        """Top wall stress sigma0 + A sin(2 pi f t)."""
         self.w_sigma0_A[0] = A
         self.w_sigma0_f[0] = f
 
@@ -1236,32 +1227,39 @@ class sim:
     def setDEMstepsPerCFDstep(self, ndem):
         self.ndem = numpy.asarray(ndem)
 
-    # material parameters, python/sphere.py:3762-4054
+    def setFluidCompressibility(self, beta_f):
+        self._darcy_only('setFluidCompressibility')
+        self.beta_f[0] = beta_f
+
+    def setFluidViscosity(self, mu):
+        self.mu[0] = mu
+
+    def setFluidDensity(self, rho_f):
+        self.rho_f[0] = rho_f
+
+    # ------------------------------------------------------------------
+    # material parameters (python/sphere.py:3762-4054)
+    # ------------------------------------------------------------------
     def defaultParams(self, mu_s=0.5, mu_d=0.5, mu_r=0.0, rho=2600, k_n=1.16e9, k_t=1.16e9,
                       k_r=0, gamma_n=0.0, gamma_t=0.0, gamma_r=0.0, gamma_wn=1.0e4,
                       gamma_wt=1.0e4, capillaryCohesion=0):
-        self.rho = numpy.ones(1, dtype=numpy.float64) * rho
-        self.k_n = numpy.ones(1, dtype=numpy.float64) * k_n
-        self.k_t = numpy.ones(1, dtype=numpy.float64) * k_t
-        self.k_r = numpy.ones(1, dtype=numpy.float64) * k_r
-        self.gamma_n = numpy.ones(1, dtype=numpy.float64) * gamma_n
-        self.gamma_t = numpy.ones(1, dtype=numpy.float64) * gamma_t
-        self.gamma_r = numpy.ones(1, dtype=numpy.float64) * gamma_r
-        self.mu_s = numpy.ones(1, dtype=numpy.float64) * mu_s
-        self.mu_d = numpy.ones(1, dtype=numpy.float64) * mu_d
-        self.mu_r = numpy.ones(1, dtype=numpy.float64) * mu_r
+        """Set the contact parameters (python/sphere.py:3762-3839): the
+        arguments as given, wall friction equal to the grain friction, and
+        with capillaryCohesion = 1 a liquid bridge of 1e-12 m^3 with
+        kappa = 2 pi gamma_t (contact angle 0) and rupture distance
+        V_b^(1/3)."""
+        values = dict(rho=rho, k_n=k_n, k_t=k_t, k_r=k_r, gamma_n=gamma_n, gamma_t=gamma_t,
+                      gamma_r=gamma_r, mu_s=mu_s, mu_d=mu_d, mu_r=mu_r)
+        for name, v in values.items():
+            setattr(self, name, _one(_F64, v))
         self.gamma_wn[0] = gamma_wn
         self.gamma_wt[0] = gamma_wt
         self.mu_ws = self.mu_s
         self.mu_wd = self.mu_d
-        theta = 0.0
-        if capillaryCohesion == 1:
-            self.kappa[0] = 2.0 * math.pi * gamma_t * numpy.cos(theta)
-            self.V_b[0] = 1e-12
-        else:
-            self.kappa[0] = 0.0
-            self.V_b[0] = 0.0
-        self.db[0] = (1.0 + theta / 2.0) * self.V_b[0]**(1.0 / 3.0)
+        cohesive = capillaryCohesion == 1
+        self.kappa[0] = 2.0 * math.pi * gamma_t if cohesive else 0.0
+        self.V_b[0] = 1e-12 if cohesive else 0.0
+        self.db[0] = self.V_b[0] ** (1.0 / 3.0)
 
     def setStiffnessNormal(self, k_n):
         self.k_n[0] = k_n
@@ -1270,7 +1268,8 @@ class sim:
         self.k_t[0] = k_t
 
     def setYoungsModulus(self, E):
-        """python/sphere.py:3895-3908"""
+        """Young's modulus; above 0.001 it sets the contact stiffness
+        (python/sphere.py:3895-3908)."""
         self.E[0] = E
 
     def setPoissonsRatio(self, nu):
@@ -1279,20 +1278,21 @@ class sim:
         the reference reads nowhere (python/sphere.py:169-171)."""
         self.k_r[0] = nu
 
+    def _damping_check(self, gamma, k, component, over_damping):
+        ratio = gamma / (2.0 * numpy.sqrt(self.smallestMass() * k))
+        if ratio > 1.0 and not over_damping:
+            raise Exception('Warning: The system is over-dampened (ratio = ' + str(ratio)
+                            + ') in the ' + component + ' component.')
+
     def setDampingNormal(self, gamma, over_damping=False):
+        """Normal damping; above the critical 2 sqrt(m k_n) raises unless
+        over_damping (the reference's smallestMass is a volume)."""
         self.gamma_n[0] = gamma
-        critical_gamma = 2.0 * numpy.sqrt(self.smallestMass() * self.k_n[0])
-        damping_ratio = gamma / critical_gamma
-        if damping_ratio > 1.0 and not over_damping:
-            raise Exception('Warning: The system is over-dampened (ratio = ' + str(damping_ratio)
-                            + ') in the normal component.')
+        self._damping_check(gamma, self.k_n[0], 'normal', over_damping)
 
     def setDampingTangential(self, gamma, over_damping=False):
         self.gamma_t[0] = gamma
-        damping_ratio = gamma / (2.0 * numpy.sqrt(self.smallestMass() * self.k_t[0]))
-        if damping_ratio > 1.0 and not over_damping:
-            raise Exception('Warning: The system is over-dampened (ratio = ' + str(damping_ratio)
-                            + ') in the tangential component.')
+        self._damping_check(gamma, self.k_t[0], 'tangential', over_damping)
 
     def setStaticFriction(self, mu_s):
         self.mu_s[0] = mu_s
@@ -1300,36 +1300,21 @@ class sim:
     def setDynamicFriction(self, mu_d):
         self.mu_d[0] = mu_d
 
-    def setFluidCompressibility(self, beta_f):
-        if self.cfd_solver[0] == 1:
-            self.beta_f[0] = beta_f
-        else:
-            raise Exception('setFluidCompressibility() only relevant for the Darcy solver')
-
-    def setFluidViscosity(self, mu):
-        self.mu[0] = mu
-
-    def setFluidDensity(self, rho_f):
-        self.rho_f[0] = rho_f
-
     def scaleSize(self, factor):
-        self.L *= factor
-        self.x *= factor
-        self.radius *= factor
-        self.xyzsum *= factor
-        self.vel *= factor
-        self.force *= factor
-        self.torque *= factor
-        self.w_x *= factor
-        self.w_m *= factor
-        self.w_vel *= factor
-        self.w_force *= factor
+        """Scale lengths, velocities, forces and wall quantities by factor."""
+        for name in ("L", "x", "radius", "xyzsum", "vel", "force", "torque", "w_x", "w_m",
+                     "w_vel", "w_force"):
+            setattr(self, name, getattr(self, name) * factor)
 
     # ------------------------------------------------------------------
-    # energies and masses, python/sphere.py:4142-4380
+    # masses, energies and bulk measures (python/sphere.py:4142-4437,
+    # 4777-4855)
     # ------------------------------------------------------------------
+    def _masses(self):
+        return self.rho[0] * V_sphere(self.radius)
+
     def surfaceArea(self, idx):
-        return 4.0 * numpy.pi * self.radius[idx]**2
+        return 4.0 * numpy.pi * self.radius[idx] ** 2
 
     def volume(self, idx):
         return V_sphere(self.radius[idx])
@@ -1338,30 +1323,31 @@ class sim:
         return self.rho[0] * self.volume(idx)
 
     def totalMass(self):
-        return float(numpy.sum(self.rho[0] * V_sphere(self.radius)))
+        return float(numpy.sum(self._masses()))
 
     def smallestMass(self):
-        # the reference returns a volume here (python/sphere.py:4186-4195)
+        """The volume of the smallest grain (the reference returns a
+        volume here, python/sphere.py:4186-4195)."""
         return V_sphere(numpy.min(self.radius))
 
     def largestMass(self):
         return V_sphere(numpy.max(self.radius))
 
     def momentOfInertia(self, idx):
-        return 2.0 / 5.0 * self.mass(idx) * self.radius[idx]**2
+        return 0.4 * self.mass(idx) * self.radius[idx] ** 2
 
     def kineticEnergy(self, idx):
-        return 0.5 * self.mass(idx) * numpy.sqrt(numpy.dot(self.vel[idx, :], self.vel[idx, :]))**2
+        return 0.5 * self.mass(idx) * float(numpy.dot(self.vel[idx], self.vel[idx]))
 
     def totalKineticEnergy(self):
-        return sum(self.kineticEnergy(i) for i in range(int(self.np[0])))
+        return self.energy('kin')
 
     def rotationalEnergy(self, idx):
-        return 0.5 * self.momentOfInertia(idx) * numpy.sqrt(
-            numpy.dot(self.angvel[idx, :], self.angvel[idx, :]))**2
+        return 0.5 * self.momentOfInertia(idx) * float(numpy.dot(self.angvel[idx],
+                                                                 self.angvel[idx]))
 
     def totalRotationalEnergy(self):
-        return sum(self.rotationalEnergy(i) for i in range(int(self.np[0])))
+        return self.energy('rot')
 
     def viscousEnergy(self, idx):
         return self.ev[idx]
@@ -1375,40 +1361,98 @@ class sim:
     def totalFrictionalEnergy(self):
         return float(numpy.sum(self.es))
 
-    def energy(self, method):
-        """python/sphere.py:4311-4380"""
-        if method == 'pot':
-            m = numpy.ones(self.np) * 4.0 / 3.0 * math.pi * self.radius**3 * self.rho
-            return numpy.sum(m * math.sqrt(numpy.dot(self.g, self.g)) * self.x[:, 2])
-        elif method == 'kin':
-            m = numpy.ones(self.np) * 4.0 / 3.0 * math.pi * self.radius**3 * self.rho
-            return float(numpy.sum(0.5 * m * numpy.sum(self.vel**2, axis=1)))
-        elif method == 'rot':
-            m = numpy.ones(self.np) * 4.0 / 3.0 * math.pi * self.radius**3 * self.rho
-            return float(numpy.sum(0.5 * 2.0 / 5.0 * m * self.radius**2
-                                   * numpy.sum(self.angvel**2, axis=1)))
-        elif method == 'shear':
-            return numpy.sum(self.es)
-        elif method == 'shearrate':
-            return numpy.sum(self.es_dot)
-        elif method == 'visc_n':
-            return numpy.sum(self.ev)
-        elif method == 'visc_n_rate':
-            return numpy.sum(self.ev_dot)
-        elif method == 'bondpot':
-            if self.nb0 > 0:
-                R_bar = self.lambda_bar * numpy.minimum(self.radius[self.bonds[:, 0]],
-                                                        self.radius[self.bonds[:, 1]])
-                A = numpy.pi * R_bar**2
-                I = 0.25 * numpy.pi * R_bar**4
-                J = I * 2.0
-                bondpot_fn = numpy.sum(0.5 * A * self.k_n * numpy.abs(self.bonds_delta_n)**2)
-                bondpot_ft = numpy.sum(0.5 * A * self.k_t * numpy.linalg.norm(self.bonds_delta_t)**2)
-                bondpot_tn = numpy.sum(0.5 * J * self.k_t * numpy.abs(self.bonds_omega_n)**2)
-                bondpot_tt = numpy.sum(0.5 * I * self.k_n * numpy.linalg.norm(self.bonds_omega_t)**2)
-                return bondpot_fn + bondpot_ft + bondpot_tn + bondpot_tt
+    def _bond_energy(self):
+        """Elastic energy of the bonds (python/sphere.py:4360-4376): with
+        R = lambda min(r_i, r_j), A = pi R^2, I = pi R^4 / 4, J = 2 I, the sum
+        over bonds of A k_n dn^2/2 + J k_t wn^2/2, plus A k_t |dt|^2/2 and
+        I k_n |wt|^2/2 where, as in the reference, |dt| and |wt| are the
+        norms of the whole tangential arrays (the per-bond norms for one
+        bond)."""
+        nb = int(self.nb0[0])
+        if nb == 0:
             return 0.0
-        raise Exception('Unknownw energy() method "' + method + '"')
+        b = numpy.asarray(self.bonds).reshape(nb, 2)
+        R = self.lambda_bar[0] * numpy.minimum(self.radius[b[:, 0]], self.radius[b[:, 1]])
+        A = numpy.pi * R ** 2
+        I = 0.25 * numpy.pi * R ** 4
+        J = 2.0 * I
+        dt2 = numpy.sum(numpy.asarray(self.bonds_delta_t) ** 2)
+        wt2 = numpy.sum(numpy.asarray(self.bonds_omega_t) ** 2)
+        return float(numpy.sum(0.5 * A * self.k_n[0] * numpy.asarray(self.bonds_delta_n) ** 2)
+                     + numpy.sum(0.5 * A * self.k_t[0] * dt2)
+                     + numpy.sum(0.5 * J * self.k_t[0] * numpy.asarray(self.bonds_omega_n) ** 2)
+                     + numpy.sum(0.5 * I * self.k_n[0] * wt2))
+
+    def energy(self, method):
+        """Energy sums of the assemblage (python/sphere.py:4311-4380): 'pot'
+        m |g| z, 'kin' m |v|^2 / 2, 'rot' (2/5 m r^2) |w|^2 / 2, 'shear' es,
+        'shearrate' es_dot, 'visc_n' ev, 'visc_n_rate' ev_dot, 'bondpot' the
+        bonds' elastic energy."""
+        sums = {'shear': 'es', 'shearrate': 'es_dot', 'visc_n': 'ev', 'visc_n_rate': 'ev_dot'}
+        if method in sums:
+            return float(numpy.sum(getattr(self, sums[method])))
+        m = self._masses()
+        if method == 'pot':
+            return float(numpy.sum(m * numpy.linalg.norm(self.g) * self.x[:, 2]))
+        if method == 'kin':
+            return float(numpy.sum(0.5 * m * numpy.sum(self.vel ** 2, axis=1)))
+        if method == 'rot':
+            return float(numpy.sum(0.2 * m * self.radius ** 2 * numpy.sum(self.angvel ** 2, axis=1)))
+        if method == 'bondpot':
+            return self._bond_energy()
+        raise Exception('Unknown energy() method "' + method + '"')
+
+    def _grain_box_volume(self):
+        lo = numpy.min(self.x - self.radius[:, None], axis=0)
+        hi = numpy.max(self.x + self.radius[:, None], axis=0)
+        return float(numpy.prod(hi - lo))
+
+    def voidRatio(self):
+        """(V_world - V_solid) / V_solid with V_world = L0 L1 L2 (no walls)
+        or L0 L1 w_x[0] (one wall) (python/sphere.py:4382-4400)."""
+        V_s = float(numpy.sum(V_sphere(self.radius)))
+        return (self._bulk_volume() - V_s) / V_s
+
+    def _bulk_volume(self):
+        if self.nw[0] == 0:
+            return float(self.L[0] * self.L[1] * self.L[2])
+        if self.nw[0] == 1:
+            V = float(self.L[0] * self.L[1] * self.w_x[0])
+            if V <= 0.0:
+                raise Exception("Could not determine total volume")
+            return V
+        return 0.0
+
+    def bulkPorosity(self, trim=True):
+        """Porosity 1 - V_solid / V of the assemblage (python/sphere.py
+        :4402-4436); V is the axis-parallel box around the grains (trim), the
+        world (no walls) or the world below the top wall (one wall)."""
+        V = self._grain_box_volume() if trim else self._bulk_volume()
+        V_s = float(numpy.sum(V_sphere(self.radius)))
+        return (V - V_s) / V
+
+    def _driven(self):
+        return numpy.nonzero(self.fixvel > 0.0)
+
+    def shearDisplacement(self):
+        """x displacement of the driven (fixvel > 0) grains: the largest
+        accumulated x movement among them (python/sphere.py:4777-4792)."""
+        return float(numpy.max(self.xyzsum[self._driven()][:, 0]))
+
+    def shearVelocity(self):
+        """Largest x velocity of the driven grains (python/sphere.py:4794-4808)."""
+        return float(numpy.max(self.vel[self._driven()][:, 0]))
+
+    def shearVel(self):
+        return self.shearVelocity()
+
+    def shearStrain(self):
+        """Shear displacement over the top wall's height (python/sphere.py:4816-4835)."""
+        return self.shearDisplacement() / self.w_x[0]
+
+    def shearStrainRate(self):
+        """Shear velocity over the top wall's height (python/sphere.py:4837-4853)."""
+        return self.shearVelocity() / self.w_x[0]
 
     # ------------------------------------------------------------------
     # execution, python/sphere.py:4482-4540
