@@ -61,52 +61,6 @@ def contact_bytes_per_particle(zbar):
     return 196.0 + 52.0 * zbar
 
 
-def prepare_shear_box(engine, n, seed, stage_steps, z_target, relax_rounds, relax_steps):
-    from sphere_amd import workloads as W
-    s, f0, rng = W.shear_box_base(n=n, seed=seed, phi=0.60)
-    s.gamma_n[0] = 5e3
-    st = W.EngineStepper(engine)
-    t0 = time.time()
-    f, z = W.grow_to_jamming(s, st, f0, z_target=z_target, steps=stage_steps, fine_df=0.005)
-    log("growth to jamming done in %.1f s: f=%.4f, Z=%.2f" % (time.time() - t0, f, z))
-    s.w_x[0] = s.L[2]
-    s.g[2] = -9.81
-    t0 = time.time()
-    W.relax_packing(s, st, relax_rounds, relax_steps)
-    log("relaxation done in %.1f s (%d rounds x %d steps); stages with slot overflow: %d" %
-        (time.time() - t0, relax_rounds, relax_steps, st.overflow_steps))
-    # contacts of the final packing: one step with the run's contact law
-    s.gamma_n[0] = 0.0
-    engine.load(s.to_state())
-    engine.step(1)
-    contacts = engine.read_contacts()[0]
-    per = numpy.count_nonzero(contacts.reshape(n, -1) != n, axis=1)
-    log("contacts per grain after preparation: mean %.2f max %d" % (per.mean(), per.max()))
-    nb = W.shear_box_finish(s, rng, contacts=contacts, bond_fraction=0.05)
-    log("packing: %d grains, %d bonds, grid %s, box %s" % (n, nb, [int(v) for v in s.num],
-                                                          [round(float(v), 4) for v in s.L]))
-    return s
-
-
-def prepare_consolidation(engine, n, seed, stage_steps, relax_rounds, relax_steps):
-    from sphere_amd import workloads as W
-    s, f0, rng = W.consolidation_base(n=n, seed=seed)
-    st = W.EngineStepper(engine)
-    t0 = time.time()
-    f, z = W.grow_to_jamming(s, st, f0, z_target=4.2, steps=stage_steps, fine_df=0.005)
-    log("consolidation box: growth to jamming done in %.1f s: f=%.4f, Z=%.2f" %
-        (time.time() - t0, f, z))
-    s.w_x[0] = s.L[2]
-    s.g[2] = -9.81
-    t0 = time.time()
-    W.relax_packing(s, st, relax_rounds, relax_steps)
-    log("consolidation box: relaxation done in %.1f s" % (time.time() - t0))
-    W.consolidation_finish(s)
-    log("consolidation box: %d grains, grid %s, box %s, wall at %.4f m" %
-        (n, [int(v) for v in s.num], [round(float(v), 4) for v in s.L], float(s.w_x[0])))
-    return s
-
-
 def percentiles(v):
     v = numpy.asarray(v, dtype=numpy.float64)
     return {"mean": float(v.mean()), "p50": float(numpy.percentile(v, 50)),
@@ -136,6 +90,7 @@ def run_darcy(engine, s, warmup, steps):
 
 def darcy_packing(args, engine):
     import sphere_amd
+    from sphere_amd import workloads as W
     if args.darcy_load:
         s = sphere_amd.sim(fluid=True)
         s.readbin(args.darcy_load, verbose=False)
@@ -143,7 +98,7 @@ def darcy_packing(args, engine):
         log("loaded the consolidation box from %s" % args.darcy_load)
     else:
         log("preparing the consolidation box (%d grains)" % args.darcy_n)
-        s = prepare_consolidation(engine, args.darcy_n, args.darcy_seed, 300, 3, 400)
+        s = W.prepare_consolidation(engine, args.darcy_n, args.darcy_seed, 300, 3, 400, log=log)
     if args.darcy_save:
         os.makedirs(args.darcy_save, exist_ok=True)
         s.writebin(folder=args.darcy_save, verbose=False)
@@ -428,6 +383,7 @@ def main():
         import torch
         torch.cuda.init()
     import sphere_amd
+    from sphere_amd import workloads as W
     if args.no_shear and args.ensemble and args.no_darcy:
         d = ensemble_section(args)
         print(json.dumps({"metric": d["metric"], "value": d["together"], "unit": "particle-steps/s",
@@ -459,8 +415,8 @@ def main():
         log("loaded the shear box from %s" % args.load_packing)
     else:
         log("preparing the shear box (%d grains)" % args.n)
-        s = prepare_shear_box(engine, args.n, args.seed, args.stage_steps, args.z_target,
-                              args.relax_rounds, args.relax_steps)
+        s = W.prepare_shear_box(engine, args.n, args.seed, args.stage_steps, args.z_target,
+                                args.relax_rounds, args.relax_steps, log=log)
     if args.save_packing:
         os.makedirs(args.save_packing, exist_ok=True)
         s.writebin(folder=args.save_packing, verbose=False)

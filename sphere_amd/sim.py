@@ -473,9 +473,10 @@ class sim:
         if self.fluid:
             w(self.cfd_solver, numpy.int32)
             w(self.mu, numpy.float64)
+            v_f, p_f, phi, dphi, p_c = self._grid_cells()
             cells = numpy.concatenate(
-                [self.v_f, self.p_f[..., None], self.phi[..., None],
-                 (self.dphi * self.time_dt * self.ndem)[..., None]], axis=3)
+                [v_f, p_f[..., None], phi[..., None],
+                 (dphi * self.time_dt * self.ndem)[..., None]], axis=3)
             w(cells.transpose(2, 1, 0, 3), numpy.float64)
             for nm in ['rho_f', 'p_mod_A', 'p_mod_f', 'p_mod_phi']:
                 w(getattr(self, nm), numpy.float64)
@@ -486,7 +487,7 @@ class sim:
                 w(getattr(self, nm), numpy.int32)
             w(self.bc_bot_flux, numpy.float64)
             w(self.bc_top_flux, numpy.float64)
-            w(numpy.asarray(self.p_f_constant).transpose(2, 1, 0), numpy.int32)
+            w(p_c.transpose(2, 1, 0), numpy.int32)
             if self.cfd_solver[0] == 0:
                 for nm in ['gamma', 'theta', 'beta', 'tolerance']:
                     w(getattr(self, nm), numpy.float64)
@@ -508,6 +509,20 @@ class sim:
         w(self.color, numpy.int32)
         return b"".join(out)
 
+    def _grid_cells(self):
+        """The cell fields over the current grid: like the reference's
+        writer, the first num[0] x num[1] x num[2] cells (a helper that
+        changes num after initFluid leaves the arrays larger)."""
+        nx, ny, nz = (int(v) for v in self.num[:3])
+        out = []
+        for name in ("v_f", "p_f", "phi", "dphi", "p_f_constant"):
+            a = numpy.asarray(getattr(self, name))
+            if a.shape[0] < nx or a.shape[1] < ny or a.shape[2] < nz:
+                raise ValueError("%s has %s cells, fewer than the grid's %s"
+                                 % (name, a.shape[:3], (nx, ny, nz)))
+            out.append(a[:nx, :ny, :nz])
+        return out
+
     def writebin(self, folder="../input/", verbose=True):
         """Write ``<folder>/<sid>.bin`` (python/sphere.py:1355-1553)."""
         targetbin = folder + "/" + self.sid + ".bin"
@@ -516,6 +531,12 @@ class sim:
         data = self._bytes()
         with open(targetbin, "wb") as fh:
             fh.write(data)
+
+    def id(self, sid=''):
+        """The simulation id, or set it when sid is given."""
+        if sid == '':
+            return self.sid
+        self.sid = sid
 
     def readfirst(self, verbose=True):
         self.readbin('../output/' + self.sid + '.output00000.bin', verbose)
@@ -612,11 +633,12 @@ class sim:
             buf.get('bonds_omega_t', nb, 3)[:] = numpy.asarray(self.bonds_omega_t).reshape(nb, 3)
         if self.fluid:
             cells = int(self.num[0] * self.num[1] * self.num[2])
-            buf.get('v_f', cells, 3)[:] = self.v_f.transpose(2, 1, 0, 3).reshape(-1, 3)
-            buf.get('p_f', cells, 1)[:] = self.p_f.transpose(2, 1, 0).ravel()
-            buf.get('phi', cells, 1)[:] = self.phi.transpose(2, 1, 0).ravel()
-            buf.get('dphi', cells, 1)[:] = (self.dphi * self.time_dt * self.ndem).transpose(2, 1, 0).ravel()
-            buf.get('p_constant', cells, 1)[:] = numpy.asarray(self.p_f_constant).transpose(2, 1, 0).ravel()
+            v_f, p_f, phi, dphi, p_c = self._grid_cells()
+            buf.get('v_f', cells, 3)[:] = v_f.transpose(2, 1, 0, 3).reshape(-1, 3)
+            buf.get('p_f', cells, 1)[:] = p_f.transpose(2, 1, 0).ravel()
+            buf.get('phi', cells, 1)[:] = phi.transpose(2, 1, 0).ravel()
+            buf.get('dphi', cells, 1)[:] = (dphi * self.time_dt * self.ndem).transpose(2, 1, 0).ravel()
+            buf.get('p_constant', cells, 1)[:] = p_c.transpose(2, 1, 0).ravel()
             if n:
                 buf.get('f_p', n, 3)[:] = self.f_p
         return buf

@@ -10,12 +10,16 @@
 * consolidation_base(n) / consolidation_finish(s)
                     -- Darcy-coupled consolidation: 200,000 grains under a
                        20 kPa wall on a 48 x 48 x 96 Darcy grid.
+* prepare_shear_box(engine, n, seed), prepare_consolidation(engine, n, seed)
+                    -- the two benchmark boxes, grown on the device.
 
 Dense packings are made by radius growth: grains start on a jittered cubic
 lattice with radii scaled down until no two touch, then the radii grow back
 to their drawn values in stages while the DEM (the stepper given) relaxes
 the overlaps; velocities are zeroed at the start of every stage.
 """
+import time
+
 import numpy
 
 from .sim import sim
@@ -481,3 +485,57 @@ def ensemble_boxes(count=64, seed=5, n_min=5000, n_max=60000, steps=300, z_targe
     params = [{"grains": int(sizes[k]), "mu": float(mus[k]), "sigma0": float(sig[k]),
                "v_top": float(vel[k])} for k in range(count)]
     return [bx["s"] for bx in boxes], params
+
+
+def prepare_shear_box(engine, n, seed, stage_steps=400, z_target=4.2, relax_rounds=4, relax_steps=400,
+                      log=None):
+    """The dense periodic shear box of the benchmark: grown to jamming on the
+    device, relaxed under gravity, then given its run configuration (Hertz-
+    Mindlin, 10 kPa top wall, driven top layer, 5% of the touching pairs
+    bonded).  Returns the sim object."""
+    log = log or (lambda m: None)
+    s, f0, rng = shear_box_base(n=n, seed=seed, phi=0.60)
+    s.gamma_n[0] = 5e3
+    st = EngineStepper(engine)
+    t0 = time.time()
+    f, z = grow_to_jamming(s, st, f0, z_target=z_target, steps=stage_steps, fine_df=0.005)
+    log("growth to jamming done in %.1f s: f=%.4f, Z=%.2f" % (time.time() - t0, f, z))
+    s.w_x[0] = s.L[2]
+    s.g[2] = -9.81
+    t0 = time.time()
+    relax_packing(s, st, relax_rounds, relax_steps)
+    log("relaxation done in %.1f s (%d rounds x %d steps); stages with slot overflow: %d" %
+        (time.time() - t0, relax_rounds, relax_steps, st.overflow_steps))
+    # contacts of the final packing: one step with the run's contact law
+    s.gamma_n[0] = 0.0
+    engine.load(s.to_state())
+    engine.step(1)
+    contacts = engine.read_contacts()[0]
+    per = numpy.count_nonzero(contacts.reshape(n, -1) != n, axis=1)
+    log("contacts per grain after preparation: mean %.2f max %d" % (per.mean(), per.max()))
+    nb = shear_box_finish(s, rng, contacts=contacts, bond_fraction=0.05)
+    log("packing: %d grains, %d bonds, grid %s, box %s" % (n, nb, [int(v) for v in s.num],
+                                                          [round(float(v), 4) for v in s.L]))
+    return s
+
+
+def prepare_consolidation(engine, n, seed, stage_steps=300, relax_rounds=3, relax_steps=400, log=None):
+    """The Darcy-coupled consolidation box of the benchmark (200,000 grains
+    on a 48 x 48 x 96 grid by default): grown to jamming, relaxed, then
+    loaded by a 20 kPa wall with the fluid set up.  Returns the sim object."""
+    log = log or (lambda m: None)
+    s, f0, rng = consolidation_base(n=n, seed=seed)
+    st = EngineStepper(engine)
+    t0 = time.time()
+    f, z = grow_to_jamming(s, st, f0, z_target=4.2, steps=stage_steps, fine_df=0.005)
+    log("consolidation box: growth to jamming done in %.1f s: f=%.4f, Z=%.2f" %
+        (time.time() - t0, f, z))
+    s.w_x[0] = s.L[2]
+    s.g[2] = -9.81
+    t0 = time.time()
+    relax_packing(s, st, relax_rounds, relax_steps)
+    log("consolidation box: relaxation done in %.1f s" % (time.time() - t0))
+    consolidation_finish(s)
+    log("consolidation box: %d grains, grid %s, box %s, wall at %.4f m" %
+        (n, [int(v) for v in s.num], [round(float(v), 4) for v in s.L], float(s.w_x[0])))
+    return s

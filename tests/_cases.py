@@ -190,3 +190,51 @@ def fluid_particle(test):
         s.g[2] = -10.0
     s.initTemporal(total=0.001, file_dt=0.0001)
     return s
+
+
+def pressed_case(n=1000, seed=0, model=2, wmode=1, side=0.0003, top=0.0003, sigma0=2e4,
+                 wall_vel=-0.01, E=0.0, nu=0.3):
+    """Grains on a cubic lattice in a closed box (periodic = 0) whose walls
+    press on them from the start: the outer grains overlap the bottom, the
+    four side boundaries and the top wall by about `side` / `top` metres.
+    The top wall is stress driven (wmode 1, sigma0), velocity driven (wmode
+    2, wall_vel) or fixed (wmode 0); wall damping and friction are on, so
+    the wall law's normal, viscous and frictional branches all act, and the
+    wall force partials are non-zero from the first step."""
+    rng = numpy.random.default_rng(seed)
+    side_n = int(round(n ** (1.0 / 3.0)))
+    nz = (n + side_n * side_n - 1) // (side_n * side_n)
+    r = 0.01
+    spacing = 2.0 * r * 0.999
+    x = lattice_positions(side_n, side_n, nz, spacing, 0.0, rng)[:n]
+    s = sphere_amd.sim(np=n, sid="pressed", nw=1)
+    s.radius[:] = rng.uniform(0.0098, 0.0102, n)
+    # the lattice starts at r; shift it so that the outer grains overlap the
+    # x, y and z = 0 boundaries by `side`
+    x -= 0.5 * spacing - r + side
+    s.x[:] = x
+    s.vel[:] = rng.uniform(-0.02, 0.02, (n, 3))
+    s.angvel[:] = rng.uniform(-2.0, 2.0, (n, 3))
+    s.defaultParams(mu_s=0.5, mu_d=0.4, k_n=1.16e7, k_t=1.16e7, gamma_n=20.0, gamma_t=10.0,
+                    gamma_wn=2e3, gamma_wt=2e3)
+    s.mu_ws[0] = 0.3
+    s.mu_wd[0] = 0.3
+    s.contactmodel[0] = model
+    if model == 3:
+        s.setYoungsModulus(E)
+        s.setPoissonsRatio(nu)
+    s.g[2] = -9.81
+    s.periodic[0] = 0
+    lx = (side_n - 1) * spacing + 2 * r - 2 * side
+    top_z = numpy.max(s.x[:, 2] + s.radius)
+    s.defineWorldBoundaries(L=[lx, lx, top_z + 0.1], dx=2.2 * 0.0102)
+    s.w_n[0, :] = [0.0, 0.0, -1.0]
+    s.w_x[0] = top_z - top
+    s.w_m[0] = 0.2 * s.totalMass()
+    s.wmode[0] = wmode
+    s.w_sigma0[0] = sigma0 if wmode == 1 else 0.0
+    s.w_vel[0] = wall_vel if wmode == 2 else 0.0
+    m_min = 2600.0 * 4.0 / 3.0 * numpy.pi * 0.0098 ** 3
+    k = 1.16e7 if model != 3 else numpy.pi / 2.0 * E * 0.0102
+    s.initTemporal(total=1.0, file_dt=1.0, dt=0.1 * numpy.sqrt(m_min / k))
+    return s

@@ -1,0 +1,415 @@
+"""The drop-in path end to end: ``sim.run()`` -> ``sph_run`` -> the files it
+writes (outputs, status.dat, conv.log), as the reference's own tests use it
+(tests/io_tests.py, io_tests_fluid.py, contactmodel_wall.py,
+cfd_tests_darcy.py, cfd_tests_darcy_particles.py).  Each test keeps the
+reference test's assertions and, where the run is short enough for the
+oracle, also compares the written state with the oracle stepped to the same
+step count (1e-10 relative per step, the parity bar of the other GPU tests).
+
+Runs happen in a scratch tree with input/, output/ and a working folder, as
+``cd ..; ./sphere input/<sid>.bin`` expects."""
+import os
+import shutil
+
+import numpy
+import pytest
+
+import sphere_amd
+from _oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL_PER_STEP = 1e-10
+
+
+@pytest.fixture
+def rundir(tmp_path):
+    for d in ("input", "output", "work"):
+        (tmp_path / d).mkdir()
+    old = os.getcwd()
+    os.chdir(tmp_path / "work")
+    yield tmp_path
+    os.chdir(old)
+
+
+def schedule(dt, total, file_dt, current=0.0):
+    """Step count of a run and the step counts at which its output files
+    1, 2, ... are written (the loop of src/device.cu:833-2834)."""
+    t, clock, n, outs = current, 0.0, 0, []
+    while t <= total:
+        t += dt
+        clock += dt
+        n += 1
+        if clock >= file_dt:
+            outs.append(n)
+            clock = 0.0
+    return n, outs
+
+
+def read_output(sid, k, fluid=False):
+    s = sphere_amd.sim(sid=sid, fluid=fluid)
+    s.readbin("../output/%s.output%05d.bin" % (sid, k), verbose=False)
+    return s
+
+
+def rel(a, b):
+    a = numpy.asarray(a, float)
+    b = numpy.asarray(b, float)
+    scale = numpy.max(numpy.abs(b)) if b.size else 0.0
+    d = numpy.max(numpy.abs(a - b)) if b.size else 0.0
+    return d / scale if scale > 0 else d
+
+
+def compare_with_oracle(out, ora, steps, fluid=False):
+    """A state read back from a written file against the oracle's state."""
+    st = ora.read()
+    n = int(out.np[0])
+    worst = 0.0
+    if n:
+        pairs = [(numpy.column_stack([out.x, out.radius]), st.get("x", n, 4)),
+                 (numpy.column_stack([out.vel, out.fixvel]), st.get("vel", n, 4)),
+                 (out.angvel, st.get("angvel", n, 3)), (out.force, st.get("force", n, 3)),
+                 (out.torque, st.get("torque", n, 3)), (out.xyzsum, st.get("xyzsum", n, 3)),
+                 (out.es, st.get("es", n, 1)), (out.ev, st.get("ev", n, 1)),
+                 (out.p, st.get("p", n, 1))]
+        for a, b in pairs:
+            worst = max(worst, rel(a, b))
+    if fluid:
+        h = st.state
+        nx, ny, nz = h.num[0], h.num[1], h.num[2]
+        p = st.get("p_f", nx * ny * nz, 1).reshape(nz, ny, nx).transpose(2, 1, 0)
+        worst = max(worst, rel(out.p_f, p))
+    assert worst <= TOL_PER_STEP * max(steps, 1), "file state differs from the oracle: %g" % worst
+    assert out.time_step_count[0] == st.state.time_step_count
+    for w in range(int(out.nw[0])):
+        assert abs(out.w_x[w] - st.state.w_nx[4 * w + 3]) <= \
+            TOL_PER_STEP * max(steps, 1) * max(1.0, abs(st.state.w_nx[4 * w + 3]))
+    return worst
+
+
+# ----------------------------------------------------------------------
+# tests/io_tests.py, tests/io_tests_fluid.py
+# ----------------------------------------------------------------------
+def test_io_roundtrip_through_run(rundir):
+    """output00000 is the input file byte for byte (the native writer
+    against Python's writebin); output00001, after one step with g = 0 and
+    no contacts, equals the input except for the time; status.dat holds the
+    last file number (tests/io_tests.py:9-41)."""
+    numpy.random.seed(11)
+    orig = sphere_amd.sim(np=100, nw=1, sid="test-initgrid")
+    orig.generateRadii(histogram=False)
+    orig.defaultParams()
+    orig.g[2] = 0.0
+    orig.initRandomGridPos()
+    orig.initTemporal(current=0.0, total=0.0)
+    orig.time_total[0] = 2.0 * orig.time_dt[0]
+    orig.time_file_dt[0] = orig.time_dt[0]
+    orig.writebin(verbose=False)
+    py = sphere_amd.sim()
+    py.readbin("../input/" + orig.sid + ".bin", verbose=False)
+    assert orig == py
+    assert orig.run(dry=True, verbose=False) == 0
+    assert not os.path.exists("../output/test-initgrid.output00000.bin")
+    assert orig.run(verbose=False) == 0
+    with open("../input/test-initgrid.bin", "rb") as a, \
+            open("../output/test-initgrid.output00000.bin", "rb") as b:
+        assert a.read() == b.read()
+    cpp = read_output(orig.sid, 0)
+    assert orig == cpp
+    cuda = read_output(orig.sid, 1)
+    cuda.time_current = orig.time_current
+    cuda.time_step_count = orig.time_step_count
+    assert orig.differences(cuda) == []
+    n, outs = schedule(orig.time_dt[0], orig.time_total[0], orig.time_file_dt[0])
+    assert orig.status() == len(outs)
+    last = open("../output/test-initgrid.status.dat").read().split()
+    assert int(last[2]) == len(outs) and float(last[1]) == pytest.approx(100.0 * n / 2.0, rel=1e-3)
+
+
+def test_io_fluid_darcy_through_run(rundir):
+    """tests/io_tests_fluid.py:55-98 (Darcy): the first file equals the
+    input; after one step (10 Jacobi sweeps at most) the fluid and grains
+    have barely changed; and that file equals the oracle's state."""
+    numpy.random.seed(12)
+    orig = sphere_amd.sim(np=100, sid="test-initgrid-fluid", fluid=True)
+    orig.generateRadii(histogram=False, mean=1.0)
+    orig.defaultParams()
+    orig.initRandomGridPos()
+    orig.initFluid(cfd_solver=1)
+    orig.setMaxIterations(10)
+    orig.initTemporal(current=0.0, total=0.0)
+    orig.time_total[0] = 2.0 * orig.time_dt[0]
+    orig.time_file_dt[0] = orig.time_dt[0]
+    orig.writebin(verbose=False)
+    py = sphere_amd.sim(fluid=True)
+    py.readbin("../input/" + orig.sid + ".bin", verbose=False)
+    assert orig == py
+    ora = Oracle(orig.to_state())
+    assert orig.run(verbose=False) == 0
+    cpp = read_output(orig.sid, 0, fluid=True)
+    assert orig.differences(cpp) == []
+    cuda = read_output(orig.sid, 1, fluid=True)
+    assert numpy.allclose(orig.v_f, cuda.v_f, rtol=0, atol=1e-5)
+    assert numpy.allclose(orig.p_f, cuda.p_f, rtol=0, atol=0.1)
+    assert numpy.allclose(orig.x, cuda.x, rtol=0.01)
+    assert numpy.max(numpy.abs(cuda.vel - orig.vel)) < 1e-5
+    ora.step(1)
+    compare_with_oracle(cuda, ora, 1, fluid=True)
+
+
+# ----------------------------------------------------------------------
+# tests/contactmodel_wall.py:16-107
+# ----------------------------------------------------------------------
+def wall_grain(vz, vx=0.0, gamma_wn=0.0, gamma_wt=0.0, total=1.0):
+    s = sphere_amd.sim(np=1, nw=0, sid="contactmodeltest")
+    s.radius[:] = 1.0
+    s.x[0, :] = [5.0, 5.0, 1.05]
+    s.vel[0, 2] = vz
+    s.vel[0, 0] = vx
+    s.defineWorldBoundaries(L=[10, 10, 10])
+    s.gamma_wn[0] = gamma_wn
+    s.gamma_wt[0] = gamma_wt
+    s.initTemporal(total=total, file_dt=0.01)
+    return s
+
+
+def run_and_check_last(s):
+    """sim.run, then the last file against the oracle at its step count."""
+    ora = Oracle(s.to_state())
+    assert s.run(verbose=False) == 0
+    n, outs = schedule(s.time_dt[0], s.time_total[0], s.time_file_dt[0])
+    last = s.status()
+    assert last == len(outs)
+    out = read_output(s.sid, last)
+    ora.step(outs[-1])
+    compare_with_oracle(out, ora, outs[-1])
+    return out
+
+
+def test_wall_elastic_normal(rundir):
+    orig = wall_grain(-0.1)
+    before = orig.totalKineticEnergy()
+    out = run_and_check_last(orig)
+    assert out.vel[0, 2] == pytest.approx(0.1, rel=1e-3)
+    assert out.totalKineticEnergy() == pytest.approx(before, rel=1e-3)
+
+
+def test_wall_elastic_oblique(rundir):
+    orig = wall_grain(-0.1, vx=0.1, total=0.3)
+    before = orig.totalKineticEnergy()
+    out = run_and_check_last(orig)
+    assert out.totalKineticEnergy() == pytest.approx(before, rel=1e-3)
+
+
+def test_wall_viscous_normal(rundir):
+    orig = wall_grain(-0.1, gamma_wn=1e6)
+    before = orig.energy('kin')
+    out = run_and_check_last(orig)
+    after = out.energy('kin')
+    assert before > after
+    assert before == pytest.approx(after + out.energy('visc_n'), rel=0.05)
+
+
+def test_wall_viscous_oblique(rundir):
+    orig = wall_grain(-0.1, vx=0.1, gamma_wn=1e6, gamma_wt=1e6)
+    before = orig.energy('kin')
+    out = run_and_check_last(orig)
+    assert before > out.energy('kin')
+    assert out.angvel[0, 0] == 0.0 and out.angvel[0, 1] > 0.0 and out.angvel[0, 2] == 0.0
+
+
+# ----------------------------------------------------------------------
+# tests/cfd_tests_darcy.py
+# ----------------------------------------------------------------------
+def cfd_box(L=(0.4, 0.3, 0.4)):
+    s = sphere_amd.sim(np=0, nd=3, nw=0, sid="cfdtest", fluid=True)
+    s.defaultParams()
+    s.defineWorldBoundaries(list(L), dx=0.1)
+    s.initFluid(cfd_solver=1)
+    s.initTemporal(total=0.2, file_dt=0.01, dt=1.0e-7)
+    s.time_file_dt[0] = s.time_dt[0] * 0.99
+    s.time_total[0] = s.time_dt[0] * 10
+    return s
+
+
+def conv_log(sid):
+    return numpy.loadtxt("../output/" + sid + "-conv.log", ndmin=2)
+
+
+def test_darcy_no_forcing_zero_iterations(rundir):
+    """cfd_tests_darcy.py:14-45: pressure stays 0, no solve iterates (the
+    conv.log iteration column sums to 0) and there is no flow."""
+    orig = cfd_box()
+    assert orig.run(verbose=False) == 0
+    py = sphere_amd.sim(sid=orig.sid, fluid=True)
+    py.readlast(verbose=False)
+    assert numpy.array_equal(py.p_f, numpy.zeros(orig.num))
+    it = conv_log(orig.sid)
+    assert it.shape[0] >= 1 and it[:, 1].sum() == 0.0
+    assert numpy.all(numpy.abs(py.v_f) < 1e-6)
+
+
+def test_darcy_pressure_gradient_and_conv_log(rundir):
+    """cfd_tests_darcy.py:48-76 with 0.01 s instead of 0.5 s (the pressure
+    diffusion time of this box is about 2 ms): the profile is linear, the
+    flow points down, the conv.log lines are those of the oracle, and the
+    last file equals the oracle's state."""
+    orig = cfd_box()
+    orig.p_f[:, :, -1] = 1.0
+    orig.initTemporal(total=0.01, file_dt=0.001, dt=1.0e-6)
+    ora = Oracle(orig.to_state())
+    assert orig.run(verbose=False) == 0
+    py = sphere_amd.sim(sid=orig.sid, fluid=True)
+    py.readlast(verbose=False)
+    ideal = numpy.linspace(orig.p_f[0, 0, 0], orig.p_f[0, 0, -1], int(orig.num[2]))
+    assert numpy.max(numpy.abs(ideal - py.p_f[0, 0, :])) < 0.1
+    assert numpy.all(py.v_f[:, :, :, 2] < 0.0) and numpy.all(py.v_f[:, :, :, 0:1] < 1e-7)
+    it = conv_log(orig.sid)
+    assert numpy.all(it[0:6, 1] < 1000) and numpy.all(it[6:, 1] < 20)
+    n, outs = schedule(orig.time_dt[0], orig.time_total[0], orig.time_file_dt[0])
+    ora.step(outs[-1])
+    compare_with_oracle(py, ora, outs[-1], fluid=True)
+    # conv.log: the oracle's (iter + 1, nijac) lines for the same run
+    ora.step(n - outs[-1])
+    assert numpy.array_equal(it.astype(numpy.int64), ora.conv_log().astype(numpy.int64))
+
+
+def test_darcy_fast_pressure_modulation(rundir):
+    """cfd_tests_darcy.py:133-150: a top pressure 1 + sin(2 pi t / T) over
+    one period T = 0.01 s keeps a smooth (near-linear) profile in every
+    output file, and each file matches the oracle."""
+    orig = cfd_box()
+    orig.p_f[:, :, -1] = 1.0
+    orig.initTemporal(total=0.5, file_dt=0.01, dt=1.0e-6)
+    orig.time_total[0] = 1.0e-2
+    orig.time_file_dt[0] = 0.101 * orig.time_total[0]
+    orig.setFluidPressureModulation(A=1.0, f=1.0 / orig.time_total[0])
+    ora = Oracle(orig.to_state())
+    assert orig.run(verbose=False) == 0
+    py = sphere_amd.sim(sid=orig.sid, fluid=True)
+    n, outs = schedule(orig.time_dt[0], orig.time_total[0], orig.time_file_dt[0])
+    assert py.status() == len(outs)
+    done = 0
+    tops = []
+    for k in range(1, py.status() + 1):
+        py.readstep(k, verbose=False)
+        ideal = numpy.linspace(py.p_f[0, 0, 0], py.p_f[0, 0, -1], int(py.num[2]))
+        assert numpy.max(numpy.abs(ideal - py.p_f[0, 0, :])) < 0.5
+        ora.step(outs[k - 1] - done)
+        done = outs[k - 1]
+        compare_with_oracle(py, ora, done, fluid=True)
+        tops.append(py.p_f[0, 0, -1])
+    # the top pressure follows 1 + sin(2 pi f t) (p_top_orig = 1)
+    t = numpy.array(outs) * orig.time_dt[0]
+    assert numpy.max(numpy.abs(numpy.array(tops) - (1.0 + numpy.sin(2 * numpy.pi * t / 1e-2)))) < 0.05
+
+
+def test_darcy_flux_top_and_x_gradient(rundir):
+    """cfd_tests_darcy.py:193-238: a prescribed top flux (bc 4) runs and
+    matches the oracle; then a pressure difference along x (bc 0 on the x
+    faces, no flow elsewhere) drives flow towards +x everywhere (0.01 s
+    instead of 0.5 s)."""
+    orig = cfd_box(L=(1.0, 1.0, 1.0))
+    orig.setFluidTopFixedFlux(1.0)
+    ora = Oracle(orig.to_state())
+    assert orig.run(verbose=False) == 0
+    py = sphere_amd.sim(sid=orig.sid, fluid=True)
+    py.readlast(verbose=False)
+    n, outs = schedule(orig.time_dt[0], orig.time_total[0], orig.time_file_dt[0])
+    ora.step(outs[-1])
+    compare_with_oracle(py, ora, outs[-1], fluid=True)
+    assert numpy.any(py.p_f != 0.0)  # the flux drives the pressure
+    x = cfd_box(L=(1.0, 1.0, 1.0))
+    x.p_f[:, :, :] = 0.0
+    x.p_f[0, :, :] = 2.0
+    x.setFluidXFixedPressure()
+    x.setFluidYNoFlow()
+    x.setFluidTopNoFlow()
+    x.setFluidBottomNoFlow()
+    x.initTemporal(total=0.01, file_dt=0.001, dt=1.0e-6)
+    assert x.run(verbose=False) == 0
+    py.readlast(verbose=False)
+    assert numpy.all(py.v_f[:, :, :, 0] > 0.0)
+
+
+# ----------------------------------------------------------------------
+# tests/cfd_tests_darcy_particles.py:117-215 on the reference's own
+# cube-init packing (1,000 grains, written by its CUDA build)
+# ----------------------------------------------------------------------
+def cube_init(rundir, sid):
+    shutil.copy(os.path.join(GOLDEN, "cube-init.output00029.bin"),
+                str(rundir / "output" / "cube-init.output00029.bin"))
+    shutil.copy(os.path.join(GOLDEN, "cube-init.status.dat"),
+                str(rundir / "output" / "cube-init.status.dat"))
+    s = sphere_amd.sim('cube-init', fluid=False)
+    s.readlast(verbose=False)
+    s.num[2] //= 2
+    s.L[2] /= 2.0
+    s.id(sid)
+    s.setStiffnessNormal(36.4e9)
+    s.setStiffnessTangential(36.4e9 / 3.0)
+    return s
+
+
+@pytest.mark.parametrize("ndem", [1, 10])
+def test_consolidation_wall_moves_down(rundir, ndem):
+    """The top wall of the fluid-saturated cube-init packing moves down
+    under a 10 kPa load with ndem = 1 and 10 (cfd_tests_darcy_particles.py
+    :117-160); the reference runs 0.5 ms, here 0.1 ms (the wall moves from
+    the first step).  The first file after 200 steps matches the oracle."""
+    orig = cube_init(rundir, "darcy_fluidization")
+    orig.initTemporal(total=0.0001, file_dt=0.0001)
+    orig.initFluid(cfd_solver=1)
+    orig.setDEMstepsPerCFDstep(ndem)
+    orig.setFluidBottomNoFlow()
+    orig.g[2] = -10.0
+    orig.consolidate()
+    ref = sphere_amd.sim(sid="darcy_fluidization", fluid=True)
+    # a short run compared with the oracle file by file
+    short = cube_init(rundir, "darcy_short")
+    short.initTemporal(total=1.0, file_dt=1.0)
+    short.initFluid(cfd_solver=1)
+    short.setDEMstepsPerCFDstep(ndem)
+    short.setFluidBottomNoFlow()
+    short.g[2] = -10.0
+    short.consolidate()
+    short.time_file_dt[0] = 99.5 * short.time_dt[0]
+    short.time_total[0] = 199.5 * short.time_dt[0]
+    ora = Oracle(short.to_state())
+    assert short.run(verbose=False) == 0
+    n, outs = schedule(short.time_dt[0], short.time_total[0], short.time_file_dt[0])
+    done = 0
+    for k, at in enumerate(outs, start=1):
+        ora.step(at - done)
+        done = at
+        compare_with_oracle(read_output("darcy_short", k, fluid=True), ora, at, fluid=True)
+    assert orig.run(verbose=False) == 0
+    ref.readlast(verbose=False)
+    assert orig.w_x[0] > ref.w_x[0]
+
+
+def test_fluidization_threshold(rundir):
+    """cfd_tests_darcy_particles.py:163-215: a pressure difference of 0.9x
+    the buoyant weight of the packing leaves the grains settling (mean
+    upward velocity below 2 mm/s); 2.0x lifts them."""
+    orig = cube_init(rundir, "darcy_fluidization")
+    orig.setDEMstepsPerCFDstep(1)
+    orig.initTemporal(total=0.0005, file_dt=0.0001)
+    orig.initFluid(cfd_solver=1)
+    orig.g[2] = -10.0
+    porosity = orig.bulkPorosity()
+    fluidize = abs((orig.rho[0] - orig.rho_f[0]) * (1.0 - porosity) * abs(orig.g[2]))
+    for factor in (0.9, 2.0):
+        orig.id("fluidization-" + str(factor))
+        dp = factor * fluidize * orig.L[2]
+        orig.p_f[:, :, 0] = dp
+        orig.p_f[:, :, -1] = 0.0
+        assert orig.run(verbose=False) == 0
+        py = sphere_amd.sim(sid=orig.sid, fluid=True)
+        py.readlast(verbose=False)
+        vz = numpy.mean(py.vel[:, 2])
+        if factor < 1.0:
+            assert vz < 0.002
+        else:
+            assert vz > 0.002

@@ -52,7 +52,9 @@ def _net_case(shape, seed):
 @pytest.mark.parametrize("shape,wrap,grads", [((16, 16, 16), 0, True), ((24, 16, 8), 0, True),
                                               ((16, 16, 24), 3, True), ((8, 24, 16), 1, True),
                                               # the full-resolution kernels of large grids
-                                              ((32, 48, 48), 3, False)])
+                                              ((32, 48, 48), 3, False),
+                                              # the consolidation box's Darcy grid
+                                              ((96, 48, 48), 3, False)])
 def test_net_forward_backward_match_oracle(torch, shape, wrap, grads):
     """On large grids the BF16 roundings of the GPU and of the oracle part
     at a few activations where FP32 summation order decides a tie, and the
