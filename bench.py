@@ -145,10 +145,36 @@ def darcy_section(args):
                                    % (sweep_bytes / 1e6)},
     }
     engine.close()
+    out["cpu_baseline"] = None if args.no_cpu or args.darcy_cpu_steps <= 0 else \
+        darcy_cpu_leg(args, s)
     if args.no_net:
         return out
     out["learned"] = learned_section(args, s, ms, sweeps)
     return out
+
+
+def darcy_cpu_leg(args, s):
+    """The consolidation box on the CPU oracle: 2 untimed fluid steps, then
+    --darcy-cpu-steps timed ones; the Jacobi solve time per fluid step (the
+    same span the GPU figure covers) and the step time."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import Oracle
+    threads = host_threads(args)
+    ora = Oracle(s.to_state(), threads=threads, nc=32)
+    ora.step(2)
+    a = ora.darcy_stats()
+    t0 = time.perf_counter()
+    ora.step(args.darcy_cpu_steps)
+    dt = time.perf_counter() - t0
+    b = ora.darcy_stats()
+    k = args.darcy_cpu_steps
+    solve_ms = (b.solve_ms - a.solve_ms) / k
+    log("consolidation box on the CPU oracle (%d threads): %.1f ms per solve, %.1f sweeps, %.1f ms "
+        "per fluid step" % (threads, solve_ms, (b.sweeps - a.sweeps) / k, 1e3 * dt / k))
+    return {"ms_per_solve": solve_ms, "sweeps_per_solve": (b.sweeps - a.sweeps) / float(k),
+            "ms_per_step": 1e3 * dt / k, "cores": threads, "kind": "port",
+            "sample": "2 untimed + %d timed fluid steps of the FP64 C oracle from the loaded box"
+                      % k, "host": host_info(threads)}
 
 
 def net_flops(nx, ny, nz):
@@ -161,14 +187,22 @@ def net_flops(nx, ny, nz):
 
 
 def learned_section(args, s, plain_ms, plain_sweeps):
-    """Train the correction network on the consolidation box's own solves,
-    then time the same run with it (same packing, warm-up and step count)."""
+    """Train the correction network on the solves of a consolidation box
+    grown from another seed (held out from the timed run), then time the
+    timed box's run with it (same packing, warm-up and step count as the
+    plain run).  The same network is also timed on its own training box
+    (in-sample) for comparison."""
     import torch
     import sphere_amd
     from sphere_amd import learn
+    from sphere_amd import workloads as W
     t0 = time.time()
     trainer = sphere_amd.Engine(0, nc=32)
-    trainer.load(s.to_state())
+    s_train = W.prepare_consolidation(trainer, args.darcy_n, args.darcy_seed + 1, 300, 3, 400)
+    log("Darcy network: training box grown from seed %d in %.1f s" % (args.darcy_seed + 1,
+                                                                       time.time() - t0))
+    t0 = time.time()
+    trainer.load(s_train.to_state())
     trainer.step(args.darcy_warmup)
     net = learn.make_net(trainer)
     X, Y = learn.collect(trainer, args.net_steps)
@@ -204,11 +238,20 @@ def learned_section(args, s, plain_ms, plain_sweeps):
     torch.cuda.synchronize()
     fwd_s = e0.elapsed_time(e1) / 20 / 1000.0
     net.close()
-    # the timed run with the network
+    # the timed run with the network (held out), and the same network on its
+    # training box (in sample) beside it, with that box's plain solve
     engine = sphere_amd.Engine(0, nc=32)
     engine.set_darcy_net(params, args.net_corrections)
     ms, sweeps, elapsed, st = run_darcy(engine, s, args.darcy_warmup, args.darcy_steps)
     engine.close()
+    insample = {}
+    for use in (False, True):
+        e = sphere_amd.Engine(0, nc=32)
+        if use:
+            e.set_darcy_net(params, args.net_corrections)
+        ims, isw, _, _ = run_darcy(e, s_train, args.darcy_warmup, args.darcy_steps)
+        e.close()
+        insample["learned" if use else "plain"] = (ims, isw)
     # the converged pressure of one solve from the same state, with and without
     pf = []
     for use in (False, True):
@@ -226,8 +269,18 @@ def learned_section(args, s, plain_ms, plain_sweeps):
         (ms.mean(), numpy.percentile(ms, 50), sweeps.mean(), st.corrections / float(args.darcy_warmup +
                                                                                      args.darcy_steps),
          plain_ms.mean(), plain_sweeps.mean(), 1e6 * fwd_s, p_err))
+    (ip_ms, ip_sw), (il_ms, il_sw) = insample["plain"], insample["learned"]
+    log("in sample (the training box): %.3f ms, %.1f sweeps with the network; plain %.3f ms, "
+        "%.1f sweeps (%.2fx)" % (il_ms.mean(), il_sw.mean(), ip_ms.mean(), ip_sw.mean(),
+                                 ip_ms.mean() / il_ms.mean()))
     return {
+        "held_out": "trained on a box grown from seed %d, timed on the seed-%d box"
+                    % (args.darcy_seed + 1, args.darcy_seed),
         "ms": percentiles(ms), "sweeps": percentiles(sweeps),
+        "in_sample": {"ms": percentiles(il_ms), "sweeps": percentiles(il_sw),
+                      "plain_ms": percentiles(ip_ms), "plain_sweeps": percentiles(ip_sw),
+                      "speedup_ms": float(ip_ms.mean() / il_ms.mean()),
+                      "sweep_ratio_plain_over_learned": float(ip_sw.mean() / il_sw.mean())},
         "corrections_per_solve": st.corrections / float(args.darcy_warmup + args.darcy_steps),
         "nonconverged": int(st.nonconverged),
         "sweep_ratio_plain_over_learned": float(plain_sweeps.mean() / sweeps.mean()),
@@ -306,10 +359,59 @@ def ensemble_section(args):
     }
 
 
-def cpu_baseline(state_buf, n, warm, steps, threads):
+def host_threads(args):
+    """Threads of the CPU baseline: all the host cores this job may use (the
+    OpenMP thread count the job is given, else every core)."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def host_info(threads):
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "threads_used": threads,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS", "")}
+
+
+def settling_leg(args, threads):
+    """Config 1 on the GPU and on the CPU oracle: dry settling of 10,000
+    grains, all 2,000 steps on each (and the first 200 on one CPU thread)."""
+    import sphere_amd
+    from sphere_amd import workloads as W
+    s = W.settling(n=10000, seed=1, steps=2000)
+    st = s.to_state()
+    e = sphere_amd.Engine(0)
+    e.load(st)
+    e.step(1)
+    e.load(st)
+    t0 = time.perf_counter()
+    e.step(2000)
+    gpu_s = time.perf_counter() - t0
+    e.close()
+    cpu_v, cpu_s = cpu_baseline(st, 10000, 0, 2000, threads, nc=32)
+    one_v, one_s = cpu_baseline(st, 10000, 0, 200, 1, nc=32)
+    log("settling 10k x 2000 steps: GPU %.3f s (%.3g particle-steps/s), CPU oracle %.2f s on %d "
+        "threads (%.3g), one thread %.3g" % (gpu_s, 2e7 / gpu_s, cpu_s, threads, cpu_v, one_v))
+    return {"workload": "dry settling: 10000 grains, radii U[0.009, 0.011] m, jittered lattice, "
+                        "linear contacts k_n=k_t=1.16e7, mu=0.5, frictional walls, 2000 steps",
+            "gpu_particle_steps_per_s": 2e7 / gpu_s, "gpu_seconds": gpu_s,
+            "cpu_particle_steps_per_s": cpu_v, "cpu_seconds": cpu_s, "cpu_threads": threads,
+            "cpu_single_core_particle_steps_per_s": one_v,
+            "cpu_single_core_sample": "the first 200 steps (%.1f s)" % one_s}
+
+
+def cpu_baseline(state_buf, n, warm, steps, threads, nc=64):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from _oracle import Oracle
-    ora = Oracle(state_buf, threads=threads, nc=64)
+    ora = Oracle(state_buf, threads=threads, nc=nc)
     ora.step(warm)
     t0 = time.perf_counter()
     ora.step(steps)
@@ -333,6 +435,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps-1", type=int, default=2,
+                    help="timed steps of the one-thread CPU baseline")
+    ap.add_argument("--no-settling", action="store_true",
+                    help="skip the 10,000-grain settling leg of the CPU baseline")
+    ap.add_argument("--darcy-cpu-steps", type=int, default=5,
+                    help="fluid steps of the consolidation box on the CPU oracle")
     ap.add_argument("--skin", type=float, default=None,
                     help="neighbour-list skin as a fraction of the largest radius (0: cell search)")
     ap.add_argument("--separate-rebuild", action="store_true",
@@ -473,12 +581,21 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log("CPU oracle baseline: %d threads, 2 + %d steps" % (threads, args.cpu_steps))
+        threads = host_threads(args)
+        log("CPU oracle baseline: %d threads, 2 + %d steps; 1 thread, 1 + %d steps" %
+            (threads, args.cpu_steps, args.cpu_steps_1))
         cv, cdt = cpu_baseline(warm_buf, n, 2, args.cpu_steps, threads)
+        cv1, cdt1 = cpu_baseline(warm_buf, n, 1, args.cpu_steps_1, 1)
         cpu = {"value": cv, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": "the same 1M-grain shear box after the GPU warm-up: 2 untimed + %d timed "
-                         "steps of the FP64 C oracle (OpenMP, %.1f s)" % (args.cpu_steps, cdt)}
+                         "steps of the FP64 C oracle (OpenMP over %d threads, %.1f s)"
+                         % (args.cpu_steps, threads, cdt),
+               "single_core": {"value": cv1, "cores": 1,
+                               "sample": "1 untimed + %d timed steps on one thread (%.1f s)"
+                                         % (args.cpu_steps_1, cdt1)},
+               "host": host_info(threads)}
+        if not args.no_settling:
+            cpu["settling_10k"] = settling_leg(args, threads)
 
     line = {
         "metric": "particle-steps/s (1M grains, FP64); Darcy ms-to-1e-8, CNN-precond vs plain",

@@ -198,6 +198,9 @@ def load_library(path=None):
         "sph_net_backward": (ctypes.c_int, [vp, vp, vp, vp, vp]),
         "sph_net_conv3d": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_uint32,
                                           ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp]),
+        "sph_net_conv3d_noalloc": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, ctypes.c_int,
+                                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                  ctypes.c_uint32, vp, vp]),
         "sph_ensemble_create": (vp, [ctypes.POINTER(vp), ctypes.c_int]),
         "sph_ensemble_destroy": (None, [vp]),
         "sph_ensemble_last_error": (ctypes.c_char_p, [vp]),

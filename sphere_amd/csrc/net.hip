@@ -15,6 +15,7 @@
 #include "net.h"
 #include "net_kernels.h"
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -34,6 +35,40 @@ int pad_out(int c) { return (c + 15) / 16 * 16; }
 std::string g_net_error;
 
 }  // namespace
+
+// Scratch of sph_net_conv3d_noalloc: the BF16 copy of one call's input
+// followed by its packed weights.  Each GPU the library runs on holds its own
+// copy of this array until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in): it uses the
+// device address hipGetSymbolAddress returns, kept per device.
+#define NET_NOALLOC_BYTES (16u << 20)
+static __device__ __align__(256) unsigned char scratch[NET_NOALLOC_BYTES];
+
+#define NET_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_dev[NET_MAX_DEVICES];
+
+// The scratch address on the current device, resolved on the first call
+// there (nullptr with g_net_error set on failure).
+static unsigned char *noalloc_scratch()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NET_MAX_DEVICES) {
+        g_net_error = "no current HIP device";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            g_net_error = std::string("hipGetSymbolAddress of the scratch array: ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_scratch_dev[dev] = (unsigned char *)ptr;
+    }
+    return g_scratch_dev[dev];
+}
 
 struct sph_net {
     int device = 0;
@@ -327,6 +362,51 @@ int sph_net_conv3d(const float *in, int cin, const float *w, const float *b, int
     if (rc) {
         g_net_error = "unsupported shape";
         return SPH_ERR_VALUE;
+    }
+    return SPH_OK;
+}
+
+int sph_net_conv3d_noalloc(const float *in, int cin, const float *w, const float *b, int cout,
+                           uint32_t nx, uint32_t ny, uint32_t nz, uint32_t periodic, float *out,
+                           void *stream)
+{
+    if (!in || !w || !out || cin <= 0 || cout <= 0 || cout > 64 || nx == 0 || ny == 0 || nz == 0) {
+        g_net_error = "bad argument";
+        return SPH_ERR_ARG;
+    }
+    const int cpad = pad_in(cin), opad = pad_out(cout);
+    if (cpad > 64) {
+        g_net_error = "too many input channels";
+        return SPH_ERR_ARG;
+    }
+    const uint64_t V64 = (uint64_t)nx * ny * nz;
+    const uint64_t in_bytes = (V64 * (uint64_t)cpad * 2u + 255u) / 256u * 256u;
+    const uint64_t w_bytes = 27ull * (uint64_t)opad * (uint64_t)cpad * 2u;
+    if (V64 > 0xffffffffull || in_bytes + w_bytes > NET_NOALLOC_BYTES) {
+        g_net_error = "sph_net_conv3d_noalloc: the input (" + std::to_string(V64) + " voxels x " +
+                      std::to_string(cpad) + " channels) and weights need " +
+                      std::to_string(in_bytes + w_bytes) + " bytes of scratch; at most " +
+                      std::to_string(NET_NOALLOC_BYTES) + " are available";
+        return SPH_ERR_ARG;
+    }
+    unsigned char *base = noalloc_scratch();
+    if (!base) return SPH_ERR_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    const NetGeom g{nx, ny, nz, periodic & 3u};
+    uint16_t *inb = (uint16_t *)base;
+    uint16_t *wf = (uint16_t *)(base + in_bytes);
+    // the staging kernels write every element they reserve (padding as
+    // zeros), so nothing in the scratch needs clearing between calls
+    launch_to_bf16_pad(s, (uint32_t)V64, in, cin, cpad, inb);
+    launch_pack_weights(s, w, cin, cout, wf, cpad, opad, nullptr, 0, 0);
+    if (launch_conv_fwd(s, g, inb, cpad, wf, opad, b, cout, out, NET_OUT_F32)) {
+        g_net_error = "unsupported shape";
+        return SPH_ERR_VALUE;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_net_error = std::string("launch failed: ") + hipGetErrorString(e);
+        return SPH_ERR_HIP;
     }
     return SPH_OK;
 }
