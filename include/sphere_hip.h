@@ -250,6 +250,19 @@ int sph_read_contacts(sph_ctx *ctx, uint32_t *contacts, double *distmod,
 /* Restore contact history (a restart side car; absent from the v2.15 file). */
 int sph_write_contacts(sph_ctx *ctx, const uint32_t *contacts,
                        const double *distmod, const double *delta_t);
+/* Restart side car (no counterpart in the reference, whose restarts drop
+ * the contact history and restart TY3 with zero previous accelerations,
+ * src/file_io.cpp:420-731, src/integration.cuh:74-81): the occupied contact
+ * slots with their tangential springs, the previous accelerations of grains
+ * and walls, the step counter, the velocity-step state and the Darcy top
+ * pressure base.  sph_run writes <sid>.contacts.NNNNN.bin beside every output
+ * file and, when <input>.contacts.bin lies beside its input file and belongs
+ * to it (same time, output number and sizes), restores it, so that a run
+ * continued from an output file equals the uninterrupted run bit for bit.
+ * sph_read_restart needs the state it was written with to be loaded. */
+int sph_write_restart(sph_ctx *ctx, const char *path);
+int sph_read_restart(sph_ctx *ctx, const char *path);
+
 /* Number of occupied contact slots summed over all particles. */
 int sph_count_contacts(sph_ctx *ctx, uint64_t *ncontacts);
 

@@ -28,6 +28,10 @@
  *   D8  setDarcyTopPressure writes only x < nx, y < ny, z < nz cells
  *       (reference operator precedence, darcy.cuh:1206-1208).
  *   D9  cellEnd of empty cells is 0xffffffff (reference leaves it stale).
+ *   D10 the Jacobi residual field is reduced to its maximum, not stored.
+ *   D11 the ghost permeability of a bc-4 (prescribed flux) face is 0: the
+ *       reference never writes it (setDarcyGhostNodes handles bc 0-2,
+ *       darcy.cuh:194-264) in memory it does not clear (darcy.cuh:32).
  */
 #include "sphere_oracle.h"
 

@@ -185,6 +185,8 @@ def load_library(path=None):
         "sph_read_contacts": (ctypes.c_int, [vp, _pu, _pd, _pd]),
         "sph_write_contacts": (ctypes.c_int, [vp, _pu, _pd, _pd]),
         "sph_count_contacts": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "sph_write_restart": (ctypes.c_int, [vp, ctypes.c_char_p]),
+        "sph_read_restart": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "sph_get_darcy_stats": (ctypes.c_int, [vp, ctypes.POINTER(SphDarcyStats)]),
         "sph_set_profiling": (ctypes.c_int, [vp, ctypes.c_int]),
         "sph_kernel_time": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
@@ -317,6 +319,14 @@ class Engine:
         self._check(self.lib.sph_write_contacts(self.ctx, _ptr(contacts, ctypes.c_uint32),
                                                 _ptr(distmod, ctypes.c_double),
                                                 _ptr(delta_t, ctypes.c_double)))
+
+    def write_restart(self, path):
+        """Write the restart side car of the current state."""
+        self._check(self.lib.sph_write_restart(self.ctx, path.encode()))
+
+    def read_restart(self, path):
+        """Restore a side car written with the loaded state."""
+        self._check(self.lib.sph_read_restart(self.ctx, path.encode()))
 
     def count_contacts(self):
         n = ctypes.c_uint64(0)

@@ -903,6 +903,163 @@ static int darcy_step(sph_ctx *ctx, std::ofstream *convlog)
     return SPH_OK;
 }
 
+// ------------------------------------------------------------------
+// Restart side car (no reference counterpart).  The v2.15 file holds the
+// particle, wall, bond and fluid state but not the contact history, the
+// previous accelerations of the TY3 integrator or the time loop's counters,
+// so a run continued from an output file differs from the uninterrupted one
+// (src/file_io.cpp:420-731; TY3 restarts with acc0 = 0,
+// src/integration.cuh:74-81).  The side car, written beside each output
+// file, carries exactly those, and a run loaded from that file continues
+// bit for bit.
+// ------------------------------------------------------------------
+#define RESTART_VERSION 1u
+
+struct RestartHeader {
+    char magic[4];            // "SPHR"
+    uint32_t version;
+    double time_current;      // of the state file it belongs to
+    uint32_t step_count, np, nc, nw;
+    int64_t iter;             // steps since the run started (0: a fresh start)
+    int32_t velocity_state, pad;
+    double p_top_orig;        // the Darcy top pressure without its modulation
+    uint64_t occupied;        // occupied contact slots that follow
+};
+// followed by: w_acc[nw], acc[np*3], angacc[np*3] (FP64), cmask[np] (u64),
+// then per occupied slot in particle and slot order the contact word (u32),
+// and then their tangential springs (3 FP64 each)
+
+static int write_restart(sph_ctx *ctx, const std::string &path)
+{
+    const sph_state &h = ctx->hs.h;
+    const size_t np = ctx->np, nc = ctx->nc;
+    int rc = sync(ctx);
+    if (rc) return rc;
+    RestartHeader H{};
+    std::memcpy(H.magic, "SPHR", 4);
+    H.version = RESTART_VERSION;
+    H.time_current = h.time_current;
+    H.step_count = h.time_step_count;
+    H.np = (uint32_t)np;
+    H.nc = (uint32_t)nc;
+    H.nw = h.nw;
+    H.iter = ctx->iter;
+    H.velocity_state = ctx->velocity_state;
+    H.p_top_orig = ctx->p_top_orig;
+    std::vector<double> wacc(MAXW), acc(np * 3), angacc(np * 3);
+    std::vector<uint64_t> mask(np);
+    std::vector<uint32_t> rows(np * nc);
+    std::vector<double> dts(np * nc * 3);
+    HIPCHK(hipMemcpy(wacc.data(), ctx->db.w_acc, MAXW * 8, hipMemcpyDeviceToHost));
+    if (np) {
+        HIPCHK(hipMemcpy(acc.data(), ctx->db.acc, np * 24, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(angacc.data(), ctx->db.angacc, np * 24, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(mask.data(), ctx->db.cmask, np * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(rows.data(), ctx->db.contacts, np * nc * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(dts.data(), ctx->db.delta_t, np * nc * 24, hipMemcpyDeviceToHost));
+    }
+    std::vector<uint32_t> words;
+    std::vector<double> springs;
+    for (size_t a = 0; a < np; ++a)
+        for (size_t k = 0; k < nc; ++k)
+            if ((mask[a] >> k) & 1ull) {
+                words.push_back(rows[a * nc + k]);
+                springs.insert(springs.end(), &dts[(a * nc + k) * 3], &dts[(a * nc + k) * 3] + 3);
+            }
+    H.occupied = words.size();
+    FILE *fp = fopen(path.c_str(), "wb");
+    if (!fp) return fail(ctx, SPH_ERR_FILE, "cannot write " + path);
+    bool ok = fwrite(&H, sizeof(H), 1, fp) == 1;
+    ok = ok && fwrite(wacc.data(), 8, h.nw, fp) == h.nw;
+    ok = ok && fwrite(acc.data(), 8, np * 3, fp) == np * 3;
+    ok = ok && fwrite(angacc.data(), 8, np * 3, fp) == np * 3;
+    ok = ok && fwrite(mask.data(), 8, np, fp) == np;
+    ok = ok && fwrite(words.data(), 4, words.size(), fp) == words.size();
+    ok = ok && fwrite(springs.data(), 8, springs.size(), fp) == springs.size();
+    ok = (fclose(fp) == 0) && ok;
+    return ok ? SPH_OK : fail(ctx, SPH_ERR_FILE, "short write of " + path);
+}
+
+static int peek_restart(const std::string &path, RestartHeader &H)
+{
+    FILE *fp = fopen(path.c_str(), "rb");
+    if (!fp) return SPH_ERR_FILE;
+    const bool ok = fread(&H, sizeof(H), 1, fp) == 1;
+    fclose(fp);
+    if (!ok || std::memcmp(H.magic, "SPHR", 4) != 0 || H.version != RESTART_VERSION)
+        return SPH_ERR_FILE;
+    return SPH_OK;
+}
+
+// Restores a side car into the loaded state, which must be the state file it
+// was written with (same time, output number, grain and wall counts and slot
+// count).
+static int read_restart(sph_ctx *ctx, const std::string &path)
+{
+    const sph_state &h = ctx->hs.h;
+    RestartHeader H{};
+    if (peek_restart(path, H)) return fail(ctx, SPH_ERR_FILE, path + " is not a restart side car");
+    if (H.np != ctx->np || H.nc != ctx->nc || H.nw != h.nw || H.time_current != h.time_current ||
+        H.step_count != h.time_step_count)
+        return fail(ctx, SPH_ERR_VALUE, path + " belongs to another state (time " +
+                                            std::to_string(H.time_current) + ", output " +
+                                            std::to_string(H.step_count) + ", " +
+                                            std::to_string(H.np) + " grains, " +
+                                            std::to_string(H.nc) + " slots)");
+    const size_t np = ctx->np, nc = ctx->nc;
+    std::vector<double> wacc(MAXW, 0.0), acc(np * 3), angacc(np * 3);
+    std::vector<uint64_t> mask(np);
+    std::vector<uint32_t> words(H.occupied);
+    std::vector<double> springs(H.occupied * 3);
+    FILE *fp = fopen(path.c_str(), "rb");
+    if (!fp) return fail(ctx, SPH_ERR_FILE, "cannot read " + path);
+    bool ok = fseek(fp, (long)sizeof(H), SEEK_SET) == 0;
+    ok = ok && fread(wacc.data(), 8, h.nw, fp) == h.nw;
+    ok = ok && fread(acc.data(), 8, np * 3, fp) == np * 3;
+    ok = ok && fread(angacc.data(), 8, np * 3, fp) == np * 3;
+    ok = ok && fread(mask.data(), 8, np, fp) == np;
+    ok = ok && fread(words.data(), 4, words.size(), fp) == words.size();
+    ok = ok && fread(springs.data(), 8, springs.size(), fp) == springs.size();
+    fclose(fp);
+    if (!ok) return fail(ctx, SPH_ERR_FILE, path + " is truncated");
+    std::vector<uint32_t> rows(np * nc, (uint32_t)np);
+    std::vector<double> dts(np * nc * 3, 0.0);
+    size_t j = 0;
+    for (size_t a = 0; a < np; ++a)
+        for (size_t k = 0; k < nc; ++k)
+            if ((mask[a] >> k) & 1ull) {
+                if (j >= words.size() || (words[j] & CT_IDX_MASK) >= np)
+                    return fail(ctx, SPH_ERR_FILE, path + ": inconsistent contact slots");
+                rows[a * nc + k] = words[j];
+                dts[(a * nc + k) * 3] = springs[3 * j];
+                dts[(a * nc + k) * 3 + 1] = springs[3 * j + 1];
+                dts[(a * nc + k) * 3 + 2] = springs[3 * j + 2];
+                ++j;
+            }
+    if (j != words.size()) return fail(ctx, SPH_ERR_FILE, path + ": inconsistent contact slots");
+    hipStream_t s = ctx->stream;
+    int rc = sync(ctx);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(ctx->db.w_acc, wacc.data(), MAXW * 8, hipMemcpyHostToDevice));
+    if (np) {
+        HIPCHK(hipMemcpy(ctx->db.acc, acc.data(), np * 24, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->db.angacc, angacc.data(), np * 24, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->db.contacts, rows.data(), np * nc * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->db.delta_t, dts.data(), np * nc * 24, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->db.cmask, mask.data(), np * 8, hipMemcpyHostToDevice));
+    }
+    // the device copy of the step counter and time (read by the kernels)
+    const StepClock c{(long long)H.iter, h.time_current};
+    HIPCHK(hipMemcpyAsync(ctx->db.clock, &c, sizeof(c), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->iter = (long)H.iter;
+    ctx->velocity_state = H.velocity_state;
+    ctx->p_top_orig = H.p_top_orig;
+    ctx->nl_force = 1;  // list and slot maps are rebuilt from the restored rows
+    ctx->hash_ready = false;
+    return SPH_OK;
+}
+
 // Output event (src/device.cu:2576-2680).  With dir == nullptr the state
 // effects are applied without writing files.
 static int output_event(sph_ctx *ctx, const std::string *dir, const std::string &sid)
@@ -919,6 +1076,9 @@ static int output_event(sph_ctx *ctx, const std::string *dir, const std::string 
         std::string err;
         rc = write_state_file((*dir + "/" + sid + name).c_str(), ctx->hs, err);
         if (rc) return fail(ctx, rc, err);
+        snprintf(name, sizeof(name), ".contacts.%05u.bin", h.time_step_count);
+        rc = write_restart(ctx, *dir + "/" + sid + name);
+        if (rc) return rc;
         FILE *fp = fopen((*dir + "/" + sid + ".status.dat").c_str(), "w");
         if (fp) {
             fprintf(fp, "%2.4e %2.4e %d\n", h.time_current, 100.0 * h.time_current / h.time_total,
@@ -1344,6 +1504,13 @@ int sph_run(const sph_run_opts *opts)
     std::string dir = opts->output_dir ? opts->output_dir : "output";
     sph_ctx *ctx = sph_create(opts->device);
     if (!ctx) return SPH_ERR_HIP;
+    // a restart side car beside the input (<input>.bin -> <input>.contacts.bin)
+    std::string side;
+    if (in.size() > 4 && in.compare(in.size() - 4, 4, ".bin") == 0)
+        side = in.substr(0, in.size() - 4) + ".contacts.bin";
+    RestartHeader rh{};
+    const bool have_side = !side.empty() && peek_restart(side, rh) == SPH_OK;
+    if (have_side && (rh.nc == 32 || rh.nc == 64)) ctx->nc = rh.nc;
     int rc = sph_load_file(ctx, opts->input_path, opts->fluid);
     if (rc) {
         g_last_error = ctx->err;
@@ -1351,6 +1518,16 @@ int sph_run(const sph_run_opts *opts)
         return rc;
     }
     sph_state &h = ctx->hs.h;
+    if (have_side) {
+        rc = read_restart(ctx, side);
+        if (rc == SPH_ERR_VALUE) {
+            fprintf(stderr, "Warning: ignoring %s: %s\n", side.c_str(), ctx->err.c_str());
+        } else if (rc) {
+            g_last_error = ctx->err;
+            sph_destroy(ctx);
+            return rc;
+        }
+    }
     if (opts->dry) {  // reportValues, src/sphere.cpp:349-465
         printf("  - Number of particles: %u\n  - Time step length: %g s\n  - Total time: %g s\n"
                "  - File output interval: %g s\n  - Grid: %u*%u*%u, periodic %d\n  - Contact model: %u\n",
@@ -1393,6 +1570,24 @@ int sph_run(const sph_run_opts *opts)
     if (rc) g_last_error = ctx->err;
     sph_destroy(ctx);
     return rc;
+}
+
+int sph_write_restart(sph_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
+    (void)hipSetDevice(ctx->device);
+    int rc = download(ctx, false, false);  // the header's time and output number
+    if (rc) return rc;
+    return write_restart(ctx, path);
+}
+
+int sph_read_restart(sph_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return fail(ctx, SPH_ERR_ARG, "null argument");
+    if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
+    (void)hipSetDevice(ctx->device);
+    return read_restart(ctx, path);
 }
 
 int sph_read_sort(sph_ctx *ctx, uint32_t *cellid, uint32_t *index, uint32_t *cell_start,

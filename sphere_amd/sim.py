@@ -12,6 +12,8 @@ libsphere_hip.so, which writes ``../output/<sid>.outputNNNNN.bin``,
 import ctypes
 import math
 import os
+import re
+import shutil
 
 import numpy
 
@@ -23,6 +25,17 @@ VERSION = 2.15  # src/version.h:5
 def V_sphere(r):
     """Volume of a sphere (python/sphere.py:7762-7769)."""
     return 4.0 / 3.0 * math.pi * r**3.0
+
+
+def _restart_side_car(path):
+    """The restart side car written beside an output file
+    (<sid>.outputNNNNN.bin -> <sid>.contacts.NNNNN.bin), or None."""
+    head, name = os.path.split(path)
+    m = re.match(r"^(.*)\.output(\d{5})\.bin$", name)
+    if not m:
+        return None
+    side = os.path.join(head, "%s.contacts.%s.bin" % (m.group(1), m.group(2)))
+    return side if os.path.exists(side) else None
 
 
 def status(project):
@@ -260,6 +273,7 @@ class sim:
             print("Input file: {0}".format(targetbin))
         with open(targetbin, "rb") as fh:
             r = _Reader(fh.read())
+        self._restart = _restart_side_car(targetbin)
         try:
             self.version = r.take(numpy.float64)
             self.nd = r.take(numpy.int32)
@@ -1484,6 +1498,14 @@ class sim:
         """Write ../input/<sid>.bin and run it through libsphere_hip.so, with
         outputs in ../output (the reference's ``cd ..; ./sphere``)."""
         self.writebin(verbose=False)
+        # a restart side car of the state read last goes beside the input
+        # file, where the run looks for it; a stale one is removed
+        side = "../input/" + self.sid + ".contacts.bin"
+        src = getattr(self, "_restart", None)
+        if src and os.path.exists(src):
+            shutil.copyfile(src, side)
+        elif os.path.exists(side):
+            os.remove(side)
         if valgrind or cudamemcheck:
             print("Warning: valgrind/cuda-memcheck wrappers are not available; running plainly")
         lib = _abi.load_library()

@@ -409,3 +409,40 @@ def test_darcy_forcing_of_two_approaching_grains():
     want = -dphi / (1e-8 * phi1 * (1.0 - phi1))
     assert want > 0.0  # compression raises the pore pressure
     assert abs(expl - want) < 1e-9 * want
+
+
+@pytest.mark.parametrize("face", ["top", "bottom"])
+def test_darcy_flux_boundary(face):
+    """bc 4 (prescribed specific flux q, src/darcy.cuh:270-312): the ghost
+    pressure p - (mu / k) q dz makes the boundary cell's Darcy velocity
+    -(k/mu)(p_ghost - p_neighbour)/(2 dz)/phi = q / (2 phi) (the central
+    difference spans two cells).  The reference never writes the ghost
+    permeability of a bc-4 face (setDarcyGhostNodes handles bc 0-2), so the
+    face conductance 2 k_g k / (k_g + k) is that of a zero ghost and the
+    pressure field does not change (declared: the ghost is held at 0)."""
+    q = 1e-3
+    s = sphere_amd.sim(np=0, nd=3, nw=0, sid="flux", fluid=True)
+    s.defaultParams()
+    s.defineWorldBoundaries([0.4, 0.4, 0.8], dx=0.1)
+    s.initFluid(mu=1e-3, cfd_solver=1)
+    s.setFluidCompressibility(1e-8)
+    s.setPermeabilityPrefactor(1e-12, verbose=False)
+    if face == "top":
+        s.setFluidBottomNoFlow()
+        s.setFluidTopFixedFlux(q)
+    else:
+        s.setFluidTopNoFlow()
+        s.setFluidBottomFixedFlux(q)
+    s.initTemporal(total=1.0, file_dt=1.0, dt=1e-4)
+    o = Oracle(s.to_state())
+    o.step(2)
+    st = o.read()
+    v = st.get("v_f", 4 * 4 * 8, 3).reshape(8, 4, 4, 3)
+    p = st.get("p_f", 4 * 4 * 8, 1)
+    phi = 0.9  # no grains
+    layer = 7 if face == "top" else 0
+    want = q / (2.0 * phi) if face == "top" else -q / (2.0 * phi)
+    assert numpy.allclose(v[layer, :, :, 2], want, rtol=1e-12, atol=0)
+    others = numpy.delete(v[:, :, :, 2], layer, axis=0)
+    assert not others.any() and not v[:, :, :, :2].any()
+    assert not p.any()
