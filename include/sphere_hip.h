@@ -310,6 +310,23 @@ int sph_net_conv3d(const float *in, int cin, const float *w, const float *b, int
                    uint32_t nx, uint32_t ny, uint32_t nz, uint32_t periodic, float *out,
                    void *stream);
 
+/* The convolution of sph_net_conv3d without any device allocation: no
+ * counterpart in the reference.  It is for callers that cannot allocate --
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  The BF16 staging of the input and weights goes to a
+ * library-owned scratch array of 16 MiB per GPU (a static device array each
+ * GPU holds until the process ends), so V * pad(cin) * 2 bytes (rounded up to
+ * 256) plus 27 * pad(cout) * pad(cin) * 2 bytes must fit in 16 MiB, with
+ * pad(c) = 4 for c <= 4, else c rounded up to 16; larger calls return
+ * SPH_ERR_ARG.  The work is enqueued on `stream` and the call returns
+ * without waiting for it.  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (different streams, or host threads) race.
+ * The first call on a device resolves the scratch address; make it outside
+ * graph capture. */
+int sph_net_conv3d_noalloc(const float *in, int cin, const float *w, const float *b, int cout,
+                           uint32_t nx, uint32_t ny, uint32_t nz, uint32_t periodic, float *out,
+                           void *stream);
+
 /* Use the network in the context's Darcy solves: params on the host
  * (count = sph_net_param_count()), at most max_corrections per solve;
  * params == NULL or max_corrections == 0 switches it off. */
