@@ -37,11 +37,13 @@ def threads():
 
 
 def run_compare(eng, s, chunks, per_chunk, nc=32, fluid=False, label="", sort=False,
-                contacts=True):
+                contacts=True, each=None):
     eng.load(s.to_state())
     ora = Oracle(s.to_state(), threads=threads(), nc=nc)
     done, drift = 0, 0.0
     for _ in range(chunks):
+        if each is not None and done == per_chunk:
+            each(ora)
         eng.step(per_chunk)
         ora.step(per_chunk)
         done += per_chunk
@@ -175,11 +177,25 @@ def test_grains_pressed_on_walls(wmode, mode):
     them)."""
     s = pressed_case(n=1000, seed=wmode, wmode=wmode)
     eng = sphere_amd.Engine(0, skin=None if mode == "lists" else 0.0)
-    ora = run_compare(eng, s, 4, 25, label="pressed wmode %d (%s)" % (wmode, mode))
-    h = ora.read().state
-    assert h.w_mvfd[2] != 0.0
+    seen = {}
+
+    def after_first_chunk(ora):
+        st = ora.read()
+        seen["wall_force"] = st.state.w_mvfd[2]
+        f = st.get("force", 1000, 3)
+        x = st.get("x", 1000, 4)
+        top = x[:, 2] + x[:, 3] > s.w_x[0]
+        seen["top_push"] = float(numpy.max(-f[top, 2])) if top.any() else 0.0
+
+    ora = run_compare(eng, s, 4, 25, label="pressed wmode %d (%s)" % (wmode, mode),
+                      each=after_first_chunk)
+    # the top grains are pushed down by the wall; a moving wall integrates
+    # the (non-zero) sum of the grains' wall forces (a fixed wall, mode 0,
+    # keeps its force entry, as in the reference)
+    assert seen["top_push"] > 0.0
     if wmode != 0:
-        assert h.w_nx[3] != s.w_x[0]  # the wall moved
+        assert seen["wall_force"] != 0.0
+        assert ora.read().state.w_nx[3] != s.w_x[0]  # the wall moved
     eng.close()
 
 
