@@ -1491,6 +1491,113 @@ class sim:
         return self.shearVelocity() / self.w_x[0]
 
     # ------------------------------------------------------------------
+    # VTK output for ParaView (python/sphere.py:1555-2191).  ASCII XML files;
+    # the array names are the reference's, which ParaView state files use.
+    # ------------------------------------------------------------------
+    def _vtk_point_arrays(self):
+        """(name, VTK type, values) of the grain arrays, in the reference's
+        order; fluid pressure forces with the Darcy solver."""
+        arrays = [("Diameter", "Float32", 2.0 * self.radius),
+                  ("Displacement [m]", "Float32", self.xyzsum),
+                  ("Velocity [m/s]", "Float32", self.vel)]
+        if self.fluid and int(numpy.asarray(self.cfd_solver).ravel()[0]) == 1:
+            arrays.append(("Fluid pressure force [N]", "Float32", self.f_p))
+        arrays += [("FixedVel", "Float32", self.fixvel), ("Force [N]", "Float32", self.force),
+                   ("Angular position[rad]", "Float32", self.angpos),
+                   ("Angular velocity [rad/s]", "Float32", self.angvel),
+                   ("Torque [Nm]", "Float32", self.torque),
+                   ("Shear Energy Rate [J/s]", "Float32", self.es_dot),
+                   ("Shear Energy [J]", "Float32", self.es),
+                   ("Viscous Energy Rate [J/s]", "Float32", self.ev_dot),
+                   ("Viscous Energy [J]", "Float32", self.ev),
+                   ("Pressure [Pa]", "Float32", self.p),
+                   ("Type color", "Int32", self.color)]
+        return arrays
+
+    @staticmethod
+    def _vtk_data(fh, name, vtype, values, indent="        "):
+        a = numpy.asarray(values)
+        comps = a.shape[1] if a.ndim == 2 else 1
+        fh.write('%s<DataArray type="%s" Name="%s"%s format="ascii">\n' % (
+            indent, vtype, name, ' NumberOfComponents="%d"' % comps if comps > 1 else ""))
+        fmt = "%d" if vtype.startswith("Int") or vtype.startswith("UInt") else "%f"
+        fh.write(indent + "  " + " ".join(fmt % v for v in a.ravel()) + "\n")
+        fh.write("%s</DataArray>\n" % indent)
+
+    def writeVTK(self, folder='../output/', verbose=True):
+        """Grains as the points of an unstructured grid, <folder>/<sid>.vtu
+        (python/sphere.py:1646-1998)."""
+        path = folder + '/' + self.sid + '.vtu'
+        if verbose:
+            print('Output file: ' + path)
+        n = int(self.np[0])
+        with open(path, 'w') as fh:
+            fh.write('<?xml version="1.0"?>\n<VTKFile type="UnstructuredGrid" version="0.1" '
+                     'byte_order="LittleEndian">\n  <UnstructuredGrid>\n'
+                     '    <Piece NumberOfPoints="%d" NumberOfCells="0">\n      <Points>\n' % n)
+            self._vtk_data(fh, "Position [m]", "Float32", self.x)
+            fh.write('      </Points>\n      <PointData Scalars="Diameter [m]" Vectors="vector">\n')
+            for name, vtype, values in self._vtk_point_arrays():
+                self._vtk_data(fh, name, vtype, values)
+            fh.write('      </PointData>\n      <Cells>\n')
+            for name, vtype in (("connectivity", "Int32"), ("offsets", "Int32"), ("types", "UInt8")):
+                self._vtk_data(fh, name, vtype, numpy.zeros(0, dtype=numpy.int32))
+            fh.write('      </Cells>\n    </Piece>\n  </UnstructuredGrid>\n</VTKFile>')
+
+    def writeFluidVTK(self, folder='../output/', cell_centered=True, verbose=True):
+        """The fluid grid as image data, <folder>/fluid-<sid>.vti: pressure,
+        velocity, porosity, porosity change and, with the Darcy solver, the
+        permeability (the Python helper's form) and the fixed-pressure flags;
+        values on the cells (cell_centered) or on points at the cell centres
+        (python/sphere.py:2000-2191)."""
+        path = folder + '/fluid-' + self.sid + '.vti'
+        if verbose:
+            print('Output file: ' + path)
+        v_f, p_f, phi, dphi, p_c = self._grid_cells()
+        dx = (self.L - self.origo) / self.num
+        if cell_centered:
+            origin, dims, where = self.origo, self.num + 1, "CellData"
+        else:
+            origin, dims, where = self.origo + 0.5 * dx, self.num, "PointData"
+        ext = " ".join("0 %d" % (int(d) - 1) for d in dims)
+
+        def flat(a):  # x fastest, as VTK orders image data
+            a = numpy.asarray(a)
+            return a.transpose(2, 1, 0, 3).reshape(-1, a.shape[3]) if a.ndim == 4 else \
+                a.transpose(2, 1, 0).ravel()
+        arrays = [("Pressure [Pa]", p_f), ("Velocity [m/s]", v_f), ("Porosity [-]", phi),
+                  ("Porosity change [1/s]", dphi)]
+        if int(numpy.asarray(self.cfd_solver).ravel()[0]) == 1:
+            phic = numpy.clip(phi, 0.1, 0.9)
+            arrays += [("Permeability [m*m]", self.k_c[0] * phic ** 3 / (1.0 - phic ** 2)),
+                       ("Constant pressure [-]", p_c)]
+        with open(path, 'w') as fh:
+            fh.write('<?xml version="1.0"?>\n<VTKFile type="ImageData" version="0.1" '
+                     'byte_order="LittleEndian">\n  <ImageData WholeExtent="%s" Origin="%s" '
+                     'Spacing="%s">\n    <Piece Extent="%s">\n      <%s>\n' % (
+                         ext, " ".join("%r" % float(o) for o in origin),
+                         " ".join("%r" % float(d) for d in dx), ext, where))
+            for name, values in arrays:
+                self._vtk_data(fh, name, "Float64", flat(values))
+            fh.write('      </%s>\n    </Piece>\n  </ImageData>\n</VTKFile>' % where)
+
+    def writeVTKall(self, cell_centered=True, verbose=True, forces=False):
+        """VTK files of every output file of this simulation, 0 to the last
+        (status): <sid>.NNNNN.vtu for the grains and fluid-<sid>.NNNNN.vti for
+        the fluid (python/sphere.py:1555-1644).  Contact-force files need the
+        vtk module in the reference and are not written."""
+        last = status(self.sid)
+        for i in range(last + 1):
+            sb = sim(fluid=self.fluid)
+            sb.readbin("../output/{0}.output{1:0=5}.bin".format(self.sid, i), verbose=False)
+            sb.sid = self.sid + ".{:0=5}".format(i)
+            loud = verbose and i in (0, last)
+            if sb.np[0] > 0:
+                sb.writeVTK(verbose=loud)
+            if self.fluid:
+                sb.writeFluidVTK(verbose=loud, cell_centered=cell_centered)
+
+    # ------------------------------------------------------------------
     # execution, python/sphere.py:4482-4540
     # ------------------------------------------------------------------
     def run(self, verbose=True, hideinputfile=False, dry=False, valgrind=False,

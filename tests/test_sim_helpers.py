@@ -268,3 +268,36 @@ def test_add_and_delete_particles():
     assert not s.angvel[2].any() and s.xyzsum.shape == (3, 3)
     s.deleteParticle([0, 1])
     assert s.np[0] == 1 and s.x.shape == (1, 3) and s.radius[0] == 0.5
+
+
+def test_vtk_files(tmp_path):
+    """writeVTK / writeFluidVTK / writeVTKall write well-formed VTK XML with
+    the grains' and cells' values (read back with the XML parser)."""
+    import xml.etree.ElementTree as ET
+    for d in ("input", "output", "work"):
+        (tmp_path / d).mkdir()
+    s = packing(n=20)
+    s.sid = "vtk"
+    s.time_dt[0] = 1e-6
+    s.initFluid(cfd_solver=1)
+    s.p_f[:] = numpy.arange(s.p_f.size).reshape(s.p_f.shape)
+    s.writebin(folder=str(tmp_path / "output"), verbose=False)
+    import os
+    os.rename(str(tmp_path / "output" / "vtk.bin"), str(tmp_path / "output" / "vtk.output00000.bin"))
+    (tmp_path / "output" / "vtk.status.dat").write_text("0.0 0.0 0\n")
+    old = os.getcwd()
+    os.chdir(str(tmp_path / "work"))
+    try:
+        s.writeVTKall(verbose=False)
+    finally:
+        os.chdir(old)
+    root = ET.parse(str(tmp_path / "output" / "vtk.00000.vtu")).getroot()
+    arrays = {a.get("Name"): a for a in root.iter("DataArray")}
+    pos = numpy.array(arrays["Position [m]"].text.split(), float).reshape(-1, 3)
+    assert numpy.allclose(pos, s.x, atol=1e-6)
+    assert numpy.allclose(numpy.array(arrays["Diameter"].text.split(), float), 2 * s.radius, atol=1e-6)
+    assert "Fluid pressure force [N]" in arrays and "Type color" in arrays
+    img = ET.parse(str(tmp_path / "output" / "fluid-vtk.00000.vti")).getroot()
+    p = {a.get("Name"): a for a in img.iter("DataArray")}["Pressure [Pa]"]
+    vals = numpy.array(p.text.split(), float)
+    assert numpy.array_equal(vals, s.p_f.transpose(2, 1, 0).ravel())
