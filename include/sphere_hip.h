@@ -348,13 +348,20 @@ int sph_darcy_net_input(sph_ctx *ctx, const double *p, float *input, double *sca
 
 /* ------------------------------------------------------------------------
  * Ensembles of independent boxes (no counterpart in the reference, which
- * runs one simulation per process).  The boxes are contexts loaded as usual
- * (any sizes and parameters, one device); sph_ensemble_step advances every
- * box by nsteps steps, enqueuing all boxes' kernels on their own streams
- * without waiting in between, and returns when all are done.  Each box's
- * results equal those of sph_step on that box alone.  Boxes with fluid, an
- * adaptive grid or an output event inside the steps are refused
- * (SPH_ERR_VALUE).
+ * runs one simulation per process and gets its parallel runs from job
+ * scripts, python/sphere.py:4549-4692,7638-7720).  The boxes are contexts
+ * loaded as usual (any sizes and parameters, one device); sph_ensemble_step
+ * advances every box by nsteps steps and returns when all are done.
+ *
+ * Boxes on the neighbour-list path without fluid (contact models 2 and 3,
+ * no adaptive grid, no shear-stress wall) are stepped together: each stage
+ * of a step is one kernel launch for all of them, with per-box task
+ * records in device memory, and pairs of steps are captured in one HIP
+ * graph and replayed.  Other boxes are stepped box by box on their own
+ * streams beside them.  Either way each box's results equal those of
+ * sph_step on that box alone, bit for bit.  Boxes with fluid, or an output
+ * event or the velocity step of t = 10 s / 15 s inside the steps, are
+ * refused (SPH_ERR_VALUE).
  * ---------------------------------------------------------------------- */
 typedef struct sph_ensemble sph_ensemble;
 
@@ -364,6 +371,29 @@ const char *sph_ensemble_last_error(const sph_ensemble *ens);
 int sph_ensemble_step(sph_ensemble *ens, int nsteps);
 /* steps taken so far by every box */
 int sph_ensemble_stats(const sph_ensemble *ens, uint64_t *steps);
+
+typedef struct {
+    uint32_t boxes;              /* boxes in the ensemble                      */
+    uint32_t multi_boxes;        /* of them stepped by the multi-box launches  */
+    uint32_t launches_per_step;  /* kernel launches per step for those boxes   */
+    uint32_t graph;              /* 1: a captured two-step graph is in use     */
+    uint64_t multi_steps;        /* steps taken by the multi-box launches      */
+    uint64_t graph_replays;      /* replays of the captured graph (2 steps)    */
+} sph_ensemble_info_t;
+int sph_ensemble_info(const sph_ensemble *ens, sph_ensemble_info_t *out);
+/* on = 0: enqueue the multi-box launches step by step instead of replaying
+ * a captured graph (same results). */
+int sph_ensemble_set_graph(sph_ensemble *ens, int on);
+/* Autotuner of each box's neighbour-list skin (fraction of its largest
+ * radius): for each of the nfrac fractions, every box runs trial_steps
+ * steps with it, timed with HIP events on its stream; each box keeps its
+ * fastest.  The lists are exact for any skin, so the trial steps are
+ * ordinary steps of the run (every box advances by nfrac * trial_steps
+ * steps) and the results do not depend on the choice.  chosen[box]
+ * (0 for a box without lists) and ms_per_step[box * nfrac + f] (-1 where
+ * the fraction does not fit the box's grid) may be NULL. */
+int sph_ensemble_autotune(sph_ensemble *ens, int trial_steps, const double *fractions,
+                          int nfrac, double *chosen, double *ms_per_step);
 
 #ifdef __cplusplus
 }

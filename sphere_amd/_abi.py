@@ -154,6 +154,12 @@ class StateBuffers:
 _lib = None
 
 
+class SphEnsembleInfo(ctypes.Structure):
+    _fields_ = [("boxes", ctypes.c_uint32), ("multi_boxes", ctypes.c_uint32),
+                ("launches_per_step", ctypes.c_uint32), ("graph", ctypes.c_uint32),
+                ("multi_steps", ctypes.c_uint64), ("graph_replays", ctypes.c_uint64)]
+
+
 def load_library(path=None):
     """Load libsphere_hip.so; raises if it was not built."""
     global _lib
@@ -208,6 +214,9 @@ def load_library(path=None):
         "sph_ensemble_last_error": (ctypes.c_char_p, [vp]),
         "sph_ensemble_step": (ctypes.c_int, [vp, ctypes.c_int]),
         "sph_ensemble_stats": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "sph_ensemble_info": (ctypes.c_int, [vp, ctypes.POINTER(SphEnsembleInfo)]),
+        "sph_ensemble_set_graph": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sph_ensemble_autotune": (ctypes.c_int, [vp, ctypes.c_int, _pd, ctypes.c_int, _pd, _pd]),
         "sph_set_darcy_net": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_uint64,
                                              ctypes.c_int]),
         "sph_darcy_export": (ctypes.c_int, [vp, ctypes.c_int, vp]),
@@ -458,8 +467,9 @@ class Net:
 
 class Ensemble:
     """Independent boxes (Engines with loaded states, one device) stepped
-    together: all boxes' kernels are enqueued on their own streams before
-    the host waits."""
+    together: boxes on the list path without fluid share one kernel launch
+    per stage (task records per box, pairs of steps replayed from a captured
+    HIP graph); the others are enqueued box by box on their own streams."""
 
     def __init__(self, engines, lib=None):
         self.lib = lib or load_library()
@@ -478,6 +488,30 @@ class Ensemble:
         a = ctypes.c_uint64(0)
         self.lib.sph_ensemble_stats(self.h, ctypes.byref(a))
         return {"steps": a.value}
+
+    def info(self):
+        i = SphEnsembleInfo()
+        self.lib.sph_ensemble_info(self.h, ctypes.byref(i))
+        return {k: getattr(i, k) for k, _ in SphEnsembleInfo._fields_}
+
+    def set_graph(self, on):
+        self.lib.sph_ensemble_set_graph(self.h, 1 if on else 0)
+
+    def autotune(self, trial_steps, fractions):
+        """Each box's list skin (fraction of its largest radius) chosen by
+        timing trial_steps steps with each fraction; every box advances by
+        len(fractions) * trial_steps steps.  Returns (chosen per box, ms per
+        step [box][fraction], -1 where a fraction does not fit)."""
+        f = numpy.ascontiguousarray(fractions, dtype=numpy.float64)
+        nb = len(self.engines)
+        chosen = numpy.zeros(nb)
+        ms = numpy.zeros((nb, len(f)))
+        rc = self.lib.sph_ensemble_autotune(self.h, int(trial_steps), _ptr(f, ctypes.c_double),
+                                            len(f), _ptr(chosen, ctypes.c_double),
+                                            _ptr(ms, ctypes.c_double))
+        if rc != SPH_OK:
+            raise SphereError(rc, self.lib.sph_ensemble_last_error(self.h).decode())
+        return chosen, ms
 
     def close(self):
         if self.h:

@@ -556,6 +556,29 @@ __device__ void nl_build_one(
 }
 
 
+__device__ __forceinline__ void nl_build_body(
+    const StepArgs &a, const NLArgs &n, const uint32_t *__restrict__ cond,
+    const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
+    const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x_s,
+    uint32_t *__restrict__ nl, uint32_t *__restrict__ nl_cnt, uint32_t *__restrict__ nl_pos,
+    double4 *__restrict__ x_ref, uint32_t *__restrict__ flags,
+    const uint32_t *__restrict__ contacts, const uint64_t *__restrict__ cmask,
+    uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand, float *__restrict__ nl_gap,
+    float *__restrict__ nl_gmin, const uint32_t *__restrict__ err, uint32_t bid, uint32_t nblk)
+{
+    SKIP_UNLESS(cond);
+    if ((*err & ERRF_BARRIER) != 0u) return;  // the sort before it did not finish
+    const uint32_t np = a.np;
+    if (bid == 0 && threadIdx.x == 0) {
+        flags[NLF_COUNT] += 1u;  // rebuild counter
+    }
+    if (bid == 0 && threadIdx.x < NLF_DSLOTS)  // displacements restart
+        ((unsigned long long *)(flags + NLF_DMAX))[threadIdx.x] = 0ull;
+    for (uint32_t pos = bid * blockDim.x + threadIdx.x; pos < np; pos += nblk * blockDim.x)
+        nl_build_one(a, n, pos, vals, cell_start, cell_end, x_s, nl, nl_cnt, nl_pos, x_ref, flags,
+                     contacts, cmask, nl_slot, slot_cand, nl_gap, nl_gmin);
+}
+
 __global__ void __launch_bounds__(256) k_nl_build(
     StepArgs a, NLArgs n, const uint32_t *__restrict__ cond, const uint32_t *__restrict__ vals,
     const uint32_t *__restrict__ cell_start, const uint32_t *__restrict__ cell_end,
@@ -565,29 +588,19 @@ __global__ void __launch_bounds__(256) k_nl_build(
     uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand, float *__restrict__ nl_gap,
     float *__restrict__ nl_gmin, const uint32_t *__restrict__ err)
 {
-    SKIP_UNLESS(cond);
-    if ((*err & ERRF_BARRIER) != 0u) return;  // the sort before it did not finish
-    const uint32_t np = a.np;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        flags[NLF_COUNT] += 1u;  // rebuild counter
-    }
-    if (blockIdx.x == 0 && threadIdx.x < NLF_DSLOTS)  // displacements restart
-        ((unsigned long long *)(flags + NLF_DMAX))[threadIdx.x] = 0ull;
-    for (uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x; pos < np;
-         pos += gridDim.x * blockDim.x)
-        nl_build_one(a, n, pos, vals, cell_start, cell_end, x_s, nl, nl_cnt, nl_pos, x_ref, flags,
-                     contacts, cmask, nl_slot, slot_cand, nl_gap, nl_gmin);
+    nl_build_body(a, n, cond, vals, cell_start, cell_end, x_s, nl, nl_cnt, nl_pos, x_ref, flags,
+                  contacts, cmask, nl_slot, slot_cand, nl_gap, nl_gmin, err, blockIdx.x,
+                  gridDim.x);
 }
 
 void launch_nl_build(hipStream_t s, const StepArgs &a, const NLArgs &n, const DemBuffers &b,
                      const uint32_t *cond)
 {
-    uint32_t nb = (a.np + 255) / 256;
-    if (nb > COND_BLOCKS) nb = COND_BLOCKS;
-    hipLaunchKernelGGL(k_nl_build, dim3(nb), dim3(256), 0, s, a, n, cond, b.vals, b.cell_start,
-                       b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_pos, b.x_ref, b.nl_flags,
-                       b.contacts, b.cmask, b.nl_slot, b.slot_cand, b.nl_gap, b.nl_gmin,
-                       b.errflags);
+    const NlBuildTask l = nl_build_task(a, n, b, cond);
+    hipLaunchKernelGGL(k_nl_build, dim3(nl_build_blocks(a.np)), dim3(256), 0, s, l.a, l.n, l.cond,
+                       l.vals, l.cell_start, l.cell_end, l.x_s, l.nl, l.nl_cnt, l.nl_pos, l.x_ref,
+                       l.flags, l.contacts, l.cmask, l.nl_slot, l.slot_cand, l.nl_gap, l.nl_gmin,
+                       l.err);
 }
 
 // ------------------------------------------------------------------
@@ -665,19 +678,19 @@ __device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t *lds, ui
 // barrier, then each block scans its chunk from the sum of the chunks before
 // it.
 __device__ bool grid_scan(uint32_t *__restrict__ a, uint32_t m, uint32_t *__restrict__ part,
-                          uint32_t *bar, uint32_t nblocks, uint32_t &gen, uint32_t *err,
-                          uint32_t limit, uint32_t *lds)
+                          uint32_t *bar, uint32_t bid, uint32_t nblocks, uint32_t &gen,
+                          uint32_t *err, uint32_t limit, uint32_t *lds)
 {
     const uint32_t t = threadIdx.x;
     const uint32_t chunk = (m + nblocks - 1u) / nblocks;
-    const uint32_t lo = min(blockIdx.x * chunk, m), hi = min(lo + chunk, m);
+    const uint32_t lo = min(bid * chunk, m), hi = min(lo + chunk, m);
     uint32_t sum = 0, tot;
     for (uint32_t i = lo + t; i < hi; i += 256u) sum += a[i];
     (void)block_scan_256(sum, lds, &tot);
-    if (t == 0) part[blockIdx.x] = tot;
+    if (t == 0) part[bid] = tot;
     if (!grid_barrier(bar, nblocks, gen, err, limit)) return false;
     uint32_t off = 0;
-    for (uint32_t b = t; b < blockIdx.x; b += 256u) off += part[b];
+    for (uint32_t b = t; b < bid; b += 256u) off += part[b];
     (void)block_scan_256(off, lds, &off);
     // each thread takes `per` consecutive entries of the chunk
     const uint32_t per = (chunk + 255u) / 256u;
@@ -693,17 +706,18 @@ __device__ bool grid_scan(uint32_t *__restrict__ a, uint32_t m, uint32_t *__rest
     return true;
 }
 
-__global__ void __launch_bounds__(256) k_rebuild(StepArgs a, RebuildArgs r)
+// Block `bid` of the G blocks that rebuild one box.
+__device__ __forceinline__ void rebuild_body(const StepArgs &a, const RebuildArgs &r, uint32_t bid,
+                                             uint32_t G)
 {
     if ((*r.cond & NLF_REBUILD) == 0u) return;
     if ((*r.err & ERRF_BARRIER) != 0u) return;  // an earlier barrier timed out
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[RS_WAVES][256];
-    const uint32_t G = gridDim.x;
     uint32_t gen = 0;
     if (threadIdx.x == 0)
         gen = __hip_atomic_load(&r.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t gtid = blockIdx.x * 256u + threadIdx.x, gsz = G * 256u;
+    const uint32_t gtid = bid * 256u + threadIdx.x, gsz = G * 256u;
     for (uint32_t c = gtid; c < r.ncells; c += gsz) {
         r.cell_start[c] = EMPTY_CELL;
         r.cell_end[c] = EMPTY_CELL;
@@ -712,13 +726,14 @@ __global__ void __launch_bounds__(256) k_rebuild(StepArgs a, RebuildArgs r)
     for (uint32_t pass = 0; pass < r.passes; ++pass) {
         const int shift = (int)(8u * pass);
         const uint32_t src = pass & 1u;
-        for (uint32_t tile = blockIdx.x; tile < r.ntiles; tile += G)
+        for (uint32_t tile = bid; tile < r.ntiles; tile += G)
             radix_hist_tile(r.keys[src], r.n, shift, r.hist, r.ntiles, tile, run);
         if (!grid_barrier(r.bar, G, gen, r.err, r.spin_limit)) return;
-        if (!grid_scan(r.hist, 256u * r.ntiles, r.part, r.bar, G, gen, r.err, r.spin_limit, run))
+        if (!grid_scan(r.hist, 256u * r.ntiles, r.part, r.bar, bid, G, gen, r.err, r.spin_limit,
+                       run))
             return;
         if (!grid_barrier(r.bar, G, gen, r.err, r.spin_limit)) return;
-        for (uint32_t tile = blockIdx.x; tile < r.ntiles; tile += G)
+        for (uint32_t tile = bid; tile < r.ntiles; tile += G)
             radix_scatter_tile(r.keys[src], r.vals[src], r.keys[src ^ 1u], r.vals[src ^ 1u], r.n,
                                shift, r.hist, r.ntiles, tile, run, wcnt);
         if (!grid_barrier(r.bar, G, gen, r.err, r.spin_limit)) return;
@@ -739,6 +754,11 @@ __global__ void __launch_bounds__(256) k_rebuild(StepArgs a, RebuildArgs r)
     }
 }
 
+__global__ void __launch_bounds__(256) k_rebuild(StepArgs a, RebuildArgs r)
+{
+    rebuild_body(a, r, blockIdx.x, gridDim.x);
+}
+
 int rebuild_blocks_per_cu()
 {
     int nb = 0;
@@ -752,34 +772,10 @@ void launch_rebuild(hipStream_t s, const StepArgs &a, const NLArgs &n, DemBuffer
                     uint32_t spin_limit)
 {
     if (a.np == 0) return;
-    int passes = (bits + 7) / 8;
-    if (passes < 1) passes = 1;
-    RebuildArgs r;
-    r.cond = cond;
-    r.keys[0] = b.keys;
-    r.keys[1] = b.keys_tmp;
-    r.vals[0] = b.vals;
-    r.vals[1] = b.vals_tmp;
-    r.hist = b.hist;
-    r.part = b.rb_part;
-    r.n = a.np;
-    r.ntiles = (a.np + RS_TILE - 1) / RS_TILE;
-    r.passes = (uint32_t)passes;
-    r.ncells = ncells;
-    r.cell_start = b.cell_start;
-    r.cell_end = b.cell_end;
-    r.x = b.x;
-    r.vel = b.vel;
-    r.angvel = b.angvel;
-    r.x_s = b.x_s;
-    r.vel_s = b.vel_s;
-    r.angvel_s = b.angvel_s;
-    r.bar = b.rb_bar;
-    r.err = b.errflags;
-    r.spin_limit = spin_limit ? spin_limit : RB_SPIN_LIMIT;
-    hipLaunchKernelGGL(k_rebuild, dim3(blocks), dim3(256), 0, s, a, r);
+    const RebuildTask t = rebuild_task(a, b, ncells, bits, cond, spin_limit);
+    hipLaunchKernelGGL(k_rebuild, dim3(blocks), dim3(256), 0, s, t.a, t.r);
     // the sorted keys and values end in the buffer the last pass wrote
-    if (passes & 1) {
+    if (t.r.passes & 1u) {
         uint32_t *tk = b.keys;
         b.keys = b.keys_tmp;
         b.keys_tmp = tk;
@@ -789,10 +785,11 @@ void launch_rebuild(hipStream_t s, const StepArgs &a, const NLArgs &n, DemBuffer
     }
     // the list build, the heaviest phase, on one thread per grain (it
     // needs no barrier after it, so its grid is not bounded by residency)
-    hipLaunchKernelGGL(k_nl_build, dim3((a.np + 255) / 256), dim3(256), 0, s, a, n, cond, b.vals,
-                       b.cell_start, b.cell_end, b.x_s, b.nl, b.nl_cnt, b.nl_pos, b.x_ref,
-                       b.nl_flags, b.contacts, b.cmask, b.nl_slot, b.slot_cand, b.nl_gap,
-                       b.nl_gmin, b.errflags);
+    const NlBuildTask l = nl_build_task(a, n, b, cond);
+    hipLaunchKernelGGL(k_nl_build, dim3((a.np + 255) / 256), dim3(256), 0, s, l.a, l.n, l.cond,
+                       l.vals, l.cell_start, l.cell_end, l.x_s, l.nl, l.nl_cnt, l.nl_pos, l.x_ref,
+                       l.flags, l.contacts, l.cmask, l.nl_slot, l.slot_cand, l.nl_gap, l.nl_gmin,
+                       l.err);
 }
 
 // Sum and maximum of the list lengths (overflowed lists count as cap + 1).
@@ -1445,8 +1442,8 @@ __device__ void owner_test_all(const Grid &g, uint32_t np, uint32_t pos, uint32_
 // NEAR: grid searched one cell deep (nearest_image applies; the cell code of
 // list entries is only read for new contacts)
 template <bool NEAR>
-__global__ void __launch_bounds__(TW) k_topology_lists(
-    StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
+__device__ __forceinline__ void topology_lists_body(
+    const StepArgs &a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
     const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x,
     const double4 *__restrict__ x_s, uint32_t *__restrict__ contacts,
     uint64_t *__restrict__ cmask, uint32_t *__restrict__ err,
@@ -1454,7 +1451,8 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     uint8_t *__restrict__ slot_cand, const uint32_t *__restrict__ nl_cnt,
     const uint32_t *__restrict__ nl_pos, const uint32_t *__restrict__ cellc,
     const float *__restrict__ nl_gap, const float *__restrict__ nl_gmin,
-    const float *__restrict__ nl_disp, uint32_t *__restrict__ flags_rw, uint32_t cur)
+    const float *__restrict__ nl_disp, uint32_t *__restrict__ flags_rw, uint32_t cur,
+    uint32_t bid, uint32_t nblk)
 {
     // the list rebuild before it did not finish (its grid barrier timed out)
     if ((*err & ERRF_BARRIER) != 0u) return;
@@ -1464,7 +1462,7 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     __shared__ uint32_t s_a[TW];
     __shared__ uint8_t s_own[TW * TU], s_k[TW * TU], s_fl[TW * TU];
     const uint32_t lane = threadIdx.x;
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t tile = xcd_tile(bid, nblk);
     const uint32_t pos = tile * TW + lane;
     const uint32_t np = a.np;
     const Grid &g = a.grid;
@@ -1484,7 +1482,7 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     // row whose surface gap at the rebuild is at most twice the largest
     // displacement since then (plus a rounding margin).
     const bool full = (flags[cur] & (NLF_REBUILD | NLF_WRAP)) != 0u || flags[NLF_STICKY] != 0u;
-    if (full && blockIdx.x == 0 && lane == 0) flags_rw[NLF_FULL] += 1u;
+    if (full && bid == 0 && lane == 0) flags_rw[NLF_FULL] += 1u;
     const double dmax = dmax_since_rebuild(flags);
     Set128 q = {0ull, 0ull};
     uint32_t n = 0;
@@ -1655,6 +1653,24 @@ __global__ void __launch_bounds__(TW) k_topology_lists(
     if (mask != mask0) cmask[a_orig] = mask;
 }
 
+
+template <bool NEAR>
+__global__ void __launch_bounds__(TW) k_topology_lists(
+    StepArgs a, const uint32_t *__restrict__ vals, const uint32_t *__restrict__ cell_start,
+    const uint32_t *__restrict__ cell_end, const double4 *__restrict__ x,
+    const double4 *__restrict__ x_s, uint32_t *__restrict__ contacts,
+    uint64_t *__restrict__ cmask, uint32_t *__restrict__ err,
+    const uint32_t *__restrict__ nl, uint8_t *__restrict__ nl_slot,
+    uint8_t *__restrict__ slot_cand, const uint32_t *__restrict__ nl_cnt,
+    const uint32_t *__restrict__ nl_pos, const uint32_t *__restrict__ cellc,
+    const float *__restrict__ nl_gap, const float *__restrict__ nl_gmin,
+    const float *__restrict__ nl_disp, uint32_t *__restrict__ flags_rw, uint32_t cur)
+{
+    topology_lists_body<NEAR>(a, vals, cell_start, cell_end, x, x_s, contacts, cmask, err, nl,
+                              nl_slot, slot_cand, nl_cnt, nl_pos, cellc, nl_gap, nl_gmin, nl_disp,
+                              flags_rw, cur, blockIdx.x, gridDim.x);
+}
+
 // Wall contacts of one particle, contactsearch.cuh:420-701: the top wall
 // (wall 0, or the plane z = L2 without walls), the fixed bottom z = origo2,
 // and for non-periodic axes walls 1-4.  Returns the normal forces on walls
@@ -1754,8 +1770,8 @@ __device__ __forceinline__ void store_acc(double (*s)[64], uint32_t lane, const 
 
 // HERTZ: contact model 3, else 2
 template <bool HERTZ>
-__global__ void __launch_bounds__(FW) k_forces(
-    StepArgs a, const uint32_t *__restrict__ order, const double4 *__restrict__ x,
+__device__ __forceinline__ void forces_body(
+    const StepArgs &a, const uint32_t *__restrict__ order, const double4 *__restrict__ x,
     const double4 *__restrict__ vel, const double3 *__restrict__ angvel,
     double3 *__restrict__ force, double3 *__restrict__ torque, double *__restrict__ es_dot_out,
     double *__restrict__ ev_dot_out, double *__restrict__ es, double *__restrict__ ev,
@@ -1765,7 +1781,7 @@ __global__ void __launch_bounds__(FW) k_forces(
     double *__restrict__ delta_t,
     uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand,
     const float *__restrict__ nl_gap, float *__restrict__ nl_gmin,
-    const uint32_t *__restrict__ err)
+    const uint32_t *__restrict__ err, uint32_t bid, uint32_t nblk)
 {
     // list mode: the list rebuild of this step did not finish
     if (err && (*err & ERRF_BARRIER) != 0u) return;
@@ -1774,7 +1790,7 @@ __global__ void __launch_bounds__(FW) k_forces(
     __shared__ double s_rec[NREC][FW];
     __shared__ double s_acc[9][FW];  // the owners' running sums (F, T, es_dot, ev_dot, p)
     const uint32_t lane = threadIdx.x;
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t tile = xcd_tile(bid, nblk);
     const uint32_t pos = tile * FW + lane;
     const uint32_t np = a.np;
     const bool valid = pos < np;
@@ -1905,6 +1921,25 @@ __global__ void __launch_bounds__(FW) k_forces(
     if (gfree < INFINITY && nl_gmin[a_orig] > gfree) nl_gmin[a_orig] = gfree;
 }
 
+template <bool HERTZ>
+__global__ void __launch_bounds__(FW) k_forces(
+    StepArgs a, const uint32_t *__restrict__ order, const double4 *__restrict__ x,
+    const double4 *__restrict__ vel, const double3 *__restrict__ angvel,
+    double3 *__restrict__ force, double3 *__restrict__ torque, double *__restrict__ es_dot_out,
+    double *__restrict__ ev_dot_out, double *__restrict__ es, double *__restrict__ ev,
+    double *__restrict__ p_out, const double4 *__restrict__ w_nx,
+    const double4 *__restrict__ w_mvfd, double *__restrict__ walls_force_pp,
+    uint32_t *__restrict__ contacts, uint64_t *__restrict__ cmask,
+    double *__restrict__ delta_t,
+    uint8_t *__restrict__ nl_slot, uint8_t *__restrict__ slot_cand,
+    const float *__restrict__ nl_gap, float *__restrict__ nl_gmin,
+    const uint32_t *__restrict__ err)
+{
+    forces_body<HERTZ>(a, order, x, vel, angvel, force, torque, es_dot_out, ev_dot_out, es, ev,
+                       p_out, w_nx, w_mvfd, walls_force_pp, contacts, cmask, delta_t, nl_slot,
+                       slot_cand, nl_gap, nl_gmin, err, blockIdx.x, gridDim.x);
+}
+
 // Model 1 (contactLinearViscous + capillary cohesion, no contact history):
 // one thread per sorted position, cells searched every step.
 __global__ void __launch_bounds__(256) k_contact_viscous(
@@ -1974,17 +2009,19 @@ void launch_topology(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool
                      bool near, uint32_t cur)
 {
     if (a.prm.contactmodel == 1) return;  // model 1 keeps no contact history
-    if (nlist && near)
-        hipLaunchKernelGGL(k_topology_lists<true>, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s, a,
-                           b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
-                           b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
-                           b.nl_pos, b.cellc, b.nl_gap, b.nl_gmin, b.nl_disp, b.nl_flags, cur);
-    else if (nlist)
-        hipLaunchKernelGGL(k_topology_lists<false>, dim3((a.np + TW - 1) / TW), dim3(TW), 0, s,
-                           a, b.vals, b.cell_start, b.cell_end, b.x, b.x_s, b.contacts, b.cmask,
-                           b.errflags, b.nl, b.nl_slot, b.slot_cand, b.nl_cnt,
-                           b.nl_pos, b.cellc, b.nl_gap, b.nl_gmin, b.nl_disp, b.nl_flags, cur);
-    else
+    if (nlist) {
+        const TopoTask t = topology_task(a, b, cur);
+        if (near)
+            hipLaunchKernelGGL(k_topology_lists<true>, dim3(topology_blocks(a.np)), dim3(TW), 0, s,
+                               t.a, t.vals, t.cell_start, t.cell_end, t.x, t.x_s, t.contacts,
+                               t.cmask, t.err, t.nl, t.nl_slot, t.slot_cand, t.nl_cnt, t.nl_pos,
+                               t.cellc, t.nl_gap, t.nl_gmin, t.nl_disp, t.flags, t.cur);
+        else
+            hipLaunchKernelGGL(k_topology_lists<false>, dim3(topology_blocks(a.np)), dim3(TW), 0,
+                               s, t.a, t.vals, t.cell_start, t.cell_end, t.x, t.x_s, t.contacts,
+                               t.cmask, t.err, t.nl, t.nl_slot, t.slot_cand, t.nl_cnt, t.nl_pos,
+                               t.cellc, t.nl_gap, t.nl_gmin, t.nl_disp, t.flags, t.cur);
+    } else
         hipLaunchKernelGGL(k_topology_cells, dim3((a.np + 255) / 256), dim3(256), 0, s, a,
                            b.vals, b.cell_start, b.cell_end, b.x_s, b.contacts, b.cmask,
                            b.errflags);
@@ -1999,23 +2036,18 @@ void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool 
                            b.walls_force_pp, b.errflags);
         return;
     }
-    const uint32_t nw = (a.np + FW - 1) / FW;
-    // list mode runs in original order (the lists are by original index)
-    const uint32_t *order = nlist ? nullptr : b.vals;
-    uint8_t *nls = nlist ? b.nl_slot : nullptr;
-    uint8_t *scd = nlist ? b.slot_cand : nullptr;
-    const float *gap = nlist ? b.nl_gap : nullptr;
-    float *gmin = nlist ? b.nl_gmin : nullptr;
+    const ForcesTask t = forces_task(a, b, nlist);
+    const uint32_t nw = forces_blocks(a.np);
     if (a.prm.contactmodel == 3)
-        hipLaunchKernelGGL(k_forces<true>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
-                           b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
-                           b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask,
-                           b.delta_t, nls, scd, gap, gmin, nlist ? b.errflags : nullptr);
+        hipLaunchKernelGGL(k_forces<true>, dim3(nw), dim3(FW), 0, s, t.a, t.order, t.x, t.vel,
+                           t.angvel, t.force, t.torque, t.es_dot, t.ev_dot, t.es, t.ev, t.p,
+                           t.w_nx, t.w_mvfd, t.walls_force_pp, t.contacts, t.cmask, t.delta_t,
+                           t.nl_slot, t.slot_cand, t.nl_gap, t.nl_gmin, t.err);
     else
-        hipLaunchKernelGGL(k_forces<false>, dim3(nw), dim3(FW), 0, s, a, order, b.x, b.vel,
-                           b.angvel, b.force, b.torque, b.es_dot, b.ev_dot, b.es, b.ev, b.p,
-                           b.w_nx, b.w_mvfd, b.walls_force_pp, b.contacts, b.cmask,
-                           b.delta_t, nls, scd, gap, gmin, nlist ? b.errflags : nullptr);
+        hipLaunchKernelGGL(k_forces<false>, dim3(nw), dim3(FW), 0, s, t.a, t.order, t.x, t.vel,
+                           t.angvel, t.force, t.torque, t.es_dot, t.ev_dot, t.es, t.ev, t.p,
+                           t.w_nx, t.w_mvfd, t.walls_force_pp, t.contacts, t.cmask, t.delta_t,
+                           t.nl_slot, t.slot_cand, t.nl_gap, t.nl_gmin, t.err);
 }
 
 // ------------------------------------------------------------------
@@ -2023,13 +2055,14 @@ void launch_contact(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool 
 // ------------------------------------------------------------------
 
 // bondsLinear body, src/cohesion.cuh:24-217, one thread per bond
-__global__ void __launch_bounds__(256) k_bond_compute(
-    StepArgs a, const uint2 *__restrict__ bonds, uint8_t *__restrict__ broken,
+__device__ __forceinline__ void bond_compute_body(
+    const StepArgs &a, const uint2 *__restrict__ bonds, uint8_t *__restrict__ broken,
     double4 *__restrict__ bdelta, double4 *__restrict__ bomega, const double4 *__restrict__ x,
     const double4 *__restrict__ vel, const double3 *__restrict__ angvel,
-    double4 *__restrict__ bf, double4 *__restrict__ bt, uint8_t *__restrict__ applied)
+    double4 *__restrict__ bf, double4 *__restrict__ bt, uint8_t *__restrict__ applied,
+    uint32_t bid)
 {
-    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t idx = bid * blockDim.x + threadIdx.x;
     if (idx >= a.nb0) return;
     applied[idx] = 0;
     if (broken[idx]) return;
@@ -2085,6 +2118,16 @@ __global__ void __launch_bounds__(256) k_bond_compute(
     applied[idx] = 1;
 }
 
+__global__ void __launch_bounds__(256) k_bond_compute(
+    StepArgs a, const uint2 *__restrict__ bonds, uint8_t *__restrict__ broken,
+    double4 *__restrict__ bdelta, double4 *__restrict__ bomega, const double4 *__restrict__ x,
+    const double4 *__restrict__ vel, const double3 *__restrict__ angvel,
+    double4 *__restrict__ bf, double4 *__restrict__ bt, uint8_t *__restrict__ applied)
+{
+    bond_compute_body(a, bonds, broken, bdelta, bomega, x, vel, angvel, bf, bt, applied,
+                      blockIdx.x);
+}
+
 // Per bonded particle: force/torque updates of cohesion.cuh:220-230 applied in
 // ascending bond order.  ent = bond index << 1 | (1 if the particle is bond.y).
 __global__ void __launch_bounds__(256) k_bond_gather(
@@ -2117,9 +2160,9 @@ __global__ void __launch_bounds__(256) k_bond_gather(
 void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool gather)
 {
     if (a.nb0 == 0) return;
-    hipLaunchKernelGGL(k_bond_compute, dim3((a.nb0 + 255) / 256), dim3(256), 0, s, a, b.bonds,
-                       b.bond_broken, b.bond_delta, b.bond_omega, b.x, b.vel, b.angvel, b.bond_f,
-                       b.bond_t, b.bond_applied);
+    const BondTask t = bond_task(a, b);
+    hipLaunchKernelGGL(k_bond_compute, dim3((a.nb0 + 255) / 256), dim3(256), 0, s, t.a, t.bonds,
+                       t.broken, t.bdelta, t.bomega, t.x, t.vel, t.angvel, t.bf, t.bt, t.applied);
     if (gather && b.nbp)
         hipLaunchKernelGGL(k_bond_gather, dim3((b.nbp + 255) / 256), dim3(256), 0, s, b.nbp,
                            b.bp_list, b.bp_off, b.bp_ent, b.bond_f, b.bond_t, b.bond_applied,
@@ -2275,23 +2318,23 @@ __device__ __forceinline__ void next_hash(const StepArgs &a, const NextHash &nh,
 // integrate (TY3), src/integration.cuh:14-305, in original order (the
 // reference reads the *_sorted copies, which hold the same values), with the
 // wall-0 force partial of summation() reduced over the same 256-particle block.
-__global__ void __launch_bounds__(256) k_integrate(
-    StepArgs a, const StepClock *__restrict__ clock, int change_velocity_state, double velocity_factor,
-    double tau_x,
+__device__ __forceinline__ void integrate_body(
+    const StepArgs &a, const StepClock *__restrict__ clock, int change_velocity_state,
+    double velocity_factor, double tau_x,
     double4 *__restrict__ x, double4 *__restrict__ vel, double3 *__restrict__ angvel,
     double3 *__restrict__ force, double3 *__restrict__ torque,
     double3 *__restrict__ angpos, double3 *__restrict__ acc_a, double3 *__restrict__ angacc_a,
     double3 *__restrict__ xyzsum_a, const double4 *__restrict__ w_mvfd,
     const double *__restrict__ tau_eff, const double *__restrict__ walls_force_pp,
-    double *__restrict__ wall_partial, BondGather bg, NextHash nh)
+    double *__restrict__ wall_partial, const BondGather &bg, const NextHash &nh, uint32_t bid)
 {
     __shared__ double cache[256];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     if (a.walls.nw > 0) {
         double temp = 0.0;
         if (i < a.np) temp += walls_force_pp[i];
         double s = block_sum_256(temp, cache);
-        if (threadIdx.x == 0) wall_partial[blockIdx.x] = s;
+        if (threadIdx.x == 0) wall_partial[bid] = s;
     }
     if (i >= a.np) return;
     const Grid &g = a.grid;
@@ -2395,30 +2438,39 @@ __global__ void __launch_bounds__(256) k_integrate(
     if (nh.keys) next_hash(a, nh, i, x_new, wrapped);
 }
 
+__global__ void __launch_bounds__(256) k_integrate(
+    StepArgs a, const StepClock *__restrict__ clock, int change_velocity_state, double velocity_factor,
+    double tau_x,
+    double4 *__restrict__ x, double4 *__restrict__ vel, double3 *__restrict__ angvel,
+    double3 *__restrict__ force, double3 *__restrict__ torque,
+    double3 *__restrict__ angpos, double3 *__restrict__ acc_a, double3 *__restrict__ angacc_a,
+    double3 *__restrict__ xyzsum_a, const double4 *__restrict__ w_mvfd,
+    const double *__restrict__ tau_eff, const double *__restrict__ walls_force_pp,
+    double *__restrict__ wall_partial, BondGather bg, NextHash nh)
+{
+    integrate_body(a, clock, change_velocity_state, velocity_factor, tau_x, x, vel, angvel, force,
+                   torque, angpos, acc_a, angacc_a, xyzsum_a, w_mvfd, tau_eff, walls_force_pp,
+                   wall_partial, bg, nh, blockIdx.x);
+}
+
 void launch_integrate(hipStream_t s, const StepArgs &a, const DemBuffers &b,
                       int change_velocity_state, double velocity_factor, double tau_x,
                       bool gather_bonds, const NextHash &nh)
 {
-    uint32_t nb = (a.np + 255) / 256;
-    BondGather bg{};
-    if (gather_bonds && a.nb0 > 0) {
-        bg.off = b.pp_off;
-        bg.ent = b.bp_ent;
-        bg.f = b.bond_f;
-        bg.t = b.bond_t;
-        bg.applied = b.bond_applied;
-    }
-    hipLaunchKernelGGL(k_integrate, dim3(nb), dim3(256), 0, s, a, b.clock, change_velocity_state,
-                       velocity_factor, tau_x, b.x, b.vel, b.angvel, b.force, b.torque, b.angpos,
-                       b.acc, b.angacc, b.xyzsum, b.w_mvfd, b.tau_eff, b.walls_force_pp,
-                       b.wall_partial, bg, nh);
+    const uint32_t nb = (a.np + 255) / 256;
+    const IntegrateTask t = integrate_task(a, b, change_velocity_state, velocity_factor, tau_x,
+                                           gather_bonds, nh);
+    hipLaunchKernelGGL(k_integrate, dim3(nb), dim3(256), 0, s, t.a, t.clock,
+                       t.change_velocity_state, t.velocity_factor, t.tau_x, t.x, t.vel, t.angvel,
+                       t.force, t.torque, t.angpos, t.acc, t.angacc, t.xyzsum, t.w_mvfd,
+                       t.tau_eff, t.walls_force_pp, t.wall_partial, t.bg, t.nh);
 }
 
 // integrateWalls, src/integration.cuh:344-450.  One wave: lane 0 forms the
 // serial sum over the block partials (the reference's order) once and every
 // wall thread uses it (the reference sums wall 0's partials for each wall).
-__global__ void __launch_bounds__(SUM_THREADS) k_integrate_walls(
-    StepArgs a, const StepClock *__restrict__ clock, double4 *__restrict__ w_nx,
+__device__ __forceinline__ void integrate_walls_body(
+    const StepArgs &a, const StepClock *__restrict__ clock, double4 *__restrict__ w_nx,
     double4 *__restrict__ w_mvfd, double *__restrict__ w_acc,
     const double *__restrict__ partial, uint32_t nparts)
 {
@@ -2449,11 +2501,19 @@ __global__ void __launch_bounds__(SUM_THREADS) k_integrate_walls(
     w_acc[idx] = acc;
 }
 
+__global__ void __launch_bounds__(SUM_THREADS) k_integrate_walls(
+    StepArgs a, const StepClock *__restrict__ clock, double4 *__restrict__ w_nx,
+    double4 *__restrict__ w_mvfd, double *__restrict__ w_acc,
+    const double *__restrict__ partial, uint32_t nparts)
+{
+    integrate_walls_body(a, clock, w_nx, w_mvfd, w_acc, partial, nparts);
+}
+
 void launch_integrate_walls(hipStream_t s, const StepArgs &a, const DemBuffers &b)
 {
-    uint32_t nb = (a.np + 255) / 256;
-    hipLaunchKernelGGL(k_integrate_walls, dim3(1), dim3(SUM_THREADS), 0, s, a, b.clock, b.w_nx,
-                       b.w_mvfd, b.w_acc, b.wall_partial, nb);
+    const WallsTask t = walls_task(a, b);
+    hipLaunchKernelGGL(k_integrate_walls, dim3(1), dim3(SUM_THREADS), 0, s, t.a, t.clock, t.w_nx,
+                       t.w_mvfd, t.w_acc, t.partial, t.nparts);
 }
 
 // iter += 1, time += dt: the host's time loop update (src/device.cu:2541-2545)
@@ -2519,4 +2579,333 @@ void launch_rebuild_mask(hipStream_t s, const DemBuffers &b, uint32_t np, uint32
     if (!np) return;
     hipLaunchKernelGGL(k_rebuild_mask, dim3((np + 255) / 256), dim3(256), 0, s, np, nc, b.contacts,
                        b.cmask, b.delta_t);
+}
+
+// ------------------------------------------------------------------
+// Multi-box launches (ensembles): task records and kernels
+// ------------------------------------------------------------------
+
+RebuildTask rebuild_task(const StepArgs &a, const DemBuffers &b, uint32_t ncells, int bits,
+                         const uint32_t *cond, uint32_t spin_limit)
+{
+    int passes = (bits + 7) / 8;
+    if (passes < 1) passes = 1;
+    RebuildTask t;
+    t.a = a;
+    RebuildArgs &r = t.r;
+    r.cond = cond;
+    r.keys[0] = b.keys;
+    r.keys[1] = b.keys_tmp;
+    r.vals[0] = b.vals;
+    r.vals[1] = b.vals_tmp;
+    r.hist = b.hist;
+    r.part = b.rb_part;
+    r.n = a.np;
+    r.ntiles = (a.np + RS_TILE - 1) / RS_TILE;
+    r.passes = (uint32_t)passes;
+    r.ncells = ncells;
+    r.cell_start = b.cell_start;
+    r.cell_end = b.cell_end;
+    r.x = b.x;
+    r.vel = b.vel;
+    r.angvel = b.angvel;
+    r.x_s = b.x_s;
+    r.vel_s = b.vel_s;
+    r.angvel_s = b.angvel_s;
+    r.bar = b.rb_bar;
+    r.err = b.errflags;
+    r.spin_limit = spin_limit ? spin_limit : RB_SPIN_LIMIT;
+    return t;
+}
+
+NlBuildTask nl_build_task(const StepArgs &a, const NLArgs &n, const DemBuffers &b,
+                          const uint32_t *cond)
+{
+    NlBuildTask t;
+    t.a = a;
+    t.n = n;
+    t.cond = cond;
+    t.vals = b.vals;
+    t.cell_start = b.cell_start;
+    t.cell_end = b.cell_end;
+    t.x_s = b.x_s;
+    t.nl = b.nl;
+    t.nl_cnt = b.nl_cnt;
+    t.nl_pos = b.nl_pos;
+    t.x_ref = b.x_ref;
+    t.flags = b.nl_flags;
+    t.contacts = b.contacts;
+    t.cmask = b.cmask;
+    t.nl_slot = b.nl_slot;
+    t.slot_cand = b.slot_cand;
+    t.nl_gap = b.nl_gap;
+    t.nl_gmin = b.nl_gmin;
+    t.err = b.errflags;
+    return t;
+}
+
+TopoTask topology_task(const StepArgs &a, const DemBuffers &b, uint32_t cur)
+{
+    TopoTask t;
+    t.a = a;
+    t.vals = b.vals;
+    t.cell_start = b.cell_start;
+    t.cell_end = b.cell_end;
+    t.x = b.x;
+    t.x_s = b.x_s;
+    t.contacts = b.contacts;
+    t.cmask = b.cmask;
+    t.err = b.errflags;
+    t.nl = b.nl;
+    t.nl_slot = b.nl_slot;
+    t.slot_cand = b.slot_cand;
+    t.nl_cnt = b.nl_cnt;
+    t.nl_pos = b.nl_pos;
+    t.cellc = b.cellc;
+    t.nl_gap = b.nl_gap;
+    t.nl_gmin = b.nl_gmin;
+    t.nl_disp = b.nl_disp;
+    t.flags = b.nl_flags;
+    t.cur = cur;
+    return t;
+}
+
+ForcesTask forces_task(const StepArgs &a, const DemBuffers &b, bool nlist)
+{
+    ForcesTask t;
+    t.a = a;
+    t.order = nlist ? nullptr : b.vals;  // list mode runs in original order
+    t.x = b.x;
+    t.vel = b.vel;
+    t.angvel = b.angvel;
+    t.force = b.force;
+    t.torque = b.torque;
+    t.es_dot = b.es_dot;
+    t.ev_dot = b.ev_dot;
+    t.es = b.es;
+    t.ev = b.ev;
+    t.p = b.p;
+    t.w_nx = b.w_nx;
+    t.w_mvfd = b.w_mvfd;
+    t.walls_force_pp = b.walls_force_pp;
+    t.contacts = b.contacts;
+    t.cmask = b.cmask;
+    t.delta_t = b.delta_t;
+    t.nl_slot = nlist ? b.nl_slot : nullptr;
+    t.slot_cand = nlist ? b.slot_cand : nullptr;
+    t.nl_gap = nlist ? b.nl_gap : nullptr;
+    t.nl_gmin = nlist ? b.nl_gmin : nullptr;
+    t.err = nlist ? b.errflags : nullptr;
+    return t;
+}
+
+BondTask bond_task(const StepArgs &a, const DemBuffers &b)
+{
+    BondTask t;
+    t.a = a;
+    t.bonds = b.bonds;
+    t.broken = b.bond_broken;
+    t.bdelta = b.bond_delta;
+    t.bomega = b.bond_omega;
+    t.x = b.x;
+    t.vel = b.vel;
+    t.angvel = b.angvel;
+    t.bf = b.bond_f;
+    t.bt = b.bond_t;
+    t.applied = b.bond_applied;
+    return t;
+}
+
+IntegrateTask integrate_task(const StepArgs &a, const DemBuffers &b, int change_velocity_state,
+                             double velocity_factor, double tau_x, bool gather_bonds,
+                             const NextHash &nh)
+{
+    IntegrateTask t;
+    t.a = a;
+    t.clock = b.clock;
+    t.change_velocity_state = change_velocity_state;
+    t.velocity_factor = velocity_factor;
+    t.tau_x = tau_x;
+    t.x = b.x;
+    t.vel = b.vel;
+    t.angvel = b.angvel;
+    t.force = b.force;
+    t.torque = b.torque;
+    t.angpos = b.angpos;
+    t.acc = b.acc;
+    t.angacc = b.angacc;
+    t.xyzsum = b.xyzsum;
+    t.w_mvfd = b.w_mvfd;
+    t.tau_eff = b.tau_eff;
+    t.walls_force_pp = b.walls_force_pp;
+    t.wall_partial = b.wall_partial;
+    t.bg = BondGather{};
+    if (gather_bonds && a.nb0 > 0) {
+        t.bg.off = b.pp_off;
+        t.bg.ent = b.bp_ent;
+        t.bg.f = b.bond_f;
+        t.bg.t = b.bond_t;
+        t.bg.applied = b.bond_applied;
+    }
+    t.nh = nh;
+    return t;
+}
+
+WallsTask walls_task(const StepArgs &a, const DemBuffers &b)
+{
+    WallsTask t;
+    t.a = a;
+    t.clock = b.clock;
+    t.w_nx = b.w_nx;
+    t.w_mvfd = b.w_mvfd;
+    t.w_acc = b.w_acc;
+    t.partial = b.wall_partial;
+    t.nparts = (a.np + 255) / 256;
+    return t;
+}
+
+uint32_t topology_blocks(uint32_t np) { return (np + TW - 1) / TW; }
+uint32_t forces_blocks(uint32_t np) { return (np + FW - 1) / FW; }
+uint32_t nl_build_blocks(uint32_t np)
+{
+    uint32_t nb = (np + 255) / 256;
+    return nb > COND_BLOCKS ? COND_BLOCKS : nb;
+}
+
+// The box of this block, the block's index within the box and the box's
+// block count; false for the padding blocks after a box's last block.
+__device__ __forceinline__ bool multi_block(const MultiMap &m, uint32_t &box, uint32_t &bid,
+                                            uint32_t &nblk)
+{
+    const uint32_t b = blockIdx.x;
+    uint32_t lo = 0, hi = m.nbox;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (m.start[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    box = lo;
+    bid = b - m.start[lo];
+    nblk = m.start[m.nbox + lo];
+    return bid < nblk;
+}
+
+__global__ void __launch_bounds__(256) k_rebuild_multi(const RebuildTask *__restrict__ t, MultiMap m)
+{
+    uint32_t k, bid, nblk;
+    if (!multi_block(m, k, bid, nblk)) return;
+    rebuild_body(t[k].a, t[k].r, bid, nblk);
+}
+
+__global__ void __launch_bounds__(256) k_nl_build_multi(const NlBuildTask *__restrict__ t, MultiMap m)
+{
+    uint32_t k, bid, nblk;
+    if (!multi_block(m, k, bid, nblk)) return;
+    const NlBuildTask &T = t[k];
+    nl_build_body(T.a, T.n, T.cond, T.vals, T.cell_start, T.cell_end, T.x_s, T.nl, T.nl_cnt,
+                  T.nl_pos, T.x_ref, T.flags, T.contacts, T.cmask, T.nl_slot, T.slot_cand,
+                  T.nl_gap, T.nl_gmin, T.err, bid, nblk);
+}
+
+template <bool NEAR>
+__global__ void __launch_bounds__(TW) k_topology_multi(const TopoTask *__restrict__ t, MultiMap m)
+{
+    uint32_t k, bid, nblk;
+    if (!multi_block(m, k, bid, nblk)) return;
+    const TopoTask &T = t[k];
+    topology_lists_body<NEAR>(T.a, T.vals, T.cell_start, T.cell_end, T.x, T.x_s, T.contacts,
+                              T.cmask, T.err, T.nl, T.nl_slot, T.slot_cand, T.nl_cnt, T.nl_pos,
+                              T.cellc, T.nl_gap, T.nl_gmin, T.nl_disp, T.flags, T.cur, bid, nblk);
+}
+
+template <bool HERTZ>
+__global__ void __launch_bounds__(FW) k_forces_multi(const ForcesTask *__restrict__ t, MultiMap m)
+{
+    uint32_t k, bid, nblk;
+    if (!multi_block(m, k, bid, nblk)) return;
+    const ForcesTask &T = t[k];
+    forces_body<HERTZ>(T.a, T.order, T.x, T.vel, T.angvel, T.force, T.torque, T.es_dot, T.ev_dot,
+                       T.es, T.ev, T.p, T.w_nx, T.w_mvfd, T.walls_force_pp, T.contacts, T.cmask,
+                       T.delta_t, T.nl_slot, T.slot_cand, T.nl_gap, T.nl_gmin, T.err, bid, nblk);
+}
+
+__global__ void __launch_bounds__(256) k_bonds_multi(const BondTask *__restrict__ t, MultiMap m)
+{
+    uint32_t k, bid, nblk;
+    if (!multi_block(m, k, bid, nblk)) return;
+    const BondTask &T = t[k];
+    bond_compute_body(T.a, T.bonds, T.broken, T.bdelta, T.bomega, T.x, T.vel, T.angvel, T.bf,
+                      T.bt, T.applied, bid);
+}
+
+__global__ void __launch_bounds__(256) k_integrate_multi(const IntegrateTask *__restrict__ t,
+                                                         MultiMap m)
+{
+    uint32_t k, bid, nblk;
+    if (!multi_block(m, k, bid, nblk)) return;
+    const IntegrateTask &T = t[k];
+    integrate_body(T.a, T.clock, T.change_velocity_state, T.velocity_factor, T.tau_x, T.x, T.vel,
+                   T.angvel, T.force, T.torque, T.angpos, T.acc, T.angacc, T.xyzsum, T.w_mvfd,
+                   T.tau_eff, T.walls_force_pp, T.wall_partial, T.bg, T.nh, bid);
+}
+
+__global__ void __launch_bounds__(SUM_THREADS) k_walls_multi(const WallsTask *__restrict__ t)
+{
+    const WallsTask &T = t[blockIdx.x];
+    integrate_walls_body(T.a, T.clock, T.w_nx, T.w_mvfd, T.w_acc, T.partial, T.nparts);
+}
+
+__global__ void k_clock_multi(const ClockTask *__restrict__ t, uint32_t n)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        t[k].clock->iter += 1;
+        t[k].clock->time += t[k].dt;
+    }
+}
+
+void launch_rebuild_multi(hipStream_t s, const RebuildTask *t, MultiMap m, uint32_t blocks)
+{
+    if (blocks) hipLaunchKernelGGL(k_rebuild_multi, dim3(blocks), dim3(256), 0, s, t, m);
+}
+
+void launch_nl_build_multi(hipStream_t s, const NlBuildTask *t, MultiMap m, uint32_t blocks)
+{
+    if (blocks) hipLaunchKernelGGL(k_nl_build_multi, dim3(blocks), dim3(256), 0, s, t, m);
+}
+
+void launch_topology_multi(hipStream_t s, const TopoTask *t, MultiMap m, uint32_t blocks,
+                           bool near)
+{
+    if (!blocks) return;
+    if (near) hipLaunchKernelGGL(k_topology_multi<true>, dim3(blocks), dim3(TW), 0, s, t, m);
+    else hipLaunchKernelGGL(k_topology_multi<false>, dim3(blocks), dim3(TW), 0, s, t, m);
+}
+
+void launch_forces_multi(hipStream_t s, const ForcesTask *t, MultiMap m, uint32_t blocks,
+                         bool hertz)
+{
+    if (!blocks) return;
+    if (hertz) hipLaunchKernelGGL(k_forces_multi<true>, dim3(blocks), dim3(FW), 0, s, t, m);
+    else hipLaunchKernelGGL(k_forces_multi<false>, dim3(blocks), dim3(FW), 0, s, t, m);
+}
+
+void launch_bonds_multi(hipStream_t s, const BondTask *t, MultiMap m, uint32_t blocks)
+{
+    if (blocks) hipLaunchKernelGGL(k_bonds_multi, dim3(blocks), dim3(256), 0, s, t, m);
+}
+
+void launch_integrate_multi(hipStream_t s, const IntegrateTask *t, MultiMap m, uint32_t blocks)
+{
+    if (blocks) hipLaunchKernelGGL(k_integrate_multi, dim3(blocks), dim3(256), 0, s, t, m);
+}
+
+void launch_walls_multi(hipStream_t s, const WallsTask *t, uint32_t n)
+{
+    if (n) hipLaunchKernelGGL(k_walls_multi, dim3(n), dim3(SUM_THREADS), 0, s, t);
+}
+
+void launch_clock_multi(hipStream_t s, const ClockTask *t, uint32_t n)
+{
+    if (n) hipLaunchKernelGGL(k_clock_multi, dim3((n + 63) / 64), dim3(64), 0, s, t, n);
 }

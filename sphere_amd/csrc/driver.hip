@@ -58,6 +58,7 @@ struct sph_ctx {
     int key_bits = 1;
     // neighbour lists
     double nl_frac = 0.1;    // requested skin / largest radius
+    double nl_rmax = 0.0;    // largest radius of the loaded state
     uint32_t nl_cap = 128;   // list entries per particle
     bool rb_fused = true;    // list rebuild as one launch (k_rebuild)
     uint32_t rb_blocks = 0;  // its grid; 0: separate sort, reorder and list launches
@@ -96,6 +97,13 @@ struct sph_ctx {
     size_t ev_used = 0;
     std::vector<std::tuple<int, size_t, size_t>> ev_pending;
 };
+
+#define HIPCHK_E(call)                                                            \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return ens_fail(e, SPH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
 
 #define HIPCHK(call)                                                              \
     do {                                                                          \
@@ -334,6 +342,23 @@ static void pack4(const std::vector<double> &src, size_t n, int w, std::vector<d
         for (int c = 0; c < w; ++c) dst[4 * i + c] = src[(size_t)w * i + c];
 }
 
+// Cells searched on each side at a list rebuild for this skin: 1 or 2 if a
+// cell (or two) holds a diameter plus the skin and periodic axes keep at
+// least 2 range + 1 cells; 0 if lists cannot be used.
+static uint32_t nl_range(const sph_state &h, double rmax, double skin)
+{
+    double cmin = 1e300;
+    for (int d = 0; d < 3; ++d) cmin = std::min(cmin, h.L[d] / h.num[d]);
+    uint32_t range = 0;
+    if (cmin >= (2.0 * rmax + skin) * (1.0 + 1e-9)) range = 1;
+    else if (2.0 * cmin >= (2.0 * rmax + skin) * (1.0 + 1e-9)) range = 2;
+    const uint32_t need = 2 * range + 1;
+    bool ok = range > 0;
+    if (h.periodic == 1 || h.periodic == 2) ok = ok && h.num[0] >= need;
+    if (h.periodic == 1) ok = ok && h.num[1] >= need;
+    return ok ? range : 0u;
+}
+
 static int upload(sph_ctx *ctx)
 {
     HostState &hs = ctx->hs;
@@ -519,17 +544,10 @@ static int upload(sph_ctx *ctx)
         h.num[2] <= 1024) {
         double rmax = 0.0;
         for (size_t i = 0; i < np; ++i) rmax = std::max(rmax, hs.x[4 * i + 3]);
-        double cmin = 1e300;
-        for (int d = 0; d < 3; ++d) cmin = std::min(cmin, h.L[d] / h.num[d]);
+        ctx->nl_rmax = rmax;
         const double skin = ctx->nl_frac * rmax;
-        uint32_t range = 0;
-        if (cmin >= (2.0 * rmax + skin) * (1.0 + 1e-9)) range = 1;
-        else if (2.0 * cmin >= (2.0 * rmax + skin) * (1.0 + 1e-9)) range = 2;
-        const uint32_t need = 2 * range + 1;
-        bool ok = range > 0;
-        if (h.periodic == 1 || h.periodic == 2) ok = ok && h.num[0] >= need;
-        if (h.periodic == 1) ok = ok && h.num[1] >= need;
-        if (ok) {
+        const uint32_t range = nl_range(h, rmax, skin);
+        if (range > 0) {
             ctx->nla.skin = skin;
             // both partners move at most 0.45 skin before a rebuild
             ctx->nla.disp2 = (0.45 * skin) * (0.45 * skin);
@@ -1793,18 +1811,58 @@ int sph_kernel_time(sph_ctx *ctx, int kernel_id, double *total_ms, uint64_t *lau
 }
 
 // ------------------------------------------------------------------
-// Ensembles of independent boxes (no reference counterpart): the steps of
-// all boxes are enqueued without waiting, box after box for each step, on
-// every box's own streams, so that the device runs the boxes' kernels side
-// by side; the host waits once at the end.  Each box runs exactly the
-// kernels of sph_step, so its results equal those of sph_step on that box.
+// Ensembles of independent boxes (no reference counterpart)
+//
+// Boxes on the list path without fluid (the shear and consolidation boxes
+// of a DEM ensemble) are stepped together: every stage of a step is one
+// launch for all of them (dem_kernels.hip, "multi-box launches"), whose
+// blocks are divided among the boxes and whose per-box arguments are task
+// records in device memory.  Each box runs the body of its single-box
+// kernel, so its results equal those of sph_step on that box alone.  The
+// task records depend on two host values that alternate from step to step
+// (the rebuild-flag word and, for an odd number of radix passes, which key
+// buffer is current), so a plan holds the records of two consecutive
+// steps, and two steps are captured in one HIP graph that is replayed.
+// Other boxes (fluid, cell search, shear-stress walls, ...) are stepped
+// box by box on their own streams beside the multi-box launches.
 // ------------------------------------------------------------------
+
+}  // extern "C"
+
+struct EnsStage {
+    void *tasks = nullptr;      // device task records
+    uint32_t *map = nullptr;    // device: block starts, then block counts
+    uint32_t nbox = 0, blocks = 0;
+};
+
+struct EnsPhase {  // the launches of one step
+    EnsStage rebuild, nlbuild, topo[2], forces[2], bonds, integ;
+    void *walls = nullptr;
+    uint32_t nwalls = 0;
+    void *clocks = nullptr;
+    uint32_t nclocks = 0;
+};
 
 struct sph_ensemble {
     std::vector<sph_ctx *> boxes;
     int device = 0;
     std::string err;
     uint64_t steps = 0;
+    hipStream_t stream = nullptr;
+    // plan of the multi-box boxes
+    std::vector<int> multi;             // indices of the boxes stepped by multi-box launches
+    std::vector<uint64_t> plan_gen;     // their load generation at planning
+    std::vector<uint32_t> plan_parity;  // their rebuild-flag word at planning
+    std::vector<uint32_t *> plan_keys;  // their current key buffer at planning
+    bool planned = false;
+    uint32_t phase = 0;                 // steps since planning, mod 2
+    EnsPhase ph[2];
+    std::vector<void *> dmem;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = true;
+    uint64_t graph_replays = 0, multi_steps = 0;
+    uint32_t launches_per_step = 0;
 };
 
 static int ens_fail(sph_ensemble *e, int code, const std::string &msg)
@@ -1812,6 +1870,384 @@ static int ens_fail(sph_ensemble *e, int code, const std::string &msg)
     e->err = msg;
     g_last_error = msg;
     return code;
+}
+
+static void ens_drop_plan(sph_ensemble *e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+    if (!e->dmem.empty() && e->stream) (void)hipStreamSynchronize(e->stream);
+    for (void *p : e->dmem) (void)hipFree(p);
+    e->dmem.clear();
+    for (EnsPhase &p : e->ph) p = EnsPhase{};
+    e->planned = false;
+    e->multi.clear();
+}
+
+// A box the multi-box launches can step: the list path with the fused
+// rebuild, contact model 2 or 3, no fluid, no adaptive grid, no wmode-3
+// wall, no profiling.
+static bool ens_multi_eligible(const sph_ctx *c)
+{
+    const sph_state &h = c->hs.h;
+    return c->loaded && h.np > 0 && !h.fluid && c->nl_on && c->rb_blocks > 0 &&
+           (h.contactmodel == 2 || h.contactmodel == 3) && !(h.adaptive == 1 && h.nw > 0) &&
+           !(h.nw > 0 && h.wmode[0] == 3) && !c->prof && c->db.rb_bar && c->db.rb_part;
+}
+
+static int radix_passes(const sph_ctx *c)
+{
+    const int p = (c->key_bits + 7) / 8;
+    return p < 1 ? 1 : p;
+}
+
+// The device copy of one stage: task records in box order and the map of
+// block starts (multiples of 8) and counts.
+template <class T>
+static int ens_stage(sph_ensemble *e, EnsStage &st, const std::vector<T> &tasks,
+                     const std::vector<uint32_t> &counts)
+{
+    st.nbox = (uint32_t)tasks.size();
+    st.blocks = 0;
+    if (tasks.empty()) return SPH_OK;
+    std::vector<uint32_t> map(2 * tasks.size());
+    uint32_t at = 0;
+    for (size_t k = 0; k < tasks.size(); ++k) {
+        map[k] = at;
+        map[tasks.size() + k] = counts[k];
+        at += (counts[k] + 7u) & ~7u;
+    }
+    st.blocks = at;
+    void *dt = nullptr, *dm = nullptr;
+    if (hipMalloc(&dt, sizeof(T) * tasks.size()) != hipSuccess)
+        return ens_fail(e, SPH_ERR_HIP, "ensemble: device allocation failed");
+    e->dmem.push_back(dt);
+    if (hipMalloc(&dm, 4 * map.size()) != hipSuccess)
+        return ens_fail(e, SPH_ERR_HIP, "ensemble: device allocation failed");
+    e->dmem.push_back(dm);
+    if (hipMemcpy(dt, tasks.data(), sizeof(T) * tasks.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dm, map.data(), 4 * map.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return ens_fail(e, SPH_ERR_HIP, "ensemble: copy of the task records failed");
+    st.tasks = dt;
+    st.map = (uint32_t *)dm;
+    return SPH_OK;
+}
+
+template <class T>
+static int ens_array(sph_ensemble *e, void *&dst, const std::vector<T> &v)
+{
+    dst = nullptr;
+    if (v.empty()) return SPH_OK;
+    if (hipMalloc(&dst, sizeof(T) * v.size()) != hipSuccess)
+        return ens_fail(e, SPH_ERR_HIP, "ensemble: device allocation failed");
+    e->dmem.push_back(dst);
+    if (hipMemcpy(dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return ens_fail(e, SPH_ERR_HIP, "ensemble: copy of the task records failed");
+    return SPH_OK;
+}
+
+// Rebuild blocks of each multi-box box: its single-box grid, scaled down so
+// that the whole launch stays within a quarter of the device's resident
+// capacity for the kernel (every box's grid barrier needs its blocks
+// running together).
+static std::vector<uint32_t> ens_rebuild_blocks(const sph_ensemble *e)
+{
+    std::vector<uint32_t> nb(e->multi.size(), 1u);
+    int cus = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device);
+    const uint64_t budget = std::max<uint64_t>(e->multi.size(),
+                                               (uint64_t)rebuild_blocks_per_cu() * std::max(cus, 1) / 4);
+    uint64_t total_np = 0, want = 0;
+    for (int k : e->multi) {
+        total_np += e->boxes[k]->hs.h.np;
+        want += e->boxes[k]->rb_blocks;
+    }
+    for (size_t i = 0; i < e->multi.size(); ++i) {
+        const sph_ctx *c = e->boxes[e->multi[i]];
+        uint64_t b = c->rb_blocks;
+        if (want > budget) b = std::max<uint64_t>(1, (budget - e->multi.size()) * c->hs.h.np / std::max<uint64_t>(total_np, 1));
+        nb[i] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, c->rb_blocks));
+    }
+    return nb;
+}
+
+static int ens_plan(sph_ensemble *e)
+{
+    ens_drop_plan(e);
+    for (size_t k = 0; k < e->boxes.size(); ++k)
+        if (ens_multi_eligible(e->boxes[k])) e->multi.push_back((int)k);
+    e->plan_gen.clear();
+    e->plan_parity.clear();
+    e->plan_keys.clear();
+    for (int k : e->multi) {
+        const sph_ctx *c = e->boxes[k];
+        e->plan_gen.push_back(c->gen);
+        e->plan_parity.push_back(c->nl_parity);
+        e->plan_keys.push_back(c->db.keys);
+    }
+    e->phase = 0;
+    const std::vector<uint32_t> rbn = ens_rebuild_blocks(e);
+    e->launches_per_step = 0;
+    for (int j = 0; j < 2; ++j) {
+        std::vector<RebuildTask> rb;
+        std::vector<NlBuildTask> nlb;
+        std::vector<TopoTask> topo[2];
+        std::vector<ForcesTask> frc[2];
+        std::vector<BondTask> bnd;
+        std::vector<IntegrateTask> itg;
+        std::vector<WallsTask> wls;
+        std::vector<ClockTask> clk;
+        std::vector<uint32_t> n_rb, n_nlb, n_topo[2], n_frc[2], n_bnd, n_itg;
+        for (size_t i = 0; i < e->multi.size(); ++i) {
+            const sph_ctx *c = e->boxes[e->multi[i]];
+            const sph_state &h = c->hs.h;
+            const StepArgs a = step_args(c);
+            const uint32_t cur = c->nl_parity ^ (uint32_t)j;
+            DemBuffers b = c->db;
+            const bool odd = (radix_passes(c) & 1) != 0;
+            if (odd && j == 1) {
+                std::swap(b.keys, b.keys_tmp);
+                std::swap(b.vals, b.vals_tmp);
+            }
+            const uint32_t *cond = c->db.nl_flags + cur;
+            rb.push_back(rebuild_task(a, b, c->ncells, c->key_bits, cond, c->rb_spin));
+            n_rb.push_back(rbn[i]);
+            if (odd) {  // the sorted keys end in the other buffer (launch_rebuild)
+                std::swap(b.keys, b.keys_tmp);
+                std::swap(b.vals, b.vals_tmp);
+            }
+            nlb.push_back(nl_build_task(a, c->nla, b, cond));
+            n_nlb.push_back((h.np + 255) / 256);
+            const int near = c->nla.range == 1 ? 1 : 0;
+            topo[near].push_back(topology_task(a, b, cur));
+            n_topo[near].push_back(topology_blocks(h.np));
+            const int hz = h.contactmodel == 3 ? 1 : 0;
+            frc[hz].push_back(forces_task(a, b, true));
+            n_frc[hz].push_back(forces_blocks(h.np));
+            if (h.nb0 > 0) {
+                bnd.push_back(bond_task(a, b));
+                n_bnd.push_back((h.nb0 + 255) / 256);
+            }
+            NextHash nh{};
+            nh.keys = b.keys;
+            nh.vals = b.vals;
+            nh.cellc = b.cellc;
+            nh.x_ref = b.x_ref;
+            nh.disp = b.nl_disp;
+            nh.flags = b.nl_flags;
+            nh.cur = cur ^ 1u;  // the next step's word
+            nh.disp2 = c->nla.disp2;
+            itg.push_back(integrate_task(a, b, 0, 10.0, h.tau_x, true, nh));
+            n_itg.push_back((h.np + 255) / 256);
+            if (h.nw > 0) wls.push_back(walls_task(a, b));
+            clk.push_back(ClockTask{b.clock, h.time_dt});
+        }
+        EnsPhase &P = e->ph[j];
+        int rc;
+        if ((rc = ens_stage(e, P.rebuild, rb, n_rb)) || (rc = ens_stage(e, P.nlbuild, nlb, n_nlb)) ||
+            (rc = ens_stage(e, P.topo[0], topo[0], n_topo[0])) ||
+            (rc = ens_stage(e, P.topo[1], topo[1], n_topo[1])) ||
+            (rc = ens_stage(e, P.forces[0], frc[0], n_frc[0])) ||
+            (rc = ens_stage(e, P.forces[1], frc[1], n_frc[1])) ||
+            (rc = ens_stage(e, P.bonds, bnd, n_bnd)) || (rc = ens_stage(e, P.integ, itg, n_itg)) ||
+            (rc = ens_array(e, P.walls, wls)) || (rc = ens_array(e, P.clocks, clk)))
+            return rc;
+        P.nwalls = (uint32_t)wls.size();
+        P.nclocks = (uint32_t)clk.size();
+        if (j == 0) {
+            const EnsStage *all[] = {&P.rebuild, &P.nlbuild, &P.topo[0], &P.topo[1], &P.forces[0],
+                                     &P.forces[1], &P.bonds, &P.integ};
+            for (const EnsStage *st : all) e->launches_per_step += st->blocks ? 1u : 0u;
+            e->launches_per_step += (P.nwalls ? 1u : 0u) + (P.nclocks ? 1u : 0u);
+        }
+    }
+    e->planned = true;
+    return SPH_OK;
+}
+
+// The plan still describes the boxes: same boxes eligible, same loads, and
+// the host values it assumed for the current phase.
+static bool ens_plan_current(const sph_ensemble *e)
+{
+    if (!e->planned) return false;
+    size_t i = 0;
+    for (size_t k = 0; k < e->boxes.size(); ++k) {
+        const sph_ctx *c = e->boxes[k];
+        const bool in = i < e->multi.size() && e->multi[i] == (int)k;
+        if (ens_multi_eligible(c) != in) return false;
+        if (!in) continue;
+        if (c->gen != e->plan_gen[i]) return false;
+        if (c->nl_parity != (e->plan_parity[i] ^ e->phase)) return false;
+        const bool odd = (radix_passes(c) & 1) != 0;
+        const bool swapped = odd && e->phase == 1;
+        if ((c->db.keys == e->plan_keys[i]) == swapped) return false;
+        ++i;
+    }
+    return i == e->multi.size();
+}
+
+static void ens_enqueue_phase(sph_ensemble *e, const EnsPhase &P)
+{
+    hipStream_t s = e->stream;
+    auto mm = [](const EnsStage &st) { return MultiMap{st.map, st.nbox}; };
+    launch_rebuild_multi(s, (const RebuildTask *)P.rebuild.tasks, mm(P.rebuild), P.rebuild.blocks);
+    launch_nl_build_multi(s, (const NlBuildTask *)P.nlbuild.tasks, mm(P.nlbuild), P.nlbuild.blocks);
+    launch_topology_multi(s, (const TopoTask *)P.topo[0].tasks, mm(P.topo[0]), P.topo[0].blocks, false);
+    launch_topology_multi(s, (const TopoTask *)P.topo[1].tasks, mm(P.topo[1]), P.topo[1].blocks, true);
+    launch_forces_multi(s, (const ForcesTask *)P.forces[0].tasks, mm(P.forces[0]), P.forces[0].blocks, false);
+    launch_forces_multi(s, (const ForcesTask *)P.forces[1].tasks, mm(P.forces[1]), P.forces[1].blocks, true);
+    launch_bonds_multi(s, (const BondTask *)P.bonds.tasks, mm(P.bonds), P.bonds.blocks);
+    launch_integrate_multi(s, (const IntegrateTask *)P.integ.tasks, mm(P.integ), P.integ.blocks);
+    launch_walls_multi(s, (const WallsTask *)P.walls, P.nwalls);
+    launch_clock_multi(s, (const ClockTask *)P.clocks, P.nclocks);
+}
+
+// The host side of one step of a multi-box box (what step_once changes).
+static void ens_host_advance(sph_ctx *c)
+{
+    sph_state &h = c->hs.h;
+    c->nl_parity ^= 1u;
+    if (radix_passes(c) & 1) {
+        std::swap(c->db.keys, c->db.keys_tmp);
+        std::swap(c->db.vals, c->db.vals_tmp);
+    }
+    c->nl_force = 0;
+    c->hash_ready = true;
+    h.time_current += h.time_dt;
+    c->filetimeclock += h.time_dt;
+    c->iter++;
+}
+
+static void ens_advance_all(sph_ensemble *e)
+{
+    for (int k : e->multi) ens_host_advance(e->boxes[k]);
+    e->phase ^= 1u;
+    e->multi_steps++;
+}
+
+// The steps that can be enqueued without the host: no fluid on the
+// box-by-box path is checked by step_once itself; here, no output event
+// and no change of the velocity-step state (src/device.cu:2461-2469) inside
+// the steps, and no adaptive grid.
+static int ens_check(sph_ensemble *e, int nsteps)
+{
+    for (size_t i = 0; i < e->boxes.size(); ++i) {
+        sph_ctx *c = e->boxes[i];
+        const sph_state &h = c->hs.h;
+        const std::string b = "box " + std::to_string(i) + ": ";
+        if (!c->loaded) return ens_fail(e, SPH_ERR_STATE, b + "no state loaded");
+        if (h.fluid) return ens_fail(e, SPH_ERR_VALUE, b + "fluid boxes are stepped with sph_step");
+        if (h.adaptive == 1 && h.nw > 0) return ens_fail(e, SPH_ERR_VALUE, b + "adaptive grid");
+        double clk = c->filetimeclock, t = h.time_current;
+        int vs = c->velocity_state;
+        for (int k = 0; k < nsteps; ++k) {
+            if ((t >= 10.0 && t < 15.0 && vs == 1) || (t >= 15.0 && vs == 2))
+                return ens_fail(e, SPH_ERR_VALUE, b + "the velocity step at t = 10 s or 15 s falls in the steps");
+            clk += h.time_dt;
+            t += h.time_dt;
+        }
+        if (clk >= h.time_file_dt) return ens_fail(e, SPH_ERR_VALUE, b + "an output event falls in the steps");
+    }
+    return SPH_OK;
+}
+
+// Two steps of every multi-box box as one graph.  Nothing is pending on
+// the stream when capture begins, and the captured work is kernel launches
+// on this one stream only (no events, memsets or host calls).
+static int ens_capture(sph_ensemble *e)
+{
+    HIPCHK_E(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    ens_enqueue_phase(e, e->ph[0]);
+    ens_enqueue_phase(e, e->ph[1]);
+    hipGraph_t g = nullptr;
+    HIPCHK_E(hipStreamEndCapture(e->stream, &g));
+    e->graph = g;
+    HIPCHK_E(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
+    return SPH_OK;
+}
+
+extern "C" {
+
+// A new list skin for a loaded context on the list path: the next step
+// rebuilds the lists (exact for any skin, so the results do not change).
+static int nl_set_skin(sph_ctx *c, double frac)
+{
+    const sph_state &h = c->hs.h;
+    const double skin = frac * c->nl_rmax;
+    const uint32_t range = frac > 0.0 ? nl_range(h, c->nl_rmax, skin) : 0u;
+    if (range == 0) return SPH_ERR_VALUE;
+    c->nl_frac = frac;
+    c->nla.skin = skin;
+    c->nla.disp2 = (0.45 * skin) * (0.45 * skin);
+    c->nla.range = range;
+    c->nl_force = 1;
+    return SPH_OK;
+}
+
+int sph_ensemble_autotune(sph_ensemble *e, int trial_steps, const double *fractions, int nfrac,
+                          double *chosen, double *ms_per_step)
+{
+    if (!e || trial_steps <= 0 || !fractions || nfrac <= 0) return SPH_ERR_ARG;
+    (void)hipSetDevice(e->device);
+    for (int f = 0; f < nfrac; ++f)
+        if (!(fractions[f] > 0.0 && fractions[f] <= 1.0))
+            return ens_fail(e, SPH_ERR_ARG, "autotune: skin fractions must lie in (0, 1]");
+    int rc = ens_check(e, trial_steps * nfrac);
+    if (rc) return rc;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIPCHK_E(hipEventCreate(&t0));
+    HIPCHK_E(hipEventCreate(&t1));
+    for (size_t k = 0; k < e->boxes.size(); ++k) {
+        sph_ctx *c = e->boxes[k];
+        double best = 1e300, best_f = c->nl_frac;
+        const bool tunable = c->loaded && c->nl_on;
+        for (int f = 0; f < nfrac; ++f) {
+            double ms = -1.0;
+            if (tunable && nl_set_skin(c, fractions[f]) == SPH_OK) {
+                // the first step rebuilds for the new skin; it is timed with
+                // the rest, as a run with this skin would pay it once
+                if ((rc = sync(c))) break;
+                if (hipEventRecord(t0, c->stream) != hipSuccess) { rc = SPH_ERR_HIP; break; }
+                for (int s = 0; s < trial_steps && !rc; ++s)
+                    rc = step_once(c, nullptr, nullptr, std::string());
+                if (rc) break;
+                if ((rc = wait_walls(c))) break;
+                float el = 0.0f;
+                if (hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                    hipEventElapsedTime(&el, t0, t1) != hipSuccess) { rc = SPH_ERR_HIP; break; }
+                ms = el / trial_steps;
+                if (ms < best) {
+                    best = ms;
+                    best_f = fractions[f];
+                }
+            } else {
+                // not tunable: the steps still run, so that every box keeps
+                // the same step count
+                for (int s = 0; s < trial_steps && !rc; ++s)
+                    rc = step_once(c, nullptr, nullptr, std::string());
+                if (rc) break;
+            }
+            if (ms_per_step) ms_per_step[k * nfrac + f] = ms;
+        }
+        if (rc) {
+            (void)hipEventDestroy(t0);
+            (void)hipEventDestroy(t1);
+            return ens_fail(e, rc, c->err.empty() ? "autotune: step failed" : c->err);
+        }
+        if (tunable && best_f != c->nl_frac) (void)nl_set_skin(c, best_f);
+        if (chosen) chosen[k] = tunable ? best_f : 0.0;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    for (sph_ctx *c : e->boxes) {
+        if ((rc = check_flags(c))) return ens_fail(e, rc, c->err);
+        if ((rc = sync(c))) return ens_fail(e, rc, c->err);
+    }
+    e->steps += (uint64_t)trial_steps * (uint64_t)nfrac;
+    ens_drop_plan(e);  // the task records hold the old list parameters
+    return SPH_OK;
 }
 
 sph_ensemble *sph_ensemble_create(sph_ctx **boxes, int n)
@@ -1828,10 +2264,23 @@ sph_ensemble *sph_ensemble_create(sph_ctx **boxes, int n)
     sph_ensemble *e = new sph_ensemble();
     e->boxes.assign(boxes, boxes + n);
     e->device = boxes[0]->device;
+    (void)hipSetDevice(e->device);
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_last_error = "ensemble: stream creation failed";
+        delete e;
+        return nullptr;
+    }
     return e;
 }
 
-void sph_ensemble_destroy(sph_ensemble *e) { delete e; }
+void sph_ensemble_destroy(sph_ensemble *e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    ens_drop_plan(e);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
 
 const char *sph_ensemble_last_error(const sph_ensemble *e)
 {
@@ -1845,21 +2294,29 @@ int sph_ensemble_stats(const sph_ensemble *e, uint64_t *steps)
     return SPH_OK;
 }
 
-// Enqueued steps must not need the host between them: no fluid (the Darcy
-// loop decides on the host), no adaptive grid (host read-back) and no output
-// event inside the steps.
-static int ens_check(sph_ensemble *e, int nsteps)
+int sph_ensemble_info(const sph_ensemble *e, sph_ensemble_info_t *out)
 {
-    for (size_t i = 0; i < e->boxes.size(); ++i) {
-        sph_ctx *c = e->boxes[i];
-        const sph_state &h = c->hs.h;
-        const std::string b = "box " + std::to_string(i) + ": ";
-        if (!c->loaded) return ens_fail(e, SPH_ERR_STATE, b + "no state loaded");
-        if (h.fluid) return ens_fail(e, SPH_ERR_VALUE, b + "fluid boxes are stepped with sph_step");
-        if (h.adaptive == 1 && h.nw > 0) return ens_fail(e, SPH_ERR_VALUE, b + "adaptive grid");
-        double clk = c->filetimeclock;
-        for (int k = 0; k < nsteps; ++k) clk += h.time_dt;
-        if (clk >= h.time_file_dt) return ens_fail(e, SPH_ERR_VALUE, b + "an output event falls in the steps");
+    if (!e || !out) return SPH_ERR_ARG;
+    *out = sph_ensemble_info_t{};
+    out->boxes = (uint32_t)e->boxes.size();
+    out->multi_boxes = (uint32_t)e->multi.size();
+    out->launches_per_step = e->launches_per_step;
+    out->graph = e->exec ? 1 : 0;
+    out->multi_steps = e->multi_steps;
+    out->graph_replays = e->graph_replays;
+    return SPH_OK;
+}
+
+int sph_ensemble_set_graph(sph_ensemble *e, int on)
+{
+    if (!e) return SPH_ERR_ARG;
+    e->use_graph = on != 0;
+    if (!e->use_graph && e->exec) {
+        (void)hipStreamSynchronize(e->stream);
+        (void)hipGraphExecDestroy(e->exec);
+        (void)hipGraphDestroy(e->graph);
+        e->exec = nullptr;
+        e->graph = nullptr;
     }
     return SPH_OK;
 }
@@ -1871,9 +2328,70 @@ int sph_ensemble_step(sph_ensemble *e, int nsteps)
     (void)hipSetDevice(e->device);
     int rc = ens_check(e, nsteps);
     if (rc) return rc;
+    // nothing of the boxes' own streams may be pending under the
+    // multi-box launches
+    for (sph_ctx *c : e->boxes)
+        if ((rc = sync(c))) return ens_fail(e, rc, c->err);
+    if (!ens_plan_current(e) && (rc = ens_plan(e))) return rc;
+    std::vector<sph_ctx *> single;
+    {
+        size_t i = 0;
+        for (size_t k = 0; k < e->boxes.size(); ++k) {
+            if (i < e->multi.size() && e->multi[i] == (int)k) ++i;
+            else single.push_back(e->boxes[k]);
+        }
+    }
+    int left = nsteps;
+    if (!e->multi.empty()) {
+        // the hash of a box's first step after a load (later ones are
+        // formed by its integrate)
+        bool hashed = false;
+        for (int k : e->multi) {
+            sph_ctx *c = e->boxes[k];
+            if (c->nl_force || !c->hash_ready) {
+                launch_hash_nl(e->stream, step_args(c), c->nla, c->db, c->nl_parity, c->nl_force ? 1u : 0u);
+                hashed = true;
+            }
+        }
+        if (hashed || e->phase == 1) {
+            ens_enqueue_phase(e, e->ph[e->phase]);
+            ens_advance_all(e);
+            --left;
+        }
+        if (left >= 2 && e->use_graph && !e->exec) {
+            HIPCHK_E(hipStreamSynchronize(e->stream));
+            if ((rc = ens_capture(e))) {
+                // capture refused: run the launches without the graph
+                if (e->exec) (void)hipGraphExecDestroy(e->exec);
+                if (e->graph) (void)hipGraphDestroy(e->graph);
+                e->exec = nullptr;
+                e->graph = nullptr;
+                e->use_graph = false;
+                (void)hipGetLastError();
+            }
+        }
+        while (left >= 2) {
+            if (e->exec) {
+                HIPCHK_E(hipGraphLaunch(e->exec, e->stream));
+                e->graph_replays++;
+            } else {
+                ens_enqueue_phase(e, e->ph[0]);
+                ens_enqueue_phase(e, e->ph[1]);
+            }
+            ens_advance_all(e);
+            ens_advance_all(e);
+            left -= 2;
+        }
+        if (left == 1) {
+            ens_enqueue_phase(e, e->ph[e->phase]);
+            ens_advance_all(e);
+        }
+    }
+    // the other boxes, box by box on their own streams
     for (int k = 0; k < nsteps; ++k)
-        for (sph_ctx *c : e->boxes)
+        for (sph_ctx *c : single)
             if ((rc = step_once(c, nullptr, nullptr, std::string()))) return ens_fail(e, rc, c->err);
+    HIPCHK_E(hipStreamSynchronize(e->stream));
     for (sph_ctx *c : e->boxes) {
         if ((rc = check_flags(c))) return ens_fail(e, rc, c->err);
         if ((rc = sync(c))) return ens_fail(e, rc, c->err);

@@ -131,6 +131,145 @@ void launch_count_contacts(hipStream_t s, const uint64_t *cmask, uint32_t np,
                            unsigned long long *out);
 void launch_rebuild_mask(hipStream_t s, const DemBuffers &b, uint32_t np, uint32_t nc);
 
+// ------------------------------------------------------------------
+// Multi-box launches (ensembles of independent boxes, no counterpart in
+// the reference): one launch runs one stage of a step for every box.  The
+// launch's blocks are divided among the boxes: box k owns the blocks
+// [start[k], start[k] + count[k]).  The starts are multiples of 8, so that a
+// box's blocks meet the eight XCDs in the order of a launch of its own;
+// the blocks between one box's last block and the next start return at
+// once.  Each box's arguments are one task record in device memory, the
+// arguments of the single-box kernel, which runs the same body.
+// ------------------------------------------------------------------
+struct MultiMap {
+    const uint32_t *start;  // nbox block starts (ascending), then nbox counts
+    uint32_t nbox;
+};
+
+struct RebuildTask {
+    StepArgs a;
+    RebuildArgs r;
+};
+
+struct NlBuildTask {
+    StepArgs a;
+    NLArgs n;
+    const uint32_t *cond, *vals, *cell_start, *cell_end;
+    const double4 *x_s;
+    uint32_t *nl, *nl_cnt, *nl_pos;
+    double4 *x_ref;
+    uint32_t *flags;
+    const uint32_t *contacts;
+    const uint64_t *cmask;
+    uint8_t *nl_slot, *slot_cand;
+    float *nl_gap, *nl_gmin;
+    const uint32_t *err;
+};
+
+struct TopoTask {
+    StepArgs a;
+    const uint32_t *vals, *cell_start, *cell_end;
+    const double4 *x, *x_s;
+    uint32_t *contacts;
+    uint64_t *cmask;
+    uint32_t *err;
+    const uint32_t *nl;
+    uint8_t *nl_slot, *slot_cand;
+    const uint32_t *nl_cnt, *nl_pos, *cellc;
+    const float *nl_gap, *nl_gmin, *nl_disp;
+    uint32_t *flags;
+    uint32_t cur;
+};
+
+struct ForcesTask {
+    StepArgs a;
+    const uint32_t *order;
+    const double4 *x, *vel;
+    const double3 *angvel;
+    double3 *force, *torque;
+    double *es_dot, *ev_dot, *es, *ev, *p;
+    const double4 *w_nx, *w_mvfd;
+    double *walls_force_pp;
+    uint32_t *contacts;
+    uint64_t *cmask;
+    double *delta_t;
+    uint8_t *nl_slot, *slot_cand;
+    const float *nl_gap;
+    float *nl_gmin;
+    const uint32_t *err;
+};
+
+struct BondTask {
+    StepArgs a;
+    const uint2 *bonds;
+    uint8_t *broken;
+    double4 *bdelta, *bomega;
+    const double4 *x, *vel;
+    const double3 *angvel;
+    double4 *bf, *bt;
+    uint8_t *applied;
+};
+
+struct IntegrateTask {
+    StepArgs a;
+    const StepClock *clock;
+    int change_velocity_state;
+    double velocity_factor, tau_x;
+    double4 *x, *vel;
+    double3 *angvel, *force, *torque, *angpos, *acc, *angacc, *xyzsum;
+    const double4 *w_mvfd;
+    const double *tau_eff, *walls_force_pp;
+    double *wall_partial;
+    BondGather bg;
+    NextHash nh;
+};
+
+struct WallsTask {
+    StepArgs a;
+    const StepClock *clock;
+    double4 *w_nx, *w_mvfd;
+    double *w_acc;
+    const double *partial;
+    uint32_t nparts;
+};
+
+struct ClockTask {
+    StepClock *clock;
+    double dt;
+};
+
+// Task records of one box for one step, as the single-box launchers form
+// them (rebuild: the sort passes' key buffers as they are before the step).
+RebuildTask rebuild_task(const StepArgs &a, const DemBuffers &b, uint32_t ncells, int bits,
+                         const uint32_t *cond, uint32_t spin_limit);
+NlBuildTask nl_build_task(const StepArgs &a, const NLArgs &n, const DemBuffers &b,
+                          const uint32_t *cond);
+TopoTask topology_task(const StepArgs &a, const DemBuffers &b, uint32_t cur);
+ForcesTask forces_task(const StepArgs &a, const DemBuffers &b, bool nlist);
+BondTask bond_task(const StepArgs &a, const DemBuffers &b);
+IntegrateTask integrate_task(const StepArgs &a, const DemBuffers &b, int change_velocity_state,
+                             double velocity_factor, double tau_x, bool gather_bonds,
+                             const NextHash &nh);
+WallsTask walls_task(const StepArgs &a, const DemBuffers &b);
+// Blocks a box needs in each multi-box stage.
+uint32_t topology_blocks(uint32_t np);
+uint32_t forces_blocks(uint32_t np);
+uint32_t nl_build_blocks(uint32_t np);
+// Multi-box launches of `blocks` blocks (the last start plus its count,
+// rounded up) over device task arrays; near/hertz pick the kernel variant
+// (all boxes of one launch share it).
+void launch_rebuild_multi(hipStream_t s, const RebuildTask *t, MultiMap m, uint32_t blocks);
+void launch_nl_build_multi(hipStream_t s, const NlBuildTask *t, MultiMap m, uint32_t blocks);
+void launch_topology_multi(hipStream_t s, const TopoTask *t, MultiMap m, uint32_t blocks,
+                           bool near);
+void launch_forces_multi(hipStream_t s, const ForcesTask *t, MultiMap m, uint32_t blocks,
+                         bool hertz);
+void launch_bonds_multi(hipStream_t s, const BondTask *t, MultiMap m, uint32_t blocks);
+void launch_integrate_multi(hipStream_t s, const IntegrateTask *t, MultiMap m, uint32_t blocks);
+// one block per task
+void launch_walls_multi(hipStream_t s, const WallsTask *t, uint32_t n);
+void launch_clock_multi(hipStream_t s, const ClockTask *t, uint32_t n);
+
 // words of DarcyBuffers::maxres: [0], [1] alternating check slots;
 // DARCY_STATE: solve state (ended, nijac of the last check, its max |norm|
 // bits, corrections applied); DARCY_ZERO: always 0 (the state of sweeps run

@@ -1,6 +1,7 @@
 // net_kernels.hip -- gfx950 kernels of the Darcy correction network: 3x3x3
 // convolutions (stride 1, zero padding 1) as implicit GEMMs on the matrix
-// cores (v_mfma_f32_16x16x16_bf16: BF16 operands, FP32 accumulation), their
+// cores (v_mfma_f32_16x16x32_bf16 for the forward pass, 16x16x16 for the
+// weight gradients: BF16 operands, FP32 accumulation), their
 // data and weight gradients, 2x average pooling, nearest upsampling with a
 // skip add, and the ReLU mask.
 //
@@ -13,7 +14,24 @@
 #include "net_kernels.h"
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// gfx950's v_mfma_f32_16x16x32_bf16: A[m][k] with m = lane % 16 and k = 8
+// (lane / 16) + 0..7 (eight BF16 per lane), B[k][n] with n = lane % 16 and k
+// likewise, D as for the 16x16x16 shape.  Half the instructions, and half
+// the dependent steps of a K loop, of the 16x16x16 shape.
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ s16x8 cat8(bf16x4 lo, bf16x4 hi)
+{
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
 
 
 // neighbour (x, y, z) of the 3x3x3 stencil: wrapped along periodic axes,
@@ -36,14 +54,14 @@ __device__ __forceinline__ void vox_xyz(const NetGeom &g, uint32_t v, int &x, in
 
 // ---------------------------------------------------------------------------
 // Forward convolution.  A wave computes MT 16-voxel M tiles by all 16*NT
-// output channels; K runs over steps = (tap, 16-channel chunk of the input)
-// (CIN4: inputs of 4 channels, four taps per step, one per 16-lane group).
-// With KS > 1 the KS waves of a block share the M tiles and take every
-// KS-th step, and their partial sums meet in LDS (small grids: more waves,
-// shorter chains).  Each wave issues the loads of G steps before their
-// MFMAs.  MFMA 16x16x16 lane layout: A[m][k] with m = lane % 16,
-// k = 4 (lane / 16) + 0..3; B[k][n] with n = lane % 16, k likewise; the
-// accumulator holds D[4 (lane / 16) + i][lane % 16].
+// output channels; K runs over 32-wide steps: two (tap, 16-channel chunk)
+// pairs of the input (CIN4: inputs of 4 channels, eight taps per step, two
+// per 16-lane group).  With KS > 1 the KS waves of a block share the M
+// tiles and take every KS-th step, and their partial sums meet in LDS
+// (small grids: more waves, shorter chains).  Each wave issues the loads of
+// G steps before their MFMAs.  MFMA 16x16x32 lane layout: A[m][k] with
+// m = lane % 16, k = 8 (lane / 16) + 0..7; B[k][n] with n = lane % 16, k
+// likewise; the accumulator holds D[4 (lane / 16) + i][lane % 16].
 // ---------------------------------------------------------------------------
 #define CONV_G 4
 // K split of the grids of at most CONV_SMALL_V voxels: CONV_KS_SMALL waves
@@ -165,46 +183,74 @@ __global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd(NetGeom g, 
     for (int j = 0; j < MT; ++j)
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[j][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // K runs over 32-wide steps u.  Inputs of 16k channels: 16-channel steps
+    // st = (tap, chunk) taken in pairs, lane group grp reading channels
+    // 8 (grp & 1) .. + 7 of step 2u + (grp >> 1).  Inputs of 4 channels: taps
+    // 8u + 2 grp and 8u + 2 grp + 1, four channels each.
     const int chunks = CIN4 ? 1 : (cin >> 4);
-    const int S = CIN4 ? 7 : 27 * chunks;
-    for (int s0 = kpart; s0 < S; s0 += KS * CONV_G) {
-        bf16x4 a[CONV_G][MT], b[CONV_G][NT];
+    const int S = CIN4 ? 27 : 27 * chunks;
+    const int U = CIN4 ? 4 : (S + 1) / 2;
+    const int half = (int)(grp & 1u), sub = (int)(grp >> 1);
+    const int cs = CIN4 ? 4 : cin;
+    const s16x8 zero8 = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    for (int u0 = kpart; u0 < U; u0 += KS * CONV_G) {
+        s16x8 a[CONV_G][MT], b[CONV_G][NT];
 #pragma unroll
         for (int q = 0; q < CONV_G; ++q) {
-            const int st = s0 + q * KS;
-            int tap, ci;
+            const int u = u0 + q * KS;
             if (CIN4) {
-                tap = 4 * st + (int)grp;
-                ci = 0;
+                bf16x4 bl[NT][2], al[MT][2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int tap = 8 * u + 2 * (int)grp + e;
+                    const bool tok = u < U && tap < 27;
+                    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        bl[t][e] = bf16x4{0, 0, 0, 0};
+                        if (tok) bl[t][e] = *(const bf16x4 *)(w + ((size_t)(tap * cout + t * 16 + (int)m) * 4));
+                    }
+#pragma unroll
+                    for (int j = 0; j < MT; ++j) {
+                        int x = vx[j] + dx, y = vy[j] + dy;
+                        const int z = vz[j] + dz;
+                        al[j][e] = bf16x4{0, 0, 0, 0};
+                        if (tok && vin[j] && nb_in(g, x, y, z))
+                            al[j][e] = *(const bf16x4 *)(in + ((size_t)((z * ny + y) * nx + x) * 4));
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < NT; ++t) b[q][t] = cat8(bl[t][0], bl[t][1]);
+#pragma unroll
+                for (int j = 0; j < MT; ++j) a[q][j] = cat8(al[j][0], al[j][1]);
             } else {
-                tap = st / chunks;
-                ci = (st - tap * chunks) * 16 + 4 * (int)grp;
-            }
-            const bool tok = st < S && tap < 27;
-            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-            const int cs = CIN4 ? 4 : cin;
+                const int st = 2 * u + sub;
+                const bool tok = u < U && st < S;
+                const int tap = st / chunks;
+                const int ci = (st - tap * chunks) * 16 + 8 * half;
+                const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                b[q][t] = bf16x4{0, 0, 0, 0};
-                if (tok) b[q][t] = *(const bf16x4 *)(w + ((size_t)(tap * cout + t * 16 + (int)m) * cs + ci));
-            }
+                for (int t = 0; t < NT; ++t) {
+                    b[q][t] = zero8;
+                    if (tok) b[q][t] = *(const s16x8 *)(w + ((size_t)(tap * cout + t * 16 + (int)m) * cs + ci));
+                }
 #pragma unroll
-            for (int j = 0; j < MT; ++j) {
-                int x = vx[j] + dx, y = vy[j] + dy;
-                const int z = vz[j] + dz;
-                a[q][j] = bf16x4{0, 0, 0, 0};
-                if (tok && vin[j] && nb_in(g, x, y, z))
-                    a[q][j] = *(const bf16x4 *)(in + ((size_t)((z * ny + y) * nx + x) * cs + ci));
+                for (int j = 0; j < MT; ++j) {
+                    int x = vx[j] + dx, y = vy[j] + dy;
+                    const int z = vz[j] + dz;
+                    a[q][j] = zero8;
+                    if (tok && vin[j] && nb_in(g, x, y, z))
+                        a[q][j] = *(const s16x8 *)(in + ((size_t)((z * ny + y) * nx + x) * cs + ci));
+                }
             }
         }
 #pragma unroll
         for (int q = 0; q < CONV_G; ++q) {
-            if (s0 + q * KS >= S) break;
+            if (u0 + q * KS >= U) break;
 #pragma unroll
             for (int j = 0; j < MT; ++j)
 #pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q][j], b[q][t], acc[j][t], 0, 0, 0);
+                for (int t = 0; t < NT; ++t) acc[j][t] = mfma32(a[q][j], b[q][t], acc[j][t]);
         }
     }
     if (KS > 1) {
@@ -357,30 +403,47 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(NetGeom g, const uint16_t 
     f32x4 acc[4][1];
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x4 zero4 = bf16x4{0, 0, 0, 0};
     if (CIN4) {
-        // every lane takes part in every MFMA; the padding tap 27 of the
-        // last step reads tap 26's operands and zeroes them
-        for (int st = 0; st < 7; ++st) {
-            const int tap = 4 * st + (int)grp;
-            const bool tok = tap < 27;
-            const int tc = tok ? tap : 26;
-            const int dz = tc / 9, dy = (tc / 3) % 3, dx = tc % 3;  // halo offsets 0..2
-            const bf16x4 zero4 = bf16x4{0, 0, 0, 0};
-            const bf16x4 b = tok ? s_w[tc * 16 + (int)m] : zero4;
+        // 32-wide steps of eight taps: lane group grp takes taps 8 st + 2 grp
+        // and + 1 (four channels each); the padding taps 27..31 of the last
+        // step read tap 26's operands and zero them
+        for (int st = 0; st < 4; ++st) {
+            bf16x4 al[4][2], bl[2];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const bf16x4 a = s_in[((j + dz) * HY + ((int)wv + dy)) * HX + ((int)m + dx)];
-                acc[j][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(tok ? a : zero4, b, acc[j][0], 0, 0, 0);
+            for (int e = 0; e < 2; ++e) {
+                const int tap = 8 * st + 2 * (int)grp + e;
+                const bool tok = tap < 27;
+                const int tc = tok ? tap : 26;
+                const int dz = tc / 9, dy = (tc / 3) % 3, dx = tc % 3;  // halo offsets 0..2
+                bl[e] = tok ? s_w[tc * 16 + (int)m] : zero4;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const bf16x4 a = s_in[((j + dz) * HY + ((int)wv + dy)) * HX + ((int)m + dx)];
+                    al[j][e] = tok ? a : zero4;
+                }
             }
+            const s16x8 b = cat8(bl[0], bl[1]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[j][0] = mfma32(cat8(al[j][0], al[j][1]), b, acc[j][0]);
         }
     } else {
-        for (int tap = 0; tap < 27; ++tap) {
-            const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-            const bf16x4 b = s_w[(tap * 16 + (int)m) * 4 + (int)grp];
+        // 32-wide steps of two taps: lane group grp reads channels
+        // 8 (grp & 1) .. + 7 of tap 2 st + (grp >> 1); the padding tap 27
+        // reads zeros
+        const int half = (int)(grp & 1u), sub = (int)(grp >> 1);
+        for (int st = 0; st < 14; ++st) {
+            const int tap = 2 * st + sub;
+            const bool tok = tap < 27;
+            const int tc = tok ? tap : 26;
+            const int dz = tc / 9, dy = (tc / 3) % 3, dx = tc % 3;
+            const bf16x4 *wp = &s_w[(tc * 16 + (int)m) * 4 + 2 * half];
+            const s16x8 b = tok ? cat8(wp[0], wp[1]) : cat8(zero4, zero4);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const bf16x4 a = s_in[(((j + dz) * HY + ((int)wv + dy)) * HX + ((int)m + dx)) * 4 + (int)grp];
-                acc[j][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j][0], 0, 0, 0);
+                const bf16x4 *ap = &s_in[(((j + dz) * HY + ((int)wv + dy)) * HX + ((int)m + dx)) * 4 + 2 * half];
+                const s16x8 a = tok ? cat8(ap[0], ap[1]) : cat8(zero4, zero4);
+                acc[j][0] = mfma32(a, b, acc[j][0]);
             }
         }
     }

@@ -1,7 +1,9 @@
 """Ensembles: boxes of different sizes, contact laws and boundaries stepped
-together (all boxes' kernels enqueued before the host waits) give, box by
-box, exactly the state that sph_step gives for that box alone (bit for bit,
-every array)."""
+together give, box by box, exactly the state that sph_step gives for that
+box alone (bit for bit, every array).  Boxes on the list path share one
+launch per stage (pairs of steps replayed from a captured HIP graph, or
+enqueued step by step); the others run box by box beside them.  The skin
+autotuner changes no result."""
 import numpy
 import pytest
 
@@ -36,15 +38,19 @@ def _same(a, b, n, label):
 
 
 def test_ensemble_matches_single_boxes():
+    """Four list-path boxes and one box searched cell by cell (stepped box
+    by box beside the multi-box launches)."""
     boxes = _boxes()
+    boxes.append(boxes[1])
+    skins = [None] * 4 + [0.0]
     singles = []
-    for s in boxes:
-        e = sphere_amd.Engine(0, nc=64)
+    for s, sk in zip(boxes, skins):
+        e = sphere_amd.Engine(0, nc=64, skin=sk)
         e.load(s.to_state())
         singles.append(e)
     engines = []
-    for s in boxes:
-        e = sphere_amd.Engine(0, nc=64)
+    for s, sk in zip(boxes, skins):
+        e = sphere_amd.Engine(0, nc=64, skin=sk)
         e.load(s.to_state())
         engines.append(e)
     ens = sphere_amd.Ensemble(engines)
@@ -57,7 +63,95 @@ def test_ensemble_matches_single_boxes():
             _same(e.read(), ref.read(), npart, "box %d after %d more steps" % (k, n))
             assert numpy.array_equal(e.read_contacts()[0], ref.read_contacts()[0]), k
             assert e.header().time_current == ref.header().time_current
+        if n == 10:
+            # a box stepped on its own between ensemble calls (the
+            # ensemble then plans its launches again)
+            engines[0].step(1)
+            singles[0].step(1)
+    info = ens.info()
+    print("ensemble info", info)
+    assert info["boxes"] == 5 and info["multi_boxes"] == 4 and info["graph_replays"] > 0
     assert ens.stats()["steps"] == 26 + 10 + 5 + 8
+    ens.close()
+
+
+def _shear_boxes(n, seed):
+    rng = numpy.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        npart = int(rng.integers(400, 1600))
+        s = dense_case(n=npart, seed=seed + k, model=3 if k % 3 else 2, periodic=1 + (k % 2),
+                       nw=1, E=7e9, mu=float(rng.uniform(0.3, 0.7)))
+        top = s.x[:, 2] > numpy.max(s.x[:, 2]) - 0.01
+        s.fixvel[top] = 1.0
+        s.vel[top, 0] = float(rng.uniform(1e-3, 1e-1))
+        s.w_sigma0[0] = float(rng.uniform(5e3, 8e4))
+        if k % 4 == 1:
+            add_bonds(s, 0.2, seed + k)
+        s.time_file_dt[0] = 1.0
+        s.time_total[0] = 1.0
+        out.append(s)
+    return out
+
+
+def _engines(boxes):
+    out = []
+    for s in boxes:
+        e = sphere_amd.Engine(0, nc=64)
+        e.load(s.to_state())
+        out.append(e)
+    return out
+
+
+def test_ensemble_graph_and_stepwise_launches_equal():
+    """Ten list-path boxes (both laws, periodic 1 and 2, bonds on some):
+    replayed graph, step-by-step multi-box launches and single-box runs give
+    the same bits."""
+    boxes = _shear_boxes(10, 11)
+    a, b, c = _engines(boxes), _engines(boxes), _engines(boxes)
+    ea, eb = sphere_amd.Ensemble(a), sphere_amd.Ensemble(b)
+    eb.set_graph(False)
+    for n in (1, 7, 12):
+        ea.step(n)
+        eb.step(n)
+        for e in c:
+            e.step(n)
+    ia, ib = ea.info(), eb.info()
+    print("graph", ia, "stepwise", ib)
+    assert ia["multi_boxes"] == 10 and ia["graph"] == 1 and ia["graph_replays"] > 0
+    assert ib["graph"] == 0 and ib["multi_steps"] == 20
+    assert ia["launches_per_step"] <= 12
+    for k, s in enumerate(boxes):
+        n = int(s.np[0])
+        ra, rb, rc = a[k].read(), b[k].read(), c[k].read()
+        _same(ra, rc, n, "graph box %d" % k)
+        _same(rb, rc, n, "stepwise box %d" % k)
+        assert numpy.array_equal(a[k].read_contacts()[0], c[k].read_contacts()[0]), k
+        assert numpy.array_equal(a[k].read_contacts()[2], c[k].read_contacts()[2]), k
+    ea.close()
+    eb.close()
+
+
+def test_ensemble_autotune_changes_no_result():
+    """The skin autotuner times every box with each fraction; its trial steps
+    are ordinary steps, and the tuned ensemble stays bit-identical to the
+    boxes run alone with the default skin."""
+    boxes = _shear_boxes(6, 23)
+    a, c = _engines(boxes), _engines(boxes)
+    ens = sphere_amd.Ensemble(a)
+    fr = [0.05, 0.1, 0.3]
+    chosen, ms = ens.autotune(4, fr)
+    print("chosen", chosen, "ms per step", ms)
+    assert all(x in fr for x in chosen)
+    assert numpy.all(ms > 0.0)
+    ens.step(9)
+    for e in c:
+        e.step(4 * len(fr) + 9)
+    for k, s in enumerate(boxes):
+        n = int(s.np[0])
+        _same(a[k].read(), c[k].read(), n, "tuned box %d" % k)
+        assert numpy.array_equal(a[k].read_contacts()[0], c[k].read_contacts()[0]), k
+    assert ens.stats()["steps"] == 4 * len(fr) + 9
     ens.close()
 
 
