@@ -147,10 +147,99 @@ def darcy_section(args):
     engine.close()
     out["cpu_baseline"] = None if args.no_cpu or args.darcy_cpu_steps <= 0 else \
         darcy_cpu_leg(args, s)
-    if args.no_net:
-        return out
-    out["learned"] = learned_section(args, s, ms, sweeps)
+    params = None
+    if not args.no_net:
+        params, out["learned"] = learned_section(args, s, ms, sweeps)
+    out["ensemble"] = None if args.no_ensemble else darcy_ensemble_section(args, params)
     return out
+
+
+def darcy_ensemble_section(args, params):
+    """Independent Darcy-coupled shear boxes (config 5: grains log-uniform in
+    [5,000, 60,000], per-box friction, wall stress and shear velocity, Darcy
+    grids scaled to the box) stepped together as one Ensemble (one launch per
+    stage of the fluid step, the Jacobi sweeps of all solves batched, the
+    learned correction's forward passes as multi-box launches) against the
+    same boxes stepped one at a time with sph_step, from the same states, with
+    the network trained on the consolidation boxes."""
+    import sphere_amd
+    from sphere_amd import workloads as W
+    t0 = time.time()
+    sims, info = W.darcy_ensemble_boxes(count=args.ens_boxes, seed=args.ens_seed + 2)
+    total = sum(p["grains"] for p in info)
+    log("Darcy ensemble: %d boxes, %d grains in all, prepared in %.1f s" % (len(sims), total,
+                                                                          time.time() - t0))
+    states = [s.to_state() for s in sims]
+
+    def engines():
+        out = []
+        for st in states:
+            e = sphere_amd.Engine(0, nc=32)
+            if params is not None:
+                e.set_darcy_net(params, args.net_corrections)
+            e.load(st)
+            out.append(e)
+        return out
+
+    steps, warm = args.dens_steps, args.dens_warmup
+    eng = engines()
+    ens = sphere_amd.Ensemble(eng)
+    ens.step(warm)
+    t0 = time.perf_counter()
+    ens.step(steps)
+    t_ens = time.perf_counter() - t0
+    einfo = ens.info()
+    ens.close()
+    one = engines()
+    for e in one:
+        e.step(warm)
+    t0 = time.perf_counter()
+    for e in one:
+        e.step(steps)
+    t_one = time.perf_counter() - t0
+    dx = dp = 0.0
+    same_sweeps = True
+    sweeps = corr = fsteps = 0
+    for k, (a, b) in enumerate(zip(eng, one)):
+        n = info[k]["grains"]
+        V = int(numpy.prod(info[k]["grid"]))
+        ra, rb = a.read(), b.read()
+        dx = max(dx, float(numpy.max(numpy.abs(ra.get("x", n, 4) - rb.get("x", n, 4)))))
+        dp = max(dp, float(numpy.max(numpy.abs(ra.get("p_f", V, 1) - rb.get("p_f", V, 1)))))
+        sa, sb = a.darcy_stats(), b.darcy_stats()
+        same_sweeps = same_sweeps and sa.sweeps == sb.sweeps
+        sweeps += sa.sweeps
+        corr += sa.corrections
+        fsteps += sa.fluid_steps
+    for e in eng + one:
+        e.close()
+    v_ens = total * steps / t_ens
+    v_one = total * steps / t_one
+    log("Darcy ensemble: %.4g particle-steps/s together, %.4g one at a time (%.2fx); max |x| "
+        "difference %g, max |p_f| difference %g, sweep counts equal: %s; %.1f sweeps and %.1f "
+        "corrections per solve" % (v_ens, v_one, v_ens / v_one, dx, dp, same_sweeps,
+                                   sweeps / max(1, fsteps), corr / max(1, fsteps)))
+    sizes = sorted(p["grains"] for p in info)
+    grids = sorted(set(tuple(p["grid"]) for p in info))
+    return {
+        "metric": "aggregate particle-steps/s (independent Darcy-coupled boxes, FP64)",
+        "together": v_ens, "one_at_a_time": v_one, "speedup": v_ens / v_one,
+        "steps": steps, "warmup": warm,
+        "max_abs_position_difference_vs_one_at_a_time": dx,
+        "max_abs_pressure_difference_vs_one_at_a_time": dp,
+        "sweep_counts_equal": bool(same_sweeps),
+        "sweeps_per_solve": sweeps / max(1, fsteps), "corrections_per_solve": corr / max(1, fsteps),
+        "learned_correction": params is not None,
+        "ensemble_info": einfo,
+        "config": {"workload": "%d independent Darcy-coupled shear boxes, grains log-uniform in "
+                               "[5000, 60000] (radii U[0.009, 0.011] m, linear contacts), mu ~ "
+                               "U[0.3, 0.7], sigma0 ~ U[5, 80] kPa, top layer driven at U[1e-3, "
+                               "1e-1] m/s; Darcy grid c x c x 2c (c = 16, 24 or 32) per box, "
+                               "tolerance 1e-8" % len(info),
+                   "boxes": len(info), "grains_total": total,
+                   "grains_min_median_max": [sizes[0], sizes[len(sizes) // 2], sizes[-1]],
+                   "grids": [list(g) for g in grids]},
+    }
 
 
 def darcy_cpu_leg(args, s):
@@ -186,40 +275,124 @@ def net_flops(nx, ny, nz):
                for (ci, co, taps), lv in zip(learn.LAYERS, levels))
 
 
+def size_check(args, params, steps=3):
+    """The trained network on a 96 x 96 x 192 grid: a pressure pulse relaxing
+    in a grain-free Darcy box (0.96 x 0.96 x 1.92 m, p = 0 at the top, no
+    flow at the base, periodic sides), solved to the 1e-8 test with and
+    without the learned correction.  Without grains the porosity is 0.9
+    in every cell (the porosity kernel recomputes it each fluid step), so
+    the consolidation box's fields cannot be carried over; the check is
+    that the network, trained at 48 x 48 x 96, runs at eight times the
+    voxels and the solve ends at the plain solve's pressure."""
+    import sphere_amd
+    s = sphere_amd.sim(np=0, nd=3, nw=0, sid="size96", fluid=True)
+    s.defaultParams()
+    s.defineWorldBoundaries([0.96, 0.96, 1.92], dx=0.01)
+    s.initFluid(mu=1e-3, cfd_solver=1)
+    s.setFluidCompressibility(1e-8)
+    s.setPermeabilityPrefactor(1e-12, verbose=False)
+    s.setTolerance(1e-8)
+    s.setMaxIterations(200000)
+    z = (numpy.arange(192) + 0.5) * 0.01
+    s.p_f[:] = 1e3 * numpy.exp(-((z - 0.9) / 0.1) ** 2)[None, None, :]
+    s.p_f[:, :, -1] = 0.0
+    s.initTemporal(total=1.0, file_dt=1.0, dt=1e-4)
+    V = 96 * 96 * 192
+    out = {}
+    pf = []
+    for use in (False, True):
+        e = sphere_amd.Engine(0, nc=32)
+        if use:
+            e.set_darcy_net(params, args.net_corrections)
+        e.load(s.to_state())
+        st0 = e.darcy_stats()
+        e.step(steps)
+        st = e.darcy_stats()
+        pf.append(e.read().get("p_f", V, 1).copy())
+        e.close()
+        out["learned" if use else "plain"] = {
+            "ms_per_solve": (st.solve_ms - st0.solve_ms) / steps,
+            "sweeps_per_solve": (st.sweeps - st0.sweeps) / float(steps),
+            "corrections_per_solve": st.corrections / float(steps), "nonconverged": int(st.nonconverged)}
+    out["final_pressure_rel_diff_vs_plain"] = float(numpy.max(numpy.abs(pf[1] - pf[0])) /
+                                                   max(1e-300, numpy.max(numpy.abs(pf[0]))))
+    out["speedup_ms"] = out["plain"]["ms_per_solve"] / max(1e-9, out["learned"]["ms_per_solve"])
+    out["workload"] = ("pressure pulse in a grain-free 96 x 96 x 192 Darcy box, %d fluid steps, "
+                       "network trained on the 48 x 48 x 96 consolidation boxes" % steps)
+    log("size check 96x96x192: plain %.2f ms, %.0f sweeps; learned %.2f ms, %.0f sweeps; pressure "
+        "difference %.2e" % (out["plain"]["ms_per_solve"], out["plain"]["sweeps_per_solve"],
+                             out["learned"]["ms_per_solve"], out["learned"]["sweeps_per_solve"],
+                             out["final_pressure_rel_diff_vs_plain"]))
+    return out
+
+
 def learned_section(args, s, plain_ms, plain_sweeps):
-    """Train the correction network on the solves of a consolidation box
-    grown from another seed (held out from the timed run), then time the
-    timed box's run with it (same packing, warm-up and step count as the
-    plain run).  The same network is also timed on its own training box
-    (in-sample) for comparison."""
+    """Train the correction network on --net-pairs (input, error) pairs from
+    the solves of --net-boxes consolidation boxes grown from other seeds
+    (held out from the timed run): about 70% from plain sweeps of each
+    solve, the rest, in --net-rounds more rounds, from the iterates the
+    corrected solve meets.  Then time the timed box's run with it (same
+    packing, warm-up and step count as the plain run).  The same network is
+    also timed on its first training box (in sample) for comparison."""
     import torch
     import sphere_amd
     from sphere_amd import learn
     from sphere_amd import workloads as W
     t0 = time.time()
-    trainer = sphere_amd.Engine(0, nc=32)
-    s_train = W.prepare_consolidation(trainer, args.darcy_n, args.darcy_seed + 1, 300, 3, 400)
-    log("Darcy network: training box grown from seed %d in %.1f s" % (args.darcy_seed + 1,
-                                                                       time.time() - t0))
+    seeds = [args.darcy_seed + 1 + k for k in range(args.net_boxes)]
+    boxes = [W.prepare_consolidation(sphere_amd.Engine(0, nc=32), args.darcy_n, sd, 300, 3, 400)
+             for sd in seeds]
+    log("Darcy network: %d training boxes grown from seeds %s in %.1f s" % (len(seeds), seeds,
+                                                                           time.time() - t0))
     t0 = time.time()
-    trainer.load(s_train.to_state())
-    trainer.step(args.darcy_warmup)
-    net = learn.make_net(trainer)
-    X, Y = learn.collect(trainer, args.net_steps)
+    trainers = []
+    for sb in boxes:
+        tr = sphere_amd.Engine(0, nc=32)
+        tr.load(sb.to_state())
+        tr.step(args.darcy_warmup)
+        trainers.append(tr)
+    net = learn.make_net(trainers[0])
+    nx, ny, nz = learn.grid_of(trainers[0])
+    store = learn.PairStore(args.net_pairs, nx * ny * nz)
+    plain_target = int(0.7 * args.net_pairs)
+    per_box = (plain_target + len(trainers) - 1) // len(trainers)
+    for tr in trainers:  # plain iterates: 7 pairs per fluid step
+        target = min(store.n + per_box, plain_target)
+        while store.n < target:
+            before = store.n
+            learn.collect(tr, 1, store=store)
+            if store.n == before:
+                break
+            if store.n > target:
+                store.n = target
+    X, Y = store.view()
     params, _ = learn.train(net, X, Y, iters=args.net_iters, lr=1e-3)
     rounds = 1
-    for _ in range(args.net_rounds):
-        X2, Y2 = learn.collect(trainer, args.net_steps // 2, sweeps=(2, 42, 162), net=net, params=params,
-                               corrections=args.net_corrections)
-        X = torch.cat([X, X2])
-        Y = torch.cat([Y, Y2])
-        params, _ = learn.train(net, X, Y, iters=args.net_iters, lr=5e-4, params=params)
+    for r in range(args.net_rounds):
+        # the iterates the corrected solve meets (8 pairs per fluid step)
+        # and 3 plain ones, round by round over the boxes
+        target = plain_target + (args.net_pairs - plain_target) * (r + 1) // args.net_rounds
+        k = 0
+        while store.n < target:
+            before = store.n
+            learn.collect(trainers[k % len(trainers)], 1, sweeps=(2, 42, 162), net=net, params=params,
+                          corrections=args.net_corrections, store=store)
+            k += 1
+            if store.n == before and k > 4 * len(trainers):
+                break
+            if store.n > target:
+                store.n = target
+        X, Y = store.view()
+        params, _ = learn.train(net, X, Y, iters=args.net_iters, lr=5e-4, params=params, seed=r + 1)
         rounds += 1
-    pairs = int(X.shape[0])
-    del X, Y
+    pairs = int(store.n)
+    del X, Y, store
     torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     train_s = time.time() - t0
-    trainer.close()
+    for tr in trainers:
+        tr.close()
+    s_train = boxes[0]
     log("Darcy network: %d rounds on %d (input, error) pairs in %.1f s" % (rounds, pairs, train_s))
     # inference time of one forward pass (HIP events on torch's stream)
     nx, ny, nz = int(s.num[0]), int(s.num[1]), int(s.num[2])
@@ -263,6 +436,7 @@ def learned_section(args, s, plain_ms, plain_sweeps):
         pf.append(e.read().get("p_f", V, 1).copy())
         e.close()
     p_err = float(numpy.max(numpy.abs(pf[1] - pf[0])) / max(1e-300, numpy.max(numpy.abs(pf[0]))))
+    size = size_check(args, params)
     flops = net_flops(nx, ny, nz)
     log("Darcy with the network: %.3f ms per solve (p50 %.3f), %.1f sweeps, %.1f corrections per "
         "solve; plain %.3f ms, %.1f sweeps; forward %.1f us; pressure difference %.2e" %
@@ -273,9 +447,9 @@ def learned_section(args, s, plain_ms, plain_sweeps):
     log("in sample (the training box): %.3f ms, %.1f sweeps with the network; plain %.3f ms, "
         "%.1f sweeps (%.2fx)" % (il_ms.mean(), il_sw.mean(), ip_ms.mean(), ip_sw.mean(),
                                  ip_ms.mean() / il_ms.mean()))
-    return {
-        "held_out": "trained on a box grown from seed %d, timed on the seed-%d box"
-                    % (args.darcy_seed + 1, args.darcy_seed),
+    return params, {
+        "held_out": "trained on boxes grown from seeds %s, timed on the seed-%d box"
+                    % (seeds, args.darcy_seed),
         "ms": percentiles(ms), "sweeps": percentiles(sweeps),
         "in_sample": {"ms": percentiles(il_ms), "sweeps": percentiles(il_sw),
                       "plain_ms": percentiles(ip_ms), "plain_sweeps": percentiles(ip_sw),
@@ -287,7 +461,9 @@ def learned_section(args, s, plain_ms, plain_sweeps):
         "speedup_ms": float(plain_ms.mean() / ms.mean()),
         "final_pressure_rel_diff_vs_plain": p_err,
         "max_corrections": args.net_corrections,
-        "training": {"pairs": pairs, "rounds": rounds, "iters_per_round": args.net_iters,
+        "size_check_96x96x192": size,
+        "training": {"pairs": pairs, "boxes": len(seeds), "rounds": rounds,
+                     "iters_per_round": args.net_iters,
                      "seconds": train_s, "params": int(sphere_amd.Net.param_count())},
         "net_roofline": {"bound": "mfma", "achieved": flops / fwd_s / 1e12, "peak": 2500.0,
                          "unit": "TFLOP/s", "frac": flops / fwd_s / 1e12 / 2500.0,
@@ -458,15 +634,21 @@ def main():
     ap.add_argument("--darcy-steps", type=int, default=500)
     ap.add_argument("--darcy-warmup", type=int, default=20)
     ap.add_argument("--no-net", action="store_true", help="skip the learned Darcy correction")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip both ensemble sections (DEM boxes; Darcy-coupled boxes)")
+    ap.add_argument("--dens-steps", type=int, default=40, help="timed steps of the Darcy ensemble")
+    ap.add_argument("--dens-warmup", type=int, default=10)
     ap.add_argument("--ensemble", action="store_true",
-                    help="also measure independent boxes stepped together")
+                    help="with --no-shear --no-darcy: only the ensemble of DEM boxes")
     ap.add_argument("--ens-boxes", type=int, default=64)
     ap.add_argument("--ens-seed", type=int, default=5)
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--ens-warmup", type=int, default=20)
-    ap.add_argument("--net-steps", type=int, default=150,
-                    help="fluid steps whose solves make the first training round's pairs")
-    ap.add_argument("--net-iters", type=int, default=3000, help="Adam steps per training round")
+    ap.add_argument("--net-iters", type=int, default=5000, help="Adam steps per training round")
+    ap.add_argument("--net-pairs", type=int, default=20000,
+                    help="(input, error) training pairs in all")
+    ap.add_argument("--net-boxes", type=int, default=3,
+                    help="consolidation boxes (other seeds) the pairs come from")
     ap.add_argument("--net-rounds", type=int, default=2,
                     help="training rounds on iterates of the corrected solve")
     ap.add_argument("--net-corrections", type=int, default=8, help="corrections per solve at most")
@@ -629,7 +811,7 @@ def main():
         run_engine.close()
     engine.close()
     line["darcy"] = None if args.no_darcy else darcy_child()
-    line["ensemble"] = ensemble_section(args) if args.ensemble else None
+    line["ensemble"] = None if args.no_ensemble else ensemble_section(args)
     print(json.dumps(line), flush=True)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

@@ -357,11 +357,14 @@ int sph_darcy_net_input(sph_ctx *ctx, const double *p, float *input, double *sca
  * no adaptive grid, no shear-stress wall) are stepped together: each stage
  * of a step is one kernel launch for all of them, with per-box task
  * records in device memory, and pairs of steps are captured in one HIP
- * graph and replayed.  Other boxes are stepped box by box on their own
- * streams beside them.  Either way each box's results equal those of
- * sph_step on that box alone, bit for bit.  Boxes with fluid, or an output
- * event or the velocity step of t = 10 s / 15 s inside the steps, are
- * refused (SPH_ERR_VALUE).
+ * graph and replayed.  Darcy-coupled boxes on the list path (one DEM step
+ * per fluid step, no modulated top pressure, no learned correction, the
+ * same maxiter) are stepped together too, one launch per stage of the
+ * fluid step, with their Jacobi sweeps batched until every box's
+ * device-side test has ended its solve.  Other boxes are stepped box by box
+ * on their own streams.  Either way each box's results equal those of
+ * sph_step on that box alone, bit for bit.  An output event or the velocity
+ * step of t = 10 s / 15 s inside the steps is refused (SPH_ERR_VALUE).
  * ---------------------------------------------------------------------- */
 typedef struct sph_ensemble sph_ensemble;
 
@@ -379,6 +382,9 @@ typedef struct {
     uint32_t graph;              /* 1: a captured two-step graph is in use     */
     uint64_t multi_steps;        /* steps taken by the multi-box launches      */
     uint64_t graph_replays;      /* replays of the captured graph (2 steps)    */
+    uint32_t fluid_boxes;        /* Darcy-coupled boxes in multi-box launches  */
+    uint32_t pad;
+    uint64_t fluid_steps;        /* steps taken by them                        */
 } sph_ensemble_info_t;
 int sph_ensemble_info(const sph_ensemble *ens, sph_ensemble_info_t *out);
 /* on = 0: enqueue the multi-box launches step by step instead of replaying

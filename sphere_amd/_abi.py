@@ -157,7 +157,9 @@ _lib = None
 class SphEnsembleInfo(ctypes.Structure):
     _fields_ = [("boxes", ctypes.c_uint32), ("multi_boxes", ctypes.c_uint32),
                 ("launches_per_step", ctypes.c_uint32), ("graph", ctypes.c_uint32),
-                ("multi_steps", ctypes.c_uint64), ("graph_replays", ctypes.c_uint64)]
+                ("multi_steps", ctypes.c_uint64), ("graph_replays", ctypes.c_uint64),
+                ("fluid_boxes", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("fluid_steps", ctypes.c_uint64)]
 
 
 def load_library(path=None):

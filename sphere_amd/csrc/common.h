@@ -34,6 +34,32 @@ __host__ __device__ inline uint32_t ct_word(uint32_t b, uint32_t code)
     return b | (code << CT_CODE_SHIFT);
 }
 
+// Contact rows (slot words, tangential springs, slot-to-list maps): slot s
+// of grain a lives at ct_slot(a, s, nc).  With CT_G = 1 (the default) that
+// is the row-major a * nc + s.  CT_G > 1 interleaves the rows of each CT_G
+// consecutive grains slot by slot, so that one slot of those grains is one
+// run of CT_G entries; occupied slots gather at the top of the rows (new
+// contacts take the highest empty slot), so a wave's reads of them fill
+// whole cache lines.  On the 1M-grain shear box CT_G = 64 (the grains of
+// one wave) made the contact kernel slower, 188.7 against 186.7 us.
+#ifndef CT_G
+#define CT_G 1u
+#endif
+__host__ __device__ inline size_t ct_base(uint32_t a, uint32_t nc)
+{
+    return (size_t)(a / CT_G) * nc * CT_G + (a % CT_G);
+}
+__host__ __device__ inline size_t ct_slot(uint32_t a, uint32_t s, uint32_t nc)
+{
+    return ct_base(a, nc) + (size_t)s * CT_G;
+}
+// entries of the contact arrays for np grains (whole groups of CT_G)
+__host__ __device__ inline size_t ct_entries(uint32_t np, uint32_t nc)
+{
+    return (size_t)((np + CT_G - 1) / CT_G) * CT_G * nc;
+}
+#define RS(s) ((size_t)(s) * CT_G)  // offset of slot s from a row's base
+
 struct Grid {
     double origo[3];
     double L[3];

@@ -22,6 +22,18 @@ __device__ __forceinline__ bool cell_xyz(const DarcyArgs &d, int &x, int &y, int
     return true;
 }
 
+// cell of thread threadIdx.x in block bid of a launch over the interior cells
+__device__ __forceinline__ bool cell_xyz_b(const DarcyArgs &d, uint32_t bid, int &x, int &y, int &z)
+{
+    uint32_t c = bid * blockDim.x + threadIdx.x;
+    uint32_t n = d.nx * d.ny * d.nz;
+    if (c >= n) return false;
+    x = (int)(c % d.nx);
+    y = (int)((c / d.nx) % d.ny);
+    z = (int)(c / (d.nx * d.ny));
+    return true;
+}
+
 static inline dim3 cell_grid(const DarcyArgs &d)
 {
     uint32_t n = d.nx * d.ny * d.nz;
@@ -29,10 +41,10 @@ static inline dim3 cell_grid(const DarcyArgs &d)
 }
 
 // setDarcyGhostNodes<T>, src/darcy.cuh:194-264
-__global__ void __launch_bounds__(256) k_ghosts(DarcyArgs d, double *__restrict__ f, int ncomp)
+__device__ __forceinline__ void ghosts_body(DarcyArgs d, double *__restrict__ f, int ncomp, uint32_t bid)
 {
     int x, y, z;
-    if (!cell_xyz(d, x, y, z)) return;
+    if (!cell_xyz_b(d, bid, x, y, z)) return;
     const int nx = (int)d.nx, ny = (int)d.ny, nz = (int)d.nz;
     const double *val = f + (size_t)ncomp * didx(d, x, y, z);
     double v0 = val[0], v1 = 0.0, v2 = 0.0;
@@ -64,17 +76,22 @@ __global__ void __launch_bounds__(256) k_ghosts(DarcyArgs d, double *__restrict_
 #undef GW
 }
 
+__global__ void __launch_bounds__(256) k_ghosts(DarcyArgs d, double *__restrict__ f, int ncomp)
+{
+    ghosts_body(d, f, ncomp, blockIdx.x);
+}
+
 void launch_darcy_ghosts(hipStream_t s, const DarcyArgs &d, double *f, int ncomp)
 {
     hipLaunchKernelGGL(k_ghosts, cell_grid(d), dim3(256), 0, s, d, f, ncomp);
 }
 
 // setDarcyGhostNodesFlux, src/darcy.cuh:270-312
-__global__ void __launch_bounds__(256) k_ghosts_flux(DarcyArgs d, double *__restrict__ f,
-                                                     const double *__restrict__ k)
+__device__ __forceinline__ void ghosts_flux_body(DarcyArgs d, double *__restrict__ f,
+                                                     const double *__restrict__ k, uint32_t bid)
 {
     int x, y, z;
-    if (!cell_xyz(d, x, y, z)) return;
+    if (!cell_xyz_b(d, bid, x, y, z)) return;
     const int nz = (int)d.nz;
     const double pv = f[didx(d, x, y, z)];
     const double kv = k[didx(d, x, y, z)];
@@ -87,6 +104,12 @@ __global__ void __launch_bounds__(256) k_ghosts_flux(DarcyArgs d, double *__rest
     if (z == nz - 1 && d.bc_top == 4) f[didx(d, x, y, nz)] = p_ghost;
 }
 
+__global__ void __launch_bounds__(256) k_ghosts_flux(DarcyArgs d, double *__restrict__ f,
+                                                     const double *__restrict__ k)
+{
+    ghosts_flux_body(d, f, k, blockIdx.x);
+}
+
 void launch_darcy_ghosts_flux(hipStream_t s, const DarcyArgs &d, double *f, const double *k)
 {
     if (!(d.bc_bot == 4 || d.bc_top == 4)) return;
@@ -94,11 +117,11 @@ void launch_darcy_ghosts_flux(hipStream_t s, const DarcyArgs &d, double *f, cons
 }
 
 // findDarcyPressureGradient, src/darcy.cuh:889-943
-__global__ void __launch_bounds__(256) k_pressure_gradient(DarcyArgs d, const double *__restrict__ p,
-                                                           double *__restrict__ grad_p)
+__device__ __forceinline__ void pressure_gradient_body(DarcyArgs d, const double *__restrict__ p,
+                                                           double *__restrict__ grad_p, uint32_t bid)
 {
     int x, y, z;
-    if (!cell_xyz(d, x, y, z)) return;
+    if (!cell_xyz_b(d, bid, x, y, z)) return;
     const double p_xn = p[didx(d, x - 1, y, z)], p_xp = p[didx(d, x + 1, y, z)];
     const double p_yn = p[didx(d, x, y - 1, z)], p_yp = p[didx(d, x, y + 1, z)];
     const double p_zn = p[didx(d, x, y, z - 1)], p_zp = p[didx(d, x, y, z + 1)];
@@ -106,6 +129,12 @@ __global__ void __launch_bounds__(256) k_pressure_gradient(DarcyArgs d, const do
     g[0] = (p_xp - p_xn) / (d.dx + d.dx);
     g[1] = (p_yp - p_yn) / (d.dy + d.dy);
     g[2] = (p_zp - p_zn) / (d.dz + d.dz);
+}
+
+__global__ void __launch_bounds__(256) k_pressure_gradient(DarcyArgs d, const double *__restrict__ p,
+                                                           double *__restrict__ grad_p)
+{
+    pressure_gradient_body(d, p, grad_p, blockIdx.x);
 }
 
 void launch_darcy_pressure_gradient(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b)
@@ -125,14 +154,14 @@ __device__ __forceinline__ double weight(d3 x_p, d3 x_f, double dx, double dy, d
 __device__ __forceinline__ double sphere_volume(double r) { return 4.0 / 3.0 * PI_M * (r * r * r); }
 
 // findDarcyPressureForceLinear, src/darcy.cuh:948-1102
-__global__ void __launch_bounds__(256) k_pressure_force(StepArgs a, DarcyArgs d,
+__device__ __forceinline__ void pressure_force_body(StepArgs a, DarcyArgs d,
                                                         const double4 *__restrict__ xarr,
                                                         const double *__restrict__ grad_parr,
                                                         double3 *__restrict__ force,
                                                         double4 *__restrict__ f_parr,
-                                                        uint32_t *__restrict__ err)
+                                                        uint32_t *__restrict__ err, uint32_t bid)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = bid * blockDim.x + threadIdx.x;
     if (i >= a.np) return;
     const Grid &g = a.grid;
     const double4 x4 = xarr[i];
@@ -179,6 +208,16 @@ __global__ void __launch_bounds__(256) k_pressure_force(StepArgs a, DarcyArgs d,
     f_parr[i] = make_double4(f_p.x, f_p.y, f_p.z, 0.0);
 }
 
+__global__ void __launch_bounds__(256) k_pressure_force(StepArgs a, DarcyArgs d,
+                                                        const double4 *__restrict__ xarr,
+                                                        const double *__restrict__ grad_parr,
+                                                        double3 *__restrict__ force,
+                                                        double4 *__restrict__ f_parr,
+                                                        uint32_t *__restrict__ err)
+{
+    pressure_force_body(a, d, xarr, grad_parr, force, f_parr, err, blockIdx.x);
+}
+
 void launch_darcy_pressure_force(hipStream_t s, const StepArgs &a, const DarcyArgs &d,
                                  const DarcyBuffers &b, const DemBuffers &db)
 {
@@ -211,17 +250,17 @@ __device__ __forceinline__ int dist_mod_p(const Grid &g, int &tx, int &ty, int t
 }
 
 // findDarcyPorositiesLinear, src/darcy.cuh:359-550
-__global__ void __launch_bounds__(256) k_porosities(StepArgs a, DarcyArgs d, long iteration,
+__device__ __forceinline__ void porosities_body(StepArgs a, DarcyArgs d, long iteration,
                                                     const uint32_t *__restrict__ cell_start,
                                                     const uint32_t *__restrict__ cell_end,
                                                     const double4 *__restrict__ x_s,
                                                     const double4 *__restrict__ vel_s,
                                                     double *__restrict__ phi_a,
                                                     double *__restrict__ dphi_a,
-                                                    double *__restrict__ vp_avg_a)
+                                                    double *__restrict__ vp_avg_a, uint32_t bid)
 {
     int x, y, z;
-    if (!cell_xyz(d, x, y, z)) return;
+    if (!cell_xyz_b(d, bid, x, y, z)) return;
     const uint32_t cellidx = didx(d, x, y, z);
     if (a.np == 0) {
         phi_a[cellidx] = 0.9;
@@ -298,6 +337,18 @@ __global__ void __launch_bounds__(256) k_porosities(StepArgs a, DarcyArgs d, lon
     vp_avg_a[3 * (size_t)cellidx + 2] = vp_avg.z;
 }
 
+__global__ void __launch_bounds__(256) k_porosities(StepArgs a, DarcyArgs d, long iteration,
+                                                    const uint32_t *__restrict__ cell_start,
+                                                    const uint32_t *__restrict__ cell_end,
+                                                    const double4 *__restrict__ x_s,
+                                                    const double4 *__restrict__ vel_s,
+                                                    double *__restrict__ phi_a,
+                                                    double *__restrict__ dphi_a,
+                                                    double *__restrict__ vp_avg_a)
+{
+    porosities_body(a, d, iteration, cell_start, cell_end, x_s, vel_s, phi_a, dphi_a, vp_avg_a, blockIdx.x);
+}
+
 void launch_darcy_porosities(hipStream_t s, const StepArgs &a, const DarcyArgs &d,
                              const DarcyBuffers &b, const DemBuffers &db, long iter)
 {
@@ -306,12 +357,12 @@ void launch_darcy_porosities(hipStream_t s, const StepArgs &a, const DarcyArgs &
 }
 
 // copyDarcyPorositiesToEdges, src/darcy.cuh:556-590
-__global__ void __launch_bounds__(256) k_por_edges(DarcyArgs d, double *__restrict__ phi,
+__device__ __forceinline__ void por_edges_body(DarcyArgs d, double *__restrict__ phi,
                                                    double *__restrict__ dphi,
-                                                   double *__restrict__ vp)
+                                                   double *__restrict__ vp, uint32_t bid)
 {
     int x, y, z;
-    if (!cell_xyz(d, x, y, z)) return;
+    if (!cell_xyz_b(d, bid, x, y, z)) return;
     const int ny = (int)d.ny;
     if (!(y == 0 || y == ny - 1)) return;
     const int y_read = y == 0 ? 1 : ny - 2;
@@ -323,13 +374,20 @@ __global__ void __launch_bounds__(256) k_por_edges(DarcyArgs d, double *__restri
     vp[3 * (size_t)w + 2] = vp[3 * (size_t)r + 2];
 }
 
+__global__ void __launch_bounds__(256) k_por_edges(DarcyArgs d, double *__restrict__ phi,
+                                                   double *__restrict__ dphi,
+                                                   double *__restrict__ vp)
+{
+    por_edges_body(d, phi, dphi, vp, blockIdx.x);
+}
+
 // copyDarcyPorositiesToBottom, src/darcy.cuh:595-628
-__global__ void __launch_bounds__(256) k_por_bottom(DarcyArgs d, double *__restrict__ phi,
+__device__ __forceinline__ void por_bottom_body(DarcyArgs d, double *__restrict__ phi,
                                                     double *__restrict__ dphi,
-                                                    double *__restrict__ vp)
+                                                    double *__restrict__ vp, uint32_t bid)
 {
     int x, y, z;
-    if (!cell_xyz(d, x, y, z)) return;
+    if (!cell_xyz_b(d, bid, x, y, z)) return;
     if (z != 0) return;
     const uint32_t r = didx(d, x, y, 1), w = didx(d, x, y, 0);
     phi[w] = phi[r];
@@ -337,6 +395,13 @@ __global__ void __launch_bounds__(256) k_por_bottom(DarcyArgs d, double *__restr
     vp[3 * (size_t)w] = vp[3 * (size_t)r];
     vp[3 * (size_t)w + 1] = vp[3 * (size_t)r + 1];
     vp[3 * (size_t)w + 2] = vp[3 * (size_t)r + 2];
+}
+
+__global__ void __launch_bounds__(256) k_por_bottom(DarcyArgs d, double *__restrict__ phi,
+                                                    double *__restrict__ dphi,
+                                                    double *__restrict__ vp)
+{
+    por_bottom_body(d, phi, dphi, vp, blockIdx.x);
 }
 
 void launch_darcy_porosities_to_edges(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b)
@@ -362,11 +427,11 @@ void launch_darcy_top_pressure(hipStream_t s, const DarcyArgs &d, const DarcyBuf
 }
 
 // findDarcyPermeabilities, src/darcy.cuh:1266-1313 (phi^3, (1-phi)^2 as products)
-__global__ void __launch_bounds__(256) k_permeabilities(DarcyArgs d, const double *__restrict__ phi_a,
-                                                        double *__restrict__ k_a)
+__device__ __forceinline__ void permeabilities_body(DarcyArgs d, const double *__restrict__ phi_a,
+                                                        double *__restrict__ k_a, uint32_t bid)
 {
     int x, y, z;
-    if (!cell_xyz(d, x, y, z)) return;
+    if (!cell_xyz_b(d, bid, x, y, z)) return;
     const uint32_t c = didx(d, x, y, z);
     double phi = phi_a[c];
     if (phi > 0.9999) phi = 0.9999;
@@ -375,19 +440,31 @@ __global__ void __launch_bounds__(256) k_permeabilities(DarcyArgs d, const doubl
     k_a[c] = k;
 }
 
+__global__ void __launch_bounds__(256) k_permeabilities(DarcyArgs d, const double *__restrict__ phi_a,
+                                                        double *__restrict__ k_a)
+{
+    permeabilities_body(d, phi_a, k_a, blockIdx.x);
+}
+
 void launch_darcy_permeabilities(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b)
 {
     hipLaunchKernelGGL(k_permeabilities, cell_grid(d), dim3(256), 0, s, d, b.phi, b.k);
 }
 
 // copyValues, src/navierstokes.cuh:2794-2827
+__device__ __forceinline__ void copy_interior_body(DarcyArgs d, const double *__restrict__ src,
+                                                       double *__restrict__ dst, uint32_t bid)
+{
+    int x, y, z;
+    if (!cell_xyz_b(d, bid, x, y, z)) return;
+    const uint32_t c = didx(d, x, y, z);
+    dst[c] = src[c];
+}
+
 __global__ void __launch_bounds__(256) k_copy_interior(DarcyArgs d, const double *__restrict__ src,
                                                        double *__restrict__ dst)
 {
-    int x, y, z;
-    if (!cell_xyz(d, x, y, z)) return;
-    const uint32_t c = didx(d, x, y, z);
-    dst[c] = src[c];
+    copy_interior_body(d, src, dst, blockIdx.x);
 }
 
 void launch_darcy_copy_interior(hipStream_t s, const DarcyArgs &d, const double *src, double *dst)
@@ -396,11 +473,16 @@ void launch_darcy_copy_interior(hipStream_t s, const DarcyArgs &d, const double 
 }
 
 // setDarcyZeros, src/darcy.cuh:172-187
-__global__ void __launch_bounds__(256) k_zero_interior(DarcyArgs d, double *__restrict__ f)
+__device__ __forceinline__ void zero_interior_body(DarcyArgs d, double *__restrict__ f, uint32_t bid)
 {
     int x, y, z;
-    if (!cell_xyz(d, x, y, z)) return;
+    if (!cell_xyz_b(d, bid, x, y, z)) return;
     f[didx(d, x, y, z)] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_zero_interior(DarcyArgs d, double *__restrict__ f)
+{
+    zero_interior_body(d, f, blockIdx.x);
 }
 
 void launch_darcy_zero_interior(hipStream_t s, const DarcyArgs &d, double *f)
@@ -494,7 +576,7 @@ __device__ __forceinline__ JLayout jlayout(const DarcyArgs &d)
 //   base = p_old + (1 - epsilon) dp_expl,
 // the sweep's dp_impl = A div - F and p* = base + epsilon dp_impl are the
 // reference's expressions, operation for operation.
-__global__ void __launch_bounds__(256) k_darcy_prepare(DarcyArgs d, const double *__restrict__ p_a,
+__device__ __forceinline__ void darcy_prepare_body(DarcyArgs d, const double *__restrict__ p_a,
                                                        const double *__restrict__ k_a,
                                                        const double *__restrict__ phi_a,
                                                        const double *__restrict__ dphi_a,
@@ -503,11 +585,11 @@ __global__ void __launch_bounds__(256) k_darcy_prepare(DarcyArgs d, const double
                                                        const double *__restrict__ p_old,
                                                        double *__restrict__ dp_expl,
                                                        double *__restrict__ jc,
-                                                       uint8_t *__restrict__ frozen)
+                                                       uint8_t *__restrict__ frozen, uint32_t bid)
 {
     int x, y, z;
-    if (!cell_xyz(d, x, y, z)) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!cell_xyz_b(d, bid, x, y, z)) return;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     const int nx = (int)d.nx, ny = (int)d.ny, nz = (int)d.nz;
     const double dx = d.dx, dy = d.dy, dz = d.dz;
     const JLayout j = jlayout(d);
@@ -540,6 +622,20 @@ __global__ void __launch_bounds__(256) k_darcy_prepare(DarcyArgs d, const double
                  ((uint32_t)z >= d.wall0_iz && d.bc_top == 0) || (d.bc_xn == 0 && x == 0) ||
                  (d.bc_xp == 0 && x == nx - 1) || (d.bc_yn == 0 && y == 0) ||
                  (d.bc_yp == 0 && y == nx - 1) || pconst[c] == 1) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256) k_darcy_prepare(DarcyArgs d, const double *__restrict__ p_a,
+                                                       const double *__restrict__ k_a,
+                                                       const double *__restrict__ phi_a,
+                                                       const double *__restrict__ dphi_a,
+                                                       const double *__restrict__ vp_a,
+                                                       const int *__restrict__ pconst,
+                                                       const double *__restrict__ p_old,
+                                                       double *__restrict__ dp_expl,
+                                                       double *__restrict__ jc,
+                                                       uint8_t *__restrict__ frozen)
+{
+    darcy_prepare_body(d, p_a, k_a, phi_a, dphi_a, vp_a, pconst, p_old, dp_expl, jc, frozen, blockIdx.x);
 }
 
 void launch_darcy_prepare(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b)
@@ -625,7 +721,7 @@ __device__ __forceinline__ double jacobi_target(const DarcyArgs &d, const double
 // once state[0] is set, every later sweep returns at once, so the host may
 // enqueue sweeps beyond the convergence point.
 #define SWEEP_THREADS 512
-__global__ void __launch_bounds__(SWEEP_THREADS) k_darcy_sweep(DarcyArgs d, const double *__restrict__ jc,
+__device__ __forceinline__ void darcy_sweep_body(DarcyArgs d, const double *__restrict__ jc,
                                                      const uint8_t *__restrict__ frozen,
                                                      const double *__restrict__ p_in,
                                                      const double *__restrict__ k_a,
@@ -635,10 +731,10 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_darcy_sweep(DarcyArgs d, cons
                                                      const unsigned long long *__restrict__ prev,
                                                      double tolerance, uint32_t prev_nijac,
                                                      unsigned long long *__restrict__ state,
-                                                     unsigned long long *__restrict__ zero_scale)
+                                                     unsigned long long *__restrict__ zero_scale, uint32_t bid)
 {
     __shared__ unsigned long long wmax[SWEEP_THREADS / 64];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     if (zero_scale && i == 0) *zero_scale = 0ull;  // the scale slot of the correction that follows
     if (state[0] != 0ull) return;
     if (prev && darcy_decide(prev, tolerance, prev_nijac, state, i == 0)) return;
@@ -698,6 +794,21 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_darcy_sweep(DarcyArgs d, cons
     }
 }
 
+__global__ void __launch_bounds__(SWEEP_THREADS) k_darcy_sweep(DarcyArgs d, const double *__restrict__ jc,
+                                                     const uint8_t *__restrict__ frozen,
+                                                     const double *__restrict__ p_in,
+                                                     const double *__restrict__ k_a,
+                                                     double *__restrict__ p_out,
+                                                     unsigned long long *__restrict__ check,
+                                                     unsigned long long *__restrict__ clear,
+                                                     const unsigned long long *__restrict__ prev,
+                                                     double tolerance, uint32_t prev_nijac,
+                                                     unsigned long long *__restrict__ state,
+                                                     unsigned long long *__restrict__ zero_scale)
+{
+    darcy_sweep_body(d, jc, frozen, p_in, k_a, p_out, check, clear, prev, tolerance, prev_nijac, state, zero_scale, blockIdx.x);
+}
+
 void launch_darcy_sweep(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b,
                         const double *p_in, double *p_out, unsigned long long *check,
                         unsigned long long *clear, const unsigned long long *prev,
@@ -738,16 +849,16 @@ void launch_darcy_decide(hipStream_t s, const DarcyBuffers &b, const unsigned lo
 // and state[0] != 0 (the solve has ended), so corrections can be enqueued
 // ahead of the convergence decision.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_darcy_residual_max(DarcyArgs d, const double *__restrict__ jc,
+__device__ __forceinline__ void darcy_residual_max_body(DarcyArgs d, const double *__restrict__ jc,
                                                             const uint8_t *__restrict__ frozen,
                                                             const double *__restrict__ p,
                                                             const unsigned long long *__restrict__ state,
-                                                            unsigned long long *__restrict__ smax)
+                                                            unsigned long long *__restrict__ smax, uint32_t bid)
 {
     __shared__ unsigned long long wmax[4];
     if (state && state[0] != 0ull) return;
     const uint32_t n = d.nx * d.ny * d.nz;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     unsigned long long bits = 0;
     if (i < n) {
         const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / (d.nx * d.ny));
@@ -768,6 +879,15 @@ __global__ void __launch_bounds__(256) k_darcy_residual_max(DarcyArgs d, const d
     }
 }
 
+__global__ void __launch_bounds__(256) k_darcy_residual_max(DarcyArgs d, const double *__restrict__ jc,
+                                                            const uint8_t *__restrict__ frozen,
+                                                            const double *__restrict__ p,
+                                                            const unsigned long long *__restrict__ state,
+                                                            unsigned long long *__restrict__ smax)
+{
+    darcy_residual_max_body(d, jc, frozen, p, state, smax, blockIdx.x);
+}
+
 __device__ __forceinline__ bool usable_scale(unsigned long long bits, double &s)
 {
     s = __longlong_as_double((long long)bits);
@@ -775,17 +895,17 @@ __device__ __forceinline__ bool usable_scale(unsigned long long bits, double &s)
 }
 
 // network input [V][3] (x fastest) for iterate p
-__global__ void __launch_bounds__(256) k_darcy_net_input(DarcyArgs d, const double *__restrict__ jc,
+__device__ __forceinline__ void darcy_net_input_body(DarcyArgs d, const double *__restrict__ jc,
                                                          const uint8_t *__restrict__ frozen,
                                                          const double *__restrict__ p,
                                                          const unsigned long long *__restrict__ state,
                                                          const unsigned long long *__restrict__ smax,
                                                          float *__restrict__ input,
-                                                         uint16_t *__restrict__ in4)
+                                                         uint16_t *__restrict__ in4, uint32_t bid)
 {
     if (state && state[0] != 0ull) return;
     const uint32_t n = d.nx * d.ny * d.nz;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / (d.nx * d.ny));
     const uint32_t c = didx(d, x, y, z);
@@ -815,19 +935,30 @@ __global__ void __launch_bounds__(256) k_darcy_net_input(DarcyArgs d, const doub
     }
 }
 
+__global__ void __launch_bounds__(256) k_darcy_net_input(DarcyArgs d, const double *__restrict__ jc,
+                                                         const uint8_t *__restrict__ frozen,
+                                                         const double *__restrict__ p,
+                                                         const unsigned long long *__restrict__ state,
+                                                         const unsigned long long *__restrict__ smax,
+                                                         float *__restrict__ input,
+                                                         uint16_t *__restrict__ in4)
+{
+    darcy_net_input_body(d, jc, frozen, p, state, smax, input, in4, blockIdx.x);
+}
+
 // p += s e on the cells that are not held fixed; counts applied corrections
 // in state[3]
-__global__ void __launch_bounds__(256) k_darcy_net_apply(DarcyArgs d, const uint8_t *__restrict__ frozen,
+__device__ __forceinline__ void darcy_net_apply_body(DarcyArgs d, const uint8_t *__restrict__ frozen,
                                                          double *__restrict__ p,
                                                          const float *__restrict__ e,
                                                          const unsigned long long *__restrict__ smax,
-                                                         unsigned long long *__restrict__ state)
+                                                         unsigned long long *__restrict__ state, uint32_t bid)
 {
     if (state[0] != 0ull) return;
     double s;
     if (!usable_scale(*smax, s)) return;
     const uint32_t n = d.nx * d.ny * d.nz;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     if (i == 0) state[3] += 1ull;
     if (i >= n || frozen[i]) return;
     const float ev = e[i];
@@ -861,6 +992,15 @@ __global__ void __launch_bounds__(256) k_darcy_net_apply(DarcyArgs d, const uint
 #undef GW
 }
 
+__global__ void __launch_bounds__(256) k_darcy_net_apply(DarcyArgs d, const uint8_t *__restrict__ frozen,
+                                                         double *__restrict__ p,
+                                                         const float *__restrict__ e,
+                                                         const unsigned long long *__restrict__ smax,
+                                                         unsigned long long *__restrict__ state)
+{
+    darcy_net_apply_body(d, frozen, p, e, smax, state, blockIdx.x);
+}
+
 void launch_darcy_residual_max(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b, const double *p,
                                bool free_running)
 {
@@ -885,13 +1025,13 @@ void launch_darcy_net_apply(hipStream_t s, const DarcyArgs &d, const DarcyBuffer
 }
 
 // findDarcyVelocities, src/darcy.cuh:1839-1897
-__global__ void __launch_bounds__(256) k_darcy_velocities(DarcyArgs d, const double *__restrict__ p,
+__device__ __forceinline__ void darcy_velocities_body(DarcyArgs d, const double *__restrict__ p,
                                                           const double *__restrict__ phi_a,
                                                           const double *__restrict__ k_a,
-                                                          double *__restrict__ v_a)
+                                                          double *__restrict__ v_a, uint32_t bid)
 {
     int x, y, z;
-    if (!cell_xyz(d, x, y, z)) return;
+    if (!cell_xyz_b(d, bid, x, y, z)) return;
     const uint32_t c = didx(d, x, y, z);
     const double p_xn = p[didx(d, x - 1, y, z)], p_xp = p[didx(d, x + 1, y, z)];
     const double p_yn = p[didx(d, x, y - 1, z)], p_yp = p[didx(d, x, y + 1, z)];
@@ -905,7 +1045,275 @@ __global__ void __launch_bounds__(256) k_darcy_velocities(DarcyArgs d, const dou
     v_a[3 * (size_t)c + 2] = v.z;
 }
 
+__global__ void __launch_bounds__(256) k_darcy_velocities(DarcyArgs d, const double *__restrict__ p,
+                                                          const double *__restrict__ phi_a,
+                                                          const double *__restrict__ k_a,
+                                                          double *__restrict__ v_a)
+{
+    darcy_velocities_body(d, p, phi_a, k_a, v_a, blockIdx.x);
+}
+
 void launch_darcy_velocities(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b)
 {
     hipLaunchKernelGGL(k_darcy_velocities, cell_grid(d), dim3(256), 0, s, d, b.p, b.phi, b.k, b.v);
+}
+
+// ---------------------------------------------------------------------------
+// Multi-box launches (ensembles of fluid boxes): every stage of a fluid step
+// is one launch for all boxes; a block finds its box (multi_block) and runs
+// the body of the single-box kernel on that box's task record.
+// ---------------------------------------------------------------------------
+uint32_t darcy_cell_blocks(const DarcyArgs &d) { return (d.nx * d.ny * d.nz + 255u) / 256u; }
+
+uint32_t darcy_sweep_blocks(const DarcyArgs &d)
+{
+    return (d.nx * d.ny * d.nz + SWEEP_THREADS - 1u) / SWEEP_THREADS;
+}
+
+uint32_t darcy_all_cell_blocks(const DarcyArgs &d)
+{
+    return ((d.nx + 2u) * (d.ny + 2u) * (d.nz + 2u) + 255u) / 256u;
+}
+
+#define DM_BOX                                   \
+    uint32_t k_, bid, nblk_;                     \
+    if (!multi_block(m, k_, bid, nblk_)) return; \
+    const DarcyTask &T = t[k_]
+
+__global__ void __launch_bounds__(256) k_dm_ghosts(const DarcyTask *__restrict__ t, MultiMap m, int field)
+{
+    DM_BOX;
+    double *f = field == DM_P ? T.b.p : field == DM_GRAD_P ? T.b.grad_p : field == DM_PHI ? T.b.phi : T.b.k;
+    ghosts_body(T.d, f, field == DM_GRAD_P ? 3 : 1, bid);
+}
+
+__global__ void __launch_bounds__(256) k_dm_ghosts_flux(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    ghosts_flux_body(T.d, T.b.p, T.b.k, bid);
+}
+
+__global__ void __launch_bounds__(256) k_dm_pressure_gradient(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    pressure_gradient_body(T.d, T.b.p, T.b.grad_p, bid);
+}
+
+__global__ void __launch_bounds__(256) k_dm_pressure_force(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    pressure_force_body(T.a, T.d, T.x, T.b.grad_p, T.force, T.b.f_p, T.err, bid);
+}
+
+// the iteration count of the step from the box's device clock (the host's
+// value in a single-box step)
+__global__ void __launch_bounds__(256) k_dm_porosities(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    porosities_body(T.a, T.d, (long)T.clock->iter, T.cell_start, T.cell_end, T.x_s, T.vel_s, T.b.phi,
+                    T.b.dphi, T.b.vp_avg, bid);
+}
+
+__global__ void __launch_bounds__(256) k_dm_por_edges(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    por_edges_body(T.d, T.b.phi, T.b.dphi, T.b.vp_avg, bid);
+}
+
+__global__ void __launch_bounds__(256) k_dm_por_bottom(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    por_bottom_body(T.d, T.b.phi, T.b.dphi, T.b.vp_avg, bid);
+}
+
+__global__ void __launch_bounds__(256) k_dm_permeabilities(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    permeabilities_body(T.d, T.b.phi, T.b.k, bid);
+}
+
+__global__ void __launch_bounds__(256) k_dm_solve_start(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    copy_interior_body(T.d, T.b.p, T.b.p_old, bid);
+}
+
+__global__ void __launch_bounds__(256) k_dm_prepare(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    darcy_prepare_body(T.d, T.b.p, T.b.k, T.b.phi, T.b.dphi, T.b.vp_avg, T.b.p_constant, T.b.p_old,
+                       T.b.dp_expl, T.b.jc, T.b.jfrozen, bid);
+}
+
+__global__ void k_dm_reset(const DarcyTask *__restrict__ t)
+{
+    if (threadIdx.x < DARCY_MAXRES_WORDS) t[blockIdx.x].b.maxres[threadIdx.x] = 0ull;
+}
+
+// The iterate buffers and the check slots of sweep nijac, as the host loop
+// of a single-box solve passes them (driver.hip, darcy_step).
+__global__ void __launch_bounds__(SWEEP_THREADS) k_dm_sweep(const DarcyTask *__restrict__ t, MultiMap m,
+                                                            uint32_t nijac, int correct)
+{
+    DM_BOX;
+    const bool even = (nijac & 1u) == 0u;
+    unsigned long long *mr = T.b.maxres;
+    const bool check = nijac % 10u == 0u;
+    const bool decides = nijac > 0u && (nijac - 1u) % 10u == 0u;
+    darcy_sweep_body(T.d, T.b.jc, T.b.jfrozen, even ? T.b.p : T.b.p_new, T.b.k, even ? T.b.p_new : T.b.p,
+                     check ? mr + ((nijac / 10u) & 1u) : nullptr,
+                     check ? mr + (((nijac / 10u) & 1u) ^ 1u) : nullptr,
+                     decides ? mr + (((nijac - 1u) / 10u) & 1u) : nullptr, T.tolerance, nijac - 1u,
+                     mr + DARCY_STATE, (correct && T.net_in4) ? mr + DARCY_SCALE : nullptr, bid);
+}
+
+__global__ void k_dm_decide(const DarcyTask *__restrict__ t, uint32_t n, uint32_t nijac)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const DarcyTask &T = t[k];
+    unsigned long long *state = T.b.maxres + DARCY_STATE;
+    if (state[0] != 0ull) return;
+    (void)darcy_decide(T.b.maxres + ((nijac / 10u) & 1u), T.tolerance, nijac, state, true);
+}
+
+// the iterate that deciding sweep nijac wrote
+__device__ __forceinline__ double *dm_iterate(const DarcyTask &T, uint32_t nijac)
+{
+    return (nijac & 1u) ? T.b.p : T.b.p_new;
+}
+
+__global__ void __launch_bounds__(256) k_dm_residual_max(const DarcyTask *__restrict__ t, MultiMap m,
+                                                         uint32_t nijac)
+{
+    DM_BOX;
+    darcy_residual_max_body(T.d, T.b.jc, T.b.jfrozen, dm_iterate(T, nijac), T.b.maxres + DARCY_STATE,
+                            T.b.maxres + DARCY_SCALE, bid);
+}
+
+__global__ void __launch_bounds__(256) k_dm_net_input(const DarcyTask *__restrict__ t, MultiMap m,
+                                                      uint32_t nijac)
+{
+    DM_BOX;
+    darcy_net_input_body(T.d, T.b.jc, T.b.jfrozen, dm_iterate(T, nijac), T.b.maxres + DARCY_STATE,
+                         T.b.maxres + DARCY_SCALE, nullptr, T.net_in4, bid);
+}
+
+__global__ void __launch_bounds__(256) k_dm_net_apply(const DarcyTask *__restrict__ t, MultiMap m,
+                                                      uint32_t nijac)
+{
+    DM_BOX;
+    darcy_net_apply_body(T.d, T.b.jfrozen, dm_iterate(T, nijac), T.net_out, T.b.maxres + DARCY_SCALE,
+                         T.b.maxres + DARCY_STATE, bid);
+}
+
+__global__ void k_dm_states(const DarcyTask *__restrict__ t, uint32_t n, unsigned long long *__restrict__ out)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    for (int j = 0; j < 4; ++j) out[4 * k + j] = t[k].b.maxres[DARCY_STATE + j];
+}
+
+// The sweeps that ran (up to the deciding check, or all maxiter) leave the
+// solution in b.p_new when their number is odd; it is copied into b.p (every
+// cell, ghosts included), so that every solve starts from b.p.
+__global__ void __launch_bounds__(256) k_dm_normalize(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    const unsigned long long *state = T.b.maxres + DARCY_STATE;
+    const unsigned long long ran = state[0] != 0ull ? state[1] + 1ull : (unsigned long long)T.maxiter;
+    if ((ran & 1ull) == 0ull) return;
+    const uint32_t c = bid * blockDim.x + threadIdx.x;
+    if (c < (T.d.nx + 2u) * (T.d.ny + 2u) * (T.d.nz + 2u)) T.b.p[c] = T.b.p_new[c];
+}
+
+__global__ void __launch_bounds__(256) k_dm_finish(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    zero_interior_body(T.d, T.b.dphi, bid);
+}
+
+__global__ void __launch_bounds__(256) k_dm_velocities(const DarcyTask *__restrict__ t, MultiMap m)
+{
+    DM_BOX;
+    darcy_velocities_body(T.d, T.b.p, T.b.phi, T.b.k, T.b.v, bid);
+}
+
+#define DM_LAUNCH(K, BS, ...) \
+    if (blocks) hipLaunchKernelGGL(K, dim3(blocks), dim3(BS), 0, s, t, m, ##__VA_ARGS__)
+
+void launch_dm_ghosts(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks, int field)
+{
+    DM_LAUNCH(k_dm_ghosts, 256, field);
+}
+void launch_dm_ghosts_flux(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks)
+{
+    DM_LAUNCH(k_dm_ghosts_flux, 256);
+}
+void launch_dm_pressure_gradient(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks)
+{
+    DM_LAUNCH(k_dm_pressure_gradient, 256);
+}
+void launch_dm_pressure_force(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks)
+{
+    DM_LAUNCH(k_dm_pressure_force, 256);
+}
+void launch_dm_porosities(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks)
+{
+    DM_LAUNCH(k_dm_porosities, 256);
+}
+void launch_dm_por_edges(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks)
+{
+    DM_LAUNCH(k_dm_por_edges, 256);
+    DM_LAUNCH(k_dm_por_bottom, 256);
+}
+void launch_dm_permeabilities(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks)
+{
+    DM_LAUNCH(k_dm_permeabilities, 256);
+}
+void launch_dm_solve_start(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks)
+{
+    DM_LAUNCH(k_dm_solve_start, 256);
+}
+void launch_dm_prepare(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks)
+{
+    DM_LAUNCH(k_dm_prepare, 256);
+}
+void launch_dm_reset(hipStream_t s, const DarcyTask *t, uint32_t n)
+{
+    if (n) hipLaunchKernelGGL(k_dm_reset, dim3(n), dim3(64), 0, s, t);
+}
+void launch_dm_sweep(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks, uint32_t nijac,
+                     bool correct)
+{
+    DM_LAUNCH(k_dm_sweep, SWEEP_THREADS, nijac, correct ? 1 : 0);
+}
+void launch_dm_decide(hipStream_t s, const DarcyTask *t, uint32_t n, uint32_t nijac)
+{
+    if (n) hipLaunchKernelGGL(k_dm_decide, dim3((n + 63u) / 64u), dim3(64), 0, s, t, n, nijac);
+}
+void launch_dm_net_input(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks, uint32_t nijac)
+{
+    DM_LAUNCH(k_dm_residual_max, 256, nijac);
+    DM_LAUNCH(k_dm_net_input, 256, nijac);
+}
+void launch_dm_net_apply(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks, uint32_t nijac)
+{
+    DM_LAUNCH(k_dm_net_apply, 256, nijac);
+}
+void launch_dm_states(hipStream_t s, const DarcyTask *t, uint32_t n, unsigned long long *out)
+{
+    if (n) hipLaunchKernelGGL(k_dm_states, dim3((n + 63u) / 64u), dim3(64), 0, s, t, n, out);
+}
+void launch_dm_normalize(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks)
+{
+    DM_LAUNCH(k_dm_normalize, 256);
+}
+void launch_dm_finish(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks)
+{
+    DM_LAUNCH(k_dm_finish, 256);
+}
+void launch_dm_velocities(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks)
+{
+    DM_LAUNCH(k_dm_velocities, 256);
 }

@@ -119,7 +119,7 @@ __device__ __forceinline__ double dmax_since_rebuild(const uint32_t *flags)
 // particle has moved more than sqrt(disp2) since the last rebuild, when the
 // host forces a rebuild, or when the sticky list-overflow bit flags[2] is
 // set; flags[cur ^ 1] (read by the previous step's kernels) is cleared here.
-__global__ void __launch_bounds__(256) k_hash_nl(StepArgs a, NLArgs n,
+__device__ __forceinline__ void hash_nl_body(StepArgs a, NLArgs n,
                                                  const double4 *__restrict__ x,
                                                  uint32_t *__restrict__ keys,
                                                  uint32_t *__restrict__ vals,
@@ -127,9 +127,9 @@ __global__ void __launch_bounds__(256) k_hash_nl(StepArgs a, NLArgs n,
                                                  const double4 *__restrict__ x_ref,
                                                  float *__restrict__ disp,
                                                  uint32_t *__restrict__ flags, uint32_t cur,
-                                                 uint32_t force, uint32_t *__restrict__ err)
+                                                 uint32_t force, uint32_t *__restrict__ err, uint32_t bid)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     if (i == 0) flags[cur ^ 1u] = 0u;
     if (i >= a.np) return;
     const Grid &g = a.grid;
@@ -154,6 +154,19 @@ __global__ void __launch_bounds__(256) k_hash_nl(StepArgs a, NLArgs n,
     bool need = !(d2 <= n.disp2);
     if (i == 0 && (force || flags[NLF_STICKY] != 0u)) need = true;
     record_displacement(flags, cur, need, false, d2);
+}
+
+__global__ void __launch_bounds__(256) k_hash_nl(StepArgs a, NLArgs n,
+                                                 const double4 *__restrict__ x,
+                                                 uint32_t *__restrict__ keys,
+                                                 uint32_t *__restrict__ vals,
+                                                 uint32_t *__restrict__ cellc,
+                                                 const double4 *__restrict__ x_ref,
+                                                 float *__restrict__ disp,
+                                                 uint32_t *__restrict__ flags, uint32_t cur,
+                                                 uint32_t force, uint32_t *__restrict__ err)
+{
+    hash_nl_body(a, n, x, keys, vals, cellc, x_ref, disp, flags, cur, force, err, blockIdx.x);
 }
 
 void launch_hash_nl(hipStream_t s, const StepArgs &a, const NLArgs &n, const DemBuffers &b,
@@ -463,7 +476,7 @@ __device__ __forceinline__ int find_slot(const uint32_t *row, uint64_t mask, uin
     while (mask) {
         const int i = __ffsll((unsigned long long)mask) - 1;
         mask &= mask - 1;
-        if ((row[i] & CT_IDX_MASK) == b) return i;
+        if ((row[RS(i)] & CT_IDX_MASK) == b) return i;
     }
     return -1;
 }
@@ -535,20 +548,20 @@ __device__ void nl_build_one(
     }
     // maps between list entries and the slots of the particle's row
     const int NCS = (int)a.nc;
-    uint8_t *scand = slot_cand + (size_t)a_orig * NCS;
-    for (int sl = 0; sl < NCS; ++sl) scand[sl] = 0xffu;
+    uint8_t *scand = slot_cand + ct_base(a_orig, NCS);
+    for (int sl = 0; sl < NCS; ++sl) scand[RS(sl)] = 0xffu;
     if (cnt > n.cap) {
         nl_cnt[a_orig] = NL_OVF;
         atomicOr(&flags[NLF_STICKY], 1u);
     } else {
         nl_cnt[a_orig] = cnt;
-        const uint32_t *row = contacts + (size_t)a_orig * NCS;
+        const uint32_t *row = contacts + ct_base(a_orig, NCS);
         const uint64_t mask = cmask[a_orig];
         float gmin = INFINITY;
         for (uint32_t k = 0; k < cnt; ++k) {
             const int sl = find_slot(row, mask, nl[(size_t)k * np + a_orig]);
             nl_slot[(size_t)k * np + a_orig] = sl < 0 ? 0xffu : (uint8_t)sl;
-            if (sl >= 0) scand[sl] = (uint8_t)k;
+            if (sl >= 0) scand[RS(sl)] = (uint8_t)k;
             else gmin = fminf(gmin, nl_gap[(size_t)k * np + a_orig]);
         }
         nl_gmin[a_orig] = gmin;
@@ -927,7 +940,7 @@ __device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 fou
                 d3 dm;
                 (void)nl_image(g, gx, gy, gz, cellc[nl[li]], offc, code, dm);
                 const uint8_t sl = nl_slot[li];
-                row[sl] = ct_word(row[sl] & CT_IDX_MASK, code);
+                row[RS(sl)] = ct_word(row[RS(sl)] & CT_IDX_MASK, code);
             }
         }
         while (any_bit(fresh)) {
@@ -935,10 +948,10 @@ __device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 fou
             const uint32_t k = nl_first(g, gx, gy, gz, fresh, nl, np, pos, cellc, b, offc, code);
             clear_bit(fresh, k);
             const int cpos = 63 - __clzll((long long)(~mask & full));
-            row[cpos] = ct_word(b, code);
+            row[RS(cpos)] = ct_word(b, code);
             mask |= 1ull << cpos;
             nl_slot[(size_t)k * np + pos] = (uint8_t)cpos;
-            scand[cpos] = (uint8_t)k;
+            scand[RS(cpos)] = (uint8_t)k;
         }
         return false;
     }
@@ -957,12 +970,12 @@ __device__ bool slot_update(const Grid &g, uint32_t np, uint32_t pos, Set128 fou
             cpos = emp ? 63 - __clzll((long long)emp) : nc;
         }
         if (cpos < NCS) {
-            const uint8_t prev = scand[cpos];
+            const uint8_t prev = scand[RS(cpos)];
             if (prev != 0xffu && prev != (uint8_t)k) nl_slot[(size_t)prev * np + pos] = 0xffu;
-            row[cpos] = ct_word(b, code);
+            row[RS(cpos)] = ct_word(b, code);
             mask |= 1ull << cpos;
             nl_slot[(size_t)k * np + pos] = (uint8_t)cpos;
-            scand[cpos] = (uint8_t)k;
+            scand[RS(cpos)] = (uint8_t)k;
         } else {
             ovf = true;
         }
@@ -1327,7 +1340,7 @@ __device__ bool topology_cells(const Grid &g, uint32_t idx_a, double4 xa, int gx
                         cpos = emp ? 63 - __clzll((long long)emp) : nc;
                     }
                     if (cpos < NCS) {
-                        row[cpos] = ct_word(b_orig, code);
+                        row[RS(cpos)] = ct_word(b_orig, code);
                         mask |= 1ull << cpos;
                     } else {
                         ovf = true;
@@ -1364,7 +1377,7 @@ __global__ void __launch_bounds__(256) k_topology_cells(
     const uint64_t mask0 = cmask[a_orig];
     uint64_t mask = mask0;
     const bool ovf = topology_cells(g, idx_a, xa, gx, gy, gz, vals, cell_start, cell_end, x_s,
-                                    contacts + (size_t)a_orig * NCS, mask, NCS, full);
+                                    contacts + ct_base(a_orig, NCS), mask, NCS, full);
     if (ovf) atomicOr(err, ERRF_OVERFLOW);
     if (mask != mask0) cmask[a_orig] = mask;
 }
@@ -1433,7 +1446,7 @@ __device__ void owner_test_all(const Grid &g, uint32_t np, uint32_t pos, uint32_
         set_bit(found, k);
         const uint8_t sl = nl_slot[li];
         if (sl == 0xffu) set_bit(fresh, k);
-        else if ((row[sl] >> CT_CODE_SHIFT) != code) set_bit(recode, k);
+        else if ((row[RS(sl)] >> CT_CODE_SHIFT) != code) set_bit(recode, k);
     }
 }
 
@@ -1606,7 +1619,7 @@ __device__ __forceinline__ void topology_lists_body(
                         fl = 1;
                         if (sq[u] == 0xffu)
                             fl |= 2;
-                        else if ((contacts[(size_t)s_a[o] * NCS + sq[u]] >> CT_CODE_SHIFT) !=
+                        else if ((contacts[ct_slot(s_a[o], sq[u], NCS)] >> CT_CODE_SHIFT) !=
                                  code)
                             fl |= 4;
                     }
@@ -1626,7 +1639,7 @@ __device__ __forceinline__ void topology_lists_body(
     }
     if (!busy) return;
     const uint64_t fullmask = NCS >= 64 ? ~0ull : ((1ull << NCS) - 1ull);
-    uint32_t *row = contacts + (size_t)a_orig * NCS;
+    uint32_t *row = contacts + ct_base(a_orig, NCS);
     const uint64_t mask0 = cmask[a_orig];
     uint64_t mask = mask0;
     bool ovf = false;
@@ -1636,7 +1649,7 @@ __device__ __forceinline__ void topology_lists_body(
     } else if (full) {
         if (any_bit(fresh) || any_bit(recode) || __popcll(found.lo) + __popcll(found.hi) > NCS)
             ovf = slot_update(g, np, pos, found, fresh, recode, nl, nl_slot, cellc, gx, gy, gz,
-                              row, slot_cand + (size_t)a_orig * NCS, mask, NCS, fullmask);
+                              row, slot_cand + ct_base(a_orig, NCS), mask, NCS, fullmask);
     } else if (any_bit(fresh)) {
         // only new contacts were tested; if they do not all fit into empty
         // slots the full rule needs every contact found: test all entries
@@ -1647,7 +1660,7 @@ __device__ __forceinline__ void topology_lists_body(
         else
             found = fresh;
         ovf = slot_update(g, np, pos, found, fresh, recode, nl, nl_slot, cellc, gx, gy, gz, row,
-                          slot_cand + (size_t)a_orig * NCS, mask, NCS, fullmask);
+                          slot_cand + ct_base(a_orig, NCS), mask, NCS, fullmask);
     }
     if (ovf) atomicOr(err, ERRF_OVERFLOW);
     if (mask != mask0) cmask[a_orig] = mask;
@@ -1830,7 +1843,7 @@ __device__ __forceinline__ void forces_body(
             // interact body for one slot, contactsearch.cuh:484-559
             const uint32_t ao = s_a[s_own[lane]];
             const uint32_t sl = s_slot[lane];
-            const size_t mempos = (size_t)ao * NCS + sl;
+            const size_t mempos = ct_slot(ao, sl, NCS);
             const double4 xa = x[ao];
             const uint32_t cw = contacts[mempos];
             const uint32_t b = cw & CT_IDX_MASK;
@@ -2130,13 +2143,13 @@ __global__ void __launch_bounds__(256) k_bond_compute(
 
 // Per bonded particle: force/torque updates of cohesion.cuh:220-230 applied in
 // ascending bond order.  ent = bond index << 1 | (1 if the particle is bond.y).
-__global__ void __launch_bounds__(256) k_bond_gather(
+__device__ __forceinline__ void bond_gather_body(
     uint32_t nbp, const uint32_t *__restrict__ plist, const uint32_t *__restrict__ off,
     const uint32_t *__restrict__ ent, const double4 *__restrict__ bf,
     const double4 *__restrict__ bt, const uint8_t *__restrict__ applied,
-    double3 *__restrict__ force, double3 *__restrict__ torque)
+    double3 *__restrict__ force, double3 *__restrict__ torque, uint32_t bid)
 {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t k = bid * blockDim.x + threadIdx.x;
     if (k >= nbp) return;
     uint32_t pi = plist[k];
     double3 F = force[pi], T = torque[pi];
@@ -2155,6 +2168,15 @@ __global__ void __launch_bounds__(256) k_bond_gather(
     }
     force[pi] = F;
     torque[pi] = T;
+}
+
+__global__ void __launch_bounds__(256) k_bond_gather(
+    uint32_t nbp, const uint32_t *__restrict__ plist, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ ent, const double4 *__restrict__ bf,
+    const double4 *__restrict__ bt, const uint8_t *__restrict__ applied,
+    double3 *__restrict__ force, double3 *__restrict__ torque)
+{
+    bond_gather_body(nbp, plist, off, ent, bf, bt, applied, force, torque, blockIdx.x);
 }
 
 void launch_bonds(hipStream_t s, const StepArgs &a, const DemBuffers &b, bool gather)
@@ -2562,7 +2584,7 @@ __global__ void __launch_bounds__(256) k_rebuild_mask(uint32_t np, uint32_t nc,
     if (i >= np) return;
     uint64_t m = 0;
     for (uint32_t k = 0; k < nc; ++k) {
-        size_t pos = (size_t)i * nc + k;
+        size_t pos = ct_slot(i, k, nc);
         if (contacts[pos] != np) {
             m |= 1ull << k;
         } else {
@@ -2772,24 +2794,6 @@ uint32_t nl_build_blocks(uint32_t np)
     return nb > COND_BLOCKS ? COND_BLOCKS : nb;
 }
 
-// The box of this block, the block's index within the box and the box's
-// block count; false for the padding blocks after a box's last block.
-__device__ __forceinline__ bool multi_block(const MultiMap &m, uint32_t &box, uint32_t &bid,
-                                            uint32_t &nblk)
-{
-    const uint32_t b = blockIdx.x;
-    uint32_t lo = 0, hi = m.nbox;
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (m.start[mid] <= b) lo = mid;
-        else hi = mid;
-    }
-    box = lo;
-    bid = b - m.start[lo];
-    nblk = m.start[m.nbox + lo];
-    return bid < nblk;
-}
-
 __global__ void __launch_bounds__(256) k_rebuild_multi(const RebuildTask *__restrict__ t, MultiMap m)
 {
     uint32_t k, bid, nblk;
@@ -2898,6 +2902,68 @@ void launch_bonds_multi(hipStream_t s, const BondTask *t, MultiMap m, uint32_t b
 void launch_integrate_multi(hipStream_t s, const IntegrateTask *t, MultiMap m, uint32_t blocks)
 {
     if (blocks) hipLaunchKernelGGL(k_integrate_multi, dim3(blocks), dim3(256), 0, s, t, m);
+}
+
+HashTask hash_task(const StepArgs &a, const NLArgs &n, const DemBuffers &b, uint32_t cur,
+                   uint32_t force)
+{
+    HashTask t;
+    t.a = a;
+    t.n = n;
+    t.x = b.x;
+    t.keys = b.keys;
+    t.vals = b.vals;
+    t.cellc = b.cellc;
+    t.x_ref = b.x_ref;
+    t.disp = b.nl_disp;
+    t.flags = b.nl_flags;
+    t.cur = cur;
+    t.force = force;
+    t.err = b.errflags;
+    return t;
+}
+
+BondGatherTask bond_gather_task(const DemBuffers &b)
+{
+    BondGatherTask t;
+    t.nbp = b.nbp;
+    t.plist = b.bp_list;
+    t.off = b.bp_off;
+    t.ent = b.bp_ent;
+    t.bf = b.bond_f;
+    t.bt = b.bond_t;
+    t.applied = b.bond_applied;
+    t.force = b.force;
+    t.torque = b.torque;
+    return t;
+}
+
+__global__ void __launch_bounds__(256) k_hash_multi(const HashTask *__restrict__ t, MultiMap m)
+{
+    uint32_t k, bid, nblk;
+    if (!multi_block(m, k, bid, nblk)) return;
+    const HashTask &T = t[k];
+    hash_nl_body(T.a, T.n, T.x, T.keys, T.vals, T.cellc, T.x_ref, T.disp, T.flags, T.cur, T.force,
+                 T.err, bid);
+}
+
+__global__ void __launch_bounds__(256) k_bond_gather_multi(const BondGatherTask *__restrict__ t,
+                                                           MultiMap m)
+{
+    uint32_t k, bid, nblk;
+    if (!multi_block(m, k, bid, nblk)) return;
+    const BondGatherTask &T = t[k];
+    bond_gather_body(T.nbp, T.plist, T.off, T.ent, T.bf, T.bt, T.applied, T.force, T.torque, bid);
+}
+
+void launch_hash_multi(hipStream_t s, const HashTask *t, MultiMap m, uint32_t blocks)
+{
+    if (blocks) hipLaunchKernelGGL(k_hash_multi, dim3(blocks), dim3(256), 0, s, t, m);
+}
+
+void launch_bond_gather_multi(hipStream_t s, const BondGatherTask *t, MultiMap m, uint32_t blocks)
+{
+    if (blocks) hipLaunchKernelGGL(k_bond_gather_multi, dim3(blocks), dim3(256), 0, s, t, m);
 }
 
 void launch_walls_multi(hipStream_t s, const WallsTask *t, uint32_t n)

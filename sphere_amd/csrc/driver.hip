@@ -62,6 +62,7 @@ struct sph_ctx {
     uint32_t nl_cap = 128;   // list entries per particle
     bool rb_fused = true;    // list rebuild as one launch (k_rebuild)
     uint32_t rb_blocks = 0;  // its grid; 0: separate sort, reorder and list launches
+    uint32_t rb_cap = 0;     // blocks its barrier words serve (the multi-box rebuild of fluid boxes too)
     uint32_t rb_spin = 0;    // its barrier wait bound in sleeps (0: the default)
     bool nl_on = false;
     NLArgs nla{};
@@ -389,8 +390,9 @@ static int upload(sph_ctx *ctx)
     AL(b.cell_start, uint32_t, ctx->ncells); AL(b.cell_end, uint32_t, ctx->ncells);
     AL(b.x_s, double4, np); AL(b.vel_s, double4, np); AL(b.angvel_s, double3, np);
     const size_t NCS = ctx->nc;
-    AL(b.contacts, uint32_t, np * NCS); AL(b.cmask, uint64_t, np);
-    AL(b.delta_t, double, np * NCS * 3);
+    const size_t NCE = ct_entries((uint32_t)np, (uint32_t)NCS);  // contact-array entries
+    AL(b.contacts, uint32_t, NCE); AL(b.cmask, uint64_t, np);
+    AL(b.delta_t, double, NCE * 3);
     AL(b.w_nx, double4, MAXW); AL(b.w_mvfd, double4, MAXW); AL(b.w_acc, double, MAXW);
     AL(b.walls_force_pp, double, np * (h.nw ? h.nw : 1));
     AL(b.wall_partial, double, nblocks);
@@ -419,9 +421,9 @@ static int upload(sph_ctx *ctx)
     HIPCHK(hipMemcpyAsync(b.p, hs.p.data(), np * 8, hipMemcpyHostToDevice, s));
     launch_fill_f64(s, (double *)b.acc, np * 3, 0.0);
     launch_fill_f64(s, (double *)b.angacc, np * 3, 0.0);
-    launch_fill_u32(s, b.contacts, np * NCS, (uint32_t)np);  // device.cu:471-473
+    launch_fill_u32(s, b.contacts, NCE, (uint32_t)np);  // device.cu:471-473
     launch_fill_u32(s, (uint32_t *)b.cmask, 2 * np, 0u);
-    launch_fill_f64(s, b.delta_t, np * NCS * 3, 0.0);
+    launch_fill_f64(s, b.delta_t, NCE * 3, 0.0);
     launch_fill_f64(s, b.walls_force_pp, np * (h.nw ? h.nw : 1), 0.0);
     launch_fill_f64(s, b.w_acc, MAXW, 0.0);
     launch_fill_u32(s, b.errflags, 1, 0u);
@@ -559,19 +561,21 @@ static int upload(sph_ctx *ctx)
             AL(b.nl_gap, float, (size_t)ctx->nl_cap * np);
             AL(b.nl_gmin, float, np); AL(b.nl_disp, float, np);
             AL(b.nl_slot, uint8_t, (size_t)ctx->nl_cap * np);
-            AL(b.slot_cand, uint8_t, np * NCS);
+            AL(b.slot_cand, uint8_t, NCE);
             launch_fill_u32(s, b.nl_flags, NLF_WORDS, 0u);
-            launch_fill_u32(s, (uint32_t *)b.slot_cand, (np * NCS + 3) / 4, 0xffffffffu);
+            launch_fill_u32(s, (uint32_t *)b.slot_cand, (NCE + 3) / 4, 0xffffffffu);
             launch_fill_f64(s, (double *)b.x_ref, np * 4, 0.0);
             launch_fill_u32(s, (uint32_t *)b.nl_disp, np, 0u);
             ctx->nl_on = true;
             // one resident block per compute unit at most (the rebuild
             // kernel's grid barriers need every block running)
             ctx->rb_blocks = 0;
+            ctx->rb_cap = 0;
             int cus = 0;
             // (a fluid run rebuilds every step: there the separate launches,
-            // on full grids, are faster)
-            if (ctx->rb_fused && !h.fluid && rebuild_blocks_per_cu() > 0 &&
+            // on full grids, are faster; its barrier words serve the
+            // multi-box launches of an ensemble)
+            if (ctx->rb_fused && rebuild_blocks_per_cu() > 0 &&
                 hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                       ctx->device) == hipSuccess && cus > 0) {
                 const uint32_t want = (uint32_t)((np + 255) / 256);
@@ -581,7 +585,8 @@ static int upload(sph_ctx *ctx)
                 AL(b.rb_bar, uint32_t, 2);
                 AL(b.rb_part, uint32_t, blocks);
                 launch_fill_u32(s, b.rb_bar, 2, 0u);
-                ctx->rb_blocks = blocks;
+                ctx->rb_cap = blocks;
+                if (!h.fluid) ctx->rb_blocks = blocks;
             }
         }
     }
@@ -947,6 +952,53 @@ struct RestartHeader {
 // then per occupied slot in particle and slot order the contact word (u32),
 // and then their tangential springs (3 FP64 each)
 
+// The contact arrays as row-major host arrays (slot k of grain a at a nc +
+// k, the order of the reference's dev_contacts), converted from and to the
+// device layout (ct_slot).  Either array may be null.
+static int rows_to_host(sph_ctx *ctx, uint32_t *rows, double *dts)
+{
+    const size_t np = ctx->np, nc = ctx->nc, ne = ct_entries(ctx->np, ctx->nc);
+    if (!np) return SPH_OK;
+    if (rows) {
+        std::vector<uint32_t> d(ne);
+        HIPCHK(hipMemcpy(d.data(), ctx->db.contacts, ne * 4, hipMemcpyDeviceToHost));
+        for (size_t a = 0; a < np; ++a)
+            for (size_t k = 0; k < nc; ++k) rows[a * nc + k] = d[ct_slot((uint32_t)a, (uint32_t)k, (uint32_t)nc)];
+    }
+    if (dts) {
+        std::vector<double> d(ne * 3);
+        HIPCHK(hipMemcpy(d.data(), ctx->db.delta_t, ne * 24, hipMemcpyDeviceToHost));
+        for (size_t a = 0; a < np; ++a)
+            for (size_t k = 0; k < nc; ++k) {
+                const size_t q = ct_slot((uint32_t)a, (uint32_t)k, (uint32_t)nc);
+                for (int c = 0; c < 3; ++c) dts[(a * nc + k) * 3 + c] = d[3 * q + c];
+            }
+    }
+    return SPH_OK;
+}
+
+static int rows_to_device(sph_ctx *ctx, const uint32_t *rows, const double *dts)
+{
+    const size_t np = ctx->np, nc = ctx->nc, ne = ct_entries(ctx->np, ctx->nc);
+    if (!np) return SPH_OK;
+    if (rows) {
+        std::vector<uint32_t> d(ne, (uint32_t)np);
+        for (size_t a = 0; a < np; ++a)
+            for (size_t k = 0; k < nc; ++k) d[ct_slot((uint32_t)a, (uint32_t)k, (uint32_t)nc)] = rows[a * nc + k];
+        HIPCHK(hipMemcpy(ctx->db.contacts, d.data(), ne * 4, hipMemcpyHostToDevice));
+    }
+    if (dts) {
+        std::vector<double> d(ne * 3, 0.0);
+        for (size_t a = 0; a < np; ++a)
+            for (size_t k = 0; k < nc; ++k) {
+                const size_t q = ct_slot((uint32_t)a, (uint32_t)k, (uint32_t)nc);
+                for (int c = 0; c < 3; ++c) d[3 * q + c] = dts[(a * nc + k) * 3 + c];
+            }
+        HIPCHK(hipMemcpy(ctx->db.delta_t, d.data(), ne * 24, hipMemcpyHostToDevice));
+    }
+    return SPH_OK;
+}
+
 static int write_restart(sph_ctx *ctx, const std::string &path)
 {
     const sph_state &h = ctx->hs.h;
@@ -973,8 +1025,8 @@ static int write_restart(sph_ctx *ctx, const std::string &path)
         HIPCHK(hipMemcpy(acc.data(), ctx->db.acc, np * 24, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(angacc.data(), ctx->db.angacc, np * 24, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(mask.data(), ctx->db.cmask, np * 8, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(rows.data(), ctx->db.contacts, np * nc * 4, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(dts.data(), ctx->db.delta_t, np * nc * 24, hipMemcpyDeviceToHost));
+        const int rc = rows_to_host(ctx, rows.data(), dts.data());
+        if (rc) return rc;
     }
     std::vector<uint32_t> words;
     std::vector<double> springs;
@@ -1062,8 +1114,8 @@ static int read_restart(sph_ctx *ctx, const std::string &path)
     if (np) {
         HIPCHK(hipMemcpy(ctx->db.acc, acc.data(), np * 24, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->db.angacc, angacc.data(), np * 24, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(ctx->db.contacts, rows.data(), np * nc * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(ctx->db.delta_t, dts.data(), np * nc * 24, hipMemcpyHostToDevice));
+        rc = rows_to_device(ctx, rows.data(), dts.data());
+        if (rc) return rc;
         HIPCHK(hipMemcpy(ctx->db.cmask, mask.data(), np * 8, hipMemcpyHostToDevice));
     }
     // the device copy of the step counter and time (read by the kernels)
@@ -1632,7 +1684,8 @@ int sph_read_contacts(sph_ctx *ctx, uint32_t *contacts, double *distmod, double 
     const size_t n = (size_t)ctx->np * ctx->nc;
     if (!n) return SPH_OK;
     std::vector<uint32_t> c(n), code(n);
-    HIPCHK(hipMemcpy(c.data(), ctx->db.contacts, n * 4, hipMemcpyDeviceToHost));
+    int rc = rows_to_host(ctx, c.data(), delta_t);
+    if (rc) return rc;
     for (size_t i = 0; i < n; ++i) {
         code[i] = c[i] >> CT_CODE_SHIFT;
         c[i] &= CT_IDX_MASK;
@@ -1653,7 +1706,6 @@ int sph_read_contacts(sph_ctx *ctx, uint32_t *contacts, double *distmod, double 
             distmod[3 * i + 2] = 0.0;
         }
     }
-    if (delta_t) HIPCHK(hipMemcpy(delta_t, ctx->db.delta_t, n * 24, hipMemcpyDeviceToHost));
     return SPH_OK;
 }
 
@@ -1680,9 +1732,9 @@ int sph_write_contacts(sph_ctx *ctx, const uint32_t *contacts, const double *dis
         }
     }
     hipStream_t s = ctx->stream;
-    HIPCHK(hipMemcpyAsync(ctx->db.contacts, words.data(), n * 4, hipMemcpyHostToDevice, s));
-    if (delta_t) HIPCHK(hipMemcpyAsync(ctx->db.delta_t, delta_t, n * 24, hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
+    int rc = rows_to_device(ctx, words.data(), delta_t);
+    if (rc) return rc;
     launch_rebuild_mask(s, ctx->db, ctx->np, ctx->nc);
     ctx->nl_force = 1;  // the list/slot maps are rebuilt from the new rows
     return sync(ctx);
@@ -1843,6 +1895,14 @@ struct EnsPhase {  // the launches of one step
     uint32_t nclocks = 0;
 };
 
+struct EnsFluidPhase {  // the particle-side launches of one step of the fluid boxes
+    EnsStage hash, rebuild, nlbuild, topo[2], forces[2], bonds, gather, integ;
+    void *walls = nullptr;
+    uint32_t nwalls = 0;
+    void *clocks = nullptr;
+    uint32_t nclocks = 0;
+};
+
 struct sph_ensemble {
     std::vector<sph_ctx *> boxes;
     int device = 0;
@@ -1863,6 +1923,27 @@ struct sph_ensemble {
     bool use_graph = true;
     uint64_t graph_replays = 0, multi_steps = 0;
     uint32_t launches_per_step = 0;
+    // fluid boxes stepped by multi-box launches (section of their own: the
+    // Jacobi loop's batches end in a host read of every box's solve state)
+    std::vector<int> fluid;
+    std::vector<char> fl_elig;  // which boxes were eligible at planning
+    std::vector<uint64_t> fl_gen;
+    std::vector<uint32_t> fl_parity;
+    std::vector<uint32_t *> fl_keys;
+    bool fl_planned = false;
+    uint32_t fl_phase = 0;
+    EnsFluidPhase fph[2];
+    // Darcy stages: every box (cells, particles, sweep blocks, cells with
+    // ghosts), and the boxes with a flux face or periodic = 2
+    EnsStage dm_cells, dm_parts, dm_sweep, dm_all, dm_flux, dm_per2, dm_net;
+    void *dm_tasks = nullptr;  // every box's record, one thread or block per box
+    NetMulti *netm = nullptr;  // the forward passes of the boxes with a network
+    int net_max_corr = 0;      // their corrections per solve at most
+    uint32_t fl_maxiter = 0;
+    unsigned long long *dm_states = nullptr;    // device: 4 state words per box
+    unsigned long long *dm_states_h = nullptr;  // pinned host copy
+    std::vector<void *> fl_dmem;
+    uint64_t fluid_steps = 0;
 };
 
 static int ens_fail(sph_ensemble *e, int code, const std::string &msg)
@@ -1892,7 +1973,7 @@ static void ens_drop_plan(sph_ensemble *e)
 static bool ens_multi_eligible(const sph_ctx *c)
 {
     const sph_state &h = c->hs.h;
-    return c->loaded && h.np > 0 && !h.fluid && c->nl_on && c->rb_blocks > 0 &&
+    return c->loaded && h.np > 0 && !h.fluid && c->nl_on && c->rb_cap > 0 &&
            (h.contactmodel == 2 || h.contactmodel == 3) && !(h.adaptive == 1 && h.nw > 0) &&
            !(h.nw > 0 && h.wmode[0] == 3) && !c->prof && c->db.rb_bar && c->db.rb_part;
 }
@@ -1907,8 +1988,9 @@ static int radix_passes(const sph_ctx *c)
 // block starts (multiples of 8) and counts.
 template <class T>
 static int ens_stage(sph_ensemble *e, EnsStage &st, const std::vector<T> &tasks,
-                     const std::vector<uint32_t> &counts)
+                     const std::vector<uint32_t> &counts, std::vector<void *> *mem = nullptr)
 {
+    std::vector<void *> &keep = mem ? *mem : e->dmem;
     st.nbox = (uint32_t)tasks.size();
     st.blocks = 0;
     if (tasks.empty()) return SPH_OK;
@@ -1923,10 +2005,10 @@ static int ens_stage(sph_ensemble *e, EnsStage &st, const std::vector<T> &tasks,
     void *dt = nullptr, *dm = nullptr;
     if (hipMalloc(&dt, sizeof(T) * tasks.size()) != hipSuccess)
         return ens_fail(e, SPH_ERR_HIP, "ensemble: device allocation failed");
-    e->dmem.push_back(dt);
+    keep.push_back(dt);
     if (hipMalloc(&dm, 4 * map.size()) != hipSuccess)
         return ens_fail(e, SPH_ERR_HIP, "ensemble: device allocation failed");
-    e->dmem.push_back(dm);
+    keep.push_back(dm);
     if (hipMemcpy(dt, tasks.data(), sizeof(T) * tasks.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dm, map.data(), 4 * map.size(), hipMemcpyHostToDevice) != hipSuccess)
         return ens_fail(e, SPH_ERR_HIP, "ensemble: copy of the task records failed");
@@ -1936,13 +2018,14 @@ static int ens_stage(sph_ensemble *e, EnsStage &st, const std::vector<T> &tasks,
 }
 
 template <class T>
-static int ens_array(sph_ensemble *e, void *&dst, const std::vector<T> &v)
+static int ens_array(sph_ensemble *e, void *&dst, const std::vector<T> &v,
+                     std::vector<void *> *mem = nullptr)
 {
     dst = nullptr;
     if (v.empty()) return SPH_OK;
     if (hipMalloc(&dst, sizeof(T) * v.size()) != hipSuccess)
         return ens_fail(e, SPH_ERR_HIP, "ensemble: device allocation failed");
-    e->dmem.push_back(dst);
+    (mem ? *mem : e->dmem).push_back(dst);
     if (hipMemcpy(dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
         return ens_fail(e, SPH_ERR_HIP, "ensemble: copy of the task records failed");
     return SPH_OK;
@@ -1952,23 +2035,23 @@ static int ens_array(sph_ensemble *e, void *&dst, const std::vector<T> &v)
 // that the whole launch stays within a quarter of the device's resident
 // capacity for the kernel (every box's grid barrier needs its blocks
 // running together).
-static std::vector<uint32_t> ens_rebuild_blocks(const sph_ensemble *e)
+static std::vector<uint32_t> ens_rebuild_blocks(const sph_ensemble *e, const std::vector<int> &ids)
 {
-    std::vector<uint32_t> nb(e->multi.size(), 1u);
+    std::vector<uint32_t> nb(ids.size(), 1u);
     int cus = 0;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device);
-    const uint64_t budget = std::max<uint64_t>(e->multi.size(),
+    const uint64_t budget = std::max<uint64_t>(ids.size(),
                                                (uint64_t)rebuild_blocks_per_cu() * std::max(cus, 1) / 4);
     uint64_t total_np = 0, want = 0;
-    for (int k : e->multi) {
+    for (int k : ids) {
         total_np += e->boxes[k]->hs.h.np;
-        want += e->boxes[k]->rb_blocks;
+        want += e->boxes[k]->rb_cap;
     }
-    for (size_t i = 0; i < e->multi.size(); ++i) {
-        const sph_ctx *c = e->boxes[e->multi[i]];
-        uint64_t b = c->rb_blocks;
-        if (want > budget) b = std::max<uint64_t>(1, (budget - e->multi.size()) * c->hs.h.np / std::max<uint64_t>(total_np, 1));
-        nb[i] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, c->rb_blocks));
+    for (size_t i = 0; i < ids.size(); ++i) {
+        const sph_ctx *c = e->boxes[ids[i]];
+        uint64_t b = c->rb_cap;
+        if (want > budget) b = std::max<uint64_t>(1, (budget - ids.size()) * c->hs.h.np / std::max<uint64_t>(total_np, 1));
+        nb[i] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, c->rb_cap));
     }
     return nb;
 }
@@ -1988,7 +2071,7 @@ static int ens_plan(sph_ensemble *e)
         e->plan_keys.push_back(c->db.keys);
     }
     e->phase = 0;
-    const std::vector<uint32_t> rbn = ens_rebuild_blocks(e);
+    const std::vector<uint32_t> rbn = ens_rebuild_blocks(e, e->multi);
     e->launches_per_step = 0;
     for (int j = 0; j < 2; ++j) {
         std::vector<RebuildTask> rb;
@@ -2127,10 +2210,9 @@ static void ens_advance_all(sph_ensemble *e)
     e->multi_steps++;
 }
 
-// The steps that can be enqueued without the host: no fluid on the
-// box-by-box path is checked by step_once itself; here, no output event
-// and no change of the velocity-step state (src/device.cu:2461-2469) inside
-// the steps, and no adaptive grid.
+// Steps whose launches need no host decision between them: no output
+// event and no change of the velocity-step state (src/device.cu:2461-2469)
+// inside the steps, and no adaptive grid.
 static int ens_check(sph_ensemble *e, int nsteps)
 {
     for (size_t i = 0; i < e->boxes.size(); ++i) {
@@ -2138,7 +2220,6 @@ static int ens_check(sph_ensemble *e, int nsteps)
         const sph_state &h = c->hs.h;
         const std::string b = "box " + std::to_string(i) + ": ";
         if (!c->loaded) return ens_fail(e, SPH_ERR_STATE, b + "no state loaded");
-        if (h.fluid) return ens_fail(e, SPH_ERR_VALUE, b + "fluid boxes are stepped with sph_step");
         if (h.adaptive == 1 && h.nw > 0) return ens_fail(e, SPH_ERR_VALUE, b + "adaptive grid");
         double clk = c->filetimeclock, t = h.time_current;
         int vs = c->velocity_state;
@@ -2169,6 +2250,8 @@ static int ens_capture(sph_ensemble *e)
 }
 
 extern "C" {
+
+static void ens_fluid_drop(sph_ensemble *e);
 
 // A new list skin for a loaded context on the list path: the next step
 // rebuilds the lists (exact for any skin, so the results do not change).
@@ -2247,6 +2330,370 @@ int sph_ensemble_autotune(sph_ensemble *e, int trial_steps, const double *fracti
     }
     e->steps += (uint64_t)trial_steps * (uint64_t)nfrac;
     ens_drop_plan(e);  // the task records hold the old list parameters
+    ens_fluid_drop(e);
+    return SPH_OK;
+}
+
+// ------------------------------------------------------------------
+// Fluid boxes in the multi-box launches: the list path with a Darcy grid,
+// one DEM step per fluid step (ndem = 1), no modulated top pressure.  Each
+// stage of darcy_step is one launch for all of them; the Jacobi sweeps run
+// in batches like darcy_step's (each box's device-side test ends its own
+// sweeps), and after each batch the host reads the solve state of every box
+// at once.  The boxes with a learned correction (the same number of
+// corrections per solve) get it after the same deciding sweeps as in
+// darcy_step, their networks' forward passes as multi-box launches.  After a solve the converged
+// iterate is copied into the first iterate buffer where a single-box solve
+// would swap the two buffers, so the task records stay the same from step to
+// step (the other buffer's content is never read before it is written).
+// ------------------------------------------------------------------
+
+static bool ens_fluid_eligible(const sph_ctx *c)
+{
+    const sph_state &h = c->hs.h;
+    return c->loaded && h.np > 0 && h.fluid && c->nl_on && c->rb_cap > 0 &&
+           (h.contactmodel == 2 || h.contactmodel == 3) && !(h.adaptive == 1 && h.nw > 0) &&
+           !(h.nw > 0 && h.wmode[0] == 3) && h.ndem == 1 &&
+           !((h.p_mod_A > 1.0e-5 || h.p_mod_A < -1.0e-5) && h.p_mod_f > 1.0e-7) && !c->prof &&
+           c->db.rb_bar && c->db.rb_part &&
+           (c->net_host.empty() || c->net_max_corr <= 0 ||
+            (h.num[0] % 8 == 0 && h.num[1] % 8 == 0 && h.num[2] % 8 == 0));
+}
+
+static bool ens_learned(const sph_ctx *c) { return !c->net_host.empty() && c->net_max_corr > 0; }
+
+static void ens_fluid_drop(sph_ensemble *e)
+{
+    if (!e->fl_dmem.empty() && e->stream) (void)hipStreamSynchronize(e->stream);
+    for (void *p : e->fl_dmem) (void)hipFree(p);
+    e->fl_dmem.clear();
+    if (e->dm_states_h) (void)hipHostFree(e->dm_states_h);
+    e->dm_states_h = nullptr;
+    net_multi_destroy(e->netm);
+    e->netm = nullptr;
+    e->net_max_corr = 0;
+    e->dm_states = nullptr;
+    e->dm_tasks = nullptr;
+    for (EnsFluidPhase &p : e->fph) p = EnsFluidPhase{};
+    e->dm_cells = e->dm_parts = e->dm_sweep = e->dm_all = e->dm_flux = e->dm_per2 = e->dm_net = EnsStage{};
+    e->fl_planned = false;
+    e->fluid.clear();
+}
+
+static int ens_fluid_plan(sph_ensemble *e)
+{
+    ens_fluid_drop(e);
+    e->fl_elig.assign(e->boxes.size(), 0);
+    for (size_t k = 0; k < e->boxes.size(); ++k) {
+        sph_ctx *c = e->boxes[k];
+        if (!ens_fluid_eligible(c)) continue;
+        e->fl_elig[k] = 1;
+        if (!e->fluid.empty() && c->hs.h.maxiter != e->boxes[e->fluid[0]]->hs.h.maxiter) continue;
+        if (ens_learned(c)) {
+            if (e->net_max_corr == 0) e->net_max_corr = c->net_max_corr;
+            else if (c->net_max_corr != e->net_max_corr) continue;
+            if (net_ready(c) != SPH_OK || !c->net) continue;  // the network and its buffers
+        }
+        e->fluid.push_back((int)k);
+    }
+    e->fl_gen.clear();
+    e->fl_parity.clear();
+    e->fl_keys.clear();
+    if (e->fluid.empty()) {
+        e->fl_planned = true;
+        return SPH_OK;
+    }
+    std::vector<void *> *mem = &e->fl_dmem;
+    for (int k : e->fluid) {
+        sph_ctx *c = e->boxes[k];
+        const sph_state &h = c->hs.h;
+        // darcy_step's top-wall cell (from the wall position at the load)
+        if (h.nw > 0 && (h.wmode[0] == 1 || h.wmode[0] == 3))
+            c->wall0_iz = (unsigned int)(c->host_wx0 / (h.L[2] / h.num[2]));
+        e->fl_gen.push_back(c->gen);
+        e->fl_parity.push_back(c->nl_parity);
+        e->fl_keys.push_back(c->db.keys);
+    }
+    e->fl_maxiter = e->boxes[e->fluid[0]]->hs.h.maxiter;
+    e->fl_phase = 0;
+    const std::vector<uint32_t> rbn = ens_rebuild_blocks(e, e->fluid);
+    int rc;
+    for (int j = 0; j < 2; ++j) {
+        std::vector<HashTask> hs;
+        std::vector<RebuildTask> rb;
+        std::vector<NlBuildTask> nlb;
+        std::vector<TopoTask> topo[2];
+        std::vector<ForcesTask> frc[2];
+        std::vector<BondTask> bnd;
+        std::vector<BondGatherTask> bgt;
+        std::vector<IntegrateTask> itg;
+        std::vector<WallsTask> wls;
+        std::vector<ClockTask> clk;
+        std::vector<uint32_t> n_hs, n_rb, n_nlb, n_topo[2], n_frc[2], n_bnd, n_bgt, n_itg;
+        for (size_t i = 0; i < e->fluid.size(); ++i) {
+            const sph_ctx *c = e->boxes[e->fluid[i]];
+            const sph_state &h = c->hs.h;
+            const StepArgs a = step_args(c);
+            const uint32_t cur = c->nl_parity ^ (uint32_t)j;
+            DemBuffers b = c->db;
+            const bool odd = (radix_passes(c) & 1) != 0;
+            if (odd && j == 1) {
+                std::swap(b.keys, b.keys_tmp);
+                std::swap(b.vals, b.vals_tmp);
+            }
+            hs.push_back(hash_task(a, c->nla, b, cur, 1u));  // a fluid step rebuilds every step
+            n_hs.push_back((h.np + 255) / 256);
+            const uint32_t *cond = c->db.nl_flags + cur;
+            rb.push_back(rebuild_task(a, b, c->ncells, c->key_bits, cond, c->rb_spin));
+            n_rb.push_back(rbn[i]);
+            if (odd) {
+                std::swap(b.keys, b.keys_tmp);
+                std::swap(b.vals, b.vals_tmp);
+            }
+            nlb.push_back(nl_build_task(a, c->nla, b, cond));
+            n_nlb.push_back((h.np + 255) / 256);
+            const int near = c->nla.range == 1 ? 1 : 0;
+            topo[near].push_back(topology_task(a, b, cur));
+            n_topo[near].push_back(topology_blocks(h.np));
+            const int hz = h.contactmodel == 3 ? 1 : 0;
+            frc[hz].push_back(forces_task(a, b, true));
+            n_frc[hz].push_back(forces_blocks(h.np));
+            if (h.nb0 > 0) {
+                bnd.push_back(bond_task(a, b));
+                n_bnd.push_back((h.nb0 + 255) / 256);
+                if (b.nbp) {
+                    bgt.push_back(bond_gather_task(b));
+                    n_bgt.push_back((b.nbp + 255) / 256);
+                }
+            }
+            itg.push_back(integrate_task(a, b, 0, 10.0, h.tau_x, false, NextHash{}));
+            n_itg.push_back((h.np + 255) / 256);
+            if (h.nw > 0) wls.push_back(walls_task(a, b));
+            clk.push_back(ClockTask{b.clock, h.time_dt});
+        }
+        EnsFluidPhase &P = e->fph[j];
+        if ((rc = ens_stage(e, P.hash, hs, n_hs, mem)) || (rc = ens_stage(e, P.rebuild, rb, n_rb, mem)) ||
+            (rc = ens_stage(e, P.nlbuild, nlb, n_nlb, mem)) ||
+            (rc = ens_stage(e, P.topo[0], topo[0], n_topo[0], mem)) ||
+            (rc = ens_stage(e, P.topo[1], topo[1], n_topo[1], mem)) ||
+            (rc = ens_stage(e, P.forces[0], frc[0], n_frc[0], mem)) ||
+            (rc = ens_stage(e, P.forces[1], frc[1], n_frc[1], mem)) ||
+            (rc = ens_stage(e, P.bonds, bnd, n_bnd, mem)) || (rc = ens_stage(e, P.gather, bgt, n_bgt, mem)) ||
+            (rc = ens_stage(e, P.integ, itg, n_itg, mem)) || (rc = ens_array(e, P.walls, wls, mem)) ||
+            (rc = ens_array(e, P.clocks, clk, mem)))
+            return rc;
+        P.nwalls = (uint32_t)wls.size();
+        P.nclocks = (uint32_t)clk.size();
+    }
+    // Darcy records (the same for every step)
+    std::vector<DarcyTask> all, flux, per2, learned;
+    std::vector<uint32_t> n_cells, n_parts, n_sweep, n_all, n_flux, n_per2, n_learned;
+    std::vector<sph_net *> nets;
+    std::vector<const float *> nparams;
+    std::vector<float *> nouts;
+    for (int k : e->fluid) {
+        const sph_ctx *c = e->boxes[k];
+        const sph_state &h = c->hs.h;
+        DarcyTask t{};
+        t.a = step_args(c);
+        t.d = darcy_args(c);
+        t.b = c->dd;
+        t.x = c->db.x;
+        t.force = c->db.force;
+        t.cell_start = c->db.cell_start;
+        t.cell_end = c->db.cell_end;
+        t.x_s = c->db.x_s;
+        t.vel_s = c->db.vel_s;
+        t.err = c->db.errflags;
+        t.clock = c->db.clock;
+        t.tolerance = h.tolerance;
+        t.maxiter = h.maxiter;
+        if (ens_learned(c) && c->net) {
+            t.net_in4 = net_input_bf16(c->net);
+            t.net_out = c->net_out;
+            learned.push_back(t);
+            n_learned.push_back(darcy_cell_blocks(t.d));
+            nets.push_back(c->net);
+            nparams.push_back(c->net_params);
+            nouts.push_back(c->net_out);
+        }
+        all.push_back(t);
+        n_cells.push_back(darcy_cell_blocks(t.d));
+        n_parts.push_back((h.np + 255) / 256);
+        n_sweep.push_back(darcy_sweep_blocks(t.d));
+        n_all.push_back(darcy_all_cell_blocks(t.d));
+        if (h.bc_bot == 4 || h.bc_top == 4) {
+            flux.push_back(t);
+            n_flux.push_back(darcy_cell_blocks(t.d));
+        }
+        if (h.periodic == 2) {
+            per2.push_back(t);
+            n_per2.push_back(darcy_cell_blocks(t.d));
+        }
+    }
+    if ((rc = ens_stage(e, e->dm_cells, all, n_cells, mem)) || (rc = ens_stage(e, e->dm_parts, all, n_parts, mem)) ||
+        (rc = ens_stage(e, e->dm_sweep, all, n_sweep, mem)) || (rc = ens_stage(e, e->dm_all, all, n_all, mem)) ||
+        (rc = ens_stage(e, e->dm_flux, flux, n_flux, mem)) || (rc = ens_stage(e, e->dm_per2, per2, n_per2, mem)) ||
+        (rc = ens_stage(e, e->dm_net, learned, n_learned, mem)))
+        return rc;
+    if (!nets.empty()) {
+        e->netm = net_multi_create(nets.data(), nparams.data(), nouts.data(), (int)nets.size());
+        if (!e->netm) return ens_fail(e, SPH_ERR_HIP, "ensemble: the networks' launch plan failed");
+    }
+    e->dm_tasks = e->dm_cells.tasks;
+    void *st = nullptr;
+    if (hipMalloc(&st, 32 * e->fluid.size()) != hipSuccess ||
+        hipHostMalloc((void **)&e->dm_states_h, 32 * e->fluid.size()) != hipSuccess)
+        return ens_fail(e, SPH_ERR_HIP, "ensemble: device allocation failed");
+    e->fl_dmem.push_back(st);
+    e->dm_states = (unsigned long long *)st;
+    e->fl_planned = true;
+    return SPH_OK;
+}
+
+static bool ens_fluid_current(const sph_ensemble *e)
+{
+    if (!e->fl_planned || e->fl_elig.size() != e->boxes.size()) return false;
+    size_t i = 0;
+    for (size_t k = 0; k < e->boxes.size(); ++k) {
+        const sph_ctx *c = e->boxes[k];
+        if (ens_fluid_eligible(c) != (e->fl_elig[k] != 0)) return false;
+        const bool in = i < e->fluid.size() && e->fluid[i] == (int)k;
+        if (!in) continue;
+        if (c->gen != e->fl_gen[i]) return false;
+        if (c->nl_parity != (e->fl_parity[i] ^ e->fl_phase)) return false;
+        const bool swapped = (radix_passes(c) & 1) != 0 && e->fl_phase == 1;
+        if ((c->db.keys == e->fl_keys[i]) == swapped) return false;
+        ++i;
+    }
+    return i == e->fluid.size();
+}
+
+static MultiMap stage_map(const EnsStage &st) { return MultiMap{st.map, st.nbox}; }
+
+// One step of every fluid box (step_once with darcy_step, multi-box).
+static int ens_fluid_step(sph_ensemble *e)
+{
+    hipStream_t s = e->stream;
+    int rc;
+    const EnsFluidPhase &P = e->fph[e->fl_phase];
+    const DarcyTask *dt = (const DarcyTask *)e->dm_tasks;
+    const uint32_t nb = (uint32_t)e->fluid.size();
+    // particles: hash, list rebuild, contacts, bonds
+    launch_hash_multi(s, (const HashTask *)P.hash.tasks, stage_map(P.hash), P.hash.blocks);
+    launch_rebuild_multi(s, (const RebuildTask *)P.rebuild.tasks, stage_map(P.rebuild), P.rebuild.blocks);
+    launch_nl_build_multi(s, (const NlBuildTask *)P.nlbuild.tasks, stage_map(P.nlbuild), P.nlbuild.blocks);
+    launch_topology_multi(s, (const TopoTask *)P.topo[0].tasks, stage_map(P.topo[0]), P.topo[0].blocks, false);
+    launch_topology_multi(s, (const TopoTask *)P.topo[1].tasks, stage_map(P.topo[1]), P.topo[1].blocks, true);
+    launch_forces_multi(s, (const ForcesTask *)P.forces[0].tasks, stage_map(P.forces[0]), P.forces[0].blocks, false);
+    launch_forces_multi(s, (const ForcesTask *)P.forces[1].tasks, stage_map(P.forces[1]), P.forces[1].blocks, true);
+    launch_bonds_multi(s, (const BondTask *)P.bonds.tasks, stage_map(P.bonds), P.bonds.blocks);
+    launch_bond_gather_multi(s, (const BondGatherTask *)P.gather.tasks, stage_map(P.gather), P.gather.blocks);
+    // darcy_step before the solve
+    const DarcyTask *tc = (const DarcyTask *)e->dm_cells.tasks;
+    const MultiMap mc = stage_map(e->dm_cells);
+    const uint32_t bc = e->dm_cells.blocks;
+    launch_dm_ghosts(s, tc, mc, bc, DM_P);
+    launch_dm_pressure_gradient(s, tc, mc, bc);
+    launch_dm_ghosts(s, tc, mc, bc, DM_GRAD_P);
+    launch_dm_pressure_force(s, (const DarcyTask *)e->dm_parts.tasks, stage_map(e->dm_parts), e->dm_parts.blocks);
+    launch_dm_porosities(s, tc, mc, bc);
+    launch_dm_por_edges(s, (const DarcyTask *)e->dm_per2.tasks, stage_map(e->dm_per2), e->dm_per2.blocks);
+    launch_dm_permeabilities(s, tc, mc, bc);
+    launch_dm_ghosts(s, tc, mc, bc, DM_PHI);
+    launch_dm_ghosts(s, tc, mc, bc, DM_K);
+    launch_dm_ghosts_flux(s, (const DarcyTask *)e->dm_flux.tasks, stage_map(e->dm_flux), e->dm_flux.blocks);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    (void)hipEventRecord(t0, s);
+    launch_dm_solve_start(s, tc, mc, bc);
+    launch_dm_ghosts(s, tc, mc, bc, DM_P);
+    launch_dm_prepare(s, tc, mc, bc);
+    launch_dm_reset(s, dt, nb);
+    // the sweeps, in batches as in darcy_step, until every box's solve ended
+    const DarcyTask *tw = (const DarcyTask *)e->dm_sweep.tasks;
+    const MultiMap mw = stage_map(e->dm_sweep);
+    const uint32_t maxiter = e->fl_maxiter;
+    unsigned int hint = 0;
+    for (int k : e->fluid) hint = std::max(hint, e->boxes[k]->jacobi_hint);
+    unsigned int nijac = 0;
+    bool all_ended = false;
+    unsigned long long want = std::max(12u, (hint + 9u) / 10u * 10u + 2u);
+    int corrections = 0;
+    const DarcyTask *tn = (const DarcyTask *)e->dm_net.tasks;
+    const MultiMap mn = stage_map(e->dm_net);
+    while (nijac < maxiter && !all_ended) {
+        const unsigned int end = (unsigned int)std::min<unsigned long long>(maxiter, want);
+        for (; nijac < end; ++nijac) {
+            const bool decides = nijac > 0 && (nijac - 1) % 10 == 0;
+            const bool correct = decides && e->netm && corrections < e->net_max_corr;
+            launch_dm_sweep(s, tw, mw, e->dm_sweep.blocks, nijac, correct);
+            if (correct) {
+                launch_dm_net_input(s, tn, mn, e->dm_net.blocks, nijac);
+                if ((rc = net_multi_forward(e->netm, s)))
+                    return ens_fail(e, rc, std::string("ensemble: ") + sph_net_last_error());
+                launch_dm_net_apply(s, tn, mn, e->dm_net.blocks, nijac);
+                ++corrections;
+            }
+        }
+        if (nijac == maxiter && (maxiter - 1) % 10 == 0) launch_dm_decide(s, dt, nb, maxiter - 1);
+        launch_dm_states(s, dt, nb, e->dm_states);
+        HIPCHK_E(hipMemcpyAsync(e->dm_states_h, e->dm_states, 32 * nb, hipMemcpyDeviceToHost, s));
+        HIPCHK_E(hipStreamSynchronize(s));
+        all_ended = true;
+        for (uint32_t i = 0; i < nb; ++i) all_ended = all_ended && e->dm_states_h[4 * i] != 0ull;
+        want = (unsigned long long)nijac + 20u;
+    }
+    launch_dm_normalize(s, (const DarcyTask *)e->dm_all.tasks, stage_map(e->dm_all), e->dm_all.blocks);
+    (void)hipEventRecord(t1, s);
+    // after the solve (the step holds no output event: dphi is zeroed)
+    launch_dm_finish(s, tc, mc, bc);
+    launch_dm_ghosts(s, tc, mc, bc, DM_P);
+    launch_dm_velocities(s, tc, mc, bc);
+    // integration
+    launch_integrate_multi(s, (const IntegrateTask *)P.integ.tasks, stage_map(P.integ), P.integ.blocks);
+    launch_walls_multi(s, (const WallsTask *)P.walls, P.nwalls);
+    launch_clock_multi(s, (const ClockTask *)P.clocks, P.nclocks);
+    (void)hipEventSynchronize(t1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    // the host side of the step, box by box (darcy_step's statistics and
+    // conv.log lines, step_once's counters)
+    for (uint32_t i = 0; i < nb; ++i) {
+        sph_ctx *c = e->boxes[e->fluid[i]];
+        const unsigned long long *st = e->dm_states_h + 4 * i;
+        c->stats.fluid_steps++;
+        c->stats.corrections += st[3];
+        const unsigned int ran = st[0] != 0 ? (unsigned int)st[1] + 1u : maxiter;
+        c->stats.sweeps += ran;
+        c->jacobi_hint = ran;
+        double max_norm_res;
+        std::memcpy(&max_norm_res, &st[2], 8);
+        c->stats.last_nijac = (uint32_t)st[1];
+        c->stats.last_max_residual = max_norm_res;
+        c->stats.solve_ms += ms;
+        if (st[0] == 2)
+            return ens_fail(e, SPH_ERR_NAN, "box " + std::to_string(e->fluid[i]) +
+                                                ": normalized residual is NaN: the system has "
+                                                "probably become unstable");
+        if (st[0] == 1) {
+            if (c->iter % 10 == 0) c->conv.emplace_back(c->iter + 1, (unsigned int)st[1]);
+        } else if (maxiter > 0) {
+            c->conv.emplace_back(c->iter + 1, maxiter - 1);
+            c->stats.nonconverged++;
+            fprintf(stderr, "\nIteration %ld, time %g s: Error, the pressure solution in the fluid "
+                            "calculations did not converge. Try increasing the value of "
+                            "'darcy.maxiter' (%u) or increase 'darcy.tolerance' (%g).\n",
+                    c->iter, c->iter * c->hs.h.time_dt, maxiter, c->hs.h.tolerance);
+        }
+        ens_host_advance(c);
+        c->hash_ready = false;  // a fluid step hashes every step
+    }
+    e->fl_phase ^= 1u;
+    e->fluid_steps++;
     return SPH_OK;
 }
 
@@ -2278,6 +2725,7 @@ void sph_ensemble_destroy(sph_ensemble *e)
     if (!e) return;
     (void)hipSetDevice(e->device);
     ens_drop_plan(e);
+    ens_fluid_drop(e);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -2304,6 +2752,8 @@ int sph_ensemble_info(const sph_ensemble *e, sph_ensemble_info_t *out)
     out->graph = e->exec ? 1 : 0;
     out->multi_steps = e->multi_steps;
     out->graph_replays = e->graph_replays;
+    out->fluid_boxes = (uint32_t)e->fluid.size();
+    out->fluid_steps = e->fluid_steps;
     return SPH_OK;
 }
 
@@ -2333,11 +2783,13 @@ int sph_ensemble_step(sph_ensemble *e, int nsteps)
     for (sph_ctx *c : e->boxes)
         if ((rc = sync(c))) return ens_fail(e, rc, c->err);
     if (!ens_plan_current(e) && (rc = ens_plan(e))) return rc;
+    if (!ens_fluid_current(e) && (rc = ens_fluid_plan(e))) return rc;
     std::vector<sph_ctx *> single;
     {
-        size_t i = 0;
+        size_t i = 0, f = 0;
         for (size_t k = 0; k < e->boxes.size(); ++k) {
             if (i < e->multi.size() && e->multi[i] == (int)k) ++i;
+            else if (f < e->fluid.size() && e->fluid[f] == (int)k) ++f;
             else single.push_back(e->boxes[k]);
         }
     }
@@ -2387,6 +2839,10 @@ int sph_ensemble_step(sph_ensemble *e, int nsteps)
             ens_advance_all(e);
         }
     }
+    // the fluid boxes, step by step
+    if (!e->fluid.empty())
+        for (int k = 0; k < nsteps; ++k)
+            if ((rc = ens_fluid_step(e))) return rc;
     // the other boxes, box by box on their own streams
     for (int k = 0; k < nsteps; ++k)
         for (sph_ctx *c : single)

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "common.h"
+#include "multi.h"
 
 // Step counter and simulation time on the device (read by integrate and the
 // wall integration, advanced by k_clock_advance after them), so that a step
@@ -141,11 +142,6 @@ void launch_rebuild_mask(hipStream_t s, const DemBuffers &b, uint32_t np, uint32
 // once.  Each box's arguments are one task record in device memory, the
 // arguments of the single-box kernel, which runs the same body.
 // ------------------------------------------------------------------
-struct MultiMap {
-    const uint32_t *start;  // nbox block starts (ascending), then nbox counts
-    uint32_t nbox;
-};
-
 struct RebuildTask {
     StepArgs a;
     RebuildArgs r;
@@ -238,6 +234,26 @@ struct ClockTask {
     double dt;
 };
 
+struct HashTask {
+    StepArgs a;
+    NLArgs n;
+    const double4 *x;
+    uint32_t *keys, *vals, *cellc;
+    const double4 *x_ref;
+    float *disp;
+    uint32_t *flags;
+    uint32_t cur, force;
+    uint32_t *err;
+};
+
+struct BondGatherTask {
+    uint32_t nbp;
+    const uint32_t *plist, *off, *ent;
+    const double4 *bf, *bt;
+    const uint8_t *applied;
+    double3 *force, *torque;
+};
+
 // Task records of one box for one step, as the single-box launchers form
 // them (rebuild: the sort passes' key buffers as they are before the step).
 RebuildTask rebuild_task(const StepArgs &a, const DemBuffers &b, uint32_t ncells, int bits,
@@ -251,6 +267,9 @@ IntegrateTask integrate_task(const StepArgs &a, const DemBuffers &b, int change_
                              double velocity_factor, double tau_x, bool gather_bonds,
                              const NextHash &nh);
 WallsTask walls_task(const StepArgs &a, const DemBuffers &b);
+HashTask hash_task(const StepArgs &a, const NLArgs &n, const DemBuffers &b, uint32_t cur,
+                   uint32_t force);
+BondGatherTask bond_gather_task(const DemBuffers &b);
 // Blocks a box needs in each multi-box stage.
 uint32_t topology_blocks(uint32_t np);
 uint32_t forces_blocks(uint32_t np);
@@ -266,6 +285,8 @@ void launch_forces_multi(hipStream_t s, const ForcesTask *t, MultiMap m, uint32_
                          bool hertz);
 void launch_bonds_multi(hipStream_t s, const BondTask *t, MultiMap m, uint32_t blocks);
 void launch_integrate_multi(hipStream_t s, const IntegrateTask *t, MultiMap m, uint32_t blocks);
+void launch_hash_multi(hipStream_t s, const HashTask *t, MultiMap m, uint32_t blocks);
+void launch_bond_gather_multi(hipStream_t s, const BondGatherTask *t, MultiMap m, uint32_t blocks);
 // one block per task
 void launch_walls_multi(hipStream_t s, const WallsTask *t, uint32_t n);
 void launch_clock_multi(hipStream_t s, const ClockTask *t, uint32_t n);
@@ -323,3 +344,59 @@ void launch_darcy_net_apply(hipStream_t s, const DarcyArgs &d, const DarcyBuffer
 void launch_darcy_decide(hipStream_t s, const DarcyBuffers &b, const unsigned long long *slot,
                          double tolerance, uint32_t nijac);
 void launch_darcy_velocities(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b);
+
+// ------------------------------------------------------------------
+// Darcy stages of the multi-box launches (ensembles of fluid boxes): one
+// task record per box; the solve's two iterate buffers are b.p (where
+// every solve starts and, after dm_normalize, ends) and b.p_new.
+// ------------------------------------------------------------------
+struct DarcyTask {
+    StepArgs a;
+    DarcyArgs d;
+    DarcyBuffers b;
+    const double4 *x;
+    double3 *force;
+    const uint32_t *cell_start, *cell_end;
+    const double4 *x_s, *vel_s;
+    uint32_t *err;
+    const StepClock *clock;
+    double tolerance;
+    uint32_t maxiter;
+    uint16_t *net_in4;    // the box's network input rows (BF16), null without a network
+    const float *net_out; // the network's output for the box
+};
+// fields of DarcyTask::b that the ghost stage updates
+#define DM_P 0
+#define DM_GRAD_P 1
+#define DM_PHI 2
+#define DM_K 3
+uint32_t darcy_cell_blocks(const DarcyArgs &d);          // 256-thread blocks over the cells
+uint32_t darcy_sweep_blocks(const DarcyArgs &d);         // the sweep's blocks
+uint32_t darcy_all_cell_blocks(const DarcyArgs &d);      // 256-thread blocks over cells and ghosts
+void launch_dm_ghosts(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks, int field);
+void launch_dm_ghosts_flux(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks);
+void launch_dm_pressure_gradient(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks);
+void launch_dm_pressure_force(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks);
+void launch_dm_porosities(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks);
+void launch_dm_por_edges(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks);
+void launch_dm_permeabilities(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks);
+void launch_dm_solve_start(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks);
+void launch_dm_prepare(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks);
+// clears each box's solve state and check slots (one block per box)
+void launch_dm_reset(hipStream_t s, const DarcyTask *t, uint32_t n);
+// sweep nijac of every box's solve (from b.p into b.p_new for even nijac,
+// back for odd); correct: the correction that follows clears its scale slot
+void launch_dm_sweep(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks, uint32_t nijac,
+                     bool correct);
+void launch_dm_decide(hipStream_t s, const DarcyTask *t, uint32_t n, uint32_t nijac);
+// the learned correction of the iterate sweep nijac wrote (boxes with a
+// network): residual maximum, the network's input, and, after the forward
+// pass, p += s e
+void launch_dm_net_input(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks, uint32_t nijac);
+void launch_dm_net_apply(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks, uint32_t nijac);
+// the solve state words of every box into out[4 k .. 4 k + 3]
+void launch_dm_states(hipStream_t s, const DarcyTask *t, uint32_t n, unsigned long long *out);
+// the converged iterate of a box whose solve ended in b.p_new copied into b.p
+void launch_dm_normalize(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks);
+void launch_dm_finish(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks);
+void launch_dm_velocities(hipStream_t s, const DarcyTask *t, MultiMap m, uint32_t blocks);

@@ -237,6 +237,154 @@ int net_forward_packed(sph_net *n, const float *params, const float *input, floa
     return SPH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Multi-box forward passes (ensembles)
+// ---------------------------------------------------------------------------
+struct NetMultiOp {  // one launch: a layer's boxes that share a kernel variant
+    bool pool = false;
+    int variant = 0, cin = 0, cout_pad = 0;
+    void *tasks = nullptr;
+    uint32_t *map = nullptr;
+    uint32_t nbox = 0, blocks = 0;
+};
+
+struct NetMulti {
+    int device = 0;
+    std::vector<NetMultiOp> ops;
+    std::vector<void *> allocs;
+};
+
+template <class T>
+static bool net_multi_op(NetMulti *m, NetMultiOp op, const std::vector<T> &tasks,
+                         const std::vector<uint32_t> &counts)
+{
+    if (tasks.empty()) return true;
+    std::vector<uint32_t> map(2 * tasks.size());
+    uint32_t at = 0;
+    for (size_t k = 0; k < tasks.size(); ++k) {
+        map[k] = at;
+        map[tasks.size() + k] = counts[k];
+        at += (counts[k] + 7u) & ~7u;
+    }
+    void *dt = nullptr, *dm = nullptr;
+    if (hipMalloc(&dt, sizeof(T) * tasks.size()) != hipSuccess) return false;
+    m->allocs.push_back(dt);
+    if (hipMalloc(&dm, 4 * map.size()) != hipSuccess) return false;
+    m->allocs.push_back(dm);
+    if (hipMemcpy(dt, tasks.data(), sizeof(T) * tasks.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dm, map.data(), 4 * map.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return false;
+    op.tasks = dt;
+    op.map = (uint32_t *)dm;
+    op.nbox = (uint32_t)tasks.size();
+    op.blocks = at;
+    m->ops.push_back(op);
+    return true;
+}
+
+void net_multi_destroy(NetMulti *m)
+{
+    if (!m) return;
+    for (void *p : m->allocs) (void)hipFree(p);
+    delete m;
+}
+
+NetMulti *net_multi_create(sph_net *const *nets, const float *const *params, float *const *outputs, int n)
+{
+    NetMulti *m = new NetMulti();
+    (void)hipGetDevice(&m->device);
+    // the layers of net_forward_packed: (level, input, cin, layer, cout_pad,
+    // output, mode, skip, fine output), with the pools between them
+    struct Step {
+        bool pool;
+        int level, cin, layer, cout_pad, mode;
+    };
+    const Step steps[11] = {{false, 0, 4, 0, 16, NET_OUT_BF16_RELU}, {true, 1, 16, 0, 0, 0},
+                            {false, 1, 16, 1, 32, NET_OUT_BF16_RELU}, {true, 2, 32, 0, 0, 0},
+                            {false, 2, 32, 2, 48, NET_OUT_BF16_RELU}, {true, 3, 48, 0, 0, 0},
+                            {false, 3, 48, 3, 64, NET_OUT_BF16_RELU}, {false, 3, 64, 4, 48, NET_OUT_RELU_UP},
+                            {false, 2, 48, 5, 32, NET_OUT_RELU_UP},   {false, 1, 32, 6, 16, NET_OUT_RELU_UP},
+                            {false, 0, 16, 7, 16, NET_OUT_PROJ}};
+    for (int q = 0; q < 11; ++q) {
+        const Step &st = steps[q];
+        if (st.pool) {
+            std::vector<PoolTask> t;
+            std::vector<uint32_t> c;
+            for (int b = 0; b < n; ++b) {
+                const sph_net *nt = nets[b];
+                const uint16_t *in = st.level == 1 ? nt->x0 : st.level == 2 ? nt->x1 : nt->x2;
+                uint16_t *out = st.level == 1 ? nt->p0 : st.level == 2 ? nt->p1 : nt->p2;
+                t.push_back(PoolTask{nt->g[st.level], in, st.cin, out});
+                c.push_back(pool_blocks(nt->g[st.level], st.cin));
+            }
+            NetMultiOp op;
+            op.pool = true;
+            if (!net_multi_op(m, op, t, c)) {
+                net_multi_destroy(m);
+                return nullptr;
+            }
+            continue;
+        }
+        // group the boxes by the kernel variant their grid takes
+        for (int v = 0; v < 5; ++v) {
+            std::vector<ConvTask> t;
+            std::vector<uint32_t> c;
+            for (int b = 0; b < n; ++b) {
+                const sph_net *nt = nets[b];
+                const NetGeom &g = nt->g[st.level];
+                if (conv_variant(g, st.cin, st.cout_pad) != v) continue;
+                ConvTask k{};
+                k.g = g;
+                k.cin = st.cin;
+                k.w = nt->wf[st.layer];
+                k.bias = params[b] + nt->boff[st.layer];
+                k.cout_real = kLayers[st.layer].cout;
+                k.mode = st.mode;
+                switch (st.layer) {
+                case 0: k.in = nt->in4; k.out = nt->x0; break;
+                case 1: k.in = nt->p0; k.out = nt->x1; break;
+                case 2: k.in = nt->p1; k.out = nt->x2; break;
+                case 3: k.in = nt->p2; k.out = nt->x3; break;
+                case 4: k.in = nt->x3; k.out = nt->a3; k.skip = nt->x2; k.yfine = nt->y2; break;
+                case 5: k.in = nt->y2; k.out = nt->a2; k.skip = nt->x1; k.yfine = nt->y1; break;
+                case 6: k.in = nt->y1; k.out = nt->a1; k.skip = nt->x0; k.yfine = nt->y0; break;
+                default:
+                    k.in = nt->y0;
+                    k.out = nt->h0;
+                    k.proj = params[b] + nt->woff[8];
+                    k.out2 = outputs[b];
+                }
+                t.push_back(k);
+                c.push_back(conv_variant_blocks(v, g));
+            }
+            NetMultiOp op;
+            op.variant = v;
+            op.cin = st.cin;
+            op.cout_pad = st.cout_pad;
+            if (!net_multi_op(m, op, t, c)) {
+                net_multi_destroy(m);
+                return nullptr;
+            }
+        }
+    }
+    return m;
+}
+
+int net_multi_forward(NetMulti *m, hipStream_t s)
+{
+    for (const NetMultiOp &op : m->ops) {
+        const MultiMap mm{op.map, op.nbox};
+        if (op.pool) {
+            launch_pool_fwd_multi(s, (const PoolTask *)op.tasks, mm, op.blocks);
+        } else if (launch_conv_fwd_multi(s, op.variant, op.cin, op.cout_pad, (const ConvTask *)op.tasks, mm,
+                                         op.blocks)) {
+            g_net_error = "unsupported layer shape";
+            return SPH_ERR_VALUE;
+        }
+    }
+    return SPH_OK;
+}
+
 extern "C" {
 
 // Forward pass: input[V0][3] FP32 -> output[V0] FP32.  params: FP32 device

@@ -5,6 +5,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "multi.h"
+
 __device__ __forceinline__ float bf2f(uint16_t b)
 {
     return __uint_as_float((uint32_t)b << 16);
@@ -56,3 +58,33 @@ void launch_pack_weights(hipStream_t s, const float *w, int cin, int cout, uint1
 void launch_unpack_wgrad(hipStream_t s, const float *gw, int cin, int cin_pad, int cout, int cout_pad,
                          float *g);
 void launch_fill_f32(hipStream_t s, float *p, size_t n, float v);
+
+// Multi-box launches of the forward pass (ensembles): one task record per
+// box and layer, the arguments of launch_conv_fwd / launch_pool_fwd.  The
+// kernel variant a single-box launch picks depends on the grid and the
+// channels (conv_variant); a multi-box launch runs one variant for all its
+// boxes, so that every box gets the same sums as in a launch of its own.
+struct ConvTask {
+    NetGeom g;
+    const uint16_t *in;
+    const uint16_t *w;
+    const float *bias;
+    void *out;
+    const float *proj;
+    float *out2;
+    const uint16_t *skip;
+    uint16_t *yfine;
+    int cin, cout_real, mode;
+};
+struct PoolTask {
+    NetGeom gc;
+    const uint16_t *in;
+    int c;
+    uint16_t *out;
+};
+int conv_variant(const NetGeom &g, int cin, int cout_pad);
+uint32_t conv_variant_blocks(int variant, const NetGeom &g);
+int launch_conv_fwd_multi(hipStream_t s, int variant, int cin, int cout_pad, const ConvTask *t, MultiMap m,
+                          uint32_t blocks);
+uint32_t pool_blocks(const NetGeom &gc, int c);
+void launch_pool_fwd_multi(hipStream_t s, const PoolTask *t, MultiMap m, uint32_t blocks);

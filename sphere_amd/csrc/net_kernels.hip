@@ -153,16 +153,17 @@ __device__ __forceinline__ void conv_store(const f32x4 (&acc)[MT][NT], const uin
 
 
 template <int NT, int MT, int KS, bool CIN4>
-__global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd(NetGeom g, const uint16_t *__restrict__ in, int cin,
-                                                  const uint16_t *__restrict__ w,
-                                                  const float *__restrict__ bias, int cout_real,
-                                                  void *__restrict__ out, int mode,
-                                                  const float *__restrict__ proj, float *__restrict__ out2,
-                                                  const uint16_t *__restrict__ skip, uint16_t *__restrict__ yfine)
+__device__ __forceinline__ void conv_fwd_body(const NetGeom &g, const uint16_t *__restrict__ in, int cin,
+                                              const uint16_t *__restrict__ w,
+                                              const float *__restrict__ bias, int cout_real,
+                                              void *__restrict__ out, int mode,
+                                              const float *__restrict__ proj, float *__restrict__ out2,
+                                              const uint16_t *__restrict__ skip,
+                                              uint16_t *__restrict__ yfine, uint32_t bid)
 {
     __shared__ float red[KS > 1 ? (KS - 1) * MT * NT * 4 * 64 : 1];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    const uint32_t tile = KS == 1 ? blockIdx.x * 4u + wv : blockIdx.x;
+    const uint32_t tile = KS == 1 ? bid * 4u + wv : bid;
     const int kpart = KS == 1 ? 0 : (int)wv;
     const uint32_t V = g.nx * g.ny * g.nz;
     const uint32_t v0 = tile * 16u * MT;
@@ -341,6 +342,18 @@ __global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd(NetGeom g, 
     conv_store<NT, MT>(acc, vbase, V, m, grp, bias, cout_real, out, mode, proj, out2, g, skip, yfine);
 }
 
+template <int NT, int MT, int KS, bool CIN4>
+__global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd(NetGeom g, const uint16_t *__restrict__ in, int cin,
+                                                  const uint16_t *__restrict__ w,
+                                                  const float *__restrict__ bias, int cout_real,
+                                                  void *__restrict__ out, int mode,
+                                                  const float *__restrict__ proj, float *__restrict__ out2,
+                                                  const uint16_t *__restrict__ skip, uint16_t *__restrict__ yfine)
+{
+    conv_fwd_body<NT, MT, KS, CIN4>(g, in, cin, w, bias, cout_real, out, mode, proj, out2, skip, yfine,
+                                    blockIdx.x);
+}
+
 // Full-resolution convolution with its operands staged in LDS: a block
 // computes a 16 x 4 x 4 voxel tile (wave w: y = w, M tiles z = 0..3) by 16
 // output channels; the input tile with its one-voxel halo (18 x 6 x 6
@@ -348,12 +361,13 @@ __global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd(NetGeom g, 
 // read from HBM/L2 about once instead of 27 times.  Requires nx % 16 == 0,
 // ny % 4 == 0, nz % 4 == 0 and 16 output channels; cin 4 or 16.
 template <bool CIN4>
-__global__ void __launch_bounds__(256) k_conv_fwd_lds(NetGeom g, const uint16_t *__restrict__ in, int cin,
-                                                      const uint16_t *__restrict__ w,
-                                                      const float *__restrict__ bias, int cout_real,
-                                                      void *__restrict__ out, int mode,
-                                                      const float *__restrict__ proj, float *__restrict__ out2,
-                                                      const uint16_t *__restrict__ skip, uint16_t *__restrict__ yfine)
+__device__ __forceinline__ void conv_fwd_lds_body(const NetGeom &g, const uint16_t *__restrict__ in, int cin,
+                                                  const uint16_t *__restrict__ w,
+                                                  const float *__restrict__ bias, int cout_real,
+                                                  void *__restrict__ out, int mode,
+                                                  const float *__restrict__ proj, float *__restrict__ out2,
+                                                  const uint16_t *__restrict__ skip,
+                                                  uint16_t *__restrict__ yfine, uint32_t bid)
 {
     constexpr int CS = CIN4 ? 4 : 16;         // channels per voxel
     constexpr int HX = 18, HY = 6, HZ = 6;    // tile with halo
@@ -362,7 +376,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(NetGeom g, const uint16_t 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wv = tid >> 6;
     const uint32_t tx = g.nx / 16u, ty = g.ny / 4u;
-    const uint32_t bx = blockIdx.x % tx, by = (blockIdx.x / tx) % ty, bz = blockIdx.x / (tx * ty);
+    const uint32_t bx = bid % tx, by = (bid / tx) % ty, bz = bid / (tx * ty);
     const int x0 = (int)bx * 16, y0 = (int)by * 4, z0 = (int)bz * 4;
     // stage the input tile (zeros or wrap-around outside the grid) and the
     // weights: every load of a thread is issued before its LDS stores
@@ -455,6 +469,17 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(NetGeom g, const uint16_t 
     conv_store<1, 4>(acc, vbase, V, m, grp, bias, cout_real, out, mode, proj, out2, g, skip, yfine);
 }
 
+template <bool CIN4>
+__global__ void __launch_bounds__(256) k_conv_fwd_lds(NetGeom g, const uint16_t *__restrict__ in, int cin,
+                                                      const uint16_t *__restrict__ w,
+                                                      const float *__restrict__ bias, int cout_real,
+                                                      void *__restrict__ out, int mode,
+                                                      const float *__restrict__ proj, float *__restrict__ out2,
+                                                      const uint16_t *__restrict__ skip, uint16_t *__restrict__ yfine)
+{
+    conv_fwd_lds_body<CIN4>(g, in, cin, w, bias, cout_real, out, mode, proj, out2, skip, yfine, blockIdx.x);
+}
+
 template <int NT, bool CIN4>
 static void conv_fwd_nt(hipStream_t s, const NetGeom &g, const uint16_t *in, int cin, const uint16_t *w,
                         const float *bias, int cout_real, void *out, int mode, const float *proj,
@@ -487,6 +512,98 @@ static void conv_fwd_nt(hipStream_t s, const NetGeom &g, const uint16_t *in, int
         hipLaunchKernelGGL((k_conv_fwd<NT, 1, CONV_KS_SMALL, CIN4>), dim3(tiles), dim3(64 * CONV_KS_SMALL), 0, s,
                            g, in, cin, w, bias, cout_real, out, mode, proj, out2, skip, yfine);
     }
+}
+
+// The kernel variants of conv_fwd_nt, in its order of choice.
+#define CV_LDS 0
+#define CV_BIG 1
+#define CV_KS4 2
+#define CV_KS16 3
+#define CV_KS8 4
+
+int conv_variant(const NetGeom &g, int cin, int cout_pad)
+{
+    const uint32_t V = g.nx * g.ny * g.nz;
+    if (cout_pad == 16 && V >= 65536u && g.nx % 16u == 0 && g.ny % 4u == 0 && g.nz % 4u == 0 &&
+        (cin == 4 || cin == 16))
+        return CV_LDS;
+    if (V >= 65536u) return CV_BIG;
+    if (V > CONV_SMALL_V) return CV_KS4;
+    if (cin >= 64) return CV_KS16;
+    return CV_KS8;
+}
+
+uint32_t conv_variant_blocks(int variant, const NetGeom &g)
+{
+    const uint32_t V = g.nx * g.ny * g.nz;
+    if (variant == CV_LDS) return (g.nx / 16u) * (g.ny / 4u) * (g.nz / 4u);
+    if (variant == CV_BIG) return ((V + 63u) / 64u + 3u) / 4u;
+    return (V + 15u) / 16u;
+}
+
+template <int NT, int MT, int KS, bool CIN4>
+__global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd_multi(const ConvTask *__restrict__ t,
+                                                                         MultiMap m)
+{
+    uint32_t k, bid, nblk;
+    if (!multi_block(m, k, bid, nblk)) return;
+    const ConvTask &T = t[k];
+    conv_fwd_body<NT, MT, KS, CIN4>(T.g, T.in, T.cin, T.w, T.bias, T.cout_real, T.out, T.mode, T.proj, T.out2,
+                                    T.skip, T.yfine, bid);
+}
+
+template <bool CIN4>
+__global__ void __launch_bounds__(256) k_conv_fwd_lds_multi(const ConvTask *__restrict__ t, MultiMap m)
+{
+    uint32_t k, bid, nblk;
+    if (!multi_block(m, k, bid, nblk)) return;
+    const ConvTask &T = t[k];
+    conv_fwd_lds_body<CIN4>(T.g, T.in, T.cin, T.w, T.bias, T.cout_real, T.out, T.mode, T.proj, T.out2,
+                            T.skip, T.yfine, bid);
+}
+
+template <int NT, bool CIN4>
+static void conv_multi_nt(hipStream_t s, int variant, const ConvTask *t, MultiMap m, uint32_t blocks)
+{
+    switch (variant) {
+    case CV_LDS:
+        if (NT == 1)
+            hipLaunchKernelGGL((k_conv_fwd_lds_multi<CIN4>), dim3(blocks), dim3(256), 0, s, t, m);
+        break;
+    case CV_BIG:
+        hipLaunchKernelGGL((k_conv_fwd_multi<NT, 4, 1, CIN4>), dim3(blocks), dim3(256), 0, s, t, m);
+        break;
+    case CV_KS4:
+        hipLaunchKernelGGL((k_conv_fwd_multi<NT, 1, 4, CIN4>), dim3(blocks), dim3(256), 0, s, t, m);
+        break;
+    case CV_KS16:
+        hipLaunchKernelGGL((k_conv_fwd_multi<NT, 1, 16, CIN4>), dim3(blocks), dim3(1024), 0, s, t, m);
+        break;
+    default:
+        hipLaunchKernelGGL((k_conv_fwd_multi<NT, 1, CONV_KS_SMALL, CIN4>), dim3(blocks),
+                           dim3(64 * CONV_KS_SMALL), 0, s, t, m);
+    }
+}
+
+int launch_conv_fwd_multi(hipStream_t s, int variant, int cin, int cout_pad, const ConvTask *t, MultiMap m,
+                          uint32_t blocks)
+{
+    if (!blocks) return 0;
+    const bool c4 = cin == 4;
+    if (!c4 && (cin % 16) != 0) return -1;
+    if (variant == CV_LDS && cout_pad != 16) return -1;
+#define CONV_NT_MULTI(NTV)                                                      \
+    (c4 ? conv_multi_nt<NTV, true>(s, variant, t, m, blocks)                    \
+        : conv_multi_nt<NTV, false>(s, variant, t, m, blocks))
+    switch (cout_pad) {
+    case 16: CONV_NT_MULTI(1); break;
+    case 32: CONV_NT_MULTI(2); break;
+    case 48: CONV_NT_MULTI(3); break;
+    case 64: CONV_NT_MULTI(4); break;
+    default: return -1;
+    }
+#undef CONV_NT_MULTI
+    return 0;
 }
 
 int launch_conv_fwd(hipStream_t s, const NetGeom &g, const uint16_t *in, int cin, const uint16_t *w,
@@ -677,10 +794,10 @@ void launch_bias_grad(hipStream_t s, uint32_t V, const uint16_t *dz, int c_strid
 // ---------------------------------------------------------------------------
 // Elementwise kernels; fine grid g, coarse grid g/2 (even sizes).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_pool_fwd(NetGeom gc, const uint16_t *__restrict__ in, int c,
-                                                  uint16_t *__restrict__ out)
+__device__ __forceinline__ void pool_fwd_body(const NetGeom &gc, const uint16_t *__restrict__ in, int c,
+                                              uint16_t *__restrict__ out, uint32_t bid)
 {
-    const uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t idx = bid * 256u + threadIdx.x;
     const uint32_t Vc = gc.nx * gc.ny * gc.nz;
     if (idx >= Vc * (uint32_t)c) return;
     const uint32_t vc = idx / (uint32_t)c, ch = idx % (uint32_t)c;
@@ -693,6 +810,27 @@ __global__ void __launch_bounds__(256) k_pool_fwd(NetGeom gc, const uint16_t *__
         sum += bf2f(in[((size_t)(fz * fny + fy) * fnx + fx) * c + ch]);
     }
     out[idx] = f2bf(0.125f * sum);
+}
+
+__global__ void __launch_bounds__(256) k_pool_fwd(NetGeom gc, const uint16_t *__restrict__ in, int c,
+                                                  uint16_t *__restrict__ out)
+{
+    pool_fwd_body(gc, in, c, out, blockIdx.x);
+}
+
+uint32_t pool_blocks(const NetGeom &gc, int c) { return (gc.nx * gc.ny * gc.nz * (uint32_t)c + 255u) / 256u; }
+
+__global__ void __launch_bounds__(256) k_pool_fwd_multi(const PoolTask *__restrict__ t, MultiMap m)
+{
+    uint32_t k, bid, nblk;
+    if (!multi_block(m, k, bid, nblk)) return;
+    const PoolTask &T = t[k];
+    pool_fwd_body(T.gc, T.in, T.c, T.out, bid);
+}
+
+void launch_pool_fwd_multi(hipStream_t s, const PoolTask *t, MultiMap m, uint32_t blocks)
+{
+    if (blocks) hipLaunchKernelGGL(k_pool_fwd_multi, dim3(blocks), dim3(256), 0, s, t, m);
 }
 
 void launch_pool_fwd(hipStream_t s, const NetGeom &gc, const uint16_t *in, int c, uint16_t *out)

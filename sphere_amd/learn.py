@@ -68,13 +68,43 @@ def correct(engine, net, params_t, p, inp, out):
     engine.darcy_sweeps(p, 0)  # ghost layer of the corrected iterate
 
 
-def collect(engine, steps, sweeps=SWEEPS, log=None, net=None, params=None, corrections=8):
+class PairStore:
+    """Training pairs in preallocated device tensors: inputs [N, V, 3] in
+    BF16 (the network rounds its input to BF16 as its first operation, so
+    this is what it sees) and targets [N, V] in FP32."""
+
+    def __init__(self, capacity, V):
+        import torch
+        dev = torch.device("cuda:0")
+        self.X = torch.empty((capacity, V, 3), dtype=torch.bfloat16, device=dev)
+        self.Y = torch.empty((capacity, V), dtype=torch.float32, device=dev)
+        self.n = 0
+
+    def full(self):
+        return self.n >= self.X.shape[0]
+
+    def add(self, inp, target):
+        if self.full():
+            return False
+        self.X[self.n].copy_(inp)
+        self.Y[self.n].copy_(target)
+        self.n += 1
+        return True
+
+    def view(self):
+        return self.X[:self.n], self.Y[:self.n]
+
+
+def collect(engine, steps, sweeps=SWEEPS, log=None, net=None, params=None, corrections=8,
+            store=None):
     """Run `steps` fluid steps of the engine's loaded state; after each,
     make training pairs for the iterates after each count in `sweeps` of
     plain sweeps from the solve's start.  With a network and its parameters,
     also for the iterates the corrected solve meets: 2 sweeps, then
     (correction, 10 sweeps) repeated, a pair before each correction.
-    Returns (inputs [S, V, 3], targets [S, V]) as float32 CUDA tensors."""
+    Returns (inputs [S, V, 3], targets [S, V]) as float32 CUDA tensors, or,
+    with a PairStore, adds the pairs to it (until it is full) and returns
+    it."""
     import torch
     nx, ny, nz = grid_of(engine)
     V = nx * ny * nz
@@ -85,6 +115,8 @@ def collect(engine, steps, sweeps=SWEEPS, log=None, net=None, params=None, corre
     inputs, targets = [], []
     order = sorted(sweeps)
     for k in range(steps):
+        if store is not None and store.full():
+            break
         engine.step(1)
         st = engine.darcy_stats()
         engine.darcy_export(0, p0)
@@ -98,6 +130,9 @@ def collect(engine, steps, sweeps=SWEEPS, log=None, net=None, params=None, corre
             if not (0.0 < s < 1e300):
                 return
             pi = p.view(nz + 2, ny + 2, nx + 2)[1:-1, 1:-1, 1:-1].reshape(-1)
+            if store is not None:
+                store.add(inp, ((pfi - pi) / s).to(torch.float32))
+                return
             inputs.append(inp)
             targets.append(((pfi - pi) / s).to(torch.float32))
 
@@ -120,7 +155,10 @@ def collect(engine, steps, sweeps=SWEEPS, log=None, net=None, params=None, corre
                 correct(engine, net, pt, pc, inp, out)
                 engine.darcy_sweeps(pc, 10)
         if log and (k + 1) % 50 == 0:
-            log("collected %d pairs after %d fluid steps" % (len(inputs), k + 1))
+            log("collected %d pairs after %d fluid steps" %
+                (store.n if store is not None else len(inputs), k + 1))
+    if store is not None:
+        return store
     return torch.stack(inputs), torch.stack(targets)
 
 
@@ -140,7 +178,7 @@ def train(net, inputs, targets, iters=2000, lr=1e-3, seed=0, params=None, log=No
     for it in range(iters):
         i = int(torch.randint(inputs.shape[0], (1,), generator=gen))
         with torch.no_grad():
-            net.forward(p, inputs[i], out)
+            net.forward(p, inputs[i].float(), out)
             diff = out - targets[i]
             loss = torch.mean(diff * diff)
             gout = (2.0 / V) * diff
@@ -166,7 +204,7 @@ def evaluate(net, params, inputs, targets):
     mse, rel = [], []
     with torch.no_grad():
         for i in range(inputs.shape[0]):
-            net.forward(p, inputs[i], out)
+            net.forward(p, inputs[i].float(), out)
             d = out - targets[i]
             mse.append(float(torch.mean(d * d)))
             rel.append(float(torch.linalg.norm(d) / torch.linalg.norm(targets[i]).clamp_min(1e-30)))

@@ -427,7 +427,24 @@ def ensemble_boxes(count=64, seed=5, n_min=5000, n_max=60000, steps=300, z_targe
         s.gamma_n[0] = 5e3
         r_final = s.radius.copy()
         boxes.append({"s": s, "f": f0, "r": r_final, "z": 0.0, "done": False, "rng": brng})
-    engines = [Engine(device, nc=64) for _ in range(count)]
+    _grow_together(boxes, steps, z_target, relax_steps, device, log)
+    for k, bx in enumerate(boxes):
+        s = bx["s"]
+        shear_box_finish(s, bx["rng"], contacts=None, bond_fraction=0.0, sigma0=float(sig[k]),
+                         v_top=float(vel[k]))
+        s.mu_s[0] = mus[k]
+        s.mu_d[0] = mus[k]
+    params = [{"grains": int(sizes[k]), "mu": float(mus[k]), "sigma0": float(sig[k]),
+               "v_top": float(vel[k])} for k in range(count)]
+    return [bx["s"] for bx in boxes], params
+
+
+def _grow_together(boxes, steps, z_target, relax_steps, device=0, log=None):
+    """Radius growth to jamming (each box from its own lattice and scale f),
+    then relaxation under gravity, of all boxes stepped as one Ensemble per
+    stage; each box's positions end in its sim, at rest."""
+    from ._abi import Engine, Ensemble
+    engines = [Engine(device, nc=64) for _ in boxes]
     ens = Ensemble(engines)
     for stage in range(400):
         active = [k for k, bx in enumerate(boxes) if not bx["done"]]
@@ -475,15 +492,53 @@ def ensemble_boxes(count=64, seed=5, n_min=5000, n_max=60000, steps=300, z_targe
         s.angvel[:] = 0.0
         s.force[:] = 0.0
         s.torque[:] = 0.0
-        shear_box_finish(s, bx["rng"], contacts=None, bond_fraction=0.0, sigma0=float(sig[k]),
-                         v_top=float(vel[k]))
-        s.mu_s[0] = mus[k]
-        s.mu_d[0] = mus[k]
     ens.close()
     for e in engines:
         e.close()
+
+
+def darcy_cells_xy(n):
+    """Horizontal Darcy cells of a consolidation-type box of n grains: the
+    smallest multiple of 8 (the correction network halves the grid three
+    times), at least 16, that leaves the settled packing below about 60% of
+    the c x c x 2c grid's height (packing height / grid height is about
+    0.283 n / c^3 for cells of 2.1 r_max at a packing fraction of 0.6)."""
+    c = int(numpy.ceil((0.283 * n / 0.6) ** (1.0 / 3.0) / 8.0)) * 8
+    return max(16, c)
+
+
+def darcy_ensemble_boxes(count=64, seed=7, n_min=5000, n_max=60000, steps=300, z_target=4.2,
+                         relax_steps=400, device=0, log=None):
+    """Seeded Darcy-coupled shear boxes for the ensemble benchmark.  Box b:
+    N_b log-uniform in [n_min, n_max] grains of the consolidation box (radii
+    U[0.009, 0.011] m, linear contacts, the same fluid), on a c x c x 2c Darcy
+    grid (the DEM grid) with c from darcy_cells_xy; friction mu ~ U[0.3,
+    0.7], top-wall stress sigma0 ~ U[5, 80] kPa, and the grains of the top
+    layer (within one largest diameter of the highest grain) driven at
+    v ~ U[1e-3, 1e-1] m/s along x.  Grown to jamming and relaxed together
+    through one Ensemble per stage.  Returns the sims and their parameters."""
+    rng = numpy.random.default_rng(seed)
+    sizes = numpy.exp(rng.uniform(numpy.log(n_min), numpy.log(n_max), count)).astype(int)
+    mus = rng.uniform(0.3, 0.7, count)
+    sig = rng.uniform(5e3, 80e3, count)
+    vel = rng.uniform(1e-3, 1e-1, count)
+    boxes, cells = [], []
+    for b in range(count):
+        c = darcy_cells_xy(int(sizes[b]))
+        s, f0, brng = consolidation_base(n=int(sizes[b]), seed=int(seed * 1000 + b), cells_xy=c)
+        boxes.append({"s": s, "f": f0, "r": s.radius.copy(), "z": 0.0, "done": False, "rng": brng})
+        cells.append((c, c, 2 * c))
+    _grow_together(boxes, steps, z_target, relax_steps, device, log)
+    for k, bx in enumerate(boxes):
+        s = bx["s"]
+        consolidation_finish(s, sigma0=float(sig[k]), cells=cells[k])
+        s.mu_s[0] = mus[k]
+        s.mu_d[0] = mus[k]
+        top = s.x[:, 2] > float(numpy.max(s.x[:, 2])) - 2.0 * float(numpy.max(s.radius))
+        s.fixvel[top] = 1.0
+        s.vel[top, 0] = float(vel[k])
     params = [{"grains": int(sizes[k]), "mu": float(mus[k]), "sigma0": float(sig[k]),
-               "v_top": float(vel[k])} for k in range(count)]
+               "v_top": float(vel[k]), "grid": list(cells[k])} for k in range(count)]
     return [bx["s"] for bx in boxes], params
 
 

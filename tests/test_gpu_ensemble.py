@@ -155,11 +155,61 @@ def test_ensemble_autotune_changes_no_result():
     ens.close()
 
 
-def test_ensemble_refuses_fluid_boxes():
+FLUID = [("p_f", 1), ("v_f", 3), ("phi", 1), ("dphi", 1), ("f_p", 3)]
+
+
+def test_ensemble_darcy_boxes_match_single_boxes():
+    """Darcy-coupled boxes of different sizes, boundaries and walls stepped
+    as one ensemble (one launch per stage of the fluid step, the sweeps of
+    all solves batched) give the single-box states bit for bit, fluid
+    fields and sweep counts included; one box has a learned network (its
+    forward passes as multi-box launches) and a DEM box rides along."""
     from _cases import darcy_case
-    e = sphere_amd.Engine(0)
-    e.load(darcy_case(n=50, nxyz=(4, 4, 8)).to_state())
-    ens = sphere_amd.Ensemble([e])
-    with pytest.raises(sphere_amd.SphereError):
-        ens.step(4)
+    from sphere_amd import learn
+    boxes = [darcy_case(n=300, seed=1, nxyz=(6, 6, 12)),
+             darcy_case(n=500, seed=2, nxyz=(8, 8, 16), wall=True),
+             darcy_case(n=200, seed=3, nxyz=(6, 6, 12), bc_bot=0, bc_top=1),
+             darcy_case(n=400, seed=4, nxyz=(8, 6, 12), periodic=2),
+             darcy_case(n=250, seed=5, nxyz=(6, 6, 12), bc_top=4, tolerance=1e-7),
+             dense_case(n=343, seed=6, model=3, periodic=1, nw=1, E=7e9),
+             darcy_case(n=300, seed=7, nxyz=(8, 8, 16))]
+    boxes[4].bc_top_flux[0] = 1e-6
+    for s in boxes:
+        s.time_file_dt[0] = 1.0
+        s.time_total[0] = 1.0
+    params = learn.init_params(3)
+
+    def make():
+        out = []
+        for k, s in enumerate(boxes):
+            e = sphere_amd.Engine(0, nc=64)
+            if k == 6:
+                e.set_darcy_net(params, 4)
+            e.load(s.to_state())
+            out.append(e)
+        return out
+
+    together, alone = make(), make()
+    ens = sphere_amd.Ensemble(together)
+    for n in (5, 4, 3):
+        ens.step(n)
+        for e in alone:
+            e.step(n)
+    info = ens.info()
+    print("ensemble info", info)
+    assert info["fluid_boxes"] == 6 and info["multi_boxes"] == 1 and info["fluid_steps"] == 12
+    for k, (a, b, s) in enumerate(zip(together, alone, boxes)):
+        n = int(s.np[0])
+        ra, rb = a.read(), b.read()
+        _same(ra, rb, n, "box %d" % k)
+        assert numpy.array_equal(a.read_contacts()[0], b.read_contacts()[0]), k
+        if s.fluid:
+            V = int(numpy.prod(s.num))
+            for name, w in FLUID:
+                m = n if name == "f_p" else V
+                assert numpy.array_equal(ra.get(name, m, w), rb.get(name, m, w)), (k, name)
+            sa, sb = a.darcy_stats(), b.darcy_stats()
+            assert (sa.sweeps, sa.fluid_steps, sa.nonconverged, sa.corrections) == (
+                sb.sweeps, sb.fluid_steps, sb.nonconverged, sb.corrections), k
+            print("box %d: %d sweeps in %d fluid steps" % (k, sa.sweeps, sa.fluid_steps))
     ens.close()
