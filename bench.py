@@ -200,6 +200,7 @@ def darcy_ensemble_section(args, params):
     dx = dp = 0.0
     same_sweeps = True
     sweeps = corr = fsteps = 0
+    per_box = []
     for k, (a, b) in enumerate(zip(eng, one)):
         n = info[k]["grains"]
         V = int(numpy.prod(info[k]["grid"]))
@@ -211,14 +212,16 @@ def darcy_ensemble_section(args, params):
         sweeps += sa.sweeps
         corr += sa.corrections
         fsteps += sa.fluid_steps
+        per_box.append(sa.sweeps / max(1, sa.fluid_steps))
     for e in eng + one:
         e.close()
     v_ens = total * steps / t_ens
     v_one = total * steps / t_one
     log("Darcy ensemble: %.4g particle-steps/s together, %.4g one at a time (%.2fx); max |x| "
         "difference %g, max |p_f| difference %g, sweep counts equal: %s; %.1f sweeps and %.1f "
-        "corrections per solve" % (v_ens, v_one, v_ens / v_one, dx, dp, same_sweeps,
-                                   sweeps / max(1, fsteps), corr / max(1, fsteps)))
+        "corrections per solve (per box: min %.0f, median %.0f, max %.0f sweeps)"
+        % (v_ens, v_one, v_ens / v_one, dx, dp, same_sweeps, sweeps / max(1, fsteps),
+           corr / max(1, fsteps), numpy.min(per_box), numpy.median(per_box), numpy.max(per_box)))
     sizes = sorted(p["grains"] for p in info)
     grids = sorted(set(tuple(p["grid"]) for p in info))
     return {
@@ -229,6 +232,9 @@ def darcy_ensemble_section(args, params):
         "max_abs_pressure_difference_vs_one_at_a_time": dp,
         "sweep_counts_equal": bool(same_sweeps),
         "sweeps_per_solve": sweeps / max(1, fsteps), "corrections_per_solve": corr / max(1, fsteps),
+        "sweeps_per_solve_per_box_min_median_max": [float(numpy.min(per_box)),
+                                                     float(numpy.median(per_box)),
+                                                     float(numpy.max(per_box))],
         "learned_correction": params is not None,
         "ensemble_info": einfo,
         "config": {"workload": "%d independent Darcy-coupled shear boxes, grains log-uniform in "

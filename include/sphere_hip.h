@@ -358,10 +358,12 @@ int sph_darcy_net_input(sph_ctx *ctx, const double *p, float *input, double *sca
  * of a step is one kernel launch for all of them, with per-box task
  * records in device memory, and pairs of steps are captured in one HIP
  * graph and replayed.  Darcy-coupled boxes on the list path (one DEM step
- * per fluid step, no modulated top pressure, no learned correction, the
- * same maxiter) are stepped together too, one launch per stage of the
- * fluid step, with their Jacobi sweeps batched until every box's
- * device-side test has ended its solve.  Other boxes are stepped box by box
+ * per fluid step, no modulated top pressure, the same maxiter; boxes with
+ * a learned correction need grids in multiples of 8 cells and the same
+ * number of corrections per solve) are stepped together too, one launch
+ * per stage of the fluid step, with their Jacobi sweeps batched until every
+ * box's device-side test has ended its solve and the networks' forward
+ * passes as multi-box launches.  Other boxes are stepped box by box
  * on their own streams.  Either way each box's results equal those of
  * sph_step on that box alone, bit for bit.  An output event or the velocity
  * step of t = 10 s / 15 s inside the steps is refused (SPH_ERR_VALUE).

@@ -1942,6 +1942,11 @@ struct sph_ensemble {
     uint32_t fl_maxiter = 0;
     unsigned long long *dm_states = nullptr;    // device: 4 state words per box
     unsigned long long *dm_states_h = nullptr;  // pinned host copy
+    // the sweep launches cover only the boxes whose solve has not ended by
+    // the last host read: their block starts, counts and box indices
+    std::vector<uint32_t> fl_sweep_blocks;   // sweep blocks of each box
+    uint32_t *fl_sweep_map = nullptr;        // device, 3 words per box
+    uint32_t *fl_sweep_map_h = nullptr;      // pinned host staging
     std::vector<void *> fl_dmem;
     uint64_t fluid_steps = 0;
 };
@@ -2369,6 +2374,10 @@ static void ens_fluid_drop(sph_ensemble *e)
     e->fl_dmem.clear();
     if (e->dm_states_h) (void)hipHostFree(e->dm_states_h);
     e->dm_states_h = nullptr;
+    if (e->fl_sweep_map_h) (void)hipHostFree(e->fl_sweep_map_h);
+    e->fl_sweep_map_h = nullptr;
+    e->fl_sweep_map = nullptr;
+    e->fl_sweep_blocks.clear();
     net_multi_destroy(e->netm);
     e->netm = nullptr;
     e->net_max_corr = 0;
@@ -2541,12 +2550,19 @@ static int ens_fluid_plan(sph_ensemble *e)
         if (!e->netm) return ens_fail(e, SPH_ERR_HIP, "ensemble: the networks' launch plan failed");
     }
     e->dm_tasks = e->dm_cells.tasks;
-    void *st = nullptr;
-    if (hipMalloc(&st, 32 * e->fluid.size()) != hipSuccess ||
-        hipHostMalloc((void **)&e->dm_states_h, 32 * e->fluid.size()) != hipSuccess)
+    void *st = nullptr, *sm = nullptr;
+    if (hipMalloc(&st, 32 * e->fluid.size()) != hipSuccess)
         return ens_fail(e, SPH_ERR_HIP, "ensemble: device allocation failed");
     e->fl_dmem.push_back(st);
+    if (hipMalloc(&sm, 12 * e->fluid.size()) != hipSuccess)
+        return ens_fail(e, SPH_ERR_HIP, "ensemble: device allocation failed");
+    e->fl_dmem.push_back(sm);
+    if (hipHostMalloc((void **)&e->dm_states_h, 32 * e->fluid.size()) != hipSuccess ||
+        hipHostMalloc((void **)&e->fl_sweep_map_h, 12 * e->fluid.size()) != hipSuccess)
+        return ens_fail(e, SPH_ERR_HIP, "ensemble: host allocation failed");
     e->dm_states = (unsigned long long *)st;
+    e->fl_sweep_map = (uint32_t *)sm;
+    e->fl_sweep_blocks = n_sweep;
     e->fl_planned = true;
     return SPH_OK;
 }
@@ -2611,15 +2627,47 @@ static int ens_fluid_step(sph_ensemble *e)
     launch_dm_ghosts(s, tc, mc, bc, DM_P);
     launch_dm_prepare(s, tc, mc, bc);
     launch_dm_reset(s, dt, nb);
-    // the sweeps, in batches as in darcy_step, until every box's solve ended
+    // The sweeps, in batches, until every box's solve ended.  A batch ends
+    // with the deciding sweep after the next sweep count that a running box
+    // needed in its last solve (at least 12 sweeps in, then at least 20 more
+    // per batch), and the next batch sweeps only the boxes whose solve had
+    // not ended by then: a box with a long solve does not keep the others'
+    // blocks in every launch.  The sweeps an ended box skips would return
+    // at once, so the results are the same.
     const DarcyTask *tw = (const DarcyTask *)e->dm_sweep.tasks;
-    const MultiMap mw = stage_map(e->dm_sweep);
     const uint32_t maxiter = e->fl_maxiter;
-    unsigned int hint = 0;
-    for (int k : e->fluid) hint = std::max(hint, e->boxes[k]->jacobi_hint);
+    std::vector<unsigned long long> target(nb);
+    for (uint32_t i = 0; i < nb; ++i)
+        target[i] = (e->boxes[e->fluid[i]]->jacobi_hint + 9ull) / 10ull * 10ull + 2ull;
+    std::vector<char> running(nb, 1);
+    auto next_end = [&](unsigned long long from) {
+        unsigned long long best = ~0ull;
+        for (uint32_t i = 0; i < nb; ++i)
+            if (running[i] && target[i] >= from && target[i] < best) best = target[i];
+        return best == ~0ull ? from : best;
+    };
+    MultiMap mw{};
+    uint32_t wblocks = 0;
+    auto remap = [&]() {
+        uint32_t n = 0, at = 0;
+        for (uint32_t i = 0; i < nb; ++i) n += running[i] ? 1u : 0u;
+        uint32_t *h = e->fl_sweep_map_h;
+        for (uint32_t i = 0, k = 0; i < nb; ++i) {
+            if (!running[i]) continue;
+            h[k] = at;
+            h[n + k] = e->fl_sweep_blocks[i];
+            h[2 * n + k] = i;
+            at += (e->fl_sweep_blocks[i] + 7u) & ~7u;
+            ++k;
+        }
+        mw = MultiMap{e->fl_sweep_map, n, e->fl_sweep_map + 2 * n};
+        wblocks = at;
+        return hipMemcpyAsync(e->fl_sweep_map, h, 12 * (size_t)n, hipMemcpyHostToDevice, s);
+    };
+    HIPCHK_E(remap());
     unsigned int nijac = 0;
     bool all_ended = false;
-    unsigned long long want = std::max(12u, (hint + 9u) / 10u * 10u + 2u);
+    unsigned long long want = next_end(12);
     int corrections = 0;
     const DarcyTask *tn = (const DarcyTask *)e->dm_net.tasks;
     const MultiMap mn = stage_map(e->dm_net);
@@ -2628,7 +2676,7 @@ static int ens_fluid_step(sph_ensemble *e)
         for (; nijac < end; ++nijac) {
             const bool decides = nijac > 0 && (nijac - 1) % 10 == 0;
             const bool correct = decides && e->netm && corrections < e->net_max_corr;
-            launch_dm_sweep(s, tw, mw, e->dm_sweep.blocks, nijac, correct);
+            launch_dm_sweep(s, tw, mw, wblocks, nijac, correct);
             if (correct) {
                 launch_dm_net_input(s, tn, mn, e->dm_net.blocks, nijac);
                 if ((rc = net_multi_forward(e->netm, s)))
@@ -2642,8 +2690,15 @@ static int ens_fluid_step(sph_ensemble *e)
         HIPCHK_E(hipMemcpyAsync(e->dm_states_h, e->dm_states, 32 * nb, hipMemcpyDeviceToHost, s));
         HIPCHK_E(hipStreamSynchronize(s));
         all_ended = true;
-        for (uint32_t i = 0; i < nb; ++i) all_ended = all_ended && e->dm_states_h[4 * i] != 0ull;
-        want = (unsigned long long)nijac + 20u;
+        bool changed = false;
+        for (uint32_t i = 0; i < nb; ++i) {
+            const bool run = e->dm_states_h[4 * i] == 0ull;
+            changed = changed || (run != (running[i] != 0));
+            running[i] = run ? 1 : 0;
+            all_ended = all_ended && !run;
+        }
+        if (changed && !all_ended) HIPCHK_E(remap());
+        want = next_end((unsigned long long)nijac + 20u);
     }
     launch_dm_normalize(s, (const DarcyTask *)e->dm_all.tasks, stage_map(e->dm_all), e->dm_all.blocks);
     (void)hipEventRecord(t1, s);

@@ -3,7 +3,8 @@
 // the blocks [start[k], start[k] + count[k]).  The starts are multiples of
 // 8, so that a box's blocks meet the eight XCDs in the order of a launch of
 // its own; the blocks between one box's last block and the next start
-// return at once.
+// return at once.  With ids, entry k of the map is box ids[k] (a launch over
+// some of the boxes); without, entry k is box k.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -11,6 +12,7 @@
 struct MultiMap {
     const uint32_t *start;  // nbox block starts (ascending), then nbox counts
     uint32_t nbox;
+    const uint32_t *ids = nullptr;  // nbox box indices, or none
 };
 
 // The box of this block, the block's index within the box and the box's
@@ -25,7 +27,7 @@ __device__ __forceinline__ bool multi_block(const MultiMap &m, uint32_t &box, ui
         if (m.start[mid] <= b) lo = mid;
         else hi = mid;
     }
-    box = lo;
+    box = m.ids ? m.ids[lo] : lo;
     bid = b - m.start[lo];
     nblk = m.start[m.nbox + lo];
     return bid < nblk;

@@ -9,10 +9,10 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU and the built HIP library")
     # torch carries its own HIP runtime; the network tests use torch for
-    # device memory, so in a GPU run it is initialised before the library is
-    # loaded and both then share that runtime
+    # device memory, so in any run that may hold GPU tests it is initialised
+    # before the library is loaded and both then share that runtime
     expr = config.getoption("markexpr", "") or ""
-    if "gpu" in expr and "not gpu" not in expr:
+    if "not gpu" not in expr:
         try:
             import torch
             torch.cuda.is_available()

@@ -162,8 +162,9 @@ def test_ensemble_darcy_boxes_match_single_boxes():
     """Darcy-coupled boxes of different sizes, boundaries and walls stepped
     as one ensemble (one launch per stage of the fluid step, the sweeps of
     all solves batched) give the single-box states bit for bit, fluid
-    fields and sweep counts included; one box has a learned network (its
-    forward passes as multi-box launches) and a DEM box rides along."""
+    fields and sweep counts included; two boxes have a learned network
+    (their forward passes as multi-box launches) and a DEM box rides
+    along."""
     from _cases import darcy_case
     from sphere_amd import learn
     boxes = [darcy_case(n=300, seed=1, nxyz=(6, 6, 12)),
@@ -183,7 +184,7 @@ def test_ensemble_darcy_boxes_match_single_boxes():
         out = []
         for k, s in enumerate(boxes):
             e = sphere_amd.Engine(0, nc=64)
-            if k == 6:
+            if k in (1, 6):
                 e.set_darcy_net(params, 4)
             e.load(s.to_state())
             out.append(e)
