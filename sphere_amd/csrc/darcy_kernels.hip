@@ -734,11 +734,13 @@ __device__ __forceinline__ void darcy_sweep_body(DarcyArgs d, const double *__re
                                                      unsigned long long *__restrict__ zero_scale, uint32_t bid)
 {
     __shared__ unsigned long long wmax[SWEEP_THREADS / 64];
-    const uint32_t i = bid * blockDim.x + threadIdx.x;
+    const uint32_t n = d.nx * d.ny * d.nz;
+    // each XCD sweeps one contiguous slab of cells, so that the stencil's
+    // neighbouring rows and planes are mostly in that XCD's L2
+    const uint32_t i = xcd_tile(bid, (n + SWEEP_THREADS - 1) / SWEEP_THREADS) * SWEEP_THREADS + threadIdx.x;
     if (zero_scale && i == 0) *zero_scale = 0ull;  // the scale slot of the correction that follows
     if (state[0] != 0ull) return;
     if (prev && darcy_decide(prev, tolerance, prev_nijac, state, i == 0)) return;
-    const uint32_t n = d.nx * d.ny * d.nz;
     unsigned long long bits = 0;
     if (clear && i == 0) *clear = 0ull;
     if (i < n) {
@@ -814,8 +816,10 @@ void launch_darcy_sweep(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b
                         unsigned long long *clear, const unsigned long long *prev,
                         double tolerance, uint32_t prev_nijac, bool free_running, bool zero_scale)
 {
-    // 512-thread blocks: 6.7 us per sweep on the 48 x 48 x 96 grid, against
-    // 7.2 with 256 and 6.8 with 1024
+    // 512-thread blocks, each XCD on one slab: 5.7-5.9 us per sweep on the
+    // 48 x 48 x 96 grid (6.7 with the blocks dealt round-robin), against
+    // 6.2 with 256 threads, 5.8 with 1024, and 6.5-8.7 with 2 or 4 cells per
+    // thread
     const uint32_t n = d.nx * d.ny * d.nz;
     hipLaunchKernelGGL(k_darcy_sweep, dim3((n + SWEEP_THREADS - 1) / SWEEP_THREADS), dim3(SWEEP_THREADS), 0, s,
                        d, b.jc, b.jfrozen, p_in,
