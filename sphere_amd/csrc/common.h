@@ -153,15 +153,6 @@ __device__ __forceinline__ d3 cross3(d3 a, d3 b)
 __device__ __forceinline__ d3 xyz(double4 v) { return mk3(v.x, v.y, v.z); }
 __device__ __forceinline__ d3 xyz(double3 v) { return mk3(v.x, v.y, v.z); }
 
-// Tile index of block b of nb such that each of the 8 XCDs, which receive
-// the blocks of a launch round-robin (b mod 8), works through one contiguous
-// range of tiles: neighbouring particles then share that XCD's L2.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb)
-{
-    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u;
-    return x * q + (x < r ? x : r) + (b >> 3);
-}
-
 // Reference constants
 #define PI_REF 3.14159265358979          // src/constants.h:15
 #define PI_M 3.14159265358979323846      // M_PI, src/darcy.cuh:317

@@ -9,6 +9,16 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Tile index of block b of nb such that each of the 8 XCDs, which receive
+// the blocks of a launch round-robin (b mod 8), works through one contiguous
+// range of tiles: neighbouring particles, cells or voxels then share that
+// XCD's L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb)
+{
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 struct MultiMap {
     const uint32_t *start;  // nbox block starts (ascending), then nbox counts
     uint32_t nbox;

@@ -159,11 +159,14 @@ __device__ __forceinline__ void conv_fwd_body(const NetGeom &g, const uint16_t *
                                               void *__restrict__ out, int mode,
                                               const float *__restrict__ proj, float *__restrict__ out2,
                                               const uint16_t *__restrict__ skip,
-                                              uint16_t *__restrict__ yfine, uint32_t bid)
+                                              uint16_t *__restrict__ yfine, uint32_t bid, uint32_t nblk)
 {
     __shared__ float red[KS > 1 ? (KS - 1) * MT * NT * 4 * 64 : 1];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    const uint32_t tile = KS == 1 ? bid * 4u + wv : bid;
+    // each XCD works through one contiguous range of tiles, whose stencil
+    // reads then mostly hit that XCD's L2
+    const uint32_t b = xcd_tile(bid, nblk);
+    const uint32_t tile = KS == 1 ? b * 4u + wv : b;
     const int kpart = KS == 1 ? 0 : (int)wv;
     const uint32_t V = g.nx * g.ny * g.nz;
     const uint32_t v0 = tile * 16u * MT;
@@ -351,7 +354,7 @@ __global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd(NetGeom g, 
                                                   const uint16_t *__restrict__ skip, uint16_t *__restrict__ yfine)
 {
     conv_fwd_body<NT, MT, KS, CIN4>(g, in, cin, w, bias, cout_real, out, mode, proj, out2, skip, yfine,
-                                    blockIdx.x);
+                                    blockIdx.x, gridDim.x);
 }
 
 // Full-resolution convolution with its operands staged in LDS: a block
@@ -367,7 +370,7 @@ __device__ __forceinline__ void conv_fwd_lds_body(const NetGeom &g, const uint16
                                                   void *__restrict__ out, int mode,
                                                   const float *__restrict__ proj, float *__restrict__ out2,
                                                   const uint16_t *__restrict__ skip,
-                                                  uint16_t *__restrict__ yfine, uint32_t bid)
+                                                  uint16_t *__restrict__ yfine, uint32_t bid, uint32_t nblk)
 {
     constexpr int CS = CIN4 ? 4 : 16;         // channels per voxel
     constexpr int HX = 18, HY = 6, HZ = 6;    // tile with halo
@@ -376,7 +379,8 @@ __device__ __forceinline__ void conv_fwd_lds_body(const NetGeom &g, const uint16
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wv = tid >> 6;
     const uint32_t tx = g.nx / 16u, ty = g.ny / 4u;
-    const uint32_t bx = bid % tx, by = (bid / tx) % ty, bz = bid / (tx * ty);
+    const uint32_t t = xcd_tile(bid, nblk);  // one contiguous slab of tiles per XCD
+    const uint32_t bx = t % tx, by = (t / tx) % ty, bz = t / (tx * ty);
     const int x0 = (int)bx * 16, y0 = (int)by * 4, z0 = (int)bz * 4;
     // stage the input tile (zeros or wrap-around outside the grid) and the
     // weights: every load of a thread is issued before its LDS stores
@@ -477,7 +481,8 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds(NetGeom g, const uint16_t 
                                                       const float *__restrict__ proj, float *__restrict__ out2,
                                                       const uint16_t *__restrict__ skip, uint16_t *__restrict__ yfine)
 {
-    conv_fwd_lds_body<CIN4>(g, in, cin, w, bias, cout_real, out, mode, proj, out2, skip, yfine, blockIdx.x);
+    conv_fwd_lds_body<CIN4>(g, in, cin, w, bias, cout_real, out, mode, proj, out2, skip, yfine, blockIdx.x,
+                            gridDim.x);
 }
 
 template <int NT, bool CIN4>
@@ -549,7 +554,7 @@ __global__ void __launch_bounds__(KS > 1 ? 64 * KS : 256) k_conv_fwd_multi(const
     if (!multi_block(m, k, bid, nblk)) return;
     const ConvTask &T = t[k];
     conv_fwd_body<NT, MT, KS, CIN4>(T.g, T.in, T.cin, T.w, T.bias, T.cout_real, T.out, T.mode, T.proj, T.out2,
-                                    T.skip, T.yfine, bid);
+                                    T.skip, T.yfine, bid, nblk);
 }
 
 template <bool CIN4>
@@ -559,7 +564,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd_lds_multi(const ConvTask *__re
     if (!multi_block(m, k, bid, nblk)) return;
     const ConvTask &T = t[k];
     conv_fwd_lds_body<CIN4>(T.g, T.in, T.cin, T.w, T.bias, T.cout_real, T.out, T.mode, T.proj, T.out2,
-                            T.skip, T.yfine, bid);
+                            T.skip, T.yfine, bid, nblk);
 }
 
 template <int NT, bool CIN4>
@@ -795,9 +800,9 @@ void launch_bias_grad(hipStream_t s, uint32_t V, const uint16_t *dz, int c_strid
 // Elementwise kernels; fine grid g, coarse grid g/2 (even sizes).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void pool_fwd_body(const NetGeom &gc, const uint16_t *__restrict__ in, int c,
-                                              uint16_t *__restrict__ out, uint32_t bid)
+                                              uint16_t *__restrict__ out, uint32_t bid, uint32_t nblk)
 {
-    const uint32_t idx = bid * 256u + threadIdx.x;
+    const uint32_t idx = xcd_tile(bid, nblk) * 256u + threadIdx.x;
     const uint32_t Vc = gc.nx * gc.ny * gc.nz;
     if (idx >= Vc * (uint32_t)c) return;
     const uint32_t vc = idx / (uint32_t)c, ch = idx % (uint32_t)c;
@@ -815,7 +820,7 @@ __device__ __forceinline__ void pool_fwd_body(const NetGeom &gc, const uint16_t 
 __global__ void __launch_bounds__(256) k_pool_fwd(NetGeom gc, const uint16_t *__restrict__ in, int c,
                                                   uint16_t *__restrict__ out)
 {
-    pool_fwd_body(gc, in, c, out, blockIdx.x);
+    pool_fwd_body(gc, in, c, out, blockIdx.x, gridDim.x);
 }
 
 uint32_t pool_blocks(const NetGeom &gc, int c) { return (gc.nx * gc.ny * gc.nz * (uint32_t)c + 255u) / 256u; }
@@ -825,7 +830,7 @@ __global__ void __launch_bounds__(256) k_pool_fwd_multi(const PoolTask *__restri
     uint32_t k, bid, nblk;
     if (!multi_block(m, k, bid, nblk)) return;
     const PoolTask &T = t[k];
-    pool_fwd_body(T.gc, T.in, T.c, T.out, bid);
+    pool_fwd_body(T.gc, T.in, T.c, T.out, bid, nblk);
 }
 
 void launch_pool_fwd_multi(hipStream_t s, const PoolTask *t, MultiMap m, uint32_t blocks)
