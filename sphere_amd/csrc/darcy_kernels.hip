@@ -649,11 +649,37 @@ void launch_darcy_prepare(hipStream_t s, const DarcyArgs &d, const DarcyBuffers 
 // ends the solve with an error, max <= tolerance ends it converged.
 // state: [0] 0 running, 1 converged, 2 NaN; [1] nijac of the last check;
 // [2] bit pattern of its max |norm|.  Returns whether the solve ends.
-__device__ __forceinline__ bool darcy_decide(const unsigned long long *__restrict__ slot,
-                                             double tolerance, uint32_t nijac,
-                                             unsigned long long *__restrict__ state, bool writer)
+// The max of a reduction slot's RED_LANES words.  Every lane of the wave
+// calls red_max_wave (one word per lane, RED_LANES being the wave size).
+__device__ __forceinline__ unsigned long long red_max_wave(const unsigned long long *__restrict__ slot)
 {
-    const unsigned long long bits = *slot;
+    unsigned long long v = slot[(threadIdx.x & 63u) * RED_STRIDE];
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long red_max_serial(const unsigned long long *__restrict__ slot)
+{
+    unsigned long long v = 0ull;
+    for (int j = 0; j < RED_LANES; ++j) {
+        const unsigned long long o = slot[j * RED_STRIDE];
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// block bid's share of a launch-wide max
+__device__ __forceinline__ void red_fold(unsigned long long *__restrict__ slot, uint32_t bid, unsigned long long m)
+{
+    atomicMax(slot + (bid % RED_LANES) * RED_STRIDE, m);
+}
+
+__device__ __forceinline__ bool darcy_decide_bits(unsigned long long bits, double tolerance, uint32_t nijac,
+                                                  unsigned long long *__restrict__ state, bool writer)
+{
     const double m = __longlong_as_double((long long)bits);
     const bool nan = m != m;
     const bool stop = nan || m <= tolerance;
@@ -663,6 +689,13 @@ __device__ __forceinline__ bool darcy_decide(const unsigned long long *__restric
         if (stop) state[0] = nan ? 2ull : 1ull;
     }
     return stop;
+}
+
+// the test of a check sweep from its slot; every lane of the wave calls it
+__device__ __forceinline__ bool darcy_decide(const unsigned long long *__restrict__ slot, double tolerance,
+                                             uint32_t nijac, unsigned long long *__restrict__ state, bool writer)
+{
+    return darcy_decide_bits(red_max_wave(slot), tolerance, nijac, state, writer);
 }
 
 // p* of updateDarcySolution (src/darcy.cuh:1646-1739) for interior cell i
@@ -738,11 +771,12 @@ __device__ __forceinline__ void darcy_sweep_body(DarcyArgs d, const double *__re
     // each XCD sweeps one contiguous slab of cells, so that the stencil's
     // neighbouring rows and planes are mostly in that XCD's L2
     const uint32_t i = xcd_tile(bid, (n + SWEEP_THREADS - 1) / SWEEP_THREADS) * SWEEP_THREADS + threadIdx.x;
-    if (zero_scale && i == 0) *zero_scale = 0ull;  // the scale slot of the correction that follows
+    // the scale slot of the correction that follows
+    if (zero_scale && i < RED_LANES) zero_scale[i * RED_STRIDE] = 0ull;
     if (state[0] != 0ull) return;
     if (prev && darcy_decide(prev, tolerance, prev_nijac, state, i == 0)) return;
     unsigned long long bits = 0;
-    if (clear && i == 0) *clear = 0ull;
+    if (clear && i < RED_LANES) clear[i * RED_STRIDE] = 0ull;
     if (i < n) {
         const int nx = (int)d.nx, ny = (int)d.ny, nz = (int)d.nz;
         const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / (d.nx * d.ny));
@@ -791,7 +825,7 @@ __device__ __forceinline__ void darcy_sweep_body(DarcyArgs d, const double *__re
         if (threadIdx.x == 0) {
             unsigned long long m = wmax[0];
             for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = wmax[w] > m ? wmax[w] : m;
-            atomicMax(check, m);
+            red_fold(check, bid, m);
         }
     }
 }
@@ -832,8 +866,8 @@ void launch_darcy_sweep(hipStream_t s, const DarcyArgs &d, const DarcyBuffers &b
 __global__ void k_darcy_decide(const unsigned long long *__restrict__ slot, double tolerance,
                                uint32_t nijac, unsigned long long *__restrict__ state)
 {
-    if (threadIdx.x != 0 || state[0] != 0ull) return;
-    (void)darcy_decide(slot, tolerance, nijac, state, true);
+    if (state[0] != 0ull) return;
+    (void)darcy_decide(slot, tolerance, nijac, state, threadIdx.x == 0);
 }
 
 void launch_darcy_decide(hipStream_t s, const DarcyBuffers &b, const unsigned long long *slot,
@@ -879,7 +913,7 @@ __device__ __forceinline__ void darcy_residual_max_body(DarcyArgs d, const doubl
     if (threadIdx.x == 0) {
         unsigned long long m = wmax[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = wmax[w] > m ? wmax[w] : m;
-        atomicMax(smax, m);
+        red_fold(smax, bid, m);
     }
 }
 
@@ -908,13 +942,14 @@ __device__ __forceinline__ void darcy_net_input_body(DarcyArgs d, const double *
                                                          uint16_t *__restrict__ in4, uint32_t bid)
 {
     if (state && state[0] != 0ull) return;
+    const unsigned long long sbits = red_max_wave(smax);
     const uint32_t n = d.nx * d.ny * d.nz;
     const uint32_t i = bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int x = (int)(i % d.nx), y = (int)((i / d.nx) % d.ny), z = (int)(i / (d.nx * d.ny));
     const uint32_t c = didx(d, x, y, z);
     double s;
-    const bool ok = usable_scale(*smax, s);
+    const bool ok = usable_scale(sbits, s);
     const double r = jacobi_target(d, jc, frozen, p, i, x, y, z, c) - p[c];
     const JLayout j = jlayout(d);
     const size_t fx = (size_t)x + j.sxf * ((size_t)y + d.ny * (size_t)z);
@@ -960,7 +995,7 @@ __device__ __forceinline__ void darcy_net_apply_body(DarcyArgs d, const uint8_t 
 {
     if (state[0] != 0ull) return;
     double s;
-    if (!usable_scale(*smax, s)) return;
+    if (!usable_scale(red_max_wave(smax), s)) return;
     const uint32_t n = d.nx * d.ny * d.nz;
     const uint32_t i = bid * blockDim.x + threadIdx.x;
     if (i == 0) state[3] += 1ull;
@@ -1151,7 +1186,7 @@ __global__ void __launch_bounds__(256) k_dm_prepare(const DarcyTask *__restrict_
 
 __global__ void k_dm_reset(const DarcyTask *__restrict__ t)
 {
-    if (threadIdx.x < DARCY_MAXRES_WORDS) t[blockIdx.x].b.maxres[threadIdx.x] = 0ull;
+    for (uint32_t w = threadIdx.x; w < DARCY_MAXRES_WORDS; w += blockDim.x) t[blockIdx.x].b.maxres[w] = 0ull;
 }
 
 // The iterate buffers and the check slots of sweep nijac, as the host loop
@@ -1165,9 +1200,9 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_dm_sweep(const DarcyTask *__r
     const bool check = nijac % 10u == 0u;
     const bool decides = nijac > 0u && (nijac - 1u) % 10u == 0u;
     darcy_sweep_body(T.d, T.b.jc, T.b.jfrozen, even ? T.b.p : T.b.p_new, T.b.k, even ? T.b.p_new : T.b.p,
-                     check ? mr + ((nijac / 10u) & 1u) : nullptr,
-                     check ? mr + (((nijac / 10u) & 1u) ^ 1u) : nullptr,
-                     decides ? mr + (((nijac - 1u) / 10u) & 1u) : nullptr, T.tolerance, nijac - 1u,
+                     check ? mr + ((nijac / 10u) & 1u) * RED_WORDS : nullptr,
+                     check ? mr + (((nijac / 10u) & 1u) ^ 1u) * RED_WORDS : nullptr,
+                     decides ? mr + (((nijac - 1u) / 10u) & 1u) * RED_WORDS : nullptr, T.tolerance, nijac - 1u,
                      mr + DARCY_STATE, (correct && T.net_in4) ? mr + DARCY_SCALE : nullptr, bid);
 }
 
@@ -1178,7 +1213,8 @@ __global__ void k_dm_decide(const DarcyTask *__restrict__ t, uint32_t n, uint32_
     const DarcyTask &T = t[k];
     unsigned long long *state = T.b.maxres + DARCY_STATE;
     if (state[0] != 0ull) return;
-    (void)darcy_decide(T.b.maxres + ((nijac / 10u) & 1u), T.tolerance, nijac, state, true);
+    (void)darcy_decide_bits(red_max_serial(T.b.maxres + ((nijac / 10u) & 1u) * RED_WORDS), T.tolerance, nijac,
+                            state, true);
 }
 
 // the iterate that deciding sweep nijac wrote

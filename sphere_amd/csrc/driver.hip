@@ -844,10 +844,10 @@ static int darcy_step(sph_ctx *ctx, std::ofstream *convlog)
         for (; nijac < end; ++nijac) {
             const bool check = (nijac % 10 == 0);
             // two check slots alternate: a check sweep zeroes the other one
-            unsigned long long *slot = d.maxres + ((nijac / 10) & 1u);
-            unsigned long long *next = d.maxres + (((nijac / 10) & 1u) ^ 1u);
+            unsigned long long *slot = d.maxres + ((nijac / 10) & 1u) * RED_WORDS;
+            unsigned long long *next = d.maxres + (((nijac / 10) & 1u) ^ 1u) * RED_WORDS;
             const bool decides = nijac > 0 && (nijac - 1) % 10 == 0;
-            const unsigned long long *prev = decides ? d.maxres + (((nijac - 1) / 10) & 1u) : nullptr;
+            const unsigned long long *prev = decides ? d.maxres + (((nijac - 1) / 10) & 1u) * RED_WORDS : nullptr;
             ProfScope ps(ctx, SPH_K_DARCY_SWEEP);
             // setDarcyTopWallFixedFlow (bc_top == 1) writes dev_darcy_p, which
             // the copy of p_new into p overwrites: it has no effect
@@ -872,7 +872,7 @@ static int darcy_step(sph_ctx *ctx, std::ofstream *convlog)
             }
         }
         if (nijac == h.maxiter && (h.maxiter - 1) % 10 == 0)
-            launch_darcy_decide(s, d, d.maxres + (((h.maxiter - 1) / 10) & 1u), h.tolerance,
+            launch_darcy_decide(s, d, d.maxres + (((h.maxiter - 1) / 10) & 1u) * RED_WORDS, h.tolerance,
                                 h.maxiter - 1);
         HIPCHK(hipMemcpyAsync(ctx->h_state, d.maxres + DARCY_STATE, 32, hipMemcpyDeviceToHost, s));
         int rc = sync(ctx);
@@ -1822,13 +1822,16 @@ int sph_darcy_net_input(sph_ctx *ctx, const double *p, float *input, double *sca
     const DarcyArgs da = darcy_args(ctx);
     DarcyBuffers &d = ctx->dd;
     hipStream_t s = ctx->stream;
-    launch_fill_u32(s, (uint32_t *)(d.maxres + DARCY_SCALE), 2, 0u);
+    launch_fill_u32(s, (uint32_t *)(d.maxres + DARCY_SCALE), 2 * RED_WORDS, 0u);
     launch_darcy_residual_max(s, da, d, p, true);
     launch_darcy_net_input(s, da, d, p, input, true);
-    HIPCHK(hipMemcpyAsync(ctx->h_u64, d.maxres + DARCY_SCALE, 8, hipMemcpyDeviceToHost, s));
     rc = sync(ctx);
     if (rc) return rc;
-    std::memcpy(scale, ctx->h_u64, 8);
+    std::vector<unsigned long long> w(RED_WORDS);
+    HIPCHK(hipMemcpy(w.data(), d.maxres + DARCY_SCALE, 8 * RED_WORDS, hipMemcpyDeviceToHost));
+    unsigned long long m = 0;
+    for (int j = 0; j < RED_LANES; ++j) m = std::max(m, w[(size_t)j * RED_STRIDE]);
+    std::memcpy(scale, &m, 8);
     return SPH_OK;
 }
 

@@ -291,14 +291,23 @@ void launch_bond_gather_multi(hipStream_t s, const BondGatherTask *t, MultiMap m
 void launch_walls_multi(hipStream_t s, const WallsTask *t, uint32_t n);
 void launch_clock_multi(hipStream_t s, const ClockTask *t, uint32_t n);
 
-// words of DarcyBuffers::maxres: [0], [1] alternating check slots;
+// A max over all the blocks of a launch (the bit pattern of a non-negative
+// double, or NaN above +inf) is kept in RED_LANES words RED_STRIDE words
+// apart: block b folds its max into word b % RED_LANES with one atomic max,
+// so that the atomics of a launch meet 64 addresses on separate 128-B lines
+// rather than one; readers take the max of the 64 words.
+#define RED_LANES 64
+#define RED_STRIDE 16
+#define RED_WORDS (RED_LANES * RED_STRIDE)
+// words of DarcyBuffers::maxres: two alternating check slots (check slot k
+// at k * RED_WORDS); DARCY_SCALE: the max |residual| slot of a correction;
 // DARCY_STATE: solve state (ended, nijac of the last check, its max |norm|
 // bits, corrections applied); DARCY_ZERO: always 0 (the state of sweeps run
-// outside a solve); DARCY_SCALE: max |residual| bits of a correction
-#define DARCY_STATE 2
-#define DARCY_ZERO 6
-#define DARCY_SCALE 7
-#define DARCY_MAXRES_WORDS 8
+// outside a solve)
+#define DARCY_SCALE (2 * RED_WORDS)
+#define DARCY_STATE (3 * RED_WORDS)
+#define DARCY_ZERO (DARCY_STATE + 4)
+#define DARCY_MAXRES_WORDS (DARCY_STATE + 8)
 
 struct DarcyBuffers {
     double *p, *p_old, *p_new, *dp_expl, *phi, *dphi, *k;
