@@ -226,32 +226,41 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t *__res
     radix_hist_tile(keys, n, shift, hist, ntiles, blockIdx.x, h);
 }
 
-// Exclusive scan of m entries in place, one block of 1024 threads.
+// Exclusive scan of m entries in place, one block of 1024 threads, row by
+// row of 1024 consecutive entries (coalesced): a wave scan by shuffles, the
+// 16 wave totals scanned by one wave, and the running total of the rows
+// before; the next row's loads are issued before the current row is
+// scanned.
 __global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t *__restrict__ a, uint32_t m,
                                                          const uint32_t *__restrict__ cond)
 {
     SKIP_UNLESS(cond);
-    __shared__ uint32_t part[1024];
-    uint32_t t = threadIdx.x;
-    uint32_t chunk = (m + 1023) / 1024;
-    uint32_t lo = t * chunk, hi = lo + chunk;
-    if (hi > m) hi = m;
-    uint32_t sum = 0;
-    for (uint32_t i = lo; i < hi; ++i) sum += a[i];
-    part[t] = sum;
-    __syncthreads();
-    // Hillis-Steele inclusive scan of the 1024 partial sums
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = t >= off ? part[t - off] : 0;
+    __shared__ uint32_t wsum[16];
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    uint32_t carry = 0;
+    uint32_t nxt = t < m ? a[t] : 0u;
+    for (uint32_t base = 0; base < m; base += 1024u) {
+        const uint32_t v = nxt;
+        if (base + 1024u + t < m) nxt = a[base + 1024u + t];
+        uint32_t x = v;  // inclusive scan over the wave
+        for (uint32_t off = 1; off < 64u; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63u) wsum[w] = x;
         __syncthreads();
-        part[t] += v;
+        if (w == 0) {  // inclusive scan of the 16 wave totals
+            uint32_t sw = lane < 16u ? wsum[lane] : 0u;
+            for (uint32_t off = 1; off < 16u; off <<= 1) {
+                const uint32_t y = __shfl_up(sw, off, 64);
+                if (lane >= off) sw += y;
+            }
+            if (lane < 16u) wsum[lane] = sw;
+        }
         __syncthreads();
-    }
-    uint32_t run = t ? part[t - 1] : 0;
-    for (uint32_t i = lo; i < hi; ++i) {
-        uint32_t v = a[i];
-        a[i] = run;
-        run += v;
+        if (base + t < m) a[base + t] = carry + (w ? wsum[w - 1] : 0u) + x - v;
+        carry += wsum[15];
+        __syncthreads();  // wsum is rewritten by the next row
     }
 }
 
