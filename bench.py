@@ -650,7 +650,7 @@ def main():
     ap.add_argument("--ens-seed", type=int, default=5)
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--ens-warmup", type=int, default=20)
-    ap.add_argument("--net-iters", type=int, default=5000, help="Adam steps per training round")
+    ap.add_argument("--net-iters", type=int, default=10000, help="Adam steps per training round")
     ap.add_argument("--net-pairs", type=int, default=20000,
                     help="(input, error) training pairs in all")
     ap.add_argument("--net-boxes", type=int, default=3,
