@@ -224,8 +224,11 @@ def darcy_ensemble_section(args, params):
            corr / max(1, fsteps), numpy.min(per_box), numpy.median(per_box), numpy.max(per_box)))
     sizes = sorted(p["grains"] for p in info)
     grids = sorted(set(tuple(p["grid"]) for p in info))
+    # the CPU oracle runs the plain solve (no network)
+    cpu = None if args.no_cpu else ensemble_cpu_leg(args, sims, info, 32, 3)
     return {
         "metric": "aggregate particle-steps/s (independent Darcy-coupled boxes, FP64)",
+        "cpu_oracle_two_boxes": cpu,
         "together": v_ens, "one_at_a_time": v_one, "speedup": v_ens / v_one,
         "steps": steps, "warmup": warm,
         "max_abs_position_difference_vs_one_at_a_time": dx,
@@ -527,7 +530,9 @@ def ensemble_section(args):
     log("ensemble: %.4g particle-steps/s together, %.4g one at a time (%.2fx); max |x| difference %g" %
         (v_ens, v_one, v_ens / v_one, diff))
     sizes = sorted(p["grains"] for p in params)
+    cpu = None if args.no_cpu else ensemble_cpu_leg(args, sims, params, 64, 10)
     return {
+        "cpu_oracle_two_boxes": cpu,
         "metric": "aggregate particle-steps/s (independent boxes, FP64)",
         "together": v_ens, "one_at_a_time": v_one, "speedup": v_ens / v_one,
         "steps": args.ens_steps, "warmup": args.ens_warmup,
@@ -539,6 +544,33 @@ def ensemble_section(args):
                    "boxes": len(params), "grains_total": total,
                    "grains_min_median_max": [sizes[0], sizes[len(sizes) // 2], sizes[-1]]},
     }
+
+
+def ensemble_cpu_leg(args, sims, params, nc, steps):
+    """Two representative boxes of an ensemble (the median-sized and the
+    largest) on the CPU oracle: 2 untimed + `steps` timed steps each, with
+    all the job's host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import Oracle
+    threads = host_threads(args)
+    order = sorted(range(len(params)), key=lambda k: params[k]["grains"])
+    out = []
+    for k in (order[len(order) // 2], order[-1]):
+        ora = Oracle(sims[k].to_state(), threads=threads, nc=nc)
+        ora.step(2)
+        t0 = time.perf_counter()
+        ora.step(steps)
+        dt = time.perf_counter() - t0
+        rec = {"box": k, "grains": params[k]["grains"],
+               "particle_steps_per_s": params[k]["grains"] * steps / dt, "ms_per_step": 1e3 * dt / steps}
+        if "grid" in params[k]:
+            rec["grid"] = params[k]["grid"]
+        out.append(rec)
+        log("ensemble box %d (%d grains) on the CPU oracle (%d threads): %.4g particle-steps/s" %
+            (k, params[k]["grains"], threads, rec["particle_steps_per_s"]))
+    return {"boxes": out, "cores": threads, "kind": "port",
+            "sample": "2 untimed + %d timed steps of the FP64 C oracle per box, from the prepared state"
+                      % steps, "host": host_info(threads)}
 
 
 def host_threads(args):
@@ -809,7 +841,11 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(),
                      "traffic_source": "profiles/shear_box_1m_hbm_traffic.md",
-                     "bytes_per_step": c_bytes, "us_per_step": 1e6 * c_avg_s},
+                     "bytes_per_step": c_bytes, "us_per_step": 1e6 * c_avg_s,
+                     "measured_copy_gbs": copy_ceiling(),
+                     "measured_copy_note": "device-to-device hipMemcpy of 1 GiB, read + write bytes "
+                                           "per second, median of 10: the stream-copy ceiling of "
+                                           "this GPU"},
         "kernels": ktimes,
         "cpu_baseline": cpu,
     }
@@ -844,6 +880,41 @@ def darcy_child():
     if r.returncode != 0:
         raise SystemExit("the Darcy section failed (exit status %d)" % r.returncode)
     return json.loads(r.stdout.strip().splitlines()[-1])["darcy"]
+
+
+def copy_ceiling(mib=1024, reps=10):
+    """Read + write bytes per second of a device-to-device hipMemcpy of
+    `mib` MiB (the median of `reps` copies timed with HIP events)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib * (1 << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or \
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    rates = []
+    for _ in range(reps):
+        hip.hipEventRecord(e0, None)
+        hip.hipMemcpyAsync(b, a, ctypes.c_size_t(n), 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float(0.0)
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        if ms.value > 0:
+            rates.append(2.0 * n / (ms.value * 1e-3) / 1e9)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    gbs = float(numpy.median(rates)) if rates else None
+    if gbs:
+        log("stream-copy ceiling: %.0f GB/s (device-to-device copy of %d MiB)" % (gbs, mib))
+    return gbs
 
 
 def copy_calibration(mib):
