@@ -170,12 +170,42 @@ def darcy_ensemble_section(args, params):
     log("Darcy ensemble: %d boxes, %d grains in all, prepared in %.1f s" % (len(sims), total,
                                                                           time.time() - t0))
     states = [s.to_state() for s in sims]
+    # Per-box autotune of the learned correction: from its state, each box
+    # runs --dens-tune-steps fluid steps with the network and without (after
+    # one untimed step each), and keeps the network where its solves took
+    # less device time.  Both the ensemble and the one-at-a-time runs use
+    # the choice, so every box's results can be compared bit for bit.
+    use_net = [params is not None] * len(states)
+    tune = None
+    if params is not None and args.dens_tune_steps > 0:
+        t0 = time.time()
+        trial = []
+        for k, st in enumerate(states):
+            ms = []
+            for on in (True, False):
+                e = sphere_amd.Engine(0, nc=32)
+                if on:
+                    e.set_darcy_net(params, args.net_corrections)
+                e.load(st)
+                e.step(1)
+                a = e.darcy_stats().solve_ms
+                e.step(args.dens_tune_steps)
+                ms.append((e.darcy_stats().solve_ms - a) / args.dens_tune_steps)
+                e.close()
+            use_net[k] = ms[0] < ms[1]
+            trial.append(ms)
+        off = [k for k in range(len(states)) if not use_net[k]]
+        tune = {"trial_steps": args.dens_tune_steps, "boxes_with_network": len(states) - len(off),
+                "boxes_without": off, "seconds": time.time() - t0,
+                "solve_ms_with_without": [[round(a, 4), round(b, 4)] for a, b in trial]}
+        log("Darcy ensemble: the network kept on %d of %d boxes (autotune, %.1f s; off on %s)" %
+            (len(states) - len(off), len(states), tune["seconds"], off))
 
     def engines():
         out = []
-        for st in states:
+        for k, st in enumerate(states):
             e = sphere_amd.Engine(0, nc=32)
-            if params is not None:
+            if use_net[k]:
                 e.set_darcy_net(params, args.net_corrections)
             e.load(st)
             out.append(e)
@@ -239,6 +269,7 @@ def darcy_ensemble_section(args, params):
                                                      float(numpy.median(per_box)),
                                                      float(numpy.max(per_box))],
         "learned_correction": params is not None,
+        "correction_autotune": tune,
         "ensemble_info": einfo,
         "config": {"workload": "%d independent Darcy-coupled shear boxes, grains log-uniform in "
                                "[5000, 60000] (radii U[0.009, 0.011] m, linear contacts), mu ~ "
@@ -676,6 +707,9 @@ def main():
                     help="skip both ensemble sections (DEM boxes; Darcy-coupled boxes)")
     ap.add_argument("--dens-steps", type=int, default=40, help="timed steps of the Darcy ensemble")
     ap.add_argument("--dens-warmup", type=int, default=10)
+    ap.add_argument("--dens-tune-steps", type=int, default=3,
+                    help="trial fluid steps per box of the learned-correction autotune (0: the "
+                         "network on every box)")
     ap.add_argument("--ensemble", action="store_true",
                     help="with --no-shear --no-darcy: only the ensemble of DEM boxes")
     ap.add_argument("--ens-boxes", type=int, default=64)
