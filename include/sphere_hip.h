@@ -403,6 +403,67 @@ int sph_ensemble_set_graph(sph_ensemble *ens, int on);
 int sph_ensemble_autotune(sph_ensemble *ens, int trial_steps, const double *fractions,
                           int nfrac, double *chosen, double *ms_per_step);
 
+/* ------------------------------------------------------------------------
+ * Navier-Stokes fluid (cfd_solver = 0) without grains (np = 0): the
+ * reference's fluid part of each time step, src/device.cu:1213-1898, on a
+ * staggered grid (cell-centred p, face velocities, one ghost layer of cells
+ * and of faces): porosity 1 (src/navierstokes.cuh:1319-1328), viscous and
+ * advective terms, predicted face velocities (Chorin projection with
+ * beta = 0, or a pressure-dependent one with beta > 0), the Jacobi
+ * iteration of the pressure change epsilon with its normalized-residual
+ * test every 10 iterations, optional smoothing, and the pressure and
+ * velocity updates (src/navierstokes.cuh:261-3514).  No walls.
+ * Boundaries: bc 0 fixed pressure, 1 no flow with free slip, 2 no flow
+ * without slip (python/sphere.py:3527-3598); x and y are periodic.
+ * ---------------------------------------------------------------------- */
+typedef struct sph_ns_params {
+    uint32_t nx, ny, nz;          /* cells, each >= 3 */
+    double L[3];                  /* domain size (grid.L) */
+    double dt;                    /* the DEM time step (time.dt) */
+    double t0;                    /* time.current at the first step */
+    double mu, rho_f;             /* dynamic viscosity, fluid density */
+    double g[3];                  /* gravity (params.g) */
+    double beta, theta, gamma;    /* projection, relaxation, smoothing */
+    double tolerance, c_v;        /* Jacobi test, velocity scaling */
+    uint32_t maxiter, ndem;       /* Jacobi limit, DEM steps per fluid step */
+    int32_t bc_bot, bc_top;       /* 0, 1 or 2, as above */
+    double p_mod_A, p_mod_f, p_mod_phi; /* top pressure modulation */
+} sph_ns_params;
+
+typedef struct sph_ns_stats {
+    uint64_t steps;               /* time steps taken */
+    uint64_t solves;              /* Jacobi solves (steps with iter % ndem == 0) */
+    uint64_t iterations;          /* Jacobi iterations in all */
+    uint64_t nonconverged;        /* solves that reached maxiter */
+    uint32_t last_nijac;          /* iterations of the last solve */
+    double last_max_norm;         /* last tested max normalized residual */
+} sph_ns_stats;
+
+typedef struct sph_ns sph_ns;
+
+/* A fluid on `device` from host arrays of the interior cells (index
+ * x + nx*(y + ny*z)): v[3*cell] the cell-centred velocity (ns.v), p[cell]
+ * the pressure (ns.p).  The first step forms the face velocities from v
+ * (src/device.cu:1242-1259).  NULL on failure (sph_ns_last_error()). */
+sph_ns *sph_ns_create(int device, const sph_ns_params *params, const double *v,
+                      const double *p);
+void sph_ns_destroy(sph_ns *ns);
+const char *sph_ns_last_error(void);
+/* nsteps time steps (DEM iterations with np = 0).  SPH_ERR_NAN: a NaN
+ * normalized residual (src/navierstokes.cpp:275-297). */
+int sph_ns_step(sph_ns *ns, int nsteps);
+/* The interior cells' v, p, phi, dphi (any may be NULL), as the reference
+ * transfers them for output (src/navierstokes.cuh:177-200,
+ * src/device.cu:2595).  The host copy
+ * of p that the reference's step reads for its boundary values
+ * (ns.p[idx(0,0,nz-1)], src/device.cu:1357-1360) is refreshed here, as
+ * there. */
+int sph_ns_read(sph_ns *ns, double *v, double *p, double *phi, double *dphi);
+/* Whole ghosted arrays for checks: which 0-2 the face velocities v_x, v_y,
+ * v_z ((nx+3)(ny+3)(nz+3) each), 3 epsilon ((nx+2)(ny+2)(nz+2)). */
+int sph_ns_read_field(sph_ns *ns, int which, double *dst);
+int sph_ns_get_stats(const sph_ns *ns, sph_ns_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,7 @@ NC = 32
 MAXWALLS = 6
 
 SPH_OK = 0
+SPH_ERR_ARG = -1
 ERRORS = {
     -1: "argument",
     -2: "file",
@@ -162,6 +163,25 @@ class SphEnsembleInfo(ctypes.Structure):
                 ("fluid_steps", ctypes.c_uint64)]
 
 
+class SphNsParams(ctypes.Structure):
+    """sph_ns_params of include/sphere_hip.h."""
+    _fields_ = [("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32), ("nz", ctypes.c_uint32),
+                ("L", ctypes.c_double * 3), ("dt", ctypes.c_double), ("t0", ctypes.c_double),
+                ("mu", ctypes.c_double), ("rho_f", ctypes.c_double), ("g", ctypes.c_double * 3),
+                ("beta", ctypes.c_double), ("theta", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("tolerance", ctypes.c_double), ("c_v", ctypes.c_double),
+                ("maxiter", ctypes.c_uint32), ("ndem", ctypes.c_uint32),
+                ("bc_bot", ctypes.c_int32), ("bc_top", ctypes.c_int32),
+                ("p_mod_A", ctypes.c_double), ("p_mod_f", ctypes.c_double),
+                ("p_mod_phi", ctypes.c_double)]
+
+
+class SphNsStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("solves", ctypes.c_uint64),
+                ("iterations", ctypes.c_uint64), ("nonconverged", ctypes.c_uint64),
+                ("last_nijac", ctypes.c_uint32), ("last_max_norm", ctypes.c_double)]
+
+
 def load_library(path=None):
     """Load libsphere_hip.so; raises if it was not built."""
     global _lib
@@ -224,6 +244,13 @@ def load_library(path=None):
         "sph_darcy_export": (ctypes.c_int, [vp, ctypes.c_int, vp]),
         "sph_darcy_sweeps": (ctypes.c_int, [vp, vp, ctypes.c_int]),
         "sph_darcy_net_input": (ctypes.c_int, [vp, vp, vp, ctypes.POINTER(ctypes.c_double)]),
+        "sph_ns_create": (vp, [ctypes.c_int, ctypes.POINTER(SphNsParams), _pd, _pd]),
+        "sph_ns_destroy": (None, [vp]),
+        "sph_ns_last_error": (ctypes.c_char_p, []),
+        "sph_ns_step": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sph_ns_read": (ctypes.c_int, [vp, _pd, _pd, _pd, _pd]),
+        "sph_ns_read_field": (ctypes.c_int, [vp, ctypes.c_int, _pd]),
+        "sph_ns_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(SphNsStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -458,6 +485,111 @@ class Net:
     def close(self):
         if self.h:
             self.lib.sph_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ns_params(nx, ny, nz, L, dt, mu=8.9e-4, rho_f=1.0e3, g=(0.0, 0.0, 0.0), beta=0.0, theta=1.0,
+              gamma=0.0, tolerance=1.0e-3, c_v=1.0, maxiter=10000, ndem=1, bc_bot=0, bc_top=0,
+              p_mod_A=0.0, p_mod_f=0.0, p_mod_phi=0.0, t0=0.0):
+    """sph_ns_params with the reference's Navier-Stokes defaults
+    (python/sphere.py:3395-3508)."""
+    P = SphNsParams()
+    P.nx, P.ny, P.nz = int(nx), int(ny), int(nz)
+    for k in range(3):
+        P.L[k] = float(L[k])
+        P.g[k] = float(g[k])
+    P.dt, P.t0, P.mu, P.rho_f = float(dt), float(t0), float(mu), float(rho_f)
+    P.beta, P.theta, P.gamma = float(beta), float(theta), float(gamma)
+    P.tolerance, P.c_v = float(tolerance), float(c_v)
+    P.maxiter, P.ndem = int(maxiter), int(ndem)
+    P.bc_bot, P.bc_top = int(bc_bot), int(bc_top)
+    P.p_mod_A, P.p_mod_f, P.p_mod_phi = float(p_mod_A), float(p_mod_f), float(p_mod_phi)
+    return P
+
+
+def _one_value(a):
+    return numpy.asarray(a).ravel()[0]
+
+
+class NavierStokes:
+    """A Navier-Stokes fluid without grains (sph_ns_*): the reference's
+    cfd_solver = 0 fluid step on the GPU.  Arrays are numpy, interior cells
+    indexed [x, y, z] like sim.p_f and sim.v_f."""
+
+    def __init__(self, params, v, p, device=0, lib=None):
+        self.lib = lib or load_library()
+        self.P = params
+        self.shape = (params.nx, params.ny, params.nz)
+        vc = numpy.ascontiguousarray(numpy.asarray(v, dtype=numpy.float64).transpose(2, 1, 0, 3))
+        pc = numpy.ascontiguousarray(numpy.asarray(p, dtype=numpy.float64).transpose(2, 1, 0))
+        self.h = self.lib.sph_ns_create(int(device), ctypes.byref(params), _ptr(vc, ctypes.c_double),
+                                        _ptr(pc, ctypes.c_double))
+        if not self.h:
+            raise SphereError(SPH_ERR_ARG, self.lib.sph_ns_last_error().decode())
+
+    @classmethod
+    def from_sim(cls, s, device=0, lib=None):
+        """The fluid of a sim with cfd_solver = 0, no grains and no walls."""
+        if int(_one_value(s.cfd_solver)) != 0 or not s.fluid:
+            raise SphereError(SPH_ERR_ARG, "the sim has no Navier-Stokes fluid (cfd_solver = 0)")
+        if int(_one_value(s.np)) != 0 or int(_one_value(s.nw)) != 0:
+            raise SphereError(SPH_ERR_ARG, "only a fluid without grains and walls is built")
+        P = ns_params(*[int(n) for n in s.num], L=s.L, dt=_one_value(s.time_dt),
+                      mu=_one_value(s.mu), rho_f=_one_value(s.rho_f), g=s.g,
+                      beta=_one_value(s.beta), theta=_one_value(s.theta), gamma=_one_value(s.gamma),
+                      tolerance=_one_value(s.tolerance), c_v=_one_value(s.c_v),
+                      maxiter=_one_value(s.maxiter), ndem=_one_value(s.ndem),
+                      bc_bot=_one_value(s.bc_bot), bc_top=_one_value(s.bc_top),
+                      p_mod_A=_one_value(s.p_mod_A), p_mod_f=_one_value(s.p_mod_f),
+                      p_mod_phi=_one_value(s.p_mod_phi), t0=_one_value(s.time_current))
+        return cls(P, s.v_f, s.p_f, device=device, lib=lib)
+
+    def _check(self, rc):
+        if rc != SPH_OK:
+            raise SphereError(rc, self.lib.sph_ns_last_error().decode())
+
+    def step(self, n=1):
+        self._check(self.lib.sph_ns_step(self.h, int(n)))
+
+    def read(self):
+        """(v [nx, ny, nz, 3], p, phi, dphi [nx, ny, nz])."""
+        nx, ny, nz = self.shape
+        v = numpy.empty((nz, ny, nx, 3))
+        p, phi, dphi = (numpy.empty((nz, ny, nx)) for _ in range(3))
+        self._check(self.lib.sph_ns_read(self.h, *[_ptr(a, ctypes.c_double) for a in (v, p, phi, dphi)]))
+        return (v.transpose(2, 1, 0, 3).copy(), p.transpose(2, 1, 0).copy(),
+                phi.transpose(2, 1, 0).copy(), dphi.transpose(2, 1, 0).copy())
+
+    def to_sim(self, s, steps_taken=None):
+        """Write the fluid fields back into a sim (as an output file holds them)."""
+        v, p, phi, dphi = self.read()
+        s.v_f[:] = v
+        s.p_f[:] = p
+        s.phi[:] = phi
+        s.dphi[:] = dphi
+
+    def field(self, which):
+        """Whole ghosted array: 0-2 face velocities, 3 epsilon."""
+        nx, ny, nz = self.shape
+        n = (nx + 3) * (ny + 3) * (nz + 3) if which < 3 else (nx + 2) * (ny + 2) * (nz + 2)
+        out = numpy.empty(n)
+        self._check(self.lib.sph_ns_read_field(self.h, int(which), _ptr(out, ctypes.c_double)))
+        return out
+
+    def stats(self):
+        st = SphNsStats()
+        self._check(self.lib.sph_ns_get_stats(self.h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            self.lib.sph_ns_destroy(self.h)
             self.h = None
 
     def __del__(self):
