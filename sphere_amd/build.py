@@ -18,7 +18,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"
              "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
 SOURCES = ["dem_kernels.hip", "darcy_kernels.hip", "net_kernels.hip", "net.hip", "driver.hip",
-           "fileio.cpp"]
+           "ns.hip", "fileio.cpp"]
 
 
 def hipcc():

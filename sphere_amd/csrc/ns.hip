@@ -1,0 +1,850 @@
+// ns.hip -- the reference's Navier-Stokes fluid step (cfd_solver = 0) for a
+// fluid without grains and walls, behind sph_ns_* (include/sphere_hip.h).
+//
+// The per-step sequence is that of src/device.cu:1213-1898; each kernel
+// below restates the reference kernel it names (src/navierstokes.cuh) with
+// the same FP64 operations in the same order, one thread per cell or face.
+// Two things differ in form, not in result:
+//   - one Jacobi iteration is one launch.  The reference's forcing, ghost,
+//     Jacobi and copy kernels (src/device.cu:1636-1748) become one kernel
+//     reading buffer A and writing buffer B: the forcing uses A's ghost
+//     layer as it stands (the reference computes it before that
+//     iteration's ghost update), the Jacobi stencil takes each ghost
+//     neighbour from the interior cell the ghost rules copy it from, and
+//     the kernel writes B's ghost layer from A's interior, which is what
+//     the reference's eps holds at the next iteration's forcing;
+//   - the max normalized residual of every tenth iteration is reduced on
+//     the device (the bit patterns of non-negative doubles order like their
+//     values; a NaN orders above every number) and read by the host once per
+//     ten iterations, where the reference copies the whole field
+//     (src/navierstokes.cpp:275-297).
+// Where the reference's ghost kernel races (bc_bot = 2 or bc_top = 2), the
+// ghost layer below z = 0 keeps the bottom rule's value and the layer above
+// z = nz-1 the top rule's; smoothing reads the field as it was before the
+// pass (the reference smooths in place).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sphere_hip.h"
+
+namespace {
+
+constexpr int NS_THREADS = 256;
+constexpr double NS_PI = 3.14159265358979323846;  // M_PI
+
+struct NsGrid {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    int bb, bt;  // bc_bot, bc_top
+};
+
+__host__ __device__ inline size_t cidx(const NsGrid &g, int x, int y, int z)
+{
+    return (size_t)(x + 1) + (size_t)(g.nx + 2) * (size_t)(y + 1) +
+           (size_t)(g.nx + 2) * (size_t)(g.ny + 2) * (size_t)(z + 1);
+}
+
+__host__ __device__ inline size_t fidx(const NsGrid &g, int x, int y, int z)
+{
+    return (size_t)(x + 1) + (size_t)(g.nx + 3) * (size_t)(y + 1) +
+           (size_t)(g.nx + 3) * (size_t)(g.ny + 3) * (size_t)(z + 1);
+}
+
+__device__ inline double amean(double a, double b) { return (a + b) * 0.5; }
+
+// interior cell of thread i (x fastest); false past the last cell
+__device__ inline bool cell_of(const NsGrid &g, int &x, int &y, int &z)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned nxy = (unsigned)g.nx * (unsigned)g.ny;
+    if (i >= nxy * (unsigned)g.nz) return false;
+    z = (int)(i / nxy);
+    const unsigned r = i - (unsigned)z * nxy;
+    y = (int)(r / (unsigned)g.nx);
+    x = (int)(r - (unsigned)y * (unsigned)g.nx);
+    return true;
+}
+
+struct Vec3 {
+    double *c[3];
+};
+struct CVec3 {
+    const double *c[3];
+};
+
+// findPorositiesVelocitiesDiametersSpherical with np = 0,
+// src/navierstokes.cuh:1319-1328
+__global__ void k_ns_porosity(NsGrid g, double *phi, double *dphi)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    phi[cidx(g, x, y, z)] = 1.0;
+    dphi[cidx(g, x, y, z)] = 0.0;
+}
+
+// setNSghostNodes<T>(a, bc_bot, bc_top), src/navierstokes.cuh:660-713, on
+// up to three arrays; the z ghost writes as stated at the top of this file
+__device__ inline void ghost_cell_writes(const NsGrid &g, int x, int y, int z, double val, double *a)
+{
+    if (x == 0) a[cidx(g, g.nx, y, z)] = val;
+    if (x == g.nx - 1) a[cidx(g, -1, y, z)] = val;
+    if (y == 0) a[cidx(g, x, g.ny, z)] = val;
+    if (y == g.ny - 1) a[cidx(g, x, -1, z)] = val;
+    if (z == 0 && (g.bb == 0 || g.bb == 1)) a[cidx(g, x, y, -1)] = val;
+    if (z == 0 && g.bb == 2 && g.bt == 2) a[cidx(g, x, y, g.nz)] = val;
+    if (z == g.nz - 1 && g.bt == 0) a[cidx(g, x, y, g.nz)] = val;
+    if (z == g.nz - 2 && g.bt == 1) a[cidx(g, x, y, g.nz)] = val;
+    if (z == g.nz - 1 && g.bt == 2 && g.bb == 2) a[cidx(g, x, y, -1)] = val;
+}
+
+__global__ void k_ns_ghost_cells(NsGrid g, Vec3 a, int n)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    for (int k = 0; k < n; ++k) ghost_cell_writes(g, x, y, z, a.c[k][cidx(g, x, y, z)], a.c[k]);
+}
+
+// setNSghostNodesFace<T>, src/navierstokes.cuh:715-911 (bc 0-2)
+__global__ void k_ns_ghost_faces(NsGrid g, Vec3 F)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz, bb = g.bb, bt = g.bt;
+    double *X = F.c[0], *Y = F.c[1], *Z = F.c[2];
+    const size_t f = fidx(g, x, y, z);
+    const double vx = X[f], vy = Y[f], vz = Z[f];
+    if (x == 0) {
+        X[fidx(g, nx, y, z)] = vx;
+        Y[fidx(g, nx, y, z)] = vy;
+        Z[fidx(g, nx, y, z)] = vz;
+    }
+    if (x == 1) X[fidx(g, nx + 1, y, z)] = vx;
+    if (x == nx - 1) {
+        X[fidx(g, -1, y, z)] = vx;
+        Y[fidx(g, -1, y, z)] = vy;
+        Z[fidx(g, -1, y, z)] = vz;
+    }
+    if (z == nz - 1 && x == nx - 1 && bt == 0) Z[fidx(g, -1, y, nz)] = vz;
+    if (z == nz - 1 && x == nx - 1 && (bt == 1 || bt == 2)) Z[fidx(g, -1, y, nz)] = 0.0;
+    if (z == nz - 1 && y == ny - 1 && bt == 0) Z[fidx(g, x, -1, nz)] = vz;
+    if (z == nz - 1 && y == ny - 1 && (bt == 1 || bt == 2)) Z[fidx(g, x, -1, nz)] = 0.0;
+    if (z == 0 && x == 0 && (bb == 0 || bb == 1)) X[fidx(g, nx, y, -1)] = vx;
+    if (z == 0 && x == 0 && bb == 2) X[fidx(g, nx, y, -1)] = -vx;
+    if (z == 0 && y == 0 && (bb == 0 || bb == 1)) Y[fidx(g, x, ny, -1)] = vy;
+    if (z == 0 && y == 0 && bb == 2) Y[fidx(g, x, ny, -1)] = -vy;
+    if (z == nz - 1 && x == 0 && bt == 0) Z[fidx(g, nx, y, nz)] = vz;
+    if (z == nz - 1 && x == 0 && (bt == 1 || bt == 2)) Z[fidx(g, nx, y, nz)] = 0.0;
+    if (z == nz - 1 && y == 0 && bt == 0) Z[fidx(g, x, ny, nz)] = vz;
+    if (z == nz - 1 && y == 0 && (bt == 1 || bt == 2)) Z[fidx(g, x, ny, nz)] = 0.0;
+    // the reference tests bc_bot for these two top corners
+    if (z == nz - 1 && x == 0 && (bb == 0 || bb == 1)) X[fidx(g, nx, y, nz)] = vx;
+    if (z == nz - 1 && x == 0 && bb == 2) X[fidx(g, nx, y, nz)] = -vx;
+    if (z == nz - 1 && y == 0 && (bb == 0 || bb == 1)) Y[fidx(g, x, ny, nz)] = vy;
+    if (z == nz - 1 && y == 0 && bb == 2) Y[fidx(g, x, ny, nz)] = -vy;
+    if (y == 0) {
+        X[fidx(g, x, ny, z)] = vx;
+        Y[fidx(g, x, ny, z)] = vy;
+        Z[fidx(g, x, ny, z)] = vz;
+    }
+    if (y == 1) Y[fidx(g, x, ny + 1, z)] = vy;
+    if (y == ny - 1) {
+        X[fidx(g, x, -1, z)] = vx;
+        Y[fidx(g, x, -1, z)] = vy;
+        Z[fidx(g, x, -1, z)] = vz;
+    }
+    if (z == 0 && bb == 0) {  // the reference swaps x and y here
+        X[fidx(g, x, y, -1)] = vy;
+        Y[fidx(g, x, y, -1)] = vx;
+        Z[fidx(g, x, y, -1)] = vz;
+    }
+    if (z == 0 && bb == 1) {
+        X[fidx(g, x, y, -1)] = vx;
+        Y[fidx(g, x, y, -1)] = vy;
+        Z[fidx(g, x, y, -1)] = 0.0;
+    }
+    if (z == 0 && bb == 2) {
+        X[fidx(g, x, y, -1)] = -vx;
+        Y[fidx(g, x, y, -1)] = -vy;
+        Z[fidx(g, x, y, -1)] = 0.0;
+    }
+    if (z == nz - 1 && bt == 0) Z[fidx(g, x, y, nz)] = vz;
+    if (z == nz - 1 && bt == 1) {
+        X[fidx(g, x, y, nz)] = vx;
+        Y[fidx(g, x, y, nz)] = vy;
+        Z[fidx(g, x, y, nz)] = 0.0;
+        Z[fidx(g, x, y, nz + 1)] = 0.0;
+    }
+    if (z == nz - 1 && bt == 2) {
+        X[fidx(g, x, y, nz)] = -vx;
+        Y[fidx(g, x, y, nz)] = -vy;
+        Z[fidx(g, x, y, nz)] = 0.0;
+        Z[fidx(g, x, y, nz + 1)] = 0.0;
+    }
+}
+
+// interpolateCenterToFace, src/navierstokes.cuh:3347-3382, over the
+// (nx+1)(ny+1)(nz+1) faces
+__global__ void k_ns_center_to_face(NsGrid g, CVec3 c, Vec3 F)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned ax = (unsigned)g.nx + 1, ay = (unsigned)g.ny + 1;
+    if (i >= ax * ay * ((unsigned)g.nz + 1)) return;
+    const int z = (int)(i / (ax * ay));
+    const unsigned r = i - (unsigned)z * ax * ay;
+    const int y = (int)(r / ax);
+    const int x = (int)(r - (unsigned)y * ax);
+    const size_t f = fidx(g, x, y, z);
+    F.c[0][f] = amean(c.c[0][cidx(g, x, y, z)], c.c[0][cidx(g, x - 1, y, z)]);
+    F.c[1][f] = amean(c.c[1][cidx(g, x, y, z)], c.c[1][cidx(g, x, y - 1, z)]);
+    F.c[2][f] = amean(c.c[2][cidx(g, x, y, z)], c.c[2][cidx(g, x, y, z - 1)]);
+}
+
+// interpolateFaceToCenter, src/navierstokes.cuh:3384-3425
+__global__ void k_ns_face_to_center(NsGrid g, CVec3 F, Vec3 c)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    const size_t i = cidx(g, x, y, z);
+    c.c[0][i] = amean(F.c[0][fidx(g, x, y, z)], F.c[0][fidx(g, x + 1, y, z)]);
+    c.c[1][i] = amean(F.c[1][fidx(g, x, y, z)], F.c[1][fidx(g, x, y + 1, z)]);
+    c.c[2][i] = amean(F.c[2][fidx(g, x, y, z)], F.c[2][fidx(g, x, y, z + 1)]);
+}
+
+// findFaceDivTau, src/navierstokes.cuh:3427-3505
+__global__ void k_ns_div_tau(NsGrid g, CVec3 V, double mu, Vec3 D)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    const size_t f = fidx(g, x, y, z);
+    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    for (int k = 0; k < 3; ++k) {
+        const double *F = V.c[k];
+        const double v = F[f];
+        D.c[k][f] = mu * ((F[fidx(g, x + 1, y, z)] - 2.0 * v + F[fidx(g, x - 1, y, z)]) / (dx * dx) +
+                          (F[fidx(g, x, y + 1, z)] - 2.0 * v + F[fidx(g, x, y - 1, z)]) / (dy * dy) +
+                          (F[fidx(g, x, y, z + 1)] - 2.0 * v + F[fidx(g, x, y, z - 1)]) / (dz * dz));
+    }
+}
+
+// findNSdivphiviv, src/navierstokes.cuh:1909-2044: half upwind, half
+// central differences of div(phi v_i v)
+__global__ void k_ns_div_phi_vv(NsGrid g, const double *ph, CVec3 V, Vec3 D)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    const size_t c = cidx(g, x, y, z);
+    const size_t n[3] = {cidx(g, x - 1, y, z), cidx(g, x, y - 1, z), cidx(g, x, y, z - 1)};
+    const size_t pp[3] = {cidx(g, x + 1, y, z), cidx(g, x, y + 1, z), cidx(g, x, y, z + 1)};
+    const double h[3] = {g.dx, g.dy, g.dz};
+    double v[3], a[3];
+    for (int k = 0; k < 3; ++k) {
+        v[k] = V.c[k][c];
+        a[k] = copysign(1.0, v[k]);
+    }
+    const double phi = ph[c];
+    for (int i = 0; i < 3; ++i) {
+        double uw = 0.0, cd = 0.0;
+        for (int j = 0; j < 3; ++j) {
+            const double vn_i = V.c[i][n[j]], vn_j = V.c[j][n[j]];
+            const double vp_i = V.c[i][pp[j]], vp_j = V.c[j][pp[j]];
+            const double tu = ((1.0 + a[j]) * (phi * v[i] * v[j] - ph[n[j]] * vn_i * vn_j) +
+                               (1.0 - a[j]) * (ph[pp[j]] * vp_i * vp_j - phi * v[i] * v[j])) / (2.0 * h[j]);
+            const double tc = (ph[pp[j]] * vp_i * vp_j - ph[n[j]] * vn_i * vn_j) / (2.0 * h[j]);
+            uw = j == 0 ? tu : uw + tu;
+            cd = j == 0 ? tc : cd + tc;
+        }
+        const double tau = 0.5;
+        D.c[i][c] = tau * uw + (1.0 - tau) * cd;
+    }
+}
+
+struct NsPhys {
+    double dt;  // ndem * time.dt
+    double mu, rho_f, beta, theta, c_v;
+    double g[3];
+};
+
+// findPredNSvelocities, src/navierstokes.cuh:2276-2495 (SET_2, np = 0)
+__global__ void k_ns_predict(NsGrid g, NsPhys q, const double *p, CVec3 VF, const double *phi_a,
+                             const double *dphi_a, CVec3 DT, CVec3 DPVV, Vec3 VP)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    const size_t f = fidx(g, x, y, z);
+    const size_t c = cidx(g, x, y, z);
+    const size_t n[3] = {cidx(g, x - 1, y, z), cidx(g, x, y - 1, z), cidx(g, x, y, z - 1)};
+    const double h[3] = {g.dx, g.dy, g.dz};
+    const double dt = q.dt;
+    double vp[3];
+    for (int k = 0; k < 3; ++k) {
+        const double v = VF.c[k][f];
+        const double div_tau = q.mu > 0.0 ? DT.c[k][f] : 0.0;
+        const double phi = amean(phi_a[c], phi_a[n[k]]);
+        const double dphi = amean(dphi_a[c], dphi_a[n[k]]);
+        const double f_i = amean(0.0, 0.0);
+        double pressure_term = 0.0;
+        if (q.beta > 0.0) {
+            const double grad_p = (p[c] - p[n[k]]) / h[k];
+            pressure_term = -q.beta * dt / q.rho_f * grad_p;
+        }
+        // the x component of the lower neighbour's div(phi v v) for every
+        // direction, as the reference averages it
+        const double dpvv = amean(DPVV.c[0][n[k]], DPVV.c[k][c]);
+        const double interaction_term = -dt / (q.rho_f * phi) * f_i;
+        const double gravity_term = q.g[k] * dt;
+        const double advection_term = -1.0 * dpvv * dt / phi;
+        const double porosity_term = -1.0 * v * dphi / phi;
+        const double diffusion_term = dt / q.rho_f * div_tau;
+        vp[k] = v + q.c_v * (pressure_term + interaction_term + diffusion_term + gravity_term +
+                             porosity_term + advection_term);
+    }
+    if (z == 0 && g.bb == 1) vp[2] = 0.0;
+    if (z == 0 && g.bb == 2) vp[0] = vp[1] = vp[2] = 0.0;
+    for (int k = 0; k < 3; ++k) VP.c[k][f] = vp[k];
+}
+
+// setNSepsilonTop / Bottom / Interior, src/navierstokes.cuh:356-446, on
+// layers z0..z1 of the current epsilon buffer
+__global__ void k_ns_set_layers(NsGrid g, double *eps, int z0, int z1, double value)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    if (z >= z0 && z <= z1) eps[cidx(g, x, y, z)] = value;
+}
+
+// setUpperPressureNS, src/navierstokes.cuh:1550-1588
+__global__ void k_ns_upper_pressure(NsGrid g, double *p, double *eps, double beta, double newp)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z) || z != g.nz - 1) return;
+    const size_t c = cidx(g, x, y, z);
+    const double e = newp - beta * p[c];
+    eps[c] = e;
+    p[c] = newp;
+}
+
+// The interior cell whose value the ghost rules copy into ghost (x, y, z),
+// for the ghost neighbours of the Jacobi stencil; a ghost no rule writes
+// keeps its own value.
+__device__ inline size_t ghost_source(const NsGrid &g, int x, int y, int z)
+{
+    if (x < 0) return cidx(g, g.nx - 1, y, z);
+    if (x >= g.nx) return cidx(g, 0, y, z);
+    if (y < 0) return cidx(g, x, g.ny - 1, z);
+    if (y >= g.ny) return cidx(g, x, 0, z);
+    if (z < 0) {
+        if (g.bb == 0 || g.bb == 1) return cidx(g, x, y, 0);
+        if (g.bb == 2 && g.bt == 2) return cidx(g, x, y, g.nz - 1);
+        return cidx(g, x, y, -1);
+    }
+    if (z >= g.nz) {
+        if (g.bt == 0) return cidx(g, x, y, g.nz - 1);
+        if (g.bt == 1) return cidx(g, x, y, g.nz - 2);
+        if (g.bt == 2 && g.bb == 2) return cidx(g, x, y, 0);
+        return cidx(g, x, y, g.nz);
+    }
+    return cidx(g, x, y, z);
+}
+
+__device__ inline unsigned long long wave_max_u64(unsigned long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// One Jacobi iteration: findNSforcing (src/navierstokes.cuh:2497-2617),
+// setNSghostNodes(epsilon), jacobiIterationNS (:2679-2792, no wall) and
+// copyValues (:2795-2827), A -> B, as described at the top of this file.
+// maxw != nullptr: the iteration's max |norm| goes there (bit pattern).
+__global__ void __launch_bounds__(NS_THREADS)
+k_ns_jacobi(NsGrid g, NsPhys q, unsigned nijac, const double *A, double *B, double *f1a, Vec3 f2a,
+            CVec3 VPc, CVec3 VPF, const double *phi_a, const double *dphi_a, unsigned long long *maxw)
+{
+    int x, y, z;
+    unsigned long long bits = 0ull;
+    if (cell_of(g, x, y, z)) {
+        const double dx = g.dx, dy = g.dy, dz = g.dz;
+        const size_t c = cidx(g, x, y, z);
+        double f1, f2[3];
+        if (nijac == 0) {
+            const double phi = phi_a[c], dphi = dphi_a[c];
+            const double div_v_p = (VPF.c[0][fidx(g, x + 1, y, z)] - VPF.c[0][fidx(g, x, y, z)]) / dx +
+                                   (VPF.c[1][fidx(g, x, y + 1, z)] - VPF.c[1][fidx(g, x, y, z)]) / dy +
+                                   (VPF.c[2][fidx(g, x, y, z + 1)] - VPF.c[2][fidx(g, x, y, z)]) / dz;
+            const double gp[3] = {(phi_a[cidx(g, x + 1, y, z)] - phi_a[cidx(g, x - 1, y, z)]) / (2.0 * dx),
+                                  (phi_a[cidx(g, x, y + 1, z)] - phi_a[cidx(g, x, y - 1, z)]) / (2.0 * dy),
+                                  (phi_a[cidx(g, x, y, z + 1)] - phi_a[cidx(g, x, y, z - 1)]) / (2.0 * dz)};
+            const double dt = q.dt;
+            const double dot_gv = gp[0] * VPc.c[0][c] + gp[1] * VPc.c[1][c] + gp[2] * VPc.c[2][c];
+            const double t1 = q.rho_f * div_v_p / (q.c_v * dt);
+            const double t2 = q.rho_f * dot_gv / (q.c_v * dt * phi);
+            const double t4 = q.rho_f * dphi / (dt * dt * phi * q.c_v);
+            f1 = t1 + t2 + t4;
+            for (int k = 0; k < 3; ++k) f2[k] = gp[k] / phi;
+            f1a[c] = f1;
+            for (int k = 0; k < 3; ++k) f2a.c[k][c] = f2[k];
+        } else {
+            f1 = f1a[c];
+            for (int k = 0; k < 3; ++k) f2[k] = f2a.c[k][c];
+        }
+        // forcing from A as it stands (its ghost layer from the previous
+        // iteration)
+        const double ge[3] = {(A[cidx(g, x + 1, y, z)] - A[cidx(g, x - 1, y, z)]) / (2.0 * dx),
+                              (A[cidx(g, x, y + 1, z)] - A[cidx(g, x, y - 1, z)]) / (2.0 * dy),
+                              (A[cidx(g, x, y, z + 1)] - A[cidx(g, x, y, z - 1)]) / (2.0 * dz)};
+        const double f = f1 - (ge[0] * f2[0] + ge[1] * f2[1] + ge[2] * f2[2]);
+        const double e = A[c];
+        const int zmin = g.bb == 0 ? 1 : 0;
+        const int zmax = g.bt == 0 ? g.nz - 2 : g.nz - 1;
+        double out = e;
+        if (z >= zmin && z <= zmax) {
+            const double dxdx = dx * dx, dydy = dy * dy, dzdz = dz * dz;
+            const double e_xn = A[ghost_source(g, x - 1, y, z)], e_xp = A[ghost_source(g, x + 1, y, z)];
+            const double e_yn = A[ghost_source(g, x, y - 1, z)], e_yp = A[ghost_source(g, x, y + 1, z)];
+            const double e_zn = A[ghost_source(g, x, y, z - 1)], e_zp = A[ghost_source(g, x, y, z + 1)];
+            const double e_new = (-dxdx * dydy * dzdz * f + dydy * dzdz * (e_xn + e_xp) +
+                                  dxdx * dzdz * (e_yn + e_yp) + dxdx * dydy * (e_zn + e_zp)) /
+                                 (2.0 * (dxdx * dydy + dxdx * dzdz + dydy * dzdz));
+            const double norm = (e_new - e) * (e_new - e) / (e_new * e_new + 1.0e-16);
+            out = e * (1.0 - q.theta) + e_new * q.theta;
+            bits = (unsigned long long)__double_as_longlong(fabs(norm));
+        }
+        B[c] = out;
+        ghost_cell_writes(g, x, y, z, e, B);
+    }
+    if (maxw) {
+        __shared__ unsigned long long part[NS_THREADS / 64];
+        bits = wave_max_u64(bits);
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        if (lane == 0) part[wv] = bits;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long m = part[0];
+            for (int w = 1; w < NS_THREADS / 64; ++w) m = part[w] > m ? part[w] : m;
+            atomicMax(maxw, m);
+        }
+    }
+}
+
+// smoothing<Float>, src/navierstokes.cuh:2619-2677, A -> B (all interior
+// cells written; ghosts follow from a ghost pass)
+__global__ void k_ns_smooth(NsGrid g, double gamma, const double *A, double *B)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    const size_t c = cidx(g, x, y, z);
+    const int zmin = g.bb == 0 ? 1 : 0;
+    const int zmax = g.bt == 0 ? g.nz - 2 : g.nz - 1;
+    double out = A[c];
+    if (z >= zmin && z <= zmax) {
+        const double avg = 1.0 / 6.0 * (A[cidx(g, x - 1, y, z)] + A[cidx(g, x + 1, y, z)] +
+                                        A[cidx(g, x, y - 1, z)] + A[cidx(g, x, y + 1, z)] +
+                                        A[cidx(g, x, y, z - 1)] + A[cidx(g, x, y, z + 1)]);
+        out = (1.0 - gamma) * A[c] + gamma * avg;
+    }
+    B[c] = out;
+}
+
+// updateNSpressure, src/navierstokes.cuh:2881-2913
+__global__ void k_ns_update_p(NsGrid g, const double *eps, double beta, double *p)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    const size_t c = cidx(g, x, y, z);
+    p[c] = beta * p[c] + eps[c];
+}
+
+// updateNSvelocity, src/navierstokes.cuh:2915-3057 (SET_2, no wall)
+__global__ void k_ns_update_v(NsGrid g, NsPhys q, double c_v_ndem_dt, CVec3 VPF, const double *eps,
+                              Vec3 VF)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    const size_t f = fidx(g, x, y, z);
+    const size_t c = cidx(g, x, y, z);
+    const size_t n[3] = {cidx(g, x - 1, y, z), cidx(g, x, y - 1, z), cidx(g, x, y, z - 1)};
+    const double h[3] = {g.dx, g.dy, g.dz};
+    double v[3];
+    for (int k = 0; k < 3; ++k) {
+        const double ge = (eps[c] - eps[n[k]]) / h[k];
+        v[k] = VPF.c[k][f] - c_v_ndem_dt / q.rho_f * ge;
+    }
+    if (z == 0 && g.bb == 1) v[2] = 0.0;
+    if (z == 0 && g.bb == 2) v[0] = v[1] = v[2] = 0.0;
+    for (int k = 0; k < 3; ++k) VF.c[k][f] = v[k];
+}
+
+thread_local std::string g_ns_error;
+
+}  // namespace
+
+struct sph_ns {
+    int device = 0;
+    hipStream_t s = nullptr;
+    sph_ns_params P{};
+    NsGrid g{};
+    NsPhys q{};
+    size_t nc = 0, nf = 0;
+    void *arena = nullptr;
+    double *p = nullptr, *phi = nullptr, *dphi = nullptr, *eps[2] = {nullptr, nullptr}, *f1 = nullptr;
+    Vec3 v{}, vp{}, dpvv{}, f2{}, vf{}, vpf{}, dtau{};
+    unsigned long long *dmax = nullptr;
+    unsigned long long *hmax = nullptr;  // pinned
+    std::vector<double> ph;              // the host copy of p
+    int cur = 0;
+    long iter = 0;
+    double t = 0.0;
+    sph_ns_stats st{};
+};
+
+namespace {
+
+int fail(int code, const std::string &msg)
+{
+    g_ns_error = msg;
+    return code;
+}
+
+int hip_ok(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return SPH_OK;
+    return fail(SPH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+unsigned cell_blocks(const sph_ns *n)
+{
+    const unsigned cells = (unsigned)n->g.nx * (unsigned)n->g.ny * (unsigned)n->g.nz;
+    return (cells + NS_THREADS - 1) / NS_THREADS;
+}
+
+CVec3 cv(const Vec3 &a) { return CVec3{{a.c[0], a.c[1], a.c[2]}}; }
+
+void ghost_cells(sph_ns *n, double *a0, double *a1 = nullptr, double *a2 = nullptr)
+{
+    Vec3 a{{a0, a1, a2}};
+    hipLaunchKernelGGL(k_ns_ghost_cells, dim3(cell_blocks(n)), dim3(NS_THREADS), 0, n->s, n->g, a,
+                       a1 ? 3 : 1);
+}
+
+void ghost_faces(sph_ns *n, const Vec3 &F)
+{
+    hipLaunchKernelGGL(k_ns_ghost_faces, dim3(cell_blocks(n)), dim3(NS_THREADS), 0, n->s, n->g, F);
+}
+
+void face_to_center(sph_ns *n, const Vec3 &F, const Vec3 &c)
+{
+    hipLaunchKernelGGL(k_ns_face_to_center, dim3(cell_blocks(n)), dim3(NS_THREADS), 0, n->s, n->g, cv(F), c);
+}
+
+void set_layers(sph_ns *n, int z0, int z1, double value)
+{
+    hipLaunchKernelGGL(k_ns_set_layers, dim3(cell_blocks(n)), dim3(NS_THREADS), 0, n->s, n->g,
+                       n->eps[n->cur], z0, z1, value);
+}
+
+// max |norm| of the iteration just enqueued with n->dmax; < 0 on a HIP error
+int read_max(sph_ns *n, double *out)
+{
+    int rc = hip_ok(hipMemcpyAsync(n->hmax, n->dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, n->s),
+                    "hipMemcpyAsync");
+    if (rc) return rc;
+    rc = hip_ok(hipStreamSynchronize(n->s), "Navier-Stokes Jacobi iterations");
+    if (rc) return rc;
+    double m;
+    std::memcpy(&m, n->hmax, sizeof m);
+    *out = m;
+    return SPH_OK;
+}
+
+// One fluid solve, src/device.cu:1354-1898
+int ns_solve(sph_ns *n)
+{
+    const sph_ns_params &P = n->P;
+    const NsGrid &g = n->g;
+    const unsigned cb = cell_blocks(n);
+    const double ptop = n->ph[cidx(g, 0, 0, g.nz - 1)];
+    set_layers(n, g.nz - 1, g.nz - 1, ptop - P.beta * ptop);
+    if ((P.p_mod_A > 1.0e-5 || P.p_mod_A < -1.0e-5) && P.p_mod_f > 1.0e-7) {
+        // the reference forms the new top pressure on the host
+        // (src/device.cu:1400-1412)
+        const double newp = n->ph[cidx(g, 0, 0, g.nz - 1)] +
+                            P.p_mod_A * std::sin(2.0 * NS_PI * P.p_mod_f * n->t + P.p_mod_phi);
+        hipLaunchKernelGGL(k_ns_upper_pressure, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->p, n->eps[n->cur],
+                           P.beta, newp);
+    }
+    ghost_cells(n, n->p);
+    ghost_faces(n, n->vpf);
+    ghost_cells(n, n->phi);
+    ghost_cells(n, n->dphi);
+    face_to_center(n, n->vf, n->v);
+    ghost_cells(n, n->v.c[0], n->v.c[1], n->v.c[2]);
+    hipLaunchKernelGGL(k_ns_div_phi_vv, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->phi, cv(n->v), n->dpvv);
+    ghost_cells(n, n->dpvv.c[0], n->dpvv.c[1], n->dpvv.c[2]);
+    hipLaunchKernelGGL(k_ns_predict, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->q, n->p, cv(n->vf), n->phi,
+                       n->dphi, cv(n->dtau), cv(n->dpvv), n->vpf);
+    ghost_faces(n, n->vpf);
+    face_to_center(n, n->vpf, n->vp);
+    if (n->iter == 0) {
+        const double p2 = n->ph[cidx(g, 2, 2, 2)];
+        set_layers(n, 1, g.nz - 2, p2 - P.beta * p2);
+        const double p0 = n->ph[cidx(g, 0, 0, 0)];
+        set_layers(n, 0, 0, p0 - P.beta * p0);
+    }
+    // Jacobi iterations: iteration 0 alone, then batches ending at each
+    // tenth iteration (the reference's test, src/device.cu:1763-1790)
+    double max_norm = 0.0;
+    unsigned nijac = 0;
+    bool converged = false;
+    while (nijac < P.maxiter && !converged) {
+        const unsigned last = nijac == 0 ? 0u : std::min<unsigned>(P.maxiter - 1, (nijac + 9) / 10 * 10);
+        bool checked = false;
+        for (unsigned it = nijac; it <= last; ++it) {
+            const bool check = it % 10 == 0;
+            if (check) {
+                int rc = hip_ok(hipMemsetAsync(n->dmax, 0, sizeof(unsigned long long), n->s), "hipMemsetAsync");
+                if (rc) return rc;
+            }
+            hipLaunchKernelGGL(k_ns_jacobi, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->q, it, n->eps[n->cur],
+                               n->eps[1 - n->cur], n->f1, n->f2, cv(n->vp), cv(n->vpf), n->phi, n->dphi,
+                               check ? n->dmax : nullptr);
+            n->cur = 1 - n->cur;
+            checked = check;
+        }
+        nijac = last + 1;
+        if (checked) {
+            int rc = read_max(n, &max_norm);
+            if (rc) return rc;
+            if (max_norm != max_norm) return fail(SPH_ERR_NAN, "normalized residual is NaN");
+            if (max_norm < P.tolerance) converged = true;
+        }
+    }
+    n->st.solves++;
+    n->st.iterations += nijac;
+    n->st.last_nijac = nijac - 1;
+    n->st.last_max_norm = max_norm;
+    if (converged) {
+        ghost_cells(n, n->eps[n->cur]);
+        if (P.gamma > 0.0) {
+            hipLaunchKernelGGL(k_ns_smooth, dim3(cb), dim3(NS_THREADS), 0, n->s, g, P.gamma, n->eps[n->cur],
+                               n->eps[1 - n->cur]);
+            n->cur = 1 - n->cur;
+            ghost_cells(n, n->eps[n->cur]);
+        }
+    } else {
+        n->st.nonconverged++;
+    }
+    hipLaunchKernelGGL(k_ns_update_p, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->eps[n->cur], P.beta, n->p);
+    const double c_v_ndem_dt = P.c_v * P.ndem * P.dt;
+    hipLaunchKernelGGL(k_ns_update_v, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->q, c_v_ndem_dt, cv(n->vpf),
+                       n->eps[n->cur], n->vf);
+    ghost_faces(n, n->vpf);
+    face_to_center(n, n->vf, n->v);
+    return hip_ok(hipGetLastError(), "Navier-Stokes kernels");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sph_ns_last_error(void) { return g_ns_error.c_str(); }
+
+sph_ns *sph_ns_create(int device, const sph_ns_params *params, const double *v, const double *p)
+{
+    if (!params || !v || !p) {
+        fail(SPH_ERR_ARG, "null argument");
+        return nullptr;
+    }
+    const sph_ns_params &P = *params;
+    if (P.nx < 3 || P.ny < 3 || P.nz < 3 || P.nx > 4096 || P.ny > 4096 || P.nz > 4096 ||
+        (uint64_t)P.nx * P.ny * P.nz > (1ull << 28)) {
+        fail(SPH_ERR_ARG, "each grid dimension must be 3..4096 (at most 2^28 cells)");
+        return nullptr;
+    }
+    if (P.ndem == 0 || P.maxiter == 0 || P.bc_bot < 0 || P.bc_bot > 2 || P.bc_top < 0 || P.bc_top > 2 ||
+        !(P.dt > 0.0) || !(P.rho_f > 0.0) || !(P.L[0] > 0.0) || !(P.L[1] > 0.0) || !(P.L[2] > 0.0)) {
+        fail(SPH_ERR_ARG, "ndem and maxiter must be positive, bc_bot and bc_top 0, 1 or 2, "
+                          "dt, rho_f and L positive");
+        return nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        fail(SPH_ERR_HIP, "no HIP device available");
+        return nullptr;
+    }
+    if (device < 0) device = 0;
+    if (device >= ndev || hipSetDevice(device) != hipSuccess) {
+        fail(SPH_ERR_HIP, "device index out of range");
+        return nullptr;
+    }
+    sph_ns *n = new sph_ns();
+    n->device = device;
+    n->P = P;
+    n->g = NsGrid{(int)P.nx, (int)P.ny, (int)P.nz, P.L[0] / P.nx, P.L[1] / P.ny, P.L[2] / P.nz,
+                  P.bc_bot, P.bc_top};
+    n->q = NsPhys{P.ndem * P.dt, P.mu, P.rho_f, P.beta, P.theta, P.c_v, {P.g[0], P.g[1], P.g[2]}};
+    n->nc = (size_t)(P.nx + 2) * (P.ny + 2) * (P.nz + 2);
+    n->nf = (size_t)(P.nx + 3) * (P.ny + 3) * (P.nz + 3);
+    const size_t ncell_arrays = 18, nface_arrays = 9;
+    const size_t bytes = (ncell_arrays * n->nc + nface_arrays * n->nf) * sizeof(double) + 256;
+    if (hipStreamCreateWithFlags(&n->s, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&n->arena, bytes) != hipSuccess ||
+        hipHostMalloc((void **)&n->hmax, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        fail(SPH_ERR_HIP, "device allocation failed");
+        sph_ns_destroy(n);
+        return nullptr;
+    }
+    if (hipMemsetAsync(n->arena, 0, bytes, n->s) != hipSuccess) {
+        fail(SPH_ERR_HIP, "hipMemsetAsync failed");
+        sph_ns_destroy(n);
+        return nullptr;
+    }
+    double *a = (double *)n->arena;
+    auto take_c = [&]() { double *r = a; a += n->nc; return r; };
+    auto take_f = [&]() { double *r = a; a += n->nf; return r; };
+    n->p = take_c();
+    n->phi = take_c();
+    n->dphi = take_c();
+    n->eps[0] = take_c();
+    n->eps[1] = take_c();
+    n->f1 = take_c();
+    for (int k = 0; k < 3; ++k) {
+        n->v.c[k] = take_c();
+        n->vp.c[k] = take_c();
+        n->dpvv.c[k] = take_c();
+        n->f2.c[k] = take_c();
+    }
+    for (int k = 0; k < 3; ++k) {
+        n->vf.c[k] = take_f();
+        n->vpf.c[k] = take_f();
+        n->dtau.c[k] = take_f();
+    }
+    n->dmax = (unsigned long long *)a;
+    // interior values: p into the host copy and the device, v into the
+    // cell-centred velocity
+    n->ph.assign(n->nc, 0.0);
+    std::vector<double> vh[3];
+    for (int k = 0; k < 3; ++k) vh[k].assign(n->nc, 0.0);
+    for (uint32_t z = 0; z < P.nz; ++z)
+        for (uint32_t y = 0; y < P.ny; ++y)
+            for (uint32_t x = 0; x < P.nx; ++x) {
+                const size_t i = x + (size_t)P.nx * (y + (size_t)P.ny * z);
+                const size_t c = cidx(n->g, (int)x, (int)y, (int)z);
+                n->ph[c] = p[i];
+                for (int k = 0; k < 3; ++k) vh[k][c] = v[3 * i + k];
+            }
+    bool ok = hipMemcpyAsync(n->p, n->ph.data(), n->nc * sizeof(double), hipMemcpyHostToDevice, n->s) == hipSuccess;
+    for (int k = 0; k < 3 && ok; ++k)
+        ok = hipMemcpyAsync(n->v.c[k], vh[k].data(), n->nc * sizeof(double), hipMemcpyHostToDevice, n->s) ==
+             hipSuccess;
+    if (!ok || hipStreamSynchronize(n->s) != hipSuccess) {
+        fail(SPH_ERR_HIP, "upload failed");
+        sph_ns_destroy(n);
+        return nullptr;
+    }
+    n->t = P.t0;
+    return n;
+}
+
+void sph_ns_destroy(sph_ns *n)
+{
+    if (!n) return;
+    if (n->s) hipStreamSynchronize(n->s);
+    if (n->arena) hipFree(n->arena);
+    if (n->hmax) hipHostFree(n->hmax);
+    if (n->s) hipStreamDestroy(n->s);
+    delete n;
+}
+
+int sph_ns_step(sph_ns *n, int nsteps)
+{
+    if (!n || nsteps < 0) return fail(SPH_ERR_ARG, "bad argument");
+    if (hipSetDevice(n->device) != hipSuccess) return fail(SPH_ERR_HIP, "hipSetDevice failed");
+    const NsGrid &g = n->g;
+    const unsigned cb = cell_blocks(n);
+    for (int s = 0; s < nsteps; ++s) {
+        hipLaunchKernelGGL(k_ns_porosity, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->phi, n->dphi);
+        if (n->iter == 0) {
+            ghost_cells(n, n->v.c[0], n->v.c[1], n->v.c[2]);
+            const unsigned faces = (unsigned)(g.nx + 1) * (unsigned)(g.ny + 1) * (unsigned)(g.nz + 1);
+            hipLaunchKernelGGL(k_ns_center_to_face, dim3((faces + NS_THREADS - 1) / NS_THREADS), dim3(NS_THREADS),
+                               0, n->s, g, cv(n->v), n->vf);
+        }
+        ghost_faces(n, n->vf);
+        hipLaunchKernelGGL(k_ns_div_tau, dim3(cb), dim3(NS_THREADS), 0, n->s, g, cv(n->vf), n->P.mu, n->dtau);
+        ghost_faces(n, n->dtau);
+        ghost_cells(n, n->p);
+        ghost_cells(n, n->phi);
+        if (n->iter % n->P.ndem == 0) {
+            const int rc = ns_solve(n);
+            if (rc) return rc;
+        }
+        n->t += n->P.dt;
+        n->iter++;
+        n->st.steps++;
+    }
+    const int rc = hip_ok(hipGetLastError(), "Navier-Stokes kernels");
+    if (rc) return rc;
+    return hip_ok(hipStreamSynchronize(n->s), "Navier-Stokes step");
+}
+
+int sph_ns_read(sph_ns *n, double *v, double *p, double *phi, double *dphi)
+{
+    if (!n) return fail(SPH_ERR_ARG, "null context");
+    if (hipSetDevice(n->device) != hipSuccess) return fail(SPH_ERR_HIP, "hipSetDevice failed");
+    std::vector<double> vh[3], phih(n->nc), dphih(n->nc);
+    bool ok = hipMemcpyAsync(n->ph.data(), n->p, n->nc * sizeof(double), hipMemcpyDeviceToHost, n->s) ==
+              hipSuccess;
+    for (int k = 0; k < 3 && ok; ++k) {
+        vh[k].resize(n->nc);
+        ok = hipMemcpyAsync(vh[k].data(), n->v.c[k], n->nc * sizeof(double), hipMemcpyDeviceToHost, n->s) ==
+             hipSuccess;
+    }
+    ok = ok && hipMemcpyAsync(phih.data(), n->phi, n->nc * sizeof(double), hipMemcpyDeviceToHost, n->s) ==
+                   hipSuccess;
+    ok = ok && hipMemcpyAsync(dphih.data(), n->dphi, n->nc * sizeof(double), hipMemcpyDeviceToHost, n->s) ==
+                   hipSuccess;
+    if (!ok || hipStreamSynchronize(n->s) != hipSuccess) return fail(SPH_ERR_HIP, "download failed");
+    const sph_ns_params &P = n->P;
+    for (uint32_t z = 0; z < P.nz; ++z)
+        for (uint32_t y = 0; y < P.ny; ++y)
+            for (uint32_t x = 0; x < P.nx; ++x) {
+                const size_t i = x + (size_t)P.nx * (y + (size_t)P.ny * z);
+                const size_t c = cidx(n->g, (int)x, (int)y, (int)z);
+                if (v)
+                    for (int k = 0; k < 3; ++k) v[3 * i + k] = vh[k][c];
+                if (p) p[i] = n->ph[c];
+                if (phi) phi[i] = phih[c];
+                if (dphi) dphi[i] = dphih[c];
+            }
+    return SPH_OK;
+}
+
+int sph_ns_read_field(sph_ns *n, int which, double *dst)
+{
+    if (!n || !dst || which < 0 || which > 3) return fail(SPH_ERR_ARG, "bad argument");
+    if (hipSetDevice(n->device) != hipSuccess) return fail(SPH_ERR_HIP, "hipSetDevice failed");
+    const double *src = which < 3 ? n->vf.c[which] : n->eps[n->cur];
+    const size_t count = which < 3 ? n->nf : n->nc;
+    if (hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, n->s) != hipSuccess ||
+        hipStreamSynchronize(n->s) != hipSuccess)
+        return fail(SPH_ERR_HIP, "download failed");
+    return SPH_OK;
+}
+
+int sph_ns_get_stats(const sph_ns *n, sph_ns_stats *out)
+{
+    if (!n || !out) return fail(SPH_ERR_ARG, "bad argument");
+    *out = n->st;
+    return SPH_OK;
+}
+
+}  // extern "C"

@@ -46,7 +46,7 @@ def test_ctypes_signatures_cover_header():
 def struct_fields(name):
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    m = re.search(r"typedef struct\s*(?:%s\s*)?\{(.*?)\}\s*%s\s*;" % (name, name), text,
+    m = re.search(r"typedef struct\s*(?:%s\s*)?\{([^{}]*)\}\s*%s\s*;" % (name, name), text,
                   flags=re.S)
     assert m, name
     fields = []
@@ -65,7 +65,9 @@ def struct_fields(name):
 @pytest.mark.parametrize("cname,pyclass", [("sph_state", _abi.SphState),
                                            ("sph_run_opts", _abi.SphRunOpts),
                                            ("sph_darcy_stats", _abi.SphDarcyStats),
-                                           ("sph_neighbor_stats", _abi.SphNeighborStats)])
+                                           ("sph_neighbor_stats", _abi.SphNeighborStats),
+                                           ("sph_ns_params", _abi.SphNsParams),
+                                           ("sph_ns_stats", _abi.SphNsStats)])
 def test_struct_field_order(cname, pyclass):
     assert struct_fields(cname) == [f[0] for f in pyclass._fields_]
 

@@ -1,0 +1,142 @@
+"""The Navier-Stokes fluid step (sph_ns_*, cfd_solver = 0, no grains) on
+the GPU against the CPU oracle (oracle/ns_oracle.c): bit-identical cell
+velocities, pressures, face velocities, epsilon and Jacobi counts after
+every step, over the boundary combinations, projection variants (beta),
+relaxation, smoothing, pressure modulation, ndem > 1, a solve that reaches
+maxiter, and a read between steps (which refreshes the host copy of p the
+step reads)."""
+import numpy
+import pytest
+
+import sphere_amd
+from sphere_amd import _abi
+from _ns_oracle import NsOracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.skip("no GPU")
+    return t
+
+
+def _case(shape, seed, amp=0.02):
+    rng = numpy.random.default_rng(seed)
+    nx, ny, nz = shape
+    X, Y, Z = numpy.meshgrid(*[(numpy.arange(n) + 0.5) / n for n in shape], indexing="ij")
+    v = numpy.zeros(shape + (3,))
+    v[..., 0] = amp * numpy.sin(2 * numpy.pi * Y) + 0.3 * amp * rng.standard_normal(shape)
+    v[..., 1] = amp * numpy.cos(2 * numpy.pi * X) * numpy.sin(numpy.pi * Z)
+    v[..., 2] = 0.5 * amp * numpy.sin(2 * numpy.pi * X) + 0.3 * amp * rng.standard_normal(shape)
+    p = 500.0 + 100.0 * rng.standard_normal(shape)
+    return v, p
+
+
+CASES = [
+    # bc_bot, bc_top, extra parameters
+    (1, 0, dict()),
+    (0, 0, dict(beta=0.4)),
+    (2, 0, dict(gamma=0.25, theta=0.8)),
+    (1, 1, dict(maxiter=35)),
+    (2, 2, dict(mu=0.0)),
+    (0, 1, dict(ndem=2)),
+    (1, 2, dict(p_mod_A=80.0, p_mod_f=40.0, p_mod_phi=0.3, beta=0.2)),
+    (1, 0, dict(p_mod_A=50.0, p_mod_f=25.0, g=(0.3, -0.2, -9.81))),
+]
+
+
+@pytest.mark.parametrize("k", range(len(CASES)))
+def test_ns_matches_oracle(torch, k):
+    bb, bt, extra = CASES[k]
+    shape = (9, 7, 10)
+    L = (0.18, 0.21, 0.15)
+    kw = dict(mu=1.0e-3, rho_f=1000.0, g=(0.0, 0.0, -9.81), tolerance=1e-9, maxiter=5000,
+              bc_bot=bb, bc_top=bt, t0=0.125)
+    kw.update(extra)
+    P = _abi.ns_params(*shape, L, 2.0e-4, **kw)
+    v, p = _case(shape, 10 + k)
+    print("ns case", k, bb, bt, extra, flush=True)
+    gpu = sphere_amd.NavierStokes(P, v, p)
+    ora = NsOracle(P, v, p)
+    for step in range(6):
+        gpu.step(1)
+        ora.step(1)
+        sg, so = gpu.stats(), ora.stats()
+        assert (sg.solves, sg.iterations, sg.nonconverged, sg.last_nijac) == \
+            (so.solves, so.iterations, so.nonconverged, so.last_nijac), step
+        assert sg.last_max_norm == so.last_max_norm, step
+        for c in range(3):
+            assert numpy.array_equal(gpu.field(c), ora.field(c)), (step, "faces", c)
+        assert numpy.array_equal(gpu.field(3), ora.field(NsOracle.EPS)), (step, "epsilon")
+        if step in (2, 4):
+            # a read refreshes the host copy of p on both sides
+            a, b = gpu.read(), ora.read()
+            for x, y in zip(a, b):
+                assert numpy.array_equal(x, y), step
+    a, b = gpu.read(), ora.read()
+    for x, y in zip(a, b):
+        assert numpy.array_equal(x, y)
+    assert numpy.all(numpy.isfinite(a[0])) and numpy.all(numpy.isfinite(a[1]))
+    if extra.get("maxiter") == 35:
+        assert sg.nonconverged > 0
+    else:
+        assert sg.nonconverged == 0 and sg.iterations > sg.solves
+    gpu.close()
+
+
+def test_ns_hydrostatic_balance(torch):
+    """12 x 12 x 24 from rest under gravity (closed bottom, fixed top): the
+    GPU equals the oracle bit for bit and reaches hydrostatic balance."""
+    shape = (12, 12, 24)
+    nz = shape[2]
+    L = (0.3, 0.3, 0.6)
+    P = _abi.ns_params(*shape, L, 1.0e-4, mu=1.0e-3, g=(0.0, 0.0, -9.81), bc_bot=1, bc_top=0,
+                       tolerance=1e-24, maxiter=400000)
+    v = numpy.zeros(shape + (3,))
+    p = numpy.zeros(shape)
+    print("ns hydrostatic 12x12x24", flush=True)
+    gpu = sphere_amd.NavierStokes(P, v, p)
+    gpu.step(1)
+    ora = NsOracle(P, v, p)
+    ora.step(1)
+    a, b = gpu.read(), ora.read()
+    for x, y in zip(a, b):
+        assert numpy.array_equal(x, y)
+    assert gpu.stats().iterations == ora.stats().iterations
+    dz = L[2] / nz
+    exp = 1000.0 * 9.81 * dz * (nz - 1 - numpy.arange(nz))
+    assert numpy.max(numpy.abs(a[1] - exp)) <= 1e-8 * exp.max()
+    assert numpy.max(numpy.abs(a[0])) <= 1e-8 * 9.81 * 1e-4
+    gpu.close()
+
+
+def test_ns_consolidation_grid(torch):
+    """The consolidation box's grid, 48 x 48 x 96: two steps from a random
+    velocity field, bit for bit (the solves stop at maxiter = 600)."""
+    shape = (48, 48, 96)
+    L = (1.1088, 1.1088, 2.2176)
+    P = _abi.ns_params(*shape, L, 1.0e-4, mu=1.0e-3, g=(0.0, 0.0, -9.81), bc_bot=1, bc_top=0,
+                       tolerance=1e-14, maxiter=600)
+    v, p = _case(shape, 77)
+    print("ns 48x48x96", flush=True)
+    gpu = sphere_amd.NavierStokes(P, v, p)
+    ora = NsOracle(P, v, p)
+    gpu.step(2)
+    ora.step(2)
+    a, b = gpu.read(), ora.read()
+    for x, y in zip(a, b):
+        assert numpy.array_equal(x, y)
+    for c in range(3):
+        assert numpy.array_equal(gpu.field(c), ora.field(c))
+    sg, so = gpu.stats(), ora.stats()
+    assert (sg.iterations, sg.nonconverged) == (so.iterations, so.nonconverged)
+    gpu.close()
+
+
+def test_ns_rejects_bad_parameters():
+    P = _abi.ns_params(2, 8, 8, (0.1, 0.1, 0.1), 1e-4)
+    with pytest.raises(sphere_amd.SphereError):
+        sphere_amd.NavierStokes(P, numpy.zeros((2, 8, 8, 3)), numpy.zeros((2, 8, 8)))
