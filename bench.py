@@ -726,6 +726,8 @@ def main():
     ap.add_argument("--net-corrections", type=int, default=8, help="corrections per solve at most")
     ap.add_argument("--darcy-save", default="", help="write the prepared consolidation box here")
     ap.add_argument("--darcy-load", default="", help="start from a box written by --darcy-save")
+    ap.add_argument("--no-ns", action="store_true", help="skip the Navier-Stokes fluid section")
+    ap.add_argument("--ns-steps", type=int, default=20, help="timed Navier-Stokes steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="finish with three device-to-device copies of MIB MiB, the "
                          "known-size kernel that scales HBM counters")
@@ -751,6 +753,13 @@ def main():
         print(json.dumps({"metric": d["metric"], "value": d["together"], "unit": "particle-steps/s",
                           "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
                           "data": "synthetic", "ensemble": d}), flush=True)
+        return
+    if args.no_shear and args.no_darcy:
+        d = navier_stokes_section(args)
+        print(json.dumps({"metric": "ms per Navier-Stokes fluid step (48 x 48 x 96, no grains)",
+                          "value": d["ms_per_step"], "unit": "ms", "n_gpus": 1,
+                          "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+                          "navier_stokes": d}), flush=True)
         return
     if args.no_shear:
         if args.prep_only:
@@ -888,9 +897,73 @@ def main():
     engine.close()
     line["darcy"] = None if args.no_darcy else darcy_child()
     line["ensemble"] = None if args.no_ensemble else ensemble_section(args)
+    line["navier_stokes"] = None if args.no_ns else navier_stokes_section(args)
     print(json.dumps(line), flush=True)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+
+
+def navier_stokes_section(args):
+    """The Navier-Stokes fluid step (cfd_solver = 0) without grains on the
+    consolidation box's 48 x 48 x 96 grid: a water column over a closed
+    bottom under a fixed-pressure top, from a hydrostatic pressure and a
+    random velocity perturbation; --ns-steps timed steps after 2 untimed
+    ones, and the CPU oracle (one thread) on the first two steps."""
+    import sphere_amd
+    from sphere_amd import _abi
+    shape = (48, 48, 96)
+    L = (1.1088, 1.1088, 2.2176)
+    dt = 1.0e-4
+    P = _abi.ns_params(*shape, L, dt, mu=8.9e-4, rho_f=1000.0, g=(0.0, 0.0, -9.81), bc_bot=1,
+                       bc_top=0, tolerance=1.0e-8, maxiter=100000)
+    rng = numpy.random.default_rng(11)
+    v = 0.01 * rng.standard_normal(shape + (3,))
+    dz = L[2] / shape[2]
+    p = numpy.broadcast_to(1000.0 * 9.81 * dz * (shape[2] - 1 - numpy.arange(shape[2])),
+                           shape).copy()
+    ns = sphere_amd.NavierStokes(P, v, p)
+    ns.step(2)
+    a = ns.stats()
+    t0 = time.perf_counter()
+    ns.step(args.ns_steps)
+    el = time.perf_counter() - t0
+    b = ns.stats()
+    ns.close()
+    iters = (b.iterations - a.iterations) / float(args.ns_steps)
+    cells = shape[0] * shape[1] * shape[2]
+    out = {"workload": "Navier-Stokes water column without grains, %d x %d x %d cells (the "
+                       "consolidation box's grid), closed bottom, fixed-pressure top, hydrostatic "
+                       "start with a 0.01 m/s random velocity perturbation, dt 1e-4 s, Jacobi "
+                       "tolerance 1e-8" % shape,
+           "steps": args.ns_steps, "ms_per_step": 1000.0 * el / args.ns_steps,
+           "jacobi_iterations_per_solve": iters,
+           "us_per_jacobi_iteration": 1e6 * el / max(1.0, b.iterations - a.iterations),
+           "nonconverged_timed": int(b.nonconverged - a.nonconverged)}
+    log("Navier-Stokes 48x48x96 without grains: %.3f ms per step, %.0f Jacobi iterations per "
+        "solve, %.2f us per iteration" % (out["ms_per_step"], iters, out["us_per_jacobi_iteration"]))
+    if not args.no_cpu:
+        # the oracle's cost per Jacobi iteration: the first step of the same
+        # column with at most 300 iterations (one thread)
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from _ns_oracle import NsOracle
+        Pc = _abi.ns_params(*shape, L, dt, mu=8.9e-4, rho_f=1000.0, g=(0.0, 0.0, -9.81), bc_bot=1,
+                            bc_top=0, tolerance=1.0e-8, maxiter=300)
+        o = NsOracle(Pc, v, p)
+        t0 = time.perf_counter()
+        o.step(1)
+        cel = time.perf_counter() - t0
+        st = o.stats()
+        o.close()
+        us_it = 1e6 * cel / st.iterations
+        out["cpu_oracle"] = {"us_per_jacobi_iteration": us_it, "cores": 1, "kind": "port",
+                             "ms_per_step_at_gpu_iterations": us_it * iters / 1000.0,
+                             "sample": "the first step of the same column on the FP64 C oracle "
+                                       "(one thread), stopped at 300 Jacobi iterations (%.1f s)"
+                                       % cel}
+        log("Navier-Stokes on the CPU oracle: %.0f us per Jacobi iteration (one thread), %.0f ms "
+            "per step at the GPU's iteration count" % (us_it, us_it * iters / 1000.0))
+    out["cells"] = cells
+    return out
 
 
 def darcy_child():

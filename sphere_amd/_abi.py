@@ -517,6 +517,24 @@ def _one_value(a):
     return numpy.asarray(a).ravel()[0]
 
 
+def ns_params_from_sim(s):
+    """sph_ns_params of a sim with a Navier-Stokes fluid (cfd_solver = 0),
+    no grains and no walls: its grid, time step and time, fluid and solver
+    fields (the ones the reference's binary file carries)."""
+    if not s.fluid or int(_one_value(s.cfd_solver)) != 0:
+        raise SphereError(SPH_ERR_ARG, "the sim has no Navier-Stokes fluid (cfd_solver = 0)")
+    if int(_one_value(s.np)) != 0 or int(_one_value(s.nw)) != 0:
+        raise SphereError(SPH_ERR_ARG, "only a fluid without grains and walls is built")
+    return ns_params(*[int(n) for n in s.num], L=s.L, dt=_one_value(s.time_dt),
+                     mu=_one_value(s.mu), rho_f=_one_value(s.rho_f), g=s.g,
+                     beta=_one_value(s.beta), theta=_one_value(s.theta), gamma=_one_value(s.gamma),
+                     tolerance=_one_value(s.tolerance), c_v=_one_value(s.c_v),
+                     maxiter=_one_value(s.maxiter), ndem=_one_value(s.ndem),
+                     bc_bot=_one_value(s.bc_bot), bc_top=_one_value(s.bc_top),
+                     p_mod_A=_one_value(s.p_mod_A), p_mod_f=_one_value(s.p_mod_f),
+                     p_mod_phi=_one_value(s.p_mod_phi), t0=_one_value(s.time_current))
+
+
 class NavierStokes:
     """A Navier-Stokes fluid without grains (sph_ns_*): the reference's
     cfd_solver = 0 fluid step on the GPU.  Arrays are numpy, interior cells
@@ -536,19 +554,7 @@ class NavierStokes:
     @classmethod
     def from_sim(cls, s, device=0, lib=None):
         """The fluid of a sim with cfd_solver = 0, no grains and no walls."""
-        if int(_one_value(s.cfd_solver)) != 0 or not s.fluid:
-            raise SphereError(SPH_ERR_ARG, "the sim has no Navier-Stokes fluid (cfd_solver = 0)")
-        if int(_one_value(s.np)) != 0 or int(_one_value(s.nw)) != 0:
-            raise SphereError(SPH_ERR_ARG, "only a fluid without grains and walls is built")
-        P = ns_params(*[int(n) for n in s.num], L=s.L, dt=_one_value(s.time_dt),
-                      mu=_one_value(s.mu), rho_f=_one_value(s.rho_f), g=s.g,
-                      beta=_one_value(s.beta), theta=_one_value(s.theta), gamma=_one_value(s.gamma),
-                      tolerance=_one_value(s.tolerance), c_v=_one_value(s.c_v),
-                      maxiter=_one_value(s.maxiter), ndem=_one_value(s.ndem),
-                      bc_bot=_one_value(s.bc_bot), bc_top=_one_value(s.bc_top),
-                      p_mod_A=_one_value(s.p_mod_A), p_mod_f=_one_value(s.p_mod_f),
-                      p_mod_phi=_one_value(s.p_mod_phi), t0=_one_value(s.time_current))
-        return cls(P, s.v_f, s.p_f, device=device, lib=lib)
+        return cls(ns_params_from_sim(s), s.v_f, s.p_f, device=device, lib=lib)
 
     def _check(self, rc):
         if rc != SPH_OK:

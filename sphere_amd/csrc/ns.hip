@@ -15,9 +15,11 @@
 //     the reference's eps holds at the next iteration's forcing;
 //   - the max normalized residual of every tenth iteration is reduced on
 //     the device (the bit patterns of non-negative doubles order like their
-//     values; a NaN orders above every number) and read by the host once per
-//     ten iterations, where the reference copies the whole field
-//     (src/navierstokes.cpp:275-297).
+//     values; a NaN orders above every number) and tested there against the
+//     tolerance, where the reference copies the whole field to the host
+//     (src/navierstokes.cpp:275-297).  Once a test passes, the iterations
+//     already enqueued return at once; the host reads the outcome after
+//     each batch of about 100 iterations.
 // Where the reference's ghost kernel races (bc_bot = 2 or bc_top = 2), the
 // ghost layer below z = 0 keeps the bottom rule's value and the layer above
 // z = nz-1 the top rule's; smoothing reads the field as it was before the
@@ -32,6 +34,7 @@
 #include <vector>
 
 #include "../../include/sphere_hip.h"
+#include "multi.h"
 
 namespace {
 
@@ -58,10 +61,11 @@ __host__ __device__ inline size_t fidx(const NsGrid &g, int x, int y, int z)
 
 __device__ inline double amean(double a, double b) { return (a + b) * 0.5; }
 
-// interior cell of thread i (x fastest); false past the last cell
+// interior cell of thread i (x fastest; each XCD takes one contiguous
+// range of blocks); false past the last cell
 __device__ inline bool cell_of(const NsGrid &g, int &x, int &y, int &z)
 {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const unsigned nxy = (unsigned)g.nx * (unsigned)g.ny;
     if (i >= nxy * (unsigned)g.nz) return false;
     z = (int)(i / nxy);
@@ -361,14 +365,23 @@ __device__ inline unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
+// Solve state on the device: done (0 running, 1 converged, 2 NaN), the
+// converged iteration, the running max of the current test and the last
+// tested max (bit patterns).
+struct NsSolveState {
+    unsigned done, conv_nijac;
+    unsigned long long maxw, lastmax;
+};
+
 // One Jacobi iteration: findNSforcing (src/navierstokes.cuh:2497-2617),
 // setNSghostNodes(epsilon), jacobiIterationNS (:2679-2792, no wall) and
 // copyValues (:2795-2827), A -> B, as described at the top of this file.
-// maxw != nullptr: the iteration's max |norm| goes there (bit pattern).
+// check: the iteration's max |norm| goes into st->maxw (bit pattern).
 __global__ void __launch_bounds__(NS_THREADS)
 k_ns_jacobi(NsGrid g, NsPhys q, unsigned nijac, const double *A, double *B, double *f1a, Vec3 f2a,
-            CVec3 VPc, CVec3 VPF, const double *phi_a, const double *dphi_a, unsigned long long *maxw)
+            CVec3 VPc, CVec3 VPF, const double *phi_a, const double *dphi_a, NsSolveState *st, int check)
 {
+    if (*(volatile unsigned *)&st->done) return;
     int x, y, z;
     unsigned long long bits = 0ull;
     if (cell_of(g, x, y, z)) {
@@ -421,7 +434,7 @@ k_ns_jacobi(NsGrid g, NsPhys q, unsigned nijac, const double *A, double *B, doub
         B[c] = out;
         ghost_cell_writes(g, x, y, z, e, B);
     }
-    if (maxw) {
+    if (check) {
         __shared__ unsigned long long part[NS_THREADS / 64];
         bits = wave_max_u64(bits);
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -430,8 +443,25 @@ k_ns_jacobi(NsGrid g, NsPhys q, unsigned nijac, const double *A, double *B, doub
         if (threadIdx.x == 0) {
             unsigned long long m = part[0];
             for (int w = 1; w < NS_THREADS / 64; ++w) m = part[w] > m ? part[w] : m;
-            atomicMax(maxw, m);
+            atomicMax(&st->maxw, m);
         }
+    }
+}
+
+// The test after iteration nijac (src/device.cu:1763-1790): converged when
+// the max is below the tolerance; a NaN ends the solve with an error.
+__global__ void k_ns_decide(NsSolveState *st, unsigned nijac, double tolerance)
+{
+    if (threadIdx.x != 0 || st->done) return;
+    const unsigned long long m = st->maxw;
+    st->lastmax = m;
+    st->maxw = 0ull;
+    const double v = __longlong_as_double((long long)m);
+    if (v != v) {
+        st->done = 2u;
+    } else if (v < tolerance) {
+        st->done = 1u;
+        st->conv_nijac = nijac;
     }
 }
 
@@ -497,8 +527,8 @@ struct sph_ns {
     void *arena = nullptr;
     double *p = nullptr, *phi = nullptr, *dphi = nullptr, *eps[2] = {nullptr, nullptr}, *f1 = nullptr;
     Vec3 v{}, vp{}, dpvv{}, f2{}, vf{}, vpf{}, dtau{};
-    unsigned long long *dmax = nullptr;
-    unsigned long long *hmax = nullptr;  // pinned
+    NsSolveState *dst = nullptr;
+    NsSolveState *hst = nullptr;  // pinned
     std::vector<double> ph;              // the host copy of p
     int cur = 0;
     long iter = 0;
@@ -551,18 +581,13 @@ void set_layers(sph_ns *n, int z0, int z1, double value)
                        n->eps[n->cur], z0, z1, value);
 }
 
-// max |norm| of the iteration just enqueued with n->dmax; < 0 on a HIP error
-int read_max(sph_ns *n, double *out)
+// the device's solve state after the iterations enqueued so far
+int read_state(sph_ns *n)
 {
-    int rc = hip_ok(hipMemcpyAsync(n->hmax, n->dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, n->s),
+    int rc = hip_ok(hipMemcpyAsync(n->hst, n->dst, sizeof(NsSolveState), hipMemcpyDeviceToHost, n->s),
                     "hipMemcpyAsync");
     if (rc) return rc;
-    rc = hip_ok(hipStreamSynchronize(n->s), "Navier-Stokes Jacobi iterations");
-    if (rc) return rc;
-    double m;
-    std::memcpy(&m, n->hmax, sizeof m);
-    *out = m;
-    return SPH_OK;
+    return hip_ok(hipStreamSynchronize(n->s), "Navier-Stokes Jacobi iterations");
 }
 
 // One fluid solve, src/device.cu:1354-1898
@@ -599,34 +624,37 @@ int ns_solve(sph_ns *n)
         const double p0 = n->ph[cidx(g, 0, 0, 0)];
         set_layers(n, 0, 0, p0 - P.beta * p0);
     }
-    // Jacobi iterations: iteration 0 alone, then batches ending at each
-    // tenth iteration (the reference's test, src/device.cu:1763-1790)
-    double max_norm = 0.0;
-    unsigned nijac = 0;
+    // Jacobi iterations in batches of about 100, each ending at a test
+    // (src/device.cu:1763-1790); the device skips what follows a passed test
+    int rc = hip_ok(hipMemsetAsync(n->dst, 0, sizeof(NsSolveState), n->s), "hipMemsetAsync");
+    if (rc) return rc;
+    const int cur0 = n->cur;
+    unsigned next = 0;
     bool converged = false;
-    while (nijac < P.maxiter && !converged) {
-        const unsigned last = nijac == 0 ? 0u : std::min<unsigned>(P.maxiter - 1, (nijac + 9) / 10 * 10);
-        bool checked = false;
-        for (unsigned it = nijac; it <= last; ++it) {
-            const bool check = it % 10 == 0;
-            if (check) {
-                int rc = hip_ok(hipMemsetAsync(n->dmax, 0, sizeof(unsigned long long), n->s), "hipMemsetAsync");
-                if (rc) return rc;
-            }
-            hipLaunchKernelGGL(k_ns_jacobi, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->q, it, n->eps[n->cur],
-                               n->eps[1 - n->cur], n->f1, n->f2, cv(n->vp), cv(n->vpf), n->phi, n->dphi,
-                               check ? n->dmax : nullptr);
-            n->cur = 1 - n->cur;
-            checked = check;
+    while (next < P.maxiter) {
+        const unsigned last = std::min<unsigned>(P.maxiter - 1, (next + 99) / 10 * 10);
+        for (unsigned it = next; it <= last; ++it) {
+            const int check = it % 10 == 0;
+            const int a = cur0 ^ (int)(it & 1u);
+            hipLaunchKernelGGL(k_ns_jacobi, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->q, it, n->eps[a],
+                               n->eps[1 - a], n->f1, n->f2, cv(n->vp), cv(n->vpf), n->phi, n->dphi, n->dst,
+                               check);
+            if (check)
+                hipLaunchKernelGGL(k_ns_decide, dim3(1), dim3(64), 0, n->s, n->dst, it, P.tolerance);
         }
-        nijac = last + 1;
-        if (checked) {
-            int rc = read_max(n, &max_norm);
-            if (rc) return rc;
-            if (max_norm != max_norm) return fail(SPH_ERR_NAN, "normalized residual is NaN");
-            if (max_norm < P.tolerance) converged = true;
+        next = last + 1;
+        rc = read_state(n);
+        if (rc) return rc;
+        if (n->hst->done == 2u) return fail(SPH_ERR_NAN, "normalized residual is NaN");
+        if (n->hst->done == 1u) {
+            converged = true;
+            break;
         }
     }
+    const unsigned nijac = converged ? n->hst->conv_nijac + 1 : P.maxiter;
+    n->cur = cur0 ^ (int)(nijac & 1u);
+    double max_norm;
+    std::memcpy(&max_norm, &n->hst->lastmax, sizeof max_norm);
     n->st.solves++;
     n->st.iterations += nijac;
     n->st.last_nijac = nijac - 1;
@@ -697,7 +725,7 @@ sph_ns *sph_ns_create(int device, const sph_ns_params *params, const double *v, 
     const size_t bytes = (ncell_arrays * n->nc + nface_arrays * n->nf) * sizeof(double) + 256;
     if (hipStreamCreateWithFlags(&n->s, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&n->arena, bytes) != hipSuccess ||
-        hipHostMalloc((void **)&n->hmax, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&n->hst, sizeof(NsSolveState), hipHostMallocDefault) != hipSuccess) {
         fail(SPH_ERR_HIP, "device allocation failed");
         sph_ns_destroy(n);
         return nullptr;
@@ -727,7 +755,7 @@ sph_ns *sph_ns_create(int device, const sph_ns_params *params, const double *v, 
         n->vpf.c[k] = take_f();
         n->dtau.c[k] = take_f();
     }
-    n->dmax = (unsigned long long *)a;
+    n->dst = (NsSolveState *)a;
     // interior values: p into the host copy and the device, v into the
     // cell-centred velocity
     n->ph.assign(n->nc, 0.0);
@@ -759,7 +787,7 @@ void sph_ns_destroy(sph_ns *n)
     if (!n) return;
     if (n->s) hipStreamSynchronize(n->s);
     if (n->arena) hipFree(n->arena);
-    if (n->hmax) hipHostFree(n->hmax);
+    if (n->hst) hipHostFree(n->hst);
     if (n->s) hipStreamDestroy(n->s);
     delete n;
 }

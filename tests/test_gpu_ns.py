@@ -140,3 +140,33 @@ def test_ns_rejects_bad_parameters():
     P = _abi.ns_params(2, 8, 8, (0.1, 0.1, 0.1), 1e-4)
     with pytest.raises(sphere_amd.SphereError):
         sphere_amd.NavierStokes(P, numpy.zeros((2, 8, 8, 3)), numpy.zeros((2, 8, 8)))
+
+
+def test_ns_from_sim_and_back(torch, tmp_path):
+    """A sim's Navier-Stokes fluid steps on the GPU and goes back into the
+    sim and its v2.15 file, equal to the oracle's result."""
+    s = sphere_amd.sim(sid="nscolumn", np=0, fluid=True, cfd_solver=0)
+    s.defineWorldBoundaries([0.2, 0.15, 0.3], dx=0.025)
+    s.initFluid(mu=1e-3, cfd_solver=0)
+    s.g[2] = -9.81
+    s.setFluidBottomNoFlow()
+    s.initTemporal(total=1e-2, dt=1e-4)
+    s.setTolerance(1e-12)
+    s.setMaxIterations(100000)
+    rng = numpy.random.default_rng(4)
+    s.v_f[:] = 0.01 * rng.standard_normal(s.v_f.shape)
+    P = _abi.ns_params_from_sim(s)
+    ora = NsOracle(P, s.v_f, s.p_f)
+    ora.step(3)
+    print("ns from sim", flush=True)
+    ns = sphere_amd.NavierStokes.from_sim(s)
+    ns.step(3)
+    ns.to_sim(s)
+    ns.close()
+    v, p, phi, dphi = ora.read()
+    assert numpy.array_equal(s.v_f, v) and numpy.array_equal(s.p_f, p)
+    assert numpy.array_equal(s.phi, phi)
+    s.writebin(folder=str(tmp_path), verbose=False)
+    r = sphere_amd.sim(fluid=True)
+    r.readbin(str(tmp_path / "nscolumn.bin"), verbose=False)
+    assert numpy.array_equal(r.v_f, v) and numpy.array_equal(r.p_f, p)

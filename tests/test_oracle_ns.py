@@ -242,3 +242,29 @@ def test_nonconvergence_is_counted():
     o.step(2)
     st = o.stats()
     assert st.solves == 2 and st.nonconverged == 2 and st.iterations == 50 and st.last_nijac == 24
+
+
+def test_params_from_a_sim():
+    """A sim set up with the reference's calls (Navier-Stokes fluid, no
+    grains) gives the parameters and the [x, y, z] fields the step reads:
+    the oracle run from them reaches hydrostatic balance."""
+    import sphere_amd
+    s = sphere_amd.sim(sid="nscolumn", np=0, fluid=True, cfd_solver=0)
+    s.defineWorldBoundaries([0.1, 0.1, 0.25], dx=0.025)
+    s.initFluid(mu=1e-3, cfd_solver=0)
+    s.g[2] = -9.81
+    s.setFluidBottomNoFlow()
+    s.setFluidTopFixedPressure()
+    s.initTemporal(total=1e-3, dt=1e-4)
+    s.setTolerance(1e-26)
+    s.setMaxIterations(200000)
+    P = _abi.ns_params_from_sim(s)
+    assert (P.nx, P.ny, P.nz) == (4, 4, 10) and P.bc_bot == 1 and P.bc_top == 0
+    assert (P.beta, P.theta, P.gamma, P.c_v, P.ndem) == (0.0, 1.0, 0.0, 1.0, 1)
+    o = NsOracle(P, s.v_f, s.p_f)
+    o.step(1)
+    p = o.read()[1]
+    assert numpy.max(numpy.abs(p - _p_hyd(10))) <= 1e-9 * _p_hyd(10).max()
+    s2 = sphere_amd.sim(sid="x", np=10, fluid=True, cfd_solver=0)
+    with pytest.raises(_abi.SphereError):
+        _abi.ns_params_from_sim(s2)
