@@ -1615,6 +1615,9 @@ class sim:
             os.remove(side)
         if valgrind or cudamemcheck:
             print("Warning: valgrind/cuda-memcheck wrappers are not available; running plainly")
+        if self.fluid and int(numpy.asarray(self.cfd_solver).ravel()[0]) == 0:
+            return _run_navier_stokes("../input/" + self.sid + ".bin", "../output", verbose, dry,
+                                      device)
         lib = _abi.load_library()
         opts = _abi.SphRunOpts()
         inp = ("../input/" + self.sid + ".bin").encode()
@@ -1636,6 +1639,82 @@ class sim:
 
     def cleanup(self):
         cleanup(self)
+
+
+def _run_navier_stokes(path, outdir, verbose, dry, device):
+    """The time loop of src/device.cu:833-2834 for an input file with a
+    Navier-Stokes fluid (cfd_solver = 0) and no grains or walls, on the
+    GPU through NavierStokes: output00000 when the step count is 0, then
+    DEM iterations until time.current passes time.total, the conv.log line
+    of each solve (every tenth iteration's, and every one that reaches
+    maxiter, src/device.cu:1792-1820), and at every file interval the
+    fields, output%05d and status.dat (src/device.cu:2576-2680)."""
+    # the simulation id is the input file's name (src/sphere.cpp:16-95)
+    sid = os.path.basename(path)
+    if sid.endswith(".bin"):
+        sid = sid[:-4]
+    r = sim(sid=sid, fluid=True)
+    r.readbin(path, verbose=False)
+    if dry:
+        print("  - Number of particles: %d\n  - Time step length: %g s\n  - Total time: %g s\n"
+              "  - File output interval: %g s\n  - Fluid grid: %d*%d*%d (Navier-Stokes)"
+              % (int(r.np[0]), r.time_dt[0], r.time_total[0], r.time_file_dt[0], *r.num))
+        return 0
+    try:
+        ns = _abi.NavierStokes.from_sim(r, device=max(0, int(device)))
+    except _abi.SphereError as e:
+        print("Warning: the sphere run returned with status %d: %s" % (e.code, e))
+        return e.code
+
+    def write(name):
+        with open(outdir + "/" + sid + name, "wb") as fh:
+            fh.write(r._bytes())
+
+    def status_file():
+        with open(outdir + "/" + sid + ".status.dat", "w") as fh:
+            fh.write("%2.4e %2.4e %d\n" % (r.time_current[0], 100.0 * r.time_current[0] / r.time_total[0],
+                                            int(r.time_step_count[0])))
+    if int(r.time_step_count[0]) == 0:
+        write(".output00000.bin")
+    status_file()
+    dt = float(r.time_dt[0])
+    maxiter = int(numpy.asarray(r.maxiter).ravel()[0])
+    filetimeclock = 0.0
+    it = 0
+    rc = 0
+    with open(outdir + "/" + sid + "-conv.log", "w") as conv:
+        while r.time_current[0] <= r.time_total[0]:
+            before = ns.stats()
+            try:
+                ns.step(1)
+            except _abi.SphereError as e:
+                print("Warning: the sphere run returned with status %d: %s" % (e.code, e))
+                rc = e.code
+                break
+            after = ns.stats()
+            if after.solves > before.solves:
+                if after.nonconverged > before.nonconverged:
+                    conv.write("%d\t%d\n" % (it + 1, maxiter - 1))
+                    print("Error, the epsilon solution in the fluid calculations did not "
+                          "converge. Try increasing the value of 'ns.maxiter' (%d) or increase "
+                          "'ns.tolerance' (%g)." % (maxiter, numpy.asarray(r.tolerance).ravel()[0]))
+                elif it % 10 == 0:
+                    conv.write("%d\t%d\n" % (it + 1, after.last_nijac))
+            r.time_current[0] += dt
+            filetimeclock += dt
+            it += 1
+            if verbose and it % 10000 == 0:
+                print("\r  Current time: %g/%g s." % (r.time_current[0], r.time_total[0]), end="")
+            if filetimeclock >= r.time_file_dt[0]:
+                ns.to_sim(r)
+                r.time_step_count[0] += 1
+                write(".output%05d.bin" % int(r.time_step_count[0]))
+                status_file()
+                filetimeclock = 0.0
+    ns.close()
+    if verbose:
+        print()
+    return rc
 
 
 def run(binary, verbose=True, hideinputfile=False, fluid=False, device=-1):

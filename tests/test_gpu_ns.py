@@ -170,3 +170,87 @@ def test_ns_from_sim_and_back(torch, tmp_path):
     r = sphere_amd.sim(fluid=True)
     r.readbin(str(tmp_path / "nscolumn.bin"), verbose=False)
     assert numpy.array_equal(r.v_f, v) and numpy.array_equal(r.p_f, p)
+
+
+@pytest.fixture
+def rundir(tmp_path):
+    import os
+    for d in ("input", "output", "work"):
+        (tmp_path / d).mkdir()
+    old = os.getcwd()
+    os.chdir(tmp_path / "work")
+    yield tmp_path
+    os.chdir(old)
+
+
+def _schedule(dt, total, file_dt):
+    t, clock, n, outs = 0.0, 0.0, 0, []
+    while t <= total:
+        t += dt
+        clock += dt
+        n += 1
+        if clock >= file_dt:
+            outs.append(n)
+            clock = 0.0
+    return n, outs
+
+
+def test_sim_run_neumann_gravity(torch, rundir):
+    """tests/cfd_tests_neumann.py:44-75 through sim.run(): a 3 x 3 x 3
+    water box under gravity over a no-flow bottom reaches the hydrostatic
+    pressure (tolerance 1e2 Pa, as there) with |v_z| < 1e-4; the last output
+    equals the oracle stepped to the same step count."""
+    orig = sphere_amd.sim("neumann", fluid=True)
+    orig.defineWorldBoundaries([0.3, 0.3, 0.3], dx=0.1)
+    orig.initFluid(mu=8.9e-4)
+    orig.initTemporal(total=0.05, file_dt=0.005, dt=1.0e-4)
+    orig.g[2] = -10.0
+    orig.bc_bot[0] = 1
+    print("sim.run neumann", flush=True)
+    assert orig.run(verbose=False) == 0
+    nsteps, outs = _schedule(1.0e-4, 0.05, 0.005)
+    py = sphere_amd.sim(sid=orig.sid, fluid=True)
+    py.readlast(verbose=False)
+    assert int(py.time_step_count[0]) == len(outs)
+    ideal = numpy.linspace(orig.p_f[0, 0, 0] + (orig.L[2] - orig.L[2] / orig.num[2]) * orig.rho_f[0] *
+                           numpy.abs(orig.g[2]), orig.p_f[0, 0, -1], orig.num[2])
+    assert numpy.max(numpy.abs(ideal - py.p_f[0, 0, :])) <= 1.0e2
+    assert numpy.all(numpy.abs(py.v_f[:, :, :, 2]) < 1.0e-4)
+    # the oracle to the last output's step count
+    P = _abi.ns_params_from_sim(orig)
+    ora = NsOracle(P, orig.v_f, orig.p_f)
+    done = 0
+    for n_out in outs:
+        ora.step(n_out - done)
+        done = n_out
+        ora.read()  # the output transfer refreshes the host copy of p
+    v, p, _, _ = ora.read()
+    assert numpy.array_equal(py.p_f, p) and numpy.array_equal(py.v_f, v)
+    conv = numpy.loadtxt("../output/neumann-conv.log")
+    assert conv.ndim == 2 and conv.shape[1] == 2
+    assert numpy.array_equal(conv[:, 0], numpy.arange(1, nsteps + 1, 10)[:len(conv)])
+    st = numpy.loadtxt("../output/neumann.status.dat")
+    assert int(st[2]) == len(outs)
+
+
+def test_sim_run_cfd_simple(torch, rundir):
+    """tests/cfd_simple.py through sim.run(): 5 x 5 x 5 cells, 1 s at
+    dt = 1e-3 with gravity over a no-flow bottom; the pressure profile of
+    the last output within 1e2 Pa of the hydrostatic one.  (That script's
+    ideal starts at L rho |g| = 3000 Pa, where cfd_tests_neumann.py uses
+    (L - dz) rho |g|: the cell-centred hydrostatic pressure of the bottom
+    layer, 2400 Pa here, which the oracle reaches too.)"""
+    orig = sphere_amd.sim("cfd_simple", fluid=True)
+    orig.defineWorldBoundaries([0.3, 0.3, 0.3], dx=0.06)
+    orig.initFluid(mu=8.9e-4)
+    orig.initTemporal(total=1.0, file_dt=0.1, dt=1.0e-3)
+    orig.bc_bot[0] = 1
+    orig.g[2] = -10.0
+    print("sim.run cfd_simple", flush=True)
+    assert orig.run(verbose=False) == 0
+    py = sphere_amd.sim(sid=orig.sid, fluid=True)
+    py.readlast(verbose=False)
+    ideal = numpy.linspace(orig.p_f[0, 0, 0] + (orig.L[2] - orig.L[2] / orig.num[2]) * orig.rho_f[0] *
+                           numpy.abs(orig.g[2]), orig.p_f[0, 0, -1], orig.num[2])
+    assert numpy.max(numpy.abs(ideal - py.p_f[0, 0, :])) <= 1.0e2
+    assert numpy.all(numpy.isfinite(py.v_f))
