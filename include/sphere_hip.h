@@ -428,6 +428,7 @@ typedef struct sph_ns_params {
     uint32_t maxiter, ndem;       /* Jacobi limit, DEM steps per fluid step */
     int32_t bc_bot, bc_top;       /* 0, 1 or 2, as above */
     double p_mod_A, p_mod_f, p_mod_phi; /* top pressure modulation */
+    double c_phi;                 /* porosity scaling (grains only) */
 } sph_ns_params;
 
 typedef struct sph_ns_stats {
@@ -463,6 +464,21 @@ int sph_ns_read(sph_ns *ns, double *v, double *p, double *phi, double *dphi);
  * v_z ((nx+3)(ny+3)(nz+3) each), 3 epsilon ((nx+2)(ny+2)(nz+2)). */
 int sph_ns_read_field(sph_ns *ns, int which, double *dst);
 int sph_ns_get_stats(const sph_ns *ns, sph_ns_stats *out);
+/* Grains held in place in the fluid (np = 0 clears them): rows of 4,
+ * x = (x, y, z, radius) and vel = (vx, vy, vz, unused), the grid's origin
+ * and periodic mode (0 none, 1 x and y, 2 x; src/contactsearch.cuh:12-84).
+ * Each later step forms the porosities from them
+ * (findPorositiesVelocitiesDiametersSpherical, src/navierstokes.cuh:
+ * 1136-1318), the drag, pressure and viscous force on each grain
+ * (findInteractionForce, :3141-3282) and the drag's reaction on the fluid
+ * (applyInteractionForceToFluid, :3284-3345), which the velocity
+ * prediction adds when mu > 0 (src/device.cu:1213-1351).  The grains do not
+ * move.  SPH_ERR_GRID: a grain outside the grid. */
+int sph_ns_set_grains(sph_ns *ns, uint32_t np, const double *x, const double *vel,
+                      const double *origo, int32_t periodic);
+/* The last step's forces on the grains, np rows of 3 each (any may be NULL):
+ * drag, pressure, viscous and their sum (ns.f_d, f_p, f_v, f_sum). */
+int sph_ns_read_grain_forces(sph_ns *ns, double *f_d, double *f_p, double *f_v, double *f_sum);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,14 @@ typedef struct {
     double *p, *ph, *phi, *dphi, *v[3], *vp[3], *dpvv[3], *eps, *eps_new, *norm, *f, *f1, *f2[3];
     /* face arrays (staggered grid) with ghosts */
     double *vf[3], *vpf[3], *dtau[3];
+    /* grains held in place (nso_set_grains): original order x (x, y, z, r)
+     * and vel rows, cell lists of the stable (cell, index) sort, the
+     * interaction forces per grain and their sum per cell */
+    uint32_t np;
+    double *gx, *gv, *gxs, origo[3];
+    int periodic;
+    uint32_t *cstart, *cend, *gidx;
+    double *fd, *fp, *fv, *fsum, *Fpf[3];
     long iter;
     double t;
     sph_ns_stats st;
@@ -51,6 +59,141 @@ typedef struct {
                          for (int x = 0; x < (o)->nx; ++x)
 
 static double amean(double a, double b) { return (a + b) * 0.5; }  /* navierstokes.cuh:17-20 */
+
+/* findDistMod, contactsearch.cuh:12-84: the periodic image of a target
+ * cell; -1 when it lies outside a non-periodic boundary */
+static int find_dist_mod(const nso *o, int *t, double *dm)
+{
+    const int nx = o->nx, ny = o->ny, nz = o->nz;
+    const double *L = o->P.L;
+    if (o->periodic == 1 || o->periodic == 2) {
+        if (t[0] < 0) { t[0] += nx; dm[0] += L[0]; }
+        if (t[0] >= nx) { t[0] -= nx; dm[0] -= L[0]; }
+    }
+    if (o->periodic == 1) {
+        if (t[1] < 0) { t[1] += ny; dm[1] += L[1]; }
+        if (t[1] >= ny) { t[1] -= ny; dm[1] -= L[1]; }
+    }
+    if (o->periodic == 2 && (t[1] < 0 || t[1] >= ny)) return -1;
+    if (o->periodic > 2 || o->periodic < 1) {
+        if (t[0] < 0 || t[0] >= nx) return -1;
+        if (t[1] < 0 || t[1] >= ny) return -1;
+    }
+    if (t[2] < 0 || t[2] >= nz) return -1;
+    return 0;
+}
+
+/* findPorositiesVelocitiesDiametersSpherical with grains,
+ * navierstokes.cuh:1136-1318: the void volume of a sphere of radius
+ * min(dx, dy, dz) at each cell centre, over the grains of the 5 x 5 x 5
+ * cells around it (the averaged grain velocities and diameters it also
+ * forms are read by no kernel of the fluid step) */
+static void porosity_grains(nso *o)
+{
+    const double dx = o->dx, dy = o->dy, dz = o->dz;
+    const double R = fmin(dx, fmin(dy, dz));
+    const double cell_volume = 4.0 / 3.0 * PI_NS * R * R * R;
+    FOR_CELLS(o) {
+        const double X[3] = {x * dx + 0.5 * dx, y * dy + 0.5 * dy, z * dz + 0.5 * dz};
+        double void_volume = cell_volume;
+        const size_t c = IDX(o, x, y, z);
+        const double phi_0 = o->phi[c];
+        for (int zd = -2; zd < 3; ++zd)
+            for (int yd = -2; yd < 3; ++yd)
+                for (int xd = -2; xd < 3; ++xd) {
+                    int t[3] = {x + xd, y + yd, z + zd};
+                    double dm[3] = {0.0, 0.0, 0.0};
+                    if (find_dist_mod(o, t, dm) == -1) continue;
+                    const uint32_t cell = (uint32_t)t[0] + (uint32_t)t[1] * (uint32_t)o->nx +
+                                          (uint32_t)(o->nx * o->ny) * (uint32_t)t[2];
+                    const uint32_t start = o->cstart[cell];
+                    if (start == 0xffffffffu) continue;
+                    for (uint32_t i = start; i < o->cend[cell]; ++i) {
+                        const double *xr = o->gxs + 4 * (size_t)i;
+                        const double r = xr[3];
+                        double dist[3] = {X[0] - xr[0], X[1] - xr[1], X[2] - xr[2]};
+                        for (int k = 0; k < 3; ++k) dist[k] = dist[k] + dm[k];
+                        const double d = sqrt(dist[0] * dist[0] + dist[1] * dist[1] + dist[2] * dist[2]);
+                        if ((R - r) < d && d < (R + r))
+                            void_volume -= 1.0 / (12.0 * d) *
+                                           (PI_NS * (R + r - d) * (R + r - d) *
+                                            (d * d + 2.0 * d * r - 3.0 * r * r + 2.0 * d * R + 6.0 * r * R -
+                                             3.0 * R * R));
+                        if (d <= R - r) void_volume -= 4.0 / 3.0 * PI_NS * r * r * r;
+                    }
+                }
+        const double phi = fmin(1.00, fmax(0.00, void_volume / cell_volume));
+        double dphi = phi - phi_0;
+        if (o->iter == 0) dphi = 0.0;
+        o->phi[c] = phi * o->P.c_phi;
+        o->dphi[c] = dphi * o->P.c_phi;
+    }
+}
+
+/* findInteractionForce, navierstokes.cuh:3141-3282 (SET_2): drag (with
+ * the radius where the reference's formula names a diameter), pressure and
+ * viscous force on each grain from the fluid of its cell */
+static void interaction_force(nso *o)
+{
+    const double dx = o->dx, dy = o->dy, dz = o->dz, mu = o->P.mu, rho_f = o->P.rho_f;
+    for (uint32_t i = 0; i < o->np; ++i) {
+        const double *x = o->gx + 4 * (size_t)i, *vel = o->gv + 4 * (size_t)i;
+        const double d = x[3];
+        const int ix = (int)(unsigned)floor((x[0] - o->origo[0]) / (o->P.L[0] / o->nx));
+        const int iy = (int)(unsigned)floor((x[1] - o->origo[1]) / (o->P.L[1] / o->ny));
+        const int iz = (int)(unsigned)floor((x[2] - o->origo[2]) / (o->P.L[2] / o->nz));
+        const size_t c = IDX(o, ix, iy, iz);
+        const double phi = o->phi[c];
+        const size_t f0 = VIDX(o, ix, iy, iz);
+        const size_t fp[3] = {VIDX(o, ix + 1, iy, iz), VIDX(o, ix, iy + 1, iz), VIDX(o, ix, iy, iz + 1)};
+        double v_rel[3], div_tau[3];
+        for (int k = 0; k < 3; ++k) {
+            const double v_f = amean(o->vf[k][f0], o->vf[k][fp[k]]);
+            div_tau[k] = amean(o->dtau[k][f0], o->dtau[k][fp[k]]);
+            v_rel[k] = v_f - vel[k];
+        }
+        const double v_rel_length = sqrt(v_rel[0] * v_rel[0] + v_rel[1] * v_rel[1] + v_rel[2] * v_rel[2]);
+        const double V_p = dx * dy * dz - phi * dx * dy * dz;
+        const double Re = rho_f * d * phi * v_rel_length / mu;
+        double Cd = pow(0.63 + 4.8 / pow(Re, 0.5), 2.0);
+        double chi = 3.7 - 0.65 * exp(-pow(1.5 - log10(Re), 2.0) / 2.0);
+        if (v_rel_length < 1.0e-6) {
+            Cd = 0.0;
+            chi = 0.0;
+        }
+        const double sd = 0.125 * Cd * rho_f * PI_NS * d * d * phi * phi * v_rel_length;
+        const double pw = pow(phi, -chi);
+        const double gp[3] = {(o->p[IDX(o, ix + 1, iy, iz)] - o->p[IDX(o, ix - 1, iy, iz)]) / (2.0 * dx),
+                              (o->p[IDX(o, ix, iy + 1, iz)] - o->p[IDX(o, ix, iy - 1, iz)]) / (2.0 * dy),
+                              (o->p[IDX(o, ix, iy, iz + 1)] - o->p[IDX(o, ix, iy, iz - 1)]) / (2.0 * dz)};
+        for (int k = 0; k < 3; ++k) {
+            const double f_d = sd * v_rel[k] * pw;
+            const double f_p = -1.0 * gp[k] * V_p;
+            const double f_v = -1.0 * div_tau[k] * V_p;
+            o->fd[3 * i + k] = f_d;
+            o->fp[3 * i + k] = f_p;
+            o->fv[3 * i + k] = f_v;
+            o->fsum[3 * i + k] = f_d + f_p + f_v;
+        }
+    }
+}
+
+/* applyInteractionForceToFluid, navierstokes.cuh:3284-3345: the grains'
+ * drag summed per cell in sorted order over the cell volume (the
+ * reference's sum starts uninitialised; here at zero) */
+static void apply_force_to_fluid(nso *o)
+{
+    const double vol = o->dx * o->dy * o->dz;
+    FOR_CELLS(o) {
+        const uint32_t cell = (uint32_t)x + (uint32_t)y * (uint32_t)o->nx + (uint32_t)(o->nx * o->ny) * (uint32_t)z;
+        double fi[3] = {0.0, 0.0, 0.0};
+        const uint32_t start = o->cstart[cell];
+        if (start != 0xffffffffu)
+            for (uint32_t i = start; i < o->cend[cell]; ++i)
+                for (int k = 0; k < 3; ++k) fi[k] += o->fd[3 * (size_t)o->gidx[i] + k];
+        for (int k = 0; k < 3; ++k) o->Fpf[k][IDX(o, x, y, z)] = fi[k] / vol;
+    }
+}
 
 /* setNSghostNodes<T>(field, bc_bot, bc_top), navierstokes.cuh:660-713;
  * the z ghost writes as stated in the header comment. */
@@ -244,7 +387,8 @@ static void predict(nso *o)
             const double div_tau = P->mu > 0.0 ? o->dtau[k][f] : 0.0;
             const double phi = amean(o->phi[c], o->phi[n[k]]);
             const double dphi = amean(o->dphi[c], o->dphi[n[k]]);
-            const double f_i = amean(0.0, 0.0);
+            const int coupled = P->mu > 0.0 && o->np > 0;
+            const double f_i = coupled ? amean(o->Fpf[k][c], o->Fpf[k][n[k]]) : amean(0.0, 0.0);
             double pressure_term = 0.0;
             if (P->beta > 0.0) {
                 const double grad_p = (o->p[c] - o->p[n[k]]) / h[k];
@@ -465,11 +609,15 @@ int nso_step(void *h, int nsteps)
     nso *o = (nso *)h;
     const sph_ns_params *P = &o->P;
     for (int s = 0; s < nsteps; ++s) {
-        /* findPorositiesVelocitiesDiametersSpherical with np = 0,
+        /* findPorositiesVelocitiesDiametersSpherical, with np = 0
          * navierstokes.cuh:1319-1328 */
-        FOR_CELLS(o) {
-            o->phi[IDX(o, x, y, z)] = 1.0;
-            o->dphi[IDX(o, x, y, z)] = 0.0;
+        if (o->np > 0) {
+            porosity_grains(o);
+        } else {
+            FOR_CELLS(o) {
+                o->phi[IDX(o, x, y, z)] = 1.0;
+                o->dphi[IDX(o, x, y, z)] = 0.0;
+            }
         }
         if (o->iter == 0) {
             ghost_cells3(o, o->v);
@@ -480,6 +628,12 @@ int nso_step(void *h, int nsteps)
         ghost_faces(o, o->dtau);
         ghost_cells(o, o->p);
         ghost_cells(o, o->phi);
+        if (o->np > 0) {  /* src/device.cu:1310-1351 */
+            interaction_force(o);
+            ghost_cells(o, o->p);
+            apply_force_to_fluid(o);
+            ghost_cells3(o, o->Fpf);
+        }
         if (o->iter % P->ndem == 0) {
             const int rc = ns_solve(o);
             if (rc) return rc;
@@ -559,6 +713,77 @@ void nso_field(void *h, int which, double *dst)
 
 void nso_stats(void *h, sph_ns_stats *out) { *out = ((nso *)h)->st; }
 
+/* Grains held in place: x and vel rows of 4 (x, y, z, radius; vx, vy, vz,
+ * unused), the grid's origin and periodic mode.  The cell lists are the
+ * reference's sort (cell ID of calcCellID, sorting.cuh:3-14; stable on the
+ * index).  -1 when a grain lies outside the grid. */
+int nso_set_grains(void *h, uint32_t np, const double *x, const double *vel, const double *origo,
+                   int periodic)
+{
+    nso *o = (nso *)h;
+    const uint32_t ncells = (uint32_t)(o->nx * o->ny * o->nz);
+    uint32_t *cid = (uint32_t *)malloc(sizeof(uint32_t) * (np ? np : 1));
+    for (uint32_t i = 0; i < np; ++i) {
+        long ic[3];
+        for (int k = 0; k < 3; ++k) {
+            const double q = floor((x[4 * (size_t)i + k] - origo[k]) / (o->P.L[k] / (k == 0 ? o->nx : k == 1 ? o->ny : o->nz)));
+            const long lim = k == 0 ? o->nx : k == 1 ? o->ny : o->nz;
+            if (!(q >= 0.0) || q >= (double)lim) {
+                free(cid);
+                return -1;
+            }
+            ic[k] = (long)q;
+        }
+        cid[i] = (uint32_t)((ic[2] * o->ny) * o->nx + ic[1] * o->nx + ic[0]);
+    }
+    free(o->gx); free(o->gv); free(o->gxs); free(o->cstart); free(o->cend); free(o->gidx);
+    free(o->fd); free(o->fp); free(o->fv); free(o->fsum);
+    o->np = np;
+    o->periodic = periodic;
+    for (int k = 0; k < 3; ++k) o->origo[k] = origo[k];
+    o->gx = (double *)malloc(sizeof(double) * 4 * (np ? np : 1));
+    o->gv = (double *)malloc(sizeof(double) * 4 * (np ? np : 1));
+    o->gxs = (double *)malloc(sizeof(double) * 4 * (np ? np : 1));
+    o->gidx = (uint32_t *)malloc(sizeof(uint32_t) * (np ? np : 1));
+    o->cstart = (uint32_t *)malloc(sizeof(uint32_t) * ncells);
+    o->cend = (uint32_t *)malloc(sizeof(uint32_t) * ncells);
+    o->fd = (double *)calloc(3 * (size_t)(np ? np : 1), sizeof(double));
+    o->fp = (double *)calloc(3 * (size_t)(np ? np : 1), sizeof(double));
+    o->fv = (double *)calloc(3 * (size_t)(np ? np : 1), sizeof(double));
+    o->fsum = (double *)calloc(3 * (size_t)(np ? np : 1), sizeof(double));
+    for (int k = 0; k < 3; ++k)
+        if (!o->Fpf[k]) o->Fpf[k] = (double *)calloc(o->nc, sizeof(double));
+    memcpy(o->gx, x, sizeof(double) * 4 * np);
+    memcpy(o->gv, vel, sizeof(double) * 4 * np);
+    /* counting sort by cell, stable on the index */
+    uint32_t *count = (uint32_t *)calloc(ncells + 1, sizeof(uint32_t));
+    for (uint32_t i = 0; i < np; ++i) count[cid[i] + 1]++;
+    for (uint32_t c = 0; c < ncells; ++c) count[c + 1] += count[c];
+    for (uint32_t c = 0; c < ncells; ++c) {
+        o->cstart[c] = count[c] == count[c + 1] ? 0xffffffffu : count[c];
+        o->cend[c] = count[c + 1];
+    }
+    for (uint32_t i = 0; i < np; ++i) {
+        const uint32_t pos = count[cid[i]]++;
+        o->gidx[pos] = i;
+        memcpy(o->gxs + 4 * (size_t)pos, x + 4 * (size_t)i, sizeof(double) * 4);
+    }
+    free(count);
+    free(cid);
+    return 0;
+}
+
+/* The last step's forces on the grains (np rows of 3 each) */
+void nso_grain_forces(void *h, double *f_d, double *f_p, double *f_v, double *f_sum)
+{
+    nso *o = (nso *)h;
+    const size_t n = 3 * (size_t)o->np;
+    if (f_d) memcpy(f_d, o->fd, n * sizeof(double));
+    if (f_p) memcpy(f_p, o->fp, n * sizeof(double));
+    if (f_v) memcpy(f_v, o->fv, n * sizeof(double));
+    if (f_sum) memcpy(f_sum, o->fsum, n * sizeof(double));
+}
+
 void nso_destroy(void *h)
 {
     nso *o = (nso *)h;
@@ -571,6 +796,17 @@ void nso_destroy(void *h)
         free(o->vf[k]);
         free(o->vpf[k]);
         free(o->dtau[k]);
+        free(o->Fpf[k]);
     }
+    free(o->gx);
+    free(o->gv);
+    free(o->gxs);
+    free(o->cstart);
+    free(o->cend);
+    free(o->gidx);
+    free(o->fd);
+    free(o->fp);
+    free(o->fv);
+    free(o->fsum);
     free(o);
 }

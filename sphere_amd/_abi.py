@@ -173,7 +173,7 @@ class SphNsParams(ctypes.Structure):
                 ("maxiter", ctypes.c_uint32), ("ndem", ctypes.c_uint32),
                 ("bc_bot", ctypes.c_int32), ("bc_top", ctypes.c_int32),
                 ("p_mod_A", ctypes.c_double), ("p_mod_f", ctypes.c_double),
-                ("p_mod_phi", ctypes.c_double)]
+                ("p_mod_phi", ctypes.c_double), ("c_phi", ctypes.c_double)]
 
 
 class SphNsStats(ctypes.Structure):
@@ -251,6 +251,8 @@ def load_library(path=None):
         "sph_ns_read": (ctypes.c_int, [vp, _pd, _pd, _pd, _pd]),
         "sph_ns_read_field": (ctypes.c_int, [vp, ctypes.c_int, _pd]),
         "sph_ns_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(SphNsStats)]),
+        "sph_ns_set_grains": (ctypes.c_int, [vp, ctypes.c_uint32, _pd, _pd, _pd, ctypes.c_int32]),
+        "sph_ns_read_grain_forces": (ctypes.c_int, [vp, _pd, _pd, _pd, _pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -496,7 +498,7 @@ class Net:
 
 def ns_params(nx, ny, nz, L, dt, mu=8.9e-4, rho_f=1.0e3, g=(0.0, 0.0, 0.0), beta=0.0, theta=1.0,
               gamma=0.0, tolerance=1.0e-3, c_v=1.0, maxiter=10000, ndem=1, bc_bot=0, bc_top=0,
-              p_mod_A=0.0, p_mod_f=0.0, p_mod_phi=0.0, t0=0.0):
+              p_mod_A=0.0, p_mod_f=0.0, p_mod_phi=0.0, t0=0.0, c_phi=1.0):
     """sph_ns_params with the reference's Navier-Stokes defaults
     (python/sphere.py:3395-3508)."""
     P = SphNsParams()
@@ -510,6 +512,7 @@ def ns_params(nx, ny, nz, L, dt, mu=8.9e-4, rho_f=1.0e3, g=(0.0, 0.0, 0.0), beta
     P.maxiter, P.ndem = int(maxiter), int(ndem)
     P.bc_bot, P.bc_top = int(bc_bot), int(bc_top)
     P.p_mod_A, P.p_mod_f, P.p_mod_phi = float(p_mod_A), float(p_mod_f), float(p_mod_phi)
+    P.c_phi = float(c_phi)
     return P
 
 
@@ -532,7 +535,8 @@ def ns_params_from_sim(s):
                      maxiter=_one_value(s.maxiter), ndem=_one_value(s.ndem),
                      bc_bot=_one_value(s.bc_bot), bc_top=_one_value(s.bc_top),
                      p_mod_A=_one_value(s.p_mod_A), p_mod_f=_one_value(s.p_mod_f),
-                     p_mod_phi=_one_value(s.p_mod_phi), t0=_one_value(s.time_current))
+                     p_mod_phi=_one_value(s.p_mod_phi), t0=_one_value(s.time_current),
+                     c_phi=_one_value(s.c_phi))
 
 
 class NavierStokes:
@@ -592,6 +596,26 @@ class NavierStokes:
         st = SphNsStats()
         self._check(self.lib.sph_ns_get_stats(self.h, ctypes.byref(st)))
         return st
+
+    def set_grains(self, x, vel, origo=(0.0, 0.0, 0.0), periodic=1):
+        """Grains held in place: x [np, 4] (position, radius), vel [np, 4]
+        or [np, 3]."""
+        x = numpy.ascontiguousarray(x, dtype=numpy.float64).reshape(-1, 4)
+        v = numpy.zeros_like(x)
+        vel = numpy.asarray(vel, dtype=numpy.float64).reshape(len(x), -1)
+        v[:, :3] = vel[:, :3]
+        o = numpy.ascontiguousarray(origo, dtype=numpy.float64)
+        self.np = len(x)
+        self._check(self.lib.sph_ns_set_grains(self.h, len(x), _ptr(x, ctypes.c_double),
+                                               _ptr(v, ctypes.c_double), _ptr(o, ctypes.c_double),
+                                               int(periodic)))
+
+    def grain_forces(self):
+        """(f_d, f_p, f_v, f_sum), each [np, 3], of the last step."""
+        n = getattr(self, "np", 0)
+        out = [numpy.zeros((n, 3)) for _ in range(4)]
+        self._check(self.lib.sph_ns_read_grain_forces(self.h, *[_ptr(a, ctypes.c_double) for a in out]))
+        return tuple(out)
 
     def close(self):
         if self.h:

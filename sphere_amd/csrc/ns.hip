@@ -276,7 +276,7 @@ struct NsPhys {
 
 // findPredNSvelocities, src/navierstokes.cuh:2276-2495 (SET_2, np = 0)
 __global__ void k_ns_predict(NsGrid g, NsPhys q, const double *p, CVec3 VF, const double *phi_a,
-                             const double *dphi_a, CVec3 DT, CVec3 DPVV, Vec3 VP)
+                             const double *dphi_a, CVec3 DT, CVec3 DPVV, CVec3 FPF, int coupled, Vec3 VP)
 {
     int x, y, z;
     if (!cell_of(g, x, y, z)) return;
@@ -291,7 +291,7 @@ __global__ void k_ns_predict(NsGrid g, NsPhys q, const double *p, CVec3 VF, cons
         const double div_tau = q.mu > 0.0 ? DT.c[k][f] : 0.0;
         const double phi = amean(phi_a[c], phi_a[n[k]]);
         const double dphi = amean(dphi_a[c], dphi_a[n[k]]);
-        const double f_i = amean(0.0, 0.0);
+        const double f_i = coupled ? amean(FPF.c[k][c], FPF.c[k][n[k]]) : amean(0.0, 0.0);
         double pressure_term = 0.0;
         if (q.beta > 0.0) {
             const double grad_p = (p[c] - p[n[k]]) / h[k];
@@ -311,6 +311,150 @@ __global__ void k_ns_predict(NsGrid g, NsPhys q, const double *p, CVec3 VF, cons
     if (z == 0 && g.bb == 1) vp[2] = 0.0;
     if (z == 0 && g.bb == 2) vp[0] = vp[1] = vp[2] = 0.0;
     for (int k = 0; k < 3; ++k) VP.c[k][f] = vp[k];
+}
+
+struct NsGrains {
+    uint32_t np;
+    int periodic;
+    double origo[3], L[3];
+    double c_phi;
+    const double *x, *vel, *xs;       // rows of 4: original order, sorted
+    const uint32_t *cstart, *cend, *gidx;
+    double *fd, *fp, *fv, *fsum;      // rows of 3
+};
+
+// findDistMod, src/contactsearch.cuh:12-84
+__device__ inline int find_dist_mod(const NsGrid &g, const NsGrains &G, int *t, double *dm)
+{
+    if (G.periodic == 1 || G.periodic == 2) {
+        if (t[0] < 0) { t[0] += g.nx; dm[0] += G.L[0]; }
+        if (t[0] >= g.nx) { t[0] -= g.nx; dm[0] -= G.L[0]; }
+    }
+    if (G.periodic == 1) {
+        if (t[1] < 0) { t[1] += g.ny; dm[1] += G.L[1]; }
+        if (t[1] >= g.ny) { t[1] -= g.ny; dm[1] -= G.L[1]; }
+    }
+    if (G.periodic == 2 && (t[1] < 0 || t[1] >= g.ny)) return -1;
+    if (G.periodic > 2 || G.periodic < 1) {
+        if (t[0] < 0 || t[0] >= g.nx) return -1;
+        if (t[1] < 0 || t[1] >= g.ny) return -1;
+    }
+    if (t[2] < 0 || t[2] >= g.nz) return -1;
+    return 0;
+}
+
+// findPorositiesVelocitiesDiametersSpherical with grains,
+// src/navierstokes.cuh:1136-1318 (the grain velocity and diameter averages
+// it also forms are read by no kernel of the fluid step)
+__global__ void k_ns_porosity_grains(NsGrid g, NsGrains G, int first, double *phi_a, double *dphi_a)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    const double R = fmin(dx, fmin(dy, dz));
+    const double cell_volume = 4.0 / 3.0 * NS_PI * R * R * R;
+    const double X[3] = {x * dx + 0.5 * dx, y * dy + 0.5 * dy, z * dz + 0.5 * dz};
+    double void_volume = cell_volume;
+    const size_t c = cidx(g, x, y, z);
+    const double phi_0 = phi_a[c];
+    for (int zd = -2; zd < 3; ++zd)
+        for (int yd = -2; yd < 3; ++yd)
+            for (int xd = -2; xd < 3; ++xd) {
+                int t[3] = {x + xd, y + yd, z + zd};
+                double dm[3] = {0.0, 0.0, 0.0};
+                if (find_dist_mod(g, G, t, dm) == -1) continue;
+                const uint32_t cell = (uint32_t)t[0] + (uint32_t)t[1] * (uint32_t)g.nx +
+                                      (uint32_t)(g.nx * g.ny) * (uint32_t)t[2];
+                const uint32_t start = G.cstart[cell];
+                if (start == 0xffffffffu) continue;
+                const uint32_t end = G.cend[cell];
+                for (uint32_t i = start; i < end; ++i) {
+                    const double *xr = G.xs + 4 * (size_t)i;
+                    const double r = xr[3];
+                    double dist[3] = {X[0] - xr[0], X[1] - xr[1], X[2] - xr[2]};
+                    for (int k = 0; k < 3; ++k) dist[k] = dist[k] + dm[k];
+                    const double d = sqrt(dist[0] * dist[0] + dist[1] * dist[1] + dist[2] * dist[2]);
+                    if ((R - r) < d && d < (R + r))
+                        void_volume -= 1.0 / (12.0 * d) *
+                                       (NS_PI * (R + r - d) * (R + r - d) *
+                                        (d * d + 2.0 * d * r - 3.0 * r * r + 2.0 * d * R + 6.0 * r * R -
+                                         3.0 * R * R));
+                    if (d <= R - r) void_volume -= 4.0 / 3.0 * NS_PI * r * r * r;
+                }
+            }
+    const double phi = fmin(1.00, fmax(0.00, void_volume / cell_volume));
+    double dphi = phi - phi_0;
+    if (first) dphi = 0.0;
+    phi_a[c] = phi * G.c_phi;
+    dphi_a[c] = dphi * G.c_phi;
+}
+
+// findInteractionForce, src/navierstokes.cuh:3141-3282 (SET_2): drag (the
+// reference's formula takes the radius where it names the diameter),
+// pressure and viscous force on each grain from its cell's fluid
+__global__ void k_ns_interaction(NsGrid g, NsPhys q, NsGrains G, const double *phi_a, const double *p,
+                                 CVec3 VF, CVec3 DT)
+{
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.np) return;
+    const double *x = G.x + 4 * (size_t)i, *vel = G.vel + 4 * (size_t)i;
+    const double dx = g.dx, dy = g.dy, dz = g.dz, mu = q.mu, rho_f = q.rho_f;
+    const double d = x[3];
+    const int ix = (int)(unsigned)floor((x[0] - G.origo[0]) / (G.L[0] / g.nx));
+    const int iy = (int)(unsigned)floor((x[1] - G.origo[1]) / (G.L[1] / g.ny));
+    const int iz = (int)(unsigned)floor((x[2] - G.origo[2]) / (G.L[2] / g.nz));
+    const size_t c = cidx(g, ix, iy, iz);
+    const double phi = phi_a[c];
+    const size_t f0 = fidx(g, ix, iy, iz);
+    const size_t fp[3] = {fidx(g, ix + 1, iy, iz), fidx(g, ix, iy + 1, iz), fidx(g, ix, iy, iz + 1)};
+    double v_rel[3], div_tau[3];
+    for (int k = 0; k < 3; ++k) {
+        const double v_f = amean(VF.c[k][f0], VF.c[k][fp[k]]);
+        div_tau[k] = amean(DT.c[k][f0], DT.c[k][fp[k]]);
+        v_rel[k] = v_f - vel[k];
+    }
+    const double v_rel_length = sqrt(v_rel[0] * v_rel[0] + v_rel[1] * v_rel[1] + v_rel[2] * v_rel[2]);
+    const double V_p = dx * dy * dz - phi * dx * dy * dz;
+    const double Re = rho_f * d * phi * v_rel_length / mu;
+    double Cd = pow(0.63 + 4.8 / pow(Re, 0.5), 2.0);
+    double chi = 3.7 - 0.65 * exp(-pow(1.5 - log10(Re), 2.0) / 2.0);
+    if (v_rel_length < 1.0e-6) {
+        Cd = 0.0;
+        chi = 0.0;
+    }
+    const double sd = 0.125 * Cd * rho_f * NS_PI * d * d * phi * phi * v_rel_length;
+    const double pw = pow(phi, -chi);
+    const double gp[3] = {(p[cidx(g, ix + 1, iy, iz)] - p[cidx(g, ix - 1, iy, iz)]) / (2.0 * dx),
+                          (p[cidx(g, ix, iy + 1, iz)] - p[cidx(g, ix, iy - 1, iz)]) / (2.0 * dy),
+                          (p[cidx(g, ix, iy, iz + 1)] - p[cidx(g, ix, iy, iz - 1)]) / (2.0 * dz)};
+    for (int k = 0; k < 3; ++k) {
+        const double f_d = sd * v_rel[k] * pw;
+        const double f_p = -1.0 * gp[k] * V_p;
+        const double f_v = -1.0 * div_tau[k] * V_p;
+        G.fd[3 * i + k] = f_d;
+        G.fp[3 * i + k] = f_p;
+        G.fv[3 * i + k] = f_v;
+        G.fsum[3 * i + k] = f_d + f_p + f_v;
+    }
+}
+
+// applyInteractionForceToFluid, src/navierstokes.cuh:3284-3345: the drag
+// summed per cell in sorted order over the cell volume (the reference's sum
+// starts uninitialised; here at zero)
+__global__ void k_ns_apply_force(NsGrid g, NsGrains G, Vec3 F)
+{
+    int x, y, z;
+    if (!cell_of(g, x, y, z)) return;
+    const double vol = g.dx * g.dy * g.dz;
+    const uint32_t cell = (uint32_t)x + (uint32_t)y * (uint32_t)g.nx + (uint32_t)(g.nx * g.ny) * (uint32_t)z;
+    double fi[3] = {0.0, 0.0, 0.0};
+    const uint32_t start = G.cstart[cell];
+    if (start != 0xffffffffu) {
+        const uint32_t end = G.cend[cell];
+        for (uint32_t i = start; i < end; ++i)
+            for (int k = 0; k < 3; ++k) fi[k] += G.fd[3 * (size_t)G.gidx[i] + k];
+    }
+    for (int k = 0; k < 3; ++k) F.c[k][cidx(g, x, y, z)] = fi[k] / vol;
 }
 
 // setNSepsilonTop / Bottom / Interior, src/navierstokes.cuh:356-446, on
@@ -530,6 +674,10 @@ struct sph_ns {
     NsSolveState *dst = nullptr;
     NsSolveState *hst = nullptr;  // pinned
     std::vector<double> ph;              // the host copy of p
+    // grains held in place (sph_ns_set_grains)
+    NsGrains G{};
+    void *grains = nullptr;
+    Vec3 Fpf{};
     int cur = 0;
     long iter = 0;
     double t = 0.0;
@@ -614,8 +762,9 @@ int ns_solve(sph_ns *n)
     ghost_cells(n, n->v.c[0], n->v.c[1], n->v.c[2]);
     hipLaunchKernelGGL(k_ns_div_phi_vv, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->phi, cv(n->v), n->dpvv);
     ghost_cells(n, n->dpvv.c[0], n->dpvv.c[1], n->dpvv.c[2]);
+    const int coupled = P.mu > 0.0 && n->G.np > 0;
     hipLaunchKernelGGL(k_ns_predict, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->q, n->p, cv(n->vf), n->phi,
-                       n->dphi, cv(n->dtau), cv(n->dpvv), n->vpf);
+                       n->dphi, cv(n->dtau), cv(n->dpvv), cv(n->Fpf), coupled, n->vpf);
     ghost_faces(n, n->vpf);
     face_to_center(n, n->vpf, n->vp);
     if (n->iter == 0) {
@@ -721,7 +870,7 @@ sph_ns *sph_ns_create(int device, const sph_ns_params *params, const double *v, 
     n->q = NsPhys{P.ndem * P.dt, P.mu, P.rho_f, P.beta, P.theta, P.c_v, {P.g[0], P.g[1], P.g[2]}};
     n->nc = (size_t)(P.nx + 2) * (P.ny + 2) * (P.nz + 2);
     n->nf = (size_t)(P.nx + 3) * (P.ny + 3) * (P.nz + 3);
-    const size_t ncell_arrays = 18, nface_arrays = 9;
+    const size_t ncell_arrays = 21, nface_arrays = 9;
     const size_t bytes = (ncell_arrays * n->nc + nface_arrays * n->nf) * sizeof(double) + 256;
     if (hipStreamCreateWithFlags(&n->s, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&n->arena, bytes) != hipSuccess ||
@@ -749,6 +898,7 @@ sph_ns *sph_ns_create(int device, const sph_ns_params *params, const double *v, 
         n->vp.c[k] = take_c();
         n->dpvv.c[k] = take_c();
         n->f2.c[k] = take_c();
+        n->Fpf.c[k] = take_c();
     }
     for (int k = 0; k < 3; ++k) {
         n->vf.c[k] = take_f();
@@ -787,6 +937,7 @@ void sph_ns_destroy(sph_ns *n)
     if (!n) return;
     if (n->s) hipStreamSynchronize(n->s);
     if (n->arena) hipFree(n->arena);
+    if (n->grains) hipFree(n->grains);
     if (n->hst) hipHostFree(n->hst);
     if (n->s) hipStreamDestroy(n->s);
     delete n;
@@ -799,7 +950,11 @@ int sph_ns_step(sph_ns *n, int nsteps)
     const NsGrid &g = n->g;
     const unsigned cb = cell_blocks(n);
     for (int s = 0; s < nsteps; ++s) {
-        hipLaunchKernelGGL(k_ns_porosity, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->phi, n->dphi);
+        if (n->G.np > 0)
+            hipLaunchKernelGGL(k_ns_porosity_grains, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->G,
+                               (int)(n->iter == 0), n->phi, n->dphi);
+        else
+            hipLaunchKernelGGL(k_ns_porosity, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->phi, n->dphi);
         if (n->iter == 0) {
             ghost_cells(n, n->v.c[0], n->v.c[1], n->v.c[2]);
             const unsigned faces = (unsigned)(g.nx + 1) * (unsigned)(g.ny + 1) * (unsigned)(g.nz + 1);
@@ -811,6 +966,13 @@ int sph_ns_step(sph_ns *n, int nsteps)
         ghost_faces(n, n->dtau);
         ghost_cells(n, n->p);
         ghost_cells(n, n->phi);
+        if (n->G.np > 0) {  // src/device.cu:1310-1351
+            hipLaunchKernelGGL(k_ns_interaction, dim3((n->G.np + NS_THREADS - 1) / NS_THREADS), dim3(NS_THREADS),
+                               0, n->s, g, n->q, n->G, n->phi, n->p, cv(n->vf), cv(n->dtau));
+            ghost_cells(n, n->p);
+            hipLaunchKernelGGL(k_ns_apply_force, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->G, n->Fpf);
+            ghost_cells(n, n->Fpf.c[0], n->Fpf.c[1], n->Fpf.c[2]);
+        }
         if (n->iter % n->P.ndem == 0) {
             const int rc = ns_solve(n);
             if (rc) return rc;
@@ -865,6 +1027,97 @@ int sph_ns_read_field(sph_ns *n, int which, double *dst)
     if (hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, n->s) != hipSuccess ||
         hipStreamSynchronize(n->s) != hipSuccess)
         return fail(SPH_ERR_HIP, "download failed");
+    return SPH_OK;
+}
+
+int sph_ns_set_grains(sph_ns *n, uint32_t np, const double *x, const double *vel, const double *origo,
+                      int32_t periodic)
+{
+    if (!n || (np && (!x || !vel)) || !origo) return fail(SPH_ERR_ARG, "null argument");
+    if (hipSetDevice(n->device) != hipSuccess) return fail(SPH_ERR_HIP, "hipSetDevice failed");
+    const NsGrid &g = n->g;
+    const uint32_t ncells = (uint32_t)(g.nx * g.ny * g.nz);
+    const int num[3] = {g.nx, g.ny, g.nz};
+    // cell IDs of calcCellID (src/sorting.cuh:3-14), each grain inside the grid
+    std::vector<uint32_t> cid(np);
+    for (uint32_t i = 0; i < np; ++i) {
+        long ic[3];
+        for (int k = 0; k < 3; ++k) {
+            const double q = std::floor((x[4 * (size_t)i + k] - origo[k]) / (n->P.L[k] / num[k]));
+            if (!(q >= 0.0) || q >= (double)num[k]) return fail(SPH_ERR_GRID, "a grain lies outside the grid");
+            ic[k] = (long)q;
+        }
+        cid[i] = (uint32_t)((ic[2] * g.ny) * g.nx + ic[1] * g.nx + ic[0]);
+    }
+    // the stable (cell, index) sort and the cell lists
+    std::vector<uint32_t> count(ncells + 1, 0u), cstart(ncells), cend(ncells), gidx(np);
+    std::vector<double> xs(4 * (size_t)np);
+    for (uint32_t i = 0; i < np; ++i) count[cid[i] + 1]++;
+    for (uint32_t c = 0; c < ncells; ++c) count[c + 1] += count[c];
+    for (uint32_t c = 0; c < ncells; ++c) {
+        cstart[c] = count[c] == count[c + 1] ? 0xffffffffu : count[c];
+        cend[c] = count[c + 1];
+    }
+    for (uint32_t i = 0; i < np; ++i) {
+        const uint32_t pos = count[cid[i]]++;
+        gidx[pos] = i;
+        std::memcpy(&xs[4 * (size_t)pos], x + 4 * (size_t)i, 4 * sizeof(double));
+    }
+    if (hipStreamSynchronize(n->s) != hipSuccess) return fail(SPH_ERR_HIP, "hipStreamSynchronize failed");
+    if (n->grains) {
+        (void)hipFree(n->grains);
+        n->grains = nullptr;
+    }
+    n->G = NsGrains{};
+    n->G.periodic = periodic;
+    n->G.c_phi = n->P.c_phi;
+    for (int k = 0; k < 3; ++k) {
+        n->G.origo[k] = origo[k];
+        n->G.L[k] = n->P.L[k];
+    }
+    if (np == 0) return SPH_OK;
+    const size_t dbl = (size_t)np * (4 + 4 + 4 + 3 * 4), u32 = 2 * (size_t)ncells + np;
+    if (hipMalloc(&n->grains, dbl * sizeof(double) + u32 * sizeof(uint32_t)) != hipSuccess)
+        return fail(SPH_ERR_HIP, "device allocation failed");
+    double *a = (double *)n->grains;
+    double *dx = a, *dv = a + 4 * (size_t)np, *dxs = a + 8 * (size_t)np;
+    n->G.fd = a + 12 * (size_t)np;
+    n->G.fp = n->G.fd + 3 * (size_t)np;
+    n->G.fv = n->G.fp + 3 * (size_t)np;
+    n->G.fsum = n->G.fv + 3 * (size_t)np;
+    uint32_t *u = (uint32_t *)(a + dbl);
+    uint32_t *dcs = u, *dce = u + ncells, *dgi = u + 2 * (size_t)ncells;
+    bool ok = hipMemcpyAsync(dx, x, 4 * (size_t)np * sizeof(double), hipMemcpyHostToDevice, n->s) == hipSuccess;
+    ok = ok && hipMemcpyAsync(dv, vel, 4 * (size_t)np * sizeof(double), hipMemcpyHostToDevice, n->s) == hipSuccess;
+    ok = ok && hipMemcpyAsync(dxs, xs.data(), xs.size() * sizeof(double), hipMemcpyHostToDevice, n->s) ==
+                   hipSuccess;
+    ok = ok && hipMemcpyAsync(dcs, cstart.data(), ncells * 4, hipMemcpyHostToDevice, n->s) == hipSuccess;
+    ok = ok && hipMemcpyAsync(dce, cend.data(), ncells * 4, hipMemcpyHostToDevice, n->s) == hipSuccess;
+    ok = ok && hipMemcpyAsync(dgi, gidx.data(), (size_t)np * 4, hipMemcpyHostToDevice, n->s) == hipSuccess;
+    ok = ok && hipMemsetAsync(n->G.fd, 0, 12 * (size_t)np * sizeof(double), n->s) == hipSuccess;
+    if (!ok || hipStreamSynchronize(n->s) != hipSuccess) return fail(SPH_ERR_HIP, "upload failed");
+    n->G.np = np;
+    n->G.x = dx;
+    n->G.vel = dv;
+    n->G.xs = dxs;
+    n->G.cstart = dcs;
+    n->G.cend = dce;
+    n->G.gidx = dgi;
+    return SPH_OK;
+}
+
+int sph_ns_read_grain_forces(sph_ns *n, double *f_d, double *f_p, double *f_v, double *f_sum)
+{
+    if (!n) return fail(SPH_ERR_ARG, "null context");
+    if (hipSetDevice(n->device) != hipSuccess) return fail(SPH_ERR_HIP, "hipSetDevice failed");
+    const size_t bytes = 3 * (size_t)n->G.np * sizeof(double);
+    if (!bytes) return SPH_OK;
+    double *dst[4] = {f_d, f_p, f_v, f_sum};
+    const double *src[4] = {n->G.fd, n->G.fp, n->G.fv, n->G.fsum};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k] && hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDeviceToHost, n->s) != hipSuccess)
+            return fail(SPH_ERR_HIP, "download failed");
+    if (hipStreamSynchronize(n->s) != hipSuccess) return fail(SPH_ERR_HIP, "download failed");
     return SPH_OK;
 }
 

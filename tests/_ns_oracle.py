@@ -23,6 +23,8 @@ def _lib():
         L.nso_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd]
         L.nso_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(_abi.SphNsStats)]
         L.nso_destroy.argtypes = [ctypes.c_void_p]
+        L.nso_set_grains.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _pd, _pd, _pd, ctypes.c_int]
+        L.nso_grain_forces.argtypes = [ctypes.c_void_p, _pd, _pd, _pd, _pd]
         L._ns_ready = True
     return L
 
@@ -77,6 +79,22 @@ class NsOracle:
         """A ghosted face field as [z + 1, y + 1, x + 1]."""
         nx, ny, nz = self.shape
         return self.field(which).reshape(nz + 3, ny + 3, nx + 3)
+
+    def set_grains(self, x, vel, origo=(0.0, 0.0, 0.0), periodic=1):
+        x = numpy.ascontiguousarray(x, dtype=numpy.float64).reshape(-1, 4)
+        v = numpy.zeros_like(x)
+        vel = numpy.asarray(vel, dtype=numpy.float64).reshape(len(x), -1)
+        v[:, :3] = vel[:, :3]
+        o = numpy.ascontiguousarray(origo, dtype=numpy.float64)
+        self.np = len(x)
+        if self.L.nso_set_grains(self.h, len(x), _p(x), _p(v), _p(o), int(periodic)) != 0:
+            raise ValueError("a grain lies outside the grid")
+
+    def grain_forces(self):
+        n = getattr(self, "np", 0)
+        out = [numpy.zeros((n, 3)) for _ in range(4)]
+        self.L.nso_grain_forces(self.h, *[_p(a) for a in out])
+        return tuple(out)
 
     def stats(self):
         st = _abi.SphNsStats()

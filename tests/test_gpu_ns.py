@@ -254,3 +254,80 @@ def test_sim_run_cfd_simple(torch, rundir):
                            numpy.abs(orig.g[2]), orig.p_f[0, 0, -1], orig.num[2])
     assert numpy.max(numpy.abs(ideal - py.p_f[0, 0, :])) <= 1.0e2
     assert numpy.all(numpy.isfinite(py.v_f))
+
+
+def _grains(n, L, seed, rmax=0.02):
+    rng = numpy.random.default_rng(seed)
+    x = numpy.column_stack([rng.uniform(0, L[0], n), rng.uniform(0, L[1], n),
+                            rng.uniform(0.02 * L[2], 0.98 * L[2], n), rng.uniform(0.3 * rmax, rmax, n)])
+    vel = 0.01 * rng.standard_normal((n, 3))
+    return x, vel
+
+
+@pytest.mark.parametrize("periodic", [1, 2, 0])
+def test_ns_grains_match_oracle_inviscid(torch, periodic):
+    """Grains held in a fluid with mu = 0 (the velocity prediction leaves
+    out their drag there): porosities, porosity changes, the fluid and the
+    pressure and viscous forces on the grains bit-identical to the oracle;
+    the drag (pow, exp and log10 of the device's math library against the
+    host's) within 1e-12 relative."""
+    shape = (10, 8, 12)
+    L = (0.3, 0.24, 0.36)
+    P = _abi.ns_params(*shape, L, 1.0e-4, mu=0.0, g=(0.0, 0.0, -9.81), bc_bot=1, bc_top=0,
+                       tolerance=1e-10, maxiter=20000)
+    v, p = _case(shape, 21)
+    x, vel = _grains(150, L, 5 + periodic)
+    print("ns grains inviscid", periodic, flush=True)
+    gpu = sphere_amd.NavierStokes(P, v, p)
+    ora = NsOracle(P, v, p)
+    gpu.set_grains(x, vel, periodic=periodic)
+    ora.set_grains(x, vel, periodic=periodic)
+    for step in range(4):
+        gpu.step(1)
+        ora.step(1)
+        a, b = gpu.read(), ora.read()
+        for u, w in zip(a, b):
+            assert numpy.array_equal(u, w), step
+        fg, fo = gpu.grain_forces(), ora.grain_forces()
+        assert numpy.array_equal(fg[1], fo[1]) and numpy.array_equal(fg[2], fo[2]), step
+        assert numpy.allclose(fg[0], fo[0], rtol=1e-12, atol=1e-12 * numpy.abs(fo[0]).max()), step
+    assert a[2].min() < 0.99 and numpy.abs(fo[0]).max() > 0
+    gpu.close()
+
+
+def test_ns_grains_coupled_close_to_oracle(torch):
+    """Grains held in a viscous fluid (their drag enters the velocity
+    prediction): every field within 1e-9 of its largest magnitude of the
+    oracle's after four steps, the porosities bit-identical."""
+    shape = (10, 8, 12)
+    L = (0.3, 0.24, 0.36)
+    P = _abi.ns_params(*shape, L, 1.0e-4, mu=1.0e-3, g=(0.0, 0.0, -9.81), bc_bot=1, bc_top=0,
+                       tolerance=1e-12, maxiter=20000)
+    v, p = _case(shape, 22)
+    x, vel = _grains(150, L, 9)
+    print("ns grains coupled", flush=True)
+    gpu = sphere_amd.NavierStokes(P, v, p)
+    ora = NsOracle(P, v, p)
+    gpu.set_grains(x, vel)
+    ora.set_grains(x, vel)
+    gpu.step(4)
+    ora.step(4)
+    a, b = gpu.read(), ora.read()
+    assert numpy.array_equal(a[2], b[2])
+    for u, w in zip(a[:2], b[:2]):
+        assert numpy.max(numpy.abs(u - w)) <= 1e-9 * numpy.abs(w).max()
+    fg, fo = gpu.grain_forces(), ora.grain_forces()
+    for u, w in zip(fg, fo):
+        assert numpy.max(numpy.abs(u - w)) <= 1e-9 * numpy.abs(w).max()
+    sg, so = gpu.stats(), ora.stats()
+    print("iterations", sg.iterations, so.iterations, flush=True)
+    assert abs(int(sg.iterations) - int(so.iterations)) <= 10 * sg.solves
+    gpu.close()
+
+
+def test_ns_grain_outside_refused(torch):
+    P = _abi.ns_params(4, 4, 4, (0.1, 0.1, 0.1), 1e-4)
+    ns = sphere_amd.NavierStokes(P, numpy.zeros((4, 4, 4, 3)), numpy.zeros((4, 4, 4)))
+    with pytest.raises(sphere_amd.SphereError):
+        ns.set_grains([[0.05, 0.05, 0.2, 0.01]], [[0, 0, 0]])
+    ns.close()

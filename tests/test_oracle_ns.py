@@ -268,3 +268,144 @@ def test_params_from_a_sim():
     s2 = sphere_amd.sim(sid="x", np=10, fluid=True, cfd_solver=0)
     with pytest.raises(_abi.SphereError):
         _abi.ns_params_from_sim(s2)
+
+
+# --- grains held in the fluid (porosity, interaction forces) ---
+
+def _grain_case(shape=(6, 6, 6), L=(0.6, 0.6, 0.6), mu=1e-3, **kw):
+    v, p = _rest(shape)
+    return NsOracle(_params(shape, L, mu=mu, bc_bot=1, bc_top=0, tolerance=1e-3, maxiter=20, **kw),
+                    v, p)
+
+
+def _lens(R, r, d):
+    """Volume of the intersection of spheres of radii R and r at distance d,
+    by numerical integration of the cross-section areas along the axis."""
+    from scipy.integrate import quad
+    def area(x):  # x along the axis from the centre of R
+        a2 = R * R - x * x
+        b2 = r * r - (x - d) ** 2
+        m = min(a2, b2)
+        return numpy.pi * m if m > 0 else 0.0
+    lo, hi = max(-R, d - r), min(R, d + r)
+    return quad(area, lo, hi, points=[(R * R - r * r + d * d) / (2 * d)], epsabs=1e-16, epsrel=1e-12)[0]
+
+
+def test_porosity_of_one_grain():
+    """Porosity 1 - V_overlap / (4/3 pi R^3), R = min(dx, dy, dz) the
+    sampling sphere at each cell centre: a grain wholly inside it, one
+    partly inside (the lens volume by numerical integration), and the cells
+    it does not reach; and dphi 0 on the first step, the change after."""
+    o = _grain_case()
+    R = 0.1
+    r = 0.03
+    # grain at the centre of cell (2, 2, 2) (centre 0.25, 0.25, 0.25)
+    o.set_grains([[0.25, 0.25, 0.25, r]], [[0.0, 0.0, 0.0]])
+    o.step(1)
+    _, _, phi, dphi = o.read()
+    Vs = 4.0 / 3.0 * numpy.pi * R ** 3
+    assert phi[2, 2, 2] == pytest.approx(1.0 - (r / R) ** 3, rel=1e-13)
+    d = 0.1  # neighbour cell centre at one cell width
+    assert phi[3, 2, 2] == pytest.approx(1.0 - _lens(R, r, d) / Vs, rel=1e-10)
+    assert phi[2, 3, 2] == phi[3, 2, 2] == phi[2, 2, 1]
+    assert phi[4, 2, 2] == 1.0 and phi[3, 3, 2] == pytest.approx(1.0 - _lens(R, r, d * 2 ** 0.5) / Vs,
+                                                                rel=1e-10)
+    assert numpy.all(dphi == 0.0)
+    o.set_grains([[0.25, 0.25, 0.25, 0.05]], [[0.0, 0.0, 0.0]])
+    o.step(1)
+    _, _, phi2, dphi2 = o.read()
+    assert dphi2[2, 2, 2] == pytest.approx(phi2[2, 2, 2] - phi[2, 2, 2], rel=1e-13)
+
+
+def test_porosity_wraps_periodically():
+    """Moving every grain by one cell along x (wrapping at the periodic
+    boundary) moves the porosity field by one cell."""
+    rng = numpy.random.default_rng(8)
+    n = 40
+    x = numpy.column_stack([rng.uniform(0, 0.6, n), rng.uniform(0, 0.6, n), rng.uniform(0.05, 0.55, n),
+                            rng.uniform(0.01, 0.04, n)])
+    a = _grain_case()
+    a.set_grains(x, numpy.zeros((n, 3)))
+    a.step(1)
+    b = _grain_case()
+    y = x.copy()
+    y[:, 0] = (y[:, 0] + 0.1) % 0.6
+    b.set_grains(y, numpy.zeros((n, 3)))
+    b.step(1)
+    pa, pb = a.read()[2], b.read()[2]
+    assert numpy.allclose(numpy.roll(pa, 1, axis=0), pb, rtol=1e-12, atol=1e-14)
+    assert pa.min() < 0.999
+
+
+def test_interaction_forces():
+    """On the first step the forces use the initial fields (they come
+    before the solve): pressure force -grad(p) V_s with V_s = (1 - phi)
+    dx dy dz (the solid volume of the grain's cell) on a linear pressure;
+    the drag on a grain moving through fluid at rest in the Stokes limit,
+    2.88 pi mu r v phi^(2 - chi) (the Di Felice correlation with the radius
+    where it names the diameter: 0.125 (4.8^2 / Re) rho pi r^2 |v| v); no
+    drag on a grain at rest; f_sum = f_d + f_p + f_v."""
+    shape = (6, 6, 6)
+    mu, rho, g, dz = 20.0, 1000.0, 9.81, 0.1
+    zc = (numpy.arange(6) + 0.5) * dz
+    p = numpy.broadcast_to(rho * g * (0.6 - zc), shape).copy()
+    o = NsOracle(_params(shape, (0.6, 0.6, 0.6), mu=mu, bc_bot=1, bc_top=0, tolerance=1e-3, maxiter=1),
+                 numpy.zeros(shape + (3,)), p)
+    r, v = 0.01, 2.0e-6
+    grains = numpy.array([[0.25, 0.25, 0.25, r], [0.35, 0.45, 0.35, r], [0.41, 0.31, 0.25, r]])
+    o.set_grains(grains, [[-v, 0, 0], [0, -v, 0], [0, 0, 0]])
+    o.step(1)
+    f_d, f_p, f_v, f_sum = o.grain_forces()
+    phi = o.read()[2]
+    vol = 0.1 ** 3
+    for k, (ix, iy, iz) in enumerate([(2, 2, 2), (3, 4, 3), (4, 3, 2)]):
+        Vs = vol - phi[ix, iy, iz] * vol
+        assert f_p[k, 2] == pytest.approx(rho * g * Vs, rel=1e-12)
+        assert f_p[k, 0] == 0.0 and f_p[k, 1] == 0.0
+    for k, (ix, iy, iz) in enumerate([(2, 2, 2), (3, 4, 3)]):
+        ph = phi[ix, iy, iz]
+        Re = rho * r * ph * v / mu
+        assert Re < 2e-6
+        chi = 3.7 - 0.65 * numpy.exp(-(1.5 - numpy.log10(Re)) ** 2 / 2)
+        stokes = 2.88 * numpy.pi * mu * r * v * ph ** (2 - chi)
+        assert f_d[k, k] == pytest.approx(stokes, rel=2e-3)
+    assert numpy.all(f_d[2] == 0.0)
+    assert numpy.all(f_v == 0.0)
+    assert numpy.array_equal(f_sum, f_d + f_p + f_v)
+
+
+def test_reaction_sums_to_the_drag():
+    """The drag's reaction on the fluid, summed over the cells times the cell
+    volume, is the total drag on the grains, and it enters the velocity
+    prediction as -dt/(rho phi) F (mu > 0): with and without the grains'
+    velocities the predicted faces differ by that term."""
+    rng = numpy.random.default_rng(9)
+    n = 30
+    x = numpy.column_stack([rng.uniform(0, 0.6, n), rng.uniform(0, 0.6, n), rng.uniform(0.05, 0.55, n),
+                            rng.uniform(0.005, 0.02, n)])
+    vel = 0.01 * rng.standard_normal((n, 3))
+    runs = []
+    for vv in (vel, numpy.zeros_like(vel)):
+        o = _grain_case(mu=1e-3)
+        o.set_grains(x, vv)
+        o.step(1)
+        runs.append(o)
+    f_d = runs[0].grain_forces()[0]
+    assert numpy.abs(f_d).max() > 0
+    # the prediction's change from the grains' drag: -dt/(rho phi_face) F_face
+    dvp = runs[0].faces(NsOracle.PRED_FACES) - runs[1].faces(NsOracle.PRED_FACES)
+    nx = ny = nz = 6
+    dvx = dvp[1:nz + 1, 1:ny + 1, 1:nx + 1]  # x faces, [z, y, x]
+    # -rho phi_face dv / dt summed over the faces times the cell volume is
+    # the reaction the fluid receives: the total drag along x with its sign
+    # reversed, as -dt/(rho phi) F removes it from the fluid
+    phi = runs[0].read()[2].transpose(2, 1, 0)
+    phi_face = 0.5 * (phi + numpy.roll(phi, 1, axis=2))
+    reaction = -(1000.0 * phi_face * dvx / 1e-4).sum() * 0.1 ** 3
+    assert reaction == pytest.approx(f_d[:, 0].sum(), rel=1e-9)
+
+
+def test_grains_outside_the_grid_are_refused():
+    o = _grain_case()
+    with pytest.raises(ValueError):
+        o.set_grains([[0.25, 0.25, 0.7, 0.01]], [[0.0, 0.0, 0.0]])
