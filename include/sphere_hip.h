@@ -479,6 +479,16 @@ int sph_ns_set_grains(sph_ns *ns, uint32_t np, const double *x, const double *ve
 /* The last step's forces on the grains, np rows of 3 each (any may be NULL):
  * drag, pressure, viscous and their sum (ns.f_d, f_p, f_v, f_sum). */
 int sph_ns_read_grain_forces(sph_ns *ns, double *f_d, double *f_p, double *f_v, double *f_sum);
+/* Couples the fluid to a loaded context without a Darcy fluid whose grid
+ * (num, L) is the fluid's: each sph_step of the context then runs the
+ * fluid's iteration with the context's grains between their contact and
+ * bond forces and the integration, and adds its force on each grain
+ * (src/device.cu:1213-1351, findInteractionForce's dev_force += f_pf); the
+ * fluid must not have stepped before.  sph_ns_read and
+ * sph_ns_read_grain_forces read it between steps; sph_ns_step and
+ * sph_ns_set_grains are refused while attached.  ns = NULL detaches.
+ * Detach (or destroy the context) before destroying the fluid. */
+int sph_ns_attach(sph_ns *ns, sph_ctx *ctx);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,7 @@ typedef struct {
     int periodic;
     uint32_t *cstart, *cend, *gidx;
     double *fd, *fp, *fv, *fsum, *Fpf[3];
+    double *force;  /* coupled to the DEM oracle: its grains' force rows */
     long iter;
     double t;
     sph_ns_stats st;
@@ -174,6 +175,7 @@ static void interaction_force(nso *o)
             o->fp[3 * i + k] = f_p;
             o->fv[3 * i + k] = f_v;
             o->fsum[3 * i + k] = f_d + f_p + f_v;
+            if (o->force) o->force[3 * i + k] += f_d + f_p + f_v;  /* dev_force[i] += f_pf */
         }
     }
 }
@@ -604,45 +606,93 @@ static int ns_solve(nso *o)
     return SPH_OK;
 }
 
+/* One DEM iteration of the fluid, src/device.cu:1213-1898 */
+static int step_body(nso *o)
+{
+    const sph_ns_params *P = &o->P;
+    /* findPorositiesVelocitiesDiametersSpherical, with np = 0
+     * navierstokes.cuh:1319-1328 */
+    if (o->np > 0) {
+        porosity_grains(o);
+    } else {
+        FOR_CELLS(o) {
+            o->phi[IDX(o, x, y, z)] = 1.0;
+            o->dphi[IDX(o, x, y, z)] = 0.0;
+        }
+    }
+    if (o->iter == 0) {
+        ghost_cells3(o, o->v);
+        center_to_face(o, o->v, o->vf);
+    }
+    ghost_faces(o, o->vf);
+    face_div_tau(o);
+    ghost_faces(o, o->dtau);
+    ghost_cells(o, o->p);
+    ghost_cells(o, o->phi);
+    if (o->np > 0) {  /* src/device.cu:1310-1351 */
+        interaction_force(o);
+        ghost_cells(o, o->p);
+        apply_force_to_fluid(o);
+        ghost_cells3(o, o->Fpf);
+    }
+    if (o->iter % P->ndem == 0) {
+        const int rc = ns_solve(o);
+        if (rc) return rc;
+    }
+    o->t += P->dt;
+    o->iter++;
+    o->st.steps++;
+    return SPH_OK;
+}
+
 int nso_step(void *h, int nsteps)
 {
     nso *o = (nso *)h;
-    const sph_ns_params *P = &o->P;
     for (int s = 0; s < nsteps; ++s) {
-        /* findPorositiesVelocitiesDiametersSpherical, with np = 0
-         * navierstokes.cuh:1319-1328 */
-        if (o->np > 0) {
-            porosity_grains(o);
-        } else {
-            FOR_CELLS(o) {
-                o->phi[IDX(o, x, y, z)] = 1.0;
-                o->dphi[IDX(o, x, y, z)] = 0.0;
-            }
-        }
-        if (o->iter == 0) {
-            ghost_cells3(o, o->v);
-            center_to_face(o, o->v, o->vf);
-        }
-        ghost_faces(o, o->vf);
-        face_div_tau(o);
-        ghost_faces(o, o->dtau);
-        ghost_cells(o, o->p);
-        ghost_cells(o, o->phi);
-        if (o->np > 0) {  /* src/device.cu:1310-1351 */
-            interaction_force(o);
-            ghost_cells(o, o->p);
-            apply_force_to_fluid(o);
-            ghost_cells3(o, o->Fpf);
-        }
-        if (o->iter % P->ndem == 0) {
-            const int rc = ns_solve(o);
-            if (rc) return rc;
-        }
-        o->t += P->dt;
-        o->iter++;
-        o->st.steps++;
+        const int rc = step_body(o);
+        if (rc) return rc;
     }
     return SPH_OK;
+}
+
+/* One iteration coupled to the DEM oracle's grains (its original-order x
+ * and vel rows of 4, the cell sort's sorted x, cell starts and ends and
+ * original indices), the fluid's force on each grain added to force
+ * (rows of 3). */
+int nso_step_coupled(void *h, uint32_t np, double *x, double *vel, double *xs, uint32_t *cstart,
+                     uint32_t *cend, uint32_t *gidx, const double *origo, int periodic, double *force)
+{
+    nso *o = (nso *)h;
+    if (np != o->np || !o->fd) {
+        free(o->fd); free(o->fp); free(o->fv); free(o->fsum);
+        o->fd = (double *)calloc(3 * (size_t)(np ? np : 1), sizeof(double));
+        o->fp = (double *)calloc(3 * (size_t)(np ? np : 1), sizeof(double));
+        o->fv = (double *)calloc(3 * (size_t)(np ? np : 1), sizeof(double));
+        o->fsum = (double *)calloc(3 * (size_t)(np ? np : 1), sizeof(double));
+        for (int k = 0; k < 3; ++k)
+            if (!o->Fpf[k]) o->Fpf[k] = (double *)calloc(o->nc, sizeof(double));
+    }
+    double *keep_x = o->gx, *keep_v = o->gv, *keep_xs = o->gxs;
+    uint32_t *keep_cs = o->cstart, *keep_ce = o->cend, *keep_gi = o->gidx;
+    o->np = np;
+    o->gx = x;
+    o->gv = vel;
+    o->gxs = xs;
+    o->cstart = cstart;
+    o->cend = cend;
+    o->gidx = gidx;
+    o->periodic = periodic;
+    for (int k = 0; k < 3; ++k) o->origo[k] = origo[k];
+    o->force = force;
+    const int rc = step_body(o);
+    o->force = NULL;
+    o->gx = keep_x;
+    o->gv = keep_v;
+    o->gxs = keep_xs;
+    o->cstart = keep_cs;
+    o->cend = keep_ce;
+    o->gidx = keep_gi;
+    return rc;
 }
 
 void *nso_create(const sph_ns_params *p, const double *v, const double *pf)

@@ -73,6 +73,11 @@ static inline void st3(double *p, v3 a) { p[0] = a.x; p[1] = a.y; p[2] = a.z; }
 
 static int g_threads = 1;
 void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
+/* ns_oracle.c: one fluid iteration with the DEM oracle's grains */
+int nso_step_coupled(void *h, uint32_t np, double *x, double *vel, double *xs, uint32_t *cstart,
+                     uint32_t *cend, uint32_t *gidx, const double *origo, int periodic, double *force);
+
 int orc_get_threads(void) { return g_threads; }
 /* contact slots per particle of the next orc_create (the reference's NC) */
 static int g_nc = SPH_NC;
@@ -80,6 +85,7 @@ void orc_set_contact_slots(int nc) { g_nc = (nc == 64) ? 64 : 32; }
 
 struct orc {
     sph_state h;            /* header scalars; array pointers unused */
+    void *ns;               /* an attached Navier-Stokes fluid (ns_oracle.c) */
     uint32_t np, nb0, nw, ncells;
     /* particle state, orig order; x/vel stride 4 (w = radius / fixvel) */
     double *x, *vel, *angvel4;
@@ -1524,6 +1530,11 @@ static int step_once(orc *o)
         rc = darcy_step(o);
         if (rc) return rc;
     }
+    if (o->ns && o->np > 0) {
+        rc = nso_step_coupled(o->ns, o->np, o->x, o->vel, o->x_s, o->cell_start, o->cell_end, o->index,
+                              h->origo, h->periodic, o->force);
+        if (rc) return rc;
+    }
     if (o->np > 0) {
         if (o->nw > 0 && h->wmode[0] == 3) {
             shear_stress_pp(o);
@@ -1705,3 +1716,8 @@ int orc_read_darcy_field(orc *o, int which, double *out)
     memcpy(out, src, (size_t)o->dcells * 8);
     return SPH_OK;
 }
+
+/* Couples a Navier-Stokes fluid of ns_oracle.c to the oracle (the
+ * reference's fluid block between the bond forces and the integration,
+ * src/device.cu:1213-1898); NULL detaches. */
+void orc_attach_ns(orc *o, void *ns) { o->ns = ns; }

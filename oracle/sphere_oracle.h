@@ -38,6 +38,7 @@ int orc_read_contacts(orc *o, uint32_t *contacts, double *distmod,
 int orc_write_contacts(orc *o, const uint32_t *contacts, const double *distmod,
                        const double *delta_t);
 int orc_darcy_stats(orc *o, sph_darcy_stats *out);
+void orc_attach_ns(orc *o, void *ns);
 /* (iter+1, nijac) pairs that the reference writes to <sid>-conv.log. */
 int orc_conv_log(orc *o, uint32_t *pairs, int max_pairs);
 /* Darcy intermediate fields with ghost layers: which = 0 p, 1 phi, 2 dphi,

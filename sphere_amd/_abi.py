@@ -253,6 +253,7 @@ def load_library(path=None):
         "sph_ns_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(SphNsStats)]),
         "sph_ns_set_grains": (ctypes.c_int, [vp, ctypes.c_uint32, _pd, _pd, _pd, ctypes.c_int32]),
         "sph_ns_read_grain_forces": (ctypes.c_int, [vp, _pd, _pd, _pd, _pd]),
+        "sph_ns_attach": (ctypes.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -372,6 +373,15 @@ class Engine:
         n = ctypes.c_uint64(0)
         self._check(self.lib.sph_count_contacts(self.ctx, ctypes.byref(n)))
         return n.value
+
+    def attach_fluid(self, ns):
+        """Couple a NavierStokes fluid on this context's grid to its grains
+        (None detaches); each step then runs the fluid and adds its force
+        on the grains."""
+        self._check(self.lib.sph_ns_attach(ns.h if ns is not None else None, self.ctx))
+        if ns is not None:
+            ns.np = int(self.header().np)  # rows of grain_forces()
+        self._fluid = ns
 
     def darcy_stats(self):
         s = SphDarcyStats()
@@ -520,14 +530,16 @@ def _one_value(a):
     return numpy.asarray(a).ravel()[0]
 
 
-def ns_params_from_sim(s):
-    """sph_ns_params of a sim with a Navier-Stokes fluid (cfd_solver = 0),
-    no grains and no walls: its grid, time step and time, fluid and solver
-    fields (the ones the reference's binary file carries)."""
+def ns_params_from_sim(s, grains=False):
+    """sph_ns_params of a sim with a Navier-Stokes fluid (cfd_solver = 0)
+    and no walls: its grid, time step and time, fluid and solver fields (the
+    ones the reference's binary file carries).  Grains only with grains=True
+    (a fluid attached to an Engine that steps them)."""
     if not s.fluid or int(_one_value(s.cfd_solver)) != 0:
         raise SphereError(SPH_ERR_ARG, "the sim has no Navier-Stokes fluid (cfd_solver = 0)")
-    if int(_one_value(s.np)) != 0 or int(_one_value(s.nw)) != 0:
-        raise SphereError(SPH_ERR_ARG, "only a fluid without grains and walls is built")
+    if (int(_one_value(s.np)) != 0 and not grains) or int(_one_value(s.nw)) != 0:
+        raise SphereError(SPH_ERR_ARG, "only a fluid without walls (and without grains unless "
+                                       "attached to an Engine) is built")
     return ns_params(*[int(n) for n in s.num], L=s.L, dt=_one_value(s.time_dt),
                      mu=_one_value(s.mu), rho_f=_one_value(s.rho_f), g=s.g,
                      beta=_one_value(s.beta), theta=_one_value(s.theta), gamma=_one_value(s.gamma),

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include "net.h"
+#include "ns.h"
 
 #include <algorithm>
 #include <cmath>
@@ -40,6 +41,7 @@ int key_bits_for(uint32_t ncells)
 
 struct sph_ctx {
     int device = 0;
+    sph_ns *ns = nullptr;  // an attached Navier-Stokes fluid (sph_ns_attach)
     hipStream_t stream = nullptr;
     // integrateWalls runs on `aux`, overlapped with the next step's contact
     // search (which does not read the walls); the contact forces wait for it
@@ -1164,9 +1166,10 @@ static int output_event(sph_ctx *ctx, const std::string *dir, const std::string 
 // bonds and integrate reads or adds to the forces: the Darcy pressure force
 // (added after the bonds, src/device.cu:1970-1990) or the wmode-3 shear
 // stress (reads the forces, src/integration.cuh:457-484).
-static bool gather_in_integrate(const sph_state &h)
+static bool gather_in_integrate(const sph_ctx *ctx)
 {
-    return !h.fluid && !(h.nw > 0 && h.wmode[0] == 3);
+    const sph_state &h = ctx->hs.h;
+    return !h.fluid && !ctx->ns && !(h.nw > 0 && h.wmode[0] == 3);
 }
 
 static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *dir,
@@ -1192,7 +1195,7 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
             // sorted arrays, so a fluid run rebuilds every step.
             const uint32_t cur = ctx->nl_parity;
             const uint32_t *cond = ctx->db.nl_flags + cur;
-            const uint32_t force = ctx->nl_force || h.fluid ? 1u : 0u;
+            const uint32_t force = ctx->nl_force || h.fluid || ctx->ns ? 1u : 0u;
             if (force || !ctx->hash_ready) {  // else the last integrate formed the hash
                 ProfScope ps(ctx, SPH_K_HASH);
                 launch_hash_nl(s, a, ctx->nla, ctx->db, cur, force);
@@ -1227,12 +1230,20 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
         }
         if (h.nb0 > 0) {
             ProfScope ps(ctx, SPH_K_BONDS);
-            launch_bonds(s, a, ctx->db, !gather_in_integrate(h));
+            launch_bonds(s, a, ctx->db, !gather_in_integrate(ctx));
         }
     }
     if (h.fluid) {
         rc = darcy_step(ctx, convlog);
         if (rc) return rc;
+    }
+    if (ctx->ns && h.np > 0) {
+        // the Navier-Stokes fluid between the grains' forces and the
+        // integration, its force on each grain added (src/device.cu:1213-1898)
+        rc = ns_coupled_step(ctx->ns, h.np, (const double *)ctx->db.x, (const double *)ctx->db.vel,
+                             (const double *)ctx->db.x_s, ctx->db.cell_start, ctx->db.cell_end, ctx->db.vals,
+                             h.origo, h.periodic, (double *)ctx->db.force, s);
+        if (rc) return fail(ctx, rc, sph_ns_last_error());
     }
     if (h.np > 0) {
         const StepArgs a = step_args(ctx);
@@ -1261,7 +1272,7 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
         }
         { ProfScope ps(ctx, SPH_K_INTEGRATE);
           launch_integrate(s, a, ctx->db, change_velocity_state, 10.0, h.tau_x,
-                           gather_in_integrate(h), nh); }
+                           gather_in_integrate(ctx), nh); }
         ctx->hash_ready = fuse_hash;
         if (h.nw > 0) {
             HIPCHK(hipEventRecord(ctx->ev_integrated, s));
@@ -1329,9 +1340,34 @@ sph_ctx *sph_create(int device)
     return ctx;
 }
 
+int sph_ns_attach(sph_ns *ns, sph_ctx *ctx)
+{
+    if (!ctx) return fail(ctx, SPH_ERR_ARG, "null context");
+    if (!ns) {
+        if (ctx->ns) ns_set_attached(ctx->ns, 0);
+        ctx->ns = nullptr;
+        return SPH_OK;
+    }
+    if (!ctx->loaded) return fail(ctx, SPH_ERR_STATE, "no state loaded");
+    const sph_state &h = ctx->hs.h;
+    if (h.fluid) return fail(ctx, SPH_ERR_VALUE, "the context already has a Darcy fluid");
+    uint32_t num[3];
+    double L[3];
+    ns_grid(ns, num, L);
+    for (int k = 0; k < 3; ++k)
+        if (num[k] != h.num[k] || L[k] != h.L[k])
+            return fail(ctx, SPH_ERR_VALUE, "the fluid grid must be the context's grid (num and L)");
+    if (h.contactmodel == 1) return fail(ctx, SPH_ERR_VALUE, "contact model 1 is not coupled to a fluid here");
+    if (ctx->ns && ctx->ns != ns) ns_set_attached(ctx->ns, 0);
+    ctx->ns = ns;
+    ns_set_attached(ns, 1);
+    return SPH_OK;
+}
+
 void sph_destroy(sph_ctx *ctx)
 {
     if (!ctx) return;
+    if (ctx->ns) ns_set_attached(ctx->ns, 0);
     (void)hipSetDevice(ctx->device);
     free_device(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);

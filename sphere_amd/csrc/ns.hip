@@ -35,6 +35,7 @@
 
 #include "../../include/sphere_hip.h"
 #include "multi.h"
+#include "ns.h"
 
 namespace {
 
@@ -321,6 +322,7 @@ struct NsGrains {
     const double *x, *vel, *xs;       // rows of 4: original order, sorted
     const uint32_t *cstart, *cend, *gidx;
     double *fd, *fp, *fv, *fsum;      // rows of 3
+    double *force;                    // rows of 3: the grains' forces (attached fluid), or null
 };
 
 // findDistMod, src/contactsearch.cuh:12-84
@@ -435,6 +437,7 @@ __global__ void k_ns_interaction(NsGrid g, NsPhys q, NsGrains G, const double *p
         G.fp[3 * i + k] = f_p;
         G.fv[3 * i + k] = f_v;
         G.fsum[3 * i + k] = f_d + f_p + f_v;
+        if (G.force) G.force[3 * i + k] += f_d + f_p + f_v;  // dev_force[i] += f_pf
     }
 }
 
@@ -678,6 +681,10 @@ struct sph_ns {
     NsGrains G{};
     void *grains = nullptr;
     Vec3 Fpf{};
+    // attached to a context (sph_ns_attach): force rows of its grains
+    int attached = 0;
+    double *own_forces = nullptr;
+    uint32_t own_cap = 0;
     int cur = 0;
     long iter = 0;
     double t = 0.0;
@@ -828,7 +835,104 @@ int ns_solve(sph_ns *n)
     return hip_ok(hipGetLastError(), "Navier-Stokes kernels");
 }
 
+// One DEM iteration of the fluid, src/device.cu:1213-1898
+int ns_step_body(sph_ns *n)
+{
+    const NsGrid &g = n->g;
+    const unsigned cb = cell_blocks(n);
+    if (n->G.np > 0)
+        hipLaunchKernelGGL(k_ns_porosity_grains, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->G,
+                           (int)(n->iter == 0), n->phi, n->dphi);
+    else
+        hipLaunchKernelGGL(k_ns_porosity, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->phi, n->dphi);
+    if (n->iter == 0) {
+        ghost_cells(n, n->v.c[0], n->v.c[1], n->v.c[2]);
+        const unsigned faces = (unsigned)(g.nx + 1) * (unsigned)(g.ny + 1) * (unsigned)(g.nz + 1);
+        hipLaunchKernelGGL(k_ns_center_to_face, dim3((faces + NS_THREADS - 1) / NS_THREADS), dim3(NS_THREADS), 0,
+                           n->s, g, cv(n->v), n->vf);
+    }
+    ghost_faces(n, n->vf);
+    hipLaunchKernelGGL(k_ns_div_tau, dim3(cb), dim3(NS_THREADS), 0, n->s, g, cv(n->vf), n->P.mu, n->dtau);
+    ghost_faces(n, n->dtau);
+    ghost_cells(n, n->p);
+    ghost_cells(n, n->phi);
+    if (n->G.np > 0) {  // src/device.cu:1310-1351
+        hipLaunchKernelGGL(k_ns_interaction, dim3((n->G.np + NS_THREADS - 1) / NS_THREADS), dim3(NS_THREADS), 0,
+                           n->s, g, n->q, n->G, n->phi, n->p, cv(n->vf), cv(n->dtau));
+        ghost_cells(n, n->p);
+        hipLaunchKernelGGL(k_ns_apply_force, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->G, n->Fpf);
+        ghost_cells(n, n->Fpf.c[0], n->Fpf.c[1], n->Fpf.c[2]);
+    }
+    if (n->iter % n->P.ndem == 0) {
+        const int rc = ns_solve(n);
+        if (rc) return rc;
+    }
+    n->t += n->P.dt;
+    n->iter++;
+    n->st.steps++;
+    return SPH_OK;
+}
+
+// grain force rows of an attached fluid, grown to np
+int ensure_force_rows(sph_ns *n, uint32_t np)
+{
+    if (np <= n->own_cap) return SPH_OK;
+    if (n->own_forces) (void)hipFree(n->own_forces);
+    n->own_forces = nullptr;
+    n->own_cap = 0;
+    if (hipMalloc(&n->own_forces, 12 * (size_t)np * sizeof(double)) != hipSuccess)
+        return fail(SPH_ERR_HIP, "device allocation failed");
+    if (hipMemsetAsync(n->own_forces, 0, 12 * (size_t)np * sizeof(double), n->s) != hipSuccess)
+        return fail(SPH_ERR_HIP, "hipMemsetAsync failed");
+    n->own_cap = np;
+    return SPH_OK;
+}
+
 }  // namespace
+
+int ns_coupled_step(sph_ns *n, uint32_t np, const double *x, const double *vel, const double *xs,
+                    const uint32_t *cstart, const uint32_t *cend, const uint32_t *gidx,
+                    const double *origo, int periodic, double *force, hipStream_t s)
+{
+    // the fluid's launches go on the context's stream, after its forces
+    hipStream_t own = n->s;
+    n->s = s;
+    int rc = ensure_force_rows(n, np);
+    if (!rc) {
+        n->G = NsGrains{};
+        n->G.np = np;
+        n->G.periodic = periodic;
+        n->G.c_phi = n->P.c_phi;
+        for (int k = 0; k < 3; ++k) {
+            n->G.origo[k] = origo[k];
+            n->G.L[k] = n->P.L[k];
+        }
+        n->G.x = x;
+        n->G.vel = vel;
+        n->G.xs = xs;
+        n->G.cstart = cstart;
+        n->G.cend = cend;
+        n->G.gidx = gidx;
+        n->G.fd = n->own_forces;
+        n->G.fp = n->G.fd + 3 * (size_t)np;
+        n->G.fv = n->G.fp + 3 * (size_t)np;
+        n->G.fsum = n->G.fv + 3 * (size_t)np;
+        n->G.force = force;
+        rc = ns_step_body(n);
+    }
+    n->s = own;
+    return rc;
+}
+
+void ns_set_attached(sph_ns *n, int on) { n->attached = on; }
+
+void ns_grid(const sph_ns *n, uint32_t num[3], double L[3])
+{
+    num[0] = n->P.nx;
+    num[1] = n->P.ny;
+    num[2] = n->P.nz;
+    for (int k = 0; k < 3; ++k) L[k] = n->P.L[k];
+}
 
 extern "C" {
 
@@ -938,6 +1042,7 @@ void sph_ns_destroy(sph_ns *n)
     if (n->s) hipStreamSynchronize(n->s);
     if (n->arena) hipFree(n->arena);
     if (n->grains) hipFree(n->grains);
+    if (n->own_forces) hipFree(n->own_forces);
     if (n->hst) hipHostFree(n->hst);
     if (n->s) hipStreamDestroy(n->s);
     delete n;
@@ -947,39 +1052,10 @@ int sph_ns_step(sph_ns *n, int nsteps)
 {
     if (!n || nsteps < 0) return fail(SPH_ERR_ARG, "bad argument");
     if (hipSetDevice(n->device) != hipSuccess) return fail(SPH_ERR_HIP, "hipSetDevice failed");
-    const NsGrid &g = n->g;
-    const unsigned cb = cell_blocks(n);
+    if (n->attached) return fail(SPH_ERR_STATE, "the fluid is attached to a context: step the context");
     for (int s = 0; s < nsteps; ++s) {
-        if (n->G.np > 0)
-            hipLaunchKernelGGL(k_ns_porosity_grains, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->G,
-                               (int)(n->iter == 0), n->phi, n->dphi);
-        else
-            hipLaunchKernelGGL(k_ns_porosity, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->phi, n->dphi);
-        if (n->iter == 0) {
-            ghost_cells(n, n->v.c[0], n->v.c[1], n->v.c[2]);
-            const unsigned faces = (unsigned)(g.nx + 1) * (unsigned)(g.ny + 1) * (unsigned)(g.nz + 1);
-            hipLaunchKernelGGL(k_ns_center_to_face, dim3((faces + NS_THREADS - 1) / NS_THREADS), dim3(NS_THREADS),
-                               0, n->s, g, cv(n->v), n->vf);
-        }
-        ghost_faces(n, n->vf);
-        hipLaunchKernelGGL(k_ns_div_tau, dim3(cb), dim3(NS_THREADS), 0, n->s, g, cv(n->vf), n->P.mu, n->dtau);
-        ghost_faces(n, n->dtau);
-        ghost_cells(n, n->p);
-        ghost_cells(n, n->phi);
-        if (n->G.np > 0) {  // src/device.cu:1310-1351
-            hipLaunchKernelGGL(k_ns_interaction, dim3((n->G.np + NS_THREADS - 1) / NS_THREADS), dim3(NS_THREADS),
-                               0, n->s, g, n->q, n->G, n->phi, n->p, cv(n->vf), cv(n->dtau));
-            ghost_cells(n, n->p);
-            hipLaunchKernelGGL(k_ns_apply_force, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->G, n->Fpf);
-            ghost_cells(n, n->Fpf.c[0], n->Fpf.c[1], n->Fpf.c[2]);
-        }
-        if (n->iter % n->P.ndem == 0) {
-            const int rc = ns_solve(n);
-            if (rc) return rc;
-        }
-        n->t += n->P.dt;
-        n->iter++;
-        n->st.steps++;
+        const int rc = ns_step_body(n);
+        if (rc) return rc;
     }
     const int rc = hip_ok(hipGetLastError(), "Navier-Stokes kernels");
     if (rc) return rc;
@@ -1034,6 +1110,7 @@ int sph_ns_set_grains(sph_ns *n, uint32_t np, const double *x, const double *vel
                       int32_t periodic)
 {
     if (!n || (np && (!x || !vel)) || !origo) return fail(SPH_ERR_ARG, "null argument");
+    if (n->attached) return fail(SPH_ERR_STATE, "the fluid is attached to a context and moves its grains");
     if (hipSetDevice(n->device) != hipSuccess) return fail(SPH_ERR_HIP, "hipSetDevice failed");
     const NsGrid &g = n->g;
     const uint32_t ncells = (uint32_t)(g.nx * g.ny * g.nz);

@@ -43,6 +43,7 @@ def lib():
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_contact_slots.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
+        L.orc_attach_ns.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -86,6 +87,16 @@ class Oracle:
     def _check(self, rc):
         if rc != 0:
             raise OracleError(rc, self.L.orc_error(self.h).decode())
+
+    def attach_ns(self, nso):
+        """Couple an NsOracle fluid (tests/_ns_oracle.py) to this oracle's
+        grains (None detaches)."""
+        self.L.orc_attach_ns(self.h, nso.h if nso is not None else None)
+        if nso is not None:
+            h = _abi.SphState()
+            self.L.orc_get_header(self.h, ctypes.byref(h))
+            nso.np = int(h.np)  # rows of grain_forces()
+        self._ns = nso
 
     def step(self, n=1):
         self._check(self.L.orc_step(self.h, int(n)))

@@ -331,3 +331,60 @@ def test_ns_grain_outside_refused(torch):
     with pytest.raises(sphere_amd.SphereError):
         ns.set_grains([[0.05, 0.05, 0.2, 0.01]], [[0, 0, 0]])
     ns.close()
+
+
+def test_ns_fluid_moves_grains(torch):
+    """A Navier-Stokes fluid attached to a context (sph_ns_attach): 512
+    grains at rest on a loose periodic lattice in water flowing along x
+    (0.01 m/s with a 0.002 m/s random part); each step runs the fluid
+    between the grains' contact forces and the integration and adds its
+    force on them.  Against the oracle with the same coupling
+    (orc_attach_ns) over 20 steps: positions, velocities, the fluid and the
+    forces on the grains within 1e-9 of their largest magnitudes (the drag
+    uses the device's pow, exp and log10), porosities within 1e-12.  (With
+    the DEM time step, the porosity-change term rho dphi / (dt^2 phi) of the
+    reference's pressure equation is large for fast grains, so the grains
+    start at rest here.)"""
+    from _cases import dense_case
+    from _oracle import Oracle
+    s = dense_case(n=512, seed=3, periodic=1, vmag=0.0, wmag=0.0, gravity=False, spacing_factor=1.1)
+    st = s.to_state()
+    nx, ny, nz = (int(v) for v in s.num)
+    P = _abi.ns_params(nx, ny, nz, s.L, float(s.time_dt[0]), mu=1.0e-3, rho_f=1000.0, g=s.g,
+                       bc_bot=1, bc_top=0, tolerance=1e-10, maxiter=100000,
+                       t0=float(s.time_current[0]))
+    rng = numpy.random.default_rng(5)
+    v = numpy.zeros((nx, ny, nz, 3))
+    v[..., 0] = 0.01
+    v += 0.002 * rng.standard_normal(v.shape)
+    p = numpy.zeros((nx, ny, nz))
+    print("ns fluid moving grains", (nx, ny, nz), flush=True)
+    e = sphere_amd.Engine(0)
+    e.load(st)
+    ns = sphere_amd.NavierStokes(P, v, p)
+    e.attach_fluid(ns)
+    o = Oracle(st)
+    nso = NsOracle(P, v, p)
+    o.attach_ns(nso)
+    for k in range(4):
+        e.step(5)
+        o.step(5)
+        ge, go = e.read(), o.read()
+        for name, w in (("x", 4), ("vel", 4)):
+            a, b = ge.get(name, 512, w), go.get(name, 512, w)
+            assert numpy.max(numpy.abs(a - b)) <= 1e-9 * numpy.abs(b).max(), (k, name)
+        fa, fb = ns.read(), nso.read()
+        for a, b in zip(fa[:2], fb[:2]):
+            assert numpy.max(numpy.abs(a - b)) <= 1e-9 * max(numpy.abs(b).max(), 1e-30), k
+        assert numpy.max(numpy.abs(fa[2] - fb[2])) <= 1e-12, k
+        for a, b in zip(ns.grain_forces(), nso.grain_forces()):
+            assert numpy.max(numpy.abs(a - b)) <= 1e-9 * max(numpy.abs(b).max(), 1e-30), k
+    assert fa[2].min() < 0.9
+    assert numpy.abs(ns.grain_forces()[0]).max() > 0
+    assert numpy.abs(ge.get("vel", 512, 4)[:, :3]).max() > 0  # the fluid moved the grains
+    e.attach_fluid(None)
+    with pytest.raises(sphere_amd.SphereError):
+        e.attach_fluid(sphere_amd.NavierStokes(_abi.ns_params(3, 3, 3, (0.1, 0.1, 0.1), 1e-4),
+                                               numpy.zeros((3, 3, 3, 3)), numpy.zeros((3, 3, 3))))
+    e.close()
+    ns.close()
