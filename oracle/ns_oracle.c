@@ -47,6 +47,11 @@ typedef struct {
     uint32_t *cstart, *cend, *gidx;
     double *fd, *fp, *fv, *fsum, *Fpf[3];
     double *force;  /* coupled to the DEM oracle: its grains' force rows */
+    /* the top wall (coupled, wmode 1 or 3): its host position (< 0: none),
+     * the layer it is in and the hydrostatic step per layer
+     * (src/device.cu:1008-1015, 1380-1392) */
+    double wall_x, dp_dz;
+    unsigned wall0_iz;
     long iter;
     double t;
     sph_ns_stats st;
@@ -409,6 +414,7 @@ static void predict(nso *o)
         }
         if (z == 0 && P->bc_bot == 1) vp[2] = 0.0;
         if (z == 0 && P->bc_bot == 2) vp[0] = vp[1] = vp[2] = 0.0;
+        if ((unsigned)z >= o->wall0_iz) vp[0] = vp[1] = vp[2] = 0.0;
         for (int k = 0; k < 3; ++k) o->vpf[k][f] = vp[k];
     }
 }
@@ -463,11 +469,13 @@ static void jacobi(nso *o)
         const size_t c = IDX(o, x, y, z);
         const double *E = o->eps;
         const double e = E[c];
-        const double e_new = (-dxdx * dydy * dzdz * o->f[c] +
-                              dydy * dzdz * (E[IDX(o, x - 1, y, z)] + E[IDX(o, x + 1, y, z)]) +
-                              dxdx * dzdz * (E[IDX(o, x, y - 1, z)] + E[IDX(o, x, y + 1, z)]) +
-                              dxdx * dydy * (E[IDX(o, x, y, z - 1)] + E[IDX(o, x, y, z + 1)])) /
-                             (2.0 * (dxdx * dydy + dxdx * dzdz + dydy * dzdz));
+        double e_new = (-dxdx * dydy * dzdz * o->f[c] +
+                        dydy * dzdz * (E[IDX(o, x - 1, y, z)] + E[IDX(o, x + 1, y, z)]) +
+                        dxdx * dzdz * (E[IDX(o, x, y - 1, z)] + E[IDX(o, x, y + 1, z)]) +
+                        dxdx * dydy * (E[IDX(o, x, y, z - 1)] + E[IDX(o, x, y, z + 1)])) /
+                       (2.0 * (dxdx * dydy + dxdx * dzdz + dydy * dzdz));
+        if ((unsigned)z == o->wall0_iz) e_new = e;
+        if ((unsigned)z == o->wall0_iz + 1u) e_new = E[IDX(o, x, y, z - 1)] + o->dp_dz;
         o->norm[c] = (e_new - e) * (e_new - e) / (e_new * e_new + 1.0e-16);
         o->eps_new[c] = e * (1.0 - theta) + e_new * theta;
     }
@@ -509,6 +517,7 @@ static void update_velocity(nso *o)
         }
         if (z == 0 && P->bc_bot == 1) v[2] = 0.0;
         if (z == 0 && P->bc_bot == 2) v[0] = v[1] = v[2] = 0.0;
+        if ((unsigned)z >= o->wall0_iz) v[0] = v[1] = v[2] = 0.0;
         for (int k = 0; k < 3; ++k) o->vf[k][f] = v[k];
     }
 }
@@ -531,7 +540,15 @@ static int ns_solve(nso *o)
     const sph_ns_params *P = &o->P;
     const int nz = o->nz;
     const double ptop = o->ph[IDX(o, 0, 0, nz - 1)];
-    set_eps_layer(o, nz - 1, nz - 1, ptop - P->beta * ptop, 1);
+    const double eps_top = ptop - P->beta * ptop;
+    set_eps_layer(o, nz - 1, nz - 1, eps_top, 1);
+    if (o->wall_x >= 0.0) {
+        /* setNSepsilonAtTopWall, navierstokes.cuh:448-516 */
+        o->wall0_iz = (unsigned)(o->wall_x / (P->L[2] / P->nz));
+        if (o->wall0_iz < (unsigned)nz) set_eps_layer(o, (int)o->wall0_iz, (int)o->wall0_iz, eps_top, 1);
+        if (o->wall0_iz + 1u < (unsigned)nz)
+            set_eps_layer(o, (int)o->wall0_iz + 1, (int)o->wall0_iz + 1, eps_top - o->dp_dz, 1);
+    }
     if ((P->p_mod_A > 1.0e-5 || P->p_mod_A < -1.0e-5) && P->p_mod_f > 1.0e-7) {
         /* setUpperPressureNS, navierstokes.cuh:1550-1588 */
         const double newp = o->ph[IDX(o, 0, 0, nz - 1)] +
@@ -660,9 +677,11 @@ int nso_step(void *h, int nsteps)
  * original indices), the fluid's force on each grain added to force
  * (rows of 3). */
 int nso_step_coupled(void *h, uint32_t np, double *x, double *vel, double *xs, uint32_t *cstart,
-                     uint32_t *cend, uint32_t *gidx, const double *origo, int periodic, double *force)
+                     uint32_t *cend, uint32_t *gidx, const double *origo, int periodic, double *force,
+                     double wall_x)
 {
     nso *o = (nso *)h;
+    o->wall_x = wall_x;
     if (np != o->np || !o->fd) {
         free(o->fd); free(o->fp); free(o->fv); free(o->fsum);
         o->fd = (double *)calloc(3 * (size_t)(np ? np : 1), sizeof(double));
@@ -727,6 +746,9 @@ void *nso_create(const sph_ns_params *p, const double *v, const double *pf)
     }
     memcpy(o->ph, o->p, o->nc * sizeof(double));
     o->t = p->t0;
+    o->wall_x = -1.0;
+    o->wall0_iz = 10000000u;
+    o->dp_dz = fabs(p->rho_f * p->g[2] * p->L[2] / p->nz);
     return o;
 }
 

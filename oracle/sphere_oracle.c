@@ -76,7 +76,8 @@ void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 
 /* ns_oracle.c: one fluid iteration with the DEM oracle's grains */
 int nso_step_coupled(void *h, uint32_t np, double *x, double *vel, double *xs, uint32_t *cstart,
-                     uint32_t *cend, uint32_t *gidx, const double *origo, int periodic, double *force);
+                     uint32_t *cend, uint32_t *gidx, const double *origo, int periodic, double *force,
+                     double wall_x);
 
 int orc_get_threads(void) { return g_threads; }
 /* contact slots per particle of the next orc_create (the reference's NC) */
@@ -1531,8 +1532,9 @@ static int step_once(orc *o)
         if (rc) return rc;
     }
     if (o->ns && o->np > 0) {
+        const int wall = o->nw > 0 && (h->wmode[0] == 1 || h->wmode[0] == 3);
         rc = nso_step_coupled(o->ns, o->np, o->x, o->vel, o->x_s, o->cell_start, o->cell_end, o->index,
-                              h->origo, h->periodic, o->force);
+                              h->origo, h->periodic, o->force, wall ? o->host_wx0 : -1.0);
         if (rc) return rc;
     }
     if (o->np > 0) {

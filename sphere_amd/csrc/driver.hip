@@ -1242,7 +1242,8 @@ static int step_once(sph_ctx *ctx, std::ofstream *convlog, const std::string *di
         // integration, its force on each grain added (src/device.cu:1213-1898)
         rc = ns_coupled_step(ctx->ns, h.np, (const double *)ctx->db.x, (const double *)ctx->db.vel,
                              (const double *)ctx->db.x_s, ctx->db.cell_start, ctx->db.cell_end, ctx->db.vals,
-                             h.origo, h.periodic, (double *)ctx->db.force, s);
+                             h.origo, h.periodic, (double *)ctx->db.force,
+                             h.nw > 0 && (h.wmode[0] == 1 || h.wmode[0] == 3) ? ctx->host_wx0 : -1.0, s);
         if (rc) return fail(ctx, rc, sph_ns_last_error());
     }
     if (h.np > 0) {

@@ -10,10 +10,12 @@
 // x and vel rows of 4, the cell sort's sorted x, cell starts and ends and
 // original indices), enqueued on stream s: porosities, the interaction
 // forces (each grain's sum added to force, rows of 3) and, every ndem
-// iterations, the solve (src/device.cu:1213-1898).  0 or an SPH_ERR_* code.
+// iterations, the solve (src/device.cu:1213-1898); wall_x >= 0 is the host
+// copy of a top wall's position (wmode 1 or 3), < 0 none.  0 or an SPH_ERR_*
+// code.
 int ns_coupled_step(sph_ns *ns, uint32_t np, const double *x, const double *vel, const double *xs,
                     const uint32_t *cstart, const uint32_t *cend, const uint32_t *gidx,
-                    const double *origo, int periodic, double *force, hipStream_t s);
+                    const double *origo, int periodic, double *force, double wall_x, hipStream_t s);
 // The fluid's grid, to check that it is the context's grid.
 void ns_grid(const sph_ns *ns, uint32_t num[3], double L[3]);
 // Marks the fluid attached to a context (1) or free (0).

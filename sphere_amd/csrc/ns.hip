@@ -277,7 +277,8 @@ struct NsPhys {
 
 // findPredNSvelocities, src/navierstokes.cuh:2276-2495 (SET_2, np = 0)
 __global__ void k_ns_predict(NsGrid g, NsPhys q, const double *p, CVec3 VF, const double *phi_a,
-                             const double *dphi_a, CVec3 DT, CVec3 DPVV, CVec3 FPF, int coupled, Vec3 VP)
+                             const double *dphi_a, CVec3 DT, CVec3 DPVV, CVec3 FPF, int coupled,
+                             unsigned wall0_iz, Vec3 VP)
 {
     int x, y, z;
     if (!cell_of(g, x, y, z)) return;
@@ -311,6 +312,7 @@ __global__ void k_ns_predict(NsGrid g, NsPhys q, const double *p, CVec3 VF, cons
     }
     if (z == 0 && g.bb == 1) vp[2] = 0.0;
     if (z == 0 && g.bb == 2) vp[0] = vp[1] = vp[2] = 0.0;
+    if ((unsigned)z >= wall0_iz) vp[0] = vp[1] = vp[2] = 0.0;
     for (int k = 0; k < 3; ++k) VP.c[k][f] = vp[k];
 }
 
@@ -526,7 +528,8 @@ struct NsSolveState {
 // check: the iteration's max |norm| goes into st->maxw (bit pattern).
 __global__ void __launch_bounds__(NS_THREADS)
 k_ns_jacobi(NsGrid g, NsPhys q, unsigned nijac, const double *A, double *B, double *f1a, Vec3 f2a,
-            CVec3 VPc, CVec3 VPF, const double *phi_a, const double *dphi_a, NsSolveState *st, int check)
+            CVec3 VPc, CVec3 VPF, const double *phi_a, const double *dphi_a, NsSolveState *st, int check,
+            unsigned wall0_iz, double dp_dz)
 {
     if (*(volatile unsigned *)&st->done) return;
     int x, y, z;
@@ -571,9 +574,11 @@ k_ns_jacobi(NsGrid g, NsPhys q, unsigned nijac, const double *A, double *B, doub
             const double e_xn = A[ghost_source(g, x - 1, y, z)], e_xp = A[ghost_source(g, x + 1, y, z)];
             const double e_yn = A[ghost_source(g, x, y - 1, z)], e_yp = A[ghost_source(g, x, y + 1, z)];
             const double e_zn = A[ghost_source(g, x, y, z - 1)], e_zp = A[ghost_source(g, x, y, z + 1)];
-            const double e_new = (-dxdx * dydy * dzdz * f + dydy * dzdz * (e_xn + e_xp) +
-                                  dxdx * dzdz * (e_yn + e_yp) + dxdx * dydy * (e_zn + e_zp)) /
-                                 (2.0 * (dxdx * dydy + dxdx * dzdz + dydy * dzdz));
+            double e_new = (-dxdx * dydy * dzdz * f + dydy * dzdz * (e_xn + e_xp) +
+                            dxdx * dzdz * (e_yn + e_yp) + dxdx * dydy * (e_zn + e_zp)) /
+                           (2.0 * (dxdx * dydy + dxdx * dzdz + dydy * dzdz));
+            if ((unsigned)z == wall0_iz) e_new = e;
+            if ((unsigned)z == wall0_iz + 1u) e_new = e_zn + dp_dz;
             const double norm = (e_new - e) * (e_new - e) / (e_new * e_new + 1.0e-16);
             out = e * (1.0 - q.theta) + e_new * q.theta;
             bits = (unsigned long long)__double_as_longlong(fabs(norm));
@@ -642,7 +647,7 @@ __global__ void k_ns_update_p(NsGrid g, const double *eps, double beta, double *
 
 // updateNSvelocity, src/navierstokes.cuh:2915-3057 (SET_2, no wall)
 __global__ void k_ns_update_v(NsGrid g, NsPhys q, double c_v_ndem_dt, CVec3 VPF, const double *eps,
-                              Vec3 VF)
+                              unsigned wall0_iz, Vec3 VF)
 {
     int x, y, z;
     if (!cell_of(g, x, y, z)) return;
@@ -657,6 +662,7 @@ __global__ void k_ns_update_v(NsGrid g, NsPhys q, double c_v_ndem_dt, CVec3 VPF,
     }
     if (z == 0 && g.bb == 1) v[2] = 0.0;
     if (z == 0 && g.bb == 2) v[0] = v[1] = v[2] = 0.0;
+    if ((unsigned)z >= wall0_iz) v[0] = v[1] = v[2] = 0.0;
     for (int k = 0; k < 3; ++k) VF.c[k][f] = v[k];
 }
 
@@ -685,6 +691,10 @@ struct sph_ns {
     int attached = 0;
     double *own_forces = nullptr;
     uint32_t own_cap = 0;
+    // the context's top wall (wmode 1 or 3): its host position (< 0: none),
+    // its layer and the hydrostatic step per layer (src/device.cu:1008-1015)
+    double wall_x = -1.0, dp_dz = 0.0;
+    unsigned wall0_iz = 10000000u;
     int cur = 0;
     long iter = 0;
     double t = 0.0;
@@ -752,7 +762,15 @@ int ns_solve(sph_ns *n)
     const NsGrid &g = n->g;
     const unsigned cb = cell_blocks(n);
     const double ptop = n->ph[cidx(g, 0, 0, g.nz - 1)];
-    set_layers(n, g.nz - 1, g.nz - 1, ptop - P.beta * ptop);
+    const double eps_top = ptop - P.beta * ptop;
+    set_layers(n, g.nz - 1, g.nz - 1, eps_top);
+    if (n->wall_x >= 0.0) {
+        // setNSepsilonAtTopWall, src/navierstokes.cuh:448-516
+        n->wall0_iz = (unsigned)(n->wall_x / (P.L[2] / P.nz));
+        if (n->wall0_iz < (unsigned)g.nz) set_layers(n, (int)n->wall0_iz, (int)n->wall0_iz, eps_top);
+        if (n->wall0_iz + 1u < (unsigned)g.nz)
+            set_layers(n, (int)n->wall0_iz + 1, (int)n->wall0_iz + 1, eps_top - n->dp_dz);
+    }
     if ((P.p_mod_A > 1.0e-5 || P.p_mod_A < -1.0e-5) && P.p_mod_f > 1.0e-7) {
         // the reference forms the new top pressure on the host
         // (src/device.cu:1400-1412)
@@ -771,7 +789,7 @@ int ns_solve(sph_ns *n)
     ghost_cells(n, n->dpvv.c[0], n->dpvv.c[1], n->dpvv.c[2]);
     const int coupled = P.mu > 0.0 && n->G.np > 0;
     hipLaunchKernelGGL(k_ns_predict, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->q, n->p, cv(n->vf), n->phi,
-                       n->dphi, cv(n->dtau), cv(n->dpvv), cv(n->Fpf), coupled, n->vpf);
+                       n->dphi, cv(n->dtau), cv(n->dpvv), cv(n->Fpf), coupled, n->wall0_iz, n->vpf);
     ghost_faces(n, n->vpf);
     face_to_center(n, n->vpf, n->vp);
     if (n->iter == 0) {
@@ -794,7 +812,7 @@ int ns_solve(sph_ns *n)
             const int a = cur0 ^ (int)(it & 1u);
             hipLaunchKernelGGL(k_ns_jacobi, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->q, it, n->eps[a],
                                n->eps[1 - a], n->f1, n->f2, cv(n->vp), cv(n->vpf), n->phi, n->dphi, n->dst,
-                               check);
+                               check, n->wall0_iz, n->dp_dz);
             if (check)
                 hipLaunchKernelGGL(k_ns_decide, dim3(1), dim3(64), 0, n->s, n->dst, it, P.tolerance);
         }
@@ -829,7 +847,7 @@ int ns_solve(sph_ns *n)
     hipLaunchKernelGGL(k_ns_update_p, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->eps[n->cur], P.beta, n->p);
     const double c_v_ndem_dt = P.c_v * P.ndem * P.dt;
     hipLaunchKernelGGL(k_ns_update_v, dim3(cb), dim3(NS_THREADS), 0, n->s, g, n->q, c_v_ndem_dt, cv(n->vpf),
-                       n->eps[n->cur], n->vf);
+                       n->eps[n->cur], n->wall0_iz, n->vf);
     ghost_faces(n, n->vpf);
     face_to_center(n, n->vf, n->v);
     return hip_ok(hipGetLastError(), "Navier-Stokes kernels");
@@ -892,8 +910,9 @@ int ensure_force_rows(sph_ns *n, uint32_t np)
 
 int ns_coupled_step(sph_ns *n, uint32_t np, const double *x, const double *vel, const double *xs,
                     const uint32_t *cstart, const uint32_t *cend, const uint32_t *gidx,
-                    const double *origo, int periodic, double *force, hipStream_t s)
+                    const double *origo, int periodic, double *force, double wall_x, hipStream_t s)
 {
+    n->wall_x = wall_x;
     // the fluid's launches go on the context's stream, after its forces
     hipStream_t own = n->s;
     n->s = s;
@@ -1033,6 +1052,7 @@ sph_ns *sph_ns_create(int device, const sph_ns_params *params, const double *v, 
         return nullptr;
     }
     n->t = P.t0;
+    n->dp_dz = std::fabs(P.rho_f * P.g[2] * P.L[2] / P.nz);
     return n;
 }
 

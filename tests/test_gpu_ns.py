@@ -388,3 +388,49 @@ def test_ns_fluid_moves_grains(torch):
                                                numpy.zeros((3, 3, 3, 3)), numpy.zeros((3, 3, 3))))
     e.close()
     ns.close()
+
+
+def test_ns_fluid_with_top_wall(torch):
+    """The attached fluid under a top wall (wmode 1): at each solve the
+    layer holding the wall and the one above take the top pressure and that
+    minus rho |g| dz (setNSepsilonAtTopWall), the Jacobi iteration holds
+    them so, and the predicted and corrected velocities vanish from the
+    wall's layer up (src/navierstokes.cuh:448-516, 2443-2444, 2758-2765,
+    3013-3014).  Within 1e-9 of the oracle over 10 steps; no flow at or
+    above the wall."""
+    from _cases import dense_case
+    from _oracle import Oracle
+    s = dense_case(n=343, seed=4, periodic=1, nw=1, vmag=0.0, wmag=0.0, gravity=True, spacing_factor=1.1)
+    st = s.to_state()
+    nx, ny, nz = (int(v) for v in s.num)
+    iz = int(s.w_x[0] / (s.L[2] / nz))
+    assert iz < nz - 1
+    P = _abi.ns_params(nx, ny, nz, s.L, float(s.time_dt[0]), mu=1.0e-3, rho_f=1000.0, g=s.g,
+                       bc_bot=1, bc_top=0, tolerance=1e-10, maxiter=100000,
+                       t0=float(s.time_current[0]))
+    rng = numpy.random.default_rng(6)
+    v = 0.002 * rng.standard_normal((nx, ny, nz, 3))
+    p = numpy.zeros((nx, ny, nz))
+    print("ns fluid under a top wall, wall layer", iz, "of", nz, flush=True)
+    e = sphere_amd.Engine(0)
+    e.load(st)
+    ns = sphere_amd.NavierStokes(P, v, p)
+    e.attach_fluid(ns)
+    o = Oracle(st)
+    nso = NsOracle(P, v, p)
+    o.attach_ns(nso)
+    e.step(10)
+    o.step(10)
+    ge, go = e.read(), o.read()
+    for name in ("x", "vel"):
+        a, b = ge.get(name, 343, 4), go.get(name, 343, 4)
+        assert numpy.max(numpy.abs(a - b)) <= 1e-9 * numpy.abs(b).max(), name
+    fa, fb = ns.read(), nso.read()
+    for a, b in zip(fa[:2], fb[:2]):
+        assert numpy.max(numpy.abs(a - b)) <= 1e-9 * max(numpy.abs(b).max(), 1e-30)
+    faces = [ns.field(k).reshape(nz + 3, ny + 3, nx + 3)[1:nz + 1, 1:ny + 1, 1:nx + 1] for k in range(3)]
+    for f in faces:
+        assert numpy.all(f[iz:] == 0.0)
+    assert numpy.abs(faces[2][:iz]).max() > 0
+    e.close()
+    ns.close()
