@@ -845,6 +845,14 @@ int nso_set_grains(void *h, uint32_t np, const double *x, const double *vel, con
     return 0;
 }
 
+/* An output event of the coupled DEM oracle: the host copy of p refreshed
+ * (src/device.cu:2576-2596) */
+void nso_output_event(void *h)
+{
+    nso *o = (nso *)h;
+    memcpy(o->ph, o->p, o->nc * sizeof(double));
+}
+
 /* The last step's forces on the grains (np rows of 3 each) */
 void nso_grain_forces(void *h, double *f_d, double *f_p, double *f_v, double *f_sum)
 {

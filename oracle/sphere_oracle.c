@@ -78,6 +78,7 @@ void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 int nso_step_coupled(void *h, uint32_t np, double *x, double *vel, double *xs, uint32_t *cstart,
                      uint32_t *cend, uint32_t *gidx, const double *origo, int periodic, double *force,
                      double wall_x);
+void nso_output_event(void *h);
 
 int orc_get_threads(void) { return g_threads; }
 /* contact slots per particle of the next orc_create (the reference's NC) */
@@ -1563,6 +1564,7 @@ static int step_once(orc *o)
         /* CS4: host copies refresh, dphi zeroed after the transfer, file
          * number advanced (src/device.cu:2576-2680) */
         o->host_wx0 = o->nw > 0 ? h->w_nx[3] : 0.0;
+        if (o->ns) nso_output_event(o->ns);
         if (h->fluid) {
             for (uint32_t z = 0; z < h->num[2]; ++z)
                 for (uint32_t y = 0; y < h->num[1]; ++y)

@@ -531,15 +531,15 @@ def _one_value(a):
 
 
 def ns_params_from_sim(s, grains=False):
-    """sph_ns_params of a sim with a Navier-Stokes fluid (cfd_solver = 0)
-    and no walls: its grid, time step and time, fluid and solver fields (the
-    ones the reference's binary file carries).  Grains only with grains=True
-    (a fluid attached to an Engine that steps them)."""
+    """sph_ns_params of a sim with a Navier-Stokes fluid (cfd_solver = 0):
+    its grid, time step and time, fluid and solver fields (the ones the
+    reference's binary file carries).  Grains and walls only with
+    grains=True (a fluid attached to an Engine that steps them)."""
     if not s.fluid or int(_one_value(s.cfd_solver)) != 0:
         raise SphereError(SPH_ERR_ARG, "the sim has no Navier-Stokes fluid (cfd_solver = 0)")
-    if (int(_one_value(s.np)) != 0 and not grains) or int(_one_value(s.nw)) != 0:
-        raise SphereError(SPH_ERR_ARG, "only a fluid without walls (and without grains unless "
-                                       "attached to an Engine) is built")
+    if (int(_one_value(s.np)) != 0 or int(_one_value(s.nw)) != 0) and not grains:
+        raise SphereError(SPH_ERR_ARG, "grains and walls need the fluid attached to an Engine "
+                                       "(grains=True)")
     return ns_params(*[int(n) for n in s.num], L=s.L, dt=_one_value(s.time_dt),
                      mu=_one_value(s.mu), rho_f=_one_value(s.rho_f), g=s.g,
                      beta=_one_value(s.beta), theta=_one_value(s.theta), gamma=_one_value(s.gamma),

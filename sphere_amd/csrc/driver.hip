@@ -1141,6 +1141,10 @@ static int output_event(sph_ctx *ctx, const std::string *dir, const std::string 
     if (rc) return rc;
     ctx->host_wx0 = h.nw > 0 ? h.w_nx[3] : 0.0;
     if (h.fluid) launch_darcy_zero_interior(ctx->stream, darcy_args(ctx), ctx->dd.dphi);
+    if (ctx->ns) {
+        rc = ns_output_event(ctx->ns, ctx->stream);
+        if (rc) return fail(ctx, rc, sph_ns_last_error());
+    }
     h.time_step_count += 1;
     if (dir) {
         char name[64];

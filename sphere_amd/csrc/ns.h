@@ -20,3 +20,7 @@ int ns_coupled_step(sph_ns *ns, uint32_t np, const double *x, const double *vel,
 void ns_grid(const sph_ns *ns, uint32_t num[3], double L[3]);
 // Marks the fluid attached to a context (1) or free (0).
 void ns_set_attached(sph_ns *ns, int on);
+// An output event of the attached context: the host copy of p that the
+// fluid's boundary guesses read is refreshed, as the reference's transfer
+// for output does (src/device.cu:2576-2596).
+int ns_output_event(sph_ns *ns, hipStream_t s);

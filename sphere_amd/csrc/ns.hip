@@ -945,6 +945,14 @@ int ns_coupled_step(sph_ns *n, uint32_t np, const double *x, const double *vel, 
 
 void ns_set_attached(sph_ns *n, int on) { n->attached = on; }
 
+int ns_output_event(sph_ns *n, hipStream_t s)
+{
+    if (hipMemcpyAsync(n->ph.data(), n->p, n->nc * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return fail(SPH_ERR_HIP, "download of the pressure failed");
+    return SPH_OK;
+}
+
 void ns_grid(const sph_ns *n, uint32_t num[3], double L[3])
 {
     num[0] = n->P.nx;

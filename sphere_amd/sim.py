@@ -1643,8 +1643,9 @@ class sim:
 
 def _run_navier_stokes(path, outdir, verbose, dry, device):
     """The time loop of src/device.cu:833-2834 for an input file with a
-    Navier-Stokes fluid (cfd_solver = 0) and no grains or walls, on the
-    GPU through NavierStokes: output00000 when the step count is 0, then
+    Navier-Stokes fluid (cfd_solver = 0), on the GPU through NavierStokes
+    (attached to an Engine that steps the grains, if any): output00000 when
+    the step count is 0, then
     DEM iterations until time.current passes time.total, the conv.log line
     of each solve (every tenth iteration's, and every one that reaches
     maxiter, src/device.cu:1792-1820), and at every file interval the
@@ -1660,8 +1661,22 @@ def _run_navier_stokes(path, outdir, verbose, dry, device):
               "  - File output interval: %g s\n  - Fluid grid: %d*%d*%d (Navier-Stokes)"
               % (int(r.np[0]), r.time_dt[0], r.time_total[0], r.time_file_dt[0], *r.num))
         return 0
+    grains = int(r.np[0]) > 0
     try:
-        ns = _abi.NavierStokes.from_sim(r, device=max(0, int(device)))
+        if grains:
+            # the grains step in a context; the fluid is attached to it and
+            # runs inside each of its steps
+            P = _abi.ns_params_from_sim(r, grains=True)
+            r.fluid = False
+            st = r.to_state()
+            r.fluid = True
+            eng = _abi.Engine(max(0, int(device)))
+            eng.load(st)
+            ns = _abi.NavierStokes(P, r.v_f, r.p_f, device=max(0, int(device)))
+            eng.attach_fluid(ns)
+        else:
+            eng = None
+            ns = _abi.NavierStokes.from_sim(r, device=max(0, int(device)))
     except _abi.SphereError as e:
         print("Warning: the sphere run returned with status %d: %s" % (e.code, e))
         return e.code
@@ -1686,7 +1701,10 @@ def _run_navier_stokes(path, outdir, verbose, dry, device):
         while r.time_current[0] <= r.time_total[0]:
             before = ns.stats()
             try:
-                ns.step(1)
+                if eng is not None:
+                    eng.step(1)
+                else:
+                    ns.step(1)
             except _abi.SphereError as e:
                 print("Warning: the sphere run returned with status %d: %s" % (e.code, e))
                 rc = e.code
@@ -1706,11 +1724,23 @@ def _run_navier_stokes(path, outdir, verbose, dry, device):
             if verbose and it % 10000 == 0:
                 print("\r  Current time: %g/%g s." % (r.time_current[0], r.time_total[0]), end="")
             if filetimeclock >= r.time_file_dt[0]:
-                ns.to_sim(r)
-                r.time_step_count[0] += 1
+                if eng is not None:
+                    # the context's output event has advanced its file
+                    # number and refreshed the fluid's host copy of p
+                    r.from_state(eng.read())
+                    r.fluid = True
+                    r.cfd_solver = numpy.zeros(1, dtype=numpy.int32)
+                    v, p, phi, dphi = ns.read()
+                    r.v_f, r.p_f, r.phi, r.dphi = v, p, phi, dphi
+                    r.f_d, r.f_p, r.f_v, r.f_sum = ns.grain_forces()
+                else:
+                    ns.to_sim(r)
+                    r.time_step_count[0] += 1
                 write(".output%05d.bin" % int(r.time_step_count[0]))
                 status_file()
                 filetimeclock = 0.0
+    if eng is not None:
+        eng.close()
     ns.close()
     if verbose:
         print()

@@ -434,3 +434,46 @@ def test_ns_fluid_with_top_wall(torch):
     assert numpy.abs(faces[2][:iz]).max() > 0
     e.close()
     ns.close()
+
+
+def test_sim_run_fluid_with_grains(torch, rundir):
+    """sim.run() of a file with a Navier-Stokes fluid, grains and a top
+    wall: the grains step in a context with the fluid attached, output
+    files at every file interval carry both; the last one within 1e-9 of
+    the oracle coupled the same way and stepped to the same step count."""
+    from _cases import dense_case
+    from _oracle import Oracle
+    s = dense_case(n=343, seed=7, periodic=1, nw=1, vmag=0.0, wmag=0.0, gravity=True, spacing_factor=1.1)
+    s.sid = "nsgrains"
+    dt = float(s.time_dt[0])
+    s.initTemporal(total=40 * dt, file_dt=10 * dt, dt=dt)
+    s.initFluid(mu=1.0e-3, cfd_solver=0)
+    s.bc_bot[0] = 1
+    s.setTolerance(1e-10)
+    s.setMaxIterations(100000)
+    rng = numpy.random.default_rng(8)
+    s.v_f[:] = 0.002 * rng.standard_normal(s.v_f.shape)
+    P = _abi.ns_params_from_sim(s, grains=True)
+    s.fluid = False
+    st = s.to_state()
+    s.fluid = True
+    print("sim.run fluid with grains", flush=True)
+    assert s.run(verbose=False) == 0
+    nsteps, outs = _schedule(dt, 40 * dt, 10 * dt)
+    py = sphere_amd.sim(sid="nsgrains", fluid=True)
+    py.readlast(verbose=False)
+    assert int(py.time_step_count[0]) == len(outs) and int(py.cfd_solver[0]) == 0
+    o = Oracle(st)
+    nso = NsOracle(P, s.v_f, s.p_f)
+    o.attach_ns(nso)
+    o.step(outs[-1])
+    go = o.read()
+    n = int(s.np[0])
+    xo = go.get("x", n, 4)
+    xg = numpy.column_stack([py.x, py.radius])
+    assert numpy.max(numpy.abs(xg - xo)) <= 1e-9 * numpy.abs(xo).max()
+    v, p, phi, _ = nso.read()
+    assert numpy.max(numpy.abs(py.p_f - p)) <= 1e-9 * max(numpy.abs(p).max(), 1e-30)
+    assert numpy.max(numpy.abs(py.phi - phi)) <= 1e-12
+    f_d = nso.grain_forces()[0]
+    assert numpy.max(numpy.abs(py.f_d - f_d)) <= 1e-9 * max(numpy.abs(f_d).max(), 1e-30)
