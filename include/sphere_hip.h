@@ -404,17 +404,19 @@ int sph_ensemble_autotune(sph_ensemble *ens, int trial_steps, const double *frac
                           int nfrac, double *chosen, double *ms_per_step);
 
 /* ------------------------------------------------------------------------
- * Navier-Stokes fluid (cfd_solver = 0) without grains (np = 0): the
- * reference's fluid part of each time step, src/device.cu:1213-1898, on a
- * staggered grid (cell-centred p, face velocities, one ghost layer of cells
- * and of faces): porosity 1 (src/navierstokes.cuh:1319-1328), viscous and
- * advective terms, predicted face velocities (Chorin projection with
- * beta = 0, or a pressure-dependent one with beta > 0), the Jacobi
+ * Navier-Stokes fluid (cfd_solver = 0): the reference's fluid part of
+ * each time step, src/device.cu:1213-1898, on a staggered grid
+ * (cell-centred p, face velocities, one ghost layer of cells and of
+ * faces): porosities (1 without grains, src/navierstokes.cuh:1319-1328),
+ * viscous and advective terms, predicted face velocities (Chorin projection
+ * with beta = 0, or a pressure-dependent one with beta > 0), the Jacobi
  * iteration of the pressure change epsilon with its normalized-residual
  * test every 10 iterations, optional smoothing, and the pressure and
- * velocity updates (src/navierstokes.cuh:261-3514).  No walls.
- * Boundaries: bc 0 fixed pressure, 1 no flow with free slip, 2 no flow
- * without slip (python/sphere.py:3527-3598); x and y are periodic.
+ * velocity updates (src/navierstokes.cuh:261-3514).  On its own a fluid
+ * has no grains (or grains held in place, sph_ns_set_grains); attached to
+ * a context (sph_ns_attach) it moves the context's grains and sees its top
+ * wall.  Boundaries: bc 0 fixed pressure, 1 no flow with free slip, 2 no
+ * flow without slip (python/sphere.py:3527-3598); x and y are periodic.
  * ---------------------------------------------------------------------- */
 typedef struct sph_ns_params {
     uint32_t nx, ny, nz;          /* cells, each >= 3 */
