@@ -963,6 +963,58 @@ def navier_stokes_section(args):
         log("Navier-Stokes on the CPU oracle: %.0f us per Jacobi iteration (one thread), %.0f ms "
             "per step at the GPU's iteration count" % (us_it, us_it * iters / 1000.0))
     out["cells"] = cells
+    out["coupled"] = navier_stokes_coupled(args)
+    return out
+
+
+def navier_stokes_coupled(args):
+    """The fluid attached to a DEM context (sph_ns_attach): a settled packing
+    of 20,000 grains (the consolidation box's recipe with 16 growth cells
+    across, periodic in x and y, a wall on top) in water on the packing's
+    grid, each DEM step running the fluid iteration and adding its force on
+    the grains; --ns-steps timed steps after 2 untimed ones."""
+    import sphere_amd
+    from sphere_amd import _abi
+    from sphere_amd import workloads as W
+    n = 20_000
+    s, f0, _ = W.consolidation_base(n=n, seed=4, cells_xy=16)
+    e = sphere_amd.Engine(0, nc=32)
+    st = W.EngineStepper(e)
+    W.grow_to_jamming(s, st, f0, z_target=4.2, steps=300, fine_df=0.005)
+    s.w_x[0] = s.L[2]
+    s.g[2] = -9.81
+    W.relax_packing(s, st, 2, 300)
+    e.close()
+    nx, ny, nz = (int(v) for v in s.num)
+    P = _abi.ns_params(nx, ny, nz, s.L, float(s.time_dt[0]), mu=8.9e-4, rho_f=1000.0, g=s.g, bc_bot=1,
+                       bc_top=0, tolerance=1.0e-8, maxiter=100000, t0=float(s.time_current[0]))
+    dz = s.L[2] / nz
+    p = numpy.broadcast_to(1000.0 * 9.81 * dz * (nz - 1 - numpy.arange(nz)), (nx, ny, nz)).copy()
+    e = sphere_amd.Engine(0, nc=32)
+    e.load(s.to_state())
+    ns = sphere_amd.NavierStokes(P, numpy.zeros((nx, ny, nz, 3)), p)
+    e.attach_fluid(ns)
+    e.step(2)
+    a = ns.stats()
+    t0 = time.perf_counter()
+    e.step(args.ns_steps)
+    el = time.perf_counter() - t0
+    b = ns.stats()
+    phi = ns.read()[2]
+    e.close()
+    ns.close()
+    out = {"workload": "%d settled grains (radii U[0.009, 0.011] m, periodic in x and y, a wall "
+                       "on top) in water on their %d x %d x %d grid, the Navier-Stokes fluid "
+                       "attached (its force moves the grains), dt %.3g s, Jacobi tolerance 1e-8"
+                       % (n, nx, ny, nz, float(s.time_dt[0])),
+           "grains": n, "grid": [nx, ny, nz], "steps": args.ns_steps,
+           "ms_per_step": 1000.0 * el / args.ns_steps,
+           "particle_steps_per_s": n * args.ns_steps / el,
+           "jacobi_iterations_per_step": (b.iterations - a.iterations) / float(args.ns_steps),
+           "porosity_min_mean": [float(phi.min()), float(phi.mean())]}
+    log("Navier-Stokes with %d grains moving: %.3f ms per step (%.3g particle-steps/s), %.0f Jacobi "
+        "iterations per step" % (n, out["ms_per_step"], out["particle_steps_per_s"],
+                                 out["jacobi_iterations_per_step"]))
     return out
 
 
