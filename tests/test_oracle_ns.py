@@ -409,3 +409,37 @@ def test_grains_outside_the_grid_are_refused():
     o = _grain_case()
     with pytest.raises(ValueError):
         o.set_grains([[0.25, 0.25, 0.7, 0.01]], [[0.0, 0.0, 0.0]])
+
+
+def test_coupled_oracle_moves_grains_by_the_fluid_force():
+    """The DEM oracle with a fluid attached (orc_attach_ns): grains apart
+    from each other, at rest, no gravity, in water flowing along x.  After
+    one step each grain's velocity change is its fluid force over its mass
+    times one and the same factor of the time step (the force enters the
+    integration like a contact force): m dv / f is 1.5 dt for every grain,
+    the third-order Taylor update dv = a dt + (a - a0) dt / 2 with a0 = 0
+    on the first step (src/integration.cuh:73-81, 225-260); the fluid's
+    force points along the flow."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from _cases import dense_case
+    from _oracle import Oracle
+    s = dense_case(n=64, seed=2, periodic=1, vmag=0.0, wmag=0.0, gravity=False, spacing_factor=1.3)
+    st = s.to_state()
+    nx, ny, nz = (int(v) for v in s.num)
+    dt = float(s.time_dt[0])
+    P = _abi.ns_params(nx, ny, nz, s.L, dt, mu=1.0e-3, rho_f=1000.0, g=(0.0, 0.0, 0.0), bc_bot=1,
+                       bc_top=0, tolerance=1e-10, maxiter=100000)
+    v = numpy.zeros((nx, ny, nz, 3))
+    v[..., 0] = 0.05
+    o = Oracle(st)
+    nso = NsOracle(P, v, numpy.zeros((nx, ny, nz)))
+    o.attach_ns(nso)
+    o.step(1)
+    f_sum = nso.grain_forces()[3]
+    vel = o.read().get("vel", 64, 4)[:, :3]
+    m = float(s.rho[0]) * 4.0 / 3.0 * numpy.pi * s.radius ** 3
+    assert numpy.all(f_sum[:, 0] > 0)
+    ratio = m * vel[:, 0] / f_sum[:, 0]
+    assert numpy.allclose(ratio, 1.5 * dt, rtol=1e-9)
